@@ -1,0 +1,612 @@
+// C ABI implementation: setup -> mesh -> device state -> graph-captured SSP-RK3 steps.
+#include "../../include/corefl_c.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <fstream>
+#include <memory>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "flow.hpp"
+#include "mesh.hpp"
+#include "parameter.hpp"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+constexpr double kGamma = 1.4, kRu = 8314.462618, kMwAir = 28.9647;
+
+struct HipError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+#define HIPCHK(x)                                                                                          \
+  do {                                                                                                     \
+    hipError_t e_ = (x);                                                                                   \
+    if (e_ != hipSuccess) throw HipError(std::string(#x) + ": " + hipGetErrorString(e_));                 \
+  } while (0)
+
+double sutherland(double T) { return 1.7894e-5 * std::pow(T / 288.16, 1.5) * (288.16 + 110) / (T + 110); }
+
+// Air state of an inflow-type struct (Inflow::Inflow, BoundCond.cpp:158-262, single species).
+struct FlowState {
+  double rho, u, v, w, p, T, c, mu;
+};
+
+FlowState inflow_state(const corefl::Struct &s) {
+  using namespace corefl;
+  double mach = struct_real(s, "mach", -1), p = struct_real(s, "pressure", -1);
+  double T = struct_real(s, "temperature", -1), vel = struct_real(s, "velocity", 0);
+  double rho = struct_real(s, "density", -1), aoa = struct_real(s, "aoa", -1000);
+  double u = struct_real(s, "u", 1), v = struct_real(s, "v", 0), w = struct_real(s, "w", 0);
+  const double re = struct_real(s, "reynolds", -1);
+  const double mw = kMwAir;
+  if (T < 0) T = p * mw / (rho * kRu);
+  const double mu = sutherland(T);
+  const double c = std::sqrt(kGamma * kRu / mw * T);
+  if (mach < 0) mach = vel / c;
+  else vel = mach * c;
+  if (aoa > -1000) {
+    u = std::cos(aoa * M_PI / 180) * vel;
+    v = std::sin(aoa * M_PI / 180) * vel;
+    w = 0;
+  } else {
+    u *= vel;
+    v *= vel;
+    w *= vel;
+  }
+  if (rho < 0) {
+    if (p < 0) {
+      if (re < 0) throw std::runtime_error("inflow struct: need density, pressure or reynolds");
+      rho = mu * re / vel;
+      p = rho * T * kRu / mw;
+    } else {
+      rho = p * mw / (kRu * T);
+    }
+  }
+  return {rho, u, v, w, p, T, c, mu};
+}
+
+struct Block {
+  corefl::HostBlock host;
+  corefl::DevBlock dev{};
+  std::vector<corefl::DevFace> faces; // in boundary-file order
+  long part_offset = 0;
+  int part_count = 0;
+};
+
+} // namespace
+
+struct corefl_ctx {
+  corefl::Parameter param;
+  corefl::Scheme scheme{};
+  std::vector<Block> blocks;
+  hipStream_t stream = nullptr;
+  double *dt_dev = nullptr, *time_dev = nullptr, *parts = nullptr;
+  long n_parts = 0;
+  bool fixed_dt = false;
+  double t_end = 1e10;
+  double host_dt = 0; // source of host-to-device copies of the step size
+  hipGraphExec_t graph_exec = nullptr;
+  hipGraph_t graph = nullptr;
+  std::vector<void *> allocations;
+
+  double *alloc(size_t n) {
+    void *p = nullptr;
+    HIPCHK(hipMalloc(&p, n * sizeof(double)));
+    HIPCHK(hipMemset(p, 0, n * sizeof(double)));
+    allocations.push_back(p);
+    return static_cast<double *>(p);
+  }
+  ~corefl_ctx() {
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (void *p : allocations) (void)hipFree(p);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  hipStream_t pick(void *s) const { return s ? static_cast<hipStream_t>(s) : stream; }
+
+  void apply_bcs(Block &b, hipStream_t s) {
+    // DBoundCond::apply_boundary_conditions order (BoundCond.cu:3489-3619): outflow, then periodic.
+    for (const auto &f : b.faces)
+      if (f.type == 6) HIPCHK(corefl::launch_boundary(b.dev, f, s));
+    for (const auto &f : b.faces)
+      if (f.type == 8) HIPCHK(corefl::launch_boundary(b.dev, f, s));
+  }
+
+  // Residual of one stage for all blocks; stage < 0 writes dq only.
+  void stage(int rk, hipStream_t s) {
+    const bool rhs_only = rk < 0;
+    if (scheme.viscous_order > 0) {
+      for (auto &b : blocks) {
+        HIPCHK(corefl::launch_viscous_flux(b.dev, scheme, s));
+        for (const auto &f : b.faces)
+          if (f.type == 8) HIPCHK(corefl::launch_viscous_exchange(b.dev, f, s));
+      }
+    }
+    for (auto &b : blocks) {
+      if (scheme.inviscid_scheme == 2) HIPCHK(corefl::launch_entropy_fix(b.dev, scheme, s));
+      HIPCHK(corefl::launch_inviscid_flux(b.dev, scheme, 0, s));
+      HIPCHK(corefl::launch_inviscid_flux(b.dev, scheme, 1, s));
+      if (b.dev.dim == 3) HIPCHK(corefl::launch_inviscid_flux(b.dev, scheme, 2, s));
+      HIPCHK(corefl::launch_update(b.dev, scheme, rhs_only ? 0 : rk, dt_dev, rhs_only, s));
+      if (!rhs_only) apply_bcs(b, s);
+    }
+  }
+
+  void enqueue_dt(hipStream_t s, int clip) {
+    for (auto &b : blocks) HIPCHK(corefl::launch_local_dt(b.dev, scheme, parts + b.part_offset, s));
+    HIPCHK(corefl::launch_dt_reduce(parts, n_parts, dt_dev, time_dev, t_end, clip, s));
+  }
+
+  void enqueue_step(hipStream_t s, bool compute_dt) {
+    if (compute_dt) enqueue_dt(s, 1);
+    for (int rk = 0; rk < 3; ++rk) stage(rk, s);
+    HIPCHK(corefl::launch_time_advance(time_dev, dt_dev, s));
+  }
+};
+
+namespace {
+
+std::string read_file(const std::string &path, bool binary = false) {
+  std::ifstream f(path, binary ? std::ios::binary : std::ios::in);
+  if (!f) throw std::runtime_error("cannot open " + path);
+  std::stringstream ss;
+  ss << f.rdbuf();
+  return ss.str();
+}
+
+// Initial primitive state on every node of one block, ghosts included (Initialize.cu:51-95 and
+// initialize_bv_with_inflow in Field.cu:280-340; canonical_problem "taylorGreen" is new).
+std::vector<double> initial_bv(const corefl::Parameter &P, const corefl::HostBlock &hb) {
+  const long sz = hb.size();
+  std::vector<double> bv(6 * sz, 0.0);
+  const int g = hb.ngg, G = g + 1;
+  const long c2 = hb.mx + 2L * G, c1 = (hb.my + 2L * G) * c2;
+  auto coord = [&](const std::vector<double> &a, int i, int j, int k) {
+    return a[(k + G) * c1 + (j + G) * c2 + (i + G)];
+  };
+  auto put = [&](long id, const FlowState &s) {
+    bv[id] = s.rho;
+    bv[id + sz] = s.u;
+    bv[id + 2 * sz] = s.v;
+    bv[id + 3 * sz] = s.w;
+    bv[id + 4 * sz] = s.p;
+    bv[id + 5 * sz] = s.T;
+  };
+  const std::string problem = P.get_string("canonical_problem");
+  if (problem == "taylorGreen") {
+    const auto &s = P.get_struct("taylorGreen");
+    const double T0 = corefl::struct_real(s, "temperature", 300.0);
+    const double p0 = corefl::struct_real(s, "pressure", 101325.0);
+    const double mach = corefl::struct_real(s, "mach", 0.1);
+    const double R = kRu / kMwAir, rho0 = p0 / (R * T0), V0 = mach * std::sqrt(kGamma * R * T0);
+    const double L = P.get_real("gridScale");
+    for (int k = -g; k < hb.mz + g; ++k)
+      for (int j = -g; j < hb.my + g; ++j)
+        for (int i = -g; i < hb.mx + g; ++i) {
+          const double X = coord(hb.x, i, j, k) / L, Y = coord(hb.y, i, j, k) / L, Z = coord(hb.z, i, j, k) / L;
+          const double p = p0 + rho0 * V0 * V0 / 16 * (std::cos(2 * X) + std::cos(2 * Y)) * (std::cos(2 * Z) + 2);
+          FlowState st{p / (R * T0), V0 * std::sin(X) * std::cos(Y) * std::cos(Z),
+                       -V0 * std::cos(X) * std::sin(Y) * std::cos(Z), 0.0, p, T0, 0, 0};
+          put(hb.idx(i, j, k), st);
+        }
+    return bv;
+  }
+  const int groups = P.get_int("groups_init");
+  std::vector<FlowState> states{inflow_state(P.get_struct(P.get_string("default_init")))};
+  std::vector<double> boxes;
+  for (int l = 0; l < groups - 1; ++l) {
+    const auto &pc = P.get_struct("init_cond_" + std::to_string(l));
+    for (const char *key : {"x0", "x1", "y0", "y1", "z0", "z1"}) boxes.push_back(corefl::struct_real(pc, key, 0));
+    const std::string name = corefl::struct_string(pc, "name", "init_cond_" + std::to_string(l));
+    states.push_back(inflow_state(P.get_struct(name)));
+  }
+  for (int k = -g; k < hb.mz + g; ++k)
+    for (int j = -g; j < hb.my + g; ++j)
+      for (int i = -g; i < hb.mx + g; ++i) {
+        const double x = coord(hb.x, i, j, k), y = coord(hb.y, i, j, k), z = coord(hb.z, i, j, k);
+        int pick = 0;
+        for (int l = 0; l < groups - 1; ++l) {
+          const double *r = &boxes[6 * l];
+          if (x >= r[0] && x <= r[1] && y >= r[2] && y <= r[3] && z >= r[4] && z <= r[5]) {
+            pick = l + 1;
+            break;
+          }
+        }
+        put(hb.idx(i, j, k), states[pick]);
+      }
+  return bv;
+}
+
+void build(corefl_ctx *ctx, std::vector<corefl::HostBlock> blocks) {
+  auto &P = ctx->param;
+  if (P.get_int("reconstruction") != 4 && P.get_int("reconstruction") != 5)
+    throw std::runtime_error("reconstruction must be 4 (WENO5) or 5 (WENO7) on this path");
+  const int inv = P.get_int("inviscid_scheme");
+  if (inv < 1 || inv > 4) throw std::runtime_error("inviscid_scheme must be 1 (LF), 2 (Roe), 3 (AUSM+) or 4 (HLLC)");
+  const int vo = P.get_int("viscous_order");
+  if (vo != 0 && vo != 4 && vo != 8) throw std::runtime_error("viscous_order must be 0, 4 or 8 on this path");
+  if (P.get_int("temporal_scheme") != 3) throw std::runtime_error("temporal_scheme must be 3 (SSP-RK3)");
+  const int ngg = P.get_int("ngg");
+  P.set_int("dimension", blocks.at(0).dim);
+  // reference state and the WENO epsilon scale (BoundCond.cpp:276-295, DParameter.cu:25-28, WENO.cu:8)
+  double rho_inf, v_inf;
+  if (P.get_string("canonical_problem") == "taylorGreen") {
+    const auto &s = P.get_struct("taylorGreen");
+    const double T0 = corefl::struct_real(s, "temperature", 300.0), p0 = corefl::struct_real(s, "pressure", 101325.0);
+    const double R = kRu / kMwAir;
+    rho_inf = p0 / (R * T0);
+    v_inf = corefl::struct_real(s, "mach", 0.1) * std::sqrt(kGamma * R * T0);
+  } else {
+    const FlowState r = inflow_state(P.get_struct(P.get_string("reference_state")));
+    rho_inf = r.rho;
+    v_inf = std::abs(std::sqrt(r.u * r.u + r.v * r.v + r.w * r.w)) < 1 ? r.c : std::sqrt(r.u * r.u + r.v * r.v + r.w * r.w);
+  }
+  P.set_real("rho_inf", rho_inf);
+  P.set_real("v_inf", v_inf);
+  P.set_real("weno_eps", 1e-40 * rho_inf * v_inf * rho_inf * v_inf);
+  ctx->scheme = {inv, P.get_int("reconstruction"), vo, P.get_real("weno_eps"), P.get_real("entropy_fix_factor"),
+                 P.get_real("prandtl_number"), P.get_real("cfl")};
+  ctx->fixed_dt = P.get_bool("fixed_time_step");
+  ctx->t_end = P.get_real("total_simulation_time");
+
+  HIPCHK(hipStreamCreate(&ctx->stream));
+  ctx->dt_dev = ctx->alloc(1);
+  ctx->time_dev = ctx->alloc(1);
+  for (auto &hb : blocks) {
+    // Every kernel indexes at most ngg nodes beyond the interior; reject blocks and faces that would
+    // make any of those accesses leave the allocation.
+    const int ext[3] = {hb.mx, hb.my, hb.mz};
+    if (hb.mx < 3 || hb.my < 3 || (hb.mz != 1 && hb.mz < 3))
+      throw std::runtime_error("block extents must be >= 3 (mz = 1 for 2D blocks)");
+    for (int d = 0; d < hb.dim; ++d)
+      if (ext[d] < ngg + 1) throw std::runtime_error("block extent smaller than ngg + 1");
+    if (vo > 0)
+      for (int d = 0; d < hb.dim; ++d)
+        if (ext[d] < 5) throw std::runtime_error("viscous closures need at least 5 nodes per direction");
+    for (auto &f : hb.faces) {
+      if (f.type != 6 && f.type != 8)
+        throw std::runtime_error("boundary type " + std::to_string(f.type) + " is not supported on this path");
+      for (int d = 0; d < 3; ++d)
+        if (f.start[d] < 0 || f.end[d] < f.start[d] || f.end[d] >= ext[d])
+          throw std::runtime_error("boundary face range outside the block");
+      const int nf = f.start[f.face];
+      if (f.end[f.face] != nf || (nf != 0 && nf != ext[f.face] - 1))
+        throw std::runtime_error("boundary face is not on a block side");
+    }
+    Block B;
+    B.host = std::move(hb);
+    auto &h = B.host;
+    auto &d = B.dev;
+    d.mx = h.mx; d.my = h.my; d.mz = h.mz; d.g = ngg; d.dim = h.dim;
+    d.d2 = h.pad2(); d.d1 = h.pad1(); d.sz = h.size(); d.n = (long)h.mx * h.my * h.mz;
+    for (int q = 0; q < 6; ++q) d.bt[q] = h.btype[q];
+    const size_t sz = d.sz;
+    d.cv = ctx->alloc(5 * sz);
+    d.qn = ctx->alloc(5 * sz);
+    d.bv = ctx->alloc(6 * sz);
+    d.jac = ctx->alloc(sz);
+    d.met = ctx->alloc(9 * sz);
+    d.fx = ctx->alloc(5 * sz);
+    d.fy = ctx->alloc(5 * sz);
+    d.fz = d.dim == 3 ? ctx->alloc(5 * sz) : d.fy;
+    if (vo > 0) {
+      d.fv = ctx->alloc(5 * sz);
+      d.gv = ctx->alloc(5 * sz);
+      d.hv = ctx->alloc(5 * sz);
+    }
+    if (inv == 2) d.efd = ctx->alloc(sz);
+    d.dq = ctx->alloc(5 * d.n);
+    HIPCHK(hipMemcpy(d.jac, h.jac.data(), sz * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d.met, h.metric.data(), 9 * sz * sizeof(double), hipMemcpyHostToDevice));
+    for (const auto &f : h.faces) {
+      corefl::DevFace df{};
+      for (int q = 0; q < 3; ++q) { df.start[q] = f.start[q]; df.end[q] = f.end[q]; }
+      df.type = f.type; df.face = f.face; df.direction = f.direction;
+      B.faces.push_back(df);
+    }
+    B.part_offset = ctx->n_parts;
+    B.part_count = corefl::dt_partial_count(d);
+    ctx->n_parts += B.part_count;
+    ctx->blocks.push_back(std::move(B));
+  }
+  ctx->parts = ctx->alloc(ctx->n_parts);
+  // initial condition, then initialize_computation (Driver.cu:126-211)
+  for (auto &B : ctx->blocks) {
+    const auto bv = initial_bv(P, B.host);
+    HIPCHK(hipMemcpy(B.dev.bv, bv.data(), bv.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(corefl::launch_cv_from_bv_all(B.dev, ctx->stream));
+    ctx->apply_bcs(B, ctx->stream);
+  }
+  ctx->host_dt = P.get_real("dt");
+  HIPCHK(hipMemcpyAsync(ctx->dt_dev, &ctx->host_dt, sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  // free the host copies of the coordinates and metrics
+  for (auto &B : ctx->blocks) {
+    std::vector<double>().swap(B.host.x);
+    std::vector<double>().swap(B.host.y);
+    std::vector<double>().swap(B.host.z);
+  }
+}
+
+int fail(const std::exception &e) {
+  g_last_error = e.what();
+  return -1;
+}
+
+std::vector<corefl::HostBlock> prepare_blocks(corefl::Parameter &P, std::vector<corefl::HostBlock> blocks,
+                                              const std::string &boundary_text) {
+  corefl::parse_boundaries(boundary_text, blocks, P.get_int("periodic_label"));
+  for (auto &b : blocks) {
+    std::string err;
+    if (!corefl::compute_metrics(b, err)) throw std::runtime_error(err);
+  }
+  return blocks;
+}
+
+} // namespace
+
+extern "C" {
+
+const char *corefl_last_error(void) { return g_last_error.c_str(); }
+
+int corefl_create(const char *input_dir, corefl_ctx **out) {
+  try {
+    auto ctx = std::make_unique<corefl_ctx>();
+    const std::string dir(input_dir);
+    ctx->param.parse(read_file(dir + "/setup.txt"));
+    ctx->param.deduce();
+    auto &P = ctx->param;
+    if (!P.get_int("gridIsBinary")) throw std::runtime_error("only binary grids (gridIsBinary = 1) are read");
+    auto blocks = corefl::read_grid_binary(dir + "/grid/grid   0.dat", P.get_int("ngg"), P.get_real("gridScale"));
+    blocks = prepare_blocks(P, std::move(blocks), read_file(dir + "/boundary_condition/boundary   0.txt"));
+    build(ctx.get(), std::move(blocks));
+    *out = ctx.release();
+    return 0;
+  } catch (const std::exception &e) {
+    return fail(e);
+  }
+}
+
+int corefl_create_mem(const char *setup_text, int n_block, const int *dims, const double *const *xyz,
+                      const char *boundary_text, corefl_ctx **out) {
+  try {
+    auto ctx = std::make_unique<corefl_ctx>();
+    ctx->param.parse(setup_text);
+    ctx->param.deduce();
+    auto &P = ctx->param;
+    std::vector<corefl::HostBlock> blocks;
+    for (int b = 0; b < n_block; ++b)
+      blocks.push_back(corefl::make_block(dims[3 * b], dims[3 * b + 1], dims[3 * b + 2], P.get_int("ngg"),
+                                          xyz[3 * b], xyz[3 * b + 1], xyz[3 * b + 2], P.get_real("gridScale")));
+    blocks = prepare_blocks(P, std::move(blocks), boundary_text);
+    build(ctx.get(), std::move(blocks));
+    *out = ctx.release();
+    return 0;
+  } catch (const std::exception &e) {
+    return fail(e);
+  }
+}
+
+void corefl_destroy(corefl_ctx *ctx) { delete ctx; }
+
+int corefl_n_blocks(const corefl_ctx *ctx) { return ctx ? static_cast<int>(ctx->blocks.size()) : -1; }
+
+int corefl_block_view_get(corefl_ctx *ctx, int block, corefl_block_view *v) {
+  if (!ctx || block < 0 || block >= (int)ctx->blocks.size() || !v) {
+    g_last_error = "corefl_block_view_get: bad arguments";
+    return -1;
+  }
+  const auto &d = ctx->blocks[block].dev;
+  v->mx = d.mx; v->my = d.my; v->mz = d.mz; v->ngg = d.g; v->n_var = 5; v->n_spec = 0;
+  v->sz = d.sz;
+  v->cv = d.cv; v->bv = d.bv; v->qn = d.qn; v->metric = d.met; v->jac = d.jac; v->dq = d.dq;
+  return 0;
+}
+
+int corefl_get_array(corefl_ctx *ctx, int block, int which, double *host, long long n) {
+  try {
+    if (!ctx || block < 0 || block >= (int)ctx->blocks.size()) throw std::runtime_error("bad block index");
+    const auto &d = ctx->blocks[block].dev;
+    const double *src = nullptr;
+    long long cnt = 0;
+    switch (which) {
+      case COREFL_CV: src = d.cv; cnt = 5 * d.sz; break;
+      case COREFL_BV: src = d.bv; cnt = 6 * d.sz; break;
+      case COREFL_DQ: src = d.dq; cnt = 5 * d.n; break;
+      case COREFL_JAC: src = d.jac; cnt = d.sz; break;
+      case COREFL_METRIC: src = d.met; cnt = 9 * d.sz; break;
+      case COREFL_FV: src = d.fv; cnt = 5 * d.sz; break;
+      case COREFL_GV: src = d.gv; cnt = 5 * d.sz; break;
+      case COREFL_HV: src = d.hv; cnt = 5 * d.sz; break;
+      default: throw std::runtime_error("unknown array selector");
+    }
+    if (!src) throw std::runtime_error("array not allocated for this scheme");
+    if (n != cnt) throw std::runtime_error("host buffer has " + std::to_string(n) + " elements, need " + std::to_string(cnt));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipMemcpy(host, src, cnt * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+  } catch (const std::exception &e) {
+    return fail(e);
+  }
+}
+
+int corefl_set_state(corefl_ctx *ctx, int block, const double *bv_host, long long n) {
+  try {
+    if (!ctx || block < 0 || block >= (int)ctx->blocks.size()) throw std::runtime_error("bad block index");
+    auto &B = ctx->blocks[block];
+    if (n != 6 * B.dev.sz) throw std::runtime_error("bv must have 6*sz elements");
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipMemcpy(B.dev.bv, bv_host, n * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(corefl::launch_cv_from_bv_all(B.dev, ctx->stream));
+    ctx->apply_bcs(B, ctx->stream);
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return 0;
+  } catch (const std::exception &e) {
+    return fail(e);
+  }
+}
+
+int corefl_rhs(corefl_ctx *ctx, void *stream) {
+  try {
+    ctx->stage(-1, ctx->pick(stream));
+    return 0;
+  } catch (const std::exception &e) {
+    return fail(e);
+  }
+}
+
+int corefl_compute_dt(corefl_ctx *ctx, double *dt_out) {
+  try {
+    ctx->enqueue_dt(ctx->stream, 0);
+    HIPCHK(hipMemcpyAsync(dt_out, ctx->dt_dev, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return 0;
+  } catch (const std::exception &e) {
+    return fail(e);
+  }
+}
+
+int corefl_rk3_step(corefl_ctx *ctx, double *dt_inout, void *stream) {
+  try {
+    hipStream_t s = ctx->pick(stream);
+    const bool given = dt_inout && *dt_inout > 0;
+    if (given) HIPCHK(hipMemcpyAsync(ctx->dt_dev, dt_inout, sizeof(double), hipMemcpyHostToDevice, s));
+    else if (ctx->fixed_dt) {
+      ctx->host_dt = ctx->param.get_real("dt");
+      HIPCHK(hipMemcpyAsync(ctx->dt_dev, &ctx->host_dt, sizeof(double), hipMemcpyHostToDevice, s));
+    }
+    ctx->enqueue_step(s, !given && !ctx->fixed_dt);
+    if (dt_inout && !given) {
+      HIPCHK(hipMemcpyAsync(dt_inout, ctx->dt_dev, sizeof(double), hipMemcpyDeviceToHost, s));
+    }
+    HIPCHK(hipStreamSynchronize(s));
+    return 0;
+  } catch (const std::exception &e) {
+    return fail(e);
+  }
+}
+
+int corefl_advance(corefl_ctx *ctx, int n_steps, int use_graph, void *stream) {
+  try {
+    hipStream_t s = ctx->pick(stream);
+    if (ctx->fixed_dt) {
+      ctx->host_dt = ctx->param.get_real("dt");
+      HIPCHK(hipMemcpyAsync(ctx->dt_dev, &ctx->host_dt, sizeof(double), hipMemcpyHostToDevice, s));
+      HIPCHK(hipStreamSynchronize(s));
+    }
+    if (!use_graph) {
+      for (int n = 0; n < n_steps; ++n) ctx->enqueue_step(s, !ctx->fixed_dt);
+      return 0;
+    }
+    if (!ctx->graph_exec) {
+      HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      try {
+        ctx->enqueue_step(s, !ctx->fixed_dt);
+      } catch (...) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(s, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+      }
+      HIPCHK(hipStreamEndCapture(s, &ctx->graph));
+      HIPCHK(hipGraphInstantiate(&ctx->graph_exec, ctx->graph, nullptr, nullptr, 0));
+    }
+    for (int n = 0; n < n_steps; ++n) HIPCHK(hipGraphLaunch(ctx->graph_exec, s));
+    return 0;
+  } catch (const std::exception &e) {
+    return fail(e);
+  }
+}
+
+int corefl_get_time(corefl_ctx *ctx, double *t) {
+  try {
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipMemcpy(t, ctx->time_dev, sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+  } catch (const std::exception &e) {
+    return fail(e);
+  }
+}
+
+int corefl_synchronize(corefl_ctx *ctx) {
+  try {
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipDeviceSynchronize());
+    return 0;
+  } catch (const std::exception &e) {
+    return fail(e);
+  }
+}
+
+int corefl_profile_kernels(corefl_ctx *ctx, int reps, double *ms_out, int n_out) {
+  try {
+    if (reps < 1 || n_out < 1) throw std::runtime_error("reps and n_out must be positive");
+    hipStream_t s = ctx->stream;
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    std::vector<double> ms(8, 0.0);
+    auto timed = [&](int slot, auto &&enqueue) {
+      HIPCHK(hipEventRecord(e0, s));
+      for (int r = 0; r < reps; ++r) enqueue();
+      HIPCHK(hipEventRecord(e1, s));
+      HIPCHK(hipEventSynchronize(e1));
+      float t = 0;
+      HIPCHK(hipEventElapsedTime(&t, e0, e1));
+      ms[slot] = t / reps;
+    };
+    auto &sc = ctx->scheme;
+    for (auto &b : ctx->blocks) {
+      if (sc.viscous_order > 0) {
+        timed(0, [&] { HIPCHK(corefl::launch_viscous_flux(b.dev, sc, s)); });
+        timed(1, [&] {
+          for (const auto &f : b.faces)
+            if (f.type == 8) HIPCHK(corefl::launch_viscous_exchange(b.dev, f, s));
+        });
+      }
+      if (sc.inviscid_scheme == 2) HIPCHK(corefl::launch_entropy_fix(b.dev, sc, s));
+      timed(2, [&] { HIPCHK(corefl::launch_inviscid_flux(b.dev, sc, 0, s)); });
+      timed(3, [&] { HIPCHK(corefl::launch_inviscid_flux(b.dev, sc, 1, s)); });
+      if (b.dev.dim == 3) timed(4, [&] { HIPCHK(corefl::launch_inviscid_flux(b.dev, sc, 2, s)); });
+      timed(5, [&] { HIPCHK(corefl::launch_update(b.dev, sc, 1, ctx->dt_dev, false, s)); });
+      timed(6, [&] { ctx->apply_bcs(b, s); });
+      timed(7, [&] { ctx->enqueue_dt(s, 0); });
+      break; // first block only
+    }
+    HIPCHK(hipEventDestroy(e0));
+    HIPCHK(hipEventDestroy(e1));
+    for (int q = 0; q < n_out && q < 8; ++q) ms_out[q] = ms[q];
+    return 0;
+  } catch (const std::exception &e) {
+    return fail(e);
+  }
+}
+
+int corefl_get_int(const corefl_ctx *ctx, const char *name, int *out) {
+  try {
+    *out = ctx->param.get_int(name);
+    return 0;
+  } catch (const std::exception &e) {
+    return fail(e);
+  }
+}
+
+int corefl_get_real(const corefl_ctx *ctx, const char *name, double *out) {
+  try {
+    *out = ctx->param.get_real(name);
+    return 0;
+  } catch (const std::exception &e) {
+    return fail(e);
+  }
+}
+
+} // extern "C"

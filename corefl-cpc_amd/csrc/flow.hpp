@@ -1,0 +1,50 @@
+// Device-side block description and the launchers of the flow kernels (flow_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace corefl {
+
+// One block's device arrays in the DZone layout (Field.h:12-110): f(i,j,k,l) = p[idx(i,j,k) + l*sz].
+struct DevBlock {
+  int mx, my, mz, g, dim;
+  long d2, d1, sz, n; // padded strides, padded size, interior node count
+  double *cv, *bv, *qn, *jac, *met; // cv/qn: 5*sz, bv: 6*sz, met: 9*sz
+  double *fx, *fy, *fz;             // interface fluxes F(i+1/2) stored at node i (5*sz each)
+  double *fv, *gv, *hv;             // collocated viscous fluxes at nodes (5*sz each, l = 1..4 used)
+  double *efd;                      // Roe entropy-fix delta (sz)
+  double *dq;                       // residual (5*n), written only by the RHS-only path
+  int bt[6];                        // physical-boundary flags il, ir, jl, jr, kl, kr
+};
+
+struct DevFace {
+  int start[3], end[3];
+  int type, face, direction;
+};
+
+struct Scheme {
+  int inviscid_scheme; // 1 LF, 2 Roe, 3 AUSM+, 4 HLLC
+  int reconstruction;  // 4 WENO5, 5 WENO7 (characteristic)
+  int viscous_order;   // 0, 4, 8
+  double weno_eps, entropy_fix_factor, prandtl, cfl;
+};
+
+// Enqueue on `s`.  Each returns hipSuccess or the launch error.
+hipError_t launch_cv_from_bv_all(const DevBlock &b, hipStream_t s);
+hipError_t launch_boundary(const DevBlock &b, const DevFace &f, hipStream_t s);
+hipError_t launch_viscous_flux(const DevBlock &b, const Scheme &sc, hipStream_t s);
+hipError_t launch_viscous_exchange(const DevBlock &b, const DevFace &f, hipStream_t s);
+hipError_t launch_entropy_fix(const DevBlock &b, const Scheme &sc, hipStream_t s);
+hipError_t launch_inviscid_flux(const DevBlock &b, const Scheme &sc, int dir, hipStream_t s);
+// Residual + SSP-RK3 stage update; rhs_only writes dq instead of updating.  dt is read from *dt_dev.
+hipError_t launch_update(const DevBlock &b, const Scheme &sc, int rk, const double *dt_dev, bool rhs_only,
+                         hipStream_t s);
+// CFL time step: per-workgroup partial minima into part[0 .. dt_partial_count(b)), then the global
+// minimum of n partials into dt_dev[0], clipped to t_end - time_dev[0] when clip != 0.
+int dt_partial_count(const DevBlock &b);
+hipError_t launch_local_dt(const DevBlock &b, const Scheme &sc, double *part, hipStream_t s);
+hipError_t launch_dt_reduce(const double *part, long n, double *dt_dev, const double *time_dev, double t_end,
+                            int clip, hipStream_t s);
+hipError_t launch_time_advance(double *time_dev, const double *dt_dev, hipStream_t s);
+
+} // namespace corefl

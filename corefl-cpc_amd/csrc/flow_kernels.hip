@@ -1,0 +1,819 @@
+// Flow kernels for CDNA4 (gfx950): characteristic WENO5/7 + Riemann interface fluxes, collocated
+// viscous fluxes, fused residual + SSP-RK3 update with primitive recovery, ghost-layer boundary
+// fills, and the CFL time-step reduction.  All arithmetic is FP64.
+//
+// Thread mapping: threadIdx.x always walks i (the unit-stride index of the DZone layout), so every
+// global load and store is coalesced along a 64-lane wavefront whatever the sweep direction.
+#include "flow.hpp"
+
+#include <cmath>
+
+namespace corefl {
+
+namespace {
+
+constexpr double kGamma = 1.4;          // gamma_air (Constants.h:21)
+constexpr double kRu = 8314.462618;     // R_u (Constants.h:17)
+constexpr double kMwAir = 28.9647;      // mw_air (Constants.h:26)
+constexpr double kRair = kRu / kMwAir;  // R_air (Constants.h:34)
+
+__device__ __forceinline__ long nid(const DevBlock &b, int i, int j, int k) {
+  return (k + b.g) * b.d1 + (j + b.g) * b.d2 + (i + b.g);
+}
+
+// ---------------------------------------------------------------- WENO halves
+// Upwind-biased value at the right face of v[2] from v[0..4] (the fPlus part of WENO.cu:3059-3114).
+// The downwind-biased value is the same function of the mirrored stencil.
+__device__ __forceinline__ double weno5_half(double v0, double v1, double v2, double v3, double v4, double eps) {
+  constexpr double o6 = 1.0 / 6, t12 = 13.0 / 12;
+  const double p0 = o6 * (2 * v2 + 5 * v3 - v4);
+  const double p1 = o6 * (-v1 + 5 * v2 + 2 * v3);
+  const double p2 = o6 * (2 * v0 - 7 * v1 + 11 * v2);
+  const double e0 = v2 + v4 - 2 * v3, f0 = 3 * v2 - 4 * v3 + v4;
+  const double e1 = v1 + v3 - 2 * v2, f1 = v1 - v3;
+  const double e2 = v0 + v2 - 2 * v1, f2 = v0 - 4 * v1 + 3 * v2;
+  const double b0 = t12 * e0 * e0 + 0.25 * f0 * f0;
+  const double b1 = t12 * e1 * e1 + 0.25 * f1 * f1;
+  const double b2 = t12 * e2 * e2 + 0.25 * f2 * f2;
+  const double a0 = 0.3 / ((eps + b0) * (eps + b0));
+  const double a1 = 0.6 / ((eps + b1) * (eps + b1));
+  const double a2 = 0.1 / ((eps + b2) * (eps + b2));
+  return (a0 * p0 + a1 * p1 + a2 * p2) / (a0 + a1 + a2);
+}
+
+// Upwind-biased value at the right face of v[3] from v[0..6] (the fPlus part of WENO.cu:3116-3222,
+// Z-type weights with tau7 = (b0 + 3 b1 - 3 b2 - b3)^2).
+__device__ __forceinline__ double weno7_half(const double *v, double eps) {
+  constexpr double o6 = 1.0 / 6, d12 = 13.0 / 12.0, d13 = 1043.0 / 960, d14 = 1.0 / 12;
+  double s1 = o6 * (-2 * v[0] + 9 * v[1] - 18 * v[2] + 11 * v[3]);
+  double s2 = -v[0] + 4 * v[1] - 5 * v[2] + 2 * v[3];
+  double s3 = -v[0] + 3 * v[1] - 3 * v[2] + v[3];
+  const double b0 = s1 * s1 + d12 * s2 * s2 + d13 * s3 * s3 + d14 * s1 * s3;
+  s1 = o6 * (v[1] - 6 * v[2] + 3 * v[3] + 2 * v[4]);
+  s2 = v[2] - 2 * v[3] + v[4];
+  s3 = -v[1] + 3 * v[2] - 3 * v[3] + v[4];
+  const double b1 = s1 * s1 + d12 * s2 * s2 + d13 * s3 * s3 + d14 * s1 * s3;
+  s1 = o6 * (-2 * v[2] - 3 * v[3] + 6 * v[4] - v[5]);
+  s3 = -v[2] + 3 * v[3] - 3 * v[4] + v[5];
+  const double b2 = s1 * s1 + d12 * s2 * s2 + d13 * s3 * s3 + d14 * s1 * s3;
+  s1 = o6 * (-11 * v[3] + 18 * v[4] - 9 * v[5] + 2 * v[6]);
+  s2 = 2 * v[3] - 5 * v[4] + 4 * v[5] - v[6];
+  s3 = -v[3] + 3 * v[4] - 3 * v[5] + v[6];
+  const double b3 = s1 * s1 + d12 * s2 * s2 + d13 * s3 * s3 + d14 * s1 * s3;
+  double tau = b0 + 3 * b1 - 3 * b2 - b3;
+  tau *= tau;
+  constexpr double c0 = 1.0 / 35, c1 = 12.0 / 35, c2 = 18.0 / 35, c3 = 4.0 / 35;
+  const double a0 = c0 + c0 * tau / ((eps + b0) * (eps + b0));
+  const double a1 = c1 + c1 * tau / ((eps + b1) * (eps + b1));
+  const double a2 = c2 + c2 * tau / ((eps + b2) * (eps + b2));
+  const double a3 = c3 + c3 * tau / ((eps + b3) * (eps + b3));
+  const double q0 = -3 * v[0] + 13 * v[1] - 23 * v[2] + 25 * v[3];
+  const double q1 = v[1] - 5 * v[2] + 13 * v[3] + 3 * v[4];
+  const double q2 = -v[2] + 7 * v[3] + 7 * v[4] - v[5];
+  const double q3 = 3 * v[3] + 13 * v[4] - 5 * v[5] + v[6];
+  return (1.0 / 12) * (a0 * q0 + a1 * q1 + a2 * q2 + a3 * q3) / (a0 + a1 + a2 + a3);
+}
+
+// ---------------------------------------------------------------- Riemann solvers
+// Primitive state: rho, u, v, w, p, E (E = total energy per unit volume).
+
+// AUSM+ on Jacobian-weighted averaged metrics (InviscidScheme.cu:397-469).
+__device__ __forceinline__ void flux_ausm(const double *L, const double *R, const double *ml, const double *mr,
+                                          double jl, double jr, double *fc) {
+  const double k1 = 0.5 * (jl * ml[0] + jr * mr[0]);
+  const double k2 = 0.5 * (jl * ml[1] + jr * mr[1]);
+  const double k3 = 0.5 * (jl * ml[2] + jr * mr[2]);
+  const double gk = sqrt(k1 * k1 + k2 * k2 + k3 * k3);
+  const double ul = (k1 * L[1] + k2 * L[2] + k3 * L[3]) / gk;
+  const double ur = (k1 * R[1] + k2 * R[2] + k3 * R[3]) / gk;
+  const double c = 0.5 * (sqrt(kGamma * L[4] / L[0]) + sqrt(kGamma * R[4] / R[0]));
+  const double Ml = ul / c, Mr = ur / c;
+  constexpr double alpha = 3 / 16.0;
+  double mlp, plp, mrn, prn;
+  if (fabs(Ml) > 1) {
+    mlp = 0.5 * (Ml + fabs(Ml));
+    plp = mlp / Ml;
+  } else {
+    const double a = (Ml + 1) * (Ml + 1) * 0.25, bb = (Ml * Ml - 1) * (Ml * Ml - 1);
+    mlp = a + 0.125 * bb;
+    plp = a * (2 - Ml) + alpha * Ml * bb;
+  }
+  if (fabs(Mr) > 1) {
+    mrn = 0.5 * (Mr - fabs(Mr));
+    prn = mrn / Mr;
+  } else {
+    const double a = (Mr - 1) * (Mr - 1) * 0.25, bb = (Mr * Mr - 1) * (Mr * Mr - 1);
+    mrn = -a - 0.125 * bb;
+    prn = a * (2 + Mr) - alpha * Mr * bb;
+  }
+  const double pc = plp * L[4] + prn * R[4];
+  const double mh = mlp + mrn;
+  const double mass = c * (L[0] * 0.5 * (mh + fabs(mh)) + R[0] * 0.5 * (mh - fabs(mh)));
+  const double co = mass * gk;
+  const double *S = mass >= 0 ? L : R;
+  fc[0] = co;
+  fc[1] = co * S[1] + pc * k1;
+  fc[2] = co * S[2] + pc * k2;
+  fc[3] = co * S[3] + pc * k3;
+  fc[4] = co * (S[5] + S[4]) / S[0];
+}
+
+// HLLC with Roe-averaged wave-speed bounds (InviscidScheme.cu:471-599, single species).
+__device__ __forceinline__ void flux_hllc(const double *L, const double *R, const double *ml, const double *mr,
+                                          double jl, double jr, double *fc) {
+  const double sql = sqrt(L[0]), sqr = sqrt(R[0]);
+  const double wl = sql / (sql + sqr), wr = sqr / (sql + sqr);
+  const double ut = L[1] * wl + R[1] * wr, vt = L[2] * wl + R[2] * wr, wt = L[3] * wl + R[3] * wr;
+  const double ht = (L[5] + L[4]) / L[0] * wl + (R[5] + R[4]) / R[0] * wr;
+  const double ct = sqrt((kGamma - 1) * (ht - 0.5 * (ut * ut + vt * vt + wt * wt)));
+  const double kx = 0.5 * (ml[0] + mr[0]), ky = 0.5 * (ml[1] + mr[1]), kz = 0.5 * (ml[2] + mr[2]);
+  const double gk = sqrt(kx * kx + ky * ky + kz * kz);
+  const double Ut = (kx * ut + ky * vt + kz * wt) / gk;
+  const double ja = 0.5 * (jl + jr);
+  const double Ul = kx * L[1] + ky * L[2] + kz * L[3];
+  const double Ur = kx * R[1] + ky * R[2] + kz * R[3];
+  const double sl = fmin(Ul / gk - sqrt(kGamma * L[4] / L[0]), Ut - ct);
+  const double sr = fmax(Ur / gk + sqrt(kGamma * R[4] / R[0]), Ut + ct);
+  if (sl >= 0 || sr < 0) { // fully supersonic: upwind physical flux
+    const double *S = sl >= 0 ? L : R;
+    const double Us = sl >= 0 ? Ul : Ur;
+    const double m = S[0] * Us;
+    fc[0] = ja * m;
+    fc[1] = ja * (m * S[1] + S[4] * kx);
+    fc[2] = ja * (m * S[2] + S[4] * ky);
+    fc[3] = ja * (m * S[3] + S[4] * kz);
+    fc[4] = ja * ((S[5] + S[4]) * Us);
+    return;
+  }
+  const double dl = sl - Ul / gk, dr = sr - Ur / gk;
+  const double sm = ((R[0] * Ur * dr - L[0] * Ul * dl) / gk + L[4] - R[4]) / (R[0] * dr - L[0] * dl);
+  const double pm = L[0] * dl * (sm - Ul / gk) + L[4];
+  const bool left = sm >= 0;
+  const double *S = left ? L : R;
+  const double ss = left ? sl : sr, Us = left ? Ul : Ur;
+  const double pco = 1.0 / (ss - sm);
+  const double Qc = ja * pco * sm * (ss * gk - Us) * S[0];
+  const double dP = (ss * pm - sm * S[4]) * pco * ja;
+  fc[0] = Qc;
+  fc[1] = Qc * S[1] + dP * kx;
+  fc[2] = Qc * S[2] + dP * ky;
+  fc[3] = Qc * S[3] + dP * kz;
+  fc[4] = Qc * S[5] / S[0] + pco * ja * (ss * pm * sm * gk - sm * S[4] * Us);
+}
+
+// Roe flux with Harten's entropy fix (InviscidScheme.cu:601-765, single species).
+__device__ __forceinline__ void flux_roe(const double *L, const double *R, const double *ml, const double *mr,
+                                         double jl, double jr, double efl, double efr, double *fc) {
+  {
+    const double ax = jl * ml[0], ay = jl * ml[1], az = jl * ml[2];
+    const double Uk = L[1] * ax + L[2] * ay + L[3] * az, co = Uk * L[0];
+    fc[0] = 0.5 * co;
+    fc[1] = 0.5 * (co * L[1] + L[4] * ax);
+    fc[2] = 0.5 * (co * L[2] + L[4] * ay);
+    fc[3] = 0.5 * (co * L[3] + L[4] * az);
+    fc[4] = 0.5 * Uk * (L[4] + L[5]);
+  }
+  {
+    const double ax = jr * mr[0], ay = jr * mr[1], az = jr * mr[2];
+    const double Uk = R[1] * ax + R[2] * ay + R[3] * az, co = Uk * R[0];
+    fc[0] += 0.5 * co;
+    fc[1] += 0.5 * (co * R[1] + R[4] * ax);
+    fc[2] += 0.5 * (co * R[2] + R[4] * ay);
+    fc[3] += 0.5 * (co * R[3] + R[4] * az);
+    fc[4] += 0.5 * Uk * (R[4] + R[5]);
+  }
+  const double dl = sqrt(L[0]), dr = sqrt(R[0]);
+  const double iv = 1.0 / (dl + dr);
+  const double u = (dl * L[1] + dr * R[1]) * iv, v = (dl * L[2] + dr * R[2]) * iv, w = (dl * L[3] + dr * R[3]) * iv;
+  const double ek = 0.5 * (u * u + v * v + w * w);
+  const double h = (dl * ((L[5] + L[4]) / L[0]) + dr * ((R[5] + R[4]) / R[0])) * iv;
+  const double c = sqrt((kGamma - 1) * (h - ek));
+  double kx = 0.5 * (ml[0] + mr[0]), ky = 0.5 * (ml[1] + mr[1]), kz = 0.5 * (ml[2] + mr[2]);
+  const double gk = sqrt(kx * kx + ky * ky + kz * kz);
+  double Uk = kx * u + ky * v + kz * w;
+  const double efa = 0.5 * (efl + efr);
+  double lam[3] = {fabs(Uk - gk * c), fabs(Uk), fabs(Uk + gk * c)};
+#pragma unroll
+  for (int m = 0; m < 3; ++m)
+    if (lam[m] < efa) lam[m] = 0.5 * (lam[m] * lam[m] / efa + efa);
+  kx /= gk;
+  ky /= gk;
+  kz /= gk;
+  Uk /= gk;
+  const double ja = 0.5 * (jl + jr);
+  const double q0 = ja * (R[0] - L[0]);
+  const double q1 = ja * (R[0] * R[1] - L[0] * L[1]);
+  const double q2 = ja * (R[0] * R[2] - L[0] * L[2]);
+  const double q3 = ja * (R[0] * R[3] - L[0] * L[3]);
+  const double q4 = ja * (R[5] - L[5]);
+  double c1 = (kGamma - 1) * (ek * q0 - u * q1 - v * q2 - w * q3 + q4) / (c * c);
+  double c2 = (kx * q1 + ky * q2 + kz * q3 - Uk * q0) / c;
+  const double c3 = q0 - c1;
+  const double b0 = -lam[0] * 0.5 * (c1 - c2);
+  const double b1 = -lam[1] * (kx * c3 - ((kz * v - ky * w) * q0 - kz * q2 + ky * q3) / c);
+  const double b2 = -lam[1] * (ky * c3 - ((kx * w - kz * u) * q0 - kx * q3 + kz * q1) / c);
+  const double b3 = -lam[1] * (kz * c3 - ((ky * u - kx * v) * q0 - ky * q1 + kx * q2) / c);
+  const double b4 = -lam[2] * 0.5 * (c1 + c2);
+  const double c0 = kx * b1 + ky * b2 + kz * b3;
+  c1 = c0 + b0 + b4;
+  c2 = c * (b4 - b0);
+  fc[0] += 0.5 * c1;
+  fc[1] += 0.5 * (u * c1 + kx * c2 - c * (kz * b2 - ky * b3));
+  fc[2] += 0.5 * (v * c1 + ky * c2 - c * (kx * b3 - kz * b1));
+  fc[3] += 0.5 * (w * c1 + kz * c2 - c * (ky * b1 - kx * b2));
+  fc[4] += 0.5 * (h * c1 + Uk * c2 - c * c * c0 / (kGamma - 1) +
+                  c * ((kz * v - ky * w) * b1 + (kx * w - kz * u) * b2 + (ky * u - kx * v) * b3));
+}
+
+// Local Lax-Friedrichs (InviscidScheme.cu:772-805).
+__device__ __forceinline__ void flux_lf(const double *L, const double *R, const double *ml, const double *mr,
+                                        double jl, double jr, double *fc) {
+  const double kx = 0.5 * (ml[0] + mr[0]), ky = 0.5 * (ml[1] + mr[1]), kz = 0.5 * (ml[2] + mr[2]);
+  const double gk = sqrt(kx * kx + ky * ky + kz * kz);
+  const double Ul = L[1] * kx + L[2] * ky + L[3] * kz, Ur = R[1] * kx + R[2] * ky + R[3] * kz;
+  const double sp = fmax(fabs(Ul) + sqrt(kGamma * L[4] / L[0]) * gk, fabs(Ur) + sqrt(kGamma * R[4] / R[0]) * gk);
+  const double hj = 0.25 * (jl + jr);
+  const double ml_ = L[0] * Ul, mr_ = R[0] * Ur;
+  fc[0] = (ml_ + mr_ - sp * (R[0] - L[0])) * hj;
+  fc[1] = (ml_ * L[1] + mr_ * R[1] + kx * (L[4] + R[4]) - sp * (R[1] * R[0] - L[1] * L[0])) * hj;
+  fc[2] = (ml_ * L[2] + mr_ * R[2] + ky * (L[4] + R[4]) - sp * (R[2] * R[0] - L[2] * L[0])) * hj;
+  fc[3] = (ml_ * L[3] + mr_ * R[3] + kz * (L[4] + R[4]) - sp * (R[3] * R[0] - L[3] * L[0])) * hj;
+  fc[4] = ((L[5] + L[4]) * Ul + (R[5] + R[4]) * Ur - sp * (R[5] - L[5])) * hj;
+}
+
+__device__ __forceinline__ void to_primitive(const double *q, double *p) {
+  const double ri = 1.0 / q[0];
+  p[0] = q[0];
+  p[1] = q[1] * ri;
+  p[2] = q[2] * ri;
+  p[3] = q[3] * ri;
+  p[4] = (kGamma - 1) * (q[4] - 0.5 * (q[1] * p[1] + q[2] * p[2] + q[3] * p[3]));
+  p[5] = q[4];
+}
+
+// ---------------------------------------------------------------- interface flux kernel
+// One thread per interface t+1/2 along DIR.  The stencil q[0..2R-1] holds conservative states of
+// nodes t-R+1 .. t+R.  Characteristic fields use the left eigenvectors of the Roe average of nodes
+// t and t+1 (rows as in WENO.cu:73-124), reconstruction is one-sided WENO in those fields,
+// back-projection is the explicit R of WENO.cu:229-243, then the chosen Riemann solver.
+template<int DIR, int R, int RS>
+__global__ __launch_bounds__(256) void k_inviscid(DevBlock b, Scheme sc, double *F) {
+  int i, j, k;
+  if constexpr (DIR == 0) {
+    i = blockIdx.x * 64 + threadIdx.x - 1;
+    j = blockIdx.y * 4 + threadIdx.y;
+    k = blockIdx.z;
+    if (i >= b.mx || j >= b.my) return;
+  } else if constexpr (DIR == 1) {
+    i = blockIdx.x * 64 + threadIdx.x;
+    j = blockIdx.y * 4 + threadIdx.y - 1;
+    k = blockIdx.z;
+    if (i >= b.mx || j >= b.my) return;
+  } else {
+    i = blockIdx.x * 64 + threadIdx.x;
+    j = blockIdx.y * 4 + threadIdx.y;
+    k = blockIdx.z - 1;
+    if (i >= b.mx || j >= b.my) return;
+  }
+  const long st = DIR == 0 ? 1 : (DIR == 1 ? b.d2 : b.d1);
+  const long id = nid(b, i, j, k);
+  const long sz = b.sz;
+  constexpr int N = 2 * R;
+  double w[N][5];
+#pragma unroll
+  for (int m = 0; m < N; ++m)
+#pragma unroll
+    for (int l = 0; l < 5; ++l) w[m][l] = b.cv[id + (m - R + 1) * st + l * sz];
+
+  double ml[3], mr[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    ml[c] = b.met[id + (3 * DIR + c) * sz];
+    mr[c] = b.met[id + st + (3 * DIR + c) * sz];
+  }
+  const double jl = b.jac[id], jr = b.jac[id + st];
+
+  // Roe average of the two interface nodes
+  const double *qa = w[R - 1], *qb = w[R];
+  const double ra = qa[0], rb = qb[0];
+  const double sab = sqrt(ra * rb);
+  const double ca = 1 / (ra + sab), cb = 1 / (sab + rb);
+  const double um = ca * qa[1] + cb * qb[1], vm = ca * qa[2] + cb * qb[2], wm = ca * qa[3] + cb * qb[3];
+  const double pa = (kGamma - 1) * (qa[4] - 0.5 * (qa[1] * qa[1] + qa[2] * qa[2] + qa[3] * qa[3]) / ra);
+  const double pb = (kGamma - 1) * (qb[4] - 0.5 * (qb[1] * qb[1] + qb[2] * qb[2] + qb[3] * qb[3]) / rb);
+  const double H = ca * (qa[4] + pa) + cb * (qb[4] + pb);
+  const double ek = 0.5 * (um * um + vm * vm + wm * wm);
+  const double c = sqrt((kGamma - 1) * (H - ek));
+  double kx = 0.5 * (ml[0] + mr[0]), ky = 0.5 * (ml[1] + mr[1]), kz = 0.5 * (ml[2] + mr[2]);
+  {
+    const double r = 1.0 / sqrt(kx * kx + ky * ky + kz * kz);
+    kx *= r; ky *= r; kz *= r;
+  }
+  double nx = 0, ny = 0, nz = 0;
+  if (fabs(kx) > 0.8) {
+    nx = -kz; nz = kx;
+  } else {
+    ny = kz; nz = -ky;
+  }
+  {
+    const double r = 1.0 / sqrt(nx * nx + ny * ny + nz * nz);
+    nx *= r; ny *= r; nz *= r;
+  }
+  double qx = ky * nz - kz * ny, qy = kz * nx - kx * nz, qz = kx * ny - ky * nx;
+  {
+    const double r = 1.0 / sqrt(qx * qx + qy * qy + qz * qz);
+    qx *= r; qy *= r; qz *= r;
+  }
+  const double Uk = kx * um + ky * vm + kz * wm;
+  const double Un = nx * um + ny * vm + nz * wm;
+  const double Uq = qx * um + qy * vm + qz * wm;
+  const double gm1 = kGamma - 1, ib = 1.0 / (c * c), al = gm1 * ek;
+  double Lm[5][5] = {
+      {(al + Uk * c) * ib * 0.5, -(gm1 * um + kx * c) * ib * 0.5, -(gm1 * vm + ky * c) * ib * 0.5,
+       -(gm1 * wm + kz * c) * ib * 0.5, gm1 * ib * 0.5},
+      {-Un, nx, ny, nz, 0.0},
+      {-Uq, qx, qy, qz, 0.0},
+      {1 - al * ib, gm1 * um * ib, gm1 * vm * ib, gm1 * wm * ib, -gm1 * ib},
+      {(al - Uk * c) * ib * 0.5, -(gm1 * um - kx * c) * ib * 0.5, -(gm1 * vm - ky * c) * ib * 0.5,
+       -(gm1 * wm - kz * c) * ib * 0.5, gm1 * ib * 0.5}};
+  // project in place: w[m] <- L q[m]
+#pragma unroll
+  for (int m = 0; m < N; ++m) {
+    double t[5];
+#pragma unroll
+    for (int l = 0; l < 5; ++l)
+      t[l] = Lm[l][0] * w[m][0] + Lm[l][1] * w[m][1] + Lm[l][2] * w[m][2] + Lm[l][3] * w[m][3] + Lm[l][4] * w[m][4];
+#pragma unroll
+    for (int l = 0; l < 5; ++l) w[m][l] = t[l];
+  }
+  double WL[5], WR[5];
+  const double eps = sc.weno_eps;
+#pragma unroll
+  for (int l = 0; l < 5; ++l) {
+    if constexpr (R == 4) {
+      const double a[7] = {w[0][l], w[1][l], w[2][l], w[3][l], w[4][l], w[5][l], w[6][l]};
+      const double bb[7] = {w[7][l], w[6][l], w[5][l], w[4][l], w[3][l], w[2][l], w[1][l]};
+      WL[l] = weno7_half(a, eps);
+      WR[l] = weno7_half(bb, eps);
+    } else {
+      WL[l] = weno5_half(w[0][l], w[1][l], w[2][l], w[3][l], w[4][l], eps);
+      WR[l] = weno5_half(w[5][l], w[4][l], w[3][l], w[2][l], w[1][l], eps);
+    }
+  }
+  // back-projection Q = R W
+  auto back = [&](const double *W, double *Q) {
+    const double s = W[0] + W[3] + W[4], d = W[0] - W[4];
+    Q[0] = s;
+    Q[1] = um * s - kx * c * d + nx * W[1] + qx * W[2];
+    Q[2] = vm * s - ky * c * d + ny * W[1] + qy * W[2];
+    Q[3] = wm * s - kz * c * d + nz * W[1] + qz * W[2];
+    Q[4] = H * s - Uk * c * d + Un * W[1] + Uq * W[2] - c * c / gm1 * W[3];
+  };
+  double QL[5], QR[5], PL[6], PR[6];
+  back(WL, QL);
+  back(WR, QR);
+  to_primitive(QL, PL);
+  to_primitive(QR, PR);
+  if (!(PL[0] > 0 && PL[4] > 0 && PR[0] > 0 && PR[4] > 0)) {
+    // positivity fallback: first-order states of the two interface nodes
+    const double Qa[5] = {b.cv[id], b.cv[id + sz], b.cv[id + 2 * sz], b.cv[id + 3 * sz], b.cv[id + 4 * sz]};
+    const long ib2 = id + st;
+    const double Qb[5] = {b.cv[ib2], b.cv[ib2 + sz], b.cv[ib2 + 2 * sz], b.cv[ib2 + 3 * sz], b.cv[ib2 + 4 * sz]};
+    to_primitive(Qa, PL);
+    to_primitive(Qb, PR);
+  }
+  double fc[5];
+  if constexpr (RS == 1) flux_lf(PL, PR, ml, mr, jl, jr, fc);
+  else if constexpr (RS == 2) flux_roe(PL, PR, ml, mr, jl, jr, b.efd[id], b.efd[id + st], fc);
+  else if constexpr (RS == 4) flux_hllc(PL, PR, ml, mr, jl, jr, fc);
+  else flux_ausm(PL, PR, ml, mr, jl, jr, fc);
+#pragma unroll
+  for (int l = 0; l < 5; ++l) F[id + l * sz] = fc[l];
+}
+
+// ---------------------------------------------------------------- viscous terms
+// Collocated first derivative along one index (ViscousScheme.cu:129-307 for ORDER 8; ORDER 4 uses the
+// reference's fourth-order rows: the i==2 row as the interior stencil and the i==0/1 rows as closures).
+template<int ORDER>
+__device__ __forceinline__ double dd(const double *f, long id, long st, int i, int n, int lb, int rb) {
+#define FF(o) f[id + (o) * st]
+  if (lb && i < ORDER / 2) {
+    if (i == 0) return -25.0 / 12 * FF(0) + 4.0 * FF(1) - 3 * FF(2) + 4.0 / 3 * FF(3) - 0.25 * FF(4);
+    if (i == 1) return -0.25 * FF(-1) - 5.0 / 6 * FF(0) + 1.5 * FF(1) - 0.5 * FF(2) + 1.0 / 12 * FF(3);
+    if (ORDER == 8) {
+      if (i == 2) return -1.0 / 12 * (FF(2) - FF(-2)) + 2.0 / 3 * (FF(1) - FF(-1));
+      return 0.75 * (FF(1) - FF(-1)) - 0.15 * (FF(2) - FF(-2)) + 1.0 / 60 * (FF(3) - FF(-3));
+    }
+  }
+  if (rb && i >= n - ORDER / 2) {
+    if (i == n - 1) return 25.0 / 12 * FF(0) - 4.0 * FF(-1) + 3 * FF(-2) - 4.0 / 3 * FF(-3) + 0.25 * FF(-4);
+    if (i == n - 2) return 0.25 * FF(1) + 5.0 / 6 * FF(0) - 1.5 * FF(-1) + 0.5 * FF(-2) - 1.0 / 12 * FF(-3);
+    if (ORDER == 8) {
+      if (i == n - 3) return -1.0 / 12 * (FF(2) - FF(-2)) + 2.0 / 3 * (FF(1) - FF(-1));
+      return 0.75 * (FF(1) - FF(-1)) - 0.15 * (FF(2) - FF(-2)) + 1.0 / 60 * (FF(3) - FF(-3));
+    }
+  }
+  if constexpr (ORDER == 8)
+    return 0.8 * (FF(1) - FF(-1)) + -0.2 * (FF(2) - FF(-2)) + 4.0 / 105 * (FF(3) - FF(-3)) +
+           -1.0 / 280 * (FF(4) - FF(-4));
+  else
+    return -1.0 / 12 * (FF(2) - FF(-2)) + 2.0 / 3 * (FF(1) - FF(-1));
+#undef FF
+}
+
+__device__ __forceinline__ double sutherland(double T) {
+  return 1.7894e-5 * pow(T / 288.16, 1.5) * (288.16 + 110) / (T + 110); // Transport.cu:6-9
+}
+
+// compute_viscous_flux_collocated<Air, ORDER> (ViscousScheme.cu:309-408): node fluxes fv/gv/hv[1..4].
+template<int ORDER>
+__global__ __launch_bounds__(256) void k_viscous_flux(DevBlock b, Scheme sc) {
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+  if (i >= b.mx || j >= b.my) return;
+  const long id = nid(b, i, j, k), sz = b.sz;
+  double g[4][3];
+  const int comp[4] = {1, 2, 3, 5};
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const double *f = b.bv + comp[q] * sz;
+    g[q][0] = dd<ORDER>(f, id, 1, i, b.mx, b.bt[0], b.bt[1]);
+    g[q][1] = dd<ORDER>(f, id, b.d2, j, b.my, b.bt[2], b.bt[3]);
+    g[q][2] = b.dim == 3 ? dd<ORDER>(f, id, b.d1, k, b.mz, b.bt[4], b.bt[5]) : 0.0;
+  }
+  const double *M = b.met + id;
+  const double xx = M[0], xy = M[sz], xz = M[2 * sz], ex = M[3 * sz], ey = M[4 * sz], ez = M[5 * sz];
+  const double zx = M[6 * sz], zy = M[7 * sz], zz = M[8 * sz];
+  double d[4][3]; // physical gradients of u, v, w, T
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    d[q][0] = g[q][0] * xx + g[q][1] * ex + g[q][2] * zx;
+    d[q][1] = g[q][0] * xy + g[q][1] * ey + g[q][2] * zy;
+    d[q][2] = g[q][0] * xz + g[q][1] * ez + g[q][2] * zz;
+  }
+  const double T = b.bv[id + 5 * sz];
+  const double mu = sutherland(T);
+  const double txx = mu * (4 * d[0][0] - 2 * d[1][1] - 2 * d[2][2]) / 3.0;
+  const double tyy = mu * (4 * d[1][1] - 2 * d[0][0] - 2 * d[2][2]) / 3.0;
+  const double tzz = mu * (4 * d[2][2] - 2 * d[0][0] - 2 * d[1][1]) / 3.0;
+  const double txy = mu * (d[0][1] + d[1][0]), txz = mu * (d[0][2] + d[2][0]), tyz = mu * (d[1][2] + d[2][1]);
+  constexpr double cp = kGamma * kRair / (kGamma - 1);
+  const double kc = mu / sc.prandtl * cp;
+  const double u = b.bv[id + sz], v = b.bv[id + 2 * sz], w = b.bv[id + 3 * sz];
+  const double Ex = u * txx + v * txy + w * txz + kc * d[3][0];
+  const double Ey = u * txy + v * tyy + w * tyz + kc * d[3][1];
+  const double Ez = u * txz + v * tyz + w * tzz + kc * d[3][2];
+  double *fv = b.fv + id, *gv = b.gv + id, *hv = b.hv + id;
+  fv[sz] = xx * txx + xy * txy + xz * txz;
+  fv[2 * sz] = xx * txy + xy * tyy + xz * tyz;
+  fv[3 * sz] = xx * txz + xy * tyz + xz * tzz;
+  fv[4 * sz] = xx * Ex + xy * Ey + xz * Ez;
+  gv[sz] = ex * txx + ey * txy + ez * txz;
+  gv[2 * sz] = ex * txy + ey * tyy + ez * tyz;
+  gv[3 * sz] = ex * txz + ey * tyz + ez * tzz;
+  gv[4 * sz] = ex * Ex + ey * Ey + ez * Ez;
+  hv[sz] = zx * txx + zy * txy + zz * txz;
+  hv[2 * sz] = zx * txy + zy * tyy + zz * tyz;
+  hv[3 * sz] = zx * txz + zy * tyz + zz * tzz;
+  hv[4 * sz] = zx * Ex + zy * Ey + zz * Ez;
+}
+
+// periodic_exchange task 1 (DataCommunication.cu:628-683): average coincident face nodes, copy ghosts.
+__global__ void k_viscous_exchange(DevBlock b, DevFace f) {
+  const int i = f.start[0] + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = f.start[1] + blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = f.start[2] + blockIdx.z * blockDim.z + threadIdx.z;
+  if (i > f.end[0] || j > f.end[1] || k > f.end[2]) return;
+  int o[3] = {i, j, k};
+  const int ext[3] = {b.mx, b.my, b.mz};
+  o[f.face] = f.direction < 0 ? ext[f.face] - 1 : 0;
+  int dv[3] = {0, 0, 0};
+  dv[f.face] = f.direction;
+  const long id = nid(b, i, j, k), io = nid(b, o[0], o[1], o[2]), sz = b.sz;
+  double *A[3] = {b.fv, b.gv, b.hv};
+  for (int l = 0; l < 5; ++l)
+    for (int a = 0; a < 3; ++a) {
+      const double av = 0.5 * (A[a][id + l * sz] + A[a][io + l * sz]);
+      A[a][id + l * sz] = av;
+      A[a][io + l * sz] = av;
+    }
+  for (int g = 1; g <= b.g; ++g) {
+    const long ig = nid(b, i + g * dv[0], j + g * dv[1], k + g * dv[2]);
+    const long is = nid(b, o[0] + g * dv[0], o[1] + g * dv[1], o[2] + g * dv[2]);
+    for (int l = 0; l < 5; ++l)
+      for (int a = 0; a < 3; ++a) A[a][ig + l * sz] = A[a][is + l * sz];
+  }
+}
+
+// compute_entropy_fix_delta<Air> (InviscidScheme.cu:269-304) on the interior plus one layer.
+__global__ __launch_bounds__(256) void k_entropy_fix(DevBlock b, Scheme sc) {
+  const int i = blockIdx.x * 64 + threadIdx.x - 1, j = blockIdx.y * 4 + threadIdx.y - 1;
+  const int k = b.dim == 3 ? (int)blockIdx.z - 1 : 0;
+  if (i > b.mx || j > b.my) return;
+  const long id = nid(b, i, j, k), sz = b.sz;
+  const double *M = b.met + id;
+  const double u = b.bv[id + sz], v = b.bv[id + 2 * sz], w = b.bv[id + 3 * sz];
+  const double U = fabs(u * M[0] + v * M[sz] + w * M[2 * sz]);
+  const double V = fabs(u * M[3 * sz] + v * M[4 * sz] + w * M[5 * sz]);
+  const double W = fabs(u * M[6 * sz] + v * M[7 * sz] + w * M[8 * sz]);
+  const double gx = sqrt(M[0] * M[0] + M[sz] * M[sz] + M[2 * sz] * M[2 * sz]);
+  const double ge = sqrt(M[3 * sz] * M[3 * sz] + M[4 * sz] * M[4 * sz] + M[5 * sz] * M[5 * sz]);
+  const double gz = sqrt(M[6 * sz] * M[6 * sz] + M[7 * sz] * M[7 * sz] + M[8 * sz] * M[8 * sz]);
+  const double c = sqrt(kGamma * b.bv[id + 4 * sz] / b.bv[id]);
+  b.efd[id] = b.dim == 2 ? sc.entropy_fix_factor * (U + V + c * 0.5 * (gx + ge))
+                         : sc.entropy_fix_factor * (U + V + W + c * (gx + ge + gz) / 3.0);
+}
+
+// ---------------------------------------------------------------- residual + RK stage
+__device__ __forceinline__ void cons_from_prim(const DevBlock &b, long id) {
+  const long sz = b.sz;
+  const double rho = b.bv[id], u = b.bv[id + sz], v = b.bv[id + 2 * sz], w = b.bv[id + 3 * sz];
+  b.cv[id] = rho;
+  b.cv[id + sz] = rho * u;
+  b.cv[id + 2 * sz] = rho * v;
+  b.cv[id + 3 * sz] = rho * w;
+  b.cv[id + 4 * sz] = 0.5 * (u * u + v * v + w * w) * rho + b.bv[id + 4 * sz] / (kGamma - 1);
+}
+
+__constant__ double kRkA[3] = {1.0, 0.75, 1.0 / 3.0}; // SSPRK3 (RK.cuh:11-15)
+__constant__ double kRkB[3] = {0.0, 0.25, 2.0 / 3.0};
+__constant__ double kRkC[3] = {1.0, 0.25, 2.0 / 3.0};
+
+// dq = J (dFv/dxi + dGv/deta + dHv/dzeta) - sum_dir (F(i+1/2) - F(i-1/2)), then
+// update_cv_and_bv_rk<Air> (RK.cuh:224-280).  Stage 0 also saves U^n into qn.
+template<int ORDER, bool RHS_ONLY>
+__global__ __launch_bounds__(256) void k_update(DevBlock b, int rk, const double *__restrict__ dtp) {
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+  if (i >= b.mx || j >= b.my) return;
+  const long id = nid(b, i, j, k), sz = b.sz;
+  double r[5];
+  r[0] = 0;
+  if constexpr (ORDER > 0) {
+    const double J = b.jac[id];
+#pragma unroll
+    for (int l = 1; l < 5; ++l) {
+      double s = dd<ORDER>(b.fv + l * sz, id, 1, i, b.mx, b.bt[0], b.bt[1]) +
+                 dd<ORDER>(b.gv + l * sz, id, b.d2, j, b.my, b.bt[2], b.bt[3]);
+      if (b.dim == 3) s += dd<ORDER>(b.hv + l * sz, id, b.d1, k, b.mz, b.bt[4], b.bt[5]);
+      r[l] = s * J;
+    }
+  } else {
+#pragma unroll
+    for (int l = 1; l < 5; ++l) r[l] = 0;
+  }
+#pragma unroll
+  for (int l = 0; l < 5; ++l) r[l] -= b.fx[id + l * sz] - b.fx[id - 1 + l * sz];
+#pragma unroll
+  for (int l = 0; l < 5; ++l) r[l] -= b.fy[id + l * sz] - b.fy[id - b.d2 + l * sz];
+  if (b.dim == 3) {
+#pragma unroll
+    for (int l = 0; l < 5; ++l) r[l] -= b.fz[id + l * sz] - b.fz[id - b.d1 + l * sz];
+  }
+  if constexpr (RHS_ONLY) {
+    const long iq = ((long)k * b.my + j) * b.mx + i;
+#pragma unroll
+    for (int l = 0; l < 5; ++l) b.dq[iq + l * b.n] = r[l];
+    return;
+  }
+  const double dj = dtp[0] / b.jac[id];
+  double q[5];
+  if (rk == 0) {
+#pragma unroll
+    for (int l = 0; l < 5; ++l) {
+      const double u0 = b.cv[id + l * sz];
+      b.qn[id + l * sz] = u0;
+      q[l] = kRkA[0] * u0 + kRkB[0] * u0 + kRkC[0] * dj * r[l];
+    }
+  } else {
+#pragma unroll
+    for (int l = 0; l < 5; ++l)
+      q[l] = kRkA[rk] * b.qn[id + l * sz] + kRkB[rk] * b.cv[id + l * sz] + kRkC[rk] * dj * r[l];
+  }
+  if (b.mz == 1) q[3] = 0;
+#pragma unroll
+  for (int l = 0; l < 5; ++l) b.cv[id + l * sz] = q[l];
+  const double di = 1.0 / q[0];
+  const double u = q[1] * di, v = q[2] * di, w = q[3] * di;
+  const double p = (kGamma - 1) * (q[4] - 0.5 * q[0] * (u * u + v * v + w * w));
+  b.bv[id] = q[0];
+  b.bv[id + sz] = u;
+  b.bv[id + 2 * sz] = v;
+  b.bv[id + 3 * sz] = w;
+  b.bv[id + 4 * sz] = p;
+  b.bv[id + 5 * sz] = p * kMwAir * di / kRu;
+}
+
+// ---------------------------------------------------------------- boundary conditions
+// apply_outflow without back pressure (BoundCond.cu:2065-2188) and apply_periodic (BoundCond.cu:3033-3083).
+__global__ void k_boundary(DevBlock b, DevFace f) {
+  const int i = f.start[0] + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = f.start[1] + blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = f.start[2] + blockIdx.z * blockDim.z + threadIdx.z;
+  if (i > f.end[0] || j > f.end[1] || k > f.end[2]) return;
+  int dv[3] = {0, 0, 0};
+  dv[f.face] = f.direction;
+  const long id = nid(b, i, j, k), sz = b.sz;
+  if (f.type == 8) {
+    int o[3] = {i, j, k};
+    const int ext[3] = {b.mx, b.my, b.mz};
+    o[f.face] = f.direction < 0 ? ext[f.face] - 1 : 0;
+    const long io = nid(b, o[0], o[1], o[2]);
+    for (int l = 0; l < 6; ++l) {
+      const double a = 0.5 * (b.bv[id + l * sz] + b.bv[io + l * sz]);
+      b.bv[id + l * sz] = a;
+      b.bv[io + l * sz] = a;
+    }
+    cons_from_prim(b, id);
+    for (int g = 1; g <= b.g; ++g) {
+      const long ig = nid(b, i + g * dv[0], j + g * dv[1], k + g * dv[2]);
+      const long is = nid(b, o[0] + g * dv[0], o[1] + g * dv[1], o[2] + g * dv[2]);
+      for (int l = 0; l < 6; ++l) b.bv[ig + l * sz] = b.bv[is + l * sz];
+      cons_from_prim(b, ig);
+    }
+  } else { // zero-gradient outflow
+    for (int g = 1; g <= b.g; ++g) {
+      const long ig = nid(b, i + g * dv[0], j + g * dv[1], k + g * dv[2]);
+      for (int l = 0; l < 6; ++l) b.bv[ig + l * sz] = b.bv[id + l * sz];
+      cons_from_prim(b, ig);
+    }
+  }
+}
+
+__global__ void k_cv_from_bv(DevBlock b) {
+  const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= b.sz) return;
+  cons_from_prim(b, id);
+}
+
+// ---------------------------------------------------------------- CFL time step
+// local_time_step<Air> (TimeAdvanceFunc.cuh:28-115) fused with a workgroup minimum.
+__global__ __launch_bounds__(256) void k_local_dt(DevBlock b, Scheme sc, double *part) {
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+  double dt = 1e+6;
+  if (i < b.mx && j < b.my) {
+    const long id = nid(b, i, j, k), sz = b.sz;
+    const double *M = b.met + id;
+    const double gx = sqrt(M[0] * M[0] + M[sz] * M[sz] + M[2 * sz] * M[2 * sz]);
+    const double ge = sqrt(M[3 * sz] * M[3 * sz] + M[4 * sz] * M[4 * sz] + M[5 * sz] * M[5 * sz]);
+    const double gz = sqrt(M[6 * sz] * M[6 * sz] + M[7 * sz] * M[7 * sz] + M[8 * sz] * M[8 * sz]);
+    const double u = b.bv[id + sz], v = b.bv[id + 2 * sz], w = b.bv[id + 3 * sz];
+    const double T = b.bv[id + 5 * sz];
+    const double c = sqrt(kGamma * kRair * T);
+    double s = fmax(fabs(u * M[0] + v * M[sz] + w * M[2 * sz]) + c * gx,
+                    fabs(u * M[3 * sz] + v * M[4 * sz] + w * M[5 * sz]) + c * ge);
+    if (b.dim == 3) s = fmax(s, fabs(u * M[6 * sz] + v * M[7 * sz] + w * M[8 * sz]) + c * gz);
+    if (sc.viscous_order > 0) {
+      double ml = fmax(gx, ge);
+      if (b.dim == 3) ml = fmax(ml, gz);
+      const double dv = fmax(kGamma, 4.0 / 3.0) / b.bv[id] * (sutherland(T) / sc.prandtl);
+      s = fmax(s, ml * ml * dv);
+    }
+    dt = sc.cfl / s;
+  }
+  // wave64 minimum via cross-lane shuffles, then across the 4 waves through LDS
+  for (int off = 32; off > 0; off >>= 1) dt = fmin(dt, __shfl_xor(dt, off, 64));
+  __shared__ double wmin[4];
+  if (threadIdx.x == 0) wmin[threadIdx.y] = dt;
+  __syncthreads();
+  if (threadIdx.x == 0 && threadIdx.y == 0) {
+    const long p = ((long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+    part[p] = fmin(fmin(wmin[0], wmin[1]), fmin(wmin[2], wmin[3]));
+  }
+}
+
+__global__ __launch_bounds__(1024) void k_dt_reduce(const double *part, long n, double *dt, const double *time,
+                                                    double t_end, int clip) {
+  double m = 1e+6;
+  for (long p = threadIdx.x; p < n; p += blockDim.x) m = fmin(m, part[p]);
+  for (int off = 32; off > 0; off >>= 1) m = fmin(m, __shfl_xor(m, off, 64));
+  __shared__ double wmin[16];
+  const int wave = threadIdx.x / 64;
+  if ((threadIdx.x & 63) == 0) wmin[wave] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r = wmin[0];
+    for (int q = 1; q < (int)(blockDim.x / 64); ++q) r = fmin(r, wmin[q]);
+    if (clip && time[0] + r > t_end) r = t_end - time[0];
+    dt[0] = r;
+  }
+}
+
+__global__ void k_time_advance(double *time, const double *dt) { time[0] += dt[0]; }
+
+dim3 node_grid(const DevBlock &b) { return dim3((b.mx + 63) / 64, (b.my + 3) / 4, b.mz); }
+
+dim3 face_grid(const DevFace &f, dim3 &tpb, int ngg) {
+  unsigned t[3], g[3];
+  for (int d = 0; d < 3; ++d) {
+    const int n = f.end[d] - f.start[d] + 1;
+    t[d] = n <= 2 * ngg + 1 ? 1 : 16;
+    g[d] = (n - 1) / t[d] + 1;
+  }
+  // keep the workgroup at most 256 lanes (16 x 16 x 1 when two tangential extents are large)
+  if (t[0] * t[1] * t[2] > 256) t[2] = 1, g[2] = f.end[2] - f.start[2] + 1;
+  tpb = dim3(t[0], t[1], t[2]);
+  return dim3(g[0], g[1], g[2]);
+}
+
+} // namespace
+
+hipError_t launch_cv_from_bv_all(const DevBlock &b, hipStream_t s) {
+  k_cv_from_bv<<<(unsigned)((b.sz + 255) / 256), 256, 0, s>>>(b);
+  return hipGetLastError();
+}
+
+hipError_t launch_boundary(const DevBlock &b, const DevFace &f, hipStream_t s) {
+  dim3 tpb;
+  const dim3 grid = face_grid(f, tpb, b.g);
+  k_boundary<<<grid, tpb, 0, s>>>(b, f);
+  return hipGetLastError();
+}
+
+hipError_t launch_viscous_flux(const DevBlock &b, const Scheme &sc, hipStream_t s) {
+  const dim3 tpb(64, 4, 1);
+  if (sc.viscous_order == 8) k_viscous_flux<8><<<node_grid(b), tpb, 0, s>>>(b, sc);
+  else k_viscous_flux<4><<<node_grid(b), tpb, 0, s>>>(b, sc);
+  return hipGetLastError();
+}
+
+hipError_t launch_viscous_exchange(const DevBlock &b, const DevFace &f, hipStream_t s) {
+  dim3 tpb;
+  const dim3 grid = face_grid(f, tpb, b.g);
+  k_viscous_exchange<<<grid, tpb, 0, s>>>(b, f);
+  return hipGetLastError();
+}
+
+hipError_t launch_entropy_fix(const DevBlock &b, const Scheme &sc, hipStream_t s) {
+  const dim3 grid((b.mx + 2 + 63) / 64, (b.my + 2 + 3) / 4, b.dim == 3 ? b.mz + 2 : 1);
+  k_entropy_fix<<<grid, dim3(64, 4, 1), 0, s>>>(b, sc);
+  return hipGetLastError();
+}
+
+template<int DIR, int R>
+static void dispatch_rs(const DevBlock &b, const Scheme &sc, dim3 grid, double *F, hipStream_t s) {
+  const dim3 tpb(64, 4, 1);
+  switch (sc.inviscid_scheme) {
+    case 1: k_inviscid<DIR, R, 1><<<grid, tpb, 0, s>>>(b, sc, F); break;
+    case 2: k_inviscid<DIR, R, 2><<<grid, tpb, 0, s>>>(b, sc, F); break;
+    case 4: k_inviscid<DIR, R, 4><<<grid, tpb, 0, s>>>(b, sc, F); break;
+    default: k_inviscid<DIR, R, 3><<<grid, tpb, 0, s>>>(b, sc, F); break;
+  }
+}
+
+hipError_t launch_inviscid_flux(const DevBlock &b, const Scheme &sc, int dir, hipStream_t s) {
+  dim3 grid;
+  double *F;
+  if (dir == 0) {
+    grid = dim3((b.mx + 1 + 63) / 64, (b.my + 3) / 4, b.mz);
+    F = b.fx;
+  } else if (dir == 1) {
+    grid = dim3((b.mx + 63) / 64, (b.my + 1 + 3) / 4, b.mz);
+    F = b.fy;
+  } else {
+    grid = dim3((b.mx + 63) / 64, (b.my + 3) / 4, b.mz + 1);
+    F = b.fz;
+  }
+  const bool w7 = sc.reconstruction == 5;
+  if (dir == 0) w7 ? dispatch_rs<0, 4>(b, sc, grid, F, s) : dispatch_rs<0, 3>(b, sc, grid, F, s);
+  else if (dir == 1) w7 ? dispatch_rs<1, 4>(b, sc, grid, F, s) : dispatch_rs<1, 3>(b, sc, grid, F, s);
+  else w7 ? dispatch_rs<2, 4>(b, sc, grid, F, s) : dispatch_rs<2, 3>(b, sc, grid, F, s);
+  return hipGetLastError();
+}
+
+hipError_t launch_update(const DevBlock &b, const Scheme &sc, int rk, const double *dt_dev, bool rhs_only,
+                         hipStream_t s) {
+  const dim3 tpb(64, 4, 1), grid = node_grid(b);
+  const int o = sc.viscous_order;
+  if (rhs_only) {
+    if (o == 8) k_update<8, true><<<grid, tpb, 0, s>>>(b, rk, dt_dev);
+    else if (o == 4) k_update<4, true><<<grid, tpb, 0, s>>>(b, rk, dt_dev);
+    else k_update<0, true><<<grid, tpb, 0, s>>>(b, rk, dt_dev);
+  } else {
+    if (o == 8) k_update<8, false><<<grid, tpb, 0, s>>>(b, rk, dt_dev);
+    else if (o == 4) k_update<4, false><<<grid, tpb, 0, s>>>(b, rk, dt_dev);
+    else k_update<0, false><<<grid, tpb, 0, s>>>(b, rk, dt_dev);
+  }
+  return hipGetLastError();
+}
+
+int dt_partial_count(const DevBlock &b) {
+  const dim3 g = node_grid(b);
+  return (int)(g.x * g.y * g.z);
+}
+
+hipError_t launch_local_dt(const DevBlock &b, const Scheme &sc, double *part, hipStream_t s) {
+  k_local_dt<<<node_grid(b), dim3(64, 4, 1), 0, s>>>(b, sc, part);
+  return hipGetLastError();
+}
+
+hipError_t launch_dt_reduce(const double *part, long n, double *dt_dev, const double *time_dev, double t_end,
+                            int clip, hipStream_t s) {
+  k_dt_reduce<<<1, 1024, 0, s>>>(part, n, dt_dev, time_dev, t_end, clip);
+  return hipGetLastError();
+}
+
+hipError_t launch_time_advance(double *time_dev, const double *dt_dev, hipStream_t s) {
+  k_time_advance<<<1, 1, 0, s>>>(time_dev, dt_dev);
+  return hipGetLastError();
+}
+
+} // namespace corefl

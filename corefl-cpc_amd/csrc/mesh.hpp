@@ -1,0 +1,44 @@
+// Host-side mesh: grid and boundary readers, mirrored ghost grid, 2nd-order metrics.
+// Reference: Mesh.cpp:244-528 (metrics), 692-925 (readers), 1013-1167 (ghost grid).
+#pragma once
+
+#include <string>
+#include <vector>
+
+namespace corefl {
+
+// One boundary face (Boundary, Mesh.h:8-31) after register_boundary (Mesh.cpp:14-32).
+struct Face {
+  int start[3], end[3];
+  int type;      // 6 outflow, 8 periodic, ...
+  int face;      // normal direction 0/1/2
+  int direction; // -1 on the small-index face, +1 otherwise
+};
+
+struct HostBlock {
+  int mx, my, mz, ngg, dim;
+  // coordinates with ngg+1 ghost layers (i fastest)
+  std::vector<double> x, y, z;
+  // metrics/Jacobian in the ngg-ghost DZone layout: metric[l*sz + idx], jac[idx]
+  std::vector<double> jac, metric;
+  std::vector<Face> faces;
+  // physical-boundary flags per side (il, ir, jl, jr, kl, kr) for one-sided viscous stencils
+  int btype[6] = {0, 0, 0, 0, 0, 0};
+
+  long pad2() const { return mx + 2L * ngg; }
+  long pad1() const { return (my + 2L * ngg) * pad2(); }
+  long size() const { return pad1() * (mz + 2L * ngg); }
+  long idx(int i, int j, int k) const { return (k + ngg) * pad1() + (j + ngg) * pad2() + (i + ngg); }
+};
+
+// Parse grid%4d.dat content (int32 n_block; n_block*(mx,my,mz); x,y,z float64 per block, i fastest).
+std::vector<HostBlock> read_grid_binary(const std::string &path, int ngg, double grid_scale);
+// Build blocks from in-memory coordinates (same ordering as the binary file).
+HostBlock make_block(int mx, int my, int mz, int ngg, const double *x, const double *y, const double *z,
+                     double grid_scale);
+// Parse boundary%4d.txt content for all blocks (Mesh.cpp:837-925).
+void parse_boundaries(const std::string &text, std::vector<HostBlock> &blocks, int periodic_label);
+// Ghost grid and metrics; returns false on a non-positive interior Jacobian.
+bool compute_metrics(HostBlock &b, std::string &err);
+
+} // namespace corefl

@@ -1,0 +1,115 @@
+/*
+ * oracle.h -- CPU restatement of COREFL's per-step right-hand side and SSP-RK3 time advance.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the parity checker for the HIP product in
+ * corefl-cpc_amd/.  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+ * load it, and only as the checker or as the timed CPU baseline -- never as a code path of
+ * the product.
+ *
+ * Every routine cites the reference file:line it restates (paths relative to the
+ * Liangerty/COREFL-CPC source tree).  Arithmetic is FP64 throughout, compiled with
+ * -ffp-contract=off so that no fused multiply-add is introduced where the reference has none.
+ *
+ * Array layout (the reference's DZone/VectorField3D layout, gxl_lib/Array.cuh:233-328):
+ *   f(i,j,k,l) = data[(k+g)*d1 + (j+g)*d2 + (i+g) + l*sz],  d2 = mx+2g, d1 = (my+2g)*d2,
+ *   sz = (mx+2g)(my+2g)(mz+2g), g = ngg.  dq has no ghosts (Field.cu:524).
+ *
+ * Pinning status per function is stated next to each declaration; see the test files in tests/
+ * that exercise each check.
+ */
+#ifndef COREFL_ORACLE_H
+#define COREFL_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Scheme selection and constants the path reads from DParameter (DParameter.cuh:13-207). */
+typedef struct {
+  int inviscid_scheme;   /* 1 LF, 2 Roe, 3 AUSM+, 4 HLLC (Parameter.cpp:218-260)                 */
+  int reconstruction;    /* 4 WENO5 / 5 WENO7 characteristic reconstruction (new pairing)         */
+  int viscous_order;     /* 0 inviscid, 4 (new), 8 collocated (ViscousScheme.cu:7-73)            */
+  double prandtl;        /* prandtl_number, default 0.72                                          */
+  double cfl;            /* cfl                                                                   */
+  double entropy_fix_factor; /* default 0.125 (Parameter.cpp:380)                                 */
+  double weno_eps;       /* eps_weno * weno_eps_scale = 1e-40 (rho_inf v_inf)^2 (WENO.cu:8,884)   */
+} orc_params;
+
+/* ---- point functions (pure; parity pinned by tests/test_oracle_points.py) ---- */
+
+/* WENO.cu:3059-3114.  Returns fPlus(vp)+fMinus(vm); if_shock=0 gives the linear upwind weights.  */
+double orc_weno5(const double *vp, const double *vm, double eps, int if_shock);
+/* WENO.cu:3116-3222 (Z-type tau7 weights). */
+double orc_weno7(const double *vp, const double *vm, double eps, int if_shock);
+
+/* Left/right characteristic matrices at an interface from the Roe average of the two
+ * conservative states ql, qr (5 each) and the (unnormalised) averaged metric k[3].
+ * L rows follow WENO.cu:73-124 (weno_ch_air_x); R is the explicit back-projection of
+ * WENO.cu:229-243 with H = c^2/(gamma-1) + |u|^2/2 so that R*L = I.  Outputs row-major 5x5.   */
+void orc_char_matrices(const double *ql, const double *qr, const double *k, double *L, double *R);
+
+/* Riemann fluxes on reconstructed primitive states pv = (rho,u,v,w,p,E).
+ * scheme: 1 LF (InviscidScheme.cu:772-805), 2 Roe (601-765), 3 AUSM+ (397-469), 4 HLLC (471-599).
+ * ml/mr: the 3 metric components of the sweep direction at the two nodes; jl/jr: Jacobians;
+ * efl/efr: Roe entropy-fix deltas at the two nodes (InviscidScheme.cu:269-304).               */
+void orc_riemann(int scheme, const double *pl, const double *pr, const double *ml, const double *mr,
+                 double jl, double jr, double efl, double efr, int n_dim_unused, double *fc);
+
+/* Interface flux of the product's characteristic WENO + Riemann path for one interface from a
+ * stencil of conservative states q[m][5], m = 0..n-1 (n = 6 for WENO5, 8 for WENO7; the
+ * interface lies between m = n/2-1 and n/2).  Returns 1 if the positivity fallback fired.      */
+int orc_interface_flux(const orc_params *p, const double (*q)[5], int n, const double *ml,
+                       const double *mr, double jl, double jr, double efl, double efr, double *fc);
+
+/* Sutherland's law, Transport.cu:6-9. */
+double orc_sutherland(double T);
+
+/* ---- mesh (Mesh.cpp) ---- */
+
+/* Mirror ghost coordinates (Mesh.cpp:1013-1167, ngg+1 layers) and 2nd-order metrics/Jacobian
+ * (Mesh.cpp:244-528) for one block.  x,y,z: raw coordinates, mx*my*mz, i fastest, already
+ * scaled by gridScale.  jac (sz) and metric (9*sz) are written in the ngg-ghost layout.
+ * Returns 0, or -1 if an interior Jacobian is non-positive (the reference aborts).            */
+int orc_metrics(int mx, int my, int mz, int ngg, const double *x, const double *y, const double *z,
+                double *jac, double *metric);
+
+/* ---- one block, full step ---- */
+
+typedef struct orc_block orc_block;
+
+/* faces: n_face rows of (i1,i2,j1,j2,k1,k2,type) as in boundary%4d.txt (Mesh.cpp:837-925);
+ * types 6 (outflow) and 8 (periodic) are supported. */
+orc_block *orc_block_create(const orc_params *p, int mx, int my, int mz, int ngg, const int *faces,
+                            int n_face, const double *jac, const double *metric);
+void orc_block_destroy(orc_block *b);
+/* Set primitive variables bv (6*sz, ghosts included); computes cv over all nodes
+ * (FieldOperation.cuh:39-64) and mul (FieldOperation.cuh:86-120), as initialize_computation does
+ * after the boundary conditions (Driver.cu:126-211: properties -> BCs -> properties).          */
+void orc_block_set_bv(orc_block *b, const double *bv);
+/* Overwrite bv (6*sz) and cv (5*sz) on every node as given, with no boundary fill: used to restart the
+ * oracle from another implementation's exact state. */
+void orc_block_set_raw(orc_block *b, const double *bv, const double *cv);
+/* Copy out an array: 0 cv, 1 bv, 2 dq, 3 mul, 4 jac, 5 metric, 6 fv, 7 gv, 8 hv. */
+void orc_block_get(const orc_block *b, int which, double *out);
+/* One RHS evaluation dq on the current state (RK.cuh:137-157 without the update). */
+void orc_block_rhs(orc_block *b);
+/* CFL time step over the interior (TimeAdvanceFunc.cuh:28-115 + TimeAdvanceFunc.cu:102-152). */
+double orc_block_cfl_dt(orc_block *b);
+/* One SSP-RK3 step with time step dt (RK.cuh:100-192 for one block). */
+void orc_block_rk3_step(orc_block *b, double dt);
+/* Number of OpenMP threads the oracle uses (for the CPU baseline report). */
+int orc_num_threads(void);
+
+/* ---- exact solution ---- */
+
+/* Exact Riemann solution of the 1D Euler equations (Toro, "Riemann Solvers and Numerical
+ * Methods for Fluid Dynamics", ch. 4: pressure function + Newton iteration, then sampling).
+ * Pinned by example/sodShockTube/exact.dat. Returns the star pressure. */
+double orc_exact_riemann(double rl, double ul, double pl, double rr, double ur, double pr,
+                         double gamma, double x0, double t, int n, const double *x, double *rho,
+                         double *u, double *p);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

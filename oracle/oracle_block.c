@@ -1,0 +1,679 @@
+/*
+ * oracle_block.c -- mesh metrics and the per-block SSP-RK3 step of the oracle
+ * (TEST INFRASTRUCTURE ONLY; see oracle.h).
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+static const double kGamma = 1.4;
+static const double kRu = 8314.462618;
+static const double kMwAir = 28.9647;
+
+int orc_num_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+/* ---------------- mesh: ghost grid + metrics ---------------- */
+
+typedef struct {
+  int mx, my, mz, g; /* g = number of ghost layers of this array */
+  long d2, d1;
+  double *v;
+} arr3;
+
+static inline double *A3(arr3 *a, int i, int j, int k) {
+  return a->v + (long)(k + a->g) * a->d1 + (long)(j + a->g) * a->d2 + (i + a->g);
+}
+
+static void arr3_init(arr3 *a, int mx, int my, int mz, int g) {
+  a->mx = mx; a->my = my; a->mz = mz; a->g = g;
+  a->d2 = mx + 2 * g;
+  a->d1 = (long)(my + 2 * g) * a->d2;
+  a->v = (double *)calloc((size_t)a->d1 * (mz + 2 * g), sizeof(double));
+}
+
+/* Mesh.cpp:1013-1167 (single block, no inner/parallel faces) */
+static void ghost_grid(arr3 *X, arr3 *Y, arr3 *Z, int ngg, int dim) {
+  const int mx = X->mx, my = X->my, mz = X->mz;
+  arr3 *c[3] = {X, Y, Z};
+  for (int q = 0; q < 3; ++q) {
+    arr3 *B = c[q];
+    for (int k = 0; k < mz; ++k)
+      for (int j = 0; j < my; ++j) {
+        for (int i = -1; i >= -ngg - 1; --i) *A3(B, i, j, k) = 2 * *A3(B, 0, j, k) - *A3(B, -i, j, k);
+        for (int i = mx; i < mx + ngg + 1; ++i)
+          *A3(B, i, j, k) = 2 * *A3(B, mx - 1, j, k) - *A3(B, 2 * (mx - 1) - i, j, k);
+      }
+    for (int k = 0; k < mz; ++k)
+      for (int i = 0; i < mx; ++i) {
+        for (int j = -1; j >= -ngg - 1; --j) *A3(B, i, j, k) = 2 * *A3(B, i, 0, k) - *A3(B, i, -j, k);
+        for (int j = my; j < my + ngg + 1; ++j)
+          *A3(B, i, j, k) = 2 * *A3(B, i, my - 1, k) - *A3(B, i, 2 * (my - 1) - j, k);
+      }
+    if (dim == 3) {
+      for (int j = 0; j < my; ++j)
+        for (int i = 0; i < mx; ++i) {
+          for (int k = -1; k >= -ngg - 1; --k) *A3(B, i, j, k) = 2 * *A3(B, i, j, 0) - *A3(B, i, j, -k);
+          for (int k = mz; k < mz + ngg + 1; ++k)
+            *A3(B, i, j, k) = 2 * *A3(B, i, j, mz - 1) - *A3(B, i, j, 2 * (mz - 1) - k);
+        }
+    } else {
+      for (int j = 0; j < my; ++j)
+        for (int i = 0; i < mx; ++i) {
+          for (int k = -1; k >= -ngg - 1; --k) *A3(B, i, j, k) = *A3(B, i, j, 0) + (q == 2 ? k : 0);
+          for (int k = mz; k < mz + ngg + 1; ++k) *A3(B, i, j, k) = *A3(B, i, j, mz - 1) + (q == 2 ? k - mz + 1 : 0);
+        }
+    }
+  }
+  /* edges and corners by the parallelogram rule */
+  for (int q = 0; q < 3; ++q) {
+    arr3 *B = c[q];
+    for (int i = 1; i <= ngg + 1; ++i)
+      for (int j = 1; j <= ngg + 1; ++j)
+        for (int k = 0; k < mz; ++k) {
+          const int i1 = -i, j1 = -j, i2 = mx - 1 + i, j2 = my - 1 + j;
+          *A3(B, i1, j1, k) = *A3(B, i1, 0, k) + *A3(B, 0, j1, k) - *A3(B, 0, 0, k);
+          *A3(B, i2, j2, k) = *A3(B, i2, my - 1, k) + *A3(B, mx - 1, j2, k) - *A3(B, mx - 1, my - 1, k);
+          *A3(B, i1, j2, k) = *A3(B, i1, my - 1, k) + *A3(B, 0, j2, k) - *A3(B, 0, my - 1, k);
+          *A3(B, i2, j1, k) = *A3(B, i2, 0, k) + *A3(B, mx - 1, j1, k) - *A3(B, mx - 1, 0, k);
+        }
+    for (int i = 1; i <= ngg + 1; ++i)
+      for (int j = 0; j < my; ++j)
+        for (int k = 1; k <= ngg + 1; ++k) {
+          const int i1 = -i, k1 = -k, i2 = mx - 1 + i, k2 = mz - 1 + k;
+          *A3(B, i1, j, k1) = *A3(B, i1, j, 0) + *A3(B, 0, j, k1) - *A3(B, 0, j, 0);
+          *A3(B, i2, j, k2) = *A3(B, i2, j, mz - 1) + *A3(B, mx - 1, j, k2) - *A3(B, mx - 1, j, mz - 1);
+          *A3(B, i1, j, k2) = *A3(B, i1, j, mz - 1) + *A3(B, 0, j, k2) - *A3(B, 0, j, mz - 1);
+          *A3(B, i2, j, k1) = *A3(B, i2, j, 0) + *A3(B, mx - 1, j, k1) - *A3(B, mx - 1, j, 0);
+        }
+    for (int i = -ngg - 1; i <= mx + ngg; ++i)
+      for (int j = 1; j <= ngg + 1; ++j)
+        for (int k = 1; k <= ngg + 1; ++k) {
+          const int j1 = -j, k1 = -k, j2 = my - 1 + j, k2 = mz - 1 + k;
+          *A3(B, i, j1, k1) = *A3(B, i, j1, 0) + *A3(B, i, 0, k1) - *A3(B, i, 0, 0);
+          *A3(B, i, j2, k2) = *A3(B, i, j2, mz - 1) + *A3(B, i, my - 1, k2) - *A3(B, i, my - 1, mz - 1);
+          *A3(B, i, j1, k2) = *A3(B, i, j1, mz - 1) + *A3(B, i, 0, k2) - *A3(B, i, 0, mz - 1);
+          *A3(B, i, j2, k1) = *A3(B, i, j2, 0) + *A3(B, i, my - 1, k1) - *A3(B, i, my - 1, 0);
+        }
+  }
+}
+
+typedef struct {
+  arr3 *X, *Y, *Z;
+  double *jac, *met;
+  long sz, d2, d1;
+  int g;
+} metric_ctx;
+
+/* One node of Mesh.cpp:255-293; returns 0 if jac <= 0 (caller applies the fallback). */
+static int metric_point(metric_ctx *m, int i, int j, int k) {
+  arr3 *X = m->X, *Y = m->Y, *Z = m->Z;
+  const double dxd1 = (*A3(X, i + 1, j, k) - *A3(X, i - 1, j, k)) * 0.5;
+  const double dyd1 = (*A3(Y, i + 1, j, k) - *A3(Y, i - 1, j, k)) * 0.5;
+  const double dzd1 = (*A3(Z, i + 1, j, k) - *A3(Z, i - 1, j, k)) * 0.5;
+  const double dxd2 = (*A3(X, i, j + 1, k) - *A3(X, i, j - 1, k)) * 0.5;
+  const double dyd2 = (*A3(Y, i, j + 1, k) - *A3(Y, i, j - 1, k)) * 0.5;
+  const double dzd2 = (*A3(Z, i, j + 1, k) - *A3(Z, i, j - 1, k)) * 0.5;
+  const double dxd3 = (*A3(X, i, j, k + 1) - *A3(X, i, j, k - 1)) * 0.5;
+  const double dyd3 = (*A3(Y, i, j, k + 1) - *A3(Y, i, j, k - 1)) * 0.5;
+  const double dzd3 = (*A3(Z, i, j, k + 1) - *A3(Z, i, j, k - 1)) * 0.5;
+  const double jac = dxd1 * dyd2 * dzd3 + dxd2 * dyd3 * dzd1 + dxd3 * dyd1 * dzd2 - dxd1 * dyd3 * dzd2 -
+                     dxd2 * dyd1 * dzd3 - dxd3 * dyd2 * dzd1;
+  if (jac <= 0) return 0;
+  const long id = (long)(k + m->g) * m->d1 + (long)(j + m->g) * m->d2 + (i + m->g);
+  m->jac[id] = jac;
+  double *M = m->met;
+  M[id + 0 * m->sz] = (dyd2 * dzd3 - dyd3 * dzd2) / jac;
+  M[id + 1 * m->sz] = (dxd3 * dzd2 - dxd2 * dzd3) / jac;
+  M[id + 2 * m->sz] = (dxd2 * dyd3 - dxd3 * dyd2) / jac;
+  M[id + 3 * m->sz] = (dyd3 * dzd1 - dyd1 * dzd3) / jac;
+  M[id + 4 * m->sz] = (dxd1 * dzd3 - dxd3 * dzd1) / jac;
+  M[id + 5 * m->sz] = (dxd3 * dyd1 - dxd1 * dyd3) / jac;
+  M[id + 6 * m->sz] = (dyd1 * dzd2 - dyd2 * dzd1) / jac;
+  M[id + 7 * m->sz] = (dxd2 * dzd1 - dxd1 * dzd2) / jac;
+  M[id + 8 * m->sz] = (dxd1 * dyd2 - dxd2 * dyd1) / jac;
+  return 1;
+}
+
+static void metric_copy(metric_ctx *m, int i, int j, int k, int si, int sj, int sk) {
+  const long id = (long)(k + m->g) * m->d1 + (long)(j + m->g) * m->d2 + (i + m->g);
+  const long is = (long)(sk + m->g) * m->d1 + (long)(sj + m->g) * m->d2 + (si + m->g);
+  m->jac[id] = m->jac[is];
+  for (int l = 0; l < 9; ++l) m->met[id + l * m->sz] = m->met[is + l * m->sz];
+}
+
+int orc_metrics(int mx, int my, int mz, int ngg, const double *x, const double *y, const double *z, double *jac,
+                double *metric) {
+  const int dim = mz == 1 ? 2 : 3;
+  arr3 X, Y, Z;
+  arr3_init(&X, mx, my, mz, ngg + 1);
+  arr3_init(&Y, mx, my, mz, ngg + 1);
+  arr3_init(&Z, mx, my, mz, ngg + 1);
+  for (int k = 0; k < mz; ++k)
+    for (int j = 0; j < my; ++j)
+      for (int i = 0; i < mx; ++i) {
+        const long s = ((long)k * my + j) * mx + i;
+        *A3(&X, i, j, k) = x[s];
+        *A3(&Y, i, j, k) = y[s];
+        *A3(&Z, i, j, k) = z[s];
+      }
+  ghost_grid(&X, &Y, &Z, ngg, dim);
+  metric_ctx m = {&X, &Y, &Z, jac, metric, 0, mx + 2 * ngg, 0, ngg};
+  m.d1 = (long)(my + 2 * ngg) * m.d2;
+  m.sz = m.d1 * (mz + 2 * ngg);
+  int rc = 0;
+  /* interior, Mesh.cpp:246-294 */
+  const int kmin = dim == 2 ? 0 : 1, kmax = dim == 2 ? 1 : mz - 1;
+  for (int i = 1; i < mx - 1; ++i)
+    for (int j = 1; j < my - 1; ++j)
+      for (int k = kmin; k < kmax; ++k)
+        if (!metric_point(&m, i, j, k)) rc = -1;
+  /* the six boundary blocks, Mesh.cpp:300-528, with the reference's neighbour fallback */
+  for (int i = 1; i < mx - 1; ++i)
+    for (int j = 1; j < my - 1; ++j) {
+      for (int k = 0; k >= -ngg; --k)
+        if (!metric_point(&m, i, j, k)) metric_copy(&m, i, j, k, i, j, k + 1);
+    }
+  for (int i = 1; i < mx - 1; ++i)
+    for (int j = 1; j < my - 1; ++j)
+      for (int k = mz - 1; k < mz + ngg; ++k)
+        if (!metric_point(&m, i, j, k)) metric_copy(&m, i, j, k, i, j, k - 1);
+  for (int i = 1; i < mx - 1; ++i)
+    for (int j = 0; j >= -ngg; --j)
+      for (int k = -ngg; k < mz + ngg; ++k)
+        if (!metric_point(&m, i, j, k)) metric_copy(&m, i, j, k, i, j + 1, k);
+  for (int i = 1; i < mx - 1; ++i)
+    for (int j = my - 1; j < my + ngg; ++j)
+      for (int k = -ngg; k < mz + ngg; ++k)
+        if (!metric_point(&m, i, j, k)) metric_copy(&m, i, j, k, i, j - 1, k);
+  for (int i = 0; i >= -ngg; --i)
+    for (int j = -ngg; j < my + ngg; ++j)
+      for (int k = -ngg; k < mz + ngg; ++k)
+        if (!metric_point(&m, i, j, k)) metric_copy(&m, i, j, k, i + 1, j, k);
+  for (int i = mx - 1; i < mx + ngg; ++i)
+    for (int j = -ngg; j < my + ngg; ++j)
+      for (int k = -ngg; k < mz + ngg; ++k)
+        if (!metric_point(&m, i, j, k)) metric_copy(&m, i, j, k, i - 1, j, k);
+  free(X.v);
+  free(Y.v);
+  free(Z.v);
+  return rc;
+}
+
+/* ---------------- block state and step ---------------- */
+
+typedef struct {
+  int face, dir, type;
+  int s[3], e[3];
+} face_t;
+
+struct orc_block {
+  orc_params p;
+  int mx, my, mz, g, dim;
+  long d2, d1, sz, n;
+  face_t faces[64];
+  int n_face;
+  int bt[6]; /* physical-boundary flags per side for the one-sided viscous closures */
+  double *cv, *bv, *qn, *dq, *mul, *jac, *met, *fv, *gv, *hv, *efd;
+};
+
+#define ID(b, i, j, k) ((long)((k) + (b)->g) * (b)->d1 + (long)((j) + (b)->g) * (b)->d2 + ((i) + (b)->g))
+#define IDQ(b, i, j, k) (((long)(k) * (b)->my + (j)) * (b)->mx + (i))
+
+orc_block *orc_block_create(const orc_params *p, int mx, int my, int mz, int ngg, const int *faces, int n_face,
+                            const double *jac, const double *metric) {
+  orc_block *b = (orc_block *)calloc(1, sizeof(orc_block));
+  b->p = *p;
+  b->mx = mx; b->my = my; b->mz = mz; b->g = ngg;
+  b->dim = mz == 1 ? 2 : 3;
+  b->d2 = mx + 2 * ngg;
+  b->d1 = (long)(my + 2 * ngg) * b->d2;
+  b->sz = b->d1 * (mz + 2 * ngg);
+  b->n = (long)mx * my * mz;
+  /* Boundary::register_boundary (Mesh.cpp:14-32) with ngg = 0; bType flags (Mesh.cpp:857-916) */
+  b->n_face = n_face;
+  for (int f = 0; f < n_face; ++f) {
+    const int *r = faces + 7 * f;
+    face_t *F = &b->faces[f];
+    F->s[0] = r[0]; F->e[0] = r[1];
+    F->s[1] = r[2]; F->e[1] = r[3];
+    F->s[2] = r[4]; F->e[2] = r[5];
+    F->type = r[6];
+    F->face = 0;
+    F->dir = 1;
+    for (int d = 0; d < b->dim; ++d)
+      if (F->s[d] == F->e[d]) {
+        F->face = d;
+        if (F->s[d] == 0) F->dir = -1;
+        break;
+      }
+    for (int d = 0; d < b->dim; ++d) {
+      if (d == F->face) continue;
+      if (F->e[d] < F->s[d]) { const int t = F->e[d]; F->e[d] = F->s[d]; F->s[d] = t; }
+    }
+    if (b->dim == 2) { F->s[2] = 0; F->e[2] = 0; }
+    if (F->type != 8) b->bt[2 * F->face + (F->dir > 0 ? 1 : 0)] = F->type;
+  }
+  b->cv = calloc((size_t)5 * b->sz, sizeof(double));
+  b->qn = calloc((size_t)5 * b->sz, sizeof(double));
+  b->bv = calloc((size_t)6 * b->sz, sizeof(double));
+  b->fv = calloc((size_t)5 * b->sz, sizeof(double));
+  b->gv = calloc((size_t)5 * b->sz, sizeof(double));
+  b->hv = calloc((size_t)5 * b->sz, sizeof(double));
+  b->mul = calloc((size_t)b->sz, sizeof(double));
+  b->efd = calloc((size_t)b->sz, sizeof(double));
+  b->dq = calloc((size_t)5 * b->n, sizeof(double));
+  b->jac = malloc((size_t)b->sz * sizeof(double));
+  b->met = malloc((size_t)9 * b->sz * sizeof(double));
+  memcpy(b->jac, jac, (size_t)b->sz * sizeof(double));
+  memcpy(b->met, metric, (size_t)9 * b->sz * sizeof(double));
+  return b;
+}
+
+void orc_block_destroy(orc_block *b) {
+  if (!b) return;
+  free(b->cv); free(b->qn); free(b->bv); free(b->fv); free(b->gv); free(b->hv);
+  free(b->mul); free(b->efd); free(b->dq); free(b->jac); free(b->met);
+  free(b);
+}
+
+/* FieldOperation.cuh:16-64 (Air) */
+static void cv_from_bv(orc_block *b, long id) {
+  const long s = b->sz;
+  const double rho = b->bv[id], u = b->bv[id + s], v = b->bv[id + 2 * s], w = b->bv[id + 3 * s];
+  b->cv[id] = rho;
+  b->cv[id + s] = rho * u;
+  b->cv[id + 2 * s] = rho * v;
+  b->cv[id + 3 * s] = rho * w;
+  double e = 0.5 * (u * u + v * v + w * w);
+  e *= rho;
+  e += b->bv[id + 4 * s] / (kGamma - 1);
+  b->cv[id + 4 * s] = e;
+}
+
+/* FieldOperation.cuh:86-120 (Air): mul over all nodes */
+static void update_properties(orc_block *b) {
+#pragma omp parallel for
+  for (long id = 0; id < b->sz; ++id) b->mul[id] = orc_sutherland(b->bv[id + 5 * b->sz]);
+}
+
+/* BoundCond.cu:2065-2188 without back pressure */
+static void apply_outflow(orc_block *b, const face_t *F) {
+  int d[3] = {0, 0, 0};
+  d[F->face] = F->dir;
+  for (int k = F->s[2]; k <= F->e[2]; ++k)
+    for (int j = F->s[1]; j <= F->e[1]; ++j)
+      for (int i = F->s[0]; i <= F->e[0]; ++i) {
+        const long id = ID(b, i, j, k);
+        for (int g = 1; g <= b->g; ++g) {
+          const long ig = ID(b, i + g * d[0], j + g * d[1], k + g * d[2]);
+          for (int l = 0; l < 6; ++l) b->bv[ig + l * b->sz] = b->bv[id + l * b->sz];
+          cv_from_bv(b, ig);
+        }
+      }
+}
+
+/* BoundCond.cu:3033-3083 */
+static void apply_periodic(orc_block *b, const face_t *F) {
+  int d[3] = {0, 0, 0};
+  d[F->face] = F->dir;
+  const int ext[3] = {b->mx, b->my, b->mz};
+  for (int k = F->s[2]; k <= F->e[2]; ++k)
+    for (int j = F->s[1]; j <= F->e[1]; ++j)
+      for (int i = F->s[0]; i <= F->e[0]; ++i) {
+        int o[3] = {i, j, k};
+        o[F->face] = F->dir < 0 ? ext[F->face] - 1 : 0;
+        const long id = ID(b, i, j, k), io = ID(b, o[0], o[1], o[2]);
+        for (int l = 0; l < 6; ++l) {
+          const double a = 0.5 * (b->bv[id + l * b->sz] + b->bv[io + l * b->sz]);
+          b->bv[id + l * b->sz] = a;
+          b->bv[io + l * b->sz] = a;
+        }
+        cv_from_bv(b, id);
+        for (int g = 1; g <= b->g; ++g) {
+          const long ig = ID(b, i + g * d[0], j + g * d[1], k + g * d[2]);
+          const long is = ID(b, o[0] + g * d[0], o[1] + g * d[1], o[2] + g * d[2]);
+          for (int l = 0; l < 6; ++l) b->bv[ig + l * b->sz] = b->bv[is + l * b->sz];
+          cv_from_bv(b, ig);
+        }
+      }
+}
+
+/* DBoundCond::apply_boundary_conditions order: outflow faces, then periodic faces (BoundCond.cu:3489-3619) */
+static void apply_bcs(orc_block *b) {
+  for (int f = 0; f < b->n_face; ++f)
+    if (b->faces[f].type == 6) apply_outflow(b, &b->faces[f]);
+  for (int f = 0; f < b->n_face; ++f)
+    if (b->faces[f].type == 8) apply_periodic(b, &b->faces[f]);
+}
+
+/* periodic_exchange task 1 for the viscous fluxes (DataCommunication.cu:628-683) */
+static void exchange_viscous_flux(orc_block *b) {
+  const int ext[3] = {b->mx, b->my, b->mz};
+  for (int f = 0; f < b->n_face; ++f) {
+    const face_t *F = &b->faces[f];
+    if (F->type != 8) continue;
+    int d[3] = {0, 0, 0};
+    d[F->face] = F->dir;
+    for (int k = F->s[2]; k <= F->e[2]; ++k)
+      for (int j = F->s[1]; j <= F->e[1]; ++j)
+        for (int i = F->s[0]; i <= F->e[0]; ++i) {
+          int o[3] = {i, j, k};
+          o[F->face] = F->dir < 0 ? ext[F->face] - 1 : 0;
+          const long id = ID(b, i, j, k), io = ID(b, o[0], o[1], o[2]);
+          double *A[3] = {b->fv, b->gv, b->hv};
+          for (int l = 0; l < 5; ++l)
+            for (int a = 0; a < 3; ++a) {
+              const double av = 0.5 * (A[a][id + l * b->sz] + A[a][io + l * b->sz]);
+              A[a][id + l * b->sz] = av;
+              A[a][io + l * b->sz] = av;
+            }
+          for (int g = 1; g <= b->g; ++g) {
+            const long ig = ID(b, i + g * d[0], j + g * d[1], k + g * d[2]);
+            const long is = ID(b, o[0] + g * d[0], o[1] + g * d[1], o[2] + g * d[2]);
+            for (int l = 0; l < 5; ++l)
+              for (int a = 0; a < 3; ++a) A[a][ig + l * b->sz] = A[a][is + l * b->sz];
+          }
+        }
+  }
+}
+
+/* d/dxi of f along a direction with stride st at index i of extent n (ViscousScheme.cu:129-177 for
+ * ORDER 8; ORDER 4 uses the reference's 4th-order closures as its interior and near-boundary rows). */
+static double deriv(const double *f, long id, long st, int i, int n, int left, int right, int order) {
+#define F(o) f[id + (long)(o) * st]
+  if (order == 8) {
+    if (left) {
+      if (i == 0) return -25.0 / 12 * F(0) + 4.0 * F(1) - 3 * F(2) + 4.0 / 3 * F(3) - 0.25 * F(4);
+      if (i == 1) return -0.25 * F(-1) - 5.0 / 6 * F(0) + 1.5 * F(1) - 0.5 * F(2) + 1.0 / 12 * F(3);
+      if (i == 2) return -1.0 / 12 * (F(2) - F(-2)) + 2.0 / 3 * (F(1) - F(-1));
+      if (i == 3) return 0.75 * (F(1) - F(-1)) - 0.15 * (F(2) - F(-2)) + 1.0 / 60 * (F(3) - F(-3));
+    }
+    if (right) {
+      if (i == n - 1) return 25.0 / 12 * F(0) - 4.0 * F(-1) + 3 * F(-2) - 4.0 / 3 * F(-3) + 0.25 * F(-4);
+      if (i == n - 2) return 0.25 * F(1) + 5.0 / 6 * F(0) - 1.5 * F(-1) + 0.5 * F(-2) - 1.0 / 12 * F(-3);
+      if (i == n - 3) return -1.0 / 12 * (F(2) - F(-2)) + 2.0 / 3 * (F(1) - F(-1));
+      if (i == n - 4) return 0.75 * (F(1) - F(-1)) - 0.15 * (F(2) - F(-2)) + 1.0 / 60 * (F(3) - F(-3));
+    }
+    return 0.8 * (F(1) - F(-1)) + -0.2 * (F(2) - F(-2)) + 4.0 / 105 * (F(3) - F(-3)) + -1.0 / 280 * (F(4) - F(-4));
+  }
+  if (left) {
+    if (i == 0) return -25.0 / 12 * F(0) + 4.0 * F(1) - 3 * F(2) + 4.0 / 3 * F(3) - 0.25 * F(4);
+    if (i == 1) return -0.25 * F(-1) - 5.0 / 6 * F(0) + 1.5 * F(1) - 0.5 * F(2) + 1.0 / 12 * F(3);
+  }
+  if (right) {
+    if (i == n - 1) return 25.0 / 12 * F(0) - 4.0 * F(-1) + 3 * F(-2) - 4.0 / 3 * F(-3) + 0.25 * F(-4);
+    if (i == n - 2) return 0.25 * F(1) + 5.0 / 6 * F(0) - 1.5 * F(-1) + 0.5 * F(-2) - 1.0 / 12 * F(-3);
+  }
+  return -1.0 / 12 * (F(2) - F(-2)) + 2.0 / 3 * (F(1) - F(-1));
+#undef F
+}
+
+/* compute_viscous_flux_collocated<Air,ORDER> (ViscousScheme.cu:309-408), periodic exchange, and
+ * compute_viscous_flux_derivative<ORDER> (ViscousScheme.cu:568-612). */
+static void viscous(orc_block *b) {
+  const int order = b->p.viscous_order;
+  const long s = b->sz;
+  const double cp = kGamma * (kRu / kMwAir) / (kGamma - 1);
+#pragma omp parallel for collapse(2)
+  for (int k = 0; k < b->mz; ++k)
+    for (int j = 0; j < b->my; ++j)
+      for (int i = 0; i < b->mx; ++i) {
+        const long id = ID(b, i, j, k);
+        double dv[4][3];
+        const int comp[4] = {1, 2, 3, 5};
+        for (int q = 0; q < 4; ++q) {
+          const double *f = b->bv + comp[q] * s;
+          dv[q][0] = deriv(f, id, 1, i, b->mx, b->bt[0], b->bt[1], order);
+          dv[q][1] = deriv(f, id, b->d2, j, b->my, b->bt[2], b->bt[3], order);
+          dv[q][2] = b->dim == 3 ? deriv(f, id, b->d1, k, b->mz, b->bt[4], b->bt[5], order) : 0.0;
+        }
+        const double *M = b->met;
+        const double xx = M[id], xy = M[id + s], xz = M[id + 2 * s];
+        const double ex = M[id + 3 * s], ey = M[id + 4 * s], ez = M[id + 5 * s];
+        const double zx = M[id + 6 * s], zy = M[id + 7 * s], zz = M[id + 8 * s];
+        const double ux = dv[0][0] * xx + dv[0][1] * ex + dv[0][2] * zx;
+        const double uy = dv[0][0] * xy + dv[0][1] * ey + dv[0][2] * zy;
+        const double uz = dv[0][0] * xz + dv[0][1] * ez + dv[0][2] * zz;
+        const double vx = dv[1][0] * xx + dv[1][1] * ex + dv[1][2] * zx;
+        const double vy = dv[1][0] * xy + dv[1][1] * ey + dv[1][2] * zy;
+        const double vz = dv[1][0] * xz + dv[1][1] * ez + dv[1][2] * zz;
+        const double wx = dv[2][0] * xx + dv[2][1] * ex + dv[2][2] * zx;
+        const double wy = dv[2][0] * xy + dv[2][1] * ey + dv[2][2] * zy;
+        const double wz = dv[2][0] * xz + dv[2][1] * ez + dv[2][2] * zz;
+        const double mu = b->mul[id];
+        const double txx = mu * (4 * ux - 2 * vy - 2 * wz) / 3.0;
+        const double tyy = mu * (4 * vy - 2 * ux - 2 * wz) / 3.0;
+        const double tzz = mu * (4 * wz - 2 * ux - 2 * vy) / 3.0;
+        const double txy = mu * (uy + vx), txz = mu * (uz + wx), tyz = mu * (vz + wy);
+        b->fv[id + s] = xx * txx + xy * txy + xz * txz;
+        b->fv[id + 2 * s] = xx * txy + xy * tyy + xz * tyz;
+        b->fv[id + 3 * s] = xx * txz + xy * tyz + xz * tzz;
+        b->gv[id + s] = ex * txx + ey * txy + ez * txz;
+        b->gv[id + 2 * s] = ex * txy + ey * tyy + ez * tyz;
+        b->gv[id + 3 * s] = ex * txz + ey * tyz + ez * tzz;
+        b->hv[id + s] = zx * txx + zy * txy + zz * txz;
+        b->hv[id + 2 * s] = zx * txy + zy * tyy + zz * tyz;
+        b->hv[id + 3 * s] = zx * txz + zy * tyz + zz * tzz;
+        const double tx = dv[3][0] * xx + dv[3][1] * ex + dv[3][2] * zx;
+        const double ty = dv[3][0] * xy + dv[3][1] * ey + dv[3][2] * zy;
+        const double tz = dv[3][0] * xz + dv[3][1] * ez + dv[3][2] * zz;
+        const double kc = mu / b->p.prandtl * cp;
+        const double u = b->bv[id + s], v = b->bv[id + 2 * s], w = b->bv[id + 3 * s];
+        const double Ex = u * txx + v * txy + w * txz + kc * tx;
+        const double Ey = u * txy + v * tyy + w * tyz + kc * ty;
+        const double Ez = u * txz + v * tyz + w * tzz + kc * tz;
+        b->fv[id + 4 * s] = xx * Ex + xy * Ey + xz * Ez;
+        b->gv[id + 4 * s] = ex * Ex + ey * Ey + ez * Ez;
+        b->hv[id + 4 * s] = zx * Ex + zy * Ey + zz * Ez;
+      }
+  exchange_viscous_flux(b);
+#pragma omp parallel for collapse(2)
+  for (int k = 0; k < b->mz; ++k)
+    for (int j = 0; j < b->my; ++j)
+      for (int i = 0; i < b->mx; ++i) {
+        const long id = ID(b, i, j, k), iq = IDQ(b, i, j, k);
+        const double J = b->jac[id];
+        for (int l = 1; l < 5; ++l) {
+          double d = deriv(b->fv + l * s, id, 1, i, b->mx, b->bt[0], b->bt[1], order) +
+                     deriv(b->gv + l * s, id, b->d2, j, b->my, b->bt[2], b->bt[3], order);
+          d += b->dim == 3 ? deriv(b->hv + l * s, id, b->d1, k, b->mz, b->bt[4], b->bt[5], order) : 0.0;
+          b->dq[iq + l * b->n] += d * J;
+        }
+      }
+}
+
+/* compute_entropy_fix_delta<Air> (InviscidScheme.cu:269-304), over the interior plus one layer */
+static void entropy_fix(orc_block *b) {
+  const long s = b->sz;
+  const int kl = b->dim == 3 ? -1 : 0, kh = b->dim == 3 ? b->mz + 1 : 1;
+#pragma omp parallel for collapse(2)
+  for (int k = kl; k < kh; ++k)
+    for (int j = -1; j < b->my + 1; ++j)
+      for (int i = -1; i < b->mx + 1; ++i) {
+        const long id = ID(b, i, j, k);
+        const double *M = b->met, *bv = b->bv;
+        const double u = bv[id + s], v = bv[id + 2 * s], w = bv[id + 3 * s];
+        const double U = fabs(u * M[id] + v * M[id + s] + w * M[id + 2 * s]);
+        const double V = fabs(u * M[id + 3 * s] + v * M[id + 4 * s] + w * M[id + 5 * s]);
+        const double W = fabs(u * M[id + 6 * s] + v * M[id + 7 * s] + w * M[id + 8 * s]);
+        const double kx = sqrt(M[id] * M[id] + M[id + s] * M[id + s] + M[id + 2 * s] * M[id + 2 * s]);
+        const double ky = sqrt(M[id + 3 * s] * M[id + 3 * s] + M[id + 4 * s] * M[id + 4 * s] +
+                               M[id + 5 * s] * M[id + 5 * s]);
+        const double kz = sqrt(M[id + 6 * s] * M[id + 6 * s] + M[id + 7 * s] * M[id + 7 * s] +
+                               M[id + 8 * s] * M[id + 8 * s]);
+        const double c = sqrt(kGamma * bv[id + 4 * s] / bv[id]);
+        if (b->dim == 2)
+          b->efd[id] = b->p.entropy_fix_factor * (U + V + c * 0.5 * (kx + ky));
+        else
+          b->efd[id] = b->p.entropy_fix_factor * (U + V + W + c * (kx + ky + kz) / 3.0);
+      }
+}
+
+/* Inviscid flux of one direction and its difference into dq (the role of the WENO/pv kernels plus
+ * compute_derivative_x/y/z, WENO.cu:1544-1571). */
+static void inviscid_dir(orc_block *b, int dir) {
+  const int r = b->p.reconstruction == 5 ? 4 : 3; /* stencil half width: n = 2r points */
+  const int n = 2 * r;
+  const long s = b->sz;
+  const long st = dir == 0 ? 1 : (dir == 1 ? b->d2 : b->d1);
+  const int ext[3] = {b->mx, b->my, b->mz};
+  const int m = ext[dir];
+  /* lines orthogonal to dir */
+  const int oa = dir == 0 ? 1 : 0, ob = dir == 2 ? 1 : 2;
+#pragma omp parallel for collapse(2)
+  for (int B = 0; B < ext[ob]; ++B)
+    for (int A = 0; A < ext[oa]; ++A) {
+      double *Fl = malloc(sizeof(double) * 5 * (m + 1));
+      for (int t = -1; t < m; ++t) { /* interface t+1/2 */
+        int idx[3];
+        idx[dir] = t;
+        idx[oa] = A;
+        idx[ob] = B;
+        const long id = ID(b, idx[0], idx[1], idx[2]);
+        double q[8][5];
+        for (int mm = 0; mm < n; ++mm)
+          for (int l = 0; l < 5; ++l) q[mm][l] = b->cv[id + (mm - r + 1) * st + l * s];
+        double ml[3], mr[3];
+        for (int c = 0; c < 3; ++c) {
+          ml[c] = b->met[id + (3 * dir + c) * s];
+          mr[c] = b->met[id + st + (3 * dir + c) * s];
+        }
+        orc_interface_flux(&b->p, (const double(*)[5])q, n, ml, mr, b->jac[id], b->jac[id + st], b->efd[id],
+                           b->efd[id + st], Fl + 5 * (t + 1));
+      }
+      for (int t = 0; t < m; ++t) {
+        int idx[3];
+        idx[dir] = t;
+        idx[oa] = A;
+        idx[ob] = B;
+        const long iq = IDQ(b, idx[0], idx[1], idx[2]);
+        for (int l = 0; l < 5; ++l) b->dq[iq + l * b->n] -= Fl[5 * (t + 1) + l] - Fl[5 * t + l];
+      }
+      free(Fl);
+    }
+}
+
+void orc_block_rhs(orc_block *b) {
+  memset(b->dq, 0, sizeof(double) * 5 * b->n);
+  if (b->p.viscous_order > 0) viscous(b);
+  if (b->p.inviscid_scheme == 2) entropy_fix(b);
+  inviscid_dir(b, 0);
+  inviscid_dir(b, 1);
+  if (b->dim == 3) inviscid_dir(b, 2);
+}
+
+static const double rk_a[3] = {1.0, 0.75, 1.0 / 3.0};
+static const double rk_b[3] = {0.0, 0.25, 2.0 / 3.0};
+static const double rk_c[3] = {1.0, 0.25, 2.0 / 3.0};
+
+/* update_cv_and_bv_rk<Air> (RK.cuh:224-280) */
+static void update(orc_block *b, double dt, int rk) {
+  const long s = b->sz;
+#pragma omp parallel for collapse(2)
+  for (int k = 0; k < b->mz; ++k)
+    for (int j = 0; j < b->my; ++j)
+      for (int i = 0; i < b->mx; ++i) {
+        const long id = ID(b, i, j, k), iq = IDQ(b, i, j, k);
+        const double dj = dt / b->jac[id];
+        for (int l = 0; l < 5; ++l)
+          b->cv[id + l * s] = rk_a[rk] * b->qn[id + l * s] + rk_b[rk] * b->cv[id + l * s] +
+                              rk_c[rk] * dj * b->dq[iq + l * b->n];
+        if (b->mz == 1) b->cv[id + 3 * s] = 0;
+        double *bv = b->bv, *cv = b->cv;
+        bv[id] = cv[id];
+        const double di = 1.0 / cv[id];
+        bv[id + s] = cv[id + s] * di;
+        bv[id + 2 * s] = cv[id + 2 * s] * di;
+        bv[id + 3 * s] = cv[id + 3 * s] * di;
+        const double V2 = bv[id + s] * bv[id + s] + bv[id + 2 * s] * bv[id + 2 * s] + bv[id + 3 * s] * bv[id + 3 * s];
+        bv[id + 4 * s] = (kGamma - 1) * (cv[id + 4 * s] - 0.5 * bv[id] * V2);
+        bv[id + 5 * s] = bv[id + 4 * s] * kMwAir * di / kRu;
+      }
+}
+
+void orc_block_set_bv(orc_block *b, const double *bv) {
+  memcpy(b->bv, bv, sizeof(double) * 6 * b->sz);
+#pragma omp parallel for
+  for (long id = 0; id < b->sz; ++id) cv_from_bv(b, id);
+  update_properties(b);
+  apply_bcs(b);
+  update_properties(b);
+}
+
+void orc_block_set_raw(orc_block *b, const double *bv, const double *cv) {
+  memcpy(b->bv, bv, sizeof(double) * 6 * b->sz);
+  memcpy(b->cv, cv, sizeof(double) * 5 * b->sz);
+  update_properties(b);
+}
+
+void orc_block_get(const orc_block *b, int which, double *out) {
+  switch (which) {
+    case 0: memcpy(out, b->cv, sizeof(double) * 5 * b->sz); break;
+    case 1: memcpy(out, b->bv, sizeof(double) * 6 * b->sz); break;
+    case 2: memcpy(out, b->dq, sizeof(double) * 5 * b->n); break;
+    case 3: memcpy(out, b->mul, sizeof(double) * b->sz); break;
+    case 4: memcpy(out, b->jac, sizeof(double) * b->sz); break;
+    case 5: memcpy(out, b->met, sizeof(double) * 9 * b->sz); break;
+    case 6: memcpy(out, b->fv, sizeof(double) * 5 * b->sz); break;
+    case 7: memcpy(out, b->gv, sizeof(double) * 5 * b->sz); break;
+    case 8: memcpy(out, b->hv, sizeof(double) * 5 * b->sz); break;
+    default: break;
+  }
+}
+
+/* local_time_step<Air> (TimeAdvanceFunc.cuh:28-115) and the global minimum (TimeAdvanceFunc.cu:102-152) */
+double orc_block_cfl_dt(orc_block *b) {
+  const long s = b->sz;
+  double dtmin = 1e+6;
+#pragma omp parallel for collapse(2) reduction(min : dtmin)
+  for (int k = 0; k < b->mz; ++k)
+    for (int j = 0; j < b->my; ++j)
+      for (int i = 0; i < b->mx; ++i) {
+        const long id = ID(b, i, j, k);
+        const double *M = b->met, *bv = b->bv;
+        const double gx = sqrt(M[id] * M[id] + M[id + s] * M[id + s] + M[id + 2 * s] * M[id + 2 * s]);
+        const double ge = sqrt(M[id + 3 * s] * M[id + 3 * s] + M[id + 4 * s] * M[id + 4 * s] +
+                               M[id + 5 * s] * M[id + 5 * s]);
+        const double gz = sqrt(M[id + 6 * s] * M[id + 6 * s] + M[id + 7 * s] * M[id + 7 * s] +
+                               M[id + 8 * s] * M[id + 8 * s]);
+        const double u = bv[id + s], v = bv[id + 2 * s], w = bv[id + 3 * s];
+        const double U = u * M[id] + v * M[id + s] + w * M[id + 2 * s];
+        const double V = u * M[id + 3 * s] + v * M[id + 4 * s] + w * M[id + 5 * s];
+        const double W = u * M[id + 6 * s] + v * M[id + 7 * s] + w * M[id + 8 * s];
+        const double c = sqrt(kGamma * (kRu / kMwAir) * bv[id + 5 * s]);
+        double sr = fmax(fabs(U) + c * gx, fabs(V) + c * ge);
+        if (b->dim == 3) sr = fmax(sr, fabs(W) + c * gz);
+        if (b->p.viscous_order > 0) {
+          double ml = fmax(gx, ge);
+          if (b->dim == 3) ml = fmax(ml, gz);
+          double dv = fmax(kGamma, 4.0 / 3.0) / bv[id];
+          dv *= b->mul[id] / b->p.prandtl;
+          sr = fmax(sr, ml * ml * dv);
+        }
+        const double dt = b->p.cfl / sr;
+        if (dt < dtmin) dtmin = dt;
+      }
+  return dtmin;
+}
+
+/* One SSP-RK3 step (RK.cuh:100-192) for a single block with only outflow/periodic faces. */
+void orc_block_rk3_step(orc_block *b, double dt) {
+  memcpy(b->qn, b->cv, sizeof(double) * 5 * b->sz);
+  for (int rk = 0; rk < 3; ++rk) {
+    orc_block_rhs(b);
+    update(b, dt, rk);
+    apply_bcs(b);
+    update_properties(b);
+  }
+}

@@ -1,0 +1,172 @@
+"""Input decks (setup text, grid, boundary faces) for the parity tests and the benchmark.
+
+Every case is a dict with: setup (setup.txt text), grid (x, y, z arrays of shape (mx,my,mz) as written
+to a grid file, i.e. before gridScale), scale (gridScale),
+faces (rows i1 i2 j1 j2 k1 k2 type, 0-based as in boundary%4d.txt), boundary (file text),
+scheme (inviscid_scheme, reconstruction, viscous_order), prandtl, cfl.
+"""
+import math
+
+import numpy as np
+
+R_U, MW_AIR, GAMMA = 8314.462618, 28.9647, 1.4
+R_AIR = R_U / MW_AIR
+
+
+def sutherland(T):
+    return 1.7894e-5 * (T / 288.16) ** 1.5 * (288.16 + 110) / (T + 110)
+
+
+def boundary_text(faces):
+    lines = [f"{len(faces):7d}"]
+    for f in faces:
+        lines.append("".join(f"{v:7d}" for v in f))
+    return "\n".join(lines) + "\n"
+
+
+def box_faces(mx, my, mz, kind):
+    """All six (or four, 2D) faces of one block with the same boundary type."""
+    if mz == 1:
+        return [[0, 0, 0, my - 1, 0, 0, kind], [mx - 1, mx - 1, 0, my - 1, 0, 0, kind],
+                [0, mx - 1, 0, 0, 0, 0, kind], [0, mx - 1, my - 1, my - 1, 0, 0, kind]]
+    return [[0, 0, 0, my - 1, 0, mz - 1, kind], [mx - 1, mx - 1, 0, my - 1, 0, mz - 1, kind],
+            [0, mx - 1, 0, 0, 0, mz - 1, kind], [0, mx - 1, my - 1, my - 1, 0, mz - 1, kind],
+            [0, mx - 1, 0, my - 1, 0, 0, kind], [0, mx - 1, 0, my - 1, mz - 1, mz - 1, kind]]
+
+
+def sod(n=400, my=7, recon=4, inviscid=4, cfl=0.5, t_end=0.2):
+    """1D Sod shock tube on an n x my x 1 block (the 2D block the reference needs for a 1D case)."""
+    x = np.linspace(0.0, 1.0, n)
+    dy = 1.0 / 64  # exactly representable, so every j row has bitwise identical metrics
+    y = (np.arange(my) - (my - 1) / 2) * dy
+    X, Y, Z = np.meshgrid(x, y, [0.0], indexing="ij")
+    setup = f"""// Sod shock tube: WENO characteristic reconstruction + Riemann solver
+string canonical_problem = general
+real gridScale = 1
+int gridIsBinary = 1
+bool steady = 0
+int temporal_scheme = 3
+bool fixed_time_step = 0
+real cfl = {cfl}
+real total_simulation_time = {t_end}
+int reconstruction = {recon}
+int inviscid_scheme = {inviscid}
+int viscous_order = 0
+array string boundary_conditions {{
+    outflow
+}}
+struct outflow {{
+    string type = outflow
+    int label   =   6
+}}
+string default_init = left
+string reference_state = left
+struct  left {{
+    string  type            =   inflow
+    int     label           =   5
+    real    velocity        =   0
+    real    pressure        =   1
+    real    density         =   1
+}}
+struct  right {{
+    string  type        =   inflow
+    int     label       =   5
+    real    density     =   0.125
+    real    velocity    =   0
+    real    pressure    =   0.1
+}}
+int groups_init = 2
+struct init_cond_0 {{
+	real x0 = 0.5
+	real x1 = 1.5
+	real y0 = -10
+	real y1 = 10
+	real z0 = -2
+	real z1 = 2
+	string name = right
+}}
+"""
+    faces = box_faces(n, my, 1, 6)
+    return dict(setup=setup, grid=(X, Y, Z), scale=1.0, faces=faces, boundary=boundary_text(faces),
+                scheme=(inviscid, recon, 0), prandtl=0.72, cfl=cfl, t_end=t_end,
+                eps=1e-40 * (1.0 * math.sqrt(GAMMA)) ** 2)
+
+
+def tgv_length(reynolds, mach=0.1, T0=300.0, p0=101325.0):
+    rho0 = p0 / (R_AIR * T0)
+    V0 = mach * math.sqrt(GAMMA * R_AIR * T0)
+    return reynolds * sutherland(T0) / (rho0 * V0), rho0, V0
+
+
+def wavy_grid(n, amp=0.05, dtype=np.float64):
+    """x = 2 pi (xi + amp sin 2pi eta sin 2pi zeta) and cyclic permutations, n nodes per direction
+    with coincident end nodes (xi = 0 and xi = 1 are the same periodic node)."""
+    s = np.linspace(0.0, 1.0, n)
+    xi, eta, zeta = np.meshgrid(s, s, s, indexing="ij")
+    tp = 2 * np.pi
+    se, sz_, sx = np.sin(tp * eta), np.sin(tp * zeta), np.sin(tp * xi)
+    X = tp * (xi + amp * se * sz_)
+    Y = tp * (eta + amp * sz_ * sx)
+    Z = tp * (zeta + amp * sx * se)
+    return X, Y, Z
+
+
+def tgv(n=17, reynolds=1600.0, mach=0.1, recon=5, inviscid=3, visc=4, cfl=0.5, amp=0.05, fixed_dt=None):
+    """Taylor-Green vortex on the wavy periodic grid; coordinates in units of L (gridScale = L)."""
+    L, rho0, V0 = tgv_length(reynolds, mach)
+    X, Y, Z = wavy_grid(n, amp)
+    dt_lines = f"bool fixed_time_step = 1\nreal dt = {fixed_dt!r}" if fixed_dt else "bool fixed_time_step = 0"
+    setup = f"""// Taylor-Green vortex, Mach {mach}, Re {reynolds}, wavy periodic grid
+string canonical_problem = taylorGreen
+real gridScale = {L!r}
+int gridIsBinary = 1
+bool steady = 0
+int temporal_scheme = 3
+{dt_lines}
+real cfl = {cfl}
+int reconstruction = {recon}
+int inviscid_scheme = {inviscid}
+int viscous_order = {visc}
+real prandtl_number = 0.72
+array string boundary_conditions {{
+    periodic
+}}
+struct periodic {{
+    string type = periodic
+    int label = 8
+}}
+struct taylorGreen {{
+    real mach = {mach}
+    real temperature = 300
+    real pressure = 101325
+}}
+"""
+    faces = box_faces(n, n, n, 8)
+    return dict(setup=setup, grid=(X, Y, Z), scale=L, faces=faces,
+                boundary=boundary_text(faces), scheme=(inviscid, recon, visc), prandtl=0.72, cfl=cfl,
+                L=L, rho0=rho0, V0=V0, eps=1e-40 * (rho0 * V0) ** 2)
+
+
+def physical_grid(case):
+    """Coordinates after gridScale, as Mesh::scale applies it (Mesh.cpp:657)."""
+    s = case["scale"]
+    if abs(s - 1.0) <= 1e-15:
+        return case["grid"]
+    return tuple(a * s for a in case["grid"])
+
+
+def write_input_dir(case, path):
+    """Write the case as a COREFL input directory: setup.txt, grid/grid   0.dat, boundary_condition/."""
+    import os
+    os.makedirs(os.path.join(path, "grid"), exist_ok=True)
+    os.makedirs(os.path.join(path, "boundary_condition"), exist_ok=True)
+    with open(os.path.join(path, "setup.txt"), "w") as f:
+        f.write(case["setup"])
+    X, Y, Z = case["grid"]
+    with open(os.path.join(path, "grid", "grid   0.dat"), "wb") as f:
+        np.array([1], dtype=np.int32).tofile(f)
+        np.array(X.shape, dtype=np.int32).tofile(f)
+        for a in (X, Y, Z):
+            np.asarray(a, dtype=np.float64).ravel(order="F").tofile(f)
+    with open(os.path.join(path, "boundary_condition", "boundary   0.txt"), "w") as f:
+        f.write(case["boundary"])

@@ -1,0 +1,167 @@
+"""ctypes binding of the oracle (oracle/_build/liboracle.so), the CPU checker of the HIP path.
+
+Test infrastructure only: used by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(ROOT, "oracle", "_build", "liboracle.so")
+
+
+class OrcParams(ctypes.Structure):
+    _fields_ = [("inviscid_scheme", ctypes.c_int), ("reconstruction", ctypes.c_int),
+                ("viscous_order", ctypes.c_int), ("prandtl", ctypes.c_double), ("cfl", ctypes.c_double),
+                ("entropy_fix_factor", ctypes.c_double), ("weno_eps", ctypes.c_double)]
+
+
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB):
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
+    lib = ctypes.CDLL(LIB)
+    D, I, P = ctypes.c_double, ctypes.c_int, ctypes.c_void_p
+    lib.orc_weno5.restype = D
+    lib.orc_weno5.argtypes = [P, P, D, I]
+    lib.orc_weno7.restype = D
+    lib.orc_weno7.argtypes = [P, P, D, I]
+    lib.orc_char_matrices.argtypes = [P, P, P, P, P]
+    lib.orc_riemann.argtypes = [I, P, P, P, P, D, D, D, D, I, P]
+    lib.orc_interface_flux.restype = I
+    lib.orc_interface_flux.argtypes = [ctypes.POINTER(OrcParams), P, I, P, P, D, D, D, D, P]
+    lib.orc_sutherland.restype = D
+    lib.orc_sutherland.argtypes = [D]
+    lib.orc_metrics.restype = I
+    lib.orc_metrics.argtypes = [I, I, I, I, P, P, P, P, P]
+    lib.orc_block_create.restype = P
+    lib.orc_block_create.argtypes = [ctypes.POINTER(OrcParams), I, I, I, I, P, I, P, P]
+    lib.orc_block_destroy.argtypes = [P]
+    lib.orc_block_set_bv.argtypes = [P, P]
+    lib.orc_block_get.argtypes = [P, I, P]
+    lib.orc_block_set_raw.argtypes = [P, P, P]
+    lib.orc_block_rhs.argtypes = [P]
+    lib.orc_block_cfl_dt.restype = D
+    lib.orc_block_cfl_dt.argtypes = [P]
+    lib.orc_block_rk3_step.argtypes = [P, D]
+    lib.orc_num_threads.restype = I
+    lib.orc_exact_riemann.restype = D
+    lib.orc_exact_riemann.argtypes = [D, D, D, D, D, D, D, D, D, I, P, P, P, P]
+    _lib = lib
+    return lib
+
+
+def _p(a):
+    return a.ctypes.data
+
+
+def weno5(vp, vm, eps, shock=1):
+    vp, vm = np.ascontiguousarray(vp, float), np.ascontiguousarray(vm, float)
+    return load().orc_weno5(_p(vp), _p(vm), eps, shock)
+
+
+def weno7(vp, vm, eps, shock=1):
+    vp, vm = np.ascontiguousarray(vp, float), np.ascontiguousarray(vm, float)
+    return load().orc_weno7(_p(vp), _p(vm), eps, shock)
+
+
+def char_matrices(ql, qr, k):
+    ql, qr, k = (np.ascontiguousarray(a, float) for a in (ql, qr, k))
+    L, R = np.zeros((5, 5)), np.zeros((5, 5))
+    load().orc_char_matrices(_p(ql), _p(qr), _p(k), _p(L), _p(R))
+    return L, R
+
+
+def riemann(scheme, pl, pr, ml, mr, jl=1.0, jr=1.0, efl=0.0, efr=0.0):
+    pl, pr, ml, mr = (np.ascontiguousarray(a, float) for a in (pl, pr, ml, mr))
+    fc = np.zeros(5)
+    load().orc_riemann(scheme, _p(pl), _p(pr), _p(ml), _p(mr), jl, jr, efl, efr, 3, _p(fc))
+    return fc
+
+
+def params(inviscid=3, recon=5, visc=0, prandtl=0.72, cfl=0.5, efix=0.125, eps=1e-40):
+    return OrcParams(inviscid, recon, visc, prandtl, cfl, efix, eps)
+
+
+def interface_flux(p, q, ml, mr, jl=1.0, jr=1.0, efl=0.0, efr=0.0):
+    q = np.ascontiguousarray(q, float)
+    ml, mr = np.ascontiguousarray(ml, float), np.ascontiguousarray(mr, float)
+    fc = np.zeros(5)
+    fb = load().orc_interface_flux(ctypes.byref(p), _p(q), q.shape[0], _p(ml), _p(mr), jl, jr, efl, efr, _p(fc))
+    return fc, fb
+
+
+def sutherland(T):
+    return load().orc_sutherland(T)
+
+
+def metrics(x, y, z, ngg):
+    """x, y, z: (mx,my,mz) arrays.  Returns jac (sz,) and metric (9*sz,) in the ngg-ghost layout."""
+    mx, my, mz = x.shape
+    fx, fy, fz = (np.ascontiguousarray(np.asarray(a, float).ravel(order="F")) for a in (x, y, z))
+    sz = (mx + 2 * ngg) * (my + 2 * ngg) * (mz + 2 * ngg)
+    jac, met = np.zeros(sz), np.zeros(9 * sz)
+    rc = load().orc_metrics(mx, my, mz, ngg, _p(fx), _p(fy), _p(fz), _p(jac), _p(met))
+    if rc != 0:
+        raise ValueError("non-positive interior Jacobian")
+    return jac, met
+
+
+class Block:
+    """One oracle block: metrics from the raw grid, faces as in boundary%4d.txt."""
+
+    def __init__(self, p, x, y, z, ngg, faces):
+        self.mx, self.my, self.mz = x.shape
+        self.ngg = ngg
+        self.sz = (self.mx + 2 * ngg) * (self.my + 2 * ngg) * (self.mz + 2 * ngg)
+        self.jac, self.metric = metrics(x, y, z, ngg)
+        f = np.ascontiguousarray(np.asarray(faces, dtype=np.int32).ravel())
+        self._p = p
+        self._h = load().orc_block_create(ctypes.byref(p), self.mx, self.my, self.mz, ngg, _p(f), len(faces),
+                                          _p(self.jac), _p(self.metric))
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            load().orc_block_destroy(self._h)
+            self._h = None
+
+    def set_bv(self, bv):
+        bv = np.ascontiguousarray(bv, float)
+        assert bv.size == 6 * self.sz
+        load().orc_block_set_bv(self._h, _p(bv))
+
+    def set_raw(self, bv, cv):
+        bv, cv = np.ascontiguousarray(bv, float), np.ascontiguousarray(cv, float)
+        assert bv.size == 6 * self.sz and cv.size == 5 * self.sz
+        load().orc_block_set_raw(self._h, _p(bv), _p(cv))
+
+    def get(self, which):
+        n = {0: 5 * self.sz, 1: 6 * self.sz, 2: 5 * self.mx * self.my * self.mz, 3: self.sz, 4: self.sz,
+             5: 9 * self.sz, 6: 5 * self.sz, 7: 5 * self.sz, 8: 5 * self.sz}[which]
+        out = np.zeros(n)
+        load().orc_block_get(self._h, which, _p(out))
+        return out
+
+    def rhs(self):
+        load().orc_block_rhs(self._h)
+        return self.get(2)
+
+    def cfl_dt(self):
+        return load().orc_block_cfl_dt(self._h)
+
+    def step(self, dt):
+        load().orc_block_rk3_step(self._h, dt)
+
+
+def exact_riemann(left, right, gamma, x0, t, x):
+    x = np.ascontiguousarray(x, float)
+    rho, u, p = np.zeros_like(x), np.zeros_like(x), np.zeros_like(x)
+    ps = load().orc_exact_riemann(*left, *right, gamma, x0, t, x.size, _p(x), _p(rho), _p(u), _p(p))
+    return rho, u, p, ps

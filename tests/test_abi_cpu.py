@@ -1,0 +1,33 @@
+"""The C-ABI library loads without a GPU and exports every symbol include/corefl_c.h declares."""
+import ctypes
+import os
+import re
+
+import corefl
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "corefl_c.h")).read()
+    return sorted(set(re.findall(r"\b(corefl_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_and_wrapper_agree():
+    assert sorted(corefl.SYMBOLS) == declared_symbols()
+
+
+def test_library_exports_all_symbols():
+    lib = corefl.load_library()
+    for name in declared_symbols():
+        assert hasattr(lib, name), name
+        assert ctypes.cast(getattr(lib, name), ctypes.c_void_p).value
+
+
+def test_bad_input_reports_error():
+    # a missing input directory fails in the host parser before any device call
+    lib = corefl.load_library()
+    ctx = ctypes.c_void_p()
+    rc = lib.corefl_create(b"/nonexistent/input", ctypes.byref(ctx))
+    assert rc != 0
+    assert b"setup.txt" in lib.corefl_last_error()
