@@ -27,6 +27,18 @@ STEP_BYTES = 560.0
 FLUX_BYTES = 112.0
 
 
+def measured_traffic(kernel_symbol, n):
+    """HBM bytes per launch of one kernel from the committed rocprofv3 counter summary (same workload only)."""
+    path = os.path.join(ROOT, "profiles", "flow_tgv256_hbm_traffic.md")
+    if n != 257 or not os.path.exists(path):
+        return None
+    for line in open(path):
+        cells = [c.strip() for c in line.split("|")]
+        if len(cells) > 7 and cells[1] == f"`{kernel_symbol}`":
+            return float(cells[7]) * 1e9
+    return None
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -63,7 +75,17 @@ def main():
     ap.add_argument("--n", type=int, default=257, help="nodes per direction (n-1 unique)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-reps", type=int, default=3)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a plain 1 GiB device copy (reference point for the HBM byte counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        x = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": x.numel() * 8, "copy_bytes_written": y.numel() * 8}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); multi-GPU sharding is designed, not built.",
               file=sys.stderr)
@@ -100,8 +122,10 @@ def main():
     dom = max(flux, key=flux.get)
     faces = {"flux_x": (n + 1) * n * n, "flux_y": n * (n + 1) * n, "flux_z": n * n * (n + 1)}[dom]
     achieved = FLUX_BYTES * faces / (flux[dom] * 1e-3) / 1e9
+    symbol = {"flux_x": "k_inviscid<0, 4, 3>", "flux_y": "k_inviscid<1, 4, 3>", "flux_z": "k_inviscid<2, 4, 3>"}[dom]
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom}
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(symbol, n), "kernel": dom,
+                "algorithmic_bytes": FLUX_BYTES * faces}
 
     cpu = None
     if not args.no_cpu_baseline:
