@@ -268,7 +268,7 @@ void build(corefl_ctx *ctx, std::vector<corefl::HostBlock> blocks) {
     if (hb.mx < 3 || hb.my < 3 || (hb.mz != 1 && hb.mz < 3))
       throw std::runtime_error("block extents must be >= 3 (mz = 1 for 2D blocks)");
     for (int d = 0; d < hb.dim; ++d)
-      if (ext[d] < ngg + 1) throw std::runtime_error("block extent smaller than ngg + 1");
+      if (ext[d] < ngg + 2) throw std::runtime_error("block extent smaller than ngg + 2");
     if (vo > 0)
       for (int d = 0; d < hb.dim; ++d)
         if (ext[d] < 5) throw std::runtime_error("viscous closures need at least 5 nodes per direction");

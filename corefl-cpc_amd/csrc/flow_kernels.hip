@@ -21,6 +21,57 @@ __device__ __forceinline__ long nid(const DevBlock &b, int i, int j, int k) {
   return (k + b.g) * b.d1 + (j + b.g) * b.d2 + (i + b.g);
 }
 
+// XCD-aware tile order.  Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.
+// A 1-D launch of 8*P workgroups is remapped so that the workgroups landing on one XCD take a contiguous
+// run of tiles in which the sweep axis varies fastest: neighbouring planes of a stencil are then read by
+// workgroups sharing an L2.  The map b -> (b % 8) * P + b / 8 is a bijection on [0, 8P).
+struct TileGrid {
+  int gx, gy, gz; // tiles along i (64 wide), j (4 wide) and k (1 wide)
+  int fast;       // 0: i then j then k (each XCD takes a contiguous slab of planes), 1: j fastest, 2: k fastest
+};
+
+struct Tile {
+  int x, y, z;
+};
+
+__device__ __forceinline__ bool xcd_tile(const TileGrid &t, Tile &out) {
+  const unsigned total = (unsigned)t.gx * t.gy * t.gz;
+  const unsigned per = (total + 7) / 8;
+  const unsigned n = (blockIdx.x % 8) * per + blockIdx.x / 8;
+  if (n >= total) return false;
+  if (t.fast == 0) {
+    out.x = n % t.gx;
+    const unsigned r = n / t.gx;
+    out.y = r % t.gy;
+    out.z = r / t.gy;
+  } else if (t.fast == 2) {
+    out.z = n % t.gz;
+    const unsigned r = n / t.gz;
+    out.x = r % t.gx;
+    out.y = r / t.gx;
+  } else {
+    out.y = n % t.gy;
+    const unsigned r = n / t.gy;
+    out.x = r % t.gx;
+    out.z = r / t.gx;
+  }
+  return true;
+}
+
+inline unsigned tile_blocks(const TileGrid &t) {
+  const unsigned total = (unsigned)t.gx * t.gy * t.gz;
+  return 8 * ((total + 7) / 8);
+}
+
+// 1/d from the hardware reciprocal refined by two Newton steps (relative error at the rounding level),
+// cheaper than an IEEE division; d > 0 and normal wherever it is used.
+__device__ __forceinline__ double rcp(double d) {
+  double r = __builtin_amdgcn_rcp(d);
+  r = fma(fma(-d, r, 1.0), r, r);
+  r = fma(fma(-d, r, 1.0), r, r);
+  return r;
+}
+
 // ---------------------------------------------------------------- WENO halves
 // Upwind-biased value at the right face of v[2] from v[0..4] (the fPlus part of WENO.cu:3059-3114).
 // The downwind-biased value is the same function of the mirrored stencil.
@@ -35,10 +86,10 @@ __device__ __forceinline__ double weno5_half(double v0, double v1, double v2, do
   const double b0 = t12 * e0 * e0 + 0.25 * f0 * f0;
   const double b1 = t12 * e1 * e1 + 0.25 * f1 * f1;
   const double b2 = t12 * e2 * e2 + 0.25 * f2 * f2;
-  const double a0 = 0.3 / ((eps + b0) * (eps + b0));
-  const double a1 = 0.6 / ((eps + b1) * (eps + b1));
-  const double a2 = 0.1 / ((eps + b2) * (eps + b2));
-  return (a0 * p0 + a1 * p1 + a2 * p2) / (a0 + a1 + a2);
+  const double a0 = 0.3 * rcp((eps + b0) * (eps + b0));
+  const double a1 = 0.6 * rcp((eps + b1) * (eps + b1));
+  const double a2 = 0.1 * rcp((eps + b2) * (eps + b2));
+  return (a0 * p0 + a1 * p1 + a2 * p2) * rcp(a0 + a1 + a2);
 }
 
 // Upwind-biased value at the right face of v[3] from v[0..6] (the fPlus part of WENO.cu:3116-3222,
@@ -63,15 +114,15 @@ __device__ __forceinline__ double weno7_half(const double *v, double eps) {
   double tau = b0 + 3 * b1 - 3 * b2 - b3;
   tau *= tau;
   constexpr double c0 = 1.0 / 35, c1 = 12.0 / 35, c2 = 18.0 / 35, c3 = 4.0 / 35;
-  const double a0 = c0 + c0 * tau / ((eps + b0) * (eps + b0));
-  const double a1 = c1 + c1 * tau / ((eps + b1) * (eps + b1));
-  const double a2 = c2 + c2 * tau / ((eps + b2) * (eps + b2));
-  const double a3 = c3 + c3 * tau / ((eps + b3) * (eps + b3));
+  const double a0 = c0 + c0 * tau * rcp((eps + b0) * (eps + b0));
+  const double a1 = c1 + c1 * tau * rcp((eps + b1) * (eps + b1));
+  const double a2 = c2 + c2 * tau * rcp((eps + b2) * (eps + b2));
+  const double a3 = c3 + c3 * tau * rcp((eps + b3) * (eps + b3));
   const double q0 = -3 * v[0] + 13 * v[1] - 23 * v[2] + 25 * v[3];
   const double q1 = v[1] - 5 * v[2] + 13 * v[3] + 3 * v[4];
   const double q2 = -v[2] + 7 * v[3] + 7 * v[4] - v[5];
   const double q3 = 3 * v[3] + 13 * v[4] - 5 * v[5] + v[6];
-  return (1.0 / 12) * (a0 * q0 + a1 * q1 + a2 * q2 + a3 * q3) / (a0 + a1 + a2 + a3);
+  return (1.0 / 12) * (a0 * q0 + a1 * q1 + a2 * q2 + a3 * q3) * rcp(a0 + a1 + a2 + a3);
 }
 
 // ---------------------------------------------------------------- Riemann solvers
@@ -257,24 +308,14 @@ __device__ __forceinline__ void to_primitive(const double *q, double *p) {
 // t and t+1 (rows as in WENO.cu:73-124), reconstruction is one-sided WENO in those fields,
 // back-projection is the explicit R of WENO.cu:229-243, then the chosen Riemann solver.
 template<int DIR, int R, int RS>
-__global__ __launch_bounds__(256) void k_inviscid(DevBlock b, Scheme sc, double *F) {
-  int i, j, k;
-  if constexpr (DIR == 0) {
-    i = blockIdx.x * 64 + threadIdx.x - 1;
-    j = blockIdx.y * 4 + threadIdx.y;
-    k = blockIdx.z;
-    if (i >= b.mx || j >= b.my) return;
-  } else if constexpr (DIR == 1) {
-    i = blockIdx.x * 64 + threadIdx.x;
-    j = blockIdx.y * 4 + threadIdx.y - 1;
-    k = blockIdx.z;
-    if (i >= b.mx || j >= b.my) return;
-  } else {
-    i = blockIdx.x * 64 + threadIdx.x;
-    j = blockIdx.y * 4 + threadIdx.y;
-    k = blockIdx.z - 1;
-    if (i >= b.mx || j >= b.my) return;
-  }
+__global__ __launch_bounds__(256) void k_inviscid(DevBlock b, Scheme sc, double *F, TileGrid tg) {
+  Tile t;
+  if (!xcd_tile(tg, t)) return;
+  int i = t.x * 64 + threadIdx.x, j = t.y * 4 + threadIdx.y, k = t.z;
+  if constexpr (DIR == 0) i -= 1;
+  else if constexpr (DIR == 1) j -= 1;
+  else k -= 1;
+  if (i >= b.mx || j >= b.my) return;
   const long st = DIR == 0 ? 1 : (DIR == 1 ? b.d2 : b.d1);
   const long id = nid(b, i, j, k);
   const long sz = b.sz;
@@ -427,8 +468,10 @@ __device__ __forceinline__ double sutherland(double T) {
 
 // compute_viscous_flux_collocated<Air, ORDER> (ViscousScheme.cu:309-408): node fluxes fv/gv/hv[1..4].
 template<int ORDER>
-__global__ __launch_bounds__(256) void k_viscous_flux(DevBlock b, Scheme sc) {
-  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+__global__ __launch_bounds__(256) void k_viscous_flux(DevBlock b, Scheme sc, TileGrid tg) {
+  Tile t;
+  if (!xcd_tile(tg, t)) return;
+  const int i = t.x * 64 + threadIdx.x, j = t.y * 4 + threadIdx.y, k = t.z;
   if (i >= b.mx || j >= b.my) return;
   const long id = nid(b, i, j, k), sz = b.sz;
   double g[4][3];
@@ -477,31 +520,59 @@ __global__ __launch_bounds__(256) void k_viscous_flux(DevBlock b, Scheme sc) {
   hv[4 * sz] = zx * Ex + zy * Ey + zz * Ez;
 }
 
-// periodic_exchange task 1 (DataCommunication.cu:628-683): average coincident face nodes, copy ghosts.
-__global__ void k_viscous_exchange(DevBlock b, DevFace f) {
-  const int i = f.start[0] + blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = f.start[1] + blockIdx.y * blockDim.y + threadIdx.y;
-  const int k = f.start[2] + blockIdx.z * blockDim.z + threadIdx.z;
-  if (i > f.end[0] || j > f.end[1] || k > f.end[2]) return;
-  int o[3] = {i, j, k};
+// One work item = one (face node, ghost layer g) pair.  On i faces the ghost layers of a node are
+// contiguous in memory, so g varies fastest across lanes there; on j and k faces the face nodes vary
+// fastest (along i).  The g == 1 item also averages the coincident periodic face nodes.  Ghost copies
+// read nodes at least one layer inside the opposite face, which no item of the same launch writes.
+struct FaceItem {
+  int i, j, k, g;
+};
+
+__device__ __forceinline__ bool face_item(const DevBlock &b, const DevFace &f, FaceItem &it) {
+  const long n0 = f.end[0] - f.start[0] + 1, n1 = f.end[1] - f.start[1] + 1, n2 = f.end[2] - f.start[2] + 1;
+  const long npt = n0 * n1 * n2;
+  const long item = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (item >= npt * b.g) return false;
+  long pt;
+  if (f.face == 0) {
+    it.g = (int)(item % b.g) + 1;
+    pt = item / b.g;
+  } else {
+    it.g = (int)(item / npt) + 1;
+    pt = item % npt;
+  }
+  it.i = f.start[0] + (int)(pt % n0);
+  it.j = f.start[1] + (int)((pt / n0) % n1);
+  it.k = f.start[2] + (int)(pt / (n0 * n1));
+  return true;
+}
+
+// periodic_exchange task 1 (DataCommunication.cu:628-683): average coincident face nodes, copy ghosts of
+// the collocated viscous fluxes (components 1..4; component 0 of the viscous fluxes is never used).
+__global__ __launch_bounds__(256) void k_viscous_exchange(DevBlock b, DevFace f) {
+  FaceItem it;
+  if (!face_item(b, f, it)) return;
+  int o[3] = {it.i, it.j, it.k};
   const int ext[3] = {b.mx, b.my, b.mz};
   o[f.face] = f.direction < 0 ? ext[f.face] - 1 : 0;
   int dv[3] = {0, 0, 0};
   dv[f.face] = f.direction;
-  const long id = nid(b, i, j, k), io = nid(b, o[0], o[1], o[2]), sz = b.sz;
+  const long sz = b.sz;
   double *A[3] = {b.fv, b.gv, b.hv};
-  for (int l = 0; l < 5; ++l)
-    for (int a = 0; a < 3; ++a) {
-      const double av = 0.5 * (A[a][id + l * sz] + A[a][io + l * sz]);
-      A[a][id + l * sz] = av;
-      A[a][io + l * sz] = av;
-    }
-  for (int g = 1; g <= b.g; ++g) {
-    const long ig = nid(b, i + g * dv[0], j + g * dv[1], k + g * dv[2]);
-    const long is = nid(b, o[0] + g * dv[0], o[1] + g * dv[1], o[2] + g * dv[2]);
-    for (int l = 0; l < 5; ++l)
-      for (int a = 0; a < 3; ++a) A[a][ig + l * sz] = A[a][is + l * sz];
+  if (it.g == 1) {
+    const long id = nid(b, it.i, it.j, it.k), io = nid(b, o[0], o[1], o[2]);
+    for (int l = 1; l < 5; ++l)
+      for (int a = 0; a < 3; ++a) {
+        const double av = 0.5 * (A[a][id + l * sz] + A[a][io + l * sz]);
+        A[a][id + l * sz] = av;
+        A[a][io + l * sz] = av;
+      }
   }
+  const int g = it.g;
+  const long ig = nid(b, it.i + g * dv[0], it.j + g * dv[1], it.k + g * dv[2]);
+  const long is = nid(b, o[0] + g * dv[0], o[1] + g * dv[1], o[2] + g * dv[2]);
+  for (int l = 1; l < 5; ++l)
+    for (int a = 0; a < 3; ++a) A[a][ig + l * sz] = A[a][is + l * sz];
 }
 
 // compute_entropy_fix_delta<Air> (InviscidScheme.cu:269-304) on the interior plus one layer.
@@ -541,8 +612,10 @@ __constant__ double kRkC[3] = {1.0, 0.25, 2.0 / 3.0};
 // dq = J (dFv/dxi + dGv/deta + dHv/dzeta) - sum_dir (F(i+1/2) - F(i-1/2)), then
 // update_cv_and_bv_rk<Air> (RK.cuh:224-280).  Stage 0 also saves U^n into qn.
 template<int ORDER, bool RHS_ONLY>
-__global__ __launch_bounds__(256) void k_update(DevBlock b, int rk, const double *__restrict__ dtp) {
-  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+__global__ __launch_bounds__(256) void k_update(DevBlock b, int rk, const double *__restrict__ dtp, TileGrid tg) {
+  Tile t;
+  if (!xcd_tile(tg, t)) return;
+  const int i = t.x * 64 + threadIdx.x, j = t.y * 4 + threadIdx.y, k = t.z;
   if (i >= b.mx || j >= b.my) return;
   const long id = nid(b, i, j, k), sz = b.sz;
   double r[5];
@@ -604,38 +677,33 @@ __global__ __launch_bounds__(256) void k_update(DevBlock b, int rk, const double
 
 // ---------------------------------------------------------------- boundary conditions
 // apply_outflow without back pressure (BoundCond.cu:2065-2188) and apply_periodic (BoundCond.cu:3033-3083).
-__global__ void k_boundary(DevBlock b, DevFace f) {
-  const int i = f.start[0] + blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = f.start[1] + blockIdx.y * blockDim.y + threadIdx.y;
-  const int k = f.start[2] + blockIdx.z * blockDim.z + threadIdx.z;
-  if (i > f.end[0] || j > f.end[1] || k > f.end[2]) return;
+__global__ __launch_bounds__(256) void k_boundary(DevBlock b, DevFace f) {
+  FaceItem it;
+  if (!face_item(b, f, it)) return;
   int dv[3] = {0, 0, 0};
   dv[f.face] = f.direction;
-  const long id = nid(b, i, j, k), sz = b.sz;
+  const long id = nid(b, it.i, it.j, it.k), sz = b.sz;
+  const int g = it.g;
+  const long ig = nid(b, it.i + g * dv[0], it.j + g * dv[1], it.k + g * dv[2]);
   if (f.type == 8) {
-    int o[3] = {i, j, k};
+    int o[3] = {it.i, it.j, it.k};
     const int ext[3] = {b.mx, b.my, b.mz};
     o[f.face] = f.direction < 0 ? ext[f.face] - 1 : 0;
-    const long io = nid(b, o[0], o[1], o[2]);
-    for (int l = 0; l < 6; ++l) {
-      const double a = 0.5 * (b.bv[id + l * sz] + b.bv[io + l * sz]);
-      b.bv[id + l * sz] = a;
-      b.bv[io + l * sz] = a;
+    if (g == 1) {
+      const long io = nid(b, o[0], o[1], o[2]);
+      for (int l = 0; l < 6; ++l) {
+        const double a = 0.5 * (b.bv[id + l * sz] + b.bv[io + l * sz]);
+        b.bv[id + l * sz] = a;
+        b.bv[io + l * sz] = a;
+      }
+      cons_from_prim(b, id);
     }
-    cons_from_prim(b, id);
-    for (int g = 1; g <= b.g; ++g) {
-      const long ig = nid(b, i + g * dv[0], j + g * dv[1], k + g * dv[2]);
-      const long is = nid(b, o[0] + g * dv[0], o[1] + g * dv[1], o[2] + g * dv[2]);
-      for (int l = 0; l < 6; ++l) b.bv[ig + l * sz] = b.bv[is + l * sz];
-      cons_from_prim(b, ig);
-    }
+    const long is = nid(b, o[0] + g * dv[0], o[1] + g * dv[1], o[2] + g * dv[2]);
+    for (int l = 0; l < 6; ++l) b.bv[ig + l * sz] = b.bv[is + l * sz];
   } else { // zero-gradient outflow
-    for (int g = 1; g <= b.g; ++g) {
-      const long ig = nid(b, i + g * dv[0], j + g * dv[1], k + g * dv[2]);
-      for (int l = 0; l < 6; ++l) b.bv[ig + l * sz] = b.bv[id + l * sz];
-      cons_from_prim(b, ig);
-    }
+    for (int l = 0; l < 6; ++l) b.bv[ig + l * sz] = b.bv[id + l * sz];
   }
+  cons_from_prim(b, ig);
 }
 
 __global__ void k_cv_from_bv(DevBlock b) {
@@ -701,17 +769,8 @@ __global__ void k_time_advance(double *time, const double *dt) { time[0] += dt[0
 
 dim3 node_grid(const DevBlock &b) { return dim3((b.mx + 63) / 64, (b.my + 3) / 4, b.mz); }
 
-dim3 face_grid(const DevFace &f, dim3 &tpb, int ngg) {
-  unsigned t[3], g[3];
-  for (int d = 0; d < 3; ++d) {
-    const int n = f.end[d] - f.start[d] + 1;
-    t[d] = n <= 2 * ngg + 1 ? 1 : 16;
-    g[d] = (n - 1) / t[d] + 1;
-  }
-  // keep the workgroup at most 256 lanes (16 x 16 x 1 when two tangential extents are large)
-  if (t[0] * t[1] * t[2] > 256) t[2] = 1, g[2] = f.end[2] - f.start[2] + 1;
-  tpb = dim3(t[0], t[1], t[2]);
-  return dim3(g[0], g[1], g[2]);
+TileGrid node_tiles(const DevBlock &b, int fast) {
+  return TileGrid{(b.mx + 63) / 64, (b.my + 3) / 4, b.mz, fast};
 }
 
 } // namespace
@@ -721,24 +780,26 @@ hipError_t launch_cv_from_bv_all(const DevBlock &b, hipStream_t s) {
   return hipGetLastError();
 }
 
+static unsigned face_blocks(const DevBlock &b, const DevFace &f) {
+  const long n = (long)(f.end[0] - f.start[0] + 1) * (f.end[1] - f.start[1] + 1) * (f.end[2] - f.start[2] + 1) * b.g;
+  return (unsigned)((n + 255) / 256);
+}
+
 hipError_t launch_boundary(const DevBlock &b, const DevFace &f, hipStream_t s) {
-  dim3 tpb;
-  const dim3 grid = face_grid(f, tpb, b.g);
-  k_boundary<<<grid, tpb, 0, s>>>(b, f);
+  k_boundary<<<face_blocks(b, f), 256, 0, s>>>(b, f);
   return hipGetLastError();
 }
 
 hipError_t launch_viscous_flux(const DevBlock &b, const Scheme &sc, hipStream_t s) {
   const dim3 tpb(64, 4, 1);
-  if (sc.viscous_order == 8) k_viscous_flux<8><<<node_grid(b), tpb, 0, s>>>(b, sc);
-  else k_viscous_flux<4><<<node_grid(b), tpb, 0, s>>>(b, sc);
+  const TileGrid tg = node_tiles(b, 0);
+  if (sc.viscous_order == 8) k_viscous_flux<8><<<tile_blocks(tg), tpb, 0, s>>>(b, sc, tg);
+  else k_viscous_flux<4><<<tile_blocks(tg), tpb, 0, s>>>(b, sc, tg);
   return hipGetLastError();
 }
 
 hipError_t launch_viscous_exchange(const DevBlock &b, const DevFace &f, hipStream_t s) {
-  dim3 tpb;
-  const dim3 grid = face_grid(f, tpb, b.g);
-  k_viscous_exchange<<<grid, tpb, 0, s>>>(b, f);
+  k_viscous_exchange<<<face_blocks(b, f), 256, 0, s>>>(b, f);
   return hipGetLastError();
 }
 
@@ -751,11 +812,13 @@ hipError_t launch_entropy_fix(const DevBlock &b, const Scheme &sc, hipStream_t s
 template<int DIR, int R>
 static void dispatch_rs(const DevBlock &b, const Scheme &sc, dim3 grid, double *F, hipStream_t s) {
   const dim3 tpb(64, 4, 1);
+  const TileGrid tg{(int)grid.x, (int)grid.y, (int)grid.z, DIR};
+  const unsigned nb = tile_blocks(tg);
   switch (sc.inviscid_scheme) {
-    case 1: k_inviscid<DIR, R, 1><<<grid, tpb, 0, s>>>(b, sc, F); break;
-    case 2: k_inviscid<DIR, R, 2><<<grid, tpb, 0, s>>>(b, sc, F); break;
-    case 4: k_inviscid<DIR, R, 4><<<grid, tpb, 0, s>>>(b, sc, F); break;
-    default: k_inviscid<DIR, R, 3><<<grid, tpb, 0, s>>>(b, sc, F); break;
+    case 1: k_inviscid<DIR, R, 1><<<nb, tpb, 0, s>>>(b, sc, F, tg); break;
+    case 2: k_inviscid<DIR, R, 2><<<nb, tpb, 0, s>>>(b, sc, F, tg); break;
+    case 4: k_inviscid<DIR, R, 4><<<nb, tpb, 0, s>>>(b, sc, F, tg); break;
+    default: k_inviscid<DIR, R, 3><<<nb, tpb, 0, s>>>(b, sc, F, tg); break;
   }
 }
 
@@ -781,16 +844,18 @@ hipError_t launch_inviscid_flux(const DevBlock &b, const Scheme &sc, int dir, hi
 
 hipError_t launch_update(const DevBlock &b, const Scheme &sc, int rk, const double *dt_dev, bool rhs_only,
                          hipStream_t s) {
-  const dim3 tpb(64, 4, 1), grid = node_grid(b);
+  const dim3 tpb(64, 4, 1);
+  const TileGrid tg = node_tiles(b, 0);
+  const unsigned grid = tile_blocks(tg);
   const int o = sc.viscous_order;
   if (rhs_only) {
-    if (o == 8) k_update<8, true><<<grid, tpb, 0, s>>>(b, rk, dt_dev);
-    else if (o == 4) k_update<4, true><<<grid, tpb, 0, s>>>(b, rk, dt_dev);
-    else k_update<0, true><<<grid, tpb, 0, s>>>(b, rk, dt_dev);
+    if (o == 8) k_update<8, true><<<grid, tpb, 0, s>>>(b, rk, dt_dev, tg);
+    else if (o == 4) k_update<4, true><<<grid, tpb, 0, s>>>(b, rk, dt_dev, tg);
+    else k_update<0, true><<<grid, tpb, 0, s>>>(b, rk, dt_dev, tg);
   } else {
-    if (o == 8) k_update<8, false><<<grid, tpb, 0, s>>>(b, rk, dt_dev);
-    else if (o == 4) k_update<4, false><<<grid, tpb, 0, s>>>(b, rk, dt_dev);
-    else k_update<0, false><<<grid, tpb, 0, s>>>(b, rk, dt_dev);
+    if (o == 8) k_update<8, false><<<grid, tpb, 0, s>>>(b, rk, dt_dev, tg);
+    else if (o == 4) k_update<4, false><<<grid, tpb, 0, s>>>(b, rk, dt_dev, tg);
+    else k_update<0, false><<<grid, tpb, 0, s>>>(b, rk, dt_dev, tg);
   }
   return hipGetLastError();
 }
