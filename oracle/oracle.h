@@ -100,6 +100,42 @@ void orc_block_rk3_step(orc_block *b, double dt);
 /* Number of OpenMP threads the oracle uses (for the CPU baseline report). */
 int orc_num_threads(void);
 
+/* ---- chemistry of one constant-volume cell ---- */
+
+#define ORC_MAX_SPEC 16
+#define ORC_MAX_REAC 48
+
+/* Mechanism tables as the reference keeps them in DParameter (DParameter.cuh:13-207): NASA-7 two-range
+ * coefficients, reaction types (0 irreversible, 1 reversible, 3 duplicate pair, 4 third body,
+ * 5 Lindemann, 6/7 Troe with 3/4 parameters), Arrhenius triples (Ea in K), stoichiometry, order
+ * (sum of product minus reactant coefficients), third-body efficiencies. */
+typedef struct {
+  int ns, nr;
+  double mw[ORC_MAX_SPEC];
+  double t_low[ORC_MAX_SPEC], t_mid[ORC_MAX_SPEC], t_high[ORC_MAX_SPEC];
+  double lo[ORC_MAX_SPEC][7], hi[ORC_MAX_SPEC][7];
+  int type[ORC_MAX_REAC], rev[ORC_MAX_REAC];
+  double A[ORC_MAX_REAC], b[ORC_MAX_REAC], Ea[ORC_MAX_REAC];
+  double A2[ORC_MAX_REAC], b2[ORC_MAX_REAC], Ea2[ORC_MAX_REAC];
+  double troe[ORC_MAX_REAC][4];
+  double order[ORC_MAX_REAC];
+  double nf[ORC_MAX_REAC][ORC_MAX_SPEC], nb[ORC_MAX_REAC][ORC_MAX_SPEC];
+  double tb[ORC_MAX_REAC][ORC_MAX_SPEC];
+} orc_mech;
+
+/* h_k, cp_k per unit mass (Thermo.cu:470-520). */
+void orc_chem_h_cp(const orc_mech *m, double t, double *h, double *cp);
+/* Chemical source and rates of progress (FiniteRateChem.cu:1466-1605). */
+void orc_chem_source(const orc_mech *m, double t, const double *rhoY, double *q1, double *q2, double *omega_d,
+                     double *omega);
+/* d omega / d(rho Y) (ConstVolumeReactor.cpp:1168-1187). */
+void orc_chem_jacobian(const orc_mech *m, const double *rhoY, const double *q1, const double *q2, double *jac);
+/* T from internal energy by Newton (ConstVolumeReactor.cpp:970-995). */
+double orc_chem_update_t(const orc_mech *m, double t0, const double *yk, double E);
+/* Adaptive ROK4E at constant rho and E (ConstVolumeReactor.cpp:173-497). */
+int orc_chem_rok4e(const orc_mech *m, double rho, double E, double *rhoY, double *T, double t_end, double rtol,
+                   double atol, double *h, int max_steps, int *n_rej, double *rec, int n_rec);
+
 /* ---- exact solution ---- */
 
 /* Exact Riemann solution of the 1D Euler equations (Toro, "Riemann Solvers and Numerical

@@ -165,3 +165,88 @@ def exact_riemann(left, right, gamma, x0, t, x):
     rho, u, p = np.zeros_like(x), np.zeros_like(x), np.zeros_like(x)
     ps = load().orc_exact_riemann(*left, *right, gamma, x0, t, x.size, _p(x), _p(rho), _p(u), _p(p))
     return rho, u, p, ps
+
+
+# ---------------------------------------------------------------- chemistry
+MAXS, MAXR = 16, 48
+
+
+class OrcMech(ctypes.Structure):
+    _fields_ = [("ns", ctypes.c_int), ("nr", ctypes.c_int), ("mw", ctypes.c_double * MAXS),
+                ("t_low", ctypes.c_double * MAXS), ("t_mid", ctypes.c_double * MAXS),
+                ("t_high", ctypes.c_double * MAXS), ("lo", (ctypes.c_double * 7) * MAXS),
+                ("hi", (ctypes.c_double * 7) * MAXS), ("type", ctypes.c_int * MAXR), ("rev", ctypes.c_int * MAXR),
+                ("A", ctypes.c_double * MAXR), ("b", ctypes.c_double * MAXR), ("Ea", ctypes.c_double * MAXR),
+                ("A2", ctypes.c_double * MAXR), ("b2", ctypes.c_double * MAXR), ("Ea2", ctypes.c_double * MAXR),
+                ("troe", (ctypes.c_double * 4) * MAXR), ("order", ctypes.c_double * MAXR),
+                ("nf", (ctypes.c_double * MAXS) * MAXR), ("nb", (ctypes.c_double * MAXS) * MAXR),
+                ("tb", (ctypes.c_double * MAXS) * MAXR)]
+
+
+def mech_struct(d):
+    """OrcMech from the dict of tests/chem_inputs.parse."""
+    m = OrcMech()
+    m.ns, m.nr = d["ns"], d["nr"]
+    for k in range(d["ns"]):
+        m.mw[k], m.t_low[k], m.t_mid[k], m.t_high[k] = d["mw"][k], d["t_low"][k], d["t_mid"][k], d["t_high"][k]
+        for c in range(7):
+            m.lo[k][c], m.hi[k][c] = d["lo"][k][c], d["hi"][k][c]
+    for r in range(d["nr"]):
+        m.type[r], m.rev[r] = int(d["type"][r]), int(d["rev"][r])
+        for key in ("A", "b", "Ea", "A2", "b2", "Ea2", "order"):
+            getattr(m, key)[r] = d[key][r]
+        for c in range(4):
+            m.troe[r][c] = d["troe"][r][c]
+        for k in range(d["ns"]):
+            m.nf[r][k], m.nb[r][k], m.tb[r][k] = d["nf"][r][k], d["nb"][r][k], d["tb"][r][k]
+    return m
+
+
+def _chem_protos(lib):
+    D, I, P = ctypes.c_double, ctypes.c_int, ctypes.c_void_p
+    M = ctypes.POINTER(OrcMech)
+    lib.orc_chem_h_cp.argtypes = [M, D, P, P]
+    lib.orc_chem_source.argtypes = [M, D, P, P, P, P, P]
+    lib.orc_chem_jacobian.argtypes = [M, P, P, P, P]
+    lib.orc_chem_update_t.restype = D
+    lib.orc_chem_update_t.argtypes = [M, D, P, D]
+    lib.orc_chem_rok4e.restype = I
+    lib.orc_chem_rok4e.argtypes = [M, D, D, P, ctypes.POINTER(D), D, D, D, ctypes.POINTER(D), I,
+                                   ctypes.POINTER(I), P, I]
+
+
+def chem_h_cp(m, T):
+    lib = load()
+    _chem_protos(lib)
+    h, cp = np.zeros(m.ns), np.zeros(m.ns)
+    lib.orc_chem_h_cp(ctypes.byref(m), T, _p(h), _p(cp))
+    return h, cp
+
+
+def chem_source(m, T, rhoY):
+    lib = load()
+    _chem_protos(lib)
+    rhoY = np.ascontiguousarray(rhoY, float)
+    q1, q2 = np.zeros(m.nr), np.zeros(m.nr)
+    wd, w = np.zeros(m.ns), np.zeros(m.ns)
+    lib.orc_chem_source(ctypes.byref(m), T, _p(rhoY), _p(q1), _p(q2), _p(wd), _p(w))
+    return w, wd, q1, q2
+
+
+def chem_update_t(m, T0, yk, E):
+    lib = load()
+    _chem_protos(lib)
+    yk = np.ascontiguousarray(yk, float)
+    return lib.orc_chem_update_t(ctypes.byref(m), T0, _p(yk), E)
+
+
+def chem_rok4e(m, rho, E, rhoY, T, t_end, rtol=1e-6, atol=1e-12, h=1e-8, max_steps=10 ** 7, n_rec=0):
+    """Adaptive ROK4E of one cell.  Returns (rhoY, T, h, n_accepted, n_rejected, record[(t, T)])."""
+    lib = load()
+    _chem_protos(lib)
+    y = np.ascontiguousarray(rhoY, float).copy()
+    Tc, hc, rej = ctypes.c_double(T), ctypes.c_double(h), ctypes.c_int(0)
+    rec = np.zeros(2 * max(n_rec, 1))
+    n = lib.orc_chem_rok4e(ctypes.byref(m), rho, E, _p(y), ctypes.byref(Tc), t_end, rtol, atol, ctypes.byref(hc),
+                           max_steps, ctypes.byref(rej), _p(rec), n_rec)
+    return y, Tc.value, hc.value, n, rej.value, rec[:2 * min(n, n_rec)].reshape(-1, 2)
