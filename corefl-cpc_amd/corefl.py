@@ -20,6 +20,10 @@ SYMBOLS = (
     "corefl_block_view_get", "corefl_get_array", "corefl_set_state", "corefl_rhs", "corefl_compute_dt",
     "corefl_rk3_step", "corefl_advance", "corefl_get_time", "corefl_synchronize", "corefl_get_int",
     "corefl_profile_kernels", "corefl_get_real",
+    "corefl_chem_create", "corefl_chem_create_mem", "corefl_chem_destroy", "corefl_chem_info", "corefl_chem_name",
+    "corefl_chem_table", "corefl_chem_set_cells", "corefl_chem_get_cells", "corefl_chem_view_get",
+    "corefl_chem_source", "corefl_chem_integrate", "corefl_chem_set_launch", "corefl_chem_get_launch",
+    "corefl_chem_autotune", "corefl_chem_synchronize",
 )
 
 
@@ -28,6 +32,12 @@ class BlockView(ctypes.Structure):
                 ("n_var", ctypes.c_int), ("n_spec", ctypes.c_int), ("sz", ctypes.c_longlong),
                 ("cv", ctypes.c_void_p), ("bv", ctypes.c_void_p), ("qn", ctypes.c_void_p),
                 ("metric", ctypes.c_void_p), ("jac", ctypes.c_void_p), ("dq", ctypes.c_void_p)]
+
+
+class ChemView(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int), ("n_spec", ctypes.c_int), ("rho", ctypes.c_void_p), ("e", ctypes.c_void_p),
+                ("T", ctypes.c_void_p), ("rhoY", ctypes.c_void_p), ("h", ctypes.c_void_p),
+                ("steps", ctypes.c_void_p), ("rejects", ctypes.c_void_p)]
 
 
 _lib = None
@@ -61,6 +71,23 @@ def load_library(path=LIB_PATH):
     lib.corefl_get_int.argtypes = [P, ctypes.c_char_p, ctypes.POINTER(I)]
     lib.corefl_get_real.argtypes = [P, ctypes.c_char_p, ctypes.POINTER(D)]
     lib.corefl_profile_kernels.argtypes = [P, I, ctypes.POINTER(D), I]
+    LL, PI = ctypes.c_longlong, ctypes.POINTER(I)
+    lib.corefl_chem_create.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(P)]
+    lib.corefl_chem_create_mem.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(P)]
+    lib.corefl_chem_destroy.argtypes = [P]
+    lib.corefl_chem_destroy.restype = None
+    lib.corefl_chem_info.argtypes = [P, PI, PI, PI]
+    lib.corefl_chem_name.argtypes = [P, I, I, ctypes.c_char_p, I]
+    lib.corefl_chem_table.argtypes = [P, ctypes.c_char_p, P, LL]
+    lib.corefl_chem_set_cells.argtypes = [P, LL, P, P, P, P]
+    lib.corefl_chem_get_cells.argtypes = [P, P, P, P, P, P]
+    lib.corefl_chem_view_get.argtypes = [P, ctypes.POINTER(ChemView)]
+    lib.corefl_chem_source.argtypes = [P, P]
+    lib.corefl_chem_integrate.argtypes = [P, D, D, D, I, P]
+    lib.corefl_chem_set_launch.argtypes = [P, I, I, I]
+    lib.corefl_chem_get_launch.argtypes = [P, PI, PI, PI]
+    lib.corefl_chem_autotune.argtypes = [P, D, D, D, I, I, ctypes.POINTER(D), I]
+    lib.corefl_chem_synchronize.argtypes = [P]
     _lib = lib
     return lib
 
@@ -172,3 +199,125 @@ class Solver:
         v = ctypes.c_double()
         _check(self._lib.corefl_get_real(self._ctx, name.encode(), ctypes.byref(v)))
         return v.value
+
+
+class Chemistry:
+    """Finite-rate chemistry of a batch of constant-volume cells (CHEMKIN + NASA-7 inputs).
+
+    Cell arrays are structure-of-arrays: rhoY has shape (n_spec, n)."""
+
+    AUTOTUNE_CANDIDATES = tuple([("static", t, 0) for t in (64, 128, 256)] +
+                                [("queue", t, w) for t in (64, 128, 256) for w in (4, 8, 16)])
+
+    def __init__(self, mechanism_file=None, therm_file=None, *, mech_text=None, therm_text=""):
+        lib = load_library()
+        self._lib = lib
+        self._ctx = ctypes.c_void_p()
+        if mech_text is not None:
+            _check(lib.corefl_chem_create_mem(mech_text.encode(), therm_text.encode(), ctypes.byref(self._ctx)))
+        else:
+            _check(lib.corefl_chem_create(os.fsencode(mechanism_file), os.fsencode(therm_file or ""),
+                                          ctypes.byref(self._ctx)))
+        ns, nr, ne = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _check(lib.corefl_chem_info(self._ctx, ctypes.byref(ns), ctypes.byref(nr), ctypes.byref(ne)))
+        self.n_spec, self.n_reac, self.n_elem = ns.value, nr.value, ne.value
+        self.n = 0
+
+    def close(self):
+        if self._ctx:
+            self._lib.corefl_chem_destroy(self._ctx)
+            self._ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def name(self, index, kind=0):
+        buf = ctypes.create_string_buffer(32)
+        _check(self._lib.corefl_chem_name(self._ctx, kind, index, buf, 32))
+        return buf.value.decode()
+
+    @property
+    def species(self):
+        return [self.name(k) for k in range(self.n_spec)]
+
+    @property
+    def elements(self):
+        return [self.name(k, 1) for k in range(self.n_elem)]
+
+    _SHAPES = {"lo": (-1, 7), "hi": (-1, 7), "troe": (-1, 4)}
+
+    def table(self, name):
+        ns, nr, ne = self.n_spec, self.n_reac, self.n_elem
+        size = {"lo": ns * 7, "hi": ns * 7, "troe": nr * 4, "nf": nr * ns, "nb": nr * ns, "tb": nr * ns,
+                "comp": ns * ne}.get(name, ns if name in ("mw", "t_low", "t_mid", "t_high") else nr)
+        out = np.empty(size, dtype=np.float64)
+        _check(self._lib.corefl_chem_table(self._ctx, name.encode(), out.ctypes.data, size))
+        if name in ("nf", "nb", "tb"):
+            return out.reshape(nr, ns)
+        if name == "comp":
+            return out.reshape(ns, ne)
+        return out.reshape(self._SHAPES[name]) if name in self._SHAPES else out
+
+    def set_cells(self, rho, T, rhoY, e=None):
+        rho = np.ascontiguousarray(rho, dtype=np.float64)
+        T = np.ascontiguousarray(T, dtype=np.float64)
+        rhoY = np.ascontiguousarray(rhoY, dtype=np.float64)
+        n = rho.size
+        if T.size != n or rhoY.shape != (self.n_spec, n):
+            raise ValueError("set_cells: rho (n,), T (n,), rhoY (n_spec, n) expected")
+        ep = None
+        if e is not None:
+            e = np.ascontiguousarray(e, dtype=np.float64)
+            ep = e.ctypes.data
+        _check(self._lib.corefl_chem_set_cells(self._ctx, n, rho.ctypes.data, ep, T.ctypes.data, rhoY.ctypes.data))
+        self.n = n
+
+    def get_cells(self):
+        n = self.n
+        T = np.empty(n)
+        rhoY = np.empty((self.n_spec, n))
+        e = np.empty(n)
+        steps = np.empty(n, dtype=np.int32)
+        rej = np.empty(n, dtype=np.int32)
+        _check(self._lib.corefl_chem_get_cells(self._ctx, T.ctypes.data, rhoY.ctypes.data, e.ctypes.data,
+                                               steps.ctypes.data, rej.ctypes.data))
+        return dict(T=T, rhoY=rhoY, e=e, steps=steps, rejects=rej)
+
+    def view(self):
+        v = ChemView()
+        _check(self._lib.corefl_chem_view_get(self._ctx, ctypes.byref(v)))
+        return v
+
+    def source(self):
+        out = np.empty((self.n_spec, self.n))
+        _check(self._lib.corefl_chem_source(self._ctx, out.ctypes.data))
+        return out
+
+    def integrate(self, dt, rtol=1e-6, atol=1e-8, max_attempts=100000, sync=True):
+        _check(self._lib.corefl_chem_integrate(self._ctx, float(dt), float(rtol), float(atol), int(max_attempts),
+                                               None))
+        if sync:
+            self.sync()
+
+    def set_launch(self, queue, threads=64, blocks=1024):
+        _check(self._lib.corefl_chem_set_launch(self._ctx, 1 if queue else 0, int(threads), int(blocks)))
+
+    def get_launch(self):
+        q, t, b = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _check(self._lib.corefl_chem_get_launch(self._ctx, ctypes.byref(q), ctypes.byref(t), ctypes.byref(b)))
+        return dict(queue=q.value, threads=t.value, blocks=b.value)
+
+    def autotune(self, dt, rtol=1e-6, atol=1e-8, max_attempts=100000, reps=1):
+        """Time every candidate launch shape on the current batch; keeps the fastest.  Returns ms per
+        candidate, keyed like AUTOTUNE_CANDIDATES."""
+        n_c = len(self.AUTOTUNE_CANDIDATES)
+        out = (ctypes.c_double * n_c)()
+        _check(self._lib.corefl_chem_autotune(self._ctx, float(dt), float(rtol), float(atol), int(max_attempts),
+                                              int(reps), out, n_c))
+        return dict(zip(self.AUTOTUNE_CANDIDATES, list(out)))
+
+    def sync(self):
+        _check(self._lib.corefl_chem_synchronize(self._ctx))

@@ -1,0 +1,38 @@
+// Finite-rate chemistry tables (DParameter's chemistry part, DParameter.cuh:13-207) and the parser of
+// COREFL's chemistry inputs: CHEMKIN mechanism + NASA-7 thermo (ChemData.cpp:10-440, 868-1177).
+#pragma once
+
+#include <string>
+
+namespace corefl {
+
+constexpr int kMaxSpec = 16, kMaxReac = 48, kMaxElem = 8, kMaxSide = 4;
+
+// Plain data, copied to device memory once and read uniformly by every lane.
+struct ChemMech {
+  int ns, nr, ne;
+  double mw[kMaxSpec];  // kg/kmol from atomic weights (Element.cpp)
+  double t_low[kMaxSpec], t_mid[kMaxSpec], t_high[kMaxSpec];
+  double lo[kMaxSpec][7], hi[kMaxSpec][7];
+  // 0 irreversible, 1 reversible, 3 merged DUPLICATE pair, 4 third body, 5 Lindemann, 6/7 Troe (3/4 params)
+  int type[kMaxReac], rev[kMaxReac];
+  double A[kMaxReac], b[kMaxReac], Ea[kMaxReac]; // Ea in K
+  double A2[kMaxReac], b2[kMaxReac], Ea2[kMaxReac];
+  double troe[kMaxReac][4];
+  double order[kMaxReac]; // sum of product minus reactant coefficients
+  int need_gibbs;         // some reversible reaction lacks REV parameters: use equilibrium constants
+  // sparse stoichiometry: distinct species on each side with integer coefficients
+  int nfs[kMaxReac], nbs[kMaxReac];
+  int fsp[kMaxReac][kMaxSide], bsp[kMaxReac][kMaxSide];
+  int fnu[kMaxReac][kMaxSide], bnu[kMaxReac][kMaxSide];
+  double nf[kMaxReac][kMaxSpec], nb[kMaxReac][kMaxSpec]; // dense stoichiometric coefficients
+  double tb[kMaxReac][kMaxSpec]; // third-body efficiencies (default 1)
+  int comp[kMaxSpec][kMaxElem];  // element composition
+  char species[kMaxSpec][16];
+  char elements[kMaxElem][4];
+};
+
+// Throws std::runtime_error with a message on malformed input or table overflow.
+ChemMech parse_mechanism(const std::string &mech_text, const std::string &therm_text);
+
+} // namespace corefl

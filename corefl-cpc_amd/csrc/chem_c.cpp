@@ -1,0 +1,406 @@
+// C ABI of the batched chemistry: mechanism tables, cell batches in device memory, the source term,
+// the ROK4E sub-step and the launch-shape autotuner.
+#include "../../include/corefl_c.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <fstream>
+#include <memory>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "chem.hpp"
+#include "chem_kernels.hpp"
+
+namespace corefl {
+void set_last_error(const std::string &msg);
+}
+
+struct corefl_chem {
+  corefl::ChemMech mech{};
+  corefl::ChemMech *mech_dev = nullptr;
+  hipStream_t stream = nullptr;
+  int *counter = nullptr;
+  int cap = 0;
+  corefl::ChemBatch b{};
+  double *save = nullptr; // autotuner copy of T, rhoY, h
+  corefl::ChemLaunch cfg{1, 64, 0};
+  bool tuned = false;
+  ~corefl_chem();
+};
+
+namespace {
+
+#define HIPCHK(x)                                                                                          \
+  do {                                                                                                     \
+    hipError_t e_ = (x);                                                                                   \
+    if (e_ != hipSuccess) throw std::runtime_error(std::string(#x) + ": " + hipGetErrorString(e_));       \
+  } while (0)
+
+std::string read_file(const std::string &path) {
+  std::ifstream f(path);
+  if (!f) throw std::runtime_error("cannot open " + path);
+  std::stringstream ss;
+  ss << f.rdbuf();
+  return ss.str();
+}
+
+int fail(const std::exception &e) {
+  corefl::set_last_error(e.what());
+  return -1;
+}
+
+template<class F> int guarded(F &&f) {
+  try {
+    f();
+    return 0;
+  } catch (const std::exception &e) {
+    return fail(e);
+  }
+}
+
+void free_batch(corefl_chem *c) {
+  for (void *p : {(void *)c->b.rho, (void *)c->b.e, (void *)c->b.T, (void *)c->b.rhoY, (void *)c->b.h,
+                  (void *)c->b.steps, (void *)c->b.rejects, (void *)c->save})
+    if (p) (void)hipFree(p);
+  c->b = corefl::ChemBatch{};
+  c->save = nullptr;
+  c->cap = 0;
+}
+
+void ensure_device(corefl_chem *c) {
+  if (c->mech_dev) return;
+  if (c->mech.ns != 9)
+    throw std::runtime_error("the device chemistry kernels are instantiated for 9 species; this mechanism has " +
+                             std::to_string(c->mech.ns));
+  HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  HIPCHK(hipMalloc(&c->mech_dev, sizeof(corefl::ChemMech)));
+  HIPCHK(hipMalloc(&c->counter, sizeof(int)));
+  HIPCHK(hipMemcpy(c->mech_dev, &c->mech, sizeof(corefl::ChemMech), hipMemcpyHostToDevice));
+  int cus = 0;
+  HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0));
+  c->cfg = corefl::ChemLaunch{1, 64, 4 * cus};
+}
+
+void ensure_batch(corefl_chem *c, long long n) {
+  if (n < 0 || n > (1LL << 30)) throw std::runtime_error("batch size out of range [0, 2^30]");
+  ensure_device(c);
+  if (n <= c->cap) {
+    c->b.n = (int)n;
+    return;
+  }
+  free_batch(c);
+  const int ns = c->mech.ns;
+  const size_t d = sizeof(double) * (size_t)n;
+  HIPCHK(hipMalloc(&c->b.rho, d));
+  HIPCHK(hipMalloc(&c->b.e, d));
+  HIPCHK(hipMalloc(&c->b.T, d));
+  HIPCHK(hipMalloc(&c->b.rhoY, d * ns));
+  HIPCHK(hipMalloc(&c->b.h, d));
+  HIPCHK(hipMalloc(&c->b.steps, sizeof(int) * (size_t)n));
+  HIPCHK(hipMalloc(&c->b.rejects, sizeof(int) * (size_t)n));
+  c->cap = (int)n;
+  c->b.n = (int)n;
+}
+
+hipStream_t pick(corefl_chem *c, void *stream) { return stream ? (hipStream_t)stream : c->stream; }
+
+void integrate(corefl_chem *c, const corefl::ChemLaunch &cfg, double dt, double rtol, double atol, int max_attempts,
+               hipStream_t s) {
+  HIPCHK(corefl::launch_chem_integrate(c->mech_dev, c->mech.ns, c->b, dt, rtol, atol, max_attempts, cfg, c->counter,
+                                       s));
+}
+
+void check_args(double dt, double rtol, double atol, int max_attempts) {
+  if (!(dt > 0) || !(rtol > 0) || !(atol > 0)) throw std::runtime_error("dt, rtol and atol must be positive");
+  if (max_attempts < 1) throw std::runtime_error("max_attempts must be at least 1");
+}
+
+} // namespace
+
+corefl_chem::~corefl_chem() {
+  free_batch(this);
+  if (mech_dev) (void)hipFree(mech_dev);
+  if (counter) (void)hipFree(counter);
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+extern "C" {
+
+int corefl_chem_create_mem(const char *mech_text, const char *therm_text, corefl_chem **out) {
+  return guarded([&] {
+    if (!mech_text || !therm_text || !out) throw std::runtime_error("corefl_chem_create_mem: null argument");
+    auto c = std::make_unique<corefl_chem>();
+    c->mech = corefl::parse_mechanism(mech_text, therm_text);
+    *out = c.release();
+  });
+}
+
+int corefl_chem_create(const char *mechanism_file, const char *therm_file, corefl_chem **out) {
+  return guarded([&] {
+    if (!mechanism_file || !out) throw std::runtime_error("corefl_chem_create: null argument");
+    const std::string mech = read_file(mechanism_file);
+    const std::string therm = therm_file && *therm_file ? read_file(therm_file) : std::string();
+    auto c = std::make_unique<corefl_chem>();
+    c->mech = corefl::parse_mechanism(mech, therm);
+    *out = c.release();
+  });
+}
+
+void corefl_chem_destroy(corefl_chem *c) { delete c; }
+
+int corefl_chem_info(const corefl_chem *c, int *n_spec, int *n_reac, int *n_elem) {
+  return guarded([&] {
+    if (!c) throw std::runtime_error("corefl_chem_info: null context");
+    if (n_spec) *n_spec = c->mech.ns;
+    if (n_reac) *n_reac = c->mech.nr;
+    if (n_elem) *n_elem = c->mech.ne;
+  });
+}
+
+int corefl_chem_name(const corefl_chem *c, int kind, int index, char *buf, int len) {
+  return guarded([&] {
+    if (!c || !buf || len < 1) throw std::runtime_error("corefl_chem_name: bad arguments");
+    const char *src = nullptr;
+    if (kind == 0 && index >= 0 && index < c->mech.ns) src = c->mech.species[index];
+    if (kind == 1 && index >= 0 && index < c->mech.ne) src = c->mech.elements[index];
+    if (!src) throw std::runtime_error("corefl_chem_name: index out of range");
+    std::snprintf(buf, (size_t)len, "%s", src);
+  });
+}
+
+int corefl_chem_table(const corefl_chem *c, const char *name, double *out, long long n) {
+  return guarded([&] {
+    if (!c || !name || !out) throw std::runtime_error("corefl_chem_table: null argument");
+    const auto &m = c->mech;
+    const int ns = m.ns, nr = m.nr;
+    std::vector<double> v;
+    const std::string k(name);
+    auto per_reac = [&](const double *a) { v.assign(a, a + nr); };
+    auto per_spec = [&](const double *a) { v.assign(a, a + ns); };
+    if (k == "mw") per_spec(m.mw);
+    else if (k == "t_low") per_spec(m.t_low);
+    else if (k == "t_mid") per_spec(m.t_mid);
+    else if (k == "t_high") per_spec(m.t_high);
+    else if (k == "lo" || k == "hi") {
+      for (int s = 0; s < ns; ++s)
+        for (int q = 0; q < 7; ++q) v.push_back(k == "lo" ? m.lo[s][q] : m.hi[s][q]);
+    } else if (k == "A") per_reac(m.A);
+    else if (k == "b") per_reac(m.b);
+    else if (k == "Ea") per_reac(m.Ea);
+    else if (k == "A2") per_reac(m.A2);
+    else if (k == "b2") per_reac(m.b2);
+    else if (k == "Ea2") per_reac(m.Ea2);
+    else if (k == "order") per_reac(m.order);
+    else if (k == "type" || k == "rev") {
+      for (int r = 0; r < nr; ++r) v.push_back(k == "type" ? m.type[r] : m.rev[r]);
+    } else if (k == "troe") {
+      for (int r = 0; r < nr; ++r)
+        for (int q = 0; q < 4; ++q) v.push_back(m.troe[r][q]);
+    } else if (k == "nf" || k == "nb" || k == "tb") {
+      for (int r = 0; r < nr; ++r)
+        for (int s = 0; s < ns; ++s) v.push_back(k == "nf" ? m.nf[r][s] : k == "nb" ? m.nb[r][s] : m.tb[r][s]);
+    } else if (k == "comp") {
+      for (int s = 0; s < ns; ++s)
+        for (int e = 0; e < m.ne; ++e) v.push_back(m.comp[s][e]);
+    } else {
+      throw std::runtime_error("corefl_chem_table: unknown table " + k);
+    }
+    if ((long long)v.size() != n)
+      throw std::runtime_error("corefl_chem_table: " + k + " has " + std::to_string(v.size()) + " values, not " +
+                               std::to_string(n));
+    std::copy(v.begin(), v.end(), out);
+  });
+}
+
+int corefl_chem_set_cells(corefl_chem *c, long long n, const double *rho, const double *e, const double *T,
+                          const double *rhoY) {
+  return guarded([&] {
+    if (!c || (n > 0 && (!rho || !T || !rhoY))) throw std::runtime_error("corefl_chem_set_cells: null argument");
+    ensure_batch(c, n);
+    if (n == 0) return;
+    const size_t d = sizeof(double) * (size_t)n;
+    HIPCHK(hipMemcpyAsync(c->b.rho, rho, d, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->b.T, T, d, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->b.rhoY, rhoY, d * c->mech.ns, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemsetAsync(c->b.h, 0, d, c->stream));
+    HIPCHK(hipMemsetAsync(c->b.steps, 0, sizeof(int) * (size_t)n, c->stream));
+    HIPCHK(hipMemsetAsync(c->b.rejects, 0, sizeof(int) * (size_t)n, c->stream));
+    if (e) HIPCHK(hipMemcpyAsync(c->b.e, e, d, hipMemcpyHostToDevice, c->stream));
+    else HIPCHK(corefl::launch_chem_energy(c->mech_dev, c->mech.ns, c->b, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int corefl_chem_get_cells(corefl_chem *c, double *T, double *rhoY, double *e, int *steps, int *rejects) {
+  return guarded([&] {
+    if (!c) throw std::runtime_error("corefl_chem_get_cells: null context");
+    const long long n = c->b.n;
+    if (n == 0) return;
+    const size_t d = sizeof(double) * (size_t)n;
+    if (T) HIPCHK(hipMemcpyAsync(T, c->b.T, d, hipMemcpyDeviceToHost, c->stream));
+    if (rhoY) HIPCHK(hipMemcpyAsync(rhoY, c->b.rhoY, d * c->mech.ns, hipMemcpyDeviceToHost, c->stream));
+    if (e) HIPCHK(hipMemcpyAsync(e, c->b.e, d, hipMemcpyDeviceToHost, c->stream));
+    if (steps) HIPCHK(hipMemcpyAsync(steps, c->b.steps, sizeof(int) * (size_t)n, hipMemcpyDeviceToHost, c->stream));
+    if (rejects)
+      HIPCHK(hipMemcpyAsync(rejects, c->b.rejects, sizeof(int) * (size_t)n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int corefl_chem_view_get(corefl_chem *c, corefl_chem_view *v) {
+  return guarded([&] {
+    if (!c || !v) throw std::runtime_error("corefl_chem_view_get: null argument");
+    v->n = c->b.n;
+    v->n_spec = c->mech.ns;
+    v->rho = c->b.rho;
+    v->e = c->b.e;
+    v->T = c->b.T;
+    v->rhoY = c->b.rhoY;
+    v->h = c->b.h;
+    v->steps = c->b.steps;
+    v->rejects = c->b.rejects;
+  });
+}
+
+int corefl_chem_source(corefl_chem *c, double *omega_host) {
+  return guarded([&] {
+    if (!c || !omega_host) throw std::runtime_error("corefl_chem_source: null argument");
+    ensure_device(c);
+    const long long n = c->b.n;
+    if (n == 0) return;
+    double *w = nullptr;
+    const size_t bytes = sizeof(double) * (size_t)n * c->mech.ns;
+    HIPCHK(hipMallocAsync((void **)&w, bytes, c->stream));
+    const hipError_t le = corefl::launch_chem_omega(c->mech_dev, c->mech.ns, c->b, w, c->stream);
+    const hipError_t ce = hipMemcpyAsync(omega_host, w, bytes, hipMemcpyDeviceToHost, c->stream);
+    (void)hipFreeAsync(w, c->stream);
+    HIPCHK(le);
+    HIPCHK(ce);
+    HIPCHK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int corefl_chem_integrate(corefl_chem *c, double dt, double rtol, double atol, int max_attempts, void *stream) {
+  return guarded([&] {
+    if (!c) throw std::runtime_error("corefl_chem_integrate: null context");
+    check_args(dt, rtol, atol, max_attempts);
+    ensure_device(c);
+    integrate(c, c->cfg, dt, rtol, atol, max_attempts, pick(c, stream));
+  });
+}
+
+int corefl_chem_set_launch(corefl_chem *c, int queue, int threads, int blocks) {
+  return guarded([&] {
+    if (!c) throw std::runtime_error("corefl_chem_set_launch: null context");
+    if (threads != 64 && threads != 128 && threads != 256)
+      throw std::runtime_error("corefl_chem_set_launch: threads must be 64, 128 or 256");
+    if (queue && (blocks < 1 || blocks > (1 << 20)))
+      throw std::runtime_error("corefl_chem_set_launch: blocks must be in [1, 2^20]");
+    ensure_device(c);
+    c->cfg = corefl::ChemLaunch{queue ? 1 : 0, threads, queue ? blocks : 0};
+  });
+}
+
+int corefl_chem_get_launch(const corefl_chem *c, int *queue, int *threads, int *blocks) {
+  return guarded([&] {
+    if (!c) throw std::runtime_error("corefl_chem_get_launch: null context");
+    if (queue) *queue = c->cfg.queue;
+    if (threads) *threads = c->cfg.threads;
+    if (blocks) *blocks = c->cfg.blocks;
+  });
+}
+
+int corefl_chem_autotune(corefl_chem *c, double dt, double rtol, double atol, int max_attempts, int reps,
+                         double *ms_out, int n_out) {
+  return guarded([&] {
+    if (!c) throw std::runtime_error("corefl_chem_autotune: null context");
+    check_args(dt, rtol, atol, max_attempts);
+    if (reps < 1) throw std::runtime_error("corefl_chem_autotune: reps must be at least 1");
+    ensure_device(c);
+    const long long n = c->b.n;
+    if (n == 0) throw std::runtime_error("corefl_chem_autotune: empty batch");
+    int cus = 0;
+    HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0));
+    std::vector<corefl::ChemLaunch> cand;
+    for (int t : {64, 128, 256}) cand.push_back({0, t, 0});
+    for (int t : {64, 128, 256})
+      for (int waves_per_cu : {4, 8, 16}) {
+        const int blocks = std::max(1, cus * waves_per_cu / (t / 64));
+        cand.push_back({1, t, blocks});
+      }
+    // each trial starts from the same state: T, rho Y and h are saved and restored around it
+    const int ns = c->mech.ns;
+    const size_t d = sizeof(double) * (size_t)n;
+    HIPCHK(hipMalloc(&c->save, d * (ns + 2)));
+    auto save_or_restore = [&](bool save) {
+      double *T = c->save, *h = c->save + n, *y = c->save + 2 * n;
+      auto cp = [&](double *dst, double *src, size_t bytes) {
+        HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c->stream));
+      };
+      if (save) {
+        cp(T, c->b.T, d);
+        cp(h, c->b.h, d);
+        cp(y, c->b.rhoY, d * ns);
+      } else {
+        cp(c->b.T, T, d);
+        cp(c->b.h, h, d);
+        cp(c->b.rhoY, y, d * ns);
+      }
+    };
+    save_or_restore(true);
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    double best = 1e300;
+    corefl::ChemLaunch best_cfg = cand[0];
+    try {
+      for (size_t q = 0; q < cand.size(); ++q) {
+        double total = 0;
+        for (int r = 0; r < reps; ++r) {
+          save_or_restore(false);
+          HIPCHK(hipEventRecord(e0, c->stream));
+          integrate(c, cand[q], dt, rtol, atol, max_attempts, c->stream);
+          HIPCHK(hipEventRecord(e1, c->stream));
+          HIPCHK(hipEventSynchronize(e1));
+          float ms = 0;
+          HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+          total += ms;
+        }
+        const double avg = total / reps;
+        if (ms_out && (int)q < n_out) ms_out[q] = avg;
+        if (avg < best) {
+          best = avg;
+          best_cfg = cand[q];
+        }
+      }
+      save_or_restore(false);
+      HIPCHK(hipStreamSynchronize(c->stream));
+    } catch (...) {
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      (void)hipFree(c->save);
+      c->save = nullptr;
+      throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(c->save);
+    c->save = nullptr;
+    c->cfg = best_cfg;
+    c->tuned = true;
+  });
+}
+
+int corefl_chem_synchronize(corefl_chem *c) {
+  return guarded([&] {
+    if (!c) throw std::runtime_error("corefl_chem_synchronize: null context");
+    if (c->stream) HIPCHK(hipStreamSynchronize(c->stream));
+  });
+}
+
+} // extern "C"

@@ -1,0 +1,483 @@
+// Batched finite-rate chemistry of constant-volume cells on CDNA4: NASA-7 thermo, Arrhenius / third-body /
+// Lindemann / Troe kinetics with Gibbs-energy reverse rates, the analytic d(omega)/d(rho Y) Jacobian, a
+// partial-pivot LU, and the adaptive 4-stage Rosenbrock ROK4E with embedded error control.  FP64.
+//
+// Two launch shapes integrate a batch of cells over one chemistry sub-step:
+//   * static: one thread per cell;
+//   * queue: persistent waves that fetch cells from a global counter as lanes finish, one ballot and one
+//     atomic per wave per fetch round, so a stiff cell holds only its own lane, not a whole fixed tile.
+// Every cell stops after max_attempts step attempts (flagged), so each wave's loop ends.
+#include "chem_kernels.hpp"
+
+#include <cmath>
+
+namespace corefl {
+
+namespace {
+
+constexpr double kRu = 8314.462618, kPatm = 101325.0;
+
+// ROK4E coefficients (Define.h:28-43)
+constexpr double G = 0.572816062482135;
+constexpr double A21 = 0.432364435748567, A31 = -0.51421131687617, A32 = 1.38227114461736;
+constexpr double G21 = -0.602765307997356 / G, G31 = -1.389195789724843 / G, G32 = 1.072950969011413 / G;
+constexpr double G41 = 0.992356412977094 / G, G42 = -1.390032613873701 / G, G43 = -0.440875890223325 / G;
+constexpr double B1 = 0.194335256262729, B2 = 0.483167813989227, B4 = 0.322496929748044;
+constexpr double E1 = -0.217819895945721 - B1, E2 = 1.03130847478467 - B2, E3 = 0.186511421161047, E4 = -B4;
+
+// h_k and cp_k per unit mass, linear extrapolation outside [t_low, t_high] (Thermo.cu:470-520).
+template<int NS> __device__ __forceinline__ void h_cp(const ChemMech &m, double t, double *h, double *cp) {
+  const double t2 = t * t, t3 = t2 * t, t4 = t3 * t, t5 = t4 * t;
+#pragma unroll
+  for (int i = 0; i < NS; ++i) {
+    const double R = kRu / m.mw[i];
+    const bool below = t < m.t_low[i], above = t > m.t_high[i];
+    if (below || above) {
+      const double tt = below ? m.t_low[i] : m.t_high[i];
+      const double *c = below ? m.lo[i] : m.hi[i];
+      const double tt2 = tt * tt, tt3 = tt2 * tt, tt4 = tt3 * tt, tt5 = tt4 * tt;
+      const double cc = c[0] + c[1] * tt + c[2] * tt2 + c[3] * tt3 + c[4] * tt4;
+      const double hh = c[0] * tt + 0.5 * c[1] * tt2 + c[2] * tt3 / 3 + 0.25 * c[3] * tt4 + 0.2 * c[4] * tt5 + c[5];
+      h[i] = (hh + cc * (t - tt)) * R;
+      cp[i] = cc * R;
+    } else {
+      const double *c = t < m.t_mid[i] ? m.lo[i] : m.hi[i];
+      h[i] = (c[0] * t + 0.5 * c[1] * t2 + c[2] * t3 / 3 + 0.25 * c[3] * t4 + 0.2 * c[4] * t5 + c[5]) * R;
+      cp[i] = (c[0] + c[1] * t + c[2] * t2 + c[3] * t3 + c[4] * t4) * R;
+    }
+  }
+}
+
+// g_k / (R T), clamped to the fit range (Thermo.cu:574-598).
+template<int NS> __device__ __forceinline__ void gibbs(const ChemMech &m, double t, double *g) {
+#pragma unroll
+  for (int i = 0; i < NS; ++i) {
+    const bool below = t < m.t_low[i], above = t > m.t_high[i];
+    const double tt = below ? m.t_low[i] : (above ? m.t_high[i] : t);
+    const double *c = below ? m.lo[i] : (above ? m.hi[i] : (t < m.t_mid[i] ? m.lo[i] : m.hi[i]));
+    const double tt2 = tt * tt, tt3 = tt2 * tt, tt4 = tt3 * tt;
+    g[i] = c[0] * (1.0 - log(tt)) - 0.5 * c[1] * tt - c[2] * tt2 / 6.0 - c[3] * tt3 / 12.0 - c[4] * tt4 * 0.05 +
+           c[5] / tt - c[6];
+  }
+}
+
+__device__ __forceinline__ double arrhenius(double t, double A, double b, double Ea) {
+  return A * pow(t, b) * exp(-Ea / t);
+}
+
+__device__ __forceinline__ double mass_action(double c, double nu) {
+  // integer coefficients by repeated products (equal to pow() to rounding), others through pow()
+  const int k = (int)nu;
+  if ((double)k != nu) return pow(c, nu);
+  double r = 1.0;
+  for (int q = 0; q < k; ++q) r *= c;
+  return r;
+}
+
+// Chemical source omega [kg/(m^3 s)] of one cell (FiniteRateChem.cu:1466-1605).  With JAC, also the
+// Jacobian d omega / d(rho Y) with the rates of progress frozen (ConstVolumeReactor.cpp:1168-1187).
+template<int NS, bool JAC>
+__device__ void source(const ChemMech &m, double t, const double *rhoY, double *w, double *J) {
+  double c[NS], g[NS], cr[NS], de[NS];
+#pragma unroll
+  for (int l = 0; l < NS; ++l) {
+    c[l] = rhoY[l] / m.mw[l] * 1e-3;
+    cr[l] = 0;
+    de[l] = 0;
+  }
+  if constexpr (JAC) {
+#pragma unroll
+    for (int q = 0; q < NS * NS; ++q) J[q] = 0;
+  }
+  // Equilibrium constants are needed unless every reversible reaction carries explicit REV parameters
+  // (FiniteRateChem.cu:1519-1557); then they are applied to every reaction of type other than 0 and 2.
+  if (m.need_gibbs) gibbs<NS>(m, t, g);
+  const double tp = kPatm / kRu * 1e-3 / t;
+  for (int r = 0; r < m.nr; ++r) {
+    const int ty = m.type[r];
+    double kf = arrhenius(t, m.A[r], m.b[r], m.Ea[r]);
+    double cc = 0;
+    if (ty >= 4) {
+#pragma unroll
+      for (int l = 0; l < NS; ++l) cc += c[l] * m.tb[r][l];
+    }
+    if (ty == 3) {
+      kf += arrhenius(t, m.A2[r], m.b2[r], m.Ea2[r]);
+    } else if (ty == 4) {
+      kf *= cc;
+    } else if (ty > 4) {
+      const double klow = arrhenius(t, m.A2[r], m.b2[r], m.Ea2[r]), khigh = kf;
+      if (khigh < 1e-25 && klow < 1e-25) {
+        kf = 0;
+      } else {
+        const double pr = klow * cc / khigh;
+        double F = 1.0;
+        if (ty > 5) {
+          double fc = (1 - m.troe[r][0]) * exp(-t / m.troe[r][1]) + m.troe[r][0] * exp(-t / m.troe[r][2]);
+          if (ty == 7) fc += exp(-m.troe[r][3] / t);
+          const double lfc = log10(fc);
+          const double cq = -0.4 - 0.67 * lfc, nq = 0.75 - 1.27 * lfc;
+          const double lpr = log10(pr);
+          const double tmp = (lpr + cq) / (nq - 0.14 * (lpr + cq));
+          F = pow(10.0, lfc / (1.0 + tmp * tmp));
+        }
+        kf = khigh * pr / (1.0 + pr) * F;
+      }
+    }
+    double kb = 0;
+    if (ty != 0 && m.rev[r] == 1) {
+      kb = arrhenius(t, m.A2[r], m.b2[r], m.Ea2[r]);
+      if (ty == 4) kb *= cc;
+    }
+    if (m.need_gibbs && ty != 0 && ty != 2) {
+      double dg = 0;
+#pragma unroll
+      for (int l = 0; l < NS; ++l) dg += g[l] * (m.nb[r][l] - m.nf[r][l]);
+      kb = kf / (pow(tp, m.order[r]) * exp(-dg));
+    }
+    double q1 = 1.0, q2 = ty != 0 ? 1.0 : 0.0;
+#pragma unroll
+    for (int l = 0; l < NS; ++l) {
+      if (m.nf[r][l] != 0) q1 *= mass_action(c[l], m.nf[r][l]);
+      if (ty != 0 && m.nb[r][l] != 0) q2 *= mass_action(c[l], m.nb[r][l]);
+    }
+    q1 *= kf;
+    q2 *= kb;
+#pragma unroll
+    for (int l = 0; l < NS; ++l) {
+      cr[l] += q2 * m.nf[r][l] + q1 * m.nb[r][l];
+      de[l] += q1 * m.nf[r][l] + q2 * m.nb[r][l];
+    }
+    if constexpr (JAC) {
+#pragma unroll
+      for (int a = 0; a < NS; ++a) {
+        const double dn = m.nb[r][a] - m.nf[r][a];
+        if (dn == 0) continue;
+#pragma unroll
+        for (int n = 0; n < NS; ++n) J[a * NS + n] += dn * (m.nf[r][n] * q1 - m.nb[r][n] * q2);
+      }
+    }
+  }
+#pragma unroll
+  for (int l = 0; l < NS; ++l) w[l] = cr[l] * (1e+3 * m.mw[l]) - de[l] * (1e+3 * m.mw[l]);
+  if constexpr (JAC) {
+#pragma unroll
+    for (int n = 0; n < NS; ++n) {
+      const double inv = rhoY[n] > 1e-30 ? 1.0 / rhoY[n] : 0.0;
+#pragma unroll
+      for (int a = 0; a < NS; ++a) J[a * NS + n] = J[a * NS + n] * inv * (1e+3 * m.mw[a]);
+    }
+  }
+}
+
+// Newton iteration for T at fixed specific internal energy E (ConstVolumeReactor.cpp:970-995).
+template<int NS> __device__ double update_t(const ChemMech &m, double t, const double *yk, double E) {
+  double imw = 0;
+#pragma unroll
+  for (int l = 0; l < NS; ++l) imw += yk[l] / m.mw[l];
+  const double R = kRu * imw;
+  double err = 1e+6;
+  for (int it = 0; it < 100 && err > 1e-8; ++it) {
+    double h[NS], cp[NS];
+    h_cp<NS>(m, t, h, cp);
+    double cpt = 0, hh = 0;
+#pragma unroll
+    for (int l = 0; l < NS; ++l) {
+      cpt += cp[l] * yk[l];
+      hh += h[l] * yk[l];
+    }
+    const double t1 = t - (hh - R * t - E) / (cpt - R);
+    err = fabs(1 - t1 / t);
+    t = t1;
+  }
+  return t;
+}
+
+// Doolittle LU with partial pivoting, row-major (ConstVolumeReactor.cpp:1231-1262), and its solve.
+template<int NS> __device__ __forceinline__ void lu(double *a, int *piv) {
+#pragma unroll
+  for (int k = 0; k < NS; ++k) {
+    int ik = k;
+#pragma unroll
+    for (int r = k; r < NS; ++r) {
+#pragma unroll
+      for (int t = 0; t < k; ++t) a[r * NS + k] -= a[r * NS + t] * a[t * NS + k];
+      if (fabs(a[r * NS + k]) > fabs(a[ik * NS + k])) ik = r;
+    }
+    piv[k] = ik;
+    // row swap through selects keeps every index static
+#pragma unroll
+    for (int r = k + 1; r < NS; ++r) {
+      if (r == ik) {
+#pragma unroll
+        for (int t = 0; t < NS; ++t) {
+          const double s = a[r * NS + t];
+          a[r * NS + t] = a[k * NS + t];
+          a[k * NS + t] = s;
+        }
+      }
+    }
+#pragma unroll
+    for (int p = k + 1; p < NS; ++p)
+#pragma unroll
+      for (int t = 0; t < k; ++t) a[k * NS + p] -= a[k * NS + t] * a[t * NS + p];
+    const double inv = 1.0 / a[k * NS + k];
+#pragma unroll
+    for (int r = k + 1; r < NS; ++r) a[r * NS + k] *= inv;
+  }
+}
+
+template<int NS> __device__ __forceinline__ void lu_solve(const double *a, const int *piv, double *b) {
+#pragma unroll
+  for (int r = 0; r < NS; ++r) {
+#pragma unroll
+    for (int q = r + 1; q < NS; ++q)
+      if (piv[r] == q) {
+        const double s = b[q];
+        b[q] = b[r];
+        b[r] = s;
+      }
+  }
+#pragma unroll
+  for (int r = 1; r < NS; ++r)
+#pragma unroll
+    for (int t = 0; t < r; ++t) b[r] -= a[r * NS + t] * b[t];
+  b[NS - 1] /= a[NS * NS - 1];
+#pragma unroll
+  for (int r = NS - 2; r >= 0; --r) {
+#pragma unroll
+    for (int t = r + 1; t < NS; ++t) b[r] -= a[r * NS + t] * b[t];
+    b[r] /= a[r * NS + r];
+  }
+}
+
+// Integration state of one cell.
+template<int NS> struct Cell {
+  double y[NS]; // rho Y
+  double rho, E, T, time, h, err_last;
+  int steps, rejects, attempts;
+};
+
+// One ROK4E attempt (ConstVolumeReactor.cpp:173-476 with adaptive_time_step = 1).
+template<int NS>
+__device__ void rok4e_attempt(const ChemMech &m, Cell<NS> &s, double dt, double rtol, double atol) {
+  double w[NS], J[NS * NS], k1[NS], k2[NS], k3[NS], k4[NS], ys[NS], yk[NS];
+  int piv[NS];
+  source<NS, true>(m, s.T, s.y, w, J);
+  const double hs = fmin(s.h, dt - s.time);
+#pragma unroll
+  for (int q = 0; q < NS * NS; ++q) J[q] = -hs * G * J[q];
+#pragma unroll
+  for (int l = 0; l < NS; ++l) J[l * NS + l] += 1.0;
+  lu<NS>(J, piv);
+#pragma unroll
+  for (int l = 0; l < NS; ++l) k1[l] = w[l];
+  lu_solve<NS>(J, piv, k1);
+  const double irho = 1.0 / s.rho;
+#pragma unroll
+  for (int l = 0; l < NS; ++l) {
+    ys[l] = s.y[l] + hs * A21 * k1[l];
+    yk[l] = ys[l] * irho;
+  }
+  double Ts = update_t<NS>(m, s.T, yk, s.E);
+  source<NS, false>(m, Ts, ys, w, nullptr);
+#pragma unroll
+  for (int l = 0; l < NS; ++l) k2[l] = w[l] + G21 * k1[l];
+  lu_solve<NS>(J, piv, k2);
+#pragma unroll
+  for (int l = 0; l < NS; ++l) {
+    k2[l] -= G21 * k1[l];
+    ys[l] = s.y[l] + hs * (A31 * k1[l] + A32 * k2[l]);
+    yk[l] = ys[l] * irho;
+  }
+  Ts = update_t<NS>(m, Ts, yk, s.E);
+  source<NS, false>(m, Ts, ys, w, nullptr);
+#pragma unroll
+  for (int l = 0; l < NS; ++l) k3[l] = w[l] + G31 * k1[l] + G32 * k2[l];
+  lu_solve<NS>(J, piv, k3);
+#pragma unroll
+  for (int l = 0; l < NS; ++l) {
+    k3[l] -= G31 * k1[l] + G32 * k2[l];
+    k4[l] = w[l] + G41 * k1[l] + G42 * k2[l] + G43 * k3[l];
+  }
+  lu_solve<NS>(J, piv, k4);
+  double err = 0, ynew[NS];
+#pragma unroll
+  for (int l = 0; l < NS; ++l) {
+    k4[l] -= G41 * k1[l] + G42 * k2[l] + G43 * k3[l];
+    ynew[l] = s.y[l] + hs * (B1 * k1[l] + B2 * k2[l] + B4 * k4[l]);
+    const double dy = hs * (E1 * k1[l] + E2 * k2[l] + E3 * k3[l] + E4 * k4[l]) / (rtol * ynew[l] + atol);
+    err += dy * dy;
+  }
+  err = sqrt(err / NS);
+  double hstar = hs;
+  if (s.steps > 0) hstar *= fmin(5.0, fmax(0.2, 0.8 * pow(s.err_last, 0.1) * pow(err + 1e-20, -0.175)));
+  else if (err > 1) hstar *= 0.2;
+  s.h = hstar;
+  ++s.attempts;
+  if (err <= 1) {
+#pragma unroll
+    for (int l = 0; l < NS; ++l) {
+      s.y[l] = ynew[l];
+      yk[l] = ynew[l] * irho;
+    }
+    s.time += hs;
+    s.err_last = err;
+    s.T = update_t<NS>(m, s.T, yk, s.E);
+    ++s.steps;
+  } else {
+    ++s.rejects;
+  }
+}
+
+template<int NS> __device__ __forceinline__ void load_cell(const ChemBatch &b, int c, double dt, Cell<NS> &s) {
+#pragma unroll
+  for (int l = 0; l < NS; ++l) s.y[l] = b.rhoY[(long)l * b.n + c];
+  s.rho = b.rho[c];
+  s.E = b.e[c];
+  s.T = b.T[c];
+  s.h = b.h[c] > 0 ? fmin(b.h[c], dt) : dt;
+  s.time = 0;
+  s.err_last = 0;
+  s.steps = s.rejects = s.attempts = 0;
+}
+
+template<int NS> __device__ __forceinline__ void store_cell(const ChemBatch &b, int c, const Cell<NS> &s, bool ok) {
+#pragma unroll
+  for (int l = 0; l < NS; ++l) b.rhoY[(long)l * b.n + c] = s.y[l];
+  b.T[c] = s.T;
+  b.h[c] = s.h;
+  b.steps[c] = ok ? s.steps : -1;
+  b.rejects[c] = s.rejects;
+}
+
+template<int NS>
+__global__ __launch_bounds__(256) void k_chem_static(const ChemMech *__restrict__ mp, ChemBatch b, double dt,
+                                                     double rtol, double atol, int max_attempts) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= b.n) return;
+  const ChemMech &m = *mp;
+  Cell<NS> s;
+  load_cell<NS>(b, c, dt, s);
+  while (s.time < dt && s.attempts < max_attempts) rok4e_attempt<NS>(m, s, dt, rtol, atol);
+  store_cell<NS>(b, c, s, s.time >= dt);
+}
+
+template<int NS>
+__global__ __launch_bounds__(256) void k_chem_queue(const ChemMech *__restrict__ mp, ChemBatch b, double dt,
+                                                    double rtol, double atol, int max_attempts, int *counter) {
+  const ChemMech &m = *mp;
+  const int lane = __lane_id();
+  Cell<NS> s;
+  int cell = -1;
+  bool exhausted = false;
+  while (true) {
+    const bool need = cell < 0 && !exhausted;
+    const unsigned long long mask = __ballot(need);
+    if (mask) {
+      const int leader = __ffsll((long long)mask) - 1;
+      int base = 0;
+      if (lane == leader) base = atomicAdd(counter, __popcll(mask));
+      base = __shfl(base, leader, 64);
+      if (need) {
+        const int got = base + __popcll(mask & ((1ull << lane) - 1));
+        if (got < b.n) {
+          cell = got;
+          load_cell<NS>(b, cell, dt, s);
+        } else {
+          exhausted = true;
+        }
+      }
+    }
+    if (__ballot(cell >= 0) == 0) break;
+    if (cell >= 0) {
+      rok4e_attempt<NS>(m, s, dt, rtol, atol);
+      if (s.time >= dt || s.attempts >= max_attempts) {
+        store_cell<NS>(b, cell, s, s.time >= dt);
+        cell = -1;
+      }
+    }
+  }
+}
+
+template<int NS>
+__global__ __launch_bounds__(256) void k_chem_omega(const ChemMech *__restrict__ mp, ChemBatch b, double *omega) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= b.n) return;
+  double y[NS], w[NS];
+#pragma unroll
+  for (int l = 0; l < NS; ++l) y[l] = b.rhoY[(long)l * b.n + c];
+  source<NS, false>(*mp, b.T[c], y, w, nullptr);
+#pragma unroll
+  for (int l = 0; l < NS; ++l) omega[(long)l * b.n + c] = w[l];
+}
+
+template<int NS>
+__global__ __launch_bounds__(256) void k_chem_temperature(const ChemMech *__restrict__ mp, ChemBatch b) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= b.n) return;
+  double yk[NS];
+  const double irho = 1.0 / b.rho[c];
+#pragma unroll
+  for (int l = 0; l < NS; ++l) yk[l] = b.rhoY[(long)l * b.n + c] * irho;
+  b.T[c] = update_t<NS>(*mp, b.T[c], yk, b.e[c]);
+}
+
+template<int NS> __global__ __launch_bounds__(256) void k_chem_energy(const ChemMech *__restrict__ mp, ChemBatch b) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= b.n) return;
+  const ChemMech &m = *mp;
+  double h[NS], cp[NS];
+  const double t = b.T[c], irho = 1.0 / b.rho[c];
+  h_cp<NS>(m, t, h, cp);
+  double e = 0, imw = 0;
+#pragma unroll
+  for (int l = 0; l < NS; ++l) {
+    const double y = b.rhoY[(long)l * b.n + c] * irho;
+    e += h[l] * y;
+    imw += y / m.mw[l];
+  }
+  b.e[c] = e - kRu * imw * t;
+}
+
+__global__ void k_zero_int(int *p) { *p = 0; }
+
+} // namespace
+
+hipError_t launch_chem_energy(const ChemMech *mech_dev, int ns, const ChemBatch &b, hipStream_t s) {
+  if (ns != 9) return hipErrorNotSupported;
+  if (b.n == 0) return hipSuccess;
+  k_chem_energy<9><<<(b.n + 255) / 256, 256, 0, s>>>(mech_dev, b);
+  return hipGetLastError();
+}
+
+hipError_t launch_chem_integrate(const ChemMech *mech_dev, int ns, const ChemBatch &b, double dt, double rtol,
+                                 double atol, int max_attempts, const ChemLaunch &cfg, int *counter,
+                                 hipStream_t s) {
+  if (ns != 9) return hipErrorNotSupported;
+  if (b.n == 0) return hipSuccess;
+  if (cfg.queue) {
+    k_zero_int<<<1, 1, 0, s>>>(counter);
+    k_chem_queue<9><<<cfg.blocks, cfg.threads, 0, s>>>(mech_dev, b, dt, rtol, atol, max_attempts, counter);
+  } else {
+    k_chem_static<9><<<(b.n + cfg.threads - 1) / cfg.threads, cfg.threads, 0, s>>>(mech_dev, b, dt, rtol, atol,
+                                                                                  max_attempts);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_chem_omega(const ChemMech *mech_dev, int ns, const ChemBatch &b, double *omega, hipStream_t s) {
+  if (ns != 9) return hipErrorNotSupported;
+  if (b.n == 0) return hipSuccess;
+  k_chem_omega<9><<<(b.n + 255) / 256, 256, 0, s>>>(mech_dev, b, omega);
+  return hipGetLastError();
+}
+
+hipError_t launch_chem_temperature(const ChemMech *mech_dev, int ns, const ChemBatch &b, hipStream_t s) {
+  if (ns != 9) return hipErrorNotSupported;
+  if (b.n == 0) return hipSuccess;
+  k_chem_temperature<9><<<(b.n + 255) / 256, 256, 0, s>>>(mech_dev, b);
+  return hipGetLastError();
+}
+
+} // namespace corefl

@@ -354,6 +354,10 @@ std::vector<corefl::HostBlock> prepare_blocks(corefl::Parameter &P, std::vector<
 
 } // namespace
 
+namespace corefl {
+void set_last_error(const std::string &msg) { g_last_error = msg; }
+} // namespace corefl
+
 extern "C" {
 
 const char *corefl_last_error(void) { return g_last_error.c_str(); }

@@ -106,6 +106,79 @@ int corefl_profile_kernels(corefl_ctx *ctx, int reps, double *ms_out, int n_out)
 int corefl_get_int(const corefl_ctx *ctx, const char *name, int *out);
 int corefl_get_real(const corefl_ctx *ctx, const char *name, double *out);
 
+/* ---------------------------------------------------------------------------------------------------
+ * Finite-rate chemistry of cell batches: CHEMKIN mechanism + NASA-7 thermo, the chemical source, and the
+ * stiff per-cell sub-step (4-stage Rosenbrock ROK4E with embedded error control) batched over cells.
+ * The context owns its device tables, a batch of cells (structure of arrays) and a stream.
+ * ------------------------------------------------------------------------------------------------- */
+typedef struct corefl_chem corefl_chem;
+
+/* Device pointers of the current batch (n cells); rhoY is [k * n + cell]. */
+typedef struct {
+  int n, n_spec;
+  double *rho;   /* kg/m^3 */
+  double *e;     /* specific internal energy, J/kg, constant over a sub-step */
+  double *T;     /* K */
+  double *rhoY;  /* kg/m^3 */
+  double *h;     /* step size carried between sub-steps (0: start from dt) */
+  int *steps;    /* accepted steps of the last sub-step, -1 if the attempt limit was reached */
+  int *rejects;  /* rejected attempts of the last sub-step */
+} corefl_chem_view;
+
+/* Replaces Species::Species + Reaction::Reaction (src/ChemData.cpp:10-100, 868-1177) and the chemistry
+ * tables of DParameter::DParameter (src/DParameter.cu:6-394): reads a CHEMKIN mechanism file (ELEMENTS,
+ * SPECIES, optional THERMO, REACTIONS) and, when therm_file is non-empty, a NASA-7 thermo file.
+ * Activation energies are converted to K with R_c = 1.98726 cal/(mol K); DUPLICATE pairs are merged. */
+int corefl_chem_create(const char *mechanism_file, const char *therm_file, corefl_chem **out);
+/* Same with the two files' contents in memory (therm_text may be ""). */
+int corefl_chem_create_mem(const char *mech_text, const char *therm_text, corefl_chem **out);
+void corefl_chem_destroy(corefl_chem *chem);
+/* Species, reaction (after DUPLICATE merging) and element counts (Species::n_spec, Reaction::n_reac). */
+int corefl_chem_info(const corefl_chem *chem, int *n_spec, int *n_reac, int *n_elem);
+/* Name of species (kind 0) or element (kind 1) `index` (Species::spec_name, src/ChemData.cpp:30-60). */
+int corefl_chem_name(const corefl_chem *chem, int kind, int index, char *buf, int len);
+/* Host copy of one parsed table, as DParameter holds it (src/DParameter.cuh:100-180): "mw", "t_low",
+ * "t_mid", "t_high", "lo", "hi" (n_spec x 7), "A", "b", "Ea", "A2", "b2", "Ea2", "order", "type", "rev",
+ * "troe" (n_reac x 4), "nf", "nb", "tb" (n_reac x n_spec), "comp" (n_spec x n_elem).  n must equal the
+ * table's element count.  Needs no GPU. */
+int corefl_chem_table(const corefl_chem *chem, const char *name, double *out, long long n);
+
+/* Host-to-device batch of n cells (replaces the per-cell state of DZone sv/bv that
+ * finite_rate_chemistry reads, src/FiniteRateChem.cu:1388-1400).  e may be NULL: then
+ * e = sum_k Y_k h_k(T) - R T is computed on the device (src/ConstVolumeReactor.cpp:60-68).
+ * Resets the carried step sizes and counters. */
+int corefl_chem_set_cells(corefl_chem *chem, long long n, const double *rho, const double *e, const double *T,
+                          const double *rhoY);
+/* Device-to-host copy of the batch; any pointer may be NULL. */
+int corefl_chem_get_cells(corefl_chem *chem, double *T, double *rhoY, double *e, int *steps, int *rejects);
+/* Zero-copy device pointers of the batch. */
+int corefl_chem_view_get(corefl_chem *chem, corefl_chem_view *view);
+
+/* Chemical source omega_k [kg/(m^3 s)] of every cell at its current T and rhoY, copied to
+ * omega_host[k * n + cell] (forward_reaction_rate + backward_reaction_rate + rate_of_progress +
+ * chemical_source, src/FiniteRateChem.cu:1466-1605). */
+int corefl_chem_source(corefl_chem *chem, double *omega_host);
+
+/* One chemistry sub-step of length dt for every cell at constant density and internal energy: adaptive
+ * ROK4E (src/ConstVolumeReactor.cpp:173-476 with adaptive_time_step = 1, constants src/Define.h:28-43,
+ * temperature by update_t, 970-995), error norm RMS(err_k / (rtol * rhoY_k + atol)).  Each cell stops
+ * after max_attempts step attempts (steps = -1 flags it).  Launch shape: the autotuned or set one.
+ * Stream-ordered on `stream` (NULL: the context's stream). */
+int corefl_chem_integrate(corefl_chem *chem, double dt, double rtol, double atol, int max_attempts, void *stream);
+
+/* Launch shape of corefl_chem_integrate (no reference counterpart: the reference integrates one cell per
+ * thread, src/FiniteRateChem.cu:1388).  queue = 1: persistent waves fetching cells from a device counter,
+ * `blocks` workgroups; queue = 0: one thread per cell.  threads is 64, 128 or 256. */
+int corefl_chem_set_launch(corefl_chem *chem, int queue, int threads, int blocks);
+int corefl_chem_get_launch(const corefl_chem *chem, int *queue, int *threads, int *blocks);
+/* Times corefl_chem_integrate(dt, rtol, atol, max_attempts) on the current batch for each candidate launch
+ * shape (static 64/128/256 threads, then queue 64/128/256 threads x 4/8/16 waves per CU), `reps` runs
+ * each from the same saved state, keeps the fastest and restores the batch.  ms_out[q] (q < n_out) gets
+ * the average milliseconds of candidate q.  No reference counterpart. */
+int corefl_chem_autotune(corefl_chem *chem, double dt, double rtol, double atol, int max_attempts, int reps,
+                         double *ms_out, int n_out);
+int corefl_chem_synchronize(corefl_chem *chem);
+
 #ifdef __cplusplus
 }
 #endif
