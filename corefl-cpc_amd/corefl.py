@@ -21,7 +21,7 @@ SYMBOLS = (
     "corefl_rk3_step", "corefl_advance", "corefl_get_time", "corefl_synchronize", "corefl_get_int",
     "corefl_profile_kernels", "corefl_get_real",
     "corefl_chem_create", "corefl_chem_create_mem", "corefl_chem_destroy", "corefl_chem_info", "corefl_chem_name",
-    "corefl_chem_table", "corefl_chem_set_cells", "corefl_chem_get_cells", "corefl_chem_view_get",
+    "corefl_chem_table", "corefl_chem_set_cells", "corefl_chem_set_cell_dt", "corefl_chem_get_cells", "corefl_chem_view_get",
     "corefl_chem_source", "corefl_chem_integrate", "corefl_chem_set_launch", "corefl_chem_get_launch",
     "corefl_chem_autotune", "corefl_chem_synchronize",
 )
@@ -81,6 +81,7 @@ def load_library(path=LIB_PATH):
     lib.corefl_chem_table.argtypes = [P, ctypes.c_char_p, P, LL]
     lib.corefl_chem_set_cells.argtypes = [P, LL, P, P, P, P]
     lib.corefl_chem_get_cells.argtypes = [P, P, P, P, P, P]
+    lib.corefl_chem_set_cell_dt.argtypes = [P, P]
     lib.corefl_chem_view_get.argtypes = [P, ctypes.POINTER(ChemView)]
     lib.corefl_chem_source.argtypes = [P, P]
     lib.corefl_chem_integrate.argtypes = [P, D, D, D, I, P]
@@ -274,6 +275,16 @@ class Chemistry:
             ep = e.ctypes.data
         _check(self._lib.corefl_chem_set_cells(self._ctx, n, rho.ctypes.data, ep, T.ctypes.data, rhoY.ctypes.data))
         self.n = n
+
+    def set_cell_dt(self, dt):
+        """Per-cell sub-step lengths for integrate(0.0); None clears them."""
+        if dt is None:
+            _check(self._lib.corefl_chem_set_cell_dt(self._ctx, None))
+            return
+        dt = np.ascontiguousarray(dt, dtype=np.float64)
+        if dt.size != self.n:
+            raise ValueError("set_cell_dt: one length per cell expected")
+        _check(self._lib.corefl_chem_set_cell_dt(self._ctx, dt.ctypes.data))
 
     def get_cells(self):
         n = self.n

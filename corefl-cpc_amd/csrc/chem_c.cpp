@@ -28,6 +28,8 @@ struct corefl_chem {
   int cap = 0;
   corefl::ChemBatch b{};
   double *save = nullptr; // autotuner copy of T, rhoY, h
+  double *dtc = nullptr;  // per-cell sub-step lengths (capacity cap)
+  bool have_dtc = false;
   corefl::ChemLaunch cfg{1, 64, 0};
   bool tuned = false;
   ~corefl_chem();
@@ -65,10 +67,12 @@ template<class F> int guarded(F &&f) {
 
 void free_batch(corefl_chem *c) {
   for (void *p : {(void *)c->b.rho, (void *)c->b.e, (void *)c->b.T, (void *)c->b.rhoY, (void *)c->b.h,
-                  (void *)c->b.steps, (void *)c->b.rejects, (void *)c->save})
+                  (void *)c->b.steps, (void *)c->b.rejects, (void *)c->save, (void *)c->dtc})
     if (p) (void)hipFree(p);
   c->b = corefl::ChemBatch{};
   c->save = nullptr;
+  c->dtc = nullptr;
+  c->have_dtc = false;
   c->cap = 0;
 }
 
@@ -103,20 +107,24 @@ void ensure_batch(corefl_chem *c, long long n) {
   HIPCHK(hipMalloc(&c->b.h, d));
   HIPCHK(hipMalloc(&c->b.steps, sizeof(int) * (size_t)n));
   HIPCHK(hipMalloc(&c->b.rejects, sizeof(int) * (size_t)n));
+  HIPCHK(hipMalloc(&c->dtc, d));
   c->cap = (int)n;
   c->b.n = (int)n;
 }
 
 hipStream_t pick(corefl_chem *c, void *stream) { return stream ? (hipStream_t)stream : c->stream; }
 
+// dt > 0: every cell advances by dt; dt == 0: by its own length (corefl_chem_set_cell_dt).
 void integrate(corefl_chem *c, const corefl::ChemLaunch &cfg, double dt, double rtol, double atol, int max_attempts,
                hipStream_t s) {
-  HIPCHK(corefl::launch_chem_integrate(c->mech_dev, c->mech.ns, c->b, dt, rtol, atol, max_attempts, cfg, c->counter,
-                                       s));
+  corefl::ChemBatch b = c->b;
+  b.dtc = dt > 0 ? nullptr : c->dtc;
+  HIPCHK(corefl::launch_chem_integrate(c->mech_dev, c->mech.ns, b, dt, rtol, atol, max_attempts, cfg, c->counter, s));
 }
 
-void check_args(double dt, double rtol, double atol, int max_attempts) {
-  if (!(dt > 0) || !(rtol > 0) || !(atol > 0)) throw std::runtime_error("dt, rtol and atol must be positive");
+void check_args(corefl_chem *c, double dt, double rtol, double atol, int max_attempts) {
+  if (!(dt >= 0) || !(rtol > 0) || !(atol > 0)) throw std::runtime_error("dt >= 0, rtol > 0 and atol > 0 required");
+  if (dt == 0 && !c->have_dtc) throw std::runtime_error("dt == 0 needs per-cell lengths from corefl_chem_set_cell_dt");
   if (max_attempts < 1) throw std::runtime_error("max_attempts must be at least 1");
 }
 
@@ -230,9 +238,28 @@ int corefl_chem_set_cells(corefl_chem *c, long long n, const double *rho, const 
     HIPCHK(hipMemsetAsync(c->b.h, 0, d, c->stream));
     HIPCHK(hipMemsetAsync(c->b.steps, 0, sizeof(int) * (size_t)n, c->stream));
     HIPCHK(hipMemsetAsync(c->b.rejects, 0, sizeof(int) * (size_t)n, c->stream));
+    c->have_dtc = false;
     if (e) HIPCHK(hipMemcpyAsync(c->b.e, e, d, hipMemcpyHostToDevice, c->stream));
     else HIPCHK(corefl::launch_chem_energy(c->mech_dev, c->mech.ns, c->b, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int corefl_chem_set_cell_dt(corefl_chem *c, const double *dt_host) {
+  return guarded([&] {
+    if (!c) throw std::runtime_error("corefl_chem_set_cell_dt: null context");
+    if (!dt_host) {
+      c->have_dtc = false;
+      return;
+    }
+    const long long n = c->b.n;
+    for (long long q = 0; q < n; ++q)
+      if (!(dt_host[q] > 0)) throw std::runtime_error("corefl_chem_set_cell_dt: lengths must be positive");
+    if (n) {
+      HIPCHK(hipMemcpyAsync(c->dtc, dt_host, sizeof(double) * (size_t)n, hipMemcpyHostToDevice, c->stream));
+      HIPCHK(hipStreamSynchronize(c->stream));
+    }
+    c->have_dtc = true;
   });
 }
 
@@ -288,7 +315,7 @@ int corefl_chem_source(corefl_chem *c, double *omega_host) {
 int corefl_chem_integrate(corefl_chem *c, double dt, double rtol, double atol, int max_attempts, void *stream) {
   return guarded([&] {
     if (!c) throw std::runtime_error("corefl_chem_integrate: null context");
-    check_args(dt, rtol, atol, max_attempts);
+    check_args(c, dt, rtol, atol, max_attempts);
     ensure_device(c);
     integrate(c, c->cfg, dt, rtol, atol, max_attempts, pick(c, stream));
   });
@@ -319,7 +346,7 @@ int corefl_chem_autotune(corefl_chem *c, double dt, double rtol, double atol, in
                          double *ms_out, int n_out) {
   return guarded([&] {
     if (!c) throw std::runtime_error("corefl_chem_autotune: null context");
-    check_args(dt, rtol, atol, max_attempts);
+    check_args(c, dt, rtol, atol, max_attempts);
     if (reps < 1) throw std::runtime_error("corefl_chem_autotune: reps must be at least 1");
     ensure_device(c);
     const long long n = c->b.n;

@@ -254,13 +254,14 @@ template<int NS> __device__ __forceinline__ void lu_solve(const double *a, const
 // Integration state of one cell.
 template<int NS> struct Cell {
   double y[NS]; // rho Y
-  double rho, E, T, time, h, err_last;
+  double rho, E, T, time, h, err_last, dt;
   int steps, rejects, attempts;
 };
 
 // One ROK4E attempt (ConstVolumeReactor.cpp:173-476 with adaptive_time_step = 1).
 template<int NS>
-__device__ void rok4e_attempt(const ChemMech &m, Cell<NS> &s, double dt, double rtol, double atol) {
+__device__ void rok4e_attempt(const ChemMech &m, Cell<NS> &s, double rtol, double atol) {
+  const double dt = s.dt;
   double w[NS], J[NS * NS], k1[NS], k2[NS], k3[NS], k4[NS], ys[NS], yk[NS];
   int piv[NS];
   source<NS, true>(m, s.T, s.y, w, J);
@@ -312,7 +313,7 @@ __device__ void rok4e_attempt(const ChemMech &m, Cell<NS> &s, double dt, double 
   err = sqrt(err / NS);
   double hstar = hs;
   if (s.steps > 0) hstar *= fmin(5.0, fmax(0.2, 0.8 * pow(s.err_last, 0.1) * pow(err + 1e-20, -0.175)));
-  else if (err > 1) hstar *= 0.2;
+  else if (!(err <= 1)) hstar *= 0.2; // a NaN first-step error also cuts the step (the reference would repeat it)
   s.h = hstar;
   ++s.attempts;
   if (err <= 1) {
@@ -333,6 +334,8 @@ __device__ void rok4e_attempt(const ChemMech &m, Cell<NS> &s, double dt, double 
 template<int NS> __device__ __forceinline__ void load_cell(const ChemBatch &b, int c, double dt, Cell<NS> &s) {
 #pragma unroll
   for (int l = 0; l < NS; ++l) s.y[l] = b.rhoY[(long)l * b.n + c];
+  if (b.dtc) dt = b.dtc[c];
+  s.dt = dt;
   s.rho = b.rho[c];
   s.E = b.e[c];
   s.T = b.T[c];
@@ -359,8 +362,8 @@ __global__ __launch_bounds__(256) void k_chem_static(const ChemMech *__restrict_
   const ChemMech &m = *mp;
   Cell<NS> s;
   load_cell<NS>(b, c, dt, s);
-  while (s.time < dt && s.attempts < max_attempts) rok4e_attempt<NS>(m, s, dt, rtol, atol);
-  store_cell<NS>(b, c, s, s.time >= dt);
+  while (s.time < s.dt && s.attempts < max_attempts) rok4e_attempt<NS>(m, s, rtol, atol);
+  store_cell<NS>(b, c, s, s.time >= s.dt);
 }
 
 template<int NS>
@@ -391,9 +394,9 @@ __global__ __launch_bounds__(256) void k_chem_queue(const ChemMech *__restrict__
     }
     if (__ballot(cell >= 0) == 0) break;
     if (cell >= 0) {
-      rok4e_attempt<NS>(m, s, dt, rtol, atol);
-      if (s.time >= dt || s.attempts >= max_attempts) {
-        store_cell<NS>(b, cell, s, s.time >= dt);
+      rok4e_attempt<NS>(m, s, rtol, atol);
+      if (s.time >= s.dt || s.attempts >= max_attempts) {
+        store_cell<NS>(b, cell, s, s.time >= s.dt);
         cell = -1;
       }
     }

@@ -17,6 +17,7 @@ struct ChemBatch {
   double *h;     // per-cell step size carried between sub-steps (<= 0: start from dt)
   int *steps;    // accepted steps of the last sub-step (-1: attempt limit reached)
   int *rejects;  // rejected attempts of the last sub-step
+  const double *dtc; // per-cell sub-step length (nullptr: the launch's dt for every cell)
 };
 
 // Launch shape picked by the autotuner.

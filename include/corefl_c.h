@@ -149,6 +149,9 @@ int corefl_chem_table(const corefl_chem *chem, const char *name, double *out, lo
  * Resets the carried step sizes and counters. */
 int corefl_chem_set_cells(corefl_chem *chem, long long n, const double *rho, const double *e, const double *T,
                           const double *rhoY);
+/* Per-cell sub-step lengths (n values > 0) used by corefl_chem_integrate with dt == 0; NULL clears them.
+ * corefl_chem_set_cells clears them too. */
+int corefl_chem_set_cell_dt(corefl_chem *chem, const double *dt_host);
 /* Device-to-host copy of the batch; any pointer may be NULL. */
 int corefl_chem_get_cells(corefl_chem *chem, double *T, double *rhoY, double *e, int *steps, int *rejects);
 /* Zero-copy device pointers of the batch. */
@@ -159,7 +162,8 @@ int corefl_chem_view_get(corefl_chem *chem, corefl_chem_view *view);
  * chemical_source, src/FiniteRateChem.cu:1466-1605). */
 int corefl_chem_source(corefl_chem *chem, double *omega_host);
 
-/* One chemistry sub-step of length dt for every cell at constant density and internal energy: adaptive
+/* One chemistry sub-step of length dt (dt == 0: each cell's own length) for every cell at constant
+ * density and internal energy: adaptive
  * ROK4E (src/ConstVolumeReactor.cpp:173-476 with adaptive_time_step = 1, constants src/Define.h:28-43,
  * temperature by update_t, 970-995), error norm RMS(err_k / (rtol * rhoY_k + atol)).  Each cell stops
  * after max_attempts step attempts (steps = -1 flags it).  Launch shape: the autotuned or set one.

@@ -132,9 +132,10 @@ void orc_chem_source(const orc_mech *m, double t, const double *rhoY, double *q1
 void orc_chem_jacobian(const orc_mech *m, const double *rhoY, const double *q1, const double *q2, double *jac);
 /* T from internal energy by Newton (ConstVolumeReactor.cpp:970-995). */
 double orc_chem_update_t(const orc_mech *m, double t0, const double *yk, double E);
-/* Adaptive ROK4E at constant rho and E (ConstVolumeReactor.cpp:173-497). */
+/* Adaptive ROK4E at constant rho and E (ConstVolumeReactor.cpp:173-497); -1 if max_attempts attempts
+ * (accepted + rejected) end before t_end. */
 int orc_chem_rok4e(const orc_mech *m, double rho, double E, double *rhoY, double *T, double t_end, double rtol,
-                   double atol, double *h, int max_steps, int *n_rej, double *rec, int n_rec);
+                   double atol, double *h, int max_attempts, int *n_rej, double *rec, int n_rec);
 
 /* ---- exact solution ---- */
 

@@ -227,10 +227,13 @@ static const double E3 = 0.186511421161047;
 
 /* Adaptive ROK4E over [0, t_end] at constant density and internal energy (ConstVolumeReactor.cpp:173-497,
  * build_arnoldi = 0, adaptive_time_step = 1).  rhoY and *T are advanced in place; *h carries the step size
- * from call to call.  Returns the number of accepted steps; *n_rej gets the rejections.  If n_rec > 0,
- * (time, T) after each accepted step is written to rec[2*s], rec[2*s+1] for s < n_rec. */
+ * from call to call.  Returns the number of accepted steps, or -1 when max_attempts (accepted + rejected)
+ * attempts did not reach t_end; *n_rej gets the rejections.  If n_rec > 0, (time, T) after each accepted
+ * step is written to rec[2*s], rec[2*s+1] for s < n_rec.
+ * One deviation: a first-step error that is not a number is treated as err > 1 (step cut to 0.2 h); the
+ * reference leaves h unchanged there (ConstVolumeReactor.cpp:421-425) and repeats the same attempt forever. */
 int orc_chem_rok4e(const orc_mech *m, double rho, double E, double *rhoY, double *T, double t_end, double rtol,
-                   double atol, double *h, int max_steps, int *n_rej, double *rec, int n_rec) {
+                   double atol, double *h, int max_attempts, int *n_rej, double *rec, int n_rec) {
   const int ns = m->ns, nr = m->nr;
   double q1[ORC_MAX_REAC], q2[ORC_MAX_REAC], wd[ORC_MAX_SPEC], w[ORC_MAX_SPEC];
   double jac[ORC_MAX_SPEC * ORC_MAX_SPEC], lhs[ORC_MAX_SPEC * ORC_MAX_SPEC];
@@ -240,7 +243,7 @@ int orc_chem_rok4e(const orc_mech *m, double rho, double E, double *rhoY, double
   double time = 0, err_last = 0;
   int step = 0, rej = 0;
   (void)nr;
-  while (time < t_end && step < max_steps) {
+  while (time < t_end && step + rej < max_attempts) {
     orc_chem_source(m, *T, rhoY, q1, q2, wd, w);
     const double hs = fmin(*h, t_end - time);
     orc_chem_jacobian(m, rhoY, q1, q2, jac);
@@ -281,7 +284,7 @@ int orc_chem_rok4e(const orc_mech *m, double rho, double E, double *rhoY, double
     double hstar = hs;
     if (step > 0)
       hstar *= fmin(5.0, fmax(0.2, 0.8 * pow(err_last, 0.1) * pow(err + 1e-20, -0.175)));
-    else if (err > 1)
+    else if (!(err <= 1))
       hstar *= 0.2;
     *h = hstar;
     if (err <= 1) {
@@ -302,5 +305,5 @@ int orc_chem_rok4e(const orc_mech *m, double rho, double E, double *rhoY, double
     }
   }
   if (n_rej) *n_rej = rej;
-  return step;
+  return time < t_end ? -1 : step;
 }

@@ -240,7 +240,7 @@ def chem_update_t(m, T0, yk, E):
     return lib.orc_chem_update_t(ctypes.byref(m), T0, _p(yk), E)
 
 
-def chem_rok4e(m, rho, E, rhoY, T, t_end, rtol=1e-6, atol=1e-12, h=1e-8, max_steps=10 ** 7, n_rec=0):
+def chem_rok4e(m, rho, E, rhoY, T, t_end, rtol=1e-6, atol=1e-12, h=1e-8, max_attempts=10 ** 7, n_rec=0):
     """Adaptive ROK4E of one cell.  Returns (rhoY, T, h, n_accepted, n_rejected, record[(t, T)])."""
     lib = load()
     _chem_protos(lib)
@@ -248,5 +248,5 @@ def chem_rok4e(m, rho, E, rhoY, T, t_end, rtol=1e-6, atol=1e-12, h=1e-8, max_ste
     Tc, hc, rej = ctypes.c_double(T), ctypes.c_double(h), ctypes.c_int(0)
     rec = np.zeros(2 * max(n_rec, 1))
     n = lib.orc_chem_rok4e(ctypes.byref(m), rho, E, _p(y), ctypes.byref(Tc), t_end, rtol, atol, ctypes.byref(hc),
-                           max_steps, ctypes.byref(rej), _p(rec), n_rec)
-    return y, Tc.value, hc.value, n, rej.value, rec[:2 * min(n, n_rec)].reshape(-1, 2)
+                           max_attempts, ctypes.byref(rej), _p(rec), n_rec)
+    return y, Tc.value, hc.value, n, rej.value, rec[:2 * min(max(n, 0), n_rec)].reshape(-1, 2)

@@ -1,0 +1,75 @@
+"""Pins of the surrogate MLP restatement (oracle/mlp_oracle.py), CPU only: backward pass against central
+finite differences of the loss, GELU derivative against finite differences, BF16 rounding against
+hand-checked bit patterns, and an Adam step against its closed form for a single scalar."""
+import os
+import sys
+
+import numpy as np
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
+import mlp_oracle as M  # noqa: E402
+
+
+def test_bf16_rounding():
+    x = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 2 ** -7, 1.0 + 3 * 2 ** -8, -2.5, 3.140625, 1e-3])
+    got = M.bf16(x)
+    # 1 + 2^-8 is a tie -> even (1.0); 1 + 3*2^-8 is a tie -> even (1 + 2^-6)
+    np.testing.assert_array_equal(got[:5], [1.0, 1.0, 1.0 + 2 ** -7, 1.0 + 2 ** -6, -2.5])
+    assert got[5] == 3.140625
+    assert abs(got[6] - 1e-3) / 1e-3 < 2 ** -8
+
+
+def test_gelu_grad_matches_finite_difference():
+    z = np.linspace(-6, 6, 101)
+    h = 1e-6
+    fd = (M.gelu(z + h) - M.gelu(z - h)) / (2 * h)
+    np.testing.assert_allclose(M.gelu_grad(z), fd, rtol=1e-7, atol=1e-9)
+    # known values: gelu(0) = 0, gelu(x) -> x for large x
+    assert M.gelu(0.0) == 0.0 and abs(M.gelu(10.0) - 10.0) < 1e-12
+
+
+def test_backward_matches_finite_differences():
+    sizes = (12, 16, 16, 16, 16, 10)
+    params = M.init_params(1, sizes)
+    rng = np.random.default_rng(2)
+    X = rng.standard_normal((7, 12))
+    Y = rng.standard_normal((7, 10))
+    _, grads = M.loss_and_grads(params, X, Y, emulate=False)
+    h = 1e-6
+    for l in range(len(params)):
+        W, b = params[l]
+        for (i, j) in [(0, 0), (3, 5), (W.shape[0] - 1, W.shape[1] - 1)]:
+            Wp, Wm = W.copy(), W.copy()
+            Wp[i, j] += h
+            Wm[i, j] -= h
+            pp = list(params); pp[l] = (Wp, b)
+            pm = list(params); pm[l] = (Wm, b)
+            fd = (M.loss_and_grads(pp, X, Y, False)[0] - M.loss_and_grads(pm, X, Y, False)[0]) / (2 * h)
+            assert abs(grads[l][0][i, j] - fd) <= 1e-6 * max(1e-3, abs(fd)), (l, i, j)
+        for j in [0, b.size - 1]:
+            bp, bm = b.copy(), b.copy()
+            bp[j] += h
+            bm[j] -= h
+            pp = list(params); pp[l] = (W, bp)
+            pm = list(params); pm[l] = (W, bm)
+            fd = (M.loss_and_grads(pp, X, Y, False)[0] - M.loss_and_grads(pm, X, Y, False)[0]) / (2 * h)
+            assert abs(grads[l][1][j] - fd) <= 1e-6 * max(1e-3, abs(fd)), (l, j)
+
+
+def test_emulated_rounding_stays_close_to_fp64():
+    params = M.init_params(3)
+    rng = np.random.default_rng(4)
+    X = rng.standard_normal((64, 12))
+    o64, _ = M.forward(params, X, emulate=False)
+    o16, _ = M.forward(params, X, emulate=True)
+    assert np.max(np.abs(o16 - o64)) / np.max(np.abs(o64)) < 0.05
+
+
+def test_adam_closed_form_first_step():
+    # first Adam step moves every parameter by -lr * sign(g) (up to eps)
+    params = [(np.array([[1.0, -2.0]]), np.array([0.5, 0.0]))]
+    grads = [(np.array([[0.3, -4.0]]), np.array([1e-3, -2.0]))]
+    st = {}
+    M.adam_step(params, grads, st, lr=0.01)
+    np.testing.assert_allclose(params[0][0], [[0.99, -1.99]], rtol=0, atol=1e-7)
+    np.testing.assert_allclose(params[0][1], [0.49, 0.01], rtol=0, atol=1e-5)
