@@ -24,6 +24,10 @@ SYMBOLS = (
     "corefl_chem_table", "corefl_chem_set_cells", "corefl_chem_set_cell_dt", "corefl_chem_get_cells", "corefl_chem_view_get",
     "corefl_chem_source", "corefl_chem_integrate", "corefl_chem_set_launch", "corefl_chem_get_launch",
     "corefl_chem_autotune", "corefl_chem_synchronize",
+    "corefl_surrogate_n_params", "corefl_surrogate_create", "corefl_surrogate_destroy", "corefl_surrogate_set_params",
+    "corefl_surrogate_get_params", "corefl_surrogate_set_scaling", "corefl_surrogate_set_gate",
+    "corefl_surrogate_set_data", "corefl_surrogate_train", "corefl_surrogate_grad", "corefl_surrogate_forward",
+    "corefl_chem_surrogate_step",
 )
 
 
@@ -89,6 +93,21 @@ def load_library(path=LIB_PATH):
     lib.corefl_chem_get_launch.argtypes = [P, PI, PI, PI]
     lib.corefl_chem_autotune.argtypes = [P, D, D, D, I, I, ctypes.POINTER(D), I]
     lib.corefl_chem_synchronize.argtypes = [P]
+    F = ctypes.c_float
+    lib.corefl_surrogate_n_params.restype = LL
+    lib.corefl_surrogate_n_params.argtypes = []
+    lib.corefl_surrogate_create.argtypes = [I, ctypes.POINTER(P)]
+    lib.corefl_surrogate_destroy.argtypes = [P]
+    lib.corefl_surrogate_destroy.restype = None
+    lib.corefl_surrogate_set_params.argtypes = [P, P, LL]
+    lib.corefl_surrogate_get_params.argtypes = [P, P, LL]
+    lib.corefl_surrogate_set_scaling.argtypes = [P, P, P, P, P]
+    lib.corefl_surrogate_set_gate.argtypes = [P, P]
+    lib.corefl_surrogate_set_data.argtypes = [P, LL, P, P]
+    lib.corefl_surrogate_train.argtypes = [P, I, I, F, ctypes.c_ulonglong, P]
+    lib.corefl_surrogate_grad.argtypes = [P, LL, P, P, P, P]
+    lib.corefl_surrogate_forward.argtypes = [P, LL, P, P, I]
+    lib.corefl_chem_surrogate_step.argtypes = [P, P, D, D, D, I, ctypes.POINTER(I), P]
     _lib = lib
     return lib
 
@@ -332,3 +351,98 @@ class Chemistry:
 
     def sync(self):
         _check(self._lib.corefl_chem_synchronize(self._ctx))
+
+
+SURROGATE_SIZES = (12, 128, 128, 128, 128, 10)
+
+
+def flatten_params(params):
+    """[(W_l (K, N), b_l (N,))] -> flat float32 vector in the library's layout."""
+    return np.concatenate([np.concatenate([np.asarray(W, np.float32).ravel(), np.asarray(b, np.float32).ravel()])
+                           for W, b in params]).astype(np.float32)
+
+
+def unflatten_params(flat, sizes=SURROGATE_SIZES):
+    out, o = [], 0
+    for l in range(len(sizes) - 1):
+        k, n = sizes[l], sizes[l + 1]
+        W = flat[o:o + k * n].reshape(k, n)
+        o += k * n
+        b = flat[o:o + n]
+        o += n
+        out.append((W, b))
+    return out
+
+
+class Surrogate:
+    """Chemistry surrogate MLP (12 -> 4x128 GELU -> 10) trained and run on the GPU."""
+
+    def __init__(self, batch_capacity=8192):
+        lib = load_library()
+        self._lib = lib
+        self._ctx = ctypes.c_void_p()
+        _check(lib.corefl_surrogate_create(int(batch_capacity), ctypes.byref(self._ctx)))
+        self.n_params = lib.corefl_surrogate_n_params()
+        self.capacity = batch_capacity
+
+    def close(self):
+        if self._ctx:
+            self._lib.corefl_surrogate_destroy(self._ctx)
+            self._ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, params):
+        flat = flatten_params(params) if isinstance(params, (list, tuple)) else np.ascontiguousarray(params, np.float32)
+        _check(self._lib.corefl_surrogate_set_params(self._ctx, flat.ctypes.data, flat.size))
+
+    def get_params(self, flat=False):
+        out = np.empty(self.n_params, dtype=np.float32)
+        _check(self._lib.corefl_surrogate_get_params(self._ctx, out.ctypes.data, out.size))
+        return out if flat else unflatten_params(out)
+
+    def set_scaling(self, in_mu, in_sig, out_mu, out_sig):
+        arrs = [np.ascontiguousarray(a, np.float64) for a in (in_mu, in_sig, out_mu, out_sig)]
+        _check(self._lib.corefl_surrogate_set_scaling(self._ctx, *[a.ctypes.data for a in arrs]))
+
+    def set_gate(self, t_min, t_max, p_min, p_max, dt_min, dt_max, dT_max):
+        g = np.array([t_min, t_max, p_min, p_max, dt_min, dt_max, dT_max], dtype=np.float64)
+        _check(self._lib.corefl_surrogate_set_gate(self._ctx, g.ctypes.data))
+
+    def set_data(self, X, Y):
+        X = np.ascontiguousarray(X, np.float32)
+        Y = np.ascontiguousarray(Y, np.float32)
+        _check(self._lib.corefl_surrogate_set_data(self._ctx, X.shape[0], X.ctypes.data, Y.ctypes.data))
+
+    def train(self, batch, steps, lr, seed=1):
+        loss = np.empty(max(steps, 1), dtype=np.float32)
+        _check(self._lib.corefl_surrogate_train(self._ctx, int(batch), int(steps), float(lr), int(seed),
+                                                loss.ctypes.data))
+        return loss[:steps]
+
+    def grad(self, X, Y):
+        X = np.ascontiguousarray(X, np.float32)
+        Y = np.ascontiguousarray(Y, np.float32)
+        loss = np.empty(1, dtype=np.float32)
+        g = np.empty(self.n_params, dtype=np.float32)
+        _check(self._lib.corefl_surrogate_grad(self._ctx, X.shape[0], X.ctypes.data, Y.ctypes.data, loss.ctypes.data,
+                                               g.ctypes.data))
+        return float(loss[0]), unflatten_params(g)
+
+    def forward(self, X, fused=True):
+        X = np.ascontiguousarray(X, np.float32)
+        out = np.empty((X.shape[0], 10), dtype=np.float32)
+        _check(self._lib.corefl_surrogate_forward(self._ctx, X.shape[0], X.ctypes.data, out.ctypes.data,
+                                                  1 if fused else 0))
+        return out
+
+    def chem_step(self, chem, dt, rtol=1e-6, atol=1e-8, max_attempts=100000):
+        """Surrogate sub-step of chem's batch with exact fallback; returns the number of fallback cells."""
+        nf = ctypes.c_int()
+        _check(self._lib.corefl_chem_surrogate_step(chem._ctx, self._ctx, float(dt), float(rtol), float(atol),
+                                                    int(max_attempts), ctypes.byref(nf), None))
+        return nf.value

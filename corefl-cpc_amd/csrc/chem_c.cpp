@@ -14,26 +14,8 @@
 #include <vector>
 
 #include "chem.hpp"
+#include "chem_ctx.hpp"
 #include "chem_kernels.hpp"
-
-namespace corefl {
-void set_last_error(const std::string &msg);
-}
-
-struct corefl_chem {
-  corefl::ChemMech mech{};
-  corefl::ChemMech *mech_dev = nullptr;
-  hipStream_t stream = nullptr;
-  int *counter = nullptr;
-  int cap = 0;
-  corefl::ChemBatch b{};
-  double *save = nullptr; // autotuner copy of T, rhoY, h
-  double *dtc = nullptr;  // per-cell sub-step lengths (capacity cap)
-  bool have_dtc = false;
-  corefl::ChemLaunch cfg{1, 64, 0};
-  bool tuned = false;
-  ~corefl_chem();
-};
 
 namespace {
 
@@ -67,16 +49,21 @@ template<class F> int guarded(F &&f) {
 
 void free_batch(corefl_chem *c) {
   for (void *p : {(void *)c->b.rho, (void *)c->b.e, (void *)c->b.T, (void *)c->b.rhoY, (void *)c->b.h,
-                  (void *)c->b.steps, (void *)c->b.rejects, (void *)c->save, (void *)c->dtc})
+                  (void *)c->b.steps, (void *)c->b.rejects, (void *)c->save, (void *)c->dtc, (void *)c->list})
     if (p) (void)hipFree(p);
   c->b = corefl::ChemBatch{};
   c->save = nullptr;
   c->dtc = nullptr;
+  c->list = nullptr;
   c->have_dtc = false;
   c->cap = 0;
 }
 
-void ensure_device(corefl_chem *c) {
+} // namespace
+
+namespace corefl {
+
+void chem_ensure_device(corefl_chem *c) {
   if (c->mech_dev) return;
   if (c->mech.ns != 9)
     throw std::runtime_error("the device chemistry kernels are instantiated for 9 species; this mechanism has " +
@@ -84,15 +71,16 @@ void ensure_device(corefl_chem *c) {
   HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   HIPCHK(hipMalloc(&c->mech_dev, sizeof(corefl::ChemMech)));
   HIPCHK(hipMalloc(&c->counter, sizeof(int)));
+  HIPCHK(hipMalloc(&c->list_count, sizeof(int)));
   HIPCHK(hipMemcpy(c->mech_dev, &c->mech, sizeof(corefl::ChemMech), hipMemcpyHostToDevice));
   int cus = 0;
   HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0));
   c->cfg = corefl::ChemLaunch{1, 64, 4 * cus};
 }
 
-void ensure_batch(corefl_chem *c, long long n) {
+void chem_ensure_batch(corefl_chem *c, long long n) {
   if (n < 0 || n > (1LL << 30)) throw std::runtime_error("batch size out of range [0, 2^30]");
-  ensure_device(c);
+  chem_ensure_device(c);
   if (n <= c->cap) {
     c->b.n = (int)n;
     return;
@@ -108,32 +96,37 @@ void ensure_batch(corefl_chem *c, long long n) {
   HIPCHK(hipMalloc(&c->b.steps, sizeof(int) * (size_t)n));
   HIPCHK(hipMalloc(&c->b.rejects, sizeof(int) * (size_t)n));
   HIPCHK(hipMalloc(&c->dtc, d));
+  HIPCHK(hipMalloc(&c->list, sizeof(int) * (size_t)n));
   c->cap = (int)n;
   c->b.n = (int)n;
 }
 
-hipStream_t pick(corefl_chem *c, void *stream) { return stream ? (hipStream_t)stream : c->stream; }
+hipStream_t chem_stream(corefl_chem *c, void *stream) { return stream ? (hipStream_t)stream : c->stream; }
 
-// dt > 0: every cell advances by dt; dt == 0: by its own length (corefl_chem_set_cell_dt).
-void integrate(corefl_chem *c, const corefl::ChemLaunch &cfg, double dt, double rtol, double atol, int max_attempts,
-               hipStream_t s) {
-  corefl::ChemBatch b = c->b;
+void chem_integrate(corefl_chem *c, const ChemLaunch &cfg, double dt, double rtol, double atol, int max_attempts,
+                    hipStream_t s, const int *list, const int *n_list) {
+  ChemBatch b = c->b;
   b.dtc = dt > 0 ? nullptr : c->dtc;
-  HIPCHK(corefl::launch_chem_integrate(c->mech_dev, c->mech.ns, b, dt, rtol, atol, max_attempts, cfg, c->counter, s));
+  b.list = list;
+  b.n_list = n_list;
+  HIPCHK(launch_chem_integrate(c->mech_dev, c->mech.ns, b, dt, rtol, atol, max_attempts, cfg, c->counter, s));
 }
 
-void check_args(corefl_chem *c, double dt, double rtol, double atol, int max_attempts) {
+void chem_check_args(corefl_chem *c, double dt, double rtol, double atol, int max_attempts) {
   if (!(dt >= 0) || !(rtol > 0) || !(atol > 0)) throw std::runtime_error("dt >= 0, rtol > 0 and atol > 0 required");
   if (dt == 0 && !c->have_dtc) throw std::runtime_error("dt == 0 needs per-cell lengths from corefl_chem_set_cell_dt");
   if (max_attempts < 1) throw std::runtime_error("max_attempts must be at least 1");
 }
 
-} // namespace
+} // namespace corefl
+
+using namespace corefl;
 
 corefl_chem::~corefl_chem() {
   free_batch(this);
   if (mech_dev) (void)hipFree(mech_dev);
   if (counter) (void)hipFree(counter);
+  if (list_count) (void)hipFree(list_count);
   if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -229,7 +222,7 @@ int corefl_chem_set_cells(corefl_chem *c, long long n, const double *rho, const 
                           const double *rhoY) {
   return guarded([&] {
     if (!c || (n > 0 && (!rho || !T || !rhoY))) throw std::runtime_error("corefl_chem_set_cells: null argument");
-    ensure_batch(c, n);
+    chem_ensure_batch(c, n);
     if (n == 0) return;
     const size_t d = sizeof(double) * (size_t)n;
     HIPCHK(hipMemcpyAsync(c->b.rho, rho, d, hipMemcpyHostToDevice, c->stream));
@@ -297,7 +290,7 @@ int corefl_chem_view_get(corefl_chem *c, corefl_chem_view *v) {
 int corefl_chem_source(corefl_chem *c, double *omega_host) {
   return guarded([&] {
     if (!c || !omega_host) throw std::runtime_error("corefl_chem_source: null argument");
-    ensure_device(c);
+    chem_ensure_device(c);
     const long long n = c->b.n;
     if (n == 0) return;
     double *w = nullptr;
@@ -315,9 +308,9 @@ int corefl_chem_source(corefl_chem *c, double *omega_host) {
 int corefl_chem_integrate(corefl_chem *c, double dt, double rtol, double atol, int max_attempts, void *stream) {
   return guarded([&] {
     if (!c) throw std::runtime_error("corefl_chem_integrate: null context");
-    check_args(c, dt, rtol, atol, max_attempts);
-    ensure_device(c);
-    integrate(c, c->cfg, dt, rtol, atol, max_attempts, pick(c, stream));
+    chem_check_args(c, dt, rtol, atol, max_attempts);
+    chem_ensure_device(c);
+    chem_integrate(c, c->cfg, dt, rtol, atol, max_attempts, chem_stream(c, stream), nullptr, nullptr);
   });
 }
 
@@ -328,7 +321,7 @@ int corefl_chem_set_launch(corefl_chem *c, int queue, int threads, int blocks) {
       throw std::runtime_error("corefl_chem_set_launch: threads must be 64, 128 or 256");
     if (queue && (blocks < 1 || blocks > (1 << 20)))
       throw std::runtime_error("corefl_chem_set_launch: blocks must be in [1, 2^20]");
-    ensure_device(c);
+    chem_ensure_device(c);
     c->cfg = corefl::ChemLaunch{queue ? 1 : 0, threads, queue ? blocks : 0};
   });
 }
@@ -346,9 +339,9 @@ int corefl_chem_autotune(corefl_chem *c, double dt, double rtol, double atol, in
                          double *ms_out, int n_out) {
   return guarded([&] {
     if (!c) throw std::runtime_error("corefl_chem_autotune: null context");
-    check_args(c, dt, rtol, atol, max_attempts);
+    chem_check_args(c, dt, rtol, atol, max_attempts);
     if (reps < 1) throw std::runtime_error("corefl_chem_autotune: reps must be at least 1");
-    ensure_device(c);
+    chem_ensure_device(c);
     const long long n = c->b.n;
     if (n == 0) throw std::runtime_error("corefl_chem_autotune: empty batch");
     int cus = 0;
@@ -391,7 +384,7 @@ int corefl_chem_autotune(corefl_chem *c, double dt, double rtol, double atol, in
         for (int r = 0; r < reps; ++r) {
           save_or_restore(false);
           HIPCHK(hipEventRecord(e0, c->stream));
-          integrate(c, cand[q], dt, rtol, atol, max_attempts, c->stream);
+          chem_integrate(c, cand[q], dt, rtol, atol, max_attempts, c->stream, nullptr, nullptr);
           HIPCHK(hipEventRecord(e1, c->stream));
           HIPCHK(hipEventSynchronize(e1));
           float ms = 0;

@@ -1,0 +1,49 @@
+// The chemistry context behind the corefl_chem_* C ABI, shared by the chemistry and surrogate entry points.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <stdexcept>
+#include <string>
+
+#include "../../include/corefl_c.h"
+#include "chem.hpp"
+#include "chem_kernels.hpp"
+
+#define COREFL_HIPCHK(x)                                                                                   \
+  do {                                                                                                     \
+    hipError_t e_ = (x);                                                                                   \
+    if (e_ != hipSuccess) throw std::runtime_error(std::string(#x) + ": " + hipGetErrorString(e_));       \
+  } while (0)
+
+struct corefl_chem {
+  corefl::ChemMech mech{};
+  corefl::ChemMech *mech_dev = nullptr;
+  hipStream_t stream = nullptr;
+  int *counter = nullptr;
+  int cap = 0;
+  corefl::ChemBatch b{};
+  double *save = nullptr; // autotuner copy of T, rhoY, h
+  double *dtc = nullptr;  // per-cell sub-step lengths (capacity cap)
+  bool have_dtc = false;
+  int *list = nullptr;       // cell list of the surrogate's fallback cells (capacity cap)
+  int *list_count = nullptr; // its device-side length
+  corefl::ChemLaunch cfg{1, 64, 0};
+  bool tuned = false;
+  ~corefl_chem();
+};
+
+namespace corefl {
+
+void set_last_error(const std::string &msg);
+// Device tables, stream and counters on first use (needs a GPU).
+void chem_ensure_device(corefl_chem *c);
+// Batch arrays for n cells (grown when needed; contents undefined after growth).
+void chem_ensure_batch(corefl_chem *c, long long n);
+hipStream_t chem_stream(corefl_chem *c, void *stream);
+void chem_check_args(corefl_chem *c, double dt, double rtol, double atol, int max_attempts);
+// Exact sub-step of every cell, or of list[0 .. *n_list) when list is given.
+void chem_integrate(corefl_chem *c, const ChemLaunch &cfg, double dt, double rtol, double atol, int max_attempts,
+                    hipStream_t s, const int *list, const int *n_list);
+
+} // namespace corefl

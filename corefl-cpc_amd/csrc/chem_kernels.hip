@@ -11,11 +11,13 @@
 
 #include <cmath>
 
+#include "chem_device.hpp"
+
 namespace corefl {
 
 namespace {
 
-constexpr double kRu = 8314.462618, kPatm = 101325.0;
+using namespace chemdev;
 
 // ROK4E coefficients (Define.h:28-43)
 constexpr double G = 0.572816062482135;
@@ -24,174 +26,6 @@ constexpr double G21 = -0.602765307997356 / G, G31 = -1.389195789724843 / G, G32
 constexpr double G41 = 0.992356412977094 / G, G42 = -1.390032613873701 / G, G43 = -0.440875890223325 / G;
 constexpr double B1 = 0.194335256262729, B2 = 0.483167813989227, B4 = 0.322496929748044;
 constexpr double E1 = -0.217819895945721 - B1, E2 = 1.03130847478467 - B2, E3 = 0.186511421161047, E4 = -B4;
-
-// h_k and cp_k per unit mass, linear extrapolation outside [t_low, t_high] (Thermo.cu:470-520).
-template<int NS> __device__ __forceinline__ void h_cp(const ChemMech &m, double t, double *h, double *cp) {
-  const double t2 = t * t, t3 = t2 * t, t4 = t3 * t, t5 = t4 * t;
-#pragma unroll
-  for (int i = 0; i < NS; ++i) {
-    const double R = kRu / m.mw[i];
-    const bool below = t < m.t_low[i], above = t > m.t_high[i];
-    if (below || above) {
-      const double tt = below ? m.t_low[i] : m.t_high[i];
-      const double *c = below ? m.lo[i] : m.hi[i];
-      const double tt2 = tt * tt, tt3 = tt2 * tt, tt4 = tt3 * tt, tt5 = tt4 * tt;
-      const double cc = c[0] + c[1] * tt + c[2] * tt2 + c[3] * tt3 + c[4] * tt4;
-      const double hh = c[0] * tt + 0.5 * c[1] * tt2 + c[2] * tt3 / 3 + 0.25 * c[3] * tt4 + 0.2 * c[4] * tt5 + c[5];
-      h[i] = (hh + cc * (t - tt)) * R;
-      cp[i] = cc * R;
-    } else {
-      const double *c = t < m.t_mid[i] ? m.lo[i] : m.hi[i];
-      h[i] = (c[0] * t + 0.5 * c[1] * t2 + c[2] * t3 / 3 + 0.25 * c[3] * t4 + 0.2 * c[4] * t5 + c[5]) * R;
-      cp[i] = (c[0] + c[1] * t + c[2] * t2 + c[3] * t3 + c[4] * t4) * R;
-    }
-  }
-}
-
-// g_k / (R T), clamped to the fit range (Thermo.cu:574-598).
-template<int NS> __device__ __forceinline__ void gibbs(const ChemMech &m, double t, double *g) {
-#pragma unroll
-  for (int i = 0; i < NS; ++i) {
-    const bool below = t < m.t_low[i], above = t > m.t_high[i];
-    const double tt = below ? m.t_low[i] : (above ? m.t_high[i] : t);
-    const double *c = below ? m.lo[i] : (above ? m.hi[i] : (t < m.t_mid[i] ? m.lo[i] : m.hi[i]));
-    const double tt2 = tt * tt, tt3 = tt2 * tt, tt4 = tt3 * tt;
-    g[i] = c[0] * (1.0 - log(tt)) - 0.5 * c[1] * tt - c[2] * tt2 / 6.0 - c[3] * tt3 / 12.0 - c[4] * tt4 * 0.05 +
-           c[5] / tt - c[6];
-  }
-}
-
-__device__ __forceinline__ double arrhenius(double t, double A, double b, double Ea) {
-  return A * pow(t, b) * exp(-Ea / t);
-}
-
-__device__ __forceinline__ double mass_action(double c, double nu) {
-  // integer coefficients by repeated products (equal to pow() to rounding), others through pow()
-  const int k = (int)nu;
-  if ((double)k != nu) return pow(c, nu);
-  double r = 1.0;
-  for (int q = 0; q < k; ++q) r *= c;
-  return r;
-}
-
-// Chemical source omega [kg/(m^3 s)] of one cell (FiniteRateChem.cu:1466-1605).  With JAC, also the
-// Jacobian d omega / d(rho Y) with the rates of progress frozen (ConstVolumeReactor.cpp:1168-1187).
-template<int NS, bool JAC>
-__device__ void source(const ChemMech &m, double t, const double *rhoY, double *w, double *J) {
-  double c[NS], g[NS], cr[NS], de[NS];
-#pragma unroll
-  for (int l = 0; l < NS; ++l) {
-    c[l] = rhoY[l] / m.mw[l] * 1e-3;
-    cr[l] = 0;
-    de[l] = 0;
-  }
-  if constexpr (JAC) {
-#pragma unroll
-    for (int q = 0; q < NS * NS; ++q) J[q] = 0;
-  }
-  // Equilibrium constants are needed unless every reversible reaction carries explicit REV parameters
-  // (FiniteRateChem.cu:1519-1557); then they are applied to every reaction of type other than 0 and 2.
-  if (m.need_gibbs) gibbs<NS>(m, t, g);
-  const double tp = kPatm / kRu * 1e-3 / t;
-  for (int r = 0; r < m.nr; ++r) {
-    const int ty = m.type[r];
-    double kf = arrhenius(t, m.A[r], m.b[r], m.Ea[r]);
-    double cc = 0;
-    if (ty >= 4) {
-#pragma unroll
-      for (int l = 0; l < NS; ++l) cc += c[l] * m.tb[r][l];
-    }
-    if (ty == 3) {
-      kf += arrhenius(t, m.A2[r], m.b2[r], m.Ea2[r]);
-    } else if (ty == 4) {
-      kf *= cc;
-    } else if (ty > 4) {
-      const double klow = arrhenius(t, m.A2[r], m.b2[r], m.Ea2[r]), khigh = kf;
-      if (khigh < 1e-25 && klow < 1e-25) {
-        kf = 0;
-      } else {
-        const double pr = klow * cc / khigh;
-        double F = 1.0;
-        if (ty > 5) {
-          double fc = (1 - m.troe[r][0]) * exp(-t / m.troe[r][1]) + m.troe[r][0] * exp(-t / m.troe[r][2]);
-          if (ty == 7) fc += exp(-m.troe[r][3] / t);
-          const double lfc = log10(fc);
-          const double cq = -0.4 - 0.67 * lfc, nq = 0.75 - 1.27 * lfc;
-          const double lpr = log10(pr);
-          const double tmp = (lpr + cq) / (nq - 0.14 * (lpr + cq));
-          F = pow(10.0, lfc / (1.0 + tmp * tmp));
-        }
-        kf = khigh * pr / (1.0 + pr) * F;
-      }
-    }
-    double kb = 0;
-    if (ty != 0 && m.rev[r] == 1) {
-      kb = arrhenius(t, m.A2[r], m.b2[r], m.Ea2[r]);
-      if (ty == 4) kb *= cc;
-    }
-    if (m.need_gibbs && ty != 0 && ty != 2) {
-      double dg = 0;
-#pragma unroll
-      for (int l = 0; l < NS; ++l) dg += g[l] * (m.nb[r][l] - m.nf[r][l]);
-      kb = kf / (pow(tp, m.order[r]) * exp(-dg));
-    }
-    double q1 = 1.0, q2 = ty != 0 ? 1.0 : 0.0;
-#pragma unroll
-    for (int l = 0; l < NS; ++l) {
-      if (m.nf[r][l] != 0) q1 *= mass_action(c[l], m.nf[r][l]);
-      if (ty != 0 && m.nb[r][l] != 0) q2 *= mass_action(c[l], m.nb[r][l]);
-    }
-    q1 *= kf;
-    q2 *= kb;
-#pragma unroll
-    for (int l = 0; l < NS; ++l) {
-      cr[l] += q2 * m.nf[r][l] + q1 * m.nb[r][l];
-      de[l] += q1 * m.nf[r][l] + q2 * m.nb[r][l];
-    }
-    if constexpr (JAC) {
-#pragma unroll
-      for (int a = 0; a < NS; ++a) {
-        const double dn = m.nb[r][a] - m.nf[r][a];
-        if (dn == 0) continue;
-#pragma unroll
-        for (int n = 0; n < NS; ++n) J[a * NS + n] += dn * (m.nf[r][n] * q1 - m.nb[r][n] * q2);
-      }
-    }
-  }
-#pragma unroll
-  for (int l = 0; l < NS; ++l) w[l] = cr[l] * (1e+3 * m.mw[l]) - de[l] * (1e+3 * m.mw[l]);
-  if constexpr (JAC) {
-#pragma unroll
-    for (int n = 0; n < NS; ++n) {
-      const double inv = rhoY[n] > 1e-30 ? 1.0 / rhoY[n] : 0.0;
-#pragma unroll
-      for (int a = 0; a < NS; ++a) J[a * NS + n] = J[a * NS + n] * inv * (1e+3 * m.mw[a]);
-    }
-  }
-}
-
-// Newton iteration for T at fixed specific internal energy E (ConstVolumeReactor.cpp:970-995).
-template<int NS> __device__ double update_t(const ChemMech &m, double t, const double *yk, double E) {
-  double imw = 0;
-#pragma unroll
-  for (int l = 0; l < NS; ++l) imw += yk[l] / m.mw[l];
-  const double R = kRu * imw;
-  double err = 1e+6;
-  for (int it = 0; it < 100 && err > 1e-8; ++it) {
-    double h[NS], cp[NS];
-    h_cp<NS>(m, t, h, cp);
-    double cpt = 0, hh = 0;
-#pragma unroll
-    for (int l = 0; l < NS; ++l) {
-      cpt += cp[l] * yk[l];
-      hh += h[l] * yk[l];
-    }
-    const double t1 = t - (hh - R * t - E) / (cpt - R);
-    err = fabs(1 - t1 / t);
-    t = t1;
-  }
-  return t;
-}
 
 // Doolittle LU with partial pivoting, row-major (ConstVolumeReactor.cpp:1231-1262), and its solve.
 template<int NS> __device__ __forceinline__ void lu(double *a, int *piv) {
@@ -354,11 +188,16 @@ template<int NS> __device__ __forceinline__ void store_cell(const ChemBatch &b, 
   b.rejects[c] = s.rejects;
 }
 
+// Work item w of a launch: cell w, or cell list[w] of a device-side list of *n_list cells.
+__device__ __forceinline__ int work_count(const ChemBatch &b) { return b.list ? *b.n_list : b.n; }
+__device__ __forceinline__ int work_cell(const ChemBatch &b, int w) { return b.list ? b.list[w] : w; }
+
 template<int NS>
 __global__ __launch_bounds__(256) void k_chem_static(const ChemMech *__restrict__ mp, ChemBatch b, double dt,
                                                      double rtol, double atol, int max_attempts) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= b.n) return;
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= work_count(b)) return;
+  const int c = work_cell(b, w);
   const ChemMech &m = *mp;
   Cell<NS> s;
   load_cell<NS>(b, c, dt, s);
@@ -371,6 +210,7 @@ __global__ __launch_bounds__(256) void k_chem_queue(const ChemMech *__restrict__
                                                     double rtol, double atol, int max_attempts, int *counter) {
   const ChemMech &m = *mp;
   const int lane = __lane_id();
+  const int n_work = work_count(b);
   Cell<NS> s;
   int cell = -1;
   bool exhausted = false;
@@ -384,8 +224,8 @@ __global__ __launch_bounds__(256) void k_chem_queue(const ChemMech *__restrict__
       base = __shfl(base, leader, 64);
       if (need) {
         const int got = base + __popcll(mask & ((1ull << lane) - 1));
-        if (got < b.n) {
-          cell = got;
+        if (got < n_work) {
+          cell = work_cell(b, got);
           load_cell<NS>(b, cell, dt, s);
         } else {
           exhausted = true;

@@ -18,6 +18,8 @@ struct ChemBatch {
   int *steps;    // accepted steps of the last sub-step (-1: attempt limit reached)
   int *rejects;  // rejected attempts of the last sub-step
   const double *dtc; // per-cell sub-step length (nullptr: the launch's dt for every cell)
+  const int *list;   // integrate only cells list[0 .. *n_list) (nullptr: all n cells)
+  const int *n_list; // device-side count, at most n
 };
 
 // Launch shape picked by the autotuner.

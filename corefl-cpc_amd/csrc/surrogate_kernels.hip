@@ -1,0 +1,522 @@
+// Surrogate MLP kernels for gfx950: BF16 MFMA (v_mfma_f32_32x32x16_bf16) with FP32 accumulation.
+//
+// MFMA 32x32x16 operand layout (wave64): lane l supplies A[i][k] for i = l % 32, k = 8 (l / 32) + 0..7 and
+// B[k][j] for j = l % 32, k = 8 (l / 32) + 0..7; it receives D[i][j] for j = l % 32 in 16 registers,
+// register r holding row i = 8 (r / 4) + 4 (l / 32) + r % 4.  Both operands are therefore read as eight
+// consecutive k values of one row (A) or one column (B): activations are stored [cell][k] and weights as
+// W^T [n][k], so every operand fetch is one 16-byte LDS read.
+#include "surrogate.hpp"
+
+#include <cmath>
+
+#include "chem_device.hpp"
+
+namespace corefl {
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef short shortx8 __attribute__((ext_vector_type(8)));
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ floatx16 mfma(const shortx8 &a, const shortx8 &b, const floatx16 &c) {
+  return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c,
+                                                 0, 0, 0);
+}
+
+// FP32 -> BF16, round to nearest even (finite inputs).
+__device__ __forceinline__ unsigned short f2bf(float f) {
+  unsigned u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+
+// GELU, tanh form: 0.5 z (1 + tanh(u)) = z / (1 + exp(-2u)), u = sqrt(2/pi) (z + 0.044715 z^3).
+constexpr float kGs = 0.7978845608028654f, kGa = 0.044715f;
+__device__ __forceinline__ float sig2u(float z) {
+  const float u = kGs * (z + kGa * z * z * z);
+  return 1.0f / (1.0f + __expf(-2.0f * u));
+}
+__device__ __forceinline__ float gelu(float z) { return z * sig2u(z); }
+__device__ __forceinline__ float gelu_grad(float z) {
+  const float s = sig2u(z);
+  return s + z * s * (1.0f - s) * 2.0f * kGs * (1.0f + 3.0f * kGa * z * z);
+}
+
+// ------------------------------------------------------------------------------------------------ GEMM
+constexpr int kTS = 40; // LDS row stride (elements) of a 32-deep k chunk, padded against bank conflicts
+
+__global__ __launch_bounds__(256) void k_gemm(GemmArgs a) {
+  __shared__ __attribute__((aligned(16))) unsigned short As[64 * kTS];
+  __shared__ __attribute__((aligned(16))) unsigned short Bs[64 * kTS];
+  const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
+  const int m0 = blockIdx.x * 64, n0 = blockIdx.y * 64;
+  const int kb = blockIdx.z * a.k_chunk;
+  const int ke = min(a.K, kb + a.k_chunk);
+  const int wm = wave >> 1, wn = wave & 1;
+  const int lr = t >> 2, lk = (t & 3) * 8; // staging: row lr of the tile, k offset lk
+  floatx16 acc;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
+  for (int k0 = kb; k0 < ke; k0 += 32) {
+    {
+      const int m = m0 + lr;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int k = k0 + lk + q;
+        As[lr * kTS + lk + q] = (m < a.M && k < ke) ? a.A[m * a.sam + k * a.sak] : (unsigned short)0;
+      }
+      const int n = n0 + lr;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int k = k0 + lk + q;
+        Bs[lr * kTS + lk + q] = (n < a.N && k < ke) ? a.B[k * a.sbk + n * a.sbn] : (unsigned short)0;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < 32; ks += 16) {
+      const int kk = ks + 8 * (lane >> 5);
+      const shortx8 av = *reinterpret_cast<const shortx8 *>(&As[(wm * 32 + (lane & 31)) * kTS + kk]);
+      const shortx8 bv = *reinterpret_cast<const shortx8 *>(&Bs[(wn * 32 + (lane & 31)) * kTS + kk]);
+      acc = mfma(av, bv, acc);
+    }
+    __syncthreads();
+  }
+  const int n = n0 + wn * 32 + (lane & 31);
+  if (n >= a.N) return;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int m = m0 + wm * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+    if (m >= a.M) continue;
+    const float v = acc[r];
+    if (a.ep == kEpFwdHidden) {
+      const float z = v + a.bias[n];
+      a.C[m * a.ldc + n] = z;
+      a.Cb[m * a.ldcb + n] = f2bf(gelu(z));
+    } else if (a.ep == kEpFwdOut) {
+      a.C[m * a.ldc + n] = v + a.bias[n];
+    } else if (a.ep == kEpBwdData) {
+      a.Cb[m * a.ldcb + n] = f2bf(v * gelu_grad(a.zprev[m * a.ldz + n]));
+    } else {
+      atomicAdd(&a.C[m * a.ldc + n], v);
+    }
+  }
+}
+
+__device__ __forceinline__ unsigned long long mix64(unsigned long long x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
+__global__ void k_gather(const float *X, const float *Y, long n, unsigned long long seed, long step, int B,
+                         unsigned short *H0, float *Yb) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= B) return;
+  const long idx = seed ? (long)(mix64(seed ^ mix64((unsigned long long)step * 0x100000001B3ull + i)) % n) : i;
+#pragma unroll
+  for (int f = 0; f < 16; ++f) H0[(long)i * 16 + f] = f < kMlpIn ? f2bf(X[idx * kMlpIn + f]) : (unsigned short)0;
+#pragma unroll
+  for (int o = 0; o < kMlpOut; ++o) Yb[(long)i * kMlpOut + o] = Y[idx * kMlpOut + o];
+}
+
+__global__ __launch_bounds__(256) void k_loss(const float *out, const float *Yb, int B, unsigned short *G,
+                                              float *loss) {
+  __shared__ float part[4];
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const float inv = 1.0f / (float)((long)B * kMlpOut);
+  float sq = 0.0f;
+  if (i < (long)B * kMlpOut) {
+    const float d = out[i] - Yb[i];
+    G[i] = f2bf(2.0f * d * inv);
+    sq = d * d * inv;
+  }
+  for (int o = 32; o > 0; o >>= 1) sq += __shfl_xor(sq, o, 64);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = sq;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicAdd(loss, part[0] + part[1] + part[2] + part[3]);
+}
+
+__device__ __forceinline__ float bf2f(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
+
+// 256 threads: column n = t % 128, rows b0 + t / 128 + 2q of a 256-row chunk.
+__global__ __launch_bounds__(256) void k_colsum(const unsigned short *G, int B, int N, long ldg, float *db) {
+  const int n = threadIdx.x & 127;
+  if (n >= N) return;
+  const int b0 = blockIdx.x * 256 + (threadIdx.x >> 7);
+  float s = 0.0f;
+  for (int q = 0; q < 128; ++q) {
+    const int b = b0 + 2 * q;
+    if (b < B) s += bf2f(G[(long)b * ldg + n]);
+  }
+  atomicAdd(&db[n], s);
+}
+
+__global__ void k_adam(float *p, float *m, float *v, const float *g, long n, float lr, float b1, float b2, float eps,
+                       float c1, float c2, unsigned short *wt, int K, int N, int Kp) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.0f - b1) * gi;
+  const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  const float pi = p[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+  p[i] = pi;
+  if (wt) wt[(i % N) * Kp + i / N] = f2bf(pi);
+}
+
+__global__ void k_to_bf16t(const float *W, int K, int N, int Kp, unsigned short *wt) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long)K * N) return;
+  wt[(i % N) * Kp + i / N] = f2bf(W[i]);
+}
+
+__global__ void k_fill(float *p, long n, float v) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------------------- fused inference
+constexpr int kS0 = 24;  // LDS row stride of W0^T (16 + 8)
+constexpr int kSH = 136; // LDS row stride of 128-deep rows (128 + 8)
+
+struct FusedArgs {
+  long n;
+  const float *x; // raw mode
+  float *y;
+  ChemBatch b;    // chemistry mode
+  double dt;
+  const ChemMech *mech;
+  SurrogateScaling sc;
+  SurrogateGate gate;
+  ElementMatrix em;
+  int *fb_list, *fb_count;
+};
+
+// copy rows x cols BF16 from global [rows][cols] into LDS with row stride `stride` (cols multiple of 8)
+__device__ __forceinline__ void load_rows(unsigned short *dst, int stride, const unsigned short *src, int rows,
+                                          int cols) {
+  const int per_row = cols / 8;
+  for (int q = threadIdx.x; q < rows * per_row; q += blockDim.x) {
+    const int r = q / per_row, c = (q % per_row) * 8;
+    *reinterpret_cast<shortx8 *>(&dst[r * stride + c]) = *reinterpret_cast<const shortx8 *>(&src[r * cols + c]);
+  }
+}
+
+// One layer for the wave's 32 rows: acc[j] (column tile j of 32) = h[32][KSTEPS*16] * W^T rows j*32..
+template<int KSTEPS, int NT>
+__device__ __forceinline__ void mma_layer(const unsigned short *h, const unsigned short *w, int ws, floatx16 *acc,
+                                          int lane) {
+#pragma unroll
+  for (int j = 0; j < NT; ++j)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[j][r] = 0.0f;
+  const int row = lane & 31, kh = 8 * (lane >> 5);
+#pragma unroll
+  for (int ks = 0; ks < KSTEPS; ++ks) {
+    const shortx8 av = *reinterpret_cast<const shortx8 *>(&h[row * kSH + ks * 16 + kh]);
+#pragma unroll
+    for (int j = 0; j < NT; ++j) {
+      const shortx8 bv = *reinterpret_cast<const shortx8 *>(&w[(j * 32 + row) * ws + ks * 16 + kh]);
+      acc[j] = mfma(av, bv, acc[j]);
+    }
+  }
+}
+
+__device__ __forceinline__ void store_hidden(unsigned short *h, const floatx16 *acc, const float *bias, int lane) {
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int n = j * 32 + (lane & 31);
+    const float bn = bias[n];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int m = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+      h[m * kSH + n] = f2bf(gelu(acc[j][r] + bn));
+    }
+  }
+}
+
+// Mass fractions, pressure and the in-range flag of cell c; features scaled into f[0..12).
+__device__ __forceinline__ bool chem_features(const FusedArgs &a, long c, float *f) {
+  const ChemBatch &b = a.b;
+  const ChemMech &m = *a.mech;
+  const double rho = b.rho[c], T = b.T[c], dt = b.dtc ? b.dtc[c] : a.dt;
+  double imw = 0.0;
+  double y[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    y[k] = b.rhoY[(long)k * b.n + c] / rho;
+    imw += y[k] / m.mw[k];
+  }
+  const double p = rho * chemdev::kRu * imw * T;
+  const SurrogateScaling &s = a.sc;
+  f[0] = (float)((T - s.in_mu[0]) / s.in_sig[0]);
+  f[1] = (float)((log(p) - s.in_mu[1]) / s.in_sig[1]);
+#pragma unroll
+  for (int k = 0; k < 9; ++k) f[2 + k] = (float)((y[k] - s.in_mu[2 + k]) / s.in_sig[2 + k]);
+  f[11] = (float)((log(dt) - s.in_mu[11]) / s.in_sig[11]);
+  const SurrogateGate &g = a.gate;
+  return T >= g.t_min && T <= g.t_max && p >= g.p_min && p <= g.p_max && dt >= g.dt_min && dt <= g.dt_max;
+}
+
+// Solve the symmetric positive definite 4 x 4 system M x = r in place (elimination without pivoting, which
+// is stable for SPD matrices); false if a pivot is not positive.
+__device__ __forceinline__ bool spd_solve4(double (&M)[4][4], double (&r)[4]) {
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    if (!(M[k][k] > 1e-300)) return false;
+    const double inv = 1.0 / M[k][k];
+#pragma unroll
+    for (int i = k + 1; i < 4; ++i) {
+      const double f = M[i][k] * inv;
+#pragma unroll
+      for (int j = k; j < 4; ++j) M[i][j] -= f * M[k][j];
+      r[i] -= f * r[k];
+    }
+  }
+#pragma unroll
+  for (int k = 3; k >= 0; --k) {
+#pragma unroll
+    for (int j = k + 1; j < 4; ++j) r[k] -= M[k][j] * r[j];
+    r[k] /= M[k][k];
+  }
+  return true;
+}
+
+// Network outputs o[10] of cell c -> projected mass fractions and energy-consistent temperature; writes the
+// cell if the gate passes, else appends it to the fallback list.
+__device__ void chem_finish(const FusedArgs &a, long c, const float *o, bool in_range) {
+  const ChemBatch &b = a.b;
+  const ChemMech &m = *a.mech;
+  const double rho = b.rho[c], T = b.T[c], dt = b.dtc ? b.dtc[c] : a.dt;
+  double y0[9], dy[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    y0[k] = b.rhoY[(long)k * b.n + c] / rho;
+    dy[k] = dt * ((double)o[k] * a.sc.out_sig[k] + a.sc.out_mu[k]);
+  }
+  const double t_net = T + dt * ((double)o[9] * a.sc.out_sig[9] + a.sc.out_mu[9]);
+  // least-squares projection onto element conservation (rows e >= ne are identity padding), species
+  // driven below zero held at zero
+  unsigned clamp = 0;
+  bool ok = in_range;
+  double yn[9];
+  for (int pass = 0; pass < 4 && ok; ++pass) {
+    double res[4], M[4][4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      double r = 0.0;
+#pragma unroll
+      for (int k = 0; k < 9; ++k) r += a.em.E[e][k] * ((clamp >> k) & 1u ? -y0[k] : dy[k]);
+      res[e] = r;
+#pragma unroll
+      for (int f = 0; f < 4; ++f) {
+        double s = 0.0;
+#pragma unroll
+        for (int k = 0; k < 9; ++k)
+          if (!((clamp >> k) & 1u)) s += a.em.E[e][k] * a.em.E[f][k];
+        M[e][f] = (e >= a.em.ne || f >= a.em.ne) ? (e == f ? 1.0 : 0.0) : s;
+      }
+    }
+    if (!spd_solve4(M, res)) {
+      ok = false;
+      break;
+    }
+    unsigned neg = 0;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      if ((clamp >> k) & 1u) {
+        yn[k] = 0.0;
+      } else {
+        double corr = 0.0;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) corr += a.em.E[e][k] * res[e];
+        yn[k] = y0[k] + (dy[k] - corr);
+        if (yn[k] < 0.0) neg |= 1u << k;
+      }
+    }
+    if (!neg) break;
+    clamp |= neg;
+    if (pass == 3) ok = false;
+  }
+  double t_new = t_net;
+  if (ok) {
+    t_new = chemdev::update_t<9>(m, fmin(fmax(t_net, 200.0), 5000.0), yn, b.e[c]);
+    ok = isfinite(t_new) && fabs(t_new - t_net) <= a.gate.dT_max;
+  }
+  if (ok) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) b.rhoY[(long)k * b.n + c] = rho * yn[k];
+    b.T[c] = t_new;
+    b.steps[c] = 0;
+    b.rejects[c] = 0;
+  } else {
+    const int slot = atomicAdd(a.fb_count, 1);
+    a.fb_list[slot] = (int)c;
+  }
+}
+
+template<bool CHEM>
+__global__ __launch_bounds__(256) void k_mlp_fused(MlpDev net, FusedArgs a) {
+  __shared__ __attribute__((aligned(16))) unsigned short w0[128 * kS0];
+  __shared__ __attribute__((aligned(16))) unsigned short wh[3][128 * kSH];
+  __shared__ __attribute__((aligned(16))) unsigned short w4[32 * kSH];
+  __shared__ __attribute__((aligned(16))) unsigned short act[4][32 * kSH];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  load_rows(w0, kS0, net.wt[0], 128, 16);
+  for (int l = 1; l < 4; ++l) load_rows(wh[l - 1], kSH, net.wt[l], 128, 128);
+  load_rows(w4, kSH, net.wt[4], 32, 128);
+  __syncthreads();
+  unsigned short *h = act[wave];
+  float *hf = reinterpret_cast<float *>(h); // [32][10] outputs, over the wave's activation rows
+  const long n_tiles = (a.n + 127) / 128;
+  for (long tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+    const long c = tile * 128 + wave * 32 + (lane & 31);
+    const bool own = lane < 32 && c < a.n;
+    bool in_range = false;
+    if (lane < 32) {
+      float f[16];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) f[q] = 0.0f;
+      if (own) {
+        if constexpr (CHEM) {
+          in_range = chem_features(a, c, f);
+        } else {
+#pragma unroll
+          for (int q = 0; q < kMlpIn; ++q) f[q] = a.x[c * kMlpIn + q];
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < 16; ++q) h[lane * kSH + q] = f2bf(f[q]);
+    }
+    __syncthreads();
+    floatx16 acc[4];
+    mma_layer<1, 4>(h, w0, kS0, acc, lane);
+    __syncthreads();
+    store_hidden(h, acc, net.bias[0], lane);
+    __syncthreads();
+    for (int l = 0; l < 3; ++l) {
+      mma_layer<8, 4>(h, wh[l], kSH, acc, lane);
+      __syncthreads();
+      store_hidden(h, acc, net.bias[l + 1], lane);
+      __syncthreads();
+    }
+    mma_layer<8, 1>(h, w4, kSH, acc, lane);
+    __syncthreads();
+    {
+      const int j = lane & 31;
+      if (j < kMlpOut) {
+        const float bj = net.bias[4][j];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const int m = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+          hf[m * kMlpOut + j] = acc[0][r] + bj;
+        }
+      }
+    }
+    __syncthreads();
+    if (own) {
+      float o[kMlpOut];
+#pragma unroll
+      for (int q = 0; q < kMlpOut; ++q) o[q] = hf[lane * kMlpOut + q];
+      if constexpr (CHEM) {
+        chem_finish(a, c, o, in_range);
+      } else {
+#pragma unroll
+        for (int q = 0; q < kMlpOut; ++q) a.y[c * kMlpOut + q] = o[q];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void k_zero_count(int *p) { *p = 0; }
+
+inline unsigned grid1(long n, int t) { return (unsigned)((n + t - 1) / t); }
+
+} // namespace
+
+hipError_t launch_gemm(const GemmArgs &a, hipStream_t s) {
+  if (a.M <= 0 || a.N <= 0 || a.K <= 0) return hipSuccess;
+  if (a.k_chunk <= 0 || a.k_chunk % 32) return hipErrorInvalidValue;
+  const int zs = (a.K + a.k_chunk - 1) / a.k_chunk;
+  if (zs > 1 && a.ep != kEpAtomic) return hipErrorInvalidValue;
+  dim3 grid((a.M + 63) / 64, (a.N + 63) / 64, zs);
+  k_gemm<<<grid, 256, 0, s>>>(a);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather(const float *X, const float *Y, long n, unsigned long long seed, long step, int B,
+                         unsigned short *H0, float *Yb, hipStream_t s) {
+  if (B <= 0) return hipSuccess;
+  if (n <= 0) return hipErrorInvalidValue;
+  k_gather<<<grid1(B, 256), 256, 0, s>>>(X, Y, n, seed, step, B, H0, Yb);
+  return hipGetLastError();
+}
+
+hipError_t launch_loss(const float *out, const float *Yb, int B, unsigned short *G, float *loss, hipStream_t s) {
+  if (B <= 0) return hipSuccess;
+  k_loss<<<grid1((long)B * kMlpOut, 256), 256, 0, s>>>(out, Yb, B, G, loss);
+  return hipGetLastError();
+}
+
+hipError_t launch_colsum(const unsigned short *G, int B, int N, long ldg, float *db, hipStream_t s) {
+  if (B <= 0) return hipSuccess;
+  if (N > 128) return hipErrorInvalidValue;
+  k_colsum<<<grid1(B, 256), 256, 0, s>>>(G, B, N, ldg, db);
+  return hipGetLastError();
+}
+
+hipError_t launch_adam(float *p, float *m, float *v, const float *g, long n, float lr, float b1, float b2, float eps,
+                       float c1, float c2, unsigned short *wt, int K, int N, int Kp, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  k_adam<<<grid1(n, 256), 256, 0, s>>>(p, m, v, g, n, lr, b1, b2, eps, c1, c2, wt, K, N, Kp);
+  return hipGetLastError();
+}
+
+hipError_t launch_to_bf16t(const float *W, int K, int N, int Kp, unsigned short *wt, hipStream_t s) {
+  k_to_bf16t<<<grid1((long)K * N, 256), 256, 0, s>>>(W, K, N, Kp, wt);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill(float *p, long n, float v, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  k_fill<<<grid1(n, 256), 256, 0, s>>>(p, n, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_mlp_raw(const MlpDev &net, long n, const float *x, float *y, int blocks, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  FusedArgs a{};
+  a.n = n;
+  a.x = x;
+  a.y = y;
+  const long tiles = (n + 127) / 128;
+  k_mlp_fused<false><<<(unsigned)std::min<long>(tiles, blocks), 256, 0, s>>>(net, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_mlp_chem(const MlpDev &net, const ChemMech *mech_dev, const ChemBatch &b, double dt,
+                           const SurrogateScaling &sc, const SurrogateGate &gate, const ElementMatrix &em,
+                           int *fb_list, int *fb_count, int blocks, hipStream_t s) {
+  k_zero_count<<<1, 1, 0, s>>>(fb_count);
+  if (b.n <= 0) return hipGetLastError();
+  FusedArgs a{};
+  a.n = b.n;
+  a.b = b;
+  a.dt = dt;
+  a.mech = mech_dev;
+  a.sc = sc;
+  a.gate = gate;
+  a.em = em;
+  a.fb_list = fb_list;
+  a.fb_count = fb_count;
+  const long tiles = (b.n + 127) / 128;
+  k_mlp_fused<true><<<(unsigned)std::min<long>(tiles, blocks), 256, 0, s>>>(net, a);
+  return hipGetLastError();
+}
+
+} // namespace corefl

@@ -183,6 +183,50 @@ int corefl_chem_autotune(corefl_chem *chem, double dt, double rtol, double atol,
                          double *ms_out, int n_out);
 int corefl_chem_synchronize(corefl_chem *chem);
 
+/* ---------------------------------------------------------------------------------------------------
+ * Chemistry surrogate (no reference counterpart: a method change).  An MLP 12 -> 4 x 128 (GELU, tanh
+ * form) -> 10 with BF16 weights and FP32 accumulation on the matrix cores maps the features
+ * (T, ln p, Y_1..Y_9, ln dt), each scaled as (f - mu) / sig, to the scaled rates (dY_k/dt, dT/dt).  It is
+ * trained on the GPU (forward, backward and Adam kernels of this library) on states labelled by the exact
+ * integrator, and replaces the exact chemistry sub-step in cells that pass its gate.
+ * ------------------------------------------------------------------------------------------------- */
+typedef struct corefl_surrogate corefl_surrogate;
+
+/* Number of parameters (51968 weights + 522 biases).  Flat layout: W_0 (12 x 128, row-major), b_0 (128),
+ * W_1, b_1, W_2, b_2, W_3 (128 x 128), b_1..b_3 (128), W_4 (128 x 10), b_4 (10), in that order. */
+long long corefl_surrogate_n_params(void);
+/* Context with mini-batch buffers for up to batch_capacity rows. */
+int corefl_surrogate_create(int batch_capacity, corefl_surrogate **out);
+void corefl_surrogate_destroy(corefl_surrogate *s);
+/* FP32 master parameters (also resets the Adam moments and step count). */
+int corefl_surrogate_set_params(corefl_surrogate *s, const float *params, long long n);
+int corefl_surrogate_get_params(corefl_surrogate *s, float *params, long long n);
+/* Feature scaling (12 values each) and target scaling (10 values each). */
+int corefl_surrogate_set_scaling(corefl_surrogate *s, const double *in_mu, const double *in_sig, const double *out_mu,
+                                 const double *out_sig);
+/* Gate: gate[0..7) = T_min, T_max, p_min, p_max, dt_min, dt_max (K, Pa, s) and dT_max (K): a cell uses the
+ * network only inside the box and when the network's temperature change and the one recovered from
+ * energy conservation differ by at most dT_max. */
+int corefl_surrogate_set_gate(corefl_surrogate *s, const double *gate);
+/* Training set on the device: n rows of scaled features X (n x 12) and scaled targets Y (n x 10). */
+int corefl_surrogate_set_data(corefl_surrogate *s, long long n, const float *X, const float *Y);
+/* `steps` Adam steps (beta 0.9 / 0.999, eps 1e-8, learning rate lr) on mini-batches of `batch` rows drawn
+ * with replacement by a hash of (seed, step); mean-squared-error loss.  loss_out[steps] (may be NULL). */
+int corefl_surrogate_train(corefl_surrogate *s, int batch, int steps, float lr, unsigned long long seed,
+                           float *loss_out);
+/* Loss and its gradient (flat layout) on the n given rows (n <= capacity); no parameter update. */
+int corefl_surrogate_grad(corefl_surrogate *s, long long n, const float *X, const float *Y, float *loss, float *grads);
+/* Network outputs (n x 10) of n scaled feature rows: fused = 1 runs the fused inference kernel, 0 the
+ * training kernels (n <= capacity). */
+int corefl_surrogate_forward(corefl_surrogate *s, long long n, const float *X, float *out, int fused);
+/* Surrogate chemistry sub-step of every cell of the batch (dt == 0: per-cell lengths): network rates,
+ * least-squares projection onto element conservation (which also keeps sum Y = 1) with species that would
+ * turn negative held at zero, temperature from the conserved internal energy.  Cells failing the gate are
+ * integrated by the exact ROK4E sub-step (as corefl_chem_integrate) in the same stream; their count goes
+ * to *n_fallback when it is not NULL (this synchronises).  Accepted cells report steps = 0. */
+int corefl_chem_surrogate_step(corefl_chem *chem, corefl_surrogate *s, double dt, double rtol, double atol,
+                               int max_attempts, int *n_fallback, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -137,6 +137,28 @@ double orc_chem_update_t(const orc_mech *m, double t0, const double *yk, double 
 int orc_chem_rok4e(const orc_mech *m, double rho, double E, double *rhoY, double *T, double t_end, double rtol,
                    double atol, double *h, int max_attempts, int *n_rej, double *rec, int n_rec);
 
+/* ---- mixture block (oracle_mix.c): primitive reconstruction + mixture Riemann solvers, RK3,
+ *      Strang-split ROK4E chemistry.  params.reconstruction: 1 first order, 2 MUSCL, 3 NND2 (minmod),
+ *      4 / 5 component-wise WENO5 / WENO7; params.inviscid_scheme: 2 Roe, 3 AUSM+, 4 HLLC. ---- */
+typedef struct orc_mix orc_mix;
+orc_mix *orc_mix_create(const orc_params *p, const orc_mech *m, int mx, int my, int mz, int ngg, const int *faces,
+                        int n_face, const double *jac, const double *metric);
+void orc_mix_destroy(orc_mix *b);
+/* bv (6*sz) and sv (ns*sz) on every node; cv from them, boundary conditions, acoustic speed. */
+void orc_mix_set_state(orc_mix *b, const double *bv, const double *sv);
+void orc_mix_set_raw(orc_mix *b, const double *bv, const double *sv, const double *cv);
+/* which: 0 cv ((5+ns)*sz), 1 bv (6*sz), 2 dq ((5+ns)*n), 3 sv (ns*sz), 4 acoustic speed (sz) */
+void orc_mix_get(const orc_mix *b, int which, double *out);
+void orc_mix_rhs(orc_mix *b);
+double orc_mix_cfl_dt(orc_mix *b);
+void orc_mix_rk3_step(orc_mix *b, double dt);
+int orc_mix_chem_substep(orc_mix *b, double dt, double rtol, double atol, int max_attempts);
+int orc_mix_strang_step(orc_mix *b, double dt, double rtol, double atol, int max_attempts);
+/* One interface flux of the mixture Riemann solver params->inviscid_scheme from primitive states
+ * (rho,u,v,w,p,Y_1..Y_ns), after the reference post-processing for E and gamma. */
+void orc_mix_point_flux(const orc_params *p, const orc_mech *m, const double *pl, const double *pr,
+                        const double *ml, const double *mr, double jl, double jr, double efl, double efr, double *fc);
+
 /* ---- exact solution ---- */
 
 /* Exact Riemann solution of the 1D Euler equations (Toro, "Riemann Solvers and Numerical

@@ -1,0 +1,740 @@
+/*
+ * oracle_mix.c -- multi-species (mixture) inviscid right-hand side, SSP-RK3 step and Strang-split
+ * chemistry of one block (TEST INFRASTRUCTURE ONLY; see oracle.h).
+ *
+ * Primitive-variable path of the reference (compute_convective_term_pv / Roe_compute_inviscid_flux,
+ * InviscidScheme.cu:28-219, 221-347): reconstruction of (rho, u, v, w, p, Y_k) by first order, MUSCL or
+ * NND2 (Reconstruction.cu:5-44, minmod Limiter.cuh:8-41) or, as the new pairing, component-wise WENO5-JS /
+ * WENO7-Z one-sided halves (WENO.cu:3059-3222 with a zero opposite stencil), then the reference
+ * post-processing for E and gamma (InviscidScheme.cu:184-218) and the mixture Riemann solvers (Roe
+ * 635-765 with the entropy fix 269-304, AUSM+ 397-469, HLLC 471-599).
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+static const double kRu = 8314.462618;
+
+#define MS ORC_MAX_SPEC
+#define NV (5 + MS)
+
+typedef struct {
+  int face, dir, type;
+  int s[3], e[3];
+} mface_t;
+
+struct orc_mix {
+  orc_params p;
+  orc_mech m;
+  int ns, nv, mx, my, mz, g, dim;
+  long d2, d1, sz, n;
+  mface_t faces[64];
+  int n_face;
+  double *cv, *qn, *bv, *sv, *c, *efd, *dq, *jac, *met, *hstep;
+};
+
+#define ID(b, i, j, k) ((long)((k) + (b)->g) * (b)->d1 + (long)((j) + (b)->g) * (b)->d2 + ((i) + (b)->g))
+#define IDQ(b, i, j, k) (((long)(k) * (b)->my + (j)) * (b)->mx + (i))
+
+orc_mix *orc_mix_create(const orc_params *p, const orc_mech *m, int mx, int my, int mz, int ngg, const int *faces,
+                        int n_face, const double *jac, const double *metric) {
+  orc_mix *b = (orc_mix *)calloc(1, sizeof(orc_mix));
+  b->p = *p;
+  b->m = *m;
+  b->ns = m->ns;
+  b->nv = 5 + m->ns;
+  b->mx = mx; b->my = my; b->mz = mz; b->g = ngg;
+  b->dim = mz == 1 ? 2 : 3;
+  b->d2 = mx + 2 * ngg;
+  b->d1 = (long)(my + 2 * ngg) * b->d2;
+  b->sz = b->d1 * (mz + 2 * ngg);
+  b->n = (long)mx * my * mz;
+  b->n_face = n_face;
+  for (int f = 0; f < n_face; ++f) {
+    const int *r = faces + 7 * f;
+    mface_t *F = &b->faces[f];
+    F->s[0] = r[0]; F->e[0] = r[1];
+    F->s[1] = r[2]; F->e[1] = r[3];
+    F->s[2] = r[4]; F->e[2] = r[5];
+    F->type = r[6];
+    F->face = 0;
+    F->dir = 1;
+    for (int d = 0; d < b->dim; ++d)
+      if (F->s[d] == F->e[d]) {
+        F->face = d;
+        if (F->s[d] == 0) F->dir = -1;
+        break;
+      }
+    for (int d = 0; d < b->dim; ++d) {
+      if (d == F->face) continue;
+      if (F->e[d] < F->s[d]) { const int t = F->e[d]; F->e[d] = F->s[d]; F->s[d] = t; }
+    }
+    if (b->dim == 2) { F->s[2] = 0; F->e[2] = 0; }
+  }
+  b->cv = calloc((size_t)b->nv * b->sz, sizeof(double));
+  b->qn = calloc((size_t)b->nv * b->sz, sizeof(double));
+  b->bv = calloc((size_t)6 * b->sz, sizeof(double));
+  b->sv = calloc((size_t)b->ns * b->sz, sizeof(double));
+  b->c = calloc((size_t)b->sz, sizeof(double));
+  b->efd = calloc((size_t)b->sz, sizeof(double));
+  b->dq = calloc((size_t)b->nv * b->n, sizeof(double));
+  b->hstep = calloc((size_t)b->n, sizeof(double));
+  b->jac = malloc((size_t)b->sz * sizeof(double));
+  b->met = malloc((size_t)9 * b->sz * sizeof(double));
+  memcpy(b->jac, jac, (size_t)b->sz * sizeof(double));
+  memcpy(b->met, metric, (size_t)9 * b->sz * sizeof(double));
+  return b;
+}
+
+void orc_mix_destroy(orc_mix *b) {
+  if (!b) return;
+  free(b->cv); free(b->qn); free(b->bv); free(b->sv); free(b->c); free(b->efd); free(b->dq); free(b->hstep);
+  free(b->jac); free(b->met);
+  free(b);
+}
+
+/* compute_cv_from_bv_1_point<Mixture> + compute_total_energy (FieldOperation.cuh:16-84) */
+static void cv_from_bv(orc_mix *b, long id) {
+  const long s = b->sz;
+  const double rho = b->bv[id], u = b->bv[id + s], v = b->bv[id + 2 * s], w = b->bv[id + 3 * s];
+  b->cv[id] = rho;
+  b->cv[id + s] = rho * u;
+  b->cv[id + 2 * s] = rho * v;
+  b->cv[id + 3 * s] = rho * w;
+  for (int l = 0; l < b->ns; ++l) b->cv[id + (5 + l) * s] = rho * b->sv[id + l * s];
+  double h[MS], cp[MS];
+  orc_chem_h_cp(&b->m, b->bv[id + 5 * s], h, cp);
+  double E = 0.5 * (u * u + v * v + w * w);
+  for (int l = 0; l < b->ns; ++l) E += h[l] * b->sv[id + l * s];
+  E *= rho;
+  E -= b->bv[id + 4 * s];
+  b->cv[id + 4 * s] = E;
+}
+
+/* update_physical_properties<Mixture> (FieldOperation.cuh:86-120): the acoustic speed */
+static void properties(orc_mix *b, long id) {
+  const long s = b->sz;
+  double h[MS], cp[MS];
+  const double T = b->bv[id + 5 * s];
+  orc_chem_h_cp(&b->m, T, h, cp);
+  double imw = 0, cpt = 0;
+  for (int l = 0; l < b->ns; ++l) {
+    const double y = b->sv[id + l * s];
+    imw += y / b->m.mw[l];
+    cpt += y * cp[l];
+  }
+  const double R = kRu * imw;
+  const double gam = cpt / (cpt - R);
+  b->c[id] = sqrt(gam * R * T);
+}
+
+static void all_properties(orc_mix *b) {
+#pragma omp parallel for
+  for (long id = 0; id < b->sz; ++id) properties(b, id);
+}
+
+static void copy_node(orc_mix *b, long dst, long src) {
+  for (int l = 0; l < 6; ++l) b->bv[dst + l * b->sz] = b->bv[src + l * b->sz];
+  for (int l = 0; l < b->ns; ++l) b->sv[dst + l * b->sz] = b->sv[src + l * b->sz];
+}
+
+/* apply_outflow (BoundCond.cu:2065-2188) with the species */
+static void apply_outflow(orc_mix *b, const mface_t *F) {
+  int d[3] = {0, 0, 0};
+  d[F->face] = F->dir;
+  for (int k = F->s[2]; k <= F->e[2]; ++k)
+    for (int j = F->s[1]; j <= F->e[1]; ++j)
+      for (int i = F->s[0]; i <= F->e[0]; ++i) {
+        const long id = ID(b, i, j, k);
+        for (int g = 1; g <= b->g; ++g) {
+          const long ig = ID(b, i + g * d[0], j + g * d[1], k + g * d[2]);
+          copy_node(b, ig, id);
+          cv_from_bv(b, ig);
+        }
+      }
+}
+
+/* apply_periodic (BoundCond.cu:3033-3083) with the species */
+static void apply_periodic(orc_mix *b, const mface_t *F) {
+  int d[3] = {0, 0, 0};
+  d[F->face] = F->dir;
+  const int ext[3] = {b->mx, b->my, b->mz};
+  for (int k = F->s[2]; k <= F->e[2]; ++k)
+    for (int j = F->s[1]; j <= F->e[1]; ++j)
+      for (int i = F->s[0]; i <= F->e[0]; ++i) {
+        int o[3] = {i, j, k};
+        o[F->face] = F->dir < 0 ? ext[F->face] - 1 : 0;
+        const long id = ID(b, i, j, k), io = ID(b, o[0], o[1], o[2]);
+        for (int l = 0; l < 6; ++l) {
+          const double a = 0.5 * (b->bv[id + l * b->sz] + b->bv[io + l * b->sz]);
+          b->bv[id + l * b->sz] = a;
+          b->bv[io + l * b->sz] = a;
+        }
+        for (int l = 0; l < b->ns; ++l) {
+          const double a = 0.5 * (b->sv[id + l * b->sz] + b->sv[io + l * b->sz]);
+          b->sv[id + l * b->sz] = a;
+          b->sv[io + l * b->sz] = a;
+        }
+        cv_from_bv(b, id);
+        for (int g = 1; g <= b->g; ++g) {
+          const long ig = ID(b, i + g * d[0], j + g * d[1], k + g * d[2]);
+          const long is = ID(b, o[0] + g * d[0], o[1] + g * d[1], o[2] + g * d[2]);
+          copy_node(b, ig, is);
+          cv_from_bv(b, ig);
+        }
+      }
+}
+
+static void apply_bcs(orc_mix *b) {
+  for (int f = 0; f < b->n_face; ++f)
+    if (b->faces[f].type == 6) apply_outflow(b, &b->faces[f]);
+  for (int f = 0; f < b->n_face; ++f)
+    if (b->faces[f].type == 8) apply_periodic(b, &b->faces[f]);
+}
+
+void orc_mix_set_state(orc_mix *b, const double *bv, const double *sv) {
+  memcpy(b->bv, bv, sizeof(double) * 6 * b->sz);
+  memcpy(b->sv, sv, sizeof(double) * b->ns * b->sz);
+#pragma omp parallel for
+  for (long id = 0; id < b->sz; ++id) cv_from_bv(b, id);
+  apply_bcs(b);
+  all_properties(b);
+}
+
+void orc_mix_set_raw(orc_mix *b, const double *bv, const double *sv, const double *cv) {
+  memcpy(b->bv, bv, sizeof(double) * 6 * b->sz);
+  memcpy(b->sv, sv, sizeof(double) * b->ns * b->sz);
+  memcpy(b->cv, cv, sizeof(double) * b->nv * b->sz);
+  all_properties(b);
+}
+
+void orc_mix_get(const orc_mix *b, int which, double *out) {
+  switch (which) {
+    case 0: memcpy(out, b->cv, sizeof(double) * b->nv * b->sz); break;
+    case 1: memcpy(out, b->bv, sizeof(double) * 6 * b->sz); break;
+    case 2: memcpy(out, b->dq, sizeof(double) * b->nv * b->n); break;
+    case 3: memcpy(out, b->sv, sizeof(double) * b->ns * b->sz); break;
+    case 4: memcpy(out, b->c, sizeof(double) * b->sz); break;
+    default: break;
+  }
+}
+
+/* compute_entropy_fix_delta<Mixture> (InviscidScheme.cu:269-304) */
+static void entropy_fix(orc_mix *b) {
+  const long s = b->sz;
+  const int kl = b->dim == 3 ? -1 : 0, kh = b->dim == 3 ? b->mz + 1 : 1;
+#pragma omp parallel for collapse(2)
+  for (int k = kl; k < kh; ++k)
+    for (int j = -1; j < b->my + 1; ++j)
+      for (int i = -1; i < b->mx + 1; ++i) {
+        const long id = ID(b, i, j, k);
+        const double *M = b->met, *bv = b->bv;
+        const double u = bv[id + s], v = bv[id + 2 * s], w = bv[id + 3 * s];
+        const double U = fabs(u * M[id] + v * M[id + s] + w * M[id + 2 * s]);
+        const double V = fabs(u * M[id + 3 * s] + v * M[id + 4 * s] + w * M[id + 5 * s]);
+        const double W = fabs(u * M[id + 6 * s] + v * M[id + 7 * s] + w * M[id + 8 * s]);
+        const double kx = sqrt(M[id] * M[id] + M[id + s] * M[id + s] + M[id + 2 * s] * M[id + 2 * s]);
+        const double ky = sqrt(M[id + 3 * s] * M[id + 3 * s] + M[id + 4 * s] * M[id + 4 * s] +
+                               M[id + 5 * s] * M[id + 5 * s]);
+        const double kz = sqrt(M[id + 6 * s] * M[id + 6 * s] + M[id + 7 * s] * M[id + 7 * s] +
+                               M[id + 8 * s] * M[id + 8 * s]);
+        const double c = b->c[id];
+        if (b->dim == 2)
+          b->efd[id] = b->p.entropy_fix_factor * (U + V + c * 0.5 * (kx + ky));
+        else
+          b->efd[id] = b->p.entropy_fix_factor * (U + V + W + c * (kx + ky + kz) / 3.0);
+      }
+}
+
+static double minmod(double a, double c) {
+  const double sa = (a > 0) - (a < 0), sc = (c > 0) - (c < 0);
+  return 0.5 * (sa + sc) * fmin(fabs(a), fabs(c));
+}
+
+/* Left/right primitive states at the interface between stencil points r-1 and r of pv[2r][nv]. */
+static void reconstruct(const orc_mix *b, double (*pv)[NV], int r, double *pl, double *pr) {
+  const int nv = b->nv, rec = b->p.reconstruction;
+  const int c = r - 1; /* left node */
+  for (int l = 0; l < nv; ++l) {
+    if (rec == 1) {
+      pl[l] = pv[c][l];
+      pr[l] = pv[c + 1][l];
+    } else if (rec == 2 || rec == 3) {
+      const double d0 = pv[c][l] - pv[c - 1][l], d1 = pv[c + 1][l] - pv[c][l], d2 = pv[c + 2][l] - pv[c + 1][l];
+      if (rec == 2) {
+        const double kappa = 1.0 / 3.0;
+        pl[l] = pv[c][l] + 0.25 * ((1 - kappa) * minmod(d0, d1) + (1 + kappa) * minmod(d1, d0));
+        pr[l] = pv[c + 1][l] - 0.25 * ((1 - kappa) * minmod(d2, d1) + (1 + kappa) * minmod(d1, d2));
+      } else {
+        pl[l] = pv[c][l] + 0.5 * minmod(d0, d1);
+        pr[l] = pv[c + 1][l] - 0.5 * minmod(d1, d2);
+      }
+    } else {
+      double s[8], zero[8] = {0};
+      for (int q = 0; q < 2 * r; ++q) s[q] = pv[q][l];
+      if (rec == 4) {
+        pl[l] = orc_weno5(s, zero, b->p.weno_eps, 1);
+        pr[l] = orc_weno5(zero, s + 1, b->p.weno_eps, 1);
+      } else {
+        pl[l] = orc_weno7(s, zero, b->p.weno_eps, 1);
+        pr[l] = orc_weno7(zero, s + 1, b->p.weno_eps, 1);
+      }
+    }
+  }
+  if (rec >= 4 && (pl[0] <= 0 || pl[4] <= 0 || pr[0] <= 0 || pr[4] <= 0)) {
+    for (int l = 0; l < nv; ++l) {
+      pl[l] = pv[c][l];
+      pr[l] = pv[c + 1][l];
+    }
+  }
+}
+
+/* reconstruction<Mixture> post-processing (InviscidScheme.cu:184-210): E at [nv], gamma at [nv+1] */
+static void post(const orc_mix *b, double *pv) {
+  const int ns = b->ns, nv = b->nv;
+  double e = 0.5 * (pv[1] * pv[1] + pv[2] * pv[2] + pv[3] * pv[3]);
+  double imw = 0;
+  for (int l = 0; l < ns; ++l) imw += pv[5 + l] / b->m.mw[l];
+  const double t = pv[4] / (pv[0] * kRu * imw);
+  double h[MS], cp[MS];
+  orc_chem_h_cp(&b->m, t, h, cp);
+  double cpt = 0, cvt = 0;
+  for (int l = 0; l < ns; ++l) {
+    cpt += cp[l] * pv[5 + l];
+    cvt += pv[5 + l] * (cp[l] - kRu / b->m.mw[l]);
+    e += h[l] * pv[5 + l];
+  }
+  pv[nv] = pv[0] * e - pv[4];
+  pv[nv + 1] = cpt / cvt;
+}
+
+/* compute_half_sum_left_right_flux (InviscidScheme.cu:601-633) + riemannSolver_Roe<Mixture> (635-765) */
+static void roe(const orc_mix *b, const double *pl, const double *pr, const double *ml, const double *mr, double jl,
+                double jr, double efl, double efr, double *fc) {
+  const int ns = b->ns, nv = b->nv;
+  double JKx = jl * ml[0], JKy = jl * ml[1], JKz = jl * ml[2];
+  double Uk = pl[1] * JKx + pl[2] * JKy + pl[3] * JKz;
+  double co = Uk * pl[0];
+  fc[0] = 0.5 * co;
+  fc[1] = 0.5 * (co * pl[1] + pl[4] * JKx);
+  fc[2] = 0.5 * (co * pl[2] + pl[4] * JKy);
+  fc[3] = 0.5 * (co * pl[3] + pl[4] * JKz);
+  fc[4] = 0.5 * Uk * (pl[4] + pl[nv]);
+  for (int l = 5; l < nv; ++l) fc[l] = 0.5 * co * pl[l];
+  JKx = jr * mr[0]; JKy = jr * mr[1]; JKz = jr * mr[2];
+  Uk = pr[1] * JKx + pr[2] * JKy + pr[3] * JKz;
+  co = Uk * pr[0];
+  fc[0] += 0.5 * co;
+  fc[1] += 0.5 * (co * pr[1] + pr[4] * JKx);
+  fc[2] += 0.5 * (co * pr[2] + pr[4] * JKy);
+  fc[3] += 0.5 * (co * pr[3] + pr[4] * JKz);
+  fc[4] += 0.5 * Uk * (pr[4] + pr[nv]);
+  for (int l = 5; l < nv; ++l) fc[l] += 0.5 * co * pr[l];
+
+  const double dl = sqrt(pl[0]), dr = sqrt(pr[0]);
+  const double inv = 1.0 / (dl + dr);
+  const double u = (dl * pl[1] + dr * pr[1]) * inv;
+  const double v = (dl * pl[2] + dr * pr[2]) * inv;
+  const double w = (dl * pl[3] + dr * pr[3]) * inv;
+  const double ek = 0.5 * (u * u + v * v + w * w);
+  const double hl = (pl[nv] + pl[4]) / pl[0], hr = (pr[nv] + pr[4]) / pr[0];
+  const double h = (dl * hl + dr * hr) * inv;
+  double svm[MS];
+  for (int l = 0; l < ns; ++l) svm[l] = (dl * pl[l + 5] + dr * pr[l + 5]) * inv;
+  double mw_inv = 0;
+  for (int l = 0; l < ns; ++l) mw_inv += svm[l] / b->m.mw[l];
+  const double tl = pl[4] / pl[0], tr = pr[4] / pr[0];
+  const double t = (dl * tl + dr * tr) * inv / (kRu * mw_inv);
+  double hi[MS], cpi[MS];
+  orc_chem_h_cp(&b->m, t, hi, cpi);
+  double cvt = 0, cpt = 0;
+  for (int l = 0; l < ns; ++l) {
+    cpt += cpi[l] * svm[l];
+    cvt += svm[l] * (cpi[l] - kRu / b->m.mw[l]);
+  }
+  const double gamma = cpt / cvt;
+  const double c = sqrt(gamma * kRu * mw_inv * t);
+  const double mw = 1.0 / mw_inv;
+
+  double kx = 0.5 * (ml[0] + mr[0]), ky = 0.5 * (ml[1] + mr[1]), kz = 0.5 * (ml[2] + mr[2]);
+  const double gradK = sqrt(kx * kx + ky * ky + kz * kz);
+  double U = kx * u + ky * v + kz * w;
+  double ch[3] = {U - gradK * c, U, U + gradK * c};
+  const double efa = 0.5 * (efl + efr);
+  for (int q = 0; q < 3; ++q) {
+    ch[q] = fabs(ch[q]);
+    if (ch[q] < efa) ch[q] = 0.5 * (ch[q] * ch[q] / efa + efa);
+  }
+  kx /= gradK;
+  ky /= gradK;
+  kz /= gradK;
+  U /= gradK;
+  const double jave = 0.5 * (jl + jr);
+  double dq[NV];
+  dq[0] = jave * (pr[0] - pl[0]);
+  for (int l = 1; l < nv; ++l) dq[l] = jave * (pr[0] * pr[l] - pl[0] * pl[l]);
+  dq[4] = jave * (pr[nv] - pl[nv]);
+  double c1 = (gamma - 1) * (ek * dq[0] - u * dq[1] - v * dq[2] - w * dq[3] + dq[4]) / (c * c);
+  double c2 = (kx * dq[1] + ky * dq[2] + kz * dq[3] - U * dq[0]) / c;
+  for (int l = 0; l < ns; ++l) c1 += (mw / b->m.mw[l] - hi[l] * (gamma - 1) / (c * c)) * dq[l + 5];
+  double c3 = dq[0] - c1;
+  double LDq[NV];
+  LDq[0] = 0.5 * (c1 - c2);
+  LDq[1] = kx * c3 - ((kz * v - ky * w) * dq[0] - kz * dq[2] + ky * dq[3]) / c;
+  LDq[2] = ky * c3 - ((kx * w - kz * u) * dq[0] - kx * dq[3] + kz * dq[1]) / c;
+  LDq[3] = kz * c3 - ((ky * u - kx * v) * dq[0] - ky * dq[1] + kx * dq[2]) / c;
+  LDq[4] = 0.5 * (c1 + c2);
+  for (int l = 0; l < ns; ++l) LDq[l + 5] = dq[l + 5] - svm[l] * dq[0];
+  double bb[NV];
+  bb[0] = -ch[0] * LDq[0];
+  for (int l = 1; l < nv; ++l) bb[l] = -ch[1] * LDq[l];
+  bb[4] = -ch[2] * LDq[4];
+  const double c0 = kx * bb[1] + ky * bb[2] + kz * bb[3];
+  c1 = c0 + bb[0] + bb[4];
+  c2 = c * (bb[4] - bb[0]);
+  c3 = 0;
+  for (int l = 0; l < ns; ++l) c3 += (hi[l] - mw / b->m.mw[l] * c * c / (gamma - 1)) * bb[l + 5];
+  fc[0] += 0.5 * c1;
+  fc[1] += 0.5 * (u * c1 + kx * c2 - c * (kz * bb[2] - ky * bb[3]));
+  fc[2] += 0.5 * (v * c1 + ky * c2 - c * (kx * bb[3] - kz * bb[1]));
+  fc[3] += 0.5 * (w * c1 + kz * c2 - c * (ky * bb[1] - kx * bb[2]));
+  fc[4] += 0.5 * (h * c1 + U * c2 - c * c * c0 / (gamma - 1) +
+                  c * ((kz * v - ky * w) * bb[1] + (kx * w - kz * u) * bb[2] + (ky * u - kx * v) * bb[3]) + c3);
+  for (int l = 0; l < ns; ++l) fc[5 + l] += 0.5 * (bb[l + 5] + svm[l] * c1);
+}
+
+/* riemannSolver_ausmPlus<Mixture> (InviscidScheme.cu:397-469) */
+static void ausm(const orc_mix *b, const double *pl, const double *pr, const double *ml, const double *mr, double jl,
+                 double jr, double *fc) {
+  const int nv = b->nv;
+  const double k1 = 0.5 * (jl * ml[0] + jr * mr[0]), k2 = 0.5 * (jl * ml[1] + jr * mr[1]),
+               k3 = 0.5 * (jl * ml[2] + jr * mr[2]);
+  const double gk = sqrt(k1 * k1 + k2 * k2 + k3 * k3);
+  const double ul = (k1 * pl[1] + k2 * pl[2] + k3 * pl[3]) / gk, ur = (k1 * pr[1] + k2 * pr[2] + k3 * pr[3]) / gk;
+  const double c = 0.5 * (sqrt(pl[nv + 1] * pl[4] / pl[0]) + sqrt(pr[nv + 1] * pr[4] / pr[0]));
+  const double Ml = ul / c, Mr = ur / c, alpha = 3.0 / 16.0;
+  double mlp, mrn, plp, prn;
+  if (fabs(Ml) > 1) {
+    mlp = 0.5 * (Ml + fabs(Ml));
+    plp = mlp / Ml;
+  } else {
+    const double a = (Ml + 1) * (Ml + 1) * 0.25, q = (Ml * Ml - 1) * (Ml * Ml - 1);
+    mlp = a + 0.125 * q;
+    plp = a * (2 - Ml) + alpha * Ml * q;
+  }
+  if (fabs(Mr) > 1) {
+    mrn = 0.5 * (Mr - fabs(Mr));
+    prn = mrn / Mr;
+  } else {
+    const double a = (Mr - 1) * (Mr - 1) * 0.25, q = (Mr * Mr - 1) * (Mr * Mr - 1);
+    mrn = -a - 0.125 * q;
+    prn = a * (2 + Mr) - alpha * Mr * q;
+  }
+  const double pc = plp * pl[4] + prn * pr[4];
+  const double mh = mlp + mrn;
+  const double mp = 0.5 * (mh + fabs(mh)), mn = 0.5 * (mh - fabs(mh));
+  const double mf = c * (pl[0] * mp + pr[0] * mn);
+  const double co = mf * gk;
+  const double *s = mf >= 0 ? pl : pr;
+  fc[0] = co;
+  fc[1] = co * s[1] + pc * k1;
+  fc[2] = co * s[2] + pc * k2;
+  fc[3] = co * s[3] + pc * k3;
+  fc[4] = co * (s[nv] + s[4]) / s[0];
+  for (int l = 5; l < nv; ++l) fc[l] = co * s[l];
+}
+
+/* riemannSolver_hllc<Mixture> (InviscidScheme.cu:471-599) */
+static void hllc(const orc_mix *b, const double *pl, const double *pr, const double *ml, const double *mr, double jl,
+                 double jr, double *fc) {
+  const int ns = b->ns, nv = b->nv;
+  const double rl = sqrt(pl[0]) / (sqrt(pl[0]) + sqrt(pr[0])), rr = sqrt(pr[0]) / (sqrt(pl[0]) + sqrt(pr[0]));
+  const double ut = pl[1] * rl + pr[1] * rr, vt = pl[2] * rl + pr[2] * rr, wt = pl[3] * rl + pr[3] * rr;
+  double svm[MS], mw_inv = 0;
+  for (int l = 0; l < ns; ++l) {
+    svm[l] = rl * pl[l + 5] + rr * pr[l + 5];
+    mw_inv += svm[l] / b->m.mw[l];
+  }
+  const double t = (rl * pl[4] / pl[0] + rr * pr[4] / pr[0]) / (kRu * mw_inv);
+  double hi[MS], cpi[MS];
+  orc_chem_h_cp(&b->m, t, hi, cpi);
+  double cvt = 0, cpt = 0;
+  for (int l = 0; l < ns; ++l) {
+    cpt += cpi[l] * svm[l];
+    cvt += svm[l] * (cpi[l] - kRu / b->m.mw[l]);
+  }
+  const double ct = sqrt(cpt / cvt * kRu * mw_inv * t);
+  const double kx = 0.5 * (ml[0] + mr[0]), ky = 0.5 * (ml[1] + mr[1]), kz = 0.5 * (ml[2] + mr[2]);
+  const double gk = sqrt(kx * kx + ky * ky + kz * kz);
+  const double Ub = (kx * ut + ky * vt + kz * wt) / gk;
+  const double ja = 0.5 * (jl + jr);
+  const double Ul = kx * pl[1] + ky * pl[2] + kz * pl[3];
+  const double cl = sqrt(pl[nv + 1] * pl[4] / pl[0]);
+  const double sl = fmin(Ul / gk - cl, Ub - ct);
+  if (sl >= 0) {
+    const double rU = pl[0] * Ul;
+    fc[0] = ja * rU;
+    fc[1] = ja * (rU * pl[1] + pl[4] * kx);
+    fc[2] = ja * (rU * pl[2] + pl[4] * ky);
+    fc[3] = ja * (rU * pl[3] + pl[4] * kz);
+    fc[4] = ja * ((pl[nv] + pl[4]) * Ul);
+    for (int l = 5; l < nv; ++l) fc[l] = ja * rU * pl[l];
+    return;
+  }
+  const double Ur = kx * pr[1] + ky * pr[2] + kz * pr[3];
+  const double cr = sqrt(pr[nv + 1] * pr[4] / pr[0]);
+  const double sr = fmax(Ur / gk + cr, Ub + ct);
+  if (sr < 0) {
+    const double rU = pr[0] * Ur;
+    fc[0] = ja * rU;
+    fc[1] = ja * (rU * pr[1] + pr[4] * kx);
+    fc[2] = ja * (rU * pr[2] + pr[4] * ky);
+    fc[3] = ja * (rU * pr[3] + pr[4] * kz);
+    fc[4] = ja * ((pr[nv] + pr[4]) * Ur);
+    for (int l = 5; l < nv; ++l) fc[l] = ja * rU * pr[l];
+    return;
+  }
+  const double sm = ((pr[0] * Ur * (sr - Ur / gk) - pl[0] * Ul * (sl - Ul / gk)) / gk + pl[4] - pr[4]) /
+                    (pr[0] * (sr - Ur / gk) - pl[0] * (sl - Ul / gk));
+  const double pm = pl[0] * (sl - Ul / gk) * (sm - Ul / gk) + pl[4];
+  const int left = sm >= 0;
+  const double *q = left ? pl : pr;
+  const double S = left ? sl : sr, Uq = left ? Ul : Ur;
+  const double pC = 1.0 / (S - sm);
+  const double QC = ja * pC * sm * (S * gk - Uq) * q[0];
+  fc[0] = QC;
+  const double dP = (S * pm - sm * q[4]) * pC * ja;
+  fc[1] = QC * q[1] + dP * kx;
+  fc[2] = QC * q[2] + dP * ky;
+  fc[3] = QC * q[3] + dP * kz;
+  fc[4] = QC * q[nv] / q[0] + pC * ja * (S * pm * sm * gk - sm * q[4] * Uq);
+  for (int l = 5; l < nv; ++l) fc[l] = QC * q[l];
+}
+
+static void inviscid_dir(orc_mix *b, int dir) {
+  const int rec = b->p.reconstruction;
+  const int r = rec == 5 ? 4 : (rec == 4 ? 3 : 2); /* stencil half width */
+  const int n = 2 * r, nv = b->nv, ns = b->ns;
+  const long s = b->sz;
+  const long st = dir == 0 ? 1 : (dir == 1 ? b->d2 : b->d1);
+  const int ext[3] = {b->mx, b->my, b->mz};
+  const int m = ext[dir];
+  const int oa = dir == 0 ? 1 : 0, ob = dir == 2 ? 1 : 2;
+#pragma omp parallel for collapse(2)
+  for (int B = 0; B < ext[ob]; ++B)
+    for (int A = 0; A < ext[oa]; ++A) {
+      double *Fl = malloc(sizeof(double) * nv * (m + 1));
+      for (int t = -1; t < m; ++t) {
+        int idx[3];
+        idx[dir] = t;
+        idx[oa] = A;
+        idx[ob] = B;
+        const long id = ID(b, idx[0], idx[1], idx[2]);
+        double pv[8][NV];
+        for (int q = 0; q < n; ++q) {
+          const long iq = id + (q - r + 1) * st;
+          for (int l = 0; l < 5; ++l) pv[q][l] = b->bv[iq + l * s];
+          for (int l = 0; l < ns; ++l) pv[q][5 + l] = b->sv[iq + l * s];
+        }
+        double pl[NV + 2], pr[NV + 2];
+        reconstruct(b, pv, r, pl, pr);
+        post(b, pl);
+        post(b, pr);
+        double ml[3], mr[3];
+        for (int c = 0; c < 3; ++c) {
+          ml[c] = b->met[id + (3 * dir + c) * s];
+          mr[c] = b->met[id + st + (3 * dir + c) * s];
+        }
+        double *fc = Fl + (long)nv * (t + 1);
+        if (b->p.inviscid_scheme == 2)
+          roe(b, pl, pr, ml, mr, b->jac[id], b->jac[id + st], b->efd[id], b->efd[id + st], fc);
+        else if (b->p.inviscid_scheme == 4)
+          hllc(b, pl, pr, ml, mr, b->jac[id], b->jac[id + st], fc);
+        else
+          ausm(b, pl, pr, ml, mr, b->jac[id], b->jac[id + st], fc);
+      }
+      for (int t = 0; t < m; ++t) {
+        int idx[3];
+        idx[dir] = t;
+        idx[oa] = A;
+        idx[ob] = B;
+        const long iq = IDQ(b, idx[0], idx[1], idx[2]);
+        for (int l = 0; l < nv; ++l) b->dq[iq + l * b->n] -= Fl[(long)nv * (t + 1) + l] - Fl[(long)nv * t + l];
+      }
+      free(Fl);
+    }
+}
+
+void orc_mix_rhs(orc_mix *b) {
+  memset(b->dq, 0, sizeof(double) * b->nv * b->n);
+  if (b->p.inviscid_scheme == 2) entropy_fix(b);
+  inviscid_dir(b, 0);
+  inviscid_dir(b, 1);
+  if (b->dim == 3) inviscid_dir(b, 2);
+}
+
+static const double rk_a[3] = {1.0, 0.75, 1.0 / 3.0};
+static const double rk_b[3] = {0.0, 0.25, 2.0 / 3.0};
+static const double rk_c[3] = {1.0, 0.25, 2.0 / 3.0};
+
+/* compute_temperature_and_pressure (FieldOperation.cu:4-45) at node id from the species in sv */
+static void temperature_pressure(orc_mix *b, long id) {
+  const long s = b->sz;
+  double R = 0;
+  for (int l = 0; l < b->ns; ++l) R += b->sv[id + l * s] * kRu / b->m.mw[l];
+  const double u = b->bv[id + s], v = b->bv[id + 2 * s], w = b->bv[id + 3 * s];
+  const double e = b->cv[id + 4 * s] / b->bv[id] - 0.5 * (u * u + v * v + w * w);
+  double err = 1, t = b->bv[id + 5 * s];
+  int it = 0;
+  while (err > 1e-5 && it++ < 1000) {
+    double h[MS], cp[MS];
+    orc_chem_h_cp(&b->m, t, h, cp);
+    double cpt = 0, hh = 0;
+    for (int l = 0; l < b->ns; ++l) {
+      cpt += cp[l] * b->sv[id + l * s];
+      hh += h[l] * b->sv[id + l * s];
+    }
+    const double t1 = t - (hh - R * t - e) / (cpt - R);
+    err = fabs(1 - t1 / t);
+    t = t1;
+  }
+  b->bv[id + 5 * s] = t;
+  b->bv[id + 4 * s] = b->bv[id] * t * R;
+}
+
+/* update_cv_and_bv_rk<Mixture> (RK.cuh:224-280) */
+static void update(orc_mix *b, double dt, int rk) {
+  const long s = b->sz;
+  const int nv = b->nv, ns = b->ns;
+#pragma omp parallel for collapse(2)
+  for (int k = 0; k < b->mz; ++k)
+    for (int j = 0; j < b->my; ++j)
+      for (int i = 0; i < b->mx; ++i) {
+        const long id = ID(b, i, j, k), iq = IDQ(b, i, j, k);
+        double *cv = b->cv, *bv = b->bv;
+        const double dj = dt / b->jac[id];
+        for (int l = 0; l < nv; ++l)
+          cv[id + l * s] = rk_a[rk] * b->qn[id + l * s] + rk_b[rk] * cv[id + l * s] + rk_c[rk] * dj * b->dq[iq + l * b->n];
+        if (b->mz == 1) cv[id + 3 * s] = 0;
+        bv[id] = cv[id];
+        const double di = 1.0 / cv[id];
+        bv[id + s] = cv[id + s] * di;
+        bv[id + 2 * s] = cv[id + 2 * s] * di;
+        bv[id + 3 * s] = cv[id + 3 * s] * di;
+        double sum = 0;
+        for (int l = 0; l < ns; ++l) {
+          if (cv[id + (5 + l) * s] < 0) cv[id + (5 + l) * s] = 0;
+          else sum += cv[id + (5 + l) * s];
+        }
+        sum = 1.0 / sum;
+        const double f = cv[id] * sum;
+        for (int l = 0; l < ns; ++l) {
+          b->sv[id + l * s] = cv[id + (5 + l) * s] * sum;
+          cv[id + (5 + l) * s] *= f;
+        }
+        temperature_pressure(b, id);
+      }
+}
+
+double orc_mix_cfl_dt(orc_mix *b) {
+  const long s = b->sz;
+  double dtmin = 1e+6;
+#pragma omp parallel for collapse(2) reduction(min : dtmin)
+  for (int k = 0; k < b->mz; ++k)
+    for (int j = 0; j < b->my; ++j)
+      for (int i = 0; i < b->mx; ++i) {
+        const long id = ID(b, i, j, k);
+        const double *M = b->met, *bv = b->bv;
+        const double gx = sqrt(M[id] * M[id] + M[id + s] * M[id + s] + M[id + 2 * s] * M[id + 2 * s]);
+        const double ge = sqrt(M[id + 3 * s] * M[id + 3 * s] + M[id + 4 * s] * M[id + 4 * s] +
+                               M[id + 5 * s] * M[id + 5 * s]);
+        const double gz = sqrt(M[id + 6 * s] * M[id + 6 * s] + M[id + 7 * s] * M[id + 7 * s] +
+                               M[id + 8 * s] * M[id + 8 * s]);
+        const double u = bv[id + s], v = bv[id + 2 * s], w = bv[id + 3 * s];
+        const double U = u * M[id] + v * M[id + s] + w * M[id + 2 * s];
+        const double V = u * M[id + 3 * s] + v * M[id + 4 * s] + w * M[id + 5 * s];
+        const double W = u * M[id + 6 * s] + v * M[id + 7 * s] + w * M[id + 8 * s];
+        const double c = b->c[id];
+        double sr = fmax(fabs(U) + c * gx, fabs(V) + c * ge);
+        if (b->dim == 3) sr = fmax(sr, fabs(W) + c * gz);
+        const double dt = b->p.cfl / sr;
+        if (dt < dtmin) dtmin = dt;
+      }
+  return dtmin;
+}
+
+void orc_mix_rk3_step(orc_mix *b, double dt) {
+  memcpy(b->qn, b->cv, sizeof(double) * b->nv * b->sz);
+  for (int rk = 0; rk < 3; ++rk) {
+    orc_mix_rhs(b);
+    update(b, dt, rk);
+    apply_bcs(b);
+    all_properties(b);
+  }
+}
+
+/* Exact chemistry over dt in every interior cell at constant density and internal energy (ROK4E, as
+ * orc_chem_rok4e), the step size carried per cell; then species, temperature and pressure, boundary
+ * conditions and properties.  Returns the number of cells that did not reach dt. */
+int orc_mix_chem_substep(orc_mix *b, double dt, double rtol, double atol, int max_attempts) {
+  const long s = b->sz;
+  const int ns = b->ns;
+  int failed = 0;
+#pragma omp parallel for collapse(2) reduction(+ : failed) schedule(dynamic)
+  for (int k = 0; k < b->mz; ++k)
+    for (int j = 0; j < b->my; ++j)
+      for (int i = 0; i < b->mx; ++i) {
+        const long id = ID(b, i, j, k), iq = IDQ(b, i, j, k);
+        double *cv = b->cv, *bv = b->bv;
+        const double rho = cv[id];
+        const double u = bv[id + s], v = bv[id + 2 * s], w = bv[id + 3 * s];
+        const double e = cv[id + 4 * s] / rho - 0.5 * (u * u + v * v + w * w);
+        double y[MS], T = bv[id + 5 * s];
+        for (int l = 0; l < ns; ++l) y[l] = cv[id + (5 + l) * s];
+        double h = b->hstep[iq] > 0 ? b->hstep[iq] : dt;
+        int rej = 0;
+        const int n = orc_chem_rok4e(&b->m, rho, e, y, &T, dt, rtol, atol, &h, max_attempts, &rej, 0, 0);
+        if (n < 0) ++failed;
+        b->hstep[iq] = h;
+        double R = 0;
+        for (int l = 0; l < ns; ++l) {
+          cv[id + (5 + l) * s] = y[l];
+          b->sv[id + l * s] = y[l] / rho;
+          R += y[l] / rho * kRu / b->m.mw[l];
+        }
+        bv[id + 5 * s] = T;
+        bv[id + 4 * s] = rho * R * T;
+      }
+  apply_bcs(b);
+  all_properties(b);
+  return failed;
+}
+
+/* Strang splitting: chemistry dt/2, one SSP-RK3 flow step dt, chemistry dt/2. */
+int orc_mix_strang_step(orc_mix *b, double dt, double rtol, double atol, int max_attempts) {
+  int f = orc_mix_chem_substep(b, 0.5 * dt, rtol, atol, max_attempts);
+  orc_mix_rk3_step(b, dt);
+  f += orc_mix_chem_substep(b, 0.5 * dt, rtol, atol, max_attempts);
+  return f;
+}
+
+/* Interface flux of the mixture Riemann solvers for given primitive states (rho,u,v,w,p,Y_1..Y_ns). */
+void orc_mix_point_flux(const orc_params *p, const orc_mech *m, const double *pl_in, const double *pr_in,
+                        const double *ml, const double *mr, double jl, double jr, double efl, double efr, double *fc) {
+  static orc_mix b; /* only p, m, ns, nv are read */
+#pragma omp threadprivate(b)
+  b.p = *p;
+  b.m = *m;
+  b.ns = m->ns;
+  b.nv = 5 + m->ns;
+  double pl[NV + 2], pr[NV + 2];
+  memcpy(pl, pl_in, sizeof(double) * b.nv);
+  memcpy(pr, pr_in, sizeof(double) * b.nv);
+  post(&b, pl);
+  post(&b, pr);
+  if (p->inviscid_scheme == 2) roe(&b, pl, pr, ml, mr, jl, jr, efl, efr, fc);
+  else if (p->inviscid_scheme == 4) hllc(&b, pl, pr, ml, mr, jl, jr, fc);
+  else ausm(&b, pl, pr, ml, mr, jl, jr, fc);
+}

@@ -250,3 +250,100 @@ def chem_rok4e(m, rho, E, rhoY, T, t_end, rtol=1e-6, atol=1e-12, h=1e-8, max_att
     n = lib.orc_chem_rok4e(ctypes.byref(m), rho, E, _p(y), ctypes.byref(Tc), t_end, rtol, atol, ctypes.byref(hc),
                            max_attempts, ctypes.byref(rej), _p(rec), n_rec)
     return y, Tc.value, hc.value, n, rej.value, rec[:2 * min(max(n, 0), n_rec)].reshape(-1, 2)
+
+
+# ---------------------------------------------------------------- mixture block
+def _mix_protos(lib):
+    D, I, P = ctypes.c_double, ctypes.c_int, ctypes.c_void_p
+    _chem_protos(lib)
+    lib.orc_mix_create.restype = P
+    lib.orc_mix_create.argtypes = [ctypes.POINTER(OrcParams), ctypes.POINTER(OrcMech), I, I, I, I, P, I, P, P]
+    lib.orc_mix_destroy.argtypes = [P]
+    lib.orc_mix_set_state.argtypes = [P, P, P]
+    lib.orc_mix_set_raw.argtypes = [P, P, P, P]
+    lib.orc_mix_get.argtypes = [P, I, P]
+    lib.orc_mix_rhs.argtypes = [P]
+    lib.orc_mix_cfl_dt.restype = D
+    lib.orc_mix_cfl_dt.argtypes = [P]
+    lib.orc_mix_rk3_step.argtypes = [P, D]
+    lib.orc_mix_chem_substep.restype = I
+    lib.orc_mix_chem_substep.argtypes = [P, D, D, D, I]
+    lib.orc_mix_strang_step.restype = I
+    lib.orc_mix_strang_step.argtypes = [P, D, D, D, I]
+    lib.orc_mix_point_flux.argtypes = [ctypes.POINTER(OrcParams), ctypes.POINTER(OrcMech), P, P, P, P, D, D, D, D,
+                                       P]
+
+
+def constant_gamma_mech(n_species=1, mw=28.9647, cp_over_r=3.5):
+    """Mechanism dict of n identical species with constant cp/R (no reactions): gamma = cp/(cp - R)."""
+    ns = n_species
+    coeff = [cp_over_r, 0, 0, 0, 0, 0, 0]
+    return dict(ns=ns, nr=0, species=[f"S{k}" for k in range(ns)], mw=np.full(ns, mw), t_low=np.full(ns, 10.0),
+                t_mid=np.full(ns, 1000.0), t_high=np.full(ns, 1e5), lo=[coeff] * ns, hi=[coeff] * ns,
+                type=np.zeros(0, np.int32), rev=np.zeros(0, np.int32), A=np.zeros(0), b=np.zeros(0), Ea=np.zeros(0),
+                A2=np.zeros(0), b2=np.zeros(0), Ea2=np.zeros(0), order=np.zeros(0), troe=np.zeros((0, 4)),
+                nf=np.zeros((0, ns)), nb=np.zeros((0, ns)), tb=np.zeros((0, ns)))
+
+
+def mix_point_flux(p, m, pl, pr, ml, mr, jl=1.0, jr=1.0, efl=0.0, efr=0.0):
+    lib = load()
+    _mix_protos(lib)
+    fc = np.zeros(5 + m.ns)
+    lib.orc_mix_point_flux(ctypes.byref(p), ctypes.byref(m), _p(np.ascontiguousarray(pl, float)),
+                           _p(np.ascontiguousarray(pr, float)), _p(np.ascontiguousarray(ml, float)),
+                           _p(np.ascontiguousarray(mr, float)), jl, jr, efl, efr, _p(fc))
+    return fc
+
+
+class MixBlock:
+    """One oracle mixture block (5 + ns conservative variables)."""
+
+    def __init__(self, p, m, x, y, z, ngg, faces):
+        lib = load()
+        _mix_protos(lib)
+        self.mx, self.my, self.mz = x.shape
+        self.ngg, self.ns = ngg, m.ns
+        self.sz = (self.mx + 2 * ngg) * (self.my + 2 * ngg) * (self.mz + 2 * ngg)
+        self.n = self.mx * self.my * self.mz
+        self.jac, self.metric = metrics(x, y, z, ngg)
+        f = np.ascontiguousarray(np.asarray(faces, dtype=np.int32).ravel())
+        self._p, self._m = p, m
+        self._h = lib.orc_mix_create(ctypes.byref(p), ctypes.byref(m), self.mx, self.my, self.mz, ngg, _p(f),
+                                     len(faces), _p(self.jac), _p(self.metric))
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            load().orc_mix_destroy(self._h)
+            self._h = None
+
+    def set_state(self, bv, sv):
+        bv, sv = np.ascontiguousarray(bv, float), np.ascontiguousarray(sv, float)
+        assert bv.size == 6 * self.sz and sv.size == self.ns * self.sz
+        load().orc_mix_set_state(self._h, _p(bv), _p(sv))
+
+    def set_raw(self, bv, sv, cv):
+        bv, sv, cv = (np.ascontiguousarray(a, float) for a in (bv, sv, cv))
+        load().orc_mix_set_raw(self._h, _p(bv), _p(sv), _p(cv))
+
+    def get(self, which):
+        nv = 5 + self.ns
+        n = {0: nv * self.sz, 1: 6 * self.sz, 2: nv * self.n, 3: self.ns * self.sz, 4: self.sz}[which]
+        out = np.zeros(n)
+        load().orc_mix_get(self._h, which, _p(out))
+        return out
+
+    def rhs(self):
+        load().orc_mix_rhs(self._h)
+        return self.get(2)
+
+    def cfl_dt(self):
+        return load().orc_mix_cfl_dt(self._h)
+
+    def step(self, dt):
+        load().orc_mix_rk3_step(self._h, dt)
+
+    def chem_substep(self, dt, rtol=1e-6, atol=1e-8, max_attempts=100000):
+        return load().orc_mix_chem_substep(self._h, dt, rtol, atol, max_attempts)
+
+    def strang_step(self, dt, rtol=1e-6, atol=1e-8, max_attempts=100000):
+        return load().orc_mix_strang_step(self._h, dt, rtol, atol, max_attempts)

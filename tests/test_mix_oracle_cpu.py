@@ -1,0 +1,172 @@
+"""Pins of the oracle's mixture path (oracle/oracle_mix.c), CPU only.
+
+* constant-gamma limit: one species with cp/R = 3.5 and the molar mass of air is the calorically perfect
+  gas gamma = 1.4, so the Sod problem has its exact Riemann solution (Toro) to converge to;
+* two identical species: the mixture is one gas, so a species contact at uniform velocity and pressure
+  keeps both exactly uniform while the species is advected, and density matches the one-species run;
+* conservation: on a periodic Cartesian block the residual of every conserved variable sums to zero
+  over the unique nodes;
+* consistency: every mixture Riemann flux of equal left/right states is the physical flux, and species
+  fluxes sum to the mass flux.
+"""
+import numpy as np
+import pytest
+
+import cases
+import oracle_lib as O
+
+RU = 8314.462618
+
+
+def params(inviscid, recon, cfl=0.5):
+    return O.params(inviscid=inviscid, recon=recon, visc=0, cfl=cfl)
+
+
+def sod_block(p, m, n=200):
+    c = cases.sod(n=n, my=5)
+    X, Y, Z = c["grid"]
+    blk = O.MixBlock(p, m, X, Y, Z, 3, c["faces"])
+    g = 3
+    d2, d1 = n + 2 * g, (5 + 2 * g) * (n + 2 * g)
+    sz = blk.sz
+    bv = np.zeros((6, sz))
+    sv = np.zeros((m.ns, sz))
+    ii = (np.arange(sz) % d2) - g
+    left = ii < n // 2
+    mw = 28.9647
+    bv[0] = np.where(left, 1.0, 0.125)
+    bv[4] = np.where(left, 1.0, 0.1)
+    bv[5] = bv[4] * mw / (bv[0] * RU)
+    sv[0] = 1.0
+    blk.set_state(bv.ravel(), sv.ravel())
+    return blk, c
+
+
+def run_to(blk, t_end):
+    t = 0.0
+    while t < t_end - 1e-14:
+        dt = min(blk.cfl_dt(), t_end - t)
+        blk.step(dt)
+        t += dt
+    return t
+
+
+@pytest.mark.parametrize("inviscid,recon", [(2, 2), (4, 2), (3, 3), (4, 4), (2, 4)])
+def test_constant_gamma_sod_converges_to_exact(inviscid, recon):
+    m = O.mech_struct(O.constant_gamma_mech())
+    blk, c = sod_block(params(inviscid, recon), m, n=400)
+    t = run_to(blk, 0.2)
+    bv = blk.get(1).reshape(6, -1)
+    g, n = 3, 400
+    d2 = n + 2 * g
+    d1 = (5 + 2 * g) * d2
+    base = g * d1 + (2 + g) * d2 + g  # k = 0, middle j row
+    rho = bv[0, base:base + n]
+    x = np.linspace(0, 1, n)
+    ex = O.exact_riemann((1.0, 0.0, 1.0), (0.125, 0.0, 0.1), 1.4, 0.5, t, x)[0]
+    l1 = np.mean(np.abs(rho - ex))
+    assert l1 < (6e-3 if recon < 4 else 3e-3), l1
+
+
+def test_identical_species_keep_pressure_and_velocity_uniform():
+    m1 = O.mech_struct(O.constant_gamma_mech(1))
+    m2 = O.mech_struct(O.constant_gamma_mech(2))
+    X, Y, Z = cases.wavy_grid(17, amp=0.0)
+    faces = cases.box_faces(*X.shape, 8)
+    p = params(2, 4)
+    b1 = O.MixBlock(p, m1, X, Y, Z, 3, faces)
+    b2 = O.MixBlock(p, m2, X, Y, Z, 3, faces)
+    sz = b1.sz
+    rng = np.random.default_rng(1)
+    # random on the unique nodes, periodic copies elsewhere: coincident nodes hold equal states, so the
+    # node averaging of the periodic faces leaves rho, p and T consistent
+    nu = X.shape[0] - 1
+    g = 3
+    N = X.shape[0] + 2 * g
+    kk, jj, ii = np.meshgrid(*[(np.arange(N) - g) % nu] * 3, indexing="ij")
+    flat = ((kk * nu + jj) * nu + ii).ravel()
+    bv = np.zeros((6, sz))
+    bv[0] = (1.0 + 0.2 * rng.uniform(size=nu ** 3))[flat]  # density varies, pressure and velocity do not
+    bv[1], bv[2], bv[3] = 30.0, -10.0, 5.0
+    bv[4] = 1e5
+    bv[5] = bv[4] * 28.9647 / (bv[0] * RU)
+    ya = rng.uniform(size=nu ** 3)[flat]
+    b1.set_state(bv.ravel(), np.ones(sz))
+    b2.set_state(bv.ravel(), np.concatenate([ya, 1 - ya]))
+    dt = 0.5 * b1.cfl_dt()
+    for _ in range(3):
+        b1.step(dt)
+        b2.step(dt)
+    v1, v2 = b1.get(1).reshape(6, -1), b2.get(1).reshape(6, -1)
+    assert np.max(np.abs(v2[4] / 1e5 - 1)) < 1e-12
+    assert np.max(np.abs(v2[1] - 30.0)) < 1e-9
+    np.testing.assert_allclose(v2[0], v1[0], rtol=1e-12)
+    sv = b2.get(3).reshape(2, -1)
+    np.testing.assert_allclose(sv.sum(axis=0), 1.0, rtol=0, atol=1e-14)
+
+
+@pytest.mark.parametrize("inviscid,recon", [(2, 4), (4, 2), (3, 1)])
+def test_periodic_residual_is_conservative(inviscid, recon):
+    import chem_inputs as C
+    import chem_cases as CC
+    d = C.parse(open(CC.MECH).read(), open(CC.THERM).read())
+    m = O.mech_struct(d)
+    n = 13
+    # Cartesian: ghost coordinates are mirrored (not periodic images), so only on a uniform grid do the
+    # two copies of the periodic seam see the same metrics
+    X, Y, Z = cases.wavy_grid(n, amp=0.0)
+    faces = cases.box_faces(n, n, n, 8)
+    blk = O.MixBlock(params(inviscid, recon), m, X, Y, Z, 3, faces)
+    g = 3
+    sz = blk.sz
+    N = n + 2 * g
+    kk, jj, ii = np.meshgrid(*[np.arange(N) - g] * 3, indexing="ij")
+    th = 2 * np.pi / (n - 1)
+    s = np.sin(th * ii) * np.cos(th * jj) + 0.5 * np.sin(th * kk)
+    bv = np.zeros((6, sz))
+    bv[0] = (1.0 + 0.1 * s).ravel()
+    bv[1] = (20 * np.cos(th * jj)).ravel()
+    bv[2] = (15 * np.sin(th * kk)).ravel()
+    bv[3] = (10 * np.cos(th * ii)).ravel()
+    bv[5] = (1200 + 300 * s).ravel()
+    Yk = np.repeat(CC.Y_STOICH[:, None], sz, axis=1)
+    Yk[7] += 0.05 * (1 + s.ravel())
+    Yk /= Yk.sum(axis=0)
+    bv[4] = bv[0] * RU * np.sum(Yk / CC.MW[:, None], axis=0) * bv[5]
+    blk.set_state(bv.ravel(), Yk.ravel())
+    dq = blk.rhs().reshape(14, n, n, n)  # [l, k, j, i]
+    uniq = dq[:, :-1, :-1, :-1].reshape(14, -1)
+    scale = np.abs(dq).reshape(14, -1).max(axis=1)
+    assert np.all(np.abs(uniq.sum(axis=1)) <= 1e-10 * scale * uniq.shape[1] ** 0.5), uniq.sum(axis=1) / scale
+
+
+@pytest.mark.parametrize("scheme", [2, 3, 4])
+def test_riemann_consistency_and_species_sum(scheme):
+    import chem_inputs as C
+    import chem_cases as CC
+    d = C.parse(open(CC.MECH).read(), open(CC.THERM).read())
+    m = O.mech_struct(d)
+    p = params(scheme, 1)
+    rng = np.random.default_rng(2)
+    for _ in range(10):
+        Y = rng.dirichlet(np.ones(9))
+        T = rng.uniform(300, 2500)
+        rho = rng.uniform(0.1, 3)
+        u = rng.uniform(-300, 300, 3)
+        pres = rho * RU * np.sum(Y / CC.MW) * T
+        pv = np.concatenate([[rho], u, [pres], Y])
+        k = rng.uniform(-1, 1, 3)
+        J = rng.uniform(0.5, 2)
+        fc = O.mix_point_flux(p, m, pv, pv, k, k, J, J, 0.01, 0.01)
+        h, _ = O.chem_h_cp(m, T)
+        E = rho * (0.5 * u @ u + Y @ h) - pres
+        U = u @ k
+        phys = J * np.concatenate([[rho * U], rho * U * u + pres * k, [(E + pres) * U], rho * U * Y])
+        np.testing.assert_allclose(fc, phys, rtol=1e-12, atol=1e-12 * np.max(np.abs(phys)))
+        pl = pv.copy()
+        pr = pv.copy()
+        pr[0] *= 1.3
+        pr[4] *= 0.8
+        pr[1] += 50
+        f2 = O.mix_point_flux(p, m, pl, pr, k, k, J, J, 0.01, 0.01)
+        assert abs(f2[5:].sum() - f2[0]) <= 1e-12 * max(abs(f2[0]), 1e-300) + 1e-13 * np.max(np.abs(f2))
