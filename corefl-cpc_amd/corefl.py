@@ -12,14 +12,14 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libcorefl_amd.so")
 
-CV, BV, DQ, JAC, METRIC, FV, GV, HV = 0, 1, 2, 4, 5, 6, 7, 8
+CV, BV, DQ, SV, JAC, METRIC, FV, GV, HV = 0, 1, 2, 3, 4, 5, 6, 7, 8
 
 # exported symbols, in the order include/corefl_c.h declares them
 SYMBOLS = (
     "corefl_create", "corefl_create_mem", "corefl_destroy", "corefl_last_error", "corefl_n_blocks",
     "corefl_block_view_get", "corefl_get_array", "corefl_set_state", "corefl_rhs", "corefl_compute_dt",
     "corefl_rk3_step", "corefl_advance", "corefl_get_time", "corefl_synchronize", "corefl_get_int",
-    "corefl_profile_kernels", "corefl_get_real",
+    "corefl_profile_kernels", "corefl_get_real", "corefl_chemistry", "corefl_set_chemistry_surrogate",
     "corefl_chem_create", "corefl_chem_create_mem", "corefl_chem_destroy", "corefl_chem_info", "corefl_chem_name",
     "corefl_chem_table", "corefl_chem_set_cells", "corefl_chem_set_cell_dt", "corefl_chem_get_cells", "corefl_chem_view_get",
     "corefl_chem_source", "corefl_chem_integrate", "corefl_chem_set_launch", "corefl_chem_get_launch",
@@ -75,6 +75,8 @@ def load_library(path=LIB_PATH):
     lib.corefl_get_int.argtypes = [P, ctypes.c_char_p, ctypes.POINTER(I)]
     lib.corefl_get_real.argtypes = [P, ctypes.c_char_p, ctypes.POINTER(D)]
     lib.corefl_profile_kernels.argtypes = [P, I, ctypes.POINTER(D), I]
+    lib.corefl_chemistry.argtypes = [P, I, ctypes.POINTER(P)]
+    lib.corefl_set_chemistry_surrogate.argtypes = [P, P]
     LL, PI = ctypes.c_longlong, ctypes.POINTER(I)
     lib.corefl_chem_create.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(P)]
     lib.corefl_chem_create_mem.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(P)]
@@ -167,15 +169,28 @@ class Solver:
     def get(self, which, b=0):
         v = self.view(b)
         n_int = v.mx * v.my * v.mz
-        count = {CV: 5 * v.sz, BV: 6 * v.sz, DQ: 5 * n_int, JAC: v.sz, METRIC: 9 * v.sz,
-                 FV: 5 * v.sz, GV: 5 * v.sz, HV: 5 * v.sz}[which]
+        count = {CV: v.n_var * v.sz, BV: 6 * v.sz, DQ: v.n_var * n_int, SV: v.n_spec * v.sz, JAC: v.sz,
+                 METRIC: 9 * v.sz, FV: 5 * v.sz, GV: 5 * v.sz, HV: 5 * v.sz}[which]
         out = np.empty(count, dtype=np.float64)
         _check(self._lib.corefl_get_array(self._ctx, b, which, out.ctypes.data, count))
         return out
 
-    def set_state(self, bv, b=0):
+    def set_state(self, bv, b=0, sv=None):
+        """bv (6*sz); for mixtures also sv (n_spec*sz)."""
         bv = np.ascontiguousarray(bv, dtype=np.float64).ravel()
+        if sv is not None:
+            bv = np.concatenate([bv, np.ascontiguousarray(sv, dtype=np.float64).ravel()])
         _check(self._lib.corefl_set_state(self._ctx, b, bv.ctypes.data, bv.size))
+
+    def chemistry(self, b=0):
+        """Borrowed Chemistry view of a mixture setup, its batch holding block b's interior cells."""
+        h = ctypes.c_void_p()
+        _check(self._lib.corefl_chemistry(self._ctx, b, ctypes.byref(h)))
+        return Chemistry._borrow(h, self)
+
+    def set_chemistry_surrogate(self, surrogate):
+        self._surrogate = surrogate
+        _check(self._lib.corefl_set_chemistry_surrogate(self._ctx, surrogate._ctx if surrogate else None))
 
     def rhs(self):
         _check(self._lib.corefl_rhs(self._ctx, None))
@@ -232,6 +247,7 @@ class Chemistry:
     def __init__(self, mechanism_file=None, therm_file=None, *, mech_text=None, therm_text=""):
         lib = load_library()
         self._lib = lib
+        self._owned = True
         self._ctx = ctypes.c_void_p()
         if mech_text is not None:
             _check(lib.corefl_chem_create_mem(mech_text.encode(), therm_text.encode(), ctypes.byref(self._ctx)))
@@ -243,10 +259,25 @@ class Chemistry:
         self.n_spec, self.n_reac, self.n_elem = ns.value, nr.value, ne.value
         self.n = 0
 
+    @classmethod
+    def _borrow(cls, handle, owner):
+        self = cls.__new__(cls)
+        self._lib = load_library()
+        self._owned = False
+        self._owner = owner
+        self._ctx = handle
+        ns, nr, ne = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _check(self._lib.corefl_chem_info(self._ctx, ctypes.byref(ns), ctypes.byref(nr), ctypes.byref(ne)))
+        self.n_spec, self.n_reac, self.n_elem = ns.value, nr.value, ne.value
+        v = ChemView()
+        _check(self._lib.corefl_chem_view_get(self._ctx, ctypes.byref(v)))
+        self.n = v.n
+        return self
+
     def close(self):
-        if self._ctx:
+        if self._ctx and self._owned:
             self._lib.corefl_chem_destroy(self._ctx)
-            self._ctx = ctypes.c_void_p()
+        self._ctx = ctypes.c_void_p()
 
     def __del__(self):
         try:

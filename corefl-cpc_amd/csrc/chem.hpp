@@ -32,6 +32,21 @@ struct ChemMech {
   char elements[kMaxElem][4];
 };
 
+// h_k and cp_k per unit mass at t, linear extrapolation outside [t_low, t_high] (Thermo.cu:470-520).
+inline void host_h_cp(const ChemMech &m, double t, double *h, double *cp) {
+  for (int i = 0; i < m.ns; ++i) {
+    const double R = 8314.462618 / m.mw[i];
+    const bool below = t < m.t_low[i], above = t > m.t_high[i];
+    const double tt = below ? m.t_low[i] : (above ? m.t_high[i] : t);
+    const double *c = below ? m.lo[i] : (above ? m.hi[i] : (t < m.t_mid[i] ? m.lo[i] : m.hi[i]));
+    const double hh = c[0] * tt + 0.5 * c[1] * tt * tt + c[2] * tt * tt * tt / 3 + 0.25 * c[3] * tt * tt * tt * tt +
+                      0.2 * c[4] * tt * tt * tt * tt * tt + c[5];
+    const double cc = c[0] + c[1] * tt + c[2] * tt * tt + c[3] * tt * tt * tt + c[4] * tt * tt * tt * tt;
+    h[i] = (hh + cc * (t - tt)) * R;
+    cp[i] = cc * R;
+  }
+}
+
 // Throws std::runtime_error with a message on malformed input or table overflow.
 ChemMech parse_mechanism(const std::string &mech_text, const std::string &therm_text);
 

@@ -42,6 +42,10 @@ void chem_ensure_device(corefl_chem *c);
 void chem_ensure_batch(corefl_chem *c, long long n);
 hipStream_t chem_stream(corefl_chem *c, void *stream);
 void chem_check_args(corefl_chem *c, double dt, double rtol, double atol, int max_attempts);
+// Surrogate sub-step of batch b (the chemistry context's cells, possibly with dt_dev / dtc) with the
+// exact ROK4E fallback for gated cells, enqueued on st (implemented with the surrogate kernels).
+void surrogate_substep(corefl_chem *c, corefl_surrogate *s, const ChemBatch &b, double rtol, double atol,
+                       int max_attempts, hipStream_t st);
 // Exact sub-step of every cell, or of list[0 .. *n_list) when list is given.
 void chem_integrate(corefl_chem *c, const ChemLaunch &cfg, double dt, double rtol, double atol, int max_attempts,
                     hipStream_t s, const int *list, const int *n_list);

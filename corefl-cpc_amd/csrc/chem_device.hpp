@@ -66,7 +66,7 @@ __device__ __forceinline__ double mass_action(double c, double nu) {
 // Chemical source omega [kg/(m^3 s)] of one cell (FiniteRateChem.cu:1466-1605).  With JAC, also the
 // Jacobian d omega / d(rho Y) with the rates of progress frozen (ConstVolumeReactor.cpp:1168-1187).
 template<int NS, bool JAC>
-__device__ void source(const ChemMech &m, double t, const double *rhoY, double *w, double *J) {
+__device__ void source(const ChemMech &m, double t, const double *rhoY, double *w, double *J, double *wd = nullptr) {
   double c[NS], g[NS], cr[NS], de[NS];
 #pragma unroll
   for (int l = 0; l < NS; ++l) {
@@ -148,7 +148,10 @@ __device__ void source(const ChemMech &m, double t, const double *rhoY, double *
     }
   }
 #pragma unroll
-  for (int l = 0; l < NS; ++l) w[l] = cr[l] * (1e+3 * m.mw[l]) - de[l] * (1e+3 * m.mw[l]);
+  for (int l = 0; l < NS; ++l) {
+    w[l] = cr[l] * (1e+3 * m.mw[l]) - de[l] * (1e+3 * m.mw[l]);
+    if (wd) wd[l] = de[l] * (1e+3 * m.mw[l]);
+  }
   if constexpr (JAC) {
 #pragma unroll
     for (int n = 0; n < NS; ++n) {

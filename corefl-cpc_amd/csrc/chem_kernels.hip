@@ -168,7 +168,7 @@ __device__ void rok4e_attempt(const ChemMech &m, Cell<NS> &s, double rtol, doubl
 template<int NS> __device__ __forceinline__ void load_cell(const ChemBatch &b, int c, double dt, Cell<NS> &s) {
 #pragma unroll
   for (int l = 0; l < NS; ++l) s.y[l] = b.rhoY[(long)l * b.n + c];
-  if (b.dtc) dt = b.dtc[c];
+  dt = cell_dt(b, c, dt);
   s.dt = dt;
   s.rho = b.rho[c];
   s.E = b.e[c];

@@ -20,7 +20,14 @@ struct ChemBatch {
   const double *dtc; // per-cell sub-step length (nullptr: the launch's dt for every cell)
   const int *list;   // integrate only cells list[0 .. *n_list) (nullptr: all n cells)
   const int *n_list; // device-side count, at most n
+  const double *dt_dev; // when dtc is null: sub-step length dt_scale * dt_dev[0] (device-side step size)
+  double dt_scale;
 };
+
+// Sub-step length of cell c: per-cell, device-side, or the launch's value.
+__device__ __forceinline__ double cell_dt(const ChemBatch &b, long c, double dt) {
+  return b.dtc ? b.dtc[c] : (b.dt_dev ? b.dt_scale * b.dt_dev[0] : dt);
+}
 
 // Launch shape picked by the autotuner.
 struct ChemLaunch {

@@ -12,7 +12,10 @@
 #include <string>
 #include <vector>
 
+#include "chem.hpp"
+#include "chem_ctx.hpp"
 #include "flow.hpp"
+#include "flow_mix.hpp"
 #include "mesh.hpp"
 #include "parameter.hpp"
 
@@ -37,19 +40,40 @@ double sutherland(double T) { return 1.7894e-5 * std::pow(T / 288.16, 1.5) * (28
 // Air state of an inflow-type struct (Inflow::Inflow, BoundCond.cpp:158-262, single species).
 struct FlowState {
   double rho, u, v, w, p, T, c, mu;
+  std::vector<double> Y; // mass fractions (mixtures only)
 };
 
-FlowState inflow_state(const corefl::Struct &s) {
+FlowState inflow_state(const corefl::Struct &s, const corefl::ChemMech *mech = nullptr) {
   using namespace corefl;
   double mach = struct_real(s, "mach", -1), p = struct_real(s, "pressure", -1);
   double T = struct_real(s, "temperature", -1), vel = struct_real(s, "velocity", 0);
   double rho = struct_real(s, "density", -1), aoa = struct_real(s, "aoa", -1000);
   double u = struct_real(s, "u", 1), v = struct_real(s, "v", 0), w = struct_real(s, "w", 0);
   const double re = struct_real(s, "reynolds", -1);
-  const double mw = kMwAir;
+  double mw = kMwAir, gamma = kGamma;
+  std::vector<double> Y;
+  if (mech) {
+    // species mass fractions by name (Inflow::Inflow, BoundCond.cpp:158-262)
+    Y.assign(mech->ns, 0.0);
+    double sum = 0, imw = 0;
+    for (int k = 0; k < mech->ns; ++k) {
+      Y[k] = struct_real(s, mech->species[k], 0.0);
+      sum += Y[k];
+      imw += Y[k] / mech->mw[k];
+    }
+    if (!(sum > 0)) throw std::runtime_error("inflow struct of a mixture: give the species mass fractions");
+    for (auto &y : Y) y /= sum;
+    mw = sum / imw;
+  }
   if (T < 0) T = p * mw / (rho * kRu);
+  if (mech) {
+    double h[kMaxSpec], cp[kMaxSpec], cpt = 0;
+    host_h_cp(*mech, T, h, cp);
+    for (int k = 0; k < mech->ns; ++k) cpt += Y[k] * cp[k];
+    gamma = cpt / (cpt - kRu / mw);
+  }
   const double mu = sutherland(T);
-  const double c = std::sqrt(kGamma * kRu / mw * T);
+  const double c = std::sqrt(gamma * kRu / mw * T);
   if (mach < 0) mach = vel / c;
   else vel = mach * c;
   if (aoa > -1000) {
@@ -70,12 +94,14 @@ FlowState inflow_state(const corefl::Struct &s) {
       rho = p * mw / (kRu * T);
     }
   }
-  return {rho, u, v, w, p, T, c, mu};
+  return {rho, u, v, w, p, T, c, mu, Y};
 }
 
 struct Block {
   corefl::HostBlock host;
   corefl::DevBlock dev{};
+  corefl::MixArrays mix{}; // species arrays (ns = 0: single-species air)
+  double *hstep = nullptr; // per-cell chemistry step size carried between sub-steps
   std::vector<corefl::DevFace> faces; // in boundary-file order
   long part_offset = 0;
   int part_count = 0;
@@ -96,6 +122,14 @@ struct corefl_ctx {
   hipGraphExec_t graph_exec = nullptr;
   hipGraph_t graph = nullptr;
   std::vector<void *> allocations;
+  // mixtures
+  int ns = 0;
+  bool strang = false;          // temporal_scheme 5: chemistry dt/2, SSP-RK3, chemistry dt/2
+  corefl_chem *chem = nullptr;  // mechanism, device tables and the cell batch of the chemistry sub-steps
+  corefl_surrogate *surrogate = nullptr; // not owned; when set, sub-steps use it with the exact fallback
+  double chem_rtol = 1e-6, chem_atol = 1e-8;
+  int chem_max_attempts = 100000;
+  std::string input_dir;
 
   double *alloc(size_t n) {
     void *p = nullptr;
@@ -105,6 +139,7 @@ struct corefl_ctx {
     return static_cast<double *>(p);
   }
   ~corefl_ctx() {
+    delete chem;
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     if (graph) (void)hipGraphDestroy(graph);
     for (void *p : allocations) (void)hipFree(p);
@@ -115,10 +150,32 @@ struct corefl_ctx {
 
   void apply_bcs(Block &b, hipStream_t s) {
     // DBoundCond::apply_boundary_conditions order (BoundCond.cu:3489-3619): outflow, then periodic.
-    for (const auto &f : b.faces)
-      if (f.type == 6) HIPCHK(corefl::launch_boundary(b.dev, f, s));
-    for (const auto &f : b.faces)
-      if (f.type == 8) HIPCHK(corefl::launch_boundary(b.dev, f, s));
+    for (int type : {6, 8})
+      for (const auto &f : b.faces)
+        if (f.type == type) {
+          if (ns) HIPCHK(corefl::launch_mix_boundary(b.dev, b.mix, f, s));
+          else HIPCHK(corefl::launch_boundary(b.dev, f, s));
+        }
+  }
+
+  // Exact chemistry over scale * dt (dt from the device) in every interior cell of every block
+  // (constant density and internal energy), then boundary conditions.
+  void chem_substep(double scale, hipStream_t s) {
+    for (auto &b : blocks) {
+      corefl::chem_ensure_batch(chem, b.dev.n);
+      HIPCHK(corefl::launch_mix_chem_gather(b.dev, b.mix, chem->b, s));
+      corefl::ChemBatch cb = chem->b;
+      cb.h = b.hstep; // this block's carried step sizes
+      cb.dt_dev = dt_dev;
+      cb.dt_scale = scale;
+      if (surrogate)
+        corefl::surrogate_substep(chem, surrogate, cb, chem_rtol, chem_atol, chem_max_attempts, s);
+      else
+        HIPCHK(corefl::launch_chem_integrate(chem->mech_dev, ns, cb, 0.0, chem_rtol, chem_atol, chem_max_attempts,
+                                             chem->cfg, chem->counter, s));
+      HIPCHK(corefl::launch_mix_chem_scatter(b.dev, b.mix, chem->b, s));
+      apply_bcs(b, s);
+    }
   }
 
   // Residual of one stage for all blocks; stage < 0 writes dq only.
@@ -132,6 +189,13 @@ struct corefl_ctx {
       }
     }
     for (auto &b : blocks) {
+      if (ns) {
+        if (scheme.inviscid_scheme == 2) HIPCHK(corefl::launch_mix_entropy_fix(b.dev, b.mix, scheme, s));
+        for (int d = 0; d < b.dev.dim; ++d) HIPCHK(corefl::launch_mix_inviscid_flux(b.dev, b.mix, scheme, d, s));
+        HIPCHK(corefl::launch_mix_update(b.dev, b.mix, rhs_only ? 0 : rk, dt_dev, rhs_only, s));
+        if (!rhs_only) apply_bcs(b, s);
+        continue;
+      }
       if (scheme.inviscid_scheme == 2) HIPCHK(corefl::launch_entropy_fix(b.dev, scheme, s));
       HIPCHK(corefl::launch_inviscid_flux(b.dev, scheme, 0, s));
       HIPCHK(corefl::launch_inviscid_flux(b.dev, scheme, 1, s));
@@ -142,13 +206,18 @@ struct corefl_ctx {
   }
 
   void enqueue_dt(hipStream_t s, int clip) {
-    for (auto &b : blocks) HIPCHK(corefl::launch_local_dt(b.dev, scheme, parts + b.part_offset, s));
+    for (auto &b : blocks) {
+      if (ns) HIPCHK(corefl::launch_mix_local_dt(b.dev, b.mix, scheme, parts + b.part_offset, s));
+      else HIPCHK(corefl::launch_local_dt(b.dev, scheme, parts + b.part_offset, s));
+    }
     HIPCHK(corefl::launch_dt_reduce(parts, n_parts, dt_dev, time_dev, t_end, clip, s));
   }
 
   void enqueue_step(hipStream_t s, bool compute_dt) {
     if (compute_dt) enqueue_dt(s, 1);
+    if (strang) chem_substep(0.5, s);
     for (int rk = 0; rk < 3; ++rk) stage(rk, s);
+    if (strang) chem_substep(0.5, s);
     HIPCHK(corefl::launch_time_advance(time_dev, dt_dev, s));
   }
 };
@@ -165,15 +234,19 @@ std::string read_file(const std::string &path, bool binary = false) {
 
 // Initial primitive state on every node of one block, ghosts included (Initialize.cu:51-95 and
 // initialize_bv_with_inflow in Field.cu:280-340; canonical_problem "taylorGreen" is new).
-std::vector<double> initial_bv(const corefl::Parameter &P, const corefl::HostBlock &hb) {
+std::vector<double> initial_bv(const corefl::Parameter &P, const corefl::HostBlock &hb,
+                               const corefl::ChemMech *mech = nullptr, std::vector<double> *sv = nullptr) {
   const long sz = hb.size();
   std::vector<double> bv(6 * sz, 0.0);
+  if (mech) sv->assign((size_t)mech->ns * sz, 0.0);
   const int g = hb.ngg, G = g + 1;
   const long c2 = hb.mx + 2L * G, c1 = (hb.my + 2L * G) * c2;
   auto coord = [&](const std::vector<double> &a, int i, int j, int k) {
     return a[(k + G) * c1 + (j + G) * c2 + (i + G)];
   };
   auto put = [&](long id, const FlowState &s) {
+    if (mech)
+      for (int l = 0; l < mech->ns; ++l) (*sv)[id + l * sz] = s.Y[l];
     bv[id] = s.rho;
     bv[id + sz] = s.u;
     bv[id + 2 * sz] = s.v;
@@ -183,6 +256,7 @@ std::vector<double> initial_bv(const corefl::Parameter &P, const corefl::HostBlo
   };
   const std::string problem = P.get_string("canonical_problem");
   if (problem == "taylorGreen") {
+    if (mech) throw std::runtime_error("canonical_problem taylorGreen is single-species");
     const auto &s = P.get_struct("taylorGreen");
     const double T0 = corefl::struct_real(s, "temperature", 300.0);
     const double p0 = corefl::struct_real(s, "pressure", 101325.0);
@@ -201,13 +275,13 @@ std::vector<double> initial_bv(const corefl::Parameter &P, const corefl::HostBlo
     return bv;
   }
   const int groups = P.get_int("groups_init");
-  std::vector<FlowState> states{inflow_state(P.get_struct(P.get_string("default_init")))};
+  std::vector<FlowState> states{inflow_state(P.get_struct(P.get_string("default_init")), mech)};
   std::vector<double> boxes;
   for (int l = 0; l < groups - 1; ++l) {
     const auto &pc = P.get_struct("init_cond_" + std::to_string(l));
     for (const char *key : {"x0", "x1", "y0", "y1", "z0", "z1"}) boxes.push_back(corefl::struct_real(pc, key, 0));
     const std::string name = corefl::struct_string(pc, "name", "init_cond_" + std::to_string(l));
-    states.push_back(inflow_state(P.get_struct(name)));
+    states.push_back(inflow_state(P.get_struct(name), mech));
   }
   for (int k = -g; k < hb.mz + g; ++k)
     for (int j = -g; j < hb.my + g; ++j)
@@ -228,13 +302,54 @@ std::vector<double> initial_bv(const corefl::Parameter &P, const corefl::HostBlo
 
 void build(corefl_ctx *ctx, std::vector<corefl::HostBlock> blocks) {
   auto &P = ctx->param;
-  if (P.get_int("reconstruction") != 4 && P.get_int("reconstruction") != 5)
-    throw std::runtime_error("reconstruction must be 4 (WENO5) or 5 (WENO7) on this path");
+  const int species = P.get_int("species");
+  const int reaction = species ? P.get_int("reaction") : 0;
+  const int temporal = P.get_int("temporal_scheme");
+  const int rec = P.get_int("reconstruction");
   const int inv = P.get_int("inviscid_scheme");
-  if (inv < 1 || inv > 4) throw std::runtime_error("inviscid_scheme must be 1 (LF), 2 (Roe), 3 (AUSM+) or 4 (HLLC)");
   const int vo = P.get_int("viscous_order");
-  if (vo != 0 && vo != 4 && vo != 8) throw std::runtime_error("viscous_order must be 0, 4 or 8 on this path");
-  if (P.get_int("temporal_scheme") != 3) throw std::runtime_error("temporal_scheme must be 3 (SSP-RK3)");
+  if (species) {
+    // Species::Species / Reaction::Reaction read input/<mechanism_file> and input/<therm_file>
+    // (ChemData.cpp:10-100, 868-920)
+    auto path = [&](const std::string &f) { return ctx->input_dir.empty() ? f : ctx->input_dir + "/" + f; };
+    const std::string mech_text = read_file(path(P.get_string("mechanism_file")));
+    std::string therm_text;
+    {
+      std::ifstream t(path(P.get_string("therm_file")));
+      if (t) {
+        std::stringstream ss;
+        ss << t.rdbuf();
+        therm_text = ss.str();
+      }
+    }
+    ctx->chem = new corefl_chem();
+    ctx->chem->mech = corefl::parse_mechanism(mech_text, therm_text);
+    ctx->ns = ctx->chem->mech.ns;
+    if (ctx->ns != 9) throw std::runtime_error("the mixture kernels are instantiated for 9 species");
+    if (rec < 1 || rec > 4)
+      throw std::runtime_error("with species, reconstruction must be 1 (first order), 2 (MUSCL), 3 (NND2) or 4 "
+                               "(component-wise WENO5)");
+    if (inv < 2 || inv > 4) throw std::runtime_error("with species, inviscid_scheme must be 2 (Roe), 3 (AUSM+) or 4 (HLLC)");
+    if (vo != 0) throw std::runtime_error("viscous mixtures are not on this path (viscous_order must be 0)");
+    if (reaction && temporal != 3 && temporal != 5)
+      throw std::runtime_error("reacting flow: temporal_scheme 3 (explicit source) or 5 (Strang splitting)");
+    if (reaction && temporal == 3 && P.get_int("chemSrcMethod") != 0)
+      throw std::runtime_error("only chemSrcMethod = 0 (explicit source) is on this path with temporal_scheme 3");
+    if (!reaction && temporal != 3) throw std::runtime_error("temporal_scheme must be 3 (SSP-RK3)");
+    ctx->strang = reaction && temporal == 5;
+    P.set_int("n_spec", ctx->ns);
+    P.set_int("n_var", 5 + ctx->ns);
+    ctx->chem_rtol = P.get_real("chemistry_rtol");
+    ctx->chem_atol = P.get_real("chemistry_atol");
+    corefl::chem_ensure_device(ctx->chem);
+  } else {
+    if (rec != 4 && rec != 5) throw std::runtime_error("reconstruction must be 4 (WENO5) or 5 (WENO7) on this path");
+    if (inv < 1 || inv > 4)
+      throw std::runtime_error("inviscid_scheme must be 1 (LF), 2 (Roe), 3 (AUSM+) or 4 (HLLC)");
+    if (vo != 0 && vo != 4 && vo != 8) throw std::runtime_error("viscous_order must be 0, 4 or 8 on this path");
+    if (temporal != 3) throw std::runtime_error("temporal_scheme must be 3 (SSP-RK3)");
+  }
+  const corefl::ChemMech *mech = ctx->chem ? &ctx->chem->mech : nullptr;
   const int ngg = P.get_int("ngg");
   P.set_int("dimension", blocks.at(0).dim);
   // reference state and the WENO epsilon scale (BoundCond.cpp:276-295, DParameter.cu:25-28, WENO.cu:8)
@@ -246,7 +361,7 @@ void build(corefl_ctx *ctx, std::vector<corefl::HostBlock> blocks) {
     rho_inf = p0 / (R * T0);
     v_inf = corefl::struct_real(s, "mach", 0.1) * std::sqrt(kGamma * R * T0);
   } else {
-    const FlowState r = inflow_state(P.get_struct(P.get_string("reference_state")));
+    const FlowState r = inflow_state(P.get_struct(P.get_string("reference_state")), mech);
     rho_inf = r.rho;
     v_inf = std::abs(std::sqrt(r.u * r.u + r.v * r.v + r.w * r.w)) < 1 ? r.c : std::sqrt(r.u * r.u + r.v * r.v + r.w * r.w);
   }
@@ -290,21 +405,35 @@ void build(corefl_ctx *ctx, std::vector<corefl::HostBlock> blocks) {
     d.d2 = h.pad2(); d.d1 = h.pad1(); d.sz = h.size(); d.n = (long)h.mx * h.my * h.mz;
     for (int q = 0; q < 6; ++q) d.bt[q] = h.btype[q];
     const size_t sz = d.sz;
-    d.cv = ctx->alloc(5 * sz);
-    d.qn = ctx->alloc(5 * sz);
+    const size_t nv = 5 + ctx->ns;
+    d.cv = ctx->alloc(nv * sz);
+    d.qn = ctx->alloc(nv * sz);
     d.bv = ctx->alloc(6 * sz);
     d.jac = ctx->alloc(sz);
     d.met = ctx->alloc(9 * sz);
-    d.fx = ctx->alloc(5 * sz);
-    d.fy = ctx->alloc(5 * sz);
-    d.fz = d.dim == 3 ? ctx->alloc(5 * sz) : d.fy;
+    d.fx = ctx->alloc(nv * sz);
+    d.fy = ctx->alloc(nv * sz);
+    d.fz = d.dim == 3 ? ctx->alloc(nv * sz) : d.fy;
+    if (ctx->ns) {
+      B.mix.ns = ctx->ns;
+      B.mix.sv = ctx->alloc(ctx->ns * sz);
+      B.mix.snd = ctx->alloc(sz);
+      B.mix.mech = ctx->chem->mech_dev;
+      B.mix.explicit_source = reaction && temporal == 3;
+      if (B.mix.explicit_source) {
+        B.mix.tscale = ctx->alloc(d.n);
+        const std::vector<double> big(d.n, 1e+6);
+        HIPCHK(hipMemcpy(B.mix.tscale, big.data(), d.n * sizeof(double), hipMemcpyHostToDevice));
+      }
+      if (ctx->strang) B.hstep = ctx->alloc(d.n);
+    }
     if (vo > 0) {
       d.fv = ctx->alloc(5 * sz);
       d.gv = ctx->alloc(5 * sz);
       d.hv = ctx->alloc(5 * sz);
     }
     if (inv == 2) d.efd = ctx->alloc(sz);
-    d.dq = ctx->alloc(5 * d.n);
+    d.dq = ctx->alloc(nv * d.n);
     HIPCHK(hipMemcpy(d.jac, h.jac.data(), sz * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(d.met, h.metric.data(), 9 * sz * sizeof(double), hipMemcpyHostToDevice));
     for (const auto &f : h.faces) {
@@ -321,9 +450,15 @@ void build(corefl_ctx *ctx, std::vector<corefl::HostBlock> blocks) {
   ctx->parts = ctx->alloc(ctx->n_parts);
   // initial condition, then initialize_computation (Driver.cu:126-211)
   for (auto &B : ctx->blocks) {
-    const auto bv = initial_bv(P, B.host);
+    std::vector<double> sv;
+    const auto bv = initial_bv(P, B.host, mech, &sv);
     HIPCHK(hipMemcpy(B.dev.bv, bv.data(), bv.size() * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(corefl::launch_cv_from_bv_all(B.dev, ctx->stream));
+    if (ctx->ns) {
+      HIPCHK(hipMemcpy(B.mix.sv, sv.data(), sv.size() * sizeof(double), hipMemcpyHostToDevice));
+      HIPCHK(corefl::launch_mix_cv_from_bv_all(B.dev, B.mix, ctx->stream));
+    } else {
+      HIPCHK(corefl::launch_cv_from_bv_all(B.dev, ctx->stream));
+    }
     ctx->apply_bcs(B, ctx->stream);
   }
   ctx->host_dt = P.get_real("dt");
@@ -366,6 +501,7 @@ int corefl_create(const char *input_dir, corefl_ctx **out) {
   try {
     auto ctx = std::make_unique<corefl_ctx>();
     const std::string dir(input_dir);
+    ctx->input_dir = dir;
     ctx->param.parse(read_file(dir + "/setup.txt"));
     ctx->param.deduce();
     auto &P = ctx->param;
@@ -410,7 +546,7 @@ int corefl_block_view_get(corefl_ctx *ctx, int block, corefl_block_view *v) {
     return -1;
   }
   const auto &d = ctx->blocks[block].dev;
-  v->mx = d.mx; v->my = d.my; v->mz = d.mz; v->ngg = d.g; v->n_var = 5; v->n_spec = 0;
+  v->mx = d.mx; v->my = d.my; v->mz = d.mz; v->ngg = d.g; v->n_var = 5 + ctx->ns; v->n_spec = ctx->ns;
   v->sz = d.sz;
   v->cv = d.cv; v->bv = d.bv; v->qn = d.qn; v->metric = d.met; v->jac = d.jac; v->dq = d.dq;
   return 0;
@@ -420,12 +556,15 @@ int corefl_get_array(corefl_ctx *ctx, int block, int which, double *host, long l
   try {
     if (!ctx || block < 0 || block >= (int)ctx->blocks.size()) throw std::runtime_error("bad block index");
     const auto &d = ctx->blocks[block].dev;
+    const auto &mx = ctx->blocks[block].mix;
     const double *src = nullptr;
     long long cnt = 0;
+    const long nv = 5 + ctx->ns;
     switch (which) {
-      case COREFL_CV: src = d.cv; cnt = 5 * d.sz; break;
+      case COREFL_CV: src = d.cv; cnt = nv * d.sz; break;
       case COREFL_BV: src = d.bv; cnt = 6 * d.sz; break;
-      case COREFL_DQ: src = d.dq; cnt = 5 * d.n; break;
+      case COREFL_SV: src = mx.sv; cnt = ctx->ns * d.sz; break;
+      case COREFL_DQ: src = d.dq; cnt = nv * d.n; break;
       case COREFL_JAC: src = d.jac; cnt = d.sz; break;
       case COREFL_METRIC: src = d.met; cnt = 9 * d.sz; break;
       case COREFL_FV: src = d.fv; cnt = 5 * d.sz; break;
@@ -447,10 +586,15 @@ int corefl_set_state(corefl_ctx *ctx, int block, const double *bv_host, long lon
   try {
     if (!ctx || block < 0 || block >= (int)ctx->blocks.size()) throw std::runtime_error("bad block index");
     auto &B = ctx->blocks[block];
-    if (n != 6 * B.dev.sz) throw std::runtime_error("bv must have 6*sz elements");
+    if (n != (6 + ctx->ns) * B.dev.sz) throw std::runtime_error("state must have (6 + n_spec)*sz elements");
     HIPCHK(hipStreamSynchronize(ctx->stream));
-    HIPCHK(hipMemcpy(B.dev.bv, bv_host, n * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(corefl::launch_cv_from_bv_all(B.dev, ctx->stream));
+    HIPCHK(hipMemcpy(B.dev.bv, bv_host, 6 * B.dev.sz * sizeof(double), hipMemcpyHostToDevice));
+    if (ctx->ns) {
+      HIPCHK(hipMemcpy(B.mix.sv, bv_host + 6 * B.dev.sz, ctx->ns * B.dev.sz * sizeof(double), hipMemcpyHostToDevice));
+      HIPCHK(corefl::launch_mix_cv_from_bv_all(B.dev, B.mix, ctx->stream));
+    } else {
+      HIPCHK(corefl::launch_cv_from_bv_all(B.dev, ctx->stream));
+    }
     ctx->apply_bcs(B, ctx->stream);
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return 0;
@@ -589,6 +733,43 @@ int corefl_profile_kernels(corefl_ctx *ctx, int reps, double *ms_out, int n_out)
     HIPCHK(hipEventDestroy(e0));
     HIPCHK(hipEventDestroy(e1));
     for (int q = 0; q < n_out && q < 8; ++q) ms_out[q] = ms[q];
+    return 0;
+  } catch (const std::exception &e) {
+    return fail(e);
+  }
+}
+
+int corefl_chemistry(corefl_ctx *ctx, int block, corefl_chem **out) {
+  try {
+    if (!ctx || !out || block < 0 || block >= (int)ctx->blocks.size()) throw std::runtime_error("bad arguments");
+    if (!ctx->chem) throw std::runtime_error("this setup has no species (species = 0)");
+    auto &b = ctx->blocks[block];
+    corefl::chem_ensure_batch(ctx->chem, b.dev.n);
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(corefl::launch_mix_chem_gather(b.dev, b.mix, ctx->chem->b, ctx->chem->stream));
+    HIPCHK(hipMemsetAsync(ctx->chem->b.h, 0, sizeof(double) * b.dev.n, ctx->chem->stream));
+    HIPCHK(hipStreamSynchronize(ctx->chem->stream));
+    ctx->chem->have_dtc = false;
+    *out = ctx->chem;
+    return 0;
+  } catch (const std::exception &e) {
+    return fail(e);
+  }
+}
+
+int corefl_set_chemistry_surrogate(corefl_ctx *ctx, corefl_surrogate *s) {
+  try {
+    if (!ctx) throw std::runtime_error("null context");
+    if (s && !ctx->strang) throw std::runtime_error("the surrogate replaces split chemistry sub-steps (temporal_scheme 5)");
+    ctx->surrogate = s;
+    if (ctx->graph_exec) { // the captured step embeds the old choice
+      (void)hipGraphExecDestroy(ctx->graph_exec);
+      ctx->graph_exec = nullptr;
+    }
+    if (ctx->graph) {
+      (void)hipGraphDestroy(ctx->graph);
+      ctx->graph = nullptr;
+    }
     return 0;
   } catch (const std::exception &e) {
     return fail(e);

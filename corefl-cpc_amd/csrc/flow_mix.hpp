@@ -1,0 +1,39 @@
+// Multi-species (mixture) flow on one block: the reference's primitive-variable path
+// (compute_convective_term_pv / Roe_compute_inviscid_flux) with mixture thermodynamics, the mixture SSP-RK3
+// update with the Newton temperature, boundary fills with species, and the Strang-split chemistry
+// exchange with the batched chemistry kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "chem.hpp"
+#include "chem_kernels.hpp"
+#include "flow.hpp"
+
+namespace corefl {
+
+// Species arrays of a mixture block (DZone sv and acoustic_speed, Field.cu:473-570); cv, qn, fx/fy/fz
+// of the DevBlock hold 5 + ns components and dq (5 + ns) * n.
+struct MixArrays {
+  int ns;
+  double *sv;  // Y_k, ns * sz
+  double *snd; // acoustic speed, sz
+  const ChemMech *mech; // device tables
+  int explicit_source;  // add the chemical source to the residual (reaction = 1 with temporal_scheme 3)
+  double *tscale;       // reaction time scale per interior node (n), written with the explicit source
+};
+
+hipError_t launch_mix_cv_from_bv_all(const DevBlock &b, const MixArrays &x, hipStream_t s);
+hipError_t launch_mix_boundary(const DevBlock &b, const MixArrays &x, const DevFace &f, hipStream_t s);
+hipError_t launch_mix_entropy_fix(const DevBlock &b, const MixArrays &x, const Scheme &sc, hipStream_t s);
+// reconstruction 1 first order, 2 MUSCL, 3 NND2 (minmod), 4 / 5 component-wise WENO5 / WENO7;
+// inviscid_scheme 2 Roe, 3 AUSM+, 4 HLLC.
+hipError_t launch_mix_inviscid_flux(const DevBlock &b, const MixArrays &x, const Scheme &sc, int dir, hipStream_t s);
+hipError_t launch_mix_update(const DevBlock &b, const MixArrays &x, int rk, const double *dt_dev, bool rhs_only,
+                             hipStream_t s);
+hipError_t launch_mix_local_dt(const DevBlock &b, const MixArrays &x, const Scheme &sc, double *part, hipStream_t s);
+// Interior cells -> chemistry batch (rho, e, T, rho Y; n = mx*my*mz cells, i fastest) and back.
+hipError_t launch_mix_chem_gather(const DevBlock &b, const MixArrays &x, const ChemBatch &c, hipStream_t s);
+hipError_t launch_mix_chem_scatter(const DevBlock &b, const MixArrays &x, const ChemBatch &c, hipStream_t s);
+
+} // namespace corefl

@@ -1,0 +1,693 @@
+// Mixture-flow kernels (9 species) for gfx950, FP64.  One thread per interface (fluxes) or node (update,
+// properties); species thermodynamics from the NASA-7 fits of the mechanism.
+#include "flow_mix.hpp"
+
+#include <cmath>
+
+#include "chem_device.hpp"
+
+namespace corefl {
+
+namespace {
+
+constexpr int NS = 9, NV = 5 + NS;
+using chemdev::kRu;
+
+__device__ __forceinline__ long nid(const DevBlock &b, int i, int j, int k) {
+  return (long)(k + b.g) * b.d1 + (long)(j + b.g) * b.d2 + (i + b.g);
+}
+
+// compute_cv_from_bv_1_point<Mixture> + compute_total_energy (FieldOperation.cuh:16-84) and the acoustic
+// speed of update_physical_properties<Mixture> (FieldOperation.cuh:86-120) at node id.
+__device__ void mix_cons(const DevBlock &b, const MixArrays &x, long id) {
+  const long sz = b.sz;
+  const ChemMech &m = *x.mech;
+  const double rho = b.bv[id], u = b.bv[id + sz], v = b.bv[id + 2 * sz], w = b.bv[id + 3 * sz];
+  const double p = b.bv[id + 4 * sz], T = b.bv[id + 5 * sz];
+  double h[NS], cp[NS];
+  chemdev::h_cp<NS>(m, T, h, cp);
+  double E = 0.5 * (u * u + v * v + w * w), imw = 0, cpt = 0;
+#pragma unroll
+  for (int l = 0; l < NS; ++l) {
+    const double y = x.sv[id + l * sz];
+    E += h[l] * y;
+    imw += y / m.mw[l];
+    cpt += y * cp[l];
+    b.cv[id + (5 + l) * sz] = rho * y;
+  }
+  b.cv[id] = rho;
+  b.cv[id + sz] = rho * u;
+  b.cv[id + 2 * sz] = rho * v;
+  b.cv[id + 3 * sz] = rho * w;
+  b.cv[id + 4 * sz] = E * rho - p;
+  const double R = kRu * imw;
+  x.snd[id] = sqrt(cpt / (cpt - R) * R * T);
+}
+
+__global__ void k_mix_cv_from_bv(DevBlock b, MixArrays x) {
+  const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= b.sz) return;
+  mix_cons(b, x, id);
+}
+
+// apply_outflow / apply_periodic with the species (BoundCond.cu:2065-2188, 3033-3083); work items as in
+// the single-species fill: one (face node, ghost layer) pair each.
+__global__ __launch_bounds__(256) void k_mix_boundary(DevBlock b, MixArrays x, DevFace f) {
+  const long n0 = f.end[0] - f.start[0] + 1, n1 = f.end[1] - f.start[1] + 1, n2 = f.end[2] - f.start[2] + 1;
+  const long npt = n0 * n1 * n2;
+  const long item = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (item >= npt * b.g) return;
+  int g;
+  long pt;
+  if (f.face == 0) {
+    g = (int)(item % b.g) + 1;
+    pt = item / b.g;
+  } else {
+    g = (int)(item / npt) + 1;
+    pt = item % npt;
+  }
+  const int i = f.start[0] + (int)(pt % n0), j = f.start[1] + (int)((pt / n0) % n1),
+            k = f.start[2] + (int)(pt / (n0 * n1));
+  int dv[3] = {0, 0, 0};
+  dv[f.face] = f.direction;
+  const long id = nid(b, i, j, k), sz = b.sz;
+  const long ig = nid(b, i + g * dv[0], j + g * dv[1], k + g * dv[2]);
+  long src = id;
+  if (f.type == 8) {
+    int o[3] = {i, j, k};
+    const int ext[3] = {b.mx, b.my, b.mz};
+    o[f.face] = f.direction < 0 ? ext[f.face] - 1 : 0;
+    if (g == 1) {
+      const long io = nid(b, o[0], o[1], o[2]);
+      for (int l = 0; l < 6; ++l) {
+        const double a = 0.5 * (b.bv[id + l * sz] + b.bv[io + l * sz]);
+        b.bv[id + l * sz] = a;
+        b.bv[io + l * sz] = a;
+      }
+      for (int l = 0; l < NS; ++l) {
+        const double a = 0.5 * (x.sv[id + l * sz] + x.sv[io + l * sz]);
+        x.sv[id + l * sz] = a;
+        x.sv[io + l * sz] = a;
+      }
+      mix_cons(b, x, id);
+    }
+    src = nid(b, o[0] + g * dv[0], o[1] + g * dv[1], o[2] + g * dv[2]);
+  }
+  for (int l = 0; l < 6; ++l) b.bv[ig + l * sz] = b.bv[src + l * sz];
+  for (int l = 0; l < NS; ++l) x.sv[ig + l * sz] = x.sv[src + l * sz];
+  mix_cons(b, x, ig);
+}
+
+// compute_entropy_fix_delta<Mixture> (InviscidScheme.cu:269-304), interior plus one layer.
+__global__ __launch_bounds__(256) void k_mix_entropy_fix(DevBlock b, MixArrays x, Scheme sc) {
+  const int i = blockIdx.x * 64 + threadIdx.x - 1, j = blockIdx.y * 4 + threadIdx.y - 1;
+  const int k = b.dim == 3 ? (int)blockIdx.z - 1 : 0;
+  if (i > b.mx || j > b.my) return;
+  const long id = nid(b, i, j, k), sz = b.sz;
+  const double *M = b.met + id;
+  const double u = b.bv[id + sz], v = b.bv[id + 2 * sz], w = b.bv[id + 3 * sz];
+  const double U = fabs(u * M[0] + v * M[sz] + w * M[2 * sz]);
+  const double V = fabs(u * M[3 * sz] + v * M[4 * sz] + w * M[5 * sz]);
+  const double W = fabs(u * M[6 * sz] + v * M[7 * sz] + w * M[8 * sz]);
+  const double gx = sqrt(M[0] * M[0] + M[sz] * M[sz] + M[2 * sz] * M[2 * sz]);
+  const double ge = sqrt(M[3 * sz] * M[3 * sz] + M[4 * sz] * M[4 * sz] + M[5 * sz] * M[5 * sz]);
+  const double gz = sqrt(M[6 * sz] * M[6 * sz] + M[7 * sz] * M[7 * sz] + M[8 * sz] * M[8 * sz]);
+  const double c = x.snd[id];
+  b.efd[id] = b.dim == 2 ? sc.entropy_fix_factor * (U + V + c * 0.5 * (gx + ge))
+                         : sc.entropy_fix_factor * (U + V + W + c * (gx + ge + gz) / 3.0);
+}
+
+__device__ __forceinline__ double minmod(double a, double c) {
+  const double sa = (a > 0) - (a < 0), sc = (c > 0) - (c < 0);
+  return 0.5 * (sa + sc) * fmin(fabs(a), fabs(c));
+}
+
+// WENO5-JS (WENO.cu:3059-3114) of one side: the reconstruction at the right edge of v[2] from v[0..4].
+__device__ __forceinline__ double weno5_left(const double *v, double eps) {
+  const double b0 = 13.0 / 12 * (v[0] - 2 * v[1] + v[2]) * (v[0] - 2 * v[1] + v[2]) +
+                    0.25 * (v[0] - 4 * v[1] + 3 * v[2]) * (v[0] - 4 * v[1] + 3 * v[2]);
+  const double b1 = 13.0 / 12 * (v[1] - 2 * v[2] + v[3]) * (v[1] - 2 * v[2] + v[3]) +
+                    0.25 * (v[1] - v[3]) * (v[1] - v[3]);
+  const double b2 = 13.0 / 12 * (v[2] - 2 * v[3] + v[4]) * (v[2] - 2 * v[3] + v[4]) +
+                    0.25 * (3 * v[2] - 4 * v[3] + v[4]) * (3 * v[2] - 4 * v[3] + v[4]);
+  const double a0 = 0.1 / ((b0 + eps) * (b0 + eps)), a1 = 0.6 / ((b1 + eps) * (b1 + eps)),
+               a2 = 0.3 / ((b2 + eps) * (b2 + eps));
+  const double q0 = (2 * v[0] - 7 * v[1] + 11 * v[2]) / 6, q1 = (-v[1] + 5 * v[2] + 2 * v[3]) / 6,
+               q2 = (2 * v[2] + 5 * v[3] - v[4]) / 6;
+  return (a0 * q0 + a1 * q1 + a2 * q2) / (a0 + a1 + a2);
+}
+
+// Primitive reconstruction at the interface between stencil points R-1 and R of pv[2R][NV].
+template<int REC, int R>
+__device__ __forceinline__ void reconstruct(const double (&pv)[2 * R][NV], double eps, double *pl, double *pr) {
+  constexpr int c = R - 1;
+#pragma unroll
+  for (int l = 0; l < NV; ++l) {
+    if constexpr (REC == 1) {
+      pl[l] = pv[c][l];
+      pr[l] = pv[c + 1][l];
+    } else if constexpr (REC == 2 || REC == 3) {
+      const double d0 = pv[c][l] - pv[c - 1][l], d1 = pv[c + 1][l] - pv[c][l], d2 = pv[c + 2][l] - pv[c + 1][l];
+      if constexpr (REC == 2) {
+        constexpr double kappa = 1.0 / 3.0;
+        pl[l] = pv[c][l] + 0.25 * ((1 - kappa) * minmod(d0, d1) + (1 + kappa) * minmod(d1, d0));
+        pr[l] = pv[c + 1][l] - 0.25 * ((1 - kappa) * minmod(d2, d1) + (1 + kappa) * minmod(d1, d2));
+      } else {
+        pl[l] = pv[c][l] + 0.5 * minmod(d0, d1);
+        pr[l] = pv[c + 1][l] - 0.5 * minmod(d1, d2);
+      }
+    } else {
+      const double vl[5] = {pv[0][l], pv[1][l], pv[2][l], pv[3][l], pv[4][l]};
+      const double vr[5] = {pv[5][l], pv[4][l], pv[3][l], pv[2][l], pv[1][l]};
+      pl[l] = weno5_left(vl, eps);
+      pr[l] = weno5_left(vr, eps);
+    }
+  }
+  if constexpr (REC >= 4) {
+    if (pl[0] <= 0 || pl[4] <= 0 || pr[0] <= 0 || pr[4] <= 0) {
+#pragma unroll
+      for (int l = 0; l < NV; ++l) {
+        pl[l] = pv[c][l];
+        pr[l] = pv[c + 1][l];
+      }
+    }
+  }
+}
+
+// reconstruction<Mixture> post-processing (InviscidScheme.cu:184-210): pv[NV] = E, pv[NV+1] = gamma.
+__device__ __forceinline__ void post(const ChemMech &m, double *pv) {
+  double e = 0.5 * (pv[1] * pv[1] + pv[2] * pv[2] + pv[3] * pv[3]), imw = 0;
+#pragma unroll
+  for (int l = 0; l < NS; ++l) imw += pv[5 + l] / m.mw[l];
+  const double t = pv[4] / (pv[0] * kRu * imw);
+  double h[NS], cp[NS];
+  chemdev::h_cp<NS>(m, t, h, cp);
+  double cpt = 0, cvt = 0;
+#pragma unroll
+  for (int l = 0; l < NS; ++l) {
+    cpt += cp[l] * pv[5 + l];
+    cvt += pv[5 + l] * (cp[l] - kRu / m.mw[l]);
+    e += h[l] * pv[5 + l];
+  }
+  pv[NV] = pv[0] * e - pv[4];
+  pv[NV + 1] = cpt / cvt;
+}
+
+// compute_half_sum_left_right_flux + riemannSolver_Roe<Mixture> (InviscidScheme.cu:601-765)
+__device__ void roe(const ChemMech &m, const double *pl, const double *pr, const double *ml, const double *mr,
+                    double jl, double jr, double efl, double efr, double *fc) {
+  double JKx = jl * ml[0], JKy = jl * ml[1], JKz = jl * ml[2];
+  double Uk = pl[1] * JKx + pl[2] * JKy + pl[3] * JKz;
+  double co = Uk * pl[0];
+  fc[0] = 0.5 * co;
+  fc[1] = 0.5 * (co * pl[1] + pl[4] * JKx);
+  fc[2] = 0.5 * (co * pl[2] + pl[4] * JKy);
+  fc[3] = 0.5 * (co * pl[3] + pl[4] * JKz);
+  fc[4] = 0.5 * Uk * (pl[4] + pl[NV]);
+#pragma unroll
+  for (int l = 5; l < NV; ++l) fc[l] = 0.5 * co * pl[l];
+  JKx = jr * mr[0];
+  JKy = jr * mr[1];
+  JKz = jr * mr[2];
+  Uk = pr[1] * JKx + pr[2] * JKy + pr[3] * JKz;
+  co = Uk * pr[0];
+  fc[0] += 0.5 * co;
+  fc[1] += 0.5 * (co * pr[1] + pr[4] * JKx);
+  fc[2] += 0.5 * (co * pr[2] + pr[4] * JKy);
+  fc[3] += 0.5 * (co * pr[3] + pr[4] * JKz);
+  fc[4] += 0.5 * Uk * (pr[4] + pr[NV]);
+#pragma unroll
+  for (int l = 5; l < NV; ++l) fc[l] += 0.5 * co * pr[l];
+
+  const double dl = sqrt(pl[0]), dr = sqrt(pr[0]);
+  const double inv = 1.0 / (dl + dr);
+  const double u = (dl * pl[1] + dr * pr[1]) * inv, v = (dl * pl[2] + dr * pr[2]) * inv,
+               w = (dl * pl[3] + dr * pr[3]) * inv;
+  const double ek = 0.5 * (u * u + v * v + w * w);
+  const double h = (dl * (pl[NV] + pl[4]) / pl[0] + dr * (pr[NV] + pr[4]) / pr[0]) * inv;
+  double svm[NS], mw_inv = 0;
+#pragma unroll
+  for (int l = 0; l < NS; ++l) {
+    svm[l] = (dl * pl[l + 5] + dr * pr[l + 5]) * inv;
+    mw_inv += svm[l] / m.mw[l];
+  }
+  const double t = (dl * pl[4] / pl[0] + dr * pr[4] / pr[0]) * inv / (kRu * mw_inv);
+  double hi[NS], cpi[NS];
+  chemdev::h_cp<NS>(m, t, hi, cpi);
+  double cvt = 0, cpt = 0;
+#pragma unroll
+  for (int l = 0; l < NS; ++l) {
+    cpt += cpi[l] * svm[l];
+    cvt += svm[l] * (cpi[l] - kRu / m.mw[l]);
+  }
+  const double gamma = cpt / cvt;
+  const double c = sqrt(gamma * kRu * mw_inv * t);
+  const double mw = 1.0 / mw_inv;
+  double kx = 0.5 * (ml[0] + mr[0]), ky = 0.5 * (ml[1] + mr[1]), kz = 0.5 * (ml[2] + mr[2]);
+  const double gradK = sqrt(kx * kx + ky * ky + kz * kz);
+  double U = kx * u + ky * v + kz * w;
+  double ch[3] = {U - gradK * c, U, U + gradK * c};
+  const double efa = 0.5 * (efl + efr);
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    ch[q] = fabs(ch[q]);
+    if (ch[q] < efa) ch[q] = 0.5 * (ch[q] * ch[q] / efa + efa);
+  }
+  kx /= gradK;
+  ky /= gradK;
+  kz /= gradK;
+  U /= gradK;
+  const double jave = 0.5 * (jl + jr);
+  double dq[NV];
+  dq[0] = jave * (pr[0] - pl[0]);
+#pragma unroll
+  for (int l = 1; l < NV; ++l) dq[l] = jave * (pr[0] * pr[l] - pl[0] * pl[l]);
+  dq[4] = jave * (pr[NV] - pl[NV]);
+  const double ic2 = 1.0 / (c * c);
+  double c1 = (gamma - 1) * (ek * dq[0] - u * dq[1] - v * dq[2] - w * dq[3] + dq[4]) / (c * c);
+  double c2 = (kx * dq[1] + ky * dq[2] + kz * dq[3] - U * dq[0]) / c;
+#pragma unroll
+  for (int l = 0; l < NS; ++l) c1 += (mw / m.mw[l] - hi[l] * (gamma - 1) * ic2) * dq[l + 5];
+  double c3 = dq[0] - c1;
+  double L[NV];
+  L[0] = 0.5 * (c1 - c2);
+  L[1] = kx * c3 - ((kz * v - ky * w) * dq[0] - kz * dq[2] + ky * dq[3]) / c;
+  L[2] = ky * c3 - ((kx * w - kz * u) * dq[0] - kx * dq[3] + kz * dq[1]) / c;
+  L[3] = kz * c3 - ((ky * u - kx * v) * dq[0] - ky * dq[1] + kx * dq[2]) / c;
+  L[4] = 0.5 * (c1 + c2);
+#pragma unroll
+  for (int l = 0; l < NS; ++l) L[l + 5] = dq[l + 5] - svm[l] * dq[0];
+  double bb[NV];
+  bb[0] = -ch[0] * L[0];
+#pragma unroll
+  for (int l = 1; l < NV; ++l) bb[l] = -ch[1] * L[l];
+  bb[4] = -ch[2] * L[4];
+  const double c0 = kx * bb[1] + ky * bb[2] + kz * bb[3];
+  c1 = c0 + bb[0] + bb[4];
+  c2 = c * (bb[4] - bb[0]);
+  c3 = 0;
+#pragma unroll
+  for (int l = 0; l < NS; ++l) c3 += (hi[l] - mw / m.mw[l] * c * c / (gamma - 1)) * bb[l + 5];
+  fc[0] += 0.5 * c1;
+  fc[1] += 0.5 * (u * c1 + kx * c2 - c * (kz * bb[2] - ky * bb[3]));
+  fc[2] += 0.5 * (v * c1 + ky * c2 - c * (kx * bb[3] - kz * bb[1]));
+  fc[3] += 0.5 * (w * c1 + kz * c2 - c * (ky * bb[1] - kx * bb[2]));
+  fc[4] += 0.5 * (h * c1 + U * c2 - c * c * c0 / (gamma - 1) +
+                  c * ((kz * v - ky * w) * bb[1] + (kx * w - kz * u) * bb[2] + (ky * u - kx * v) * bb[3]) + c3);
+#pragma unroll
+  for (int l = 0; l < NS; ++l) fc[5 + l] += 0.5 * (bb[l + 5] + svm[l] * c1);
+}
+
+// riemannSolver_ausmPlus<Mixture> (InviscidScheme.cu:397-469)
+__device__ void ausm(const double *pl, const double *pr, const double *ml, const double *mr, double jl, double jr,
+                     double *fc) {
+  const double k1 = 0.5 * (jl * ml[0] + jr * mr[0]), k2 = 0.5 * (jl * ml[1] + jr * mr[1]),
+               k3 = 0.5 * (jl * ml[2] + jr * mr[2]);
+  const double gk = sqrt(k1 * k1 + k2 * k2 + k3 * k3);
+  const double ul = (k1 * pl[1] + k2 * pl[2] + k3 * pl[3]) / gk, ur = (k1 * pr[1] + k2 * pr[2] + k3 * pr[3]) / gk;
+  const double c = 0.5 * (sqrt(pl[NV + 1] * pl[4] / pl[0]) + sqrt(pr[NV + 1] * pr[4] / pr[0]));
+  const double Ml = ul / c, Mr = ur / c, alpha = 3.0 / 16.0;
+  double mlp, mrn, plp, prn;
+  if (fabs(Ml) > 1) {
+    mlp = 0.5 * (Ml + fabs(Ml));
+    plp = mlp / Ml;
+  } else {
+    const double a = (Ml + 1) * (Ml + 1) * 0.25, q = (Ml * Ml - 1) * (Ml * Ml - 1);
+    mlp = a + 0.125 * q;
+    plp = a * (2 - Ml) + alpha * Ml * q;
+  }
+  if (fabs(Mr) > 1) {
+    mrn = 0.5 * (Mr - fabs(Mr));
+    prn = mrn / Mr;
+  } else {
+    const double a = (Mr - 1) * (Mr - 1) * 0.25, q = (Mr * Mr - 1) * (Mr * Mr - 1);
+    mrn = -a - 0.125 * q;
+    prn = a * (2 + Mr) - alpha * Mr * q;
+  }
+  const double pc = plp * pl[4] + prn * pr[4];
+  const double mh = mlp + mrn;
+  const double mf = c * (pl[0] * 0.5 * (mh + fabs(mh)) + pr[0] * 0.5 * (mh - fabs(mh)));
+  const double co = mf * gk;
+  double s[NV + 1];
+#pragma unroll
+  for (int l = 0; l < NV + 1; ++l) s[l] = mf >= 0 ? pl[l] : pr[l];
+  fc[0] = co;
+  fc[1] = co * s[1] + pc * k1;
+  fc[2] = co * s[2] + pc * k2;
+  fc[3] = co * s[3] + pc * k3;
+  fc[4] = co * (s[NV] + s[4]) / s[0];
+#pragma unroll
+  for (int l = 5; l < NV; ++l) fc[l] = co * s[l];
+}
+
+// riemannSolver_hllc<Mixture> (InviscidScheme.cu:471-599)
+__device__ void hllc(const ChemMech &m, const double *pl, const double *pr, const double *ml, const double *mr,
+                     double jl, double jr, double *fc) {
+  const double sl0 = sqrt(pl[0]), sr0 = sqrt(pr[0]);
+  const double rl = sl0 / (sl0 + sr0), rr = sr0 / (sl0 + sr0);
+  const double ut = pl[1] * rl + pr[1] * rr, vt = pl[2] * rl + pr[2] * rr, wt = pl[3] * rl + pr[3] * rr;
+  double svm[NS], mw_inv = 0;
+#pragma unroll
+  for (int l = 0; l < NS; ++l) {
+    svm[l] = rl * pl[l + 5] + rr * pr[l + 5];
+    mw_inv += svm[l] / m.mw[l];
+  }
+  const double t = (rl * pl[4] / pl[0] + rr * pr[4] / pr[0]) / (kRu * mw_inv);
+  double hi[NS], cpi[NS];
+  chemdev::h_cp<NS>(m, t, hi, cpi);
+  double cvt = 0, cpt = 0;
+#pragma unroll
+  for (int l = 0; l < NS; ++l) {
+    cpt += cpi[l] * svm[l];
+    cvt += svm[l] * (cpi[l] - kRu / m.mw[l]);
+  }
+  const double ct = sqrt(cpt / cvt * kRu * mw_inv * t);
+  const double kx = 0.5 * (ml[0] + mr[0]), ky = 0.5 * (ml[1] + mr[1]), kz = 0.5 * (ml[2] + mr[2]);
+  const double gk = sqrt(kx * kx + ky * ky + kz * kz);
+  const double Ub = (kx * ut + ky * vt + kz * wt) / gk;
+  const double ja = 0.5 * (jl + jr);
+  const double Ul = kx * pl[1] + ky * pl[2] + kz * pl[3], Ur = kx * pr[1] + ky * pr[2] + kz * pr[3];
+  const double cl = sqrt(pl[NV + 1] * pl[4] / pl[0]), cr = sqrt(pr[NV + 1] * pr[4] / pr[0]);
+  const double sl = fmin(Ul / gk - cl, Ub - ct), sr = fmax(Ur / gk + cr, Ub + ct);
+  double q[NV + 2];
+  if (sl >= 0 || sr < 0) {
+    const bool lft = sl >= 0;
+#pragma unroll
+    for (int l = 0; l < NV + 1; ++l) q[l] = lft ? pl[l] : pr[l]; // selects keep the indexing static
+    const double Uq = lft ? Ul : Ur;
+    const double rU = q[0] * Uq;
+    fc[0] = ja * rU;
+    fc[1] = ja * (rU * q[1] + q[4] * kx);
+    fc[2] = ja * (rU * q[2] + q[4] * ky);
+    fc[3] = ja * (rU * q[3] + q[4] * kz);
+    fc[4] = ja * ((q[NV] + q[4]) * Uq);
+#pragma unroll
+    for (int l = 5; l < NV; ++l) fc[l] = ja * rU * q[l];
+    return;
+  }
+  const double sm = ((pr[0] * Ur * (sr - Ur / gk) - pl[0] * Ul * (sl - Ul / gk)) / gk + pl[4] - pr[4]) /
+                    (pr[0] * (sr - Ur / gk) - pl[0] * (sl - Ul / gk));
+  const double pm = pl[0] * (sl - Ul / gk) * (sm - Ul / gk) + pl[4];
+  const bool left = sm >= 0;
+#pragma unroll
+  for (int l = 0; l < NV + 1; ++l) q[l] = left ? pl[l] : pr[l];
+  const double S = left ? sl : sr, Uq = left ? Ul : Ur;
+  const double pC = 1.0 / (S - sm);
+  const double QC = ja * pC * sm * (S * gk - Uq) * q[0];
+  fc[0] = QC;
+  const double dP = (S * pm - sm * q[4]) * pC * ja;
+  fc[1] = QC * q[1] + dP * kx;
+  fc[2] = QC * q[2] + dP * ky;
+  fc[3] = QC * q[3] + dP * kz;
+  fc[4] = QC * q[NV] / q[0] + pC * ja * (S * pm * sm * gk - sm * q[4] * Uq);
+#pragma unroll
+  for (int l = 5; l < NV; ++l) fc[l] = QC * q[l];
+}
+
+// Interface flux F(i+1/2) along DIR, stored at node i; interfaces -1/2 .. m-1/2.
+template<int DIR, int REC, int RS>
+__global__ __launch_bounds__(256) void k_mix_flux(DevBlock b, MixArrays x, Scheme sc, double *F) {
+  constexpr int R = REC >= 4 ? 3 : 2;
+  int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+  if constexpr (DIR == 0) i -= 1;
+  else if constexpr (DIR == 1) j -= 1;
+  else k -= 1;
+  if (i >= b.mx || j >= b.my) return;
+  const long st = DIR == 0 ? 1 : (DIR == 1 ? b.d2 : b.d1);
+  const long id = nid(b, i, j, k), sz = b.sz;
+  double pv[2 * R][NV];
+#pragma unroll
+  for (int q = 0; q < 2 * R; ++q) {
+    const long iq = id + (q - R + 1) * st;
+#pragma unroll
+    for (int l = 0; l < 5; ++l) pv[q][l] = b.bv[iq + l * sz];
+#pragma unroll
+    for (int l = 0; l < NS; ++l) pv[q][5 + l] = x.sv[iq + l * sz];
+  }
+  double pl[NV + 2], pr[NV + 2];
+  reconstruct<REC, R>(pv, sc.weno_eps, pl, pr);
+  const ChemMech &m = *x.mech;
+  post(m, pl);
+  post(m, pr);
+  double ml[3], mr[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    ml[c] = b.met[id + (3 * DIR + c) * sz];
+    mr[c] = b.met[id + st + (3 * DIR + c) * sz];
+  }
+  double fc[NV];
+  if constexpr (RS == 2) roe(m, pl, pr, ml, mr, b.jac[id], b.jac[id + st], b.efd[id], b.efd[id + st], fc);
+  else if constexpr (RS == 4) hllc(m, pl, pr, ml, mr, b.jac[id], b.jac[id + st], fc);
+  else ausm(pl, pr, ml, mr, b.jac[id], b.jac[id + st], fc);
+#pragma unroll
+  for (int l = 0; l < NV; ++l) F[id + l * sz] = fc[l];
+}
+
+__constant__ double kRkA[3] = {1.0, 0.75, 1.0 / 3.0}; // SSPRK3 (RK.cuh:11-15)
+__constant__ double kRkB[3] = {0.0, 0.25, 2.0 / 3.0};
+__constant__ double kRkC[3] = {1.0, 0.25, 2.0 / 3.0};
+
+// dq = -sum_dir (F(i+1/2) - F(i-1/2)) and update_cv_and_bv_rk<Mixture> (RK.cuh:224-280) with
+// compute_temperature_and_pressure (FieldOperation.cu:4-45).  Stage 0 saves U^n into qn.
+template<bool RHS_ONLY>
+__global__ __launch_bounds__(256) void k_mix_update(DevBlock b, MixArrays x, int rk, const double *__restrict__ dtp) {
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+  if (i >= b.mx || j >= b.my) return;
+  const long id = nid(b, i, j, k), sz = b.sz;
+  double r[NV];
+#pragma unroll
+  for (int l = 0; l < NV; ++l) {
+    double s = -(b.fx[id + l * sz] - b.fx[id - 1 + l * sz]) - (b.fy[id + l * sz] - b.fy[id - b.d2 + l * sz]);
+    if (b.dim == 3) s -= b.fz[id + l * sz] - b.fz[id - b.d1 + l * sz];
+    r[l] = s;
+  }
+  const long iq = ((long)k * b.my + j) * b.mx + i;
+  if (x.explicit_source) {
+    // finite_rate_chemistry with chemSrcMethod 0 (FiniteRateChem.cu:1388-1464): dq += J omega and the
+    // reaction time scale min_k rho Y_k / omega_d,k
+    const double rho = b.bv[id];
+    double ry[NS], w[NS], wd[NS];
+#pragma unroll
+    for (int l = 0; l < NS; ++l) ry[l] = rho * x.sv[id + l * sz];
+    chemdev::source<NS, false>(*x.mech, b.bv[id + 5 * sz], ry, w, nullptr, wd);
+    const double J = b.jac[id];
+    double ts = 1e+6;
+#pragma unroll
+    for (int l = 0; l < NS; ++l) {
+      r[5 + l] += J * w[l];
+      if (x.sv[id + l * sz] > 1e-25 && fabs(wd[l]) > 1e-25) ts = fmin(ts, fabs(ry[l] / wd[l]));
+    }
+    x.tscale[iq] = ts;
+  }
+  if constexpr (RHS_ONLY) {
+#pragma unroll
+    for (int l = 0; l < NV; ++l) b.dq[iq + l * b.n] = r[l];
+    return;
+  }
+  const double dj = dtp[0] / b.jac[id];
+  double q[NV];
+#pragma unroll
+  for (int l = 0; l < NV; ++l) {
+    const double u0 = b.cv[id + l * sz];
+    if (rk == 0) {
+      b.qn[id + l * sz] = u0;
+      q[l] = kRkA[0] * u0 + kRkB[0] * u0 + kRkC[0] * dj * r[l];
+    } else {
+      q[l] = kRkA[rk] * b.qn[id + l * sz] + kRkB[rk] * u0 + kRkC[rk] * dj * r[l];
+    }
+  }
+  if (b.mz == 1) q[3] = 0;
+  const double di = 1.0 / q[0];
+  const double u = q[1] * di, v = q[2] * di, w = q[3] * di;
+  double sum = 0;
+#pragma unroll
+  for (int l = 0; l < NS; ++l) {
+    if (q[5 + l] < 0) q[5 + l] = 0;
+    else sum += q[5 + l];
+  }
+  sum = 1.0 / sum;
+  const double f = q[0] * sum;
+  double y[NS];
+  const ChemMech &m = *x.mech;
+  double R = 0;
+#pragma unroll
+  for (int l = 0; l < NS; ++l) {
+    y[l] = q[5 + l] * sum;
+    q[5 + l] *= f;
+    R += y[l] * kRu / m.mw[l];
+  }
+#pragma unroll
+  for (int l = 0; l < NV; ++l) b.cv[id + l * sz] = q[l];
+  // Newton iteration for T at fixed e, |1 - T1/T| < 1e-5, at most 1000 iterations
+  const double e = q[4] * di - 0.5 * (u * u + v * v + w * w);
+  double t = b.bv[id + 5 * sz], err = 1;
+  for (int it = 0; it < 1000 && err > 1e-5; ++it) {
+    double h[NS], cp[NS];
+    chemdev::h_cp<NS>(m, t, h, cp);
+    double cpt = 0, hh = 0;
+#pragma unroll
+    for (int l = 0; l < NS; ++l) {
+      cpt += cp[l] * y[l];
+      hh += h[l] * y[l];
+    }
+    const double t1 = t - (hh - R * t - e) / (cpt - R);
+    err = fabs(1 - t1 / t);
+    t = t1;
+  }
+  b.bv[id] = q[0];
+  b.bv[id + sz] = u;
+  b.bv[id + 2 * sz] = v;
+  b.bv[id + 3 * sz] = w;
+  b.bv[id + 4 * sz] = q[0] * t * R;
+  b.bv[id + 5 * sz] = t;
+#pragma unroll
+  for (int l = 0; l < NS; ++l) x.sv[id + l * sz] = y[l];
+}
+
+// local_time_step<Mixture> (TimeAdvanceFunc.cuh:28-115, inviscid) fused with a workgroup minimum.
+__global__ __launch_bounds__(256) void k_mix_local_dt(DevBlock b, MixArrays x, Scheme sc, double *part) {
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+  double dt = 1e+6;
+  if (i < b.mx && j < b.my) {
+    const long id = nid(b, i, j, k), sz = b.sz;
+    const double *M = b.met + id;
+    const double gx = sqrt(M[0] * M[0] + M[sz] * M[sz] + M[2 * sz] * M[2 * sz]);
+    const double ge = sqrt(M[3 * sz] * M[3 * sz] + M[4 * sz] * M[4 * sz] + M[5 * sz] * M[5 * sz]);
+    const double gz = sqrt(M[6 * sz] * M[6 * sz] + M[7 * sz] * M[7 * sz] + M[8 * sz] * M[8 * sz]);
+    const double u = b.bv[id + sz], v = b.bv[id + 2 * sz], w = b.bv[id + 3 * sz];
+    const double c = x.snd[id];
+    double s = fmax(fabs(u * M[0] + v * M[sz] + w * M[2 * sz]) + c * gx,
+                    fabs(u * M[3 * sz] + v * M[4 * sz] + w * M[5 * sz]) + c * ge);
+    if (b.dim == 3) s = fmax(s, fabs(u * M[6 * sz] + v * M[7 * sz] + w * M[8 * sz]) + c * gz);
+    dt = sc.cfl / s;
+    if (x.explicit_source) {
+      const double rdt = sc.cfl * x.tscale[((long)k * b.my + j) * b.mx + i];
+      if (rdt > 1e-20 && rdt < 1e-5) dt = fmin(dt, rdt);
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) dt = fmin(dt, __shfl_xor(dt, off, 64));
+  __shared__ double wmin[4];
+  if (threadIdx.x == 0) wmin[threadIdx.y] = dt;
+  __syncthreads();
+  if (threadIdx.x == 0 && threadIdx.y == 0) {
+    const long p = ((long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+    part[p] = fmin(fmin(wmin[0], wmin[1]), fmin(wmin[2], wmin[3]));
+  }
+}
+
+__global__ __launch_bounds__(256) void k_mix_chem_gather(DevBlock b, MixArrays x, ChemBatch c) {
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+  if (i >= b.mx || j >= b.my) return;
+  const long id = nid(b, i, j, k), sz = b.sz;
+  const long q = ((long)k * b.my + j) * b.mx + i;
+  const double rho = b.cv[id];
+  const double u = b.bv[id + sz], v = b.bv[id + 2 * sz], w = b.bv[id + 3 * sz];
+  c.rho[q] = rho;
+  c.e[q] = b.cv[id + 4 * sz] / rho - 0.5 * (u * u + v * v + w * w);
+  c.T[q] = b.bv[id + 5 * sz];
+#pragma unroll
+  for (int l = 0; l < NS; ++l) c.rhoY[(long)l * c.n + q] = b.cv[id + (5 + l) * sz];
+}
+
+__global__ __launch_bounds__(256) void k_mix_chem_scatter(DevBlock b, MixArrays x, ChemBatch c) {
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+  if (i >= b.mx || j >= b.my) return;
+  const long id = nid(b, i, j, k), sz = b.sz;
+  const long q = ((long)k * b.my + j) * b.mx + i;
+  const ChemMech &m = *x.mech;
+  const double rho = b.cv[id], T = c.T[q];
+  double R = 0;
+#pragma unroll
+  for (int l = 0; l < NS; ++l) {
+    const double ry = c.rhoY[(long)l * c.n + q];
+    b.cv[id + (5 + l) * sz] = ry;
+    x.sv[id + l * sz] = ry / rho;
+    R += ry / rho * kRu / m.mw[l];
+  }
+  b.bv[id + 5 * sz] = T;
+  b.bv[id + 4 * sz] = rho * R * T;
+  double h[NS], cp[NS];
+  chemdev::h_cp<NS>(m, T, h, cp);
+  double cpt = 0;
+#pragma unroll
+  for (int l = 0; l < NS; ++l) cpt += cp[l] * x.sv[id + l * sz];
+  x.snd[id] = sqrt(cpt / (cpt - R) * R * T);
+}
+
+dim3 node_grid(const DevBlock &b) { return dim3((b.mx + 63) / 64, (b.my + 3) / 4, b.mz); }
+
+template<int DIR, int REC>
+void dispatch_rs(const DevBlock &b, const MixArrays &x, const Scheme &sc, dim3 grid, double *F, hipStream_t s) {
+  const dim3 tpb(64, 4, 1);
+  switch (sc.inviscid_scheme) {
+    case 2: k_mix_flux<DIR, REC, 2><<<grid, tpb, 0, s>>>(b, x, sc, F); break;
+    case 4: k_mix_flux<DIR, REC, 4><<<grid, tpb, 0, s>>>(b, x, sc, F); break;
+    default: k_mix_flux<DIR, REC, 3><<<grid, tpb, 0, s>>>(b, x, sc, F); break;
+  }
+}
+
+template<int DIR>
+void dispatch_rec(const DevBlock &b, const MixArrays &x, const Scheme &sc, dim3 grid, double *F, hipStream_t s) {
+  switch (sc.reconstruction) {
+    case 1: dispatch_rs<DIR, 1>(b, x, sc, grid, F, s); break;
+    case 2: dispatch_rs<DIR, 2>(b, x, sc, grid, F, s); break;
+    case 3: dispatch_rs<DIR, 3>(b, x, sc, grid, F, s); break;
+    default: dispatch_rs<DIR, 4>(b, x, sc, grid, F, s); break;
+  }
+}
+
+} // namespace
+
+hipError_t launch_mix_cv_from_bv_all(const DevBlock &b, const MixArrays &x, hipStream_t s) {
+  if (x.ns != NS) return hipErrorNotSupported;
+  k_mix_cv_from_bv<<<(unsigned)((b.sz + 255) / 256), 256, 0, s>>>(b, x);
+  return hipGetLastError();
+}
+
+hipError_t launch_mix_boundary(const DevBlock &b, const MixArrays &x, const DevFace &f, hipStream_t s) {
+  if (x.ns != NS) return hipErrorNotSupported;
+  const long n = (long)(f.end[0] - f.start[0] + 1) * (f.end[1] - f.start[1] + 1) * (f.end[2] - f.start[2] + 1) * b.g;
+  k_mix_boundary<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(b, x, f);
+  return hipGetLastError();
+}
+
+hipError_t launch_mix_entropy_fix(const DevBlock &b, const MixArrays &x, const Scheme &sc, hipStream_t s) {
+  const dim3 grid((b.mx + 2 + 63) / 64, (b.my + 2 + 3) / 4, b.dim == 3 ? b.mz + 2 : 1);
+  k_mix_entropy_fix<<<grid, dim3(64, 4, 1), 0, s>>>(b, x, sc);
+  return hipGetLastError();
+}
+
+hipError_t launch_mix_inviscid_flux(const DevBlock &b, const MixArrays &x, const Scheme &sc, int dir, hipStream_t s) {
+  if (x.ns != NS) return hipErrorNotSupported;
+  if (dir == 0) dispatch_rec<0>(b, x, sc, dim3((b.mx + 1 + 63) / 64, (b.my + 3) / 4, b.mz), b.fx, s);
+  else if (dir == 1) dispatch_rec<1>(b, x, sc, dim3((b.mx + 63) / 64, (b.my + 1 + 3) / 4, b.mz), b.fy, s);
+  else dispatch_rec<2>(b, x, sc, dim3((b.mx + 63) / 64, (b.my + 3) / 4, b.mz + 1), b.fz, s);
+  return hipGetLastError();
+}
+
+hipError_t launch_mix_update(const DevBlock &b, const MixArrays &x, int rk, const double *dt_dev, bool rhs_only,
+                             hipStream_t s) {
+  if (x.ns != NS) return hipErrorNotSupported;
+  if (rhs_only) k_mix_update<true><<<node_grid(b), dim3(64, 4, 1), 0, s>>>(b, x, rk, dt_dev);
+  else k_mix_update<false><<<node_grid(b), dim3(64, 4, 1), 0, s>>>(b, x, rk, dt_dev);
+  return hipGetLastError();
+}
+
+hipError_t launch_mix_local_dt(const DevBlock &b, const MixArrays &x, const Scheme &sc, double *part, hipStream_t s) {
+  k_mix_local_dt<<<node_grid(b), dim3(64, 4, 1), 0, s>>>(b, x, sc, part);
+  return hipGetLastError();
+}
+
+hipError_t launch_mix_chem_gather(const DevBlock &b, const MixArrays &x, const ChemBatch &c, hipStream_t s) {
+  if (x.ns != NS || c.n != b.n) return hipErrorInvalidValue;
+  k_mix_chem_gather<<<node_grid(b), dim3(64, 4, 1), 0, s>>>(b, x, c);
+  return hipGetLastError();
+}
+
+hipError_t launch_mix_chem_scatter(const DevBlock &b, const MixArrays &x, const ChemBatch &c, hipStream_t s) {
+  if (x.ns != NS || c.n != b.n) return hipErrorInvalidValue;
+  k_mix_chem_scatter<<<node_grid(b), dim3(64, 4, 1), 0, s>>>(b, x, c);
+  return hipGetLastError();
+}
+
+} // namespace corefl

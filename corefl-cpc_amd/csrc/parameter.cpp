@@ -28,6 +28,11 @@ Parameter::Parameter() {
   bools["positive_preserving"] = false;
   ints["species"] = 0;
   ints["reaction"] = 0;
+  ints["chemSrcMethod"] = 0;
+  strings["mechanism_file"] = "chemistry/mechanism.inp";
+  strings["therm_file"] = "chemistry/therm.dat";
+  reals["chemistry_rtol"] = 1e-6; // ROK4E tolerances of the split chemistry sub-steps (new)
+  reals["chemistry_atol"] = 1e-8;
   bools["turbulence"] = false;
   strings["default_init"] = "freestream";
   strings["reference_state"] = "freestream";

@@ -57,6 +57,7 @@ struct corefl_surrogate {
   SurrogateGate gate{};
   bool have_scaling = false, have_gate = false, have_params = false;
   int cus = 0;
+  double step_dt = 0; // uniform sub-step length of the current corefl_chem_surrogate_step call
   ~corefl_surrogate();
 };
 
@@ -180,6 +181,29 @@ void need_params(const corefl_surrogate *s) {
 }
 
 } // namespace
+
+namespace corefl {
+
+void surrogate_substep(corefl_chem *c, corefl_surrogate *s, const ChemBatch &b, double rtol, double atol,
+                       int max_attempts, hipStream_t st) {
+  need_params(s);
+  if (!s->have_scaling || !s->have_gate) throw std::runtime_error("surrogate scaling and gate must be set first");
+  const ChemMech &mech = c->mech;
+  if (mech.ne > 4) throw std::runtime_error("surrogate projection: at most 4 elements are supported");
+  ElementMatrix em{};
+  em.ne = mech.ne;
+  for (int e = 0; e < mech.ne; ++e)
+    for (int k = 0; k < mech.ns; ++k) em.E[e][k] = mech.comp[k][e] / mech.mw[k];
+  // dt of the launch: only used when the batch has neither per-cell nor device-side lengths
+  const double dt = s->step_dt;
+  HIPCHK(launch_mlp_chem(net_of(s), c->mech_dev, b, dt, s->sc, s->gate, em, c->list, c->list_count, s->cus, st));
+  ChemBatch fb = b;
+  fb.list = c->list;
+  fb.n_list = c->list_count;
+  HIPCHK(launch_chem_integrate(c->mech_dev, c->mech.ns, fb, dt, rtol, atol, max_attempts, c->cfg, c->counter, st));
+}
+
+} // namespace corefl
 
 corefl_surrogate::~corefl_surrogate() {
   for (void *q : {(void *)p, (void *)m, (void *)v, (void *)g, (void *)out, (void *)Yb, (void *)G4, (void *)Ga,
@@ -387,17 +411,11 @@ int corefl_chem_surrogate_step(corefl_chem *c, corefl_surrogate *s, double dt, d
       throw std::runtime_error("corefl_chem_surrogate_step: scaling and gate must be set first");
     chem_check_args(c, dt, rtol, atol, max_attempts);
     chem_ensure_device(c);
-    const ChemMech &mech = c->mech;
-    if (mech.ne > 4) throw std::runtime_error("corefl_chem_surrogate_step: at most 4 elements are supported");
-    ElementMatrix em{};
-    em.ne = mech.ne;
-    for (int e = 0; e < mech.ne; ++e)
-      for (int k = 0; k < mech.ns; ++k) em.E[e][k] = mech.comp[k][e] / mech.mw[k];
-    const hipStream_t st = chem_stream(c, stream);
     ChemBatch b = c->b;
     b.dtc = dt > 0 ? nullptr : c->dtc;
-    HIPCHK(launch_mlp_chem(net_of(s), c->mech_dev, b, dt, s->sc, s->gate, em, c->list, c->list_count, s->cus, st));
-    chem_integrate(c, c->cfg, dt, rtol, atol, max_attempts, st, c->list, c->list_count);
+    const hipStream_t st = chem_stream(c, stream);
+    s->step_dt = dt;
+    surrogate_substep(c, s, b, rtol, atol, max_attempts, st);
     if (n_fallback) {
       HIPCHK(hipMemcpyAsync(n_fallback, c->list_count, sizeof(int), hipMemcpyDeviceToHost, st));
       HIPCHK(hipStreamSynchronize(st));

@@ -243,7 +243,7 @@ __device__ __forceinline__ void store_hidden(unsigned short *h, const floatx16 *
 __device__ __forceinline__ bool chem_features(const FusedArgs &a, long c, float *f) {
   const ChemBatch &b = a.b;
   const ChemMech &m = *a.mech;
-  const double rho = b.rho[c], T = b.T[c], dt = b.dtc ? b.dtc[c] : a.dt;
+  const double rho = b.rho[c], T = b.T[c], dt = cell_dt(b, c, a.dt);
   double imw = 0.0;
   double y[9];
 #pragma unroll
@@ -291,7 +291,7 @@ __device__ __forceinline__ bool spd_solve4(double (&M)[4][4], double (&r)[4]) {
 __device__ void chem_finish(const FusedArgs &a, long c, const float *o, bool in_range) {
   const ChemBatch &b = a.b;
   const ChemMech &m = *a.mech;
-  const double rho = b.rho[c], T = b.T[c], dt = b.dtc ? b.dtc[c] : a.dt;
+  const double rho = b.rho[c], T = b.T[c], dt = cell_dt(b, c, a.dt);
   double y0[9], dy[9];
 #pragma unroll
   for (int k = 0; k < 9; ++k) {

@@ -22,6 +22,8 @@ extern "C" {
 #endif
 
 typedef struct corefl_ctx corefl_ctx;
+typedef struct corefl_chem corefl_chem;           /* chemistry context, declared below */
+typedef struct corefl_surrogate corefl_surrogate; /* surrogate context, declared below */
 
 /* Zero-copy view of one block's device arrays (the DZone of src/Field.h:12-110). */
 typedef struct {
@@ -40,6 +42,7 @@ enum {
   COREFL_CV = 0,
   COREFL_BV = 1,
   COREFL_DQ = 2,
+  COREFL_SV = 3,
   COREFL_JAC = 4,
   COREFL_METRIC = 5,
   COREFL_FV = 6,
@@ -68,8 +71,9 @@ int corefl_n_blocks(const corefl_ctx *ctx);
 int corefl_block_view_get(corefl_ctx *ctx, int block, corefl_block_view *view);
 /* Device-to-host copy of one array (see the COREFL_* selectors); n = element count of `host`. */
 int corefl_get_array(corefl_ctx *ctx, int block, int which, double *host, long long n);
-/* Host-to-device primitive state bv (6*sz, ghosts included); recomputes cv on all nodes and
- * applies the block's boundary conditions (src/FieldOperation.cuh:39-64, src/BoundCond.cu:3489-3619). */
+/* Host-to-device primitive state (ghosts included): bv (6*sz) followed, for mixtures, by the species mass
+ * fractions sv (n_spec*sz); recomputes cv on all nodes and applies the block's boundary conditions
+ * (src/FieldOperation.cuh:39-64, src/BoundCond.cu:3489-3619). */
 int corefl_set_state(corefl_ctx *ctx, int block, const double *bv_host, long long n);
 
 /* One right-hand-side evaluation dq on every block: viscous flux (src/ViscousScheme.cu:7-73) and
@@ -106,12 +110,25 @@ int corefl_profile_kernels(corefl_ctx *ctx, int reps, double *ms_out, int n_out)
 int corefl_get_int(const corefl_ctx *ctx, const char *name, int *out);
 int corefl_get_real(const corefl_ctx *ctx, const char *name, double *out);
 
+/* Mixtures (species = 1).  Setup keys: mechanism_file, therm_file (relative to the input directory, as
+ * src/ChemData.cpp:13,82 read them), reaction, temporal_scheme 3 with reaction = 1 (explicit chemical
+ * source in the residual, finite_rate_chemistry with chemSrcMethod 0, src/FiniteRateChem.cu:1388-1464)
+ * or temporal_scheme 5 (new: Strang splitting, chemistry dt/2 - SSP-RK3 step - chemistry dt/2, each
+ * chemistry sub-step the batched ROK4E of corefl_chem_integrate; keys chemistry_rtol, chemistry_atol).
+ * The borrowed chemistry context of a mixture setup, its batch filled with the interior cells of `block`
+ * (the state before the next step), for corefl_chem_autotune / corefl_chem_set_launch, which the split
+ * sub-steps then use.  Owned by ctx. */
+int corefl_chemistry(corefl_ctx *ctx, int block, corefl_chem **out);
+/* Split chemistry sub-steps through corefl_chem_surrogate_step's path (network, projection, gate, exact
+ * fallback) instead of the exact integrator; NULL restores the exact sub-steps.  s is not owned and must
+ * outlive its use.  No reference counterpart. */
+int corefl_set_chemistry_surrogate(corefl_ctx *ctx, corefl_surrogate *s);
+
 /* ---------------------------------------------------------------------------------------------------
  * Finite-rate chemistry of cell batches: CHEMKIN mechanism + NASA-7 thermo, the chemical source, and the
  * stiff per-cell sub-step (4-stage Rosenbrock ROK4E with embedded error control) batched over cells.
  * The context owns its device tables, a batch of cells (structure of arrays) and a stream.
  * ------------------------------------------------------------------------------------------------- */
-typedef struct corefl_chem corefl_chem;
 
 /* Device pointers of the current batch (n cells); rhoY is [k * n + cell]. */
 typedef struct {
@@ -190,7 +207,6 @@ int corefl_chem_synchronize(corefl_chem *chem);
  * trained on the GPU (forward, backward and Adam kernels of this library) on states labelled by the exact
  * integrator, and replaces the exact chemistry sub-step in cells that pass its gate.
  * ------------------------------------------------------------------------------------------------- */
-typedef struct corefl_surrogate corefl_surrogate;
 
 /* Number of parameters (51968 weights + 522 biases).  Flat layout: W_0 (12 x 128, row-major), b_0 (128),
  * W_1, b_1, W_2, b_2, W_3 (128 x 128), b_1..b_3 (128), W_4 (128 x 10), b_4 (10), in that order. */

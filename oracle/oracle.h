@@ -149,6 +149,8 @@ void orc_mix_set_state(orc_mix *b, const double *bv, const double *sv);
 void orc_mix_set_raw(orc_mix *b, const double *bv, const double *sv, const double *cv);
 /* which: 0 cv ((5+ns)*sz), 1 bv (6*sz), 2 dq ((5+ns)*n), 3 sv (ns*sz), 4 acoustic speed (sz) */
 void orc_mix_get(const orc_mix *b, int which, double *out);
+/* Add the explicit chemical source to the residual and limit the CFL step by the reaction time scale. */
+void orc_mix_set_explicit_source(orc_mix *b, int on);
 void orc_mix_rhs(orc_mix *b);
 double orc_mix_cfl_dt(orc_mix *b);
 void orc_mix_rk3_step(orc_mix *b, double dt);

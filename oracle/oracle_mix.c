@@ -32,7 +32,8 @@ struct orc_mix {
   long d2, d1, sz, n;
   mface_t faces[64];
   int n_face;
-  double *cv, *qn, *bv, *sv, *c, *efd, *dq, *jac, *met, *hstep;
+  int explicit_source; /* chemical source in the residual (reaction = 1, temporal_scheme 3) */
+  double *cv, *qn, *bv, *sv, *c, *efd, *dq, *jac, *met, *hstep, *tscale;
 };
 
 #define ID(b, i, j, k) ((long)((k) + (b)->g) * (b)->d1 + (long)((j) + (b)->g) * (b)->d2 + ((i) + (b)->g))
@@ -81,6 +82,8 @@ orc_mix *orc_mix_create(const orc_params *p, const orc_mech *m, int mx, int my, 
   b->efd = calloc((size_t)b->sz, sizeof(double));
   b->dq = calloc((size_t)b->nv * b->n, sizeof(double));
   b->hstep = calloc((size_t)b->n, sizeof(double));
+  b->tscale = malloc((size_t)b->n * sizeof(double));
+  for (long q = 0; q < b->n; ++q) b->tscale[q] = 1e+6;
   b->jac = malloc((size_t)b->sz * sizeof(double));
   b->met = malloc((size_t)9 * b->sz * sizeof(double));
   memcpy(b->jac, jac, (size_t)b->sz * sizeof(double));
@@ -91,6 +94,7 @@ orc_mix *orc_mix_create(const orc_params *p, const orc_mech *m, int mx, int my, 
 void orc_mix_destroy(orc_mix *b) {
   if (!b) return;
   free(b->cv); free(b->qn); free(b->bv); free(b->sv); free(b->c); free(b->efd); free(b->dq); free(b->hstep);
+  free(b->tscale);
   free(b->jac); free(b->met);
   free(b);
 }
@@ -567,12 +571,38 @@ static void inviscid_dir(orc_mix *b, int dir) {
     }
 }
 
+/* finite_rate_chemistry with chemSrcMethod 0 (FiniteRateChem.cu:1388-1464): dq += J omega, and the
+ * reaction time scale min_k rho Y_k / omega_d,k */
+static void explicit_source(orc_mix *b) {
+  const long s = b->sz;
+  const int ns = b->ns;
+#pragma omp parallel for collapse(2)
+  for (int k = 0; k < b->mz; ++k)
+    for (int j = 0; j < b->my; ++j)
+      for (int i = 0; i < b->mx; ++i) {
+        const long id = ID(b, i, j, k), iq = IDQ(b, i, j, k);
+        const double rho = b->bv[id];
+        double ry[MS], q1[ORC_MAX_REAC], q2[ORC_MAX_REAC], wd[MS], w[MS];
+        for (int l = 0; l < ns; ++l) ry[l] = rho * b->sv[id + l * s];
+        orc_chem_source(&b->m, b->bv[id + 5 * s], ry, q1, q2, wd, w);
+        double ts = 1e+6;
+        for (int l = 0; l < ns; ++l) {
+          b->dq[iq + (5 + l) * b->n] += b->jac[id] * w[l];
+          if (b->sv[id + l * s] > 1e-25 && fabs(wd[l]) > 1e-25) ts = fmin(ts, fabs(ry[l] / wd[l]));
+        }
+        b->tscale[iq] = ts;
+      }
+}
+
+void orc_mix_set_explicit_source(orc_mix *b, int on) { b->explicit_source = on; }
+
 void orc_mix_rhs(orc_mix *b) {
   memset(b->dq, 0, sizeof(double) * b->nv * b->n);
   if (b->p.inviscid_scheme == 2) entropy_fix(b);
   inviscid_dir(b, 0);
   inviscid_dir(b, 1);
   if (b->dim == 3) inviscid_dir(b, 2);
+  if (b->explicit_source) explicit_source(b);
 }
 
 static const double rk_a[3] = {1.0, 0.75, 1.0 / 3.0};
@@ -659,7 +689,11 @@ double orc_mix_cfl_dt(orc_mix *b) {
         const double c = b->c[id];
         double sr = fmax(fabs(U) + c * gx, fabs(V) + c * ge);
         if (b->dim == 3) sr = fmax(sr, fabs(W) + c * gz);
-        const double dt = b->p.cfl / sr;
+        double dt = b->p.cfl / sr;
+        if (b->explicit_source) { /* TimeAdvanceFunc.cuh:104-111 */
+          const double rdt = b->p.cfl * b->tscale[IDQ(b, i, j, k)];
+          if (rdt > 1e-20 && rdt < 1e-5) dt = fmin(dt, rdt);
+        }
         if (dt < dtmin) dtmin = dt;
       }
   return dtmin;

@@ -170,3 +170,92 @@ def write_input_dir(case, path):
             np.asarray(a, dtype=np.float64).ravel(order="F").tofile(f)
     with open(os.path.join(path, "boundary_condition", "boundary   0.txt"), "w") as f:
         f.write(case["boundary"])
+
+
+def reactive_box(n=17, recon=4, inviscid=2, temporal=5, reaction=1, cfl=0.5, length=1e-2, amp=0.0,
+                 fixed_dt=None):
+    """Periodic box of H2/air (Li 2004 mechanism), side `length` metres, n nodes per direction with
+    coincident end nodes; the initial field comes from reactive_state() through set_state."""
+    import chem_cases as CC
+    X, Y, Z = wavy_grid(n, amp)
+    scale = length / (2 * np.pi)
+    Yf = CC.Y_STOICH
+    species = ["H2", "H", "O2", "O", "OH", "HO2", "H2O2", "H2O", "N2"]
+    comp = "\n".join(f"    real {s} = {y!r}" for s, y in zip(species, Yf) if y > 0)
+    dt_lines = f"bool fixed_time_step = 1\nreal dt = {fixed_dt!r}" if fixed_dt else "bool fixed_time_step = 0"
+    setup = f"""// H2/air periodic box
+real gridScale = {scale!r}
+int gridIsBinary = 1
+bool steady = 0
+int species = 1
+int reaction = {reaction}
+string mechanism_file = {CC.MECH}
+string therm_file = {CC.THERM}
+int temporal_scheme = {temporal}
+{dt_lines}
+real cfl = {cfl}
+int reconstruction = {recon}
+int inviscid_scheme = {inviscid}
+int viscous_order = 0
+array string boundary_conditions {{
+    periodic
+}}
+struct periodic {{
+    string type = periodic
+    int label = 8
+}}
+string default_init = fresh
+string reference_state = fresh
+struct fresh {{
+    string type = inflow
+    int label = 5
+    real temperature = 300
+    real pressure = 101325
+    real velocity = 20
+{comp}
+}}
+"""
+    faces = box_faces(n, n, n, 8)
+    return dict(setup=setup, grid=(X, Y, Z), scale=scale, faces=faces, boundary=boundary_text(faces),
+                scheme=(inviscid, recon, 0), cfl=cfl, n=n)
+
+
+def reactive_state(n, ngg, seed=3, n_kernels=2, peak=(1800.0, 2400.0), radius=(2.0, 4.0), u_rms=20.0,
+                   k_max=3, p0=101325.0):
+    """Stoichiometric H2/air at p0 and 300 K with Gaussian hot kernels (peaks, radii in cells, centres
+    uniform in the periodic box) and a random-phase velocity field of |k| <= k_max rescaled to u_rms.  The
+    field is periodic with period n - 1 in the node index (coincident end nodes hold equal states) and
+    is given on every node of the padded block: returns bv (6, sz) and sv (9, sz)."""
+    import chem_cases as CC
+    rng = np.random.default_rng(seed)
+    per = n - 1
+    N = n + 2 * ngg
+    idx = (np.arange(N) - ngg) % per
+    kk, jj, ii = np.meshgrid(idx, idx, idx, indexing="ij")
+    T = np.full(kk.shape, 300.0)
+    centres = rng.uniform(0, per, (n_kernels, 3))
+    peaks = rng.uniform(*peak, n_kernels)
+    radii = rng.uniform(*radius, n_kernels)
+    for c, pk, r in zip(centres, peaks, radii):
+        d2 = 0
+        for a, q in ((ii, 0), (jj, 1), (kk, 2)):
+            d = np.abs(a - c[q])
+            d = np.minimum(d, per - d)
+            d2 = d2 + d * d
+        T += (pk - 300.0) * np.exp(-d2 / r ** 2)
+    T = np.minimum(T, peaks.max())
+    vel = np.zeros((3,) + kk.shape)
+    th = 2 * np.pi / per
+    for kx in range(-k_max, k_max + 1):
+        for ky in range(-k_max, k_max + 1):
+            for kz in range(-k_max, k_max + 1):
+                if kx * kx + ky * ky + kz * kz == 0 or kx * kx + ky * ky + kz * kz > k_max * k_max:
+                    continue
+                ph = th * (kx * ii + ky * jj + kz * kk)
+                for q in range(3):
+                    vel[q] += rng.standard_normal() * np.cos(ph + rng.uniform(0, 2 * np.pi))
+    vel *= u_rms / np.sqrt(np.mean(np.sum(vel ** 2, axis=0)))
+    Y = np.repeat(CC.Y_STOICH[:, None], T.size, axis=1)
+    rho = CC.density(p0, T.ravel(), Y)
+    bv = np.stack([rho, vel[0].ravel(), vel[1].ravel(), vel[2].ravel(), np.full(T.size, p0), T.ravel()])
+    return bv, Y

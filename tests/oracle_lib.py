@@ -263,6 +263,7 @@ def _mix_protos(lib):
     lib.orc_mix_set_raw.argtypes = [P, P, P, P]
     lib.orc_mix_get.argtypes = [P, I, P]
     lib.orc_mix_rhs.argtypes = [P]
+    lib.orc_mix_set_explicit_source.argtypes = [P, I]
     lib.orc_mix_cfl_dt.restype = D
     lib.orc_mix_cfl_dt.argtypes = [P]
     lib.orc_mix_rk3_step.argtypes = [P, D]
@@ -331,6 +332,9 @@ class MixBlock:
         out = np.zeros(n)
         load().orc_mix_get(self._h, which, _p(out))
         return out
+
+    def set_explicit_source(self, on=True):
+        load().orc_mix_set_explicit_source(self._h, 1 if on else 0)
 
     def rhs(self):
         load().orc_mix_rhs(self._h)

@@ -45,3 +45,20 @@ def rel_err(a, b):
         d = np.max(np.abs(a[l] - b[l]))
         out.append(d / scale[l] if scale[l] > 0 else d)
     return np.array(out)
+
+
+def make_mix_pair(case, seed=3, **state_kw):
+    """Mixture solver and oracle mixture block on the same grid, scheme and initial state."""
+    import chem_cases as CC
+    import chem_inputs as C
+    S = make_solver(case)
+    ngg = S.get_int("ngg")
+    X, Y, Z = cases.physical_grid(case)
+    inv, recon, _ = case["scheme"]
+    p = oracle_lib.params(inviscid=inv, recon=recon, visc=0, cfl=case["cfl"], eps=S.get_real("weno_eps"))
+    m = oracle_lib.mech_struct(C.parse(open(CC.MECH).read(), open(CC.THERM).read()))
+    B = oracle_lib.MixBlock(p, m, X, Y, Z, ngg, case["faces"])
+    bv, sv = cases.reactive_state(case["n"], ngg, seed=seed, **state_kw)
+    S.set_state(bv, sv=sv)
+    B.set_state(bv.ravel(), sv.ravel())
+    return S, B
