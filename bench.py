@@ -4,6 +4,13 @@ Workload: 257^3 nodes (256^3 unique: periodic faces are coincident end nodes), M
 Sutherland viscosity, Pr 0.72, characteristic WENO7 + AUSM+, 4th-order collocated viscous fluxes,
 CFL 0.5 step computed on the device each step, FP64.  Each step is replayed from one hipGraph.
 Prints one JSON line.
+
+Two further objects ride on the same line (skip with --no-reactive / --no-surrogate):
+  reactive   - the reacting H2/air periodic box (128^3 unique cells, 9 species, WENO5 + Roe, Strang-split
+               batched exact chemistry with the autotuned launch shape), Mcell-steps/s;
+  surrogate  - the chemistry surrogate trained on the GPU on exact-integrator states, then one chemistry
+               sub-step of 256^3 cells by surrogate (with fallback) and by the exact integrator: speedup,
+               max |dY|, max |dT| against the exact FP64 result, fallback fraction.
 """
 import argparse
 import json
@@ -67,6 +74,98 @@ def cpu_baseline(seconds=12.0, n=65):
                       f"{el:.1f} s"}
 
 
+def reactive_bench(n=129, steps=10, warmup=2):
+    """Strang-split reacting box: n^3 nodes ((n-1)^3 unique cells)."""
+    import cases
+    import corefl
+    log(f"bench: reactive box {n}^3 nodes")
+    case = cases.reactive_box(n, recon=4, inviscid=2, temporal=5, reaction=1, cfl=0.5)
+    S = corefl.Solver(setup_text=case["setup"], blocks=[case["grid"]], boundary_text=case["boundary"])
+    ngg = S.get_int("ngg")
+    # 16 hot kernels, radius 4-10 cells (as the reacting-box configuration), velocity 20 m/s rms
+    bv, sv = cases.reactive_state(n, ngg, seed=3, n_kernels=16, radius=(4.0, 10.0), k_max=8)
+    S.set_state(bv, sv=sv)
+    del bv, sv
+    dt = S.compute_dt()
+    ch = S.chemistry()
+    ms = ch.autotune(0.5 * dt, reps=1)
+    best = min(ms, key=ms.get)
+    log(f"bench: chemistry autotune best {best} {ms[best]:.2f} ms (of {len(ms)} shapes)")
+    S.advance(warmup, graph=True)
+    S.sync()
+    t0 = time.perf_counter()
+    S.advance(steps, graph=True)
+    S.sync()
+    el = time.perf_counter() - t0
+    unique = (n - 1) ** 3
+    bvo = S.get(corefl.BV)
+    T = bvo[5 * bvo.size // 6:]
+    out = {"workload": f"H2/air periodic box {n - 1}^3 unique cells, 9 species / 19 reactions, WENO5 + Roe, "
+                       "Strang-split batched ROK4E, 16 hot kernels, hipGraph replay",
+           "value": round(unique * steps / el / 1e6, 3), "unit": "Mcell-steps/s", "steps": steps,
+           "ms_per_step": round(el / steps * 1e3, 3), "chem_autotune": {"best": list(best), "ms": round(ms[best], 3)},
+           "T_max": round(float(np.max(T)), 1), "finite": bool(np.isfinite(bvo).all())}
+    log("bench: reactive " + json.dumps(out))
+    S.close()
+    return out
+
+
+def surrogate_bench(n_states=1000000, train_steps=4000, n_eval=256, dt=5e-8):
+    """Train the surrogate on exact-integrator states, then time one chemistry sub-step of n_eval^3 cells by
+    surrogate and by the exact integrator."""
+    import chem_cases as CC
+    import corefl
+    import surrogate_train as ST
+    t0 = time.perf_counter()
+    ch = corefl.Chemistry(CC.MECH, CC.THERM)
+    mw = ch.table("mw")
+    data = ST.build_dataset(ch, n_states, 4, CC.Y_STOICH)
+    t_data = time.perf_counter() - t0
+    log(f"bench: surrogate dataset {data['T'].size} states in {t_data:.1f} s")
+    sur = corefl.Surrogate(8192)
+    t0 = time.perf_counter()
+    sc, loss = ST.train(sur, data, mw, steps=train_steps)
+    t_train = time.perf_counter() - t0
+    ST.set_gate(sur, dT_max=2.0)
+    log(f"bench: surrogate trained in {t_train:.1f} s, loss {loss[:1]} -> {np.mean(loss[-50:]):.3e}")
+    del data
+    rho, T, Y = CC.hot_kernel_box(n_eval, seed=3, n_kernels=16)
+    ch.set_cells(rho, T, rho * Y)
+    for _ in range(10):  # 20 us of exact chemistry: the hot kernels are igniting
+        ch.integrate(2e-6)
+    g = ch.get_cells()
+    T1, Y1 = g["T"], g["rhoY"] / rho
+    del g
+    ch.set_cells(rho, T1, rho * Y1)
+    ms = ch.autotune(dt, reps=1)
+    exact_ms = []
+    for _ in range(3):
+        ch.set_cells(rho, T1, rho * Y1)
+        ch.sync()
+        t0 = time.perf_counter()
+        ch.integrate(dt)
+        exact_ms.append((time.perf_counter() - t0) * 1e3)
+    x = ch.get_cells()
+    sur_ms = []
+    for _ in range(3):
+        ch.set_cells(rho, T1, rho * Y1)
+        ch.sync()
+        t0 = time.perf_counter()
+        nf = sur.chem_step(ch, dt)
+        sur_ms.append((time.perf_counter() - t0) * 1e3)
+    s = ch.get_cells()
+    dy = float(np.max(np.abs(s["rhoY"] - x["rhoY"]) / rho))
+    dT = float(np.max(np.abs(s["T"] - x["T"])))
+    out = {"workload": f"{n_eval}^3 cells of the hot-kernel H2/air box after 20 us of ignition, dt {dt:g} s",
+           "speedup": round(min(exact_ms) / min(sur_ms), 3), "exact_ms": round(min(exact_ms), 3),
+           "surrogate_ms": round(min(sur_ms), 3), "max_abs_dY": dy, "max_abs_dT_K": dT,
+           "fallback_frac": round(nf / T1.size, 6), "train_states": int(n_states), "train_steps": train_steps,
+           "train_s": round(t_train, 1), "dataset_s": round(t_data, 1), "exact_launch": ch.get_launch(),
+           "final_loss": float(np.mean(loss[-50:]))}
+    log("bench: surrogate " + json.dumps(out))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -75,6 +174,8 @@ def main():
     ap.add_argument("--n", type=int, default=257, help="nodes per direction (n-1 unique)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-reps", type=int, default=3)
+    ap.add_argument("--no-reactive", action="store_true")
+    ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device copy (reference point for the HBM byte counters)")
     args = ap.parse_args()
@@ -127,6 +228,9 @@ def main():
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(symbol, n), "kernel": dom,
                 "algorithmic_bytes": FLUX_BYTES * faces}
 
+    S.close()
+    reactive = None if args.no_reactive else reactive_bench()
+    surrogate = None if args.no_surrogate else surrogate_bench()
     cpu = None
     if not args.no_cpu_baseline:
         log("bench: cpu baseline (oracle)")
@@ -143,6 +247,8 @@ def main():
         "kernel_ms": {k: round(v, 4) for k, v in ms.items()},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "reactive": reactive,
+        "surrogate": surrogate,
     }
     print(json.dumps(line), flush=True)
 

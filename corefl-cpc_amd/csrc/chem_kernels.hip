@@ -157,7 +157,7 @@ __device__ void rok4e_attempt(const ChemMech &m, Cell<NS> &s, double rtol, doubl
       yk[l] = ynew[l] * irho;
     }
     s.time += hs;
-    s.err_last = err;
+    s.err_last = fmax(err, 1e-30); // an exactly zero error would stall the PI controller (0^0.1 = 0)
     s.T = update_t<NS>(m, s.T, yk, s.E);
     ++s.steps;
   } else {

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <fstream>
@@ -341,6 +342,8 @@ void build(corefl_ctx *ctx, std::vector<corefl::HostBlock> blocks) {
     P.set_int("n_var", 5 + ctx->ns);
     ctx->chem_rtol = P.get_real("chemistry_rtol");
     ctx->chem_atol = P.get_real("chemistry_atol");
+    ctx->chem_max_attempts = P.get_int("chemistry_max_attempts");
+    if (ctx->chem_max_attempts < 1) throw std::runtime_error("chemistry_max_attempts must be at least 1");
     corefl::chem_ensure_device(ctx->chem);
   } else {
     if (rec != 4 && rec != 5) throw std::runtime_error("reconstruction must be 4 (WENO5) or 5 (WENO7) on this path");
@@ -448,6 +451,11 @@ void build(corefl_ctx *ctx, std::vector<corefl::HostBlock> blocks) {
     ctx->blocks.push_back(std::move(B));
   }
   ctx->parts = ctx->alloc(ctx->n_parts);
+  if (ctx->strang) { // the chemistry batch is allocated here, never inside a captured step
+    long n_max = 0;
+    for (auto &B : ctx->blocks) n_max = std::max(n_max, B.dev.n);
+    corefl::chem_ensure_batch(ctx->chem, n_max);
+  }
   // initial condition, then initialize_computation (Driver.cu:126-211)
   for (auto &B : ctx->blocks) {
     std::vector<double> sv;

@@ -542,6 +542,12 @@ __global__ __launch_bounds__(256) void k_mix_update(DevBlock b, MixArrays x, int
   b.bv[id + 5 * sz] = t;
 #pragma unroll
   for (int l = 0; l < NS; ++l) x.sv[id + l * sz] = y[l];
+  // acoustic speed of the new state (update_physical_properties<Mixture>, FieldOperation.cuh:86-120)
+  double h[NS], cp[NS], cpt = 0;
+  chemdev::h_cp<NS>(m, t, h, cp);
+#pragma unroll
+  for (int l = 0; l < NS; ++l) cpt += cp[l] * y[l];
+  x.snd[id] = sqrt(cpt / (cpt - R) * R * t);
 }
 
 // local_time_step<Mixture> (TimeAdvanceFunc.cuh:28-115, inviscid) fused with a workgroup minimum.

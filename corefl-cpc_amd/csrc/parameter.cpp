@@ -33,6 +33,7 @@ Parameter::Parameter() {
   strings["therm_file"] = "chemistry/therm.dat";
   reals["chemistry_rtol"] = 1e-6; // ROK4E tolerances of the split chemistry sub-steps (new)
   reals["chemistry_atol"] = 1e-8;
+  ints["chemistry_max_attempts"] = 100000; // per-cell step-attempt limit of a split chemistry sub-step (new)
   bools["turbulence"] = false;
   strings["default_init"] = "freestream";
   strings["reference_state"] = "freestream";

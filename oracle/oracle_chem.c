@@ -230,8 +230,11 @@ static const double E3 = 0.186511421161047;
  * from call to call.  Returns the number of accepted steps, or -1 when max_attempts (accepted + rejected)
  * attempts did not reach t_end; *n_rej gets the rejections.  If n_rec > 0, (time, T) after each accepted
  * step is written to rec[2*s], rec[2*s+1] for s < n_rec.
- * One deviation: a first-step error that is not a number is treated as err > 1 (step cut to 0.2 h); the
- * reference leaves h unchanged there (ConstVolumeReactor.cpp:421-425) and repeats the same attempt forever. */
+ * Two deviations, both where the reference's controller degenerates: a first-step error that is not a number
+ * is treated as err > 1 (step cut to 0.2 h), where the reference leaves h unchanged (ConstVolumeReactor.cpp:
+ * 421-425) and repeats the same attempt forever; and the error carried into the PI controller is floored at
+ * 1e-30, because a step with an exactly zero error (frozen chemistry) makes err_last^0.1 = 0 and the reference
+ * then shrinks every later accepted step by 0.2, so the sub-step never ends. */
 int orc_chem_rok4e(const orc_mech *m, double rho, double E, double *rhoY, double *T, double t_end, double rtol,
                    double atol, double *h, int max_attempts, int *n_rej, double *rec, int n_rec) {
   const int ns = m->ns, nr = m->nr;
@@ -293,7 +296,7 @@ int orc_chem_rok4e(const orc_mech *m, double rho, double E, double *rhoY, double
         yk[l] = ynew[l] / rho;
       }
       time += hs;
-      err_last = err;
+      err_last = fmax(err, 1e-30);
       *T = orc_chem_update_t(m, *T, yk, E);
       if (step < n_rec) {
         rec[2 * step] = time;

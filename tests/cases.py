@@ -181,7 +181,7 @@ def reactive_box(n=17, recon=4, inviscid=2, temporal=5, reaction=1, cfl=0.5, len
     scale = length / (2 * np.pi)
     Yf = CC.Y_STOICH
     species = ["H2", "H", "O2", "O", "OH", "HO2", "H2O2", "H2O", "N2"]
-    comp = "\n".join(f"    real {s} = {y!r}" for s, y in zip(species, Yf) if y > 0)
+    comp = "\n".join(f"    real {s} = {float(y)!r}" for s, y in zip(species, Yf) if y > 0)
     dt_lines = f"bool fixed_time_step = 1\nreal dt = {fixed_dt!r}" if fixed_dt else "bool fixed_time_step = 0"
     setup = f"""// H2/air periodic box
 real gridScale = {scale!r}
@@ -244,16 +244,16 @@ def reactive_state(n, ngg, seed=3, n_kernels=2, peak=(1800.0, 2400.0), radius=(2
             d2 = d2 + d * d
         T += (pk - 300.0) * np.exp(-d2 / r ** 2)
     T = np.minimum(T, peaks.max())
-    vel = np.zeros((3,) + kk.shape)
-    th = 2 * np.pi / per
-    for kx in range(-k_max, k_max + 1):
-        for ky in range(-k_max, k_max + 1):
-            for kz in range(-k_max, k_max + 1):
-                if kx * kx + ky * ky + kz * kz == 0 or kx * kx + ky * ky + kz * kz > k_max * k_max:
-                    continue
-                ph = th * (kx * ii + ky * jj + kz * kk)
-                for q in range(3):
-                    vel[q] += rng.standard_normal() * np.cos(ph + rng.uniform(0, 2 * np.pi))
+    # random-phase modes 0 < |k| <= k_max on the periodic index grid, synthesised by an inverse FFT
+    kf = np.fft.fftfreq(per, 1.0 / per)
+    KX, KY, KZ = np.meshgrid(kf, kf, kf, indexing="ij")
+    K2 = KX ** 2 + KY ** 2 + KZ ** 2
+    mask = (K2 > 0) & (K2 <= k_max * k_max)
+    base = []
+    for q in range(3):
+        spec = (rng.standard_normal(K2.shape) + 1j * rng.standard_normal(K2.shape)) * mask
+        base.append(np.real(np.fft.ifftn(spec)))  # axes (i, j, k)
+    vel = np.stack([b[ii, jj, kk] for b in base])
     vel *= u_rms / np.sqrt(np.mean(np.sum(vel ** 2, axis=0)))
     Y = np.repeat(CC.Y_STOICH[:, None], T.size, axis=1)
     rho = CC.density(p0, T.ravel(), Y)
