@@ -12,6 +12,7 @@ constexpr int kMaxSpec = 16, kMaxReac = 48, kMaxElem = 8, kMaxSide = 4;
 struct ChemMech {
   int ns, nr, ne;
   double mw[kMaxSpec];  // kg/kmol from atomic weights (Element.cpp)
+  double imw[kMaxSpec]; // 1 / mw
   double t_low[kMaxSpec], t_mid[kMaxSpec], t_high[kMaxSpec];
   double lo[kMaxSpec][7], hi[kMaxSpec][7];
   // 0 irreversible, 1 reversible, 3 merged DUPLICATE pair, 4 third body, 5 Lindemann, 6/7 Troe (3/4 params)

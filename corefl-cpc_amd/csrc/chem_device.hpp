@@ -13,45 +13,62 @@ namespace corefl {
 namespace chemdev {
 
 constexpr double kRu = 8314.462618, kPatm = 101325.0;
+// constant divisions of the polynomial fits as products (a few ulp from the divided form)
+constexpr double kThird = 1.0 / 3.0, kSixth = 1.0 / 6.0, kTwelfth = 1.0 / 12.0;
 
 // h_k and cp_k per unit mass, linear extrapolation outside [t_low, t_high] (Thermo.cu:470-520).
 template<int NS> __device__ __forceinline__ void h_cp(const ChemMech &m, double t, double *h, double *cp) {
   const double t2 = t * t, t3 = t2 * t, t4 = t3 * t, t5 = t4 * t;
 #pragma unroll
   for (int i = 0; i < NS; ++i) {
-    const double R = kRu / m.mw[i];
+    const double R = kRu * m.imw[i];
     const bool below = t < m.t_low[i], above = t > m.t_high[i];
     if (below || above) {
       const double tt = below ? m.t_low[i] : m.t_high[i];
       const double *c = below ? m.lo[i] : m.hi[i];
       const double tt2 = tt * tt, tt3 = tt2 * tt, tt4 = tt3 * tt, tt5 = tt4 * tt;
       const double cc = c[0] + c[1] * tt + c[2] * tt2 + c[3] * tt3 + c[4] * tt4;
-      const double hh = c[0] * tt + 0.5 * c[1] * tt2 + c[2] * tt3 / 3 + 0.25 * c[3] * tt4 + 0.2 * c[4] * tt5 + c[5];
+      const double hh = c[0] * tt + 0.5 * c[1] * tt2 + c[2] * tt3 * kThird + 0.25 * c[3] * tt4 + 0.2 * c[4] * tt5 + c[5];
       h[i] = (hh + cc * (t - tt)) * R;
       cp[i] = cc * R;
     } else {
       const double *c = t < m.t_mid[i] ? m.lo[i] : m.hi[i];
-      h[i] = (c[0] * t + 0.5 * c[1] * t2 + c[2] * t3 / 3 + 0.25 * c[3] * t4 + 0.2 * c[4] * t5 + c[5]) * R;
+      h[i] = (c[0] * t + 0.5 * c[1] * t2 + c[2] * t3 * kThird + 0.25 * c[3] * t4 + 0.2 * c[4] * t5 + c[5]) * R;
       cp[i] = (c[0] + c[1] * t + c[2] * t2 + c[3] * t3 + c[4] * t4) * R;
     }
   }
 }
 
-// g_k / (R T), clamped to the fit range (Thermo.cu:574-598).
-template<int NS> __device__ __forceinline__ void gibbs(const ChemMech &m, double t, double *g) {
+// g_k / (R T), clamped to the fit range (Thermo.cu:574-598).  lnT = log(t): one logarithm serves every
+// species whose fit range holds t.
+template<int NS>
+__device__ __forceinline__ void gibbs(const ChemMech &m, double t, double lnT, double iT, double *g) {
 #pragma unroll
   for (int i = 0; i < NS; ++i) {
     const bool below = t < m.t_low[i], above = t > m.t_high[i];
     const double tt = below ? m.t_low[i] : (above ? m.t_high[i] : t);
     const double *c = below ? m.lo[i] : (above ? m.hi[i] : (t < m.t_mid[i] ? m.lo[i] : m.hi[i]));
     const double tt2 = tt * tt, tt3 = tt2 * tt, tt4 = tt3 * tt;
-    g[i] = c[0] * (1.0 - log(tt)) - 0.5 * c[1] * tt - c[2] * tt2 / 6.0 - c[3] * tt3 / 12.0 - c[4] * tt4 * 0.05 +
-           c[5] / tt - c[6];
+    const double lt = (below || above) ? log(tt) : lnT;
+    const double itt = (below || above) ? 1.0 / tt : iT;
+    g[i] = c[0] * (1.0 - lt) - 0.5 * c[1] * tt - c[2] * tt2 * kSixth - c[3] * tt3 * kTwelfth - c[4] * tt4 * 0.05 +
+           c[5] * itt - c[6];
   }
 }
 
-__device__ __forceinline__ double arrhenius(double t, double A, double b, double Ea) {
-  return A * pow(t, b) * exp(-Ea / t);
+// A T^b exp(-Ea/T) as A exp(b ln T - Ea/T): one exponential per rate instead of a power and an exponential
+// (the same value up to rounding).
+__device__ __forceinline__ double arrhenius(double lnT, double iT, double A, double b, double Ea) {
+  return A * exp(b * lnT - Ea * iT);
+}
+
+// x^p for the small integer reaction orders, pow() otherwise.
+__device__ __forceinline__ double order_pow(double x, double p) {
+  const int k = (int)p;
+  if ((double)k != p || k > 4 || k < -4) return pow(x, p);
+  double r = 1.0;
+  for (int q = 0; q < (k < 0 ? -k : k); ++q) r *= x;
+  return k < 0 ? 1.0 / r : r;
 }
 
 __device__ __forceinline__ double mass_action(double c, double nu) {
@@ -70,7 +87,7 @@ __device__ void source(const ChemMech &m, double t, const double *rhoY, double *
   double c[NS], g[NS], cr[NS], de[NS];
 #pragma unroll
   for (int l = 0; l < NS; ++l) {
-    c[l] = rhoY[l] / m.mw[l] * 1e-3;
+    c[l] = rhoY[l] * m.imw[l] * 1e-3;
     cr[l] = 0;
     de[l] = 0;
   }
@@ -80,22 +97,23 @@ __device__ void source(const ChemMech &m, double t, const double *rhoY, double *
   }
   // Equilibrium constants are needed unless every reversible reaction carries explicit REV parameters
   // (FiniteRateChem.cu:1519-1557); then they are applied to every reaction of type other than 0 and 2.
-  if (m.need_gibbs) gibbs<NS>(m, t, g);
+  const double lnT = log(t), iT = 1.0 / t;
+  if (m.need_gibbs) gibbs<NS>(m, t, lnT, iT, g);
   const double tp = kPatm / kRu * 1e-3 / t;
   for (int r = 0; r < m.nr; ++r) {
     const int ty = m.type[r];
-    double kf = arrhenius(t, m.A[r], m.b[r], m.Ea[r]);
+    double kf = arrhenius(lnT, iT, m.A[r], m.b[r], m.Ea[r]);
     double cc = 0;
     if (ty >= 4) {
 #pragma unroll
       for (int l = 0; l < NS; ++l) cc += c[l] * m.tb[r][l];
     }
     if (ty == 3) {
-      kf += arrhenius(t, m.A2[r], m.b2[r], m.Ea2[r]);
+      kf += arrhenius(lnT, iT, m.A2[r], m.b2[r], m.Ea2[r]);
     } else if (ty == 4) {
       kf *= cc;
     } else if (ty > 4) {
-      const double klow = arrhenius(t, m.A2[r], m.b2[r], m.Ea2[r]), khigh = kf;
+      const double klow = arrhenius(lnT, iT, m.A2[r], m.b2[r], m.Ea2[r]), khigh = kf;
       if (khigh < 1e-25 && klow < 1e-25) {
         kf = 0;
       } else {
@@ -108,21 +126,21 @@ __device__ void source(const ChemMech &m, double t, const double *rhoY, double *
           const double cq = -0.4 - 0.67 * lfc, nq = 0.75 - 1.27 * lfc;
           const double lpr = log10(pr);
           const double tmp = (lpr + cq) / (nq - 0.14 * (lpr + cq));
-          F = pow(10.0, lfc / (1.0 + tmp * tmp));
+          F = exp(2.302585092994046 * lfc / (1.0 + tmp * tmp)); // 10^x
         }
         kf = khigh * pr / (1.0 + pr) * F;
       }
     }
     double kb = 0;
     if (ty != 0 && m.rev[r] == 1) {
-      kb = arrhenius(t, m.A2[r], m.b2[r], m.Ea2[r]);
+      kb = arrhenius(lnT, iT, m.A2[r], m.b2[r], m.Ea2[r]);
       if (ty == 4) kb *= cc;
     }
     if (m.need_gibbs && ty != 0 && ty != 2) {
       double dg = 0;
 #pragma unroll
       for (int l = 0; l < NS; ++l) dg += g[l] * (m.nb[r][l] - m.nf[r][l]);
-      kb = kf / (pow(tp, m.order[r]) * exp(-dg));
+      kb = kf / (order_pow(tp, m.order[r]) * exp(-dg));
     }
     double q1 = 1.0, q2 = ty != 0 ? 1.0 : 0.0;
 #pragma unroll
@@ -166,7 +184,7 @@ __device__ void source(const ChemMech &m, double t, const double *rhoY, double *
 template<int NS> __device__ double update_t(const ChemMech &m, double t, const double *yk, double E) {
   double imw = 0;
 #pragma unroll
-  for (int l = 0; l < NS; ++l) imw += yk[l] / m.mw[l];
+  for (int l = 0; l < NS; ++l) imw += yk[l] * m.imw[l];
   const double R = kRu * imw;
   double err = 1e+6;
   for (int it = 0; it < 100 && err > 1e-8; ++it) {

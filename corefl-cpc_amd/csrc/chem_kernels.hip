@@ -58,6 +58,7 @@ template<int NS> __device__ __forceinline__ void lu(double *a, int *piv) {
     const double inv = 1.0 / a[k * NS + k];
 #pragma unroll
     for (int r = k + 1; r < NS; ++r) a[r * NS + k] *= inv;
+    a[k * NS + k] = inv; // the solve multiplies by the reciprocal pivot
   }
 }
 
@@ -76,12 +77,12 @@ template<int NS> __device__ __forceinline__ void lu_solve(const double *a, const
   for (int r = 1; r < NS; ++r)
 #pragma unroll
     for (int t = 0; t < r; ++t) b[r] -= a[r * NS + t] * b[t];
-  b[NS - 1] /= a[NS * NS - 1];
+  b[NS - 1] *= a[NS * NS - 1];
 #pragma unroll
   for (int r = NS - 2; r >= 0; --r) {
 #pragma unroll
     for (int t = r + 1; t < NS; ++t) b[r] -= a[r * NS + t] * b[t];
-    b[r] /= a[r * NS + r];
+    b[r] *= a[r * NS + r];
   }
 }
 
@@ -94,7 +95,7 @@ template<int NS> struct Cell {
 
 // One ROK4E attempt (ConstVolumeReactor.cpp:173-476 with adaptive_time_step = 1).
 template<int NS>
-__device__ void rok4e_attempt(const ChemMech &m, Cell<NS> &s, double rtol, double atol) {
+__device__ __forceinline__ void rok4e_attempt(const ChemMech &m, Cell<NS> &s, double rtol, double atol) {
   const double dt = s.dt;
   double w[NS], J[NS * NS], k1[NS], k2[NS], k3[NS], k4[NS], ys[NS], yk[NS];
   int piv[NS];
@@ -108,7 +109,7 @@ __device__ void rok4e_attempt(const ChemMech &m, Cell<NS> &s, double rtol, doubl
 #pragma unroll
   for (int l = 0; l < NS; ++l) k1[l] = w[l];
   lu_solve<NS>(J, piv, k1);
-  const double irho = 1.0 / s.rho;
+  const double irho = 1.0 / s.rho; // products by 1/rho stand for the reference's divisions
 #pragma unroll
   for (int l = 0; l < NS; ++l) {
     ys[l] = s.y[l] + hs * A21 * k1[l];
@@ -278,7 +279,7 @@ template<int NS> __global__ __launch_bounds__(256) void k_chem_energy(const Chem
   for (int l = 0; l < NS; ++l) {
     const double y = b.rhoY[(long)l * b.n + c] * irho;
     e += h[l] * y;
-    imw += y / m.mw[l];
+    imw += y * m.imw[l];
   }
   b.e[c] = e - kRu * imw * t;
 }

@@ -313,6 +313,7 @@ ChemMech parse_mechanism(const std::string &mech_text, const std::string &therm_
     ++nr;
   }
   m.nr = nr;
+  for (int k = 0; k < m.ns; ++k) m.imw[k] = 1.0 / m.mw[k];
   m.need_gibbs = 0;
   for (int r = 0; r < nr; ++r) m.need_gibbs |= m.type[r] != 0 && m.rev[r] != 1;
   return m;

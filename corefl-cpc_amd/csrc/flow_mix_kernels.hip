@@ -31,7 +31,7 @@ __device__ void mix_cons(const DevBlock &b, const MixArrays &x, long id) {
   for (int l = 0; l < NS; ++l) {
     const double y = x.sv[id + l * sz];
     E += h[l] * y;
-    imw += y / m.mw[l];
+    imw += y * m.imw[l];
     cpt += y * cp[l];
     b.cv[id + (5 + l) * sz] = rho * y;
   }
@@ -178,7 +178,7 @@ __device__ __forceinline__ void reconstruct(const double (&pv)[2 * R][NV], doubl
 __device__ __forceinline__ void post(const ChemMech &m, double *pv) {
   double e = 0.5 * (pv[1] * pv[1] + pv[2] * pv[2] + pv[3] * pv[3]), imw = 0;
 #pragma unroll
-  for (int l = 0; l < NS; ++l) imw += pv[5 + l] / m.mw[l];
+  for (int l = 0; l < NS; ++l) imw += pv[5 + l] * m.imw[l];
   const double t = pv[4] / (pv[0] * kRu * imw);
   double h[NS], cp[NS];
   chemdev::h_cp<NS>(m, t, h, cp);
@@ -186,7 +186,7 @@ __device__ __forceinline__ void post(const ChemMech &m, double *pv) {
 #pragma unroll
   for (int l = 0; l < NS; ++l) {
     cpt += cp[l] * pv[5 + l];
-    cvt += pv[5 + l] * (cp[l] - kRu / m.mw[l]);
+    cvt += pv[5 + l] * (cp[l] - kRu * m.imw[l]);
     e += h[l] * pv[5 + l];
   }
   pv[NV] = pv[0] * e - pv[4];
@@ -229,7 +229,7 @@ __device__ void roe(const ChemMech &m, const double *pl, const double *pr, const
 #pragma unroll
   for (int l = 0; l < NS; ++l) {
     svm[l] = (dl * pl[l + 5] + dr * pr[l + 5]) * inv;
-    mw_inv += svm[l] / m.mw[l];
+    mw_inv += svm[l] * m.imw[l];
   }
   const double t = (dl * pl[4] / pl[0] + dr * pr[4] / pr[0]) * inv / (kRu * mw_inv);
   double hi[NS], cpi[NS];
@@ -238,7 +238,7 @@ __device__ void roe(const ChemMech &m, const double *pl, const double *pr, const
 #pragma unroll
   for (int l = 0; l < NS; ++l) {
     cpt += cpi[l] * svm[l];
-    cvt += svm[l] * (cpi[l] - kRu / m.mw[l]);
+    cvt += svm[l] * (cpi[l] - kRu * m.imw[l]);
   }
   const double gamma = cpt / cvt;
   const double c = sqrt(gamma * kRu * mw_inv * t);
@@ -267,7 +267,7 @@ __device__ void roe(const ChemMech &m, const double *pl, const double *pr, const
   double c1 = (gamma - 1) * (ek * dq[0] - u * dq[1] - v * dq[2] - w * dq[3] + dq[4]) / (c * c);
   double c2 = (kx * dq[1] + ky * dq[2] + kz * dq[3] - U * dq[0]) / c;
 #pragma unroll
-  for (int l = 0; l < NS; ++l) c1 += (mw / m.mw[l] - hi[l] * (gamma - 1) * ic2) * dq[l + 5];
+  for (int l = 0; l < NS; ++l) c1 += (mw * m.imw[l] - hi[l] * (gamma - 1) * ic2) * dq[l + 5];
   double c3 = dq[0] - c1;
   double L[NV];
   L[0] = 0.5 * (c1 - c2);
@@ -287,7 +287,7 @@ __device__ void roe(const ChemMech &m, const double *pl, const double *pr, const
   c2 = c * (bb[4] - bb[0]);
   c3 = 0;
 #pragma unroll
-  for (int l = 0; l < NS; ++l) c3 += (hi[l] - mw / m.mw[l] * c * c / (gamma - 1)) * bb[l + 5];
+  for (int l = 0; l < NS; ++l) c3 += (hi[l] - mw * m.imw[l] * c * c / (gamma - 1)) * bb[l + 5];
   fc[0] += 0.5 * c1;
   fc[1] += 0.5 * (u * c1 + kx * c2 - c * (kz * bb[2] - ky * bb[3]));
   fc[2] += 0.5 * (v * c1 + ky * c2 - c * (kx * bb[3] - kz * bb[1]));
@@ -350,7 +350,7 @@ __device__ void hllc(const ChemMech &m, const double *pl, const double *pr, cons
 #pragma unroll
   for (int l = 0; l < NS; ++l) {
     svm[l] = rl * pl[l + 5] + rr * pr[l + 5];
-    mw_inv += svm[l] / m.mw[l];
+    mw_inv += svm[l] * m.imw[l];
   }
   const double t = (rl * pl[4] / pl[0] + rr * pr[4] / pr[0]) / (kRu * mw_inv);
   double hi[NS], cpi[NS];
@@ -359,7 +359,7 @@ __device__ void hllc(const ChemMech &m, const double *pl, const double *pr, cons
 #pragma unroll
   for (int l = 0; l < NS; ++l) {
     cpt += cpi[l] * svm[l];
-    cvt += svm[l] * (cpi[l] - kRu / m.mw[l]);
+    cvt += svm[l] * (cpi[l] - kRu * m.imw[l]);
   }
   const double ct = sqrt(cpt / cvt * kRu * mw_inv * t);
   const double kx = 0.5 * (ml[0] + mr[0]), ky = 0.5 * (ml[1] + mr[1]), kz = 0.5 * (ml[2] + mr[2]);
@@ -514,7 +514,7 @@ __global__ __launch_bounds__(256) void k_mix_update(DevBlock b, MixArrays x, int
   for (int l = 0; l < NS; ++l) {
     y[l] = q[5 + l] * sum;
     q[5 + l] *= f;
-    R += y[l] * kRu / m.mw[l];
+    R += y[l] * kRu * m.imw[l];
   }
 #pragma unroll
   for (int l = 0; l < NV; ++l) b.cv[id + l * sz] = q[l];
@@ -608,7 +608,7 @@ __global__ __launch_bounds__(256) void k_mix_chem_scatter(DevBlock b, MixArrays 
     const double ry = c.rhoY[(long)l * c.n + q];
     b.cv[id + (5 + l) * sz] = ry;
     x.sv[id + l * sz] = ry / rho;
-    R += ry / rho * kRu / m.mw[l];
+    R += ry / rho * kRu * m.imw[l];
   }
   b.bv[id + 5 * sz] = T;
   b.bv[id + 4 * sz] = rho * R * T;

@@ -249,7 +249,7 @@ __device__ __forceinline__ bool chem_features(const FusedArgs &a, long c, float 
 #pragma unroll
   for (int k = 0; k < 9; ++k) {
     y[k] = b.rhoY[(long)k * b.n + c] / rho;
-    imw += y[k] / m.mw[k];
+    imw += y[k] * m.imw[k];
   }
   const double p = rho * chemdev::kRu * imw * T;
   const SurrogateScaling &s = a.sc;
