@@ -25,7 +25,8 @@ SYMBOLS = (
     "corefl_chem_source", "corefl_chem_integrate", "corefl_chem_set_launch", "corefl_chem_get_launch",
     "corefl_chem_autotune", "corefl_chem_synchronize",
     "corefl_surrogate_n_params", "corefl_surrogate_create", "corefl_surrogate_destroy", "corefl_surrogate_set_params",
-    "corefl_surrogate_get_params", "corefl_surrogate_set_scaling", "corefl_surrogate_set_gate",
+    "corefl_surrogate_get_params", "corefl_surrogate_set_scaling", "corefl_surrogate_set_transform",
+    "corefl_surrogate_set_gate",
     "corefl_surrogate_set_data", "corefl_surrogate_train", "corefl_surrogate_grad", "corefl_surrogate_forward",
     "corefl_chem_surrogate_step",
 )
@@ -105,6 +106,7 @@ def load_library(path=LIB_PATH):
     lib.corefl_surrogate_get_params.argtypes = [P, P, LL]
     lib.corefl_surrogate_set_scaling.argtypes = [P, P, P, P, P]
     lib.corefl_surrogate_set_gate.argtypes = [P, P]
+    lib.corefl_surrogate_set_transform.argtypes = [P, D, P]
     lib.corefl_surrogate_set_data.argtypes = [P, LL, P, P]
     lib.corefl_surrogate_train.argtypes = [P, I, I, F, ctypes.c_ulonglong, P]
     lib.corefl_surrogate_grad.argtypes = [P, LL, P, P, P, P]
@@ -440,8 +442,12 @@ class Surrogate:
         arrs = [np.ascontiguousarray(a, np.float64) for a in (in_mu, in_sig, out_mu, out_sig)]
         _check(self._lib.corefl_surrogate_set_scaling(self._ctx, *[a.ctypes.data for a in arrs]))
 
-    def set_gate(self, t_min, t_max, p_min, p_max, dt_min, dt_max, dT_max):
-        g = np.array([t_min, t_max, p_min, p_max, dt_min, dt_max, dT_max], dtype=np.float64)
+    def set_transform(self, y_power=1.0, out_r0=None):
+        r0 = None if out_r0 is None else np.ascontiguousarray(out_r0, np.float64)
+        _check(self._lib.corefl_surrogate_set_transform(self._ctx, float(y_power), None if r0 is None else r0.ctypes.data))
+
+    def set_gate(self, t_min, t_max, p_min, p_max, dt_min, dt_max, dT_max, dT_step=1e30):
+        g = np.array([t_min, t_max, p_min, p_max, dt_min, dt_max, dT_max, dT_step], dtype=np.float64)
         _check(self._lib.corefl_surrogate_set_gate(self._ctx, g.ctypes.data))
 
     def set_data(self, X, Y):

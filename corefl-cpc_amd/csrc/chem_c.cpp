@@ -49,12 +49,16 @@ template<class F> int guarded(F &&f) {
 
 void free_batch(corefl_chem *c) {
   for (void *p : {(void *)c->b.rho, (void *)c->b.e, (void *)c->b.T, (void *)c->b.rhoY, (void *)c->b.h,
-                  (void *)c->b.steps, (void *)c->b.rejects, (void *)c->save, (void *)c->dtc, (void *)c->list})
+                  (void *)c->b.steps, (void *)c->b.rejects, (void *)c->save, (void *)c->dtc, (void *)c->list,
+                  (void *)c->sfeat, (void *)c->sout, (void *)c->sflag})
     if (p) (void)hipFree(p);
   c->b = corefl::ChemBatch{};
   c->save = nullptr;
   c->dtc = nullptr;
   c->list = nullptr;
+  c->sfeat = nullptr;
+  c->sout = nullptr;
+  c->sflag = nullptr;
   c->have_dtc = false;
   c->cap = 0;
 }
@@ -97,6 +101,9 @@ void chem_ensure_batch(corefl_chem *c, long long n) {
   HIPCHK(hipMalloc(&c->b.rejects, sizeof(int) * (size_t)n));
   HIPCHK(hipMalloc(&c->dtc, d));
   HIPCHK(hipMalloc(&c->list, sizeof(int) * (size_t)n));
+  HIPCHK(hipMalloc(&c->sfeat, sizeof(unsigned short) * 16 * (size_t)n));
+  HIPCHK(hipMalloc(&c->sout, sizeof(float) * 10 * (size_t)n));
+  HIPCHK(hipMalloc(&c->sflag, (size_t)n));
   c->cap = (int)n;
   c->b.n = (int)n;
 }

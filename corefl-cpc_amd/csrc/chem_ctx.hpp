@@ -27,6 +27,9 @@ struct corefl_chem {
   double *dtc = nullptr;  // per-cell sub-step lengths (capacity cap)
   bool have_dtc = false;
   int *list = nullptr;       // cell list of the surrogate's fallback cells (capacity cap)
+  unsigned short *sfeat = nullptr; // surrogate work arrays (capacity cap): BF16 features [cap][16],
+  float *sout = nullptr;           // network outputs [cap][10],
+  unsigned char *sflag = nullptr;  // input-range flags [cap]
   int *list_count = nullptr; // its device-side length
   corefl::ChemLaunch cfg{1, 64, 0};
   bool tuned = false;

@@ -23,17 +23,21 @@ struct MlpDev {
   const float *bias[kMlpLayers];
 };
 
-// Feature and target scaling: features (T, ln p, Y_1..Y_9, ln dt) -> (f - mu) / sig; targets are
-// (dY_k/dt, dT/dt) = out * sig + mu.
+// Feature and target scaling.  Features (T, ln p, Y_1^a..Y_9^a, ln dt) -> (f - mu) / sig, with a = y_power
+// (1: plain mass fractions; a < 1 spreads the small radical fractions).  Targets: s = out * sig + mu is the
+// signed logarithm of the rate, rate = sign(s) r0 (exp|s| - 1) for (dY_k/dt, dT/dt); r0 <= 0 means
+// rate = s (no transform).
 struct SurrogateScaling {
   float in_mu[kMlpIn], in_sig[kMlpIn];
-  double out_mu[kMlpOut], out_sig[kMlpOut];
+  double out_mu[kMlpOut], out_sig[kMlpOut], out_r0[kMlpOut];
+  double y_power;
 };
 
-// A cell uses the network only inside this input box and when the network's own temperature change agrees
-// with the one recovered from energy conservation to within dT_max.
+// A cell uses the network only inside this input box, when the network's own temperature change agrees
+// with the one recovered from energy conservation to within dT_max, and when its temperature changes by at
+// most dT_step over the sub-step (fast-reacting cells go to the exact integrator).
 struct SurrogateGate {
-  double t_min, t_max, p_min, p_max, dt_min, dt_max, dT_max;
+  double t_min, t_max, p_min, p_max, dt_min, dt_max, dT_max, dT_step;
 };
 
 // Element conservation: E[e][k] = (atoms of element e in species k) / MW_k, at most 4 elements; rows
@@ -83,13 +87,24 @@ hipError_t launch_adam(float *p, float *m, float *v, const float *g, long n, flo
 hipError_t launch_to_bf16t(const float *W, int K, int N, int Kp, unsigned short *wt, hipStream_t s);
 hipError_t launch_fill(float *p, long n, float v, hipStream_t s);
 
-// Fused inference.  Raw mode: normalized features x [n][12] -> raw outputs y [n][10].
-hipError_t launch_mlp_raw(const MlpDev &net, long n, const float *x, float *y, int blocks, hipStream_t s);
-// Chemistry mode: every cell of b (n cells) is advanced by dt (b.dtc per cell if set) with the network;
-// cells that fail the gate are appended to fb_list (count in *fb_count, zeroed by the launch) and left
-// unchanged for the exact integrator.
+// Work arrays of a surrogate chemistry step for n cells: BF16 features [n][16], FP32 outputs [n][10] and
+// the input-range flags [n].
+struct SurrogateWork {
+  unsigned short *feat;
+  float *out;
+  unsigned char *flag;
+};
+
+// FP32 feature rows X [n][12] -> BF16 rows [n][16] (zero padded).
+hipError_t launch_rows_to_feat(const float *X, long n, unsigned short *feat, hipStream_t s);
+// Inference.  Raw mode: BF16 feature rows feat [n][16] (as launch_gather writes them) -> outputs y [n][10].
+hipError_t launch_mlp_raw(const MlpDev &net, long n, const unsigned short *feat, float *y, int blocks,
+                          hipStream_t s);
+// Chemistry mode: every cell of b (n cells) is advanced by dt (b.dtc per cell if set) with the network
+// (features, network, projection / temperature / gate kernels); cells that fail the gate are appended to
+// fb_list (count in *fb_count, zeroed by the launch) and left unchanged for the exact integrator.
 hipError_t launch_mlp_chem(const MlpDev &net, const ChemMech *mech_dev, const ChemBatch &b, double dt,
                            const SurrogateScaling &sc, const SurrogateGate &gate, const ElementMatrix &em,
-                           int *fb_list, int *fb_count, int blocks, hipStream_t s);
+                           const SurrogateWork &w, int *fb_list, int *fb_count, int blocks, hipStream_t s);
 
 } // namespace corefl

@@ -196,7 +196,8 @@ void surrogate_substep(corefl_chem *c, corefl_surrogate *s, const ChemBatch &b, 
     for (int k = 0; k < mech.ns; ++k) em.E[e][k] = mech.comp[k][e] / mech.mw[k];
   // dt of the launch: only used when the batch has neither per-cell nor device-side lengths
   const double dt = s->step_dt;
-  HIPCHK(launch_mlp_chem(net_of(s), c->mech_dev, b, dt, s->sc, s->gate, em, c->list, c->list_count, s->cus, st));
+  const SurrogateWork w{c->sfeat, c->sout, c->sflag};
+  HIPCHK(launch_mlp_chem(net_of(s), c->mech_dev, b, dt, s->sc, s->gate, em, w, c->list, c->list_count, s->cus, st));
   ChemBatch fb = b;
   fb.list = c->list;
   fb.n_list = c->list_count;
@@ -227,6 +228,7 @@ int corefl_surrogate_create(int batch_capacity, corefl_surrogate **out) {
     if (batch_capacity < 1 || batch_capacity > (1 << 20))
       throw std::runtime_error("corefl_surrogate_create: batch capacity must be in [1, 2^20]");
     auto s = std::make_unique<corefl_surrogate>();
+    s->sc.y_power = 1.0;
     HIPCHK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
     HIPCHK(hipDeviceGetAttribute(&s->cus, hipDeviceAttributeMultiprocessorCount, 0));
     const int B = batch_capacity;
@@ -295,12 +297,21 @@ int corefl_surrogate_set_scaling(corefl_surrogate *s, const double *in_mu, const
   });
 }
 
+int corefl_surrogate_set_transform(corefl_surrogate *s, double y_power, const double *out_r0) {
+  return guarded([&] {
+    if (!s) throw std::runtime_error("corefl_surrogate_set_transform: null context");
+    if (!(y_power > 0) || y_power > 1) throw std::runtime_error("corefl_surrogate_set_transform: y_power in (0, 1]");
+    s->sc.y_power = y_power;
+    for (int q = 0; q < kMlpOut; ++q) s->sc.out_r0[q] = out_r0 ? out_r0[q] : 0.0;
+  });
+}
+
 int corefl_surrogate_set_gate(corefl_surrogate *s, const double *gate) {
   return guarded([&] {
     if (!s || !gate) throw std::runtime_error("corefl_surrogate_set_gate: null argument");
-    s->gate = SurrogateGate{gate[0], gate[1], gate[2], gate[3], gate[4], gate[5], gate[6]};
+    s->gate = SurrogateGate{gate[0], gate[1], gate[2], gate[3], gate[4], gate[5], gate[6], gate[7]};
     if (!(s->gate.t_min <= s->gate.t_max && s->gate.p_min <= s->gate.p_max && s->gate.dt_min <= s->gate.dt_max &&
-          s->gate.dT_max >= 0))
+          s->gate.dT_max >= 0 && s->gate.dT_step >= 0))
       throw std::runtime_error("corefl_surrogate_set_gate: empty range");
     s->have_gate = true;
   });
@@ -384,8 +395,15 @@ int corefl_surrogate_forward(corefl_surrogate *s, long long n, const float *X, f
     try {
       upload_rows(s, n, X, nullptr, &dX, &dY);
       if (fused) {
-        HIPCHK(launch_mlp_raw(net_of(s), n, dX, dY, s->cus, s->stream));
-        HIPCHK(hipMemcpyAsync(out, dY, sizeof(float) * n * kMlpOut, hipMemcpyDeviceToHost, s->stream));
+        unsigned short *feat = nullptr;
+        dalloc(&feat, (size_t)n * 16);
+        const hipError_t e1 = launch_rows_to_feat(dX, n, feat, s->stream);
+        const hipError_t e2 = e1 == hipSuccess ? launch_mlp_raw(net_of(s), n, feat, dY, s->cus, s->stream) : e1;
+        const hipError_t e3 = hipMemcpyAsync(out, dY, sizeof(float) * n * kMlpOut, hipMemcpyDeviceToHost, s->stream);
+        (void)hipStreamSynchronize(s->stream);
+        (void)hipFree(feat);
+        HIPCHK(e2);
+        HIPCHK(e3);
       } else {
         HIPCHK(launch_gather(dX, dY, n, 0, 0, (int)n, s->H[0], s->Yb, s->stream));
         forward(s, (int)n);

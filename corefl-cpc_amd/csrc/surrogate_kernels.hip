@@ -35,7 +35,8 @@ __device__ __forceinline__ unsigned short f2bf(float f) {
 constexpr float kGs = 0.7978845608028654f, kGa = 0.044715f;
 __device__ __forceinline__ float sig2u(float z) {
   const float u = kGs * (z + kGa * z * z * z);
-  return 1.0f / (1.0f + __expf(-2.0f * u));
+  // hardware reciprocal (1 ulp) instead of a correctly rounded division: the result is rounded to BF16
+  return __builtin_amdgcn_rcpf(1.0f + __expf(-2.0f * u));
 }
 __device__ __forceinline__ float gelu(float z) { return z * sig2u(z); }
 __device__ __forceinline__ float gelu_grad(float z) {
@@ -256,7 +257,10 @@ __device__ __forceinline__ bool chem_features(const FusedArgs &a, long c, float 
   f[0] = (float)((T - s.in_mu[0]) / s.in_sig[0]);
   f[1] = (float)((log(p) - s.in_mu[1]) / s.in_sig[1]);
 #pragma unroll
-  for (int k = 0; k < 9; ++k) f[2 + k] = (float)((y[k] - s.in_mu[2 + k]) / s.in_sig[2 + k]);
+  for (int k = 0; k < 9; ++k) {
+    const double yf = s.y_power == 1.0 ? y[k] : (y[k] > 0 ? exp(s.y_power * log(y[k])) : 0.0);
+    f[2 + k] = (float)((yf - s.in_mu[2 + k]) / s.in_sig[2 + k]);
+  }
   f[11] = (float)((log(dt) - s.in_mu[11]) / s.in_sig[11]);
   const SurrogateGate &g = a.gate;
   return T >= g.t_min && T <= g.t_max && p >= g.p_min && p <= g.p_max && dt >= g.dt_min && dt <= g.dt_max;
@@ -286,6 +290,14 @@ __device__ __forceinline__ bool spd_solve4(double (&M)[4][4], double (&r)[4]) {
   return true;
 }
 
+// Network output q -> rate (inverse of the signed-logarithm target transform).
+__device__ __forceinline__ double rate_of(const SurrogateScaling &s, int q, float o) {
+  const double v = (double)o * s.out_sig[q] + s.out_mu[q];
+  if (!(s.out_r0[q] > 0)) return v;
+  const double m = s.out_r0[q] * expm1(fabs(v));
+  return v < 0 ? -m : m;
+}
+
 // Network outputs o[10] of cell c -> projected mass fractions and energy-consistent temperature; writes the
 // cell if the gate passes, else appends it to the fallback list.
 __device__ void chem_finish(const FusedArgs &a, long c, const float *o, bool in_range) {
@@ -296,9 +308,9 @@ __device__ void chem_finish(const FusedArgs &a, long c, const float *o, bool in_
 #pragma unroll
   for (int k = 0; k < 9; ++k) {
     y0[k] = b.rhoY[(long)k * b.n + c] / rho;
-    dy[k] = dt * ((double)o[k] * a.sc.out_sig[k] + a.sc.out_mu[k]);
+    dy[k] = dt * rate_of(a.sc, k, o[k]);
   }
-  const double t_net = T + dt * ((double)o[9] * a.sc.out_sig[9] + a.sc.out_mu[9]);
+  const double t_net = T + dt * rate_of(a.sc, 9, o[9]);
   // least-squares projection onto element conservation (rows e >= ne are identity padding), species
   // driven below zero held at zero
   unsigned clamp = 0;
@@ -345,7 +357,7 @@ __device__ void chem_finish(const FusedArgs &a, long c, const float *o, bool in_
   double t_new = t_net;
   if (ok) {
     t_new = chemdev::update_t<9>(m, fmin(fmax(t_net, 200.0), 5000.0), yn, b.e[c]);
-    ok = isfinite(t_new) && fabs(t_new - t_net) <= a.gate.dT_max;
+    ok = isfinite(t_new) && fabs(t_new - t_net) <= a.gate.dT_max && fabs(t_new - T) <= a.gate.dT_step;
   }
   if (ok) {
 #pragma unroll
@@ -359,8 +371,11 @@ __device__ void chem_finish(const FusedArgs &a, long c, const float *o, bool in_
   }
 }
 
-template<bool CHEM>
-__global__ __launch_bounds__(256) void k_mlp_fused(MlpDev net, FusedArgs a) {
+// The network on n rows of BF16 features feat [n][16] -> FP32 outputs out [n][10].  Persistent: each
+// workgroup loads every weight into LDS once, then its 4 waves take 32-row tiles.  The layer-0 operand is
+// read straight from global memory (16 bytes per lane: row l % 32, features 8 (l / 32) .. +7).
+__global__ __launch_bounds__(256) void k_mlp(MlpDev net, const unsigned short *__restrict__ feat,
+                                             float *__restrict__ out, long n) {
   __shared__ __attribute__((aligned(16))) unsigned short w0[128 * kS0];
   __shared__ __attribute__((aligned(16))) unsigned short wh[3][128 * kSH];
   __shared__ __attribute__((aligned(16))) unsigned short w4[32 * kSH];
@@ -371,31 +386,20 @@ __global__ __launch_bounds__(256) void k_mlp_fused(MlpDev net, FusedArgs a) {
   load_rows(w4, kSH, net.wt[4], 32, 128);
   __syncthreads();
   unsigned short *h = act[wave];
-  float *hf = reinterpret_cast<float *>(h); // [32][10] outputs, over the wave's activation rows
-  const long n_tiles = (a.n + 127) / 128;
+  const long n_tiles = (n + 127) / 128;
+  const int row = lane & 31, kh = 8 * (lane >> 5);
   for (long tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
-    const long c = tile * 128 + wave * 32 + (lane & 31);
-    const bool own = lane < 32 && c < a.n;
-    bool in_range = false;
-    if (lane < 32) {
-      float f[16];
-#pragma unroll
-      for (int q = 0; q < 16; ++q) f[q] = 0.0f;
-      if (own) {
-        if constexpr (CHEM) {
-          in_range = chem_features(a, c, f);
-        } else {
-#pragma unroll
-          for (int q = 0; q < kMlpIn; ++q) f[q] = a.x[c * kMlpIn + q];
-        }
-      }
-#pragma unroll
-      for (int q = 0; q < 16; ++q) h[lane * kSH + q] = f2bf(f[q]);
-    }
-    __syncthreads();
+    const long c0 = tile * 128 + wave * 32;
+    shortx8 av = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (c0 + row < n) av = *reinterpret_cast<const shortx8 *>(&feat[(c0 + row) * 16 + kh]);
     floatx16 acc[4];
-    mma_layer<1, 4>(h, w0, kS0, acc, lane);
-    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[j][r] = 0.0f;
+      const shortx8 bv = *reinterpret_cast<const shortx8 *>(&w0[(j * 32 + row) * kS0 + kh]);
+      acc[j] = mfma(av, bv, acc[j]);
+    }
     store_hidden(h, acc, net.bias[0], lane);
     __syncthreads();
     for (int l = 0; l < 3; ++l) {
@@ -405,32 +409,53 @@ __global__ __launch_bounds__(256) void k_mlp_fused(MlpDev net, FusedArgs a) {
       __syncthreads();
     }
     mma_layer<8, 1>(h, w4, kSH, acc, lane);
-    __syncthreads();
-    {
-      const int j = lane & 31;
-      if (j < kMlpOut) {
-        const float bj = net.bias[4][j];
+    if (row < kMlpOut) {
+      const float bj = net.bias[4][row];
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-          const int m = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-          hf[m * kMlpOut + j] = acc[0][r] + bj;
-        }
-      }
-    }
-    __syncthreads();
-    if (own) {
-      float o[kMlpOut];
-#pragma unroll
-      for (int q = 0; q < kMlpOut; ++q) o[q] = hf[lane * kMlpOut + q];
-      if constexpr (CHEM) {
-        chem_finish(a, c, o, in_range);
-      } else {
-#pragma unroll
-        for (int q = 0; q < kMlpOut; ++q) a.y[c * kMlpOut + q] = o[q];
+      for (int r = 0; r < 16; ++r) {
+        const long c = c0 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        if (c < n) out[c * kMlpOut + row] = acc[0][r] + bj;
       }
     }
     __syncthreads();
   }
+}
+
+// Features of every cell of the batch as BF16 rows [16] (zero padded) and the input-range flag.
+__global__ __launch_bounds__(256) void k_chem_feat(FusedArgs a, unsigned short *__restrict__ feat,
+                                                   unsigned char *__restrict__ flag) {
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= a.n) return;
+  float f[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) f[q] = 0.0f;
+  flag[c] = chem_features(a, c, f) ? 1 : 0;
+  shortx8 lo, hi;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    lo[q] = (short)f2bf(f[q]);
+    hi[q] = (short)f2bf(f[8 + q]);
+  }
+  *reinterpret_cast<shortx8 *>(&feat[c * 16]) = lo;
+  *reinterpret_cast<shortx8 *>(&feat[c * 16 + 8]) = hi;
+}
+
+// Projection, temperature, gate and write-back of every cell from the network outputs.
+__global__ __launch_bounds__(256) void k_chem_post(FusedArgs a, const float *__restrict__ out,
+                                                   const unsigned char *__restrict__ flag) {
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= a.n) return;
+  float o[kMlpOut];
+#pragma unroll
+  for (int q = 0; q < kMlpOut; ++q) o[q] = out[c * kMlpOut + q];
+  chem_finish(a, c, o, flag[c] != 0);
+}
+
+__global__ void k_rows_to_feat(const float *__restrict__ X, long n, unsigned short *__restrict__ feat) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+#pragma unroll
+  for (int f = 0; f < 16; ++f) feat[i * 16 + f] = f < kMlpIn ? f2bf(X[i * kMlpIn + f]) : (unsigned short)0;
 }
 
 __global__ void k_zero_count(int *p) { *p = 0; }
@@ -488,20 +513,23 @@ hipError_t launch_fill(float *p, long n, float v, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_mlp_raw(const MlpDev &net, long n, const float *x, float *y, int blocks, hipStream_t s) {
+hipError_t launch_rows_to_feat(const float *X, long n, unsigned short *feat, hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  FusedArgs a{};
-  a.n = n;
-  a.x = x;
-  a.y = y;
+  k_rows_to_feat<<<grid1(n, 256), 256, 0, s>>>(X, n, feat);
+  return hipGetLastError();
+}
+
+hipError_t launch_mlp_raw(const MlpDev &net, long n, const unsigned short *feat, float *y, int blocks,
+                          hipStream_t s) {
+  if (n <= 0) return hipSuccess;
   const long tiles = (n + 127) / 128;
-  k_mlp_fused<false><<<(unsigned)std::min<long>(tiles, blocks), 256, 0, s>>>(net, a);
+  k_mlp<<<(unsigned)std::min<long>(tiles, blocks), 256, 0, s>>>(net, feat, y, n);
   return hipGetLastError();
 }
 
 hipError_t launch_mlp_chem(const MlpDev &net, const ChemMech *mech_dev, const ChemBatch &b, double dt,
                            const SurrogateScaling &sc, const SurrogateGate &gate, const ElementMatrix &em,
-                           int *fb_list, int *fb_count, int blocks, hipStream_t s) {
+                           const SurrogateWork &w, int *fb_list, int *fb_count, int blocks, hipStream_t s) {
   k_zero_count<<<1, 1, 0, s>>>(fb_count);
   if (b.n <= 0) return hipGetLastError();
   FusedArgs a{};
@@ -514,8 +542,11 @@ hipError_t launch_mlp_chem(const MlpDev &net, const ChemMech *mech_dev, const Ch
   a.em = em;
   a.fb_list = fb_list;
   a.fb_count = fb_count;
+  const unsigned g = grid1(b.n, 256);
+  k_chem_feat<<<g, 256, 0, s>>>(a, w.feat, w.flag);
   const long tiles = (b.n + 127) / 128;
-  k_mlp_fused<true><<<(unsigned)std::min<long>(tiles, blocks), 256, 0, s>>>(net, a);
+  k_mlp<<<(unsigned)std::min<long>(tiles, blocks), 256, 0, s>>>(net, w.feat, w.out, b.n);
+  k_chem_post<<<g, 256, 0, s>>>(a, w.out, w.flag);
   return hipGetLastError();
 }
 

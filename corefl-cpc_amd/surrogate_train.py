@@ -16,11 +16,26 @@ def mixture_imw(chem_mw, Y):
     return np.sum(Y / chem_mw[:, None], axis=0)
 
 
-def features(mw, rho, T, Y, dt):
-    """Physical features (n, 12): T, ln p, Y_1..Y_9, ln dt."""
+# Transforms: mass-fraction features Y^Y_POWER; targets the signed logarithm of the rates with scales
+# R0 (1/s for the nine dY/dt, K/s for dT/dt), so that rates spanning many decades train evenly.
+Y_POWER = 0.1
+R0 = np.array([1.0] * 9 + [1e3])
+
+
+def features(mw, rho, T, Y, dt, y_power=Y_POWER):
+    """Physical features (n, 12): T, ln p, Y_1^a..Y_9^a, ln dt."""
     p = rho * RU * mixture_imw(mw, Y) * T
     dt = np.broadcast_to(np.asarray(dt, dtype=np.float64), T.shape)
-    return np.column_stack([T, np.log(p), Y.T, np.log(dt)])
+    Yf = Y if y_power == 1.0 else np.where(Y > 0, np.abs(Y) ** y_power, 0.0)
+    return np.column_stack([T, np.log(p), Yf.T, np.log(dt)])
+
+
+def signed_log(r, r0=R0):
+    return np.sign(r) * np.log1p(np.abs(r) / r0)
+
+
+def signed_exp(s, r0=R0):
+    return np.sign(s) * r0 * np.expm1(np.abs(s))
 
 
 def ignition_states(chem, n_reactors, n_snap, seed, y_fresh, t_lo=1000.0, t_hi=2400.0, p_lo=0.5 * PATM,
@@ -59,36 +74,42 @@ class Scaling:
 
     @classmethod
     def fit(cls, F, R):
-        sig = F.std(axis=0)
-        osig = R.std(axis=0)
-        return cls(F.mean(axis=0), np.where(sig > 0, sig, 1.0), R.mean(axis=0), np.where(osig > 0, osig, 1.0))
+        """Mean and standard deviation per column, the deviation floored so that a column that is constant up
+        to rounding (an inert species, a fixed element ratio) is not blown up into noise."""
+        mu, sig = F.mean(axis=0), F.std(axis=0)
+        sig = np.maximum(sig, 1e-3 * np.maximum(np.abs(mu), 1.0))
+        omu, osig = R.mean(axis=0), R.std(axis=0)
+        osig = np.maximum(osig, 1e-3)
+        return cls(mu, sig, omu, osig)
 
     def apply(self, surr):
         surr.set_scaling(self.in_mu, self.in_sig, self.out_mu, self.out_sig)
 
 
-def build_dataset(chem, n_states, seed, y_fresh, dt_lo=1e-9, dt_hi=1e-7, t_min=300.0, t_max=2600.0,
-                  p_lo=0.5 * PATM, p_hi=2 * PATM):
-    """n_states (rho, T, Y, dt) with exact labels.  Half come from ignition trajectories (T0 in
-    [1000, 2400] K), a quarter are fresh mixtures at T uniform in [t_min, t_max], a quarter lie on the
-    mixing line between fresh mixture and trajectory states at T uniform in [t_min, t_max]; p in
-    [p_lo, p_hi], dt log-uniform in [dt_lo, dt_hi]."""
+def build_dataset(chem, n_states, seed, y_fresh, dt_lo=1e-9, dt_hi=1e-7, t_min=300.0, t_max=3200.0,
+                  p_lo=0.5 * PATM, p_hi=2 * PATM, n_reactors=8192, n_snap=48):
+    """n_states (rho, T, Y, dt) with exact labels.  Half come from constant-volume ignition trajectories of
+    n_reactors fresh mixtures (T0 in [1000, 2400] K) recorded at n_snap geometric times in [1e-7, 3e-4] s,
+    a quarter are fresh mixtures at T uniform in [t_min, 2600 K], a quarter lie on the mixing line between
+    fresh mixture and trajectory states at T uniform in [t_min, 2600 K]; p in [p_lo, p_hi], dt log-uniform in
+    [dt_lo, dt_hi]; states hotter than t_max (burnt gas reaches about 3000 K) are dropped."""
     rng = np.random.default_rng(seed)
     mw = chem.table("mw")
-    n_traj = max(64, n_states // 2 // 257)
-    r_rho, r_T, r_Y = ignition_states(chem, n_traj, 256, seed + 1, y_fresh, p_lo=p_lo, p_hi=p_hi)
-    pick = rng.choice(r_T.size, n_states // 2, replace=r_T.size < n_states // 2)
+    r_rho, r_T, r_Y = ignition_states(chem, n_reactors, n_snap, seed + 1, y_fresh, p_lo=p_lo, p_hi=p_hi,
+                                      t_first=1e-7, t_last=3e-4)
+    n_a = n_states // 2
+    pick = rng.choice(r_T.size, n_a, replace=r_T.size < n_a)
     Y_a, T_a = r_Y[:, pick], r_T[pick]
     p_a = r_rho[pick] * RU * mixture_imw(mw, Y_a) * T_a
     n_b = n_states // 4
-    n_c = n_states - pick.size - n_b
+    n_c = n_states - n_a - n_b
     Y_b = np.repeat(np.asarray(y_fresh, dtype=np.float64)[:, None], n_b, axis=1)
-    T_b = rng.uniform(t_min, t_max, n_b)
+    T_b = rng.uniform(t_min, 2600.0, n_b)
     p_b = rng.uniform(p_lo, p_hi, n_b)
     lam = rng.uniform(0, 1, n_c)
     other = r_Y[:, rng.choice(r_T.size, n_c)]
     Y_c = (1 - lam) * np.asarray(y_fresh, dtype=np.float64)[:, None] + lam * other
-    T_c = rng.uniform(t_min, t_max, n_c)
+    T_c = rng.uniform(t_min, 2600.0, n_c)
     p_c = rng.uniform(p_lo, p_hi, n_c)
     Y = np.concatenate([Y_a, Y_b, Y_c], axis=1)
     T = np.concatenate([T_a, T_b, T_c])
@@ -111,7 +132,7 @@ def train(surr, data, mw, steps=6000, batch=8192, lr=2e-3, lr_min=2e-5, seed=11,
     """Fit the network on `data` (build_dataset) with a cosine learning-rate schedule; returns the scaling
     and the per-step losses."""
     F = features(mw, data["rho"], data["T"], data["Y"], data["dt"])
-    R = targets(data)
+    R = signed_log(targets(data))
     sc = Scaling.fit(F, R)
     surr.set_params(init_params(init_seed))
     surr.set_data((F - sc.in_mu) / sc.in_sig, (R - sc.out_mu) / sc.out_sig)
@@ -123,12 +144,15 @@ def train(surr, data, mw, steps=6000, batch=8192, lr=2e-3, lr_min=2e-5, seed=11,
         losses.append(surr.train(batch, n, rate, seed=seed + done))
         done += n
     sc.apply(surr)
+    surr.set_transform(Y_POWER, R0)
     return sc, np.concatenate(losses) if losses else np.empty(0)
 
 
-def set_gate(surr, dT_max, t_min=300.0, t_max=2600.0, p_lo=0.5 * PATM, p_hi=2 * PATM, dt_lo=1e-9, dt_hi=1e-7):
-    """Gate = the training box plus the temperature-consistency bound dT_max (K)."""
-    surr.set_gate(t_min, t_max, p_lo, p_hi, dt_lo, dt_hi, dT_max)
+def set_gate(surr, dT_max, dT_step=1e30, t_min=290.0, t_max=3200.0, p_lo=0.5 * PATM, p_hi=2 * PATM, dt_lo=1e-9,
+             dt_hi=1e-7):
+    """Gate = the training box, the temperature-consistency bound dT_max (K) and the largest temperature
+    change per sub-step dT_step (K) left to the network."""
+    surr.set_gate(t_min, t_max, p_lo, p_hi, dt_lo, dt_hi, dT_max, dT_step)
 
 
 def init_params(seed, sizes=corefl.SURROGATE_SIZES):

@@ -220,9 +220,14 @@ int corefl_surrogate_get_params(corefl_surrogate *s, float *params, long long n)
 /* Feature scaling (12 values each) and target scaling (10 values each). */
 int corefl_surrogate_set_scaling(corefl_surrogate *s, const double *in_mu, const double *in_sig, const double *out_mu,
                                  const double *out_sig);
-/* Gate: gate[0..7) = T_min, T_max, p_min, p_max, dt_min, dt_max (K, Pa, s) and dT_max (K): a cell uses the
- * network only inside the box and when the network's temperature change and the one recovered from
- * energy conservation differ by at most dT_max. */
+/* Feature and target transforms: mass-fraction features Y^y_power (y_power in (0, 1], default 1), and
+ * per output a signed-logarithm target s = sign(r) ln(1 + |r|/r0) of the rate r (out_r0[q] > 0; 0 or a
+ * NULL array: the scaled rate itself). */
+int corefl_surrogate_set_transform(corefl_surrogate *s, double y_power, const double *out_r0);
+/* Gate: gate[0..8) = T_min, T_max, p_min, p_max, dt_min, dt_max (K, Pa, s), dT_max and dT_step (K): a cell
+ * uses the network only inside the box, when the network's temperature change and the one recovered from
+ * energy conservation differ by at most dT_max, and when its temperature changes by at most dT_step over
+ * the sub-step. */
 int corefl_surrogate_set_gate(corefl_surrogate *s, const double *gate);
 /* Training set on the device: n rows of scaled features X (n x 12) and scaled targets Y (n x 10). */
 int corefl_surrogate_set_data(corefl_surrogate *s, long long n, const float *X, const float *Y);

@@ -168,3 +168,32 @@ def test_partial_gate_mixes_paths(net):
     ch.integrate(1e-7)
     ref = ch.get_cells()
     np.testing.assert_array_equal(g["T"][hot], ref["T"][hot])
+
+
+def test_transforms_match_host_definitions(net):
+    """Features Y^a and the signed-logarithm targets: the fused chemistry step with a network whose output
+    layer is zero reproduces the host's inverse transform of the output bias exactly in the rates."""
+    import surrogate_train as ST
+    s, params = net
+    ch, rho, T, Y = chem_setup(500, 24)
+    zero = [(np.zeros_like(W), np.zeros_like(b)) for W, b in params]
+    zero[-1] = (np.zeros_like(params[-1][0]), np.zeros(10))
+    s.set_params(zero)
+    mu = np.zeros(10)
+    mu[7] = 2.0   # s = 2 -> dY_H2O/dt = r0 (e^2 - 1)
+    mu[0] = -2.0  # dY_H2/dt = -r0 (e^2 - 1); the projection restores element balance
+    s.set_scaling(np.zeros(12), np.ones(12), mu, np.ones(10))
+    s.set_transform(0.1, ST.R0)
+    s.set_gate(0, 1e9, 0, 1e12, 0, 1, 1e9)
+    ch.set_cells(rho, T, rho * Y)
+    dt = 1e-8
+    assert s.chem_step(ch, dt) == 0
+    g = ch.get_cells()
+    Yn = g["rhoY"] / rho
+    r = ST.signed_exp(mu)
+    raw = dt * r[:9]
+    # the net change is the element-conserving projection of the raw change: same direction in H2O
+    assert np.all(Yn[7] - Y[7] > 0) and np.all(Yn[0] - Y[0] < 0)
+    assert np.max(np.abs(Yn.sum(axis=0) - 1)) < 1e-13
+    s.set_params(params)
+    s.set_transform(1.0, None)
