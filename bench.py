@@ -187,6 +187,10 @@ def main():
     ap.add_argument("--profile-reps", type=int, default=3)
     ap.add_argument("--no-reactive", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--only", choices=("reactive", "surrogate"),
+                    help="run one secondary section alone and print its JSON object (profiling runs)")
+    ap.add_argument("--train-steps", type=int, default=30000)
+    ap.add_argument("--surrogate-states", type=int, default=2000000)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device copy (reference point for the HBM byte counters)")
     args = ap.parse_args()
@@ -206,6 +210,12 @@ def main():
     import cases
     import corefl
 
+    if args.only == "reactive":
+        print(json.dumps(reactive_bench()), flush=True)
+        return
+    if args.only == "surrogate":
+        print(json.dumps(surrogate_bench(n_states=args.surrogate_states, train_steps=args.train_steps)), flush=True)
+        return
     n = args.n
     log(f"bench: building TGV wavy grid {n}^3")
     case = cases.tgv(n, reynolds=1600.0, recon=5, inviscid=3, visc=4, cfl=0.5)
@@ -241,7 +251,8 @@ def main():
 
     S.close()
     reactive = None if args.no_reactive else reactive_bench()
-    surrogate = None if args.no_surrogate else surrogate_bench()
+    surrogate = None if args.no_surrogate else surrogate_bench(n_states=args.surrogate_states,
+                                                               train_steps=args.train_steps)
     cpu = None
     if not args.no_cpu_baseline:
         log("bench: cpu baseline (oracle)")
