@@ -92,8 +92,8 @@ def load_library(path=LIB_PATH):
     lib.corefl_chem_view_get.argtypes = [P, ctypes.POINTER(ChemView)]
     lib.corefl_chem_source.argtypes = [P, P]
     lib.corefl_chem_integrate.argtypes = [P, D, D, D, I, P]
-    lib.corefl_chem_set_launch.argtypes = [P, I, I, I]
-    lib.corefl_chem_get_launch.argtypes = [P, PI, PI, PI]
+    lib.corefl_chem_set_launch.argtypes = [P, I, I, I, I]
+    lib.corefl_chem_get_launch.argtypes = [P, PI, PI, PI, PI]
     lib.corefl_chem_autotune.argtypes = [P, D, D, D, I, I, ctypes.POINTER(D), I]
     lib.corefl_chem_synchronize.argtypes = [P]
     F = ctypes.c_float
@@ -243,8 +243,11 @@ class Chemistry:
 
     Cell arrays are structure-of-arrays: rhoY has shape (n_spec, n)."""
 
+    # (mode, threads, waves per CU), and for "handover" (threads, attempt budget)
     AUTOTUNE_CANDIDATES = tuple([("static", t, 0) for t in (64, 128, 256)] +
-                                [("queue", t, w) for t in (64, 128, 256) for w in (4, 8, 16)])
+                                [("queue", t, w) for t in (64, 128, 256) for w in (4, 8, 16)] +
+                                [("coop", 256, w) for w in (4, 8)] +
+                                [("handover", 64, k) for k in (4, 16)])
 
     def __init__(self, mechanism_file=None, therm_file=None, *, mech_text=None, therm_text=""):
         lib = load_library()
@@ -365,13 +368,16 @@ class Chemistry:
         if sync:
             self.sync()
 
-    def set_launch(self, queue, threads=64, blocks=1024):
-        _check(self._lib.corefl_chem_set_launch(self._ctx, 1 if queue else 0, int(threads), int(blocks)))
+    def set_launch(self, mode, threads=64, blocks=1024, budget=0):
+        """mode 0: one thread per cell; 1: work queue; 2: cooperative (16 lanes per cell); 3: work queue for
+        `budget` attempts, then cooperative.  A bool mode means 1 (True) or 0 (False)."""
+        _check(self._lib.corefl_chem_set_launch(self._ctx, int(mode), int(threads), int(blocks), int(budget)))
 
     def get_launch(self):
-        q, t, b = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
-        _check(self._lib.corefl_chem_get_launch(self._ctx, ctypes.byref(q), ctypes.byref(t), ctypes.byref(b)))
-        return dict(queue=q.value, threads=t.value, blocks=b.value)
+        q, t, b, k = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _check(self._lib.corefl_chem_get_launch(self._ctx, ctypes.byref(q), ctypes.byref(t), ctypes.byref(b),
+                                                ctypes.byref(k)))
+        return dict(queue=q.value, threads=t.value, blocks=b.value, budget=k.value)
 
     def autotune(self, dt, rtol=1e-6, atol=1e-8, max_attempts=100000, reps=1):
         """Time every candidate launch shape on the current batch; keeps the fastest.  Returns ms per

@@ -14,6 +14,8 @@ struct ChemMech {
   double mw[kMaxSpec];  // kg/kmol from atomic weights (Element.cpp)
   double imw[kMaxSpec]; // 1 / mw
   double t_low[kMaxSpec], t_mid[kMaxSpec], t_high[kMaxSpec];
+  double ln_t_low[kMaxSpec], ln_t_high[kMaxSpec];   // log of the fit-range ends (g/RT clamps to them)
+  double inv_t_low[kMaxSpec], inv_t_high[kMaxSpec]; // and their reciprocals
   double lo[kMaxSpec][7], hi[kMaxSpec][7];
   // 0 irreversible, 1 reversible, 3 merged DUPLICATE pair, 4 third body, 5 Lindemann, 6/7 Troe (3/4 params)
   int type[kMaxReac], rev[kMaxReac];
@@ -28,6 +30,11 @@ struct ChemMech {
   int fnu[kMaxReac][kMaxSide], bnu[kMaxReac][kMaxSide];
   double nf[kMaxReac][kMaxSpec], nb[kMaxReac][kMaxSpec]; // dense stoichiometric coefficients
   double tb[kMaxReac][kMaxSpec]; // third-body efficiencies (default 1)
+  // derived once the reactions are final: bit masks of the species on either side (umask) and of those whose
+  // amount changes (amask), the net coefficients nb - nf, and the integer order sum(nb) - sum(nf)
+  unsigned umask[kMaxReac], amask[kMaxReac];
+  int iorder[kMaxReac];
+  double dnu[kMaxReac][kMaxSpec];
   int comp[kMaxSpec][kMaxElem];  // element composition
   char species[kMaxSpec][16];
   char elements[kMaxElem][4];

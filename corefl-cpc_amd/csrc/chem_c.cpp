@@ -49,7 +49,8 @@ template<class F> int guarded(F &&f) {
 
 void free_batch(corefl_chem *c) {
   for (void *p : {(void *)c->b.rho, (void *)c->b.e, (void *)c->b.T, (void *)c->b.rhoY, (void *)c->b.h,
-                  (void *)c->b.steps, (void *)c->b.rejects, (void *)c->save, (void *)c->dtc, (void *)c->list,
+                  (void *)c->b.steps, (void *)c->b.rejects, (void *)c->b.defer, (void *)c->b.n_defer,
+                  (void *)c->b.tcur, (void *)c->b.elast, (void *)c->save, (void *)c->dtc, (void *)c->list,
                   (void *)c->sfeat, (void *)c->sout, (void *)c->sflag})
     if (p) (void)hipFree(p);
   c->b = corefl::ChemBatch{};
@@ -79,7 +80,7 @@ void chem_ensure_device(corefl_chem *c) {
   HIPCHK(hipMemcpy(c->mech_dev, &c->mech, sizeof(corefl::ChemMech), hipMemcpyHostToDevice));
   int cus = 0;
   HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0));
-  c->cfg = corefl::ChemLaunch{1, 64, 4 * cus};
+  c->cfg = corefl::ChemLaunch{3, 64, 4 * cus, 4}; // until autotuned: one-lane queue, stiff cells handed over
 }
 
 void chem_ensure_batch(corefl_chem *c, long long n) {
@@ -99,6 +100,10 @@ void chem_ensure_batch(corefl_chem *c, long long n) {
   HIPCHK(hipMalloc(&c->b.h, d));
   HIPCHK(hipMalloc(&c->b.steps, sizeof(int) * (size_t)n));
   HIPCHK(hipMalloc(&c->b.rejects, sizeof(int) * (size_t)n));
+  HIPCHK(hipMalloc(&c->b.defer, sizeof(int) * (size_t)n));
+  HIPCHK(hipMalloc(&c->b.n_defer, sizeof(int)));
+  HIPCHK(hipMalloc(&c->b.tcur, d));
+  HIPCHK(hipMalloc(&c->b.elast, d));
   HIPCHK(hipMalloc(&c->dtc, d));
   HIPCHK(hipMalloc(&c->list, sizeof(int) * (size_t)n));
   HIPCHK(hipMalloc(&c->sfeat, sizeof(unsigned short) * 16 * (size_t)n));
@@ -321,24 +326,27 @@ int corefl_chem_integrate(corefl_chem *c, double dt, double rtol, double atol, i
   });
 }
 
-int corefl_chem_set_launch(corefl_chem *c, int queue, int threads, int blocks) {
+int corefl_chem_set_launch(corefl_chem *c, int mode, int threads, int blocks, int budget) {
   return guarded([&] {
     if (!c) throw std::runtime_error("corefl_chem_set_launch: null context");
+    if (mode < 0 || mode > 3) throw std::runtime_error("corefl_chem_set_launch: mode must be 0, 1, 2 or 3");
     if (threads != 64 && threads != 128 && threads != 256)
       throw std::runtime_error("corefl_chem_set_launch: threads must be 64, 128 or 256");
-    if (queue && (blocks < 1 || blocks > (1 << 20)))
+    if (mode && (blocks < 1 || blocks > (1 << 20)))
       throw std::runtime_error("corefl_chem_set_launch: blocks must be in [1, 2^20]");
+    if (mode == 3 && budget < 1) throw std::runtime_error("corefl_chem_set_launch: mode 3 needs budget >= 1");
     chem_ensure_device(c);
-    c->cfg = corefl::ChemLaunch{queue ? 1 : 0, threads, queue ? blocks : 0};
+    c->cfg = corefl::ChemLaunch{mode, threads, mode ? blocks : 0, mode == 3 ? budget : 0};
   });
 }
 
-int corefl_chem_get_launch(const corefl_chem *c, int *queue, int *threads, int *blocks) {
+int corefl_chem_get_launch(const corefl_chem *c, int *mode, int *threads, int *blocks, int *budget) {
   return guarded([&] {
     if (!c) throw std::runtime_error("corefl_chem_get_launch: null context");
-    if (queue) *queue = c->cfg.queue;
+    if (mode) *mode = c->cfg.queue;
     if (threads) *threads = c->cfg.threads;
     if (blocks) *blocks = c->cfg.blocks;
+    if (budget) *budget = c->cfg.budget;
   });
 }
 
@@ -354,12 +362,16 @@ int corefl_chem_autotune(corefl_chem *c, double dt, double rtol, double atol, in
     int cus = 0;
     HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0));
     std::vector<corefl::ChemLaunch> cand;
-    for (int t : {64, 128, 256}) cand.push_back({0, t, 0});
+    for (int t : {64, 128, 256}) cand.push_back({0, t, 0, 0});
     for (int t : {64, 128, 256})
       for (int waves_per_cu : {4, 8, 16}) {
         const int blocks = std::max(1, cus * waves_per_cu / (t / 64));
-        cand.push_back({1, t, blocks});
+        cand.push_back({1, t, blocks, 0});
       }
+    // cooperative (four cells per wave) at 4 and 8 waves per CU, and the one-lane queue handing cells over
+    // after 4 or 16 attempts
+    for (int waves_per_cu : {4, 8}) cand.push_back({2, 256, std::max(1, cus * waves_per_cu / 4), 0});
+    for (int budget : {4, 16}) cand.push_back({3, 64, std::max(1, cus * 4), budget});
     // each trial starts from the same state: T, rho Y and h are saved and restored around it
     const int ns = c->mech.ns;
     const size_t d = sizeof(double) * (size_t)n;

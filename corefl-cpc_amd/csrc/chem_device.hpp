@@ -40,7 +40,7 @@ template<int NS> __device__ __forceinline__ void h_cp(const ChemMech &m, double 
 }
 
 // g_k / (R T), clamped to the fit range (Thermo.cu:574-598).  lnT = log(t): one logarithm serves every
-// species whose fit range holds t.
+// species whose fit range holds t; the logarithms of the range ends are tabulated.
 template<int NS>
 __device__ __forceinline__ void gibbs(const ChemMech &m, double t, double lnT, double iT, double *g) {
 #pragma unroll
@@ -49,8 +49,8 @@ __device__ __forceinline__ void gibbs(const ChemMech &m, double t, double lnT, d
     const double tt = below ? m.t_low[i] : (above ? m.t_high[i] : t);
     const double *c = below ? m.lo[i] : (above ? m.hi[i] : (t < m.t_mid[i] ? m.lo[i] : m.hi[i]));
     const double tt2 = tt * tt, tt3 = tt2 * tt, tt4 = tt3 * tt;
-    const double lt = (below || above) ? log(tt) : lnT;
-    const double itt = (below || above) ? 1.0 / tt : iT;
+    const double lt = below ? m.ln_t_low[i] : (above ? m.ln_t_high[i] : lnT);
+    const double itt = below ? m.inv_t_low[i] : (above ? m.inv_t_high[i] : iT);
     g[i] = c[0] * (1.0 - lt) - 0.5 * c[1] * tt - c[2] * tt2 * kSixth - c[3] * tt3 * kTwelfth - c[4] * tt4 * 0.05 +
            c[5] * itt - c[6];
   }
@@ -62,28 +62,23 @@ __device__ __forceinline__ double arrhenius(double lnT, double iT, double A, dou
   return A * exp(b * lnT - Ea * iT);
 }
 
-// x^p for the small integer reaction orders, pow() otherwise.
-__device__ __forceinline__ double order_pow(double x, double p) {
-  const int k = (int)p;
-  if ((double)k != p || k > 4 || k < -4) return pow(x, p);
+// x^k for an integer k (reaction orders and stoichiometric coefficients are integers: the parser rejects
+// others), by repeated products as the reference's integer branch does.
+__device__ __forceinline__ double int_pow(double x, int k) {
+  const int n = k < 0 ? -k : k;
   double r = 1.0;
-  for (int q = 0; q < (k < 0 ? -k : k); ++q) r *= x;
+  for (int q = 0; q < n; ++q) r *= x;
   return k < 0 ? 1.0 / r : r;
-}
-
-__device__ __forceinline__ double mass_action(double c, double nu) {
-  // integer coefficients by repeated products (equal to pow() to rounding), others through pow()
-  const int k = (int)nu;
-  if ((double)k != nu) return pow(c, nu);
-  double r = 1.0;
-  for (int q = 0; q < k; ++q) r *= c;
-  return r;
 }
 
 // Chemical source omega [kg/(m^3 s)] of one cell (FiniteRateChem.cu:1466-1605).  With JAC, also the
 // Jacobian d omega / d(rho Y) with the rates of progress frozen (ConstVolumeReactor.cpp:1168-1187).
+// Stoichiometry is walked through each reaction's species lists (a few entries) rather than all species;
+// the Jacobian keeps static indices (register-resident) and skips rows and columns through the reaction's
+// species bit masks.
 template<int NS, bool JAC>
-__device__ void source(const ChemMech &m, double t, const double *rhoY, double *w, double *J, double *wd = nullptr) {
+__device__ __forceinline__ void source(const ChemMech &m, double t, const double *rhoY, double *w, double *J,
+                                       double *wd = nullptr) {
   double c[NS], g[NS], cr[NS], de[NS];
 #pragma unroll
   for (int l = 0; l < NS; ++l) {
@@ -102,6 +97,7 @@ __device__ void source(const ChemMech &m, double t, const double *rhoY, double *
   const double tp = kPatm / kRu * 1e-3 / t;
   for (int r = 0; r < m.nr; ++r) {
     const int ty = m.type[r];
+    const int nfs = m.nfs[r], nbs = m.nbs[r];
     double kf = arrhenius(lnT, iT, m.A[r], m.b[r], m.Ea[r]);
     double cc = 0;
     if (ty >= 4) {
@@ -138,30 +134,40 @@ __device__ void source(const ChemMech &m, double t, const double *rhoY, double *
     }
     if (m.need_gibbs && ty != 0 && ty != 2) {
       double dg = 0;
-#pragma unroll
-      for (int l = 0; l < NS; ++l) dg += g[l] * (m.nb[r][l] - m.nf[r][l]);
-      kb = kf / (order_pow(tp, m.order[r]) * exp(-dg));
+      for (int s = 0; s < nbs; ++s) dg += g[m.bsp[r][s]] * m.bnu[r][s];
+      for (int s = 0; s < nfs; ++s) dg -= g[m.fsp[r][s]] * m.fnu[r][s];
+      kb = kf / (int_pow(tp, m.iorder[r]) * exp(-dg));
     }
     double q1 = 1.0, q2 = ty != 0 ? 1.0 : 0.0;
-#pragma unroll
-    for (int l = 0; l < NS; ++l) {
-      if (m.nf[r][l] != 0) q1 *= mass_action(c[l], m.nf[r][l]);
-      if (ty != 0 && m.nb[r][l] != 0) q2 *= mass_action(c[l], m.nb[r][l]);
-    }
+    for (int s = 0; s < nfs; ++s) q1 *= int_pow(c[m.fsp[r][s]], m.fnu[r][s]);
+    if (ty != 0)
+      for (int s = 0; s < nbs; ++s) q2 *= int_pow(c[m.bsp[r][s]], m.bnu[r][s]);
     q1 *= kf;
     q2 *= kb;
-#pragma unroll
-    for (int l = 0; l < NS; ++l) {
-      cr[l] += q2 * m.nf[r][l] + q1 * m.nb[r][l];
-      de[l] += q1 * m.nf[r][l] + q2 * m.nb[r][l];
+    for (int s = 0; s < nfs; ++s) {
+      const int l = m.fsp[r][s];
+      const double nu = m.fnu[r][s];
+      cr[l] += q2 * nu;
+      de[l] += q1 * nu;
+    }
+    for (int s = 0; s < nbs; ++s) {
+      const int l = m.bsp[r][s];
+      const double nu = m.bnu[r][s];
+      cr[l] += q1 * nu;
+      de[l] += q2 * nu;
     }
     if constexpr (JAC) {
+      const unsigned um = m.umask[r], am = m.amask[r];
+      double v[NS];
+#pragma unroll
+      for (int n = 0; n < NS; ++n) v[n] = (um >> n & 1u) ? m.nf[r][n] * q1 - m.nb[r][n] * q2 : 0.0;
 #pragma unroll
       for (int a = 0; a < NS; ++a) {
-        const double dn = m.nb[r][a] - m.nf[r][a];
-        if (dn == 0) continue;
+        if (!(am >> a & 1u)) continue;
+        const double dn = m.dnu[r][a];
 #pragma unroll
-        for (int n = 0; n < NS; ++n) J[a * NS + n] += dn * (m.nf[r][n] * q1 - m.nb[r][n] * q2);
+        for (int n = 0; n < NS; ++n)
+          if (um >> n & 1u) J[a * NS + n] += dn * v[n];
       }
     }
   }

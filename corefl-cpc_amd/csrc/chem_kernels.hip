@@ -12,43 +12,47 @@
 #include <cmath>
 
 #include "chem_device.hpp"
+#include "rok4e.hpp"
 
 namespace corefl {
 
 namespace {
 
 using namespace chemdev;
-
-// ROK4E coefficients (Define.h:28-43)
-constexpr double G = 0.572816062482135;
-constexpr double A21 = 0.432364435748567, A31 = -0.51421131687617, A32 = 1.38227114461736;
-constexpr double G21 = -0.602765307997356 / G, G31 = -1.389195789724843 / G, G32 = 1.072950969011413 / G;
-constexpr double G41 = 0.992356412977094 / G, G42 = -1.390032613873701 / G, G43 = -0.440875890223325 / G;
-constexpr double B1 = 0.194335256262729, B2 = 0.483167813989227, B4 = 0.322496929748044;
-constexpr double E1 = -0.217819895945721 - B1, E2 = 1.03130847478467 - B2, E3 = 0.186511421161047, E4 = -B4;
+using namespace rok4e;
 
 // Doolittle LU with partial pivoting, row-major (ConstVolumeReactor.cpp:1231-1262), and its solve.
 template<int NS> __device__ __forceinline__ void lu(double *a, int *piv) {
 #pragma unroll
   for (int k = 0; k < NS; ++k) {
+    // first row of largest |a[r][k]|; the running maximum keeps every array index static, so the matrix
+    // stays in registers
     int ik = k;
+    double amax = 0;
 #pragma unroll
     for (int r = k; r < NS; ++r) {
 #pragma unroll
       for (int t = 0; t < k; ++t) a[r * NS + k] -= a[r * NS + t] * a[t * NS + k];
-      if (fabs(a[r * NS + k]) > fabs(a[ik * NS + k])) ik = r;
+      const double v = fabs(a[r * NS + k]);
+      if (r == k) {
+        amax = v;
+      } else if (v > amax) {
+        amax = v;
+        ik = r;
+      }
     }
     piv[k] = ik;
-    // row swap through selects keeps every index static
+    // row exchange through value selects: every index stays static and no branch is formed (a branch per
+    // candidate row lets the optimiser merge them into stores through a chosen address, which sends the
+    // whole matrix to scratch memory)
 #pragma unroll
     for (int r = k + 1; r < NS; ++r) {
-      if (r == ik) {
+      const bool sw = r == ik;
 #pragma unroll
-        for (int t = 0; t < NS; ++t) {
-          const double s = a[r * NS + t];
-          a[r * NS + t] = a[k * NS + t];
-          a[k * NS + t] = s;
-        }
+      for (int t = 0; t < NS; ++t) {
+        const double x = a[r * NS + t], y = a[k * NS + t];
+        a[r * NS + t] = sw ? y : x;
+        a[k * NS + t] = sw ? x : y;
       }
     }
 #pragma unroll
@@ -66,12 +70,12 @@ template<int NS> __device__ __forceinline__ void lu_solve(const double *a, const
 #pragma unroll
   for (int r = 0; r < NS; ++r) {
 #pragma unroll
-    for (int q = r + 1; q < NS; ++q)
-      if (piv[r] == q) {
-        const double s = b[q];
-        b[q] = b[r];
-        b[r] = s;
-      }
+    for (int q = r + 1; q < NS; ++q) {
+      const bool sw = piv[r] == q;
+      const double x = b[q], y = b[r];
+      b[q] = sw ? y : x;
+      b[r] = sw ? x : y;
+    }
   }
 #pragma unroll
   for (int r = 1; r < NS; ++r)
@@ -206,9 +210,24 @@ __global__ __launch_bounds__(256) void k_chem_static(const ChemMech *__restrict_
   store_cell<NS>(b, c, s, s.time >= s.dt);
 }
 
+// Integration state of a cell that used up its attempt budget, for the cooperative kernel to resume.
+template<int NS> __device__ __forceinline__ void store_partial(const ChemBatch &b, int c, const Cell<NS> &s) {
+#pragma unroll
+  for (int l = 0; l < NS; ++l) b.rhoY[(long)l * b.n + c] = s.y[l];
+  b.T[c] = s.T;
+  b.h[c] = s.h;
+  b.steps[c] = s.steps;
+  b.rejects[c] = s.rejects;
+  b.tcur[c] = s.time;
+  b.elast[c] = s.err_last;
+}
+
+// budget > 0: a cell still unfinished after `budget` attempts is handed over (appended to b.defer) instead
+// of holding its lane for the rest of its chain.
 template<int NS>
 __global__ __launch_bounds__(256) void k_chem_queue(const ChemMech *__restrict__ mp, ChemBatch b, double dt,
-                                                    double rtol, double atol, int max_attempts, int *counter) {
+                                                    double rtol, double atol, int max_attempts, int *counter,
+                                                    int budget) {
   const ChemMech &m = *mp;
   const int lane = __lane_id();
   const int n_work = work_count(b);
@@ -234,10 +253,25 @@ __global__ __launch_bounds__(256) void k_chem_queue(const ChemMech *__restrict__
       }
     }
     if (__ballot(cell >= 0) == 0) break;
+    bool defer = false;
     if (cell >= 0) {
       rok4e_attempt<NS>(m, s, rtol, atol);
       if (s.time >= s.dt || s.attempts >= max_attempts) {
         store_cell<NS>(b, cell, s, s.time >= s.dt);
+        cell = -1;
+      } else {
+        defer = budget > 0 && s.attempts >= budget;
+      }
+    }
+    const unsigned long long dm = __ballot(defer);
+    if (dm) {
+      const int leader = __ffsll((long long)dm) - 1;
+      int base = 0;
+      if (lane == leader) base = atomicAdd(b.n_defer, __popcll(dm));
+      base = __shfl(base, leader, 64);
+      if (defer) {
+        b.defer[base + __popcll(dm & ((1ull << lane) - 1))] = cell;
+        store_partial<NS>(b, cell, s);
         cell = -1;
       }
     }
@@ -300,9 +334,22 @@ hipError_t launch_chem_integrate(const ChemMech *mech_dev, int ns, const ChemBat
                                  hipStream_t s) {
   if (ns != 9) return hipErrorNotSupported;
   if (b.n == 0) return hipSuccess;
+  if (cfg.queue == 2) return launch_chem_coop(mech_dev, ns, b, dt, rtol, atol, max_attempts, cfg.threads,
+                                              cfg.blocks, false, counter, s);
+  if (cfg.queue == 3) {
+    if (cfg.budget < 1) return hipErrorInvalidValue;
+    k_zero_int<<<1, 1, 0, s>>>(counter);
+    k_zero_int<<<1, 1, 0, s>>>(b.n_defer);
+    k_chem_queue<9><<<cfg.blocks, cfg.threads, 0, s>>>(mech_dev, b, dt, rtol, atol, max_attempts, counter,
+                                                       cfg.budget);
+    ChemBatch d = b;
+    d.list = b.defer;
+    d.n_list = b.n_defer;
+    return launch_chem_coop(mech_dev, ns, d, dt, rtol, atol, max_attempts, 256, cfg.blocks, true, counter, s);
+  }
   if (cfg.queue) {
     k_zero_int<<<1, 1, 0, s>>>(counter);
-    k_chem_queue<9><<<cfg.blocks, cfg.threads, 0, s>>>(mech_dev, b, dt, rtol, atol, max_attempts, counter);
+    k_chem_queue<9><<<cfg.blocks, cfg.threads, 0, s>>>(mech_dev, b, dt, rtol, atol, max_attempts, counter, 0);
   } else {
     k_chem_static<9><<<(b.n + cfg.threads - 1) / cfg.threads, cfg.threads, 0, s>>>(mech_dev, b, dt, rtol, atol,
                                                                                   max_attempts);

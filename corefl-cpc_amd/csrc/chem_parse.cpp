@@ -6,6 +6,7 @@
 #include "chem.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <cctype>
 #include <cstring>
 #include <map>
@@ -313,9 +314,25 @@ ChemMech parse_mechanism(const std::string &mech_text, const std::string &therm_
     ++nr;
   }
   m.nr = nr;
-  for (int k = 0; k < m.ns; ++k) m.imw[k] = 1.0 / m.mw[k];
+  for (int k = 0; k < m.ns; ++k) {
+    m.imw[k] = 1.0 / m.mw[k];
+    m.ln_t_low[k] = std::log(m.t_low[k]);
+    m.ln_t_high[k] = std::log(m.t_high[k]);
+    m.inv_t_low[k] = 1.0 / m.t_low[k];
+    m.inv_t_high[k] = 1.0 / m.t_high[k];
+  }
   m.need_gibbs = 0;
   for (int r = 0; r < nr; ++r) m.need_gibbs |= m.type[r] != 0 && m.rev[r] != 1;
+  for (int r = 0; r < nr; ++r) {
+    m.umask[r] = m.amask[r] = 0;
+    for (int s = 0; s < m.ns; ++s) {
+      m.dnu[r][s] = m.nb[r][s] - m.nf[r][s];
+      if (m.nf[r][s] != 0 || m.nb[r][s] != 0) m.umask[r] |= 1u << s;
+      if (m.dnu[r][s] != 0) m.amask[r] |= 1u << s;
+    }
+    m.iorder[r] = (int)m.order[r];
+    if ((double)m.iorder[r] != m.order[r]) throw std::runtime_error("non-integer reaction order");
+  }
   return m;
 }
 

@@ -201,7 +201,10 @@ void surrogate_substep(corefl_chem *c, corefl_surrogate *s, const ChemBatch &b, 
   ChemBatch fb = b;
   fb.list = c->list;
   fb.n_list = c->list_count;
-  HIPCHK(launch_chem_integrate(c->mech_dev, c->mech.ns, fb, dt, rtol, atol, max_attempts, c->cfg, c->counter, st));
+  // The gated cells are the stiff ones (igniting, or outside the training range): the one-lane queue for
+  // four attempts, then the cooperative kernel for the long chains.
+  const ChemLaunch fallback{3, 64, 4 * s->cus, 4};
+  HIPCHK(launch_chem_integrate(c->mech_dev, c->mech.ns, fb, dt, rtol, atol, max_attempts, fallback, c->counter, st));
 }
 
 } // namespace corefl

@@ -188,14 +188,18 @@ int corefl_chem_source(corefl_chem *chem, double *omega_host);
 int corefl_chem_integrate(corefl_chem *chem, double dt, double rtol, double atol, int max_attempts, void *stream);
 
 /* Launch shape of corefl_chem_integrate (no reference counterpart: the reference integrates one cell per
- * thread, src/FiniteRateChem.cu:1388).  queue = 1: persistent waves fetching cells from a device counter,
- * `blocks` workgroups; queue = 0: one thread per cell.  threads is 64, 128 or 256. */
-int corefl_chem_set_launch(corefl_chem *chem, int queue, int threads, int blocks);
-int corefl_chem_get_launch(const corefl_chem *chem, int *queue, int *threads, int *blocks);
+ * thread, src/FiniteRateChem.cu:1388).  mode 0: one thread per cell; 1: persistent waves fetching cells
+ * from a device counter, `blocks` workgroups; 2: persistent waves integrating four cells at a time, 16 lanes
+ * per cell (lower latency per step attempt, for stiff cells); 3: mode 1 for at most `budget` attempts per
+ * cell, the unfinished cells then continued by mode 2.  All modes give the same result to rounding.
+ * threads is 64, 128 or 256; budget is used by mode 3 only. */
+int corefl_chem_set_launch(corefl_chem *chem, int mode, int threads, int blocks, int budget);
+int corefl_chem_get_launch(const corefl_chem *chem, int *mode, int *threads, int *blocks, int *budget);
 /* Times corefl_chem_integrate(dt, rtol, atol, max_attempts) on the current batch for each candidate launch
- * shape (static 64/128/256 threads, then queue 64/128/256 threads x 4/8/16 waves per CU), `reps` runs
- * each from the same saved state, keeps the fastest and restores the batch.  ms_out[q] (q < n_out) gets
- * the average milliseconds of candidate q.  No reference counterpart. */
+ * shape (mode 0 with 64/128/256 threads; mode 1 with 64/128/256 threads x 4/8/16 waves per CU; mode 2 at
+ * 4 and 8 waves per CU; mode 3 with budgets 4 and 16), `reps` runs each from the same saved state, keeps the
+ * fastest and restores the batch.  ms_out[q] (q < n_out) gets the average milliseconds of candidate q.
+ * No reference counterpart. */
 int corefl_chem_autotune(corefl_chem *chem, double dt, double rtol, double atol, int max_attempts, int reps,
                          double *ms_out, int n_out);
 int corefl_chem_synchronize(corefl_chem *chem);
