@@ -33,10 +33,14 @@ __device__ __forceinline__ unsigned short f2bf(float f) {
 
 // GELU, tanh form: 0.5 z (1 + tanh(u)) = z / (1 + exp(-2u)), u = sqrt(2/pi) (z + 0.044715 z^3).
 constexpr float kGs = 0.7978845608028654f, kGa = 0.044715f;
+// -2u log2(e) = z (c1 + c3 z^2)
+constexpr float kG1 = -2.0f * 0.7978845608028654f * 1.4426950408889634f;
+constexpr float kG3 = -2.0f * 0.7978845608028654f * 0.044715f * 1.4426950408889634f;
 __device__ __forceinline__ float sig2u(float z) {
-  const float u = kGs * (z + kGa * z * z * z);
-  // hardware reciprocal (1 ulp) instead of a correctly rounded division: the result is rounded to BF16
-  return __builtin_amdgcn_rcpf(1.0f + __expf(-2.0f * u));
+  const float w = z * __builtin_fmaf(kG3, z * z, kG1);
+  // hardware exp2 and reciprocal (1 ulp each) instead of expf and a correctly rounded division: the
+  // activation is rounded to BF16; exp2 overflowing to +inf gives 0, the z -> -inf limit
+  return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(w));
 }
 __device__ __forceinline__ float gelu(float z) { return z * sig2u(z); }
 __device__ __forceinline__ float gelu_grad(float z) {
@@ -207,39 +211,6 @@ __device__ __forceinline__ void load_rows(unsigned short *dst, int stride, const
   }
 }
 
-// One layer for the wave's 32 rows: acc[j] (column tile j of 32) = h[32][KSTEPS*16] * W^T rows j*32..
-template<int KSTEPS, int NT>
-__device__ __forceinline__ void mma_layer(const unsigned short *h, const unsigned short *w, int ws, floatx16 *acc,
-                                          int lane) {
-#pragma unroll
-  for (int j = 0; j < NT; ++j)
-#pragma unroll
-    for (int r = 0; r < 16; ++r) acc[j][r] = 0.0f;
-  const int row = lane & 31, kh = 8 * (lane >> 5);
-#pragma unroll
-  for (int ks = 0; ks < KSTEPS; ++ks) {
-    const shortx8 av = *reinterpret_cast<const shortx8 *>(&h[row * kSH + ks * 16 + kh]);
-#pragma unroll
-    for (int j = 0; j < NT; ++j) {
-      const shortx8 bv = *reinterpret_cast<const shortx8 *>(&w[(j * 32 + row) * ws + ks * 16 + kh]);
-      acc[j] = mfma(av, bv, acc[j]);
-    }
-  }
-}
-
-__device__ __forceinline__ void store_hidden(unsigned short *h, const floatx16 *acc, const float *bias, int lane) {
-#pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const int n = j * 32 + (lane & 31);
-    const float bn = bias[n];
-#pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      const int m = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-      h[m * kSH + n] = f2bf(gelu(acc[j][r] + bn));
-    }
-  }
-}
-
 // Mass fractions, pressure and the in-range flag of cell c; features scaled into f[0..12).
 __device__ __forceinline__ bool chem_features(const FusedArgs &a, long c, float *f) {
   const ChemBatch &b = a.b;
@@ -374,50 +345,144 @@ __device__ void chem_finish(const FusedArgs &a, long c, const float *o, bool in_
 // The network on n rows of BF16 features feat [n][16] -> FP32 outputs out [n][10].  Persistent: each
 // workgroup loads every weight into LDS once, then its 4 waves take 32-row tiles.  The layer-0 operand is
 // read straight from global memory (16 bytes per lane: row l % 32, features 8 (l / 32) .. +7).
-__global__ __launch_bounds__(256) void k_mlp(MlpDev net, const unsigned short *__restrict__ feat,
-                                             float *__restrict__ out, long n) {
+// Inference MLP, one pass over the batch.  The product of each layer is formed transposed, out^T = W^T h^T:
+// the weights (LDS-resident for the whole persistent launch) are the MFMA's A operand and the activations
+// its B operand.  The accumulator layout of out^T (lane = cell, 16 neurons per lane) then maps onto the next
+// layer's B operand with one exchange of half a neuron group between lanes l and l + 32, so activations stay
+// in registers from the features to the outputs.  A wave runs kMlpTiles 32-cell tiles at once (each weight
+// fragment read from LDS then feeds kMlpTiles MFMAs); eight waves per workgroup, two per SIMD, overlap one
+// wave's MFMAs with another's activation arithmetic, which bounds the kernel (measured on 2^20 rows: 0.276
+// ms at one tile and eight waves, 0.370 ms at two tiles and four waves).
+constexpr int kMlpTiles = 1, kMlpWaves = 8;
+
+typedef unsigned uintx4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float floatx2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ unsigned pack_bf16(float a, float b) {
+  const floatx2 v = {a, b};
+  return __builtin_bit_cast(unsigned, __builtin_convertvector(v, bf16x2)); // round to nearest even
+}
+
+// GELU of acc + bias for one 32-neuron block, packed as BF16 pairs: p[q][0..1] holds neurons
+// 8q + 4h + 0..3 of the block (h = lane / 32).
+__device__ __forceinline__ void gelu_pack(const floatx16 &acc, const float *bias, int h, unsigned (&p)[4][2]) {
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const float4 b = *reinterpret_cast<const float4 *>(&bias[8 * q + 4 * h]);
+    p[q][0] = pack_bf16(gelu(acc[4 * q] + b.x), gelu(acc[4 * q + 1] + b.y));
+    p[q][1] = pack_bf16(gelu(acc[4 * q + 2] + b.z), gelu(acc[4 * q + 3] + b.w));
+  }
+}
+
+// B fragments of the next layer for the two k-steps 2j, 2j + 1 covered by block j (16 neurons each): lane
+// (cell, h) needs neurons 16p + 8h + 0..7 of the block, half of which its partner lane (l ^ 32) holds.
+__device__ __forceinline__ void next_operand(const unsigned (&p)[4][2], int h, shortx8 &b0, shortx8 &b1) {
+#pragma unroll
+  for (int par = 0; par < 2; ++par) {
+    const int own = 2 * par + h; // group this lane needs: its own half, and the partner's
+    const unsigned s0 = h == 0 ? p[2 * par + 1][0] : p[2 * par][0];
+    const unsigned s1 = h == 0 ? p[2 * par + 1][1] : p[2 * par][1];
+    const unsigned r0 = __shfl_xor(s0, 32), r1 = __shfl_xor(s1, 32);
+    uintx4 v;
+    if (h == 0) v = uintx4{p[own][0], p[own][1], r0, r1};
+    else v = uintx4{r0, r1, p[own][0], p[own][1]};
+    (par == 0 ? b0 : b1) = __builtin_bit_cast(shortx8, v);
+  }
+}
+
+__global__ __launch_bounds__(64 * kMlpWaves) void k_mlp(MlpDev net, const unsigned short *__restrict__ feat,
+                                                         float *__restrict__ out, long n) {
   __shared__ __attribute__((aligned(16))) unsigned short w0[128 * kS0];
   __shared__ __attribute__((aligned(16))) unsigned short wh[3][128 * kSH];
   __shared__ __attribute__((aligned(16))) unsigned short w4[32 * kSH];
-  __shared__ __attribute__((aligned(16))) unsigned short act[4][32 * kSH];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  __shared__ __attribute__((aligned(16))) float bias[4][128];
+  __shared__ __attribute__((aligned(16))) float bias4[32];
   load_rows(w0, kS0, net.wt[0], 128, 16);
   for (int l = 1; l < 4; ++l) load_rows(wh[l - 1], kSH, net.wt[l], 128, 128);
   load_rows(w4, kSH, net.wt[4], 32, 128);
+  for (int q = threadIdx.x; q < 4 * 128; q += blockDim.x) bias[q / 128][q % 128] = net.bias[q / 128][q % 128];
+  for (int q = threadIdx.x; q < 32; q += blockDim.x) bias4[q] = q < kMlpOut ? net.bias[4][q] : 0.0f;
   __syncthreads();
-  unsigned short *h = act[wave];
-  const long n_tiles = (n + 127) / 128;
-  const int row = lane & 31, kh = 8 * (lane >> 5);
-  for (long tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
-    const long c0 = tile * 128 + wave * 32;
-    shortx8 av = {0, 0, 0, 0, 0, 0, 0, 0};
-    if (c0 + row < n) av = *reinterpret_cast<const shortx8 *>(&feat[(c0 + row) * 16 + kh]);
-    floatx16 acc[4];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int cell = lane & 31, h = lane >> 5, kh = 8 * h;
+  constexpr long kCellsPerWave = 32 * kMlpTiles;
+  const long n_units = (n + kCellsPerWave - 1) / kCellsPerWave;
+  const int waves = gridDim.x * kMlpWaves;
+  for (long unit = (long)blockIdx.x * kMlpWaves + wave; unit < n_units; unit += waves) {
+    const long c0 = unit * kCellsPerWave;
+    floatx16 acc[kMlpTiles][4];
+    shortx8 bop[kMlpTiles][8]; // B operands of the current layer, k-steps 0..7
+    // layer 0: 16 features (12 used) -> 128
+    {
+      shortx8 f[kMlpTiles];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
+      for (int t = 0; t < kMlpTiles; ++t) {
+        const long c = c0 + 32 * t + cell;
+        f[t] = shortx8{0, 0, 0, 0, 0, 0, 0, 0};
+        if (c < n) f[t] = *reinterpret_cast<const shortx8 *>(&feat[c * 16 + kh]);
+      }
 #pragma unroll
-      for (int r = 0; r < 16; ++r) acc[j][r] = 0.0f;
-      const shortx8 bv = *reinterpret_cast<const shortx8 *>(&w0[(j * 32 + row) * kS0 + kh]);
-      acc[j] = mfma(av, bv, acc[j]);
-    }
-    store_hidden(h, acc, net.bias[0], lane);
-    __syncthreads();
-    for (int l = 0; l < 3; ++l) {
-      mma_layer<8, 4>(h, wh[l], kSH, acc, lane);
-      __syncthreads();
-      store_hidden(h, acc, net.bias[l + 1], lane);
-      __syncthreads();
-    }
-    mma_layer<8, 1>(h, w4, kSH, acc, lane);
-    if (row < kMlpOut) {
-      const float bj = net.bias[4][row];
+      for (int j = 0; j < 4; ++j) {
+        const shortx8 a = *reinterpret_cast<const shortx8 *>(&w0[(j * 32 + cell) * kS0 + kh]);
 #pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const long c = c0 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-        if (c < n) out[c * kMlpOut + row] = acc[0][r] + bj;
+        for (int t = 0; t < kMlpTiles; ++t) {
+#pragma unroll
+          for (int r = 0; r < 16; ++r) acc[t][j][r] = 0.0f;
+          acc[t][j] = mfma(a, f[t], acc[t][j]);
+        }
       }
     }
-    __syncthreads();
+#pragma unroll
+    for (int l = 0; l < 4; ++l) {
+      // activation of layer l's output -> B operands of layer l + 1
+#pragma unroll
+      for (int t = 0; t < kMlpTiles; ++t)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          unsigned p[4][2];
+          gelu_pack(acc[t][j], &bias[l][32 * j], h, p);
+          next_operand(p, h, bop[t][2 * j], bop[t][2 * j + 1]);
+        }
+      if (l == 3) break;
+      const unsigned short *w = wh[l];
+#pragma unroll
+      for (int t = 0; t < kMlpTiles; ++t)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+          for (int r = 0; r < 16; ++r) acc[t][j][r] = 0.0f;
+#pragma unroll
+      for (int ks = 0; ks < 8; ++ks)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const shortx8 a = *reinterpret_cast<const shortx8 *>(&w[(j * 32 + cell) * kSH + ks * 16 + kh]);
+#pragma unroll
+          for (int t = 0; t < kMlpTiles; ++t) acc[t][j] = mfma(a, bop[t][ks], acc[t][j]);
+        }
+    }
+    // output layer: 128 -> 10 (32-wide tile)
+    floatx16 o[kMlpTiles];
+#pragma unroll
+    for (int t = 0; t < kMlpTiles; ++t)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) o[t][r] = 0.0f;
+#pragma unroll
+    for (int ks = 0; ks < 8; ++ks) {
+      const shortx8 a = *reinterpret_cast<const shortx8 *>(&w4[cell * kSH + ks * 16 + kh]);
+#pragma unroll
+      for (int t = 0; t < kMlpTiles; ++t) o[t] = mfma(a, bop[t][ks], o[t]);
+    }
+#pragma unroll
+    for (int t = 0; t < kMlpTiles; ++t) {
+      const long c = c0 + 32 * t + cell;
+      if (c >= n) continue;
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int j = 8 * (r >> 2) + 4 * h + (r & 3);
+        if (j < kMlpOut) out[c * kMlpOut + j] = o[t][r] + bias4[j];
+      }
+    }
   }
 }
 
@@ -522,8 +587,8 @@ hipError_t launch_rows_to_feat(const float *X, long n, unsigned short *feat, hip
 hipError_t launch_mlp_raw(const MlpDev &net, long n, const unsigned short *feat, float *y, int blocks,
                           hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  const long tiles = (n + 127) / 128;
-  k_mlp<<<(unsigned)std::min<long>(tiles, blocks), 256, 0, s>>>(net, feat, y, n);
+  const long units = (n + 32 * kMlpTiles * kMlpWaves - 1) / (32 * kMlpTiles * kMlpWaves);
+  k_mlp<<<(unsigned)std::min<long>(units, blocks), 64 * kMlpWaves, 0, s>>>(net, feat, y, n);
   return hipGetLastError();
 }
 
@@ -544,8 +609,8 @@ hipError_t launch_mlp_chem(const MlpDev &net, const ChemMech *mech_dev, const Ch
   a.fb_count = fb_count;
   const unsigned g = grid1(b.n, 256);
   k_chem_feat<<<g, 256, 0, s>>>(a, w.feat, w.flag);
-  const long tiles = (b.n + 127) / 128;
-  k_mlp<<<(unsigned)std::min<long>(tiles, blocks), 256, 0, s>>>(net, w.feat, w.out, b.n);
+  const long units = (b.n + 32 * kMlpTiles * kMlpWaves - 1) / (32 * kMlpTiles * kMlpWaves);
+  k_mlp<<<(unsigned)std::min<long>(units, blocks), 64 * kMlpWaves, 0, s>>>(net, w.feat, w.out, b.n);
   k_chem_post<<<g, 256, 0, s>>>(a, w.out, w.flag);
   return hipGetLastError();
 }
