@@ -414,7 +414,7 @@ def unflatten_params(flat, sizes=SURROGATE_SIZES):
 
 
 class Surrogate:
-    """Chemistry surrogate MLP (12 -> 4x128 GELU -> 10) trained and run on the GPU."""
+    """Chemistry surrogate MLP (12 -> 4x128 hard swish -> 10) trained and run on the GPU."""
 
     def __init__(self, batch_capacity=8192):
         lib = load_library()

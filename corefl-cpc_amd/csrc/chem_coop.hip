@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <type_traits>
 
 #include "chem_device.hpp"
 #include "chem_kernels.hpp"
@@ -32,7 +33,7 @@ using namespace rok4e;
 
 constexpr int RW = 16; // lanes per cell (one DPP row)
 
-// value of lane k of this lane's 16-lane row (DPP row_newbcast; k is a constant after unrolling)
+// value of lane K of this lane's 16-lane row (DPP row_newbcast)
 template<int K> __device__ __forceinline__ double row_bcast_c(double x) {
   int2 v = __builtin_bit_cast(int2, x);
   v.x = __builtin_amdgcn_update_dpp(0, v.x, 0x150 + K, 0xF, 0xF, false);
@@ -40,31 +41,18 @@ template<int K> __device__ __forceinline__ double row_bcast_c(double x) {
   return __builtin_bit_cast(double, v);
 }
 
-__device__ __forceinline__ double row_bcast(double x, int k) {
-  switch (k) {
-  case 0: return row_bcast_c<0>(x);
-  case 1: return row_bcast_c<1>(x);
-  case 2: return row_bcast_c<2>(x);
-  case 3: return row_bcast_c<3>(x);
-  case 4: return row_bcast_c<4>(x);
-  case 5: return row_bcast_c<5>(x);
-  case 6: return row_bcast_c<6>(x);
-  case 7: return row_bcast_c<7>(x);
-  case 8: return row_bcast_c<8>(x);
-  case 9: return row_bcast_c<9>(x);
-  case 10: return row_bcast_c<10>(x);
-  case 11: return row_bcast_c<11>(x);
-  case 12: return row_bcast_c<12>(x);
-  case 13: return row_bcast_c<13>(x);
-  case 14: return row_bcast_c<14>(x);
-  default: return row_bcast_c<15>(x);
+// f(std::integral_constant<int, I>) for I = B .. E-1, unrolled at compile time: every DPP control operand
+// is then a constant (a loop of broadcasts would be unrolled only after the compiler's size estimate of it)
+template<int B, int E, class F> __device__ __forceinline__ void static_for(F &&f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    static_for<B + 1, E>(f);
   }
 }
 
 // all[l] = lane l's x, l < NS
 template<int NS> __device__ __forceinline__ void row_gather(double x, double *all) {
-#pragma unroll
-  for (int l = 0; l < NS; ++l) all[l] = row_bcast(x, l);
+  static_for<0, NS>([&](auto L) { all[decltype(L)::value] = row_bcast_c<decltype(L)::value>(x); });
 }
 
 // Orders the LDS traffic of one wavefront: stores before this point are seen by loads after it.
@@ -282,12 +270,10 @@ __device__ __forceinline__ double row_source(const ChemMech &m, const SpeciesLds
 // row a of a permuted right-hand side takes.
 template<int NS> __device__ __forceinline__ int row_lu(double *row, int a) {
   int piv[NS];
-#pragma unroll
-  for (int k = 0; k < NS; ++k) {
+  static_for<0, NS>([&](auto K) {
+    constexpr int k = decltype(K)::value;
     double col[NS];
-    // constant trip count: a loop of broadcasts (convergent operations) unrolls only when it is known
-#pragma unroll
-    for (int r = 0; r < NS; ++r) col[r] = r >= k ? row_bcast(row[k], r) : 0.0;
+    static_for<k, NS>([&](auto R) { col[decltype(R)::value] = row_bcast_c<decltype(R)::value>(row[k]); });
     int ik = k;
     double amax = 0;
 #pragma unroll
@@ -305,7 +291,7 @@ template<int NS> __device__ __forceinline__ int row_lu(double *row, int a) {
 #pragma unroll
     for (int t = 0; t < NS; ++t) {
       prow[t] = __shfl(row[t], ik, RW);
-      krow[t] = row_bcast(row[t], k);
+      krow[t] = row_bcast_c<k>(row[t]);
     }
 #pragma unroll
     for (int t = 0; t < NS; ++t) row[t] = a == k ? prow[t] : (a == ik ? krow[t] : row[t]);
@@ -318,7 +304,7 @@ template<int NS> __device__ __forceinline__ int row_lu(double *row, int a) {
     } else if (a == k) {
       row[k] = inv;
     }
-  }
+  });
   // the sequential exchanges b[r] <-> b[piv[r]] of the solve, traced back from position a
   int pos = a;
 #pragma unroll
@@ -332,24 +318,24 @@ template<int NS> __device__ __forceinline__ int row_lu(double *row, int a) {
 // Solve with the factors of row_lu: lane a holds b_a on entry and x_a on return.
 template<int NS> __device__ __forceinline__ double row_solve(const double *row, int a, int src, double b) {
   b = __shfl(b, src, RW);
-#pragma unroll
-  for (int t = 0; t < NS - 1; ++t) {
-    const double bt = row_bcast(b, t);
+  static_for<0, NS - 1>([&](auto T) {
+    constexpr int t = decltype(T)::value;
+    const double bt = row_bcast_c<t>(b);
     const double nb = b - row[t] * bt;
     b = a > t ? nb : b;
-  }
+  });
   if (a == NS - 1) b *= row[NS - 1];
   double fin[NS];
-  fin[NS - 1] = row_bcast(b, NS - 1);
-#pragma unroll
-  for (int r = NS - 2; r >= 0; --r) {
+  fin[NS - 1] = row_bcast_c<NS - 1>(b);
+  static_for<0, NS - 1>([&](auto I) {
+    constexpr int r = NS - 2 - decltype(I)::value;
     double v = b;
 #pragma unroll
     for (int t = r + 1; t < NS; ++t) v -= row[t] * fin[t];
     v *= row[r];
     b = a == r ? v : b;
-    fin[r] = row_bcast(b, r);
-  }
+    fin[r] = row_bcast_c<r>(b);
+  });
   return b;
 }
 

@@ -1,4 +1,4 @@
-// Chemistry surrogate: a 12 -> 4x128 (GELU) -> 10 MLP with BF16 weights and FP32 accumulation on the
+// Chemistry surrogate: a 12 -> 4x128 (hard swish) -> 10 MLP with BF16 weights and FP32 accumulation on the
 // gfx950 matrix cores.  Training (forward, backward, Adam) runs as tiled MFMA GEMMs with fused epilogues;
 // inference runs as one persistent kernel holding every weight in LDS, fused with the feature scaling, the
 // element-conserving projection, the temperature recovery and the fallback gate.
@@ -49,9 +49,9 @@ struct ElementMatrix {
 
 // C = A B with A(m,k) = A[m*sam + k*sak], B(k,n) = B[k*sbk + n*sbn] in BF16; FP32 accumulation.
 enum GemmEpilogue {
-  kEpFwdHidden = 0, // z = acc + bias: C (FP32) = z, Cb (BF16) = gelu(z)
+  kEpFwdHidden = 0, // z = acc + bias: C (FP32) = z, Cb (BF16) = act(z)
   kEpFwdOut = 1,    // C = acc + bias
-  kEpBwdData = 2,   // Cb = acc * gelu'(zprev)
+  kEpBwdData = 2,   // Cb = acc * act'(zprev)
   kEpAtomic = 3     // C += acc (split over K by gridDim.z)
 };
 

@@ -205,8 +205,8 @@ int corefl_chem_autotune(corefl_chem *chem, double dt, double rtol, double atol,
 int corefl_chem_synchronize(corefl_chem *chem);
 
 /* ---------------------------------------------------------------------------------------------------
- * Chemistry surrogate (no reference counterpart: a method change).  An MLP 12 -> 4 x 128 (GELU, tanh
- * form) -> 10 with BF16 weights and FP32 accumulation on the matrix cores maps the features
+ * Chemistry surrogate (no reference counterpart: a method change).  An MLP 12 -> 4 x 128 (hard swish,
+ * z clamp(z/6 + 1/2, 0, 1)) -> 10 with BF16 weights and FP32 accumulation on the matrix cores maps the features
  * (T, ln p, Y_1..Y_9, ln dt), each scaled as (f - mu) / sig, to the scaled rates (dY_k/dt, dT/dt).  It is
  * trained on the GPU (forward, backward and Adam kernels of this library) on states labelled by the exact
  * integrator, and replaces the exact chemistry sub-step in cells that pass its gate.

@@ -5,16 +5,15 @@ checker.  The surrogate has no counterpart in the reference: it is pinned by fin
 checks of this restatement (tests/test_surrogate_oracle_cpu.py), and the GPU kernels are compared with it
 with the GPU's BF16 operand rounding emulated (`emulate=True`).
 
-Network: 12 -> 128 -> 128 -> 128 -> 128 -> 10, tanh-form GELU on the four hidden layers, linear output.
+Network: 12 -> 128 -> 128 -> 128 -> 128 -> 10, hard swish z clamp(z/6 + 1/2, 0, 1) (Howard et al.,
+"Searching for MobileNetV3", 2019) on the four hidden layers, linear output.
 GPU numerics emulated here: every GEMM operand (layer inputs, weights, back-propagated gradients) is
 rounded to BF16 (round to nearest even), products are accumulated in FP32 on the GPU (FP64 here), and
-pre-activations, GELU and the loss are FP32 on the GPU (FP64 here).
+pre-activations, activations and the loss are FP32 on the GPU (FP64 here).
 """
 import numpy as np
 
 SIZES = (12, 128, 128, 128, 128, 10)
-GELU_S = np.sqrt(2.0 / np.pi)
-GELU_A = 0.044715
 
 
 def bf16(x):
@@ -25,13 +24,14 @@ def bf16(x):
     return u.astype(np.uint32).view(np.float32).astype(np.float64)
 
 
-def gelu(z):
-    return 0.5 * z * (1.0 + np.tanh(GELU_S * (z + GELU_A * z ** 3)))
+def act(z):
+    """Hard swish."""
+    return z * np.clip(z / 6.0 + 0.5, 0.0, 1.0)
 
 
-def gelu_grad(z):
-    t = np.tanh(GELU_S * (z + GELU_A * z ** 3))
-    return 0.5 * (1.0 + t) + 0.5 * z * (1.0 - t * t) * GELU_S * (1.0 + 3.0 * GELU_A * z * z)
+def act_grad(z):
+    """Its derivative: 0 below -3, 1 above 3, z/3 + 1/2 between (the kinks take the outer values)."""
+    return np.where(z <= -3.0, 0.0, np.where(z >= 3.0, 1.0, z / 3.0 + 0.5))
 
 
 def init_params(seed, sizes=SIZES):
@@ -56,7 +56,7 @@ def forward(params, X, emulate=True):
         Z = H @ Wr + b
         cache.append((H, Wr, Z))
         if l < L - 1:
-            H = r(gelu(Z))
+            H = r(act(Z))
     return Z, cache
 
 
@@ -73,7 +73,7 @@ def loss_and_grads(params, X, Y, emulate=True):
         grads[l] = (H.T @ G, G.sum(axis=0))
         if l > 0:
             dH = G @ Wr.T
-            G = r(dH * gelu_grad(cache[l - 1][2]))
+            G = r(dH * act_grad(cache[l - 1][2]))
     return loss, grads
 
 

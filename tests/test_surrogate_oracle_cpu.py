@@ -1,5 +1,5 @@
 """Pins of the surrogate MLP restatement (oracle/mlp_oracle.py), CPU only: backward pass against central
-finite differences of the loss, GELU derivative against finite differences, BF16 rounding against
+finite differences of the loss, activation derivative against finite differences, BF16 rounding against
 hand-checked bit patterns, and an Adam step against its closed form for a single scalar."""
 import os
 import sys
@@ -19,13 +19,16 @@ def test_bf16_rounding():
     assert abs(got[6] - 1e-3) / 1e-3 < 2 ** -8
 
 
-def test_gelu_grad_matches_finite_difference():
-    z = np.linspace(-6, 6, 101)
+def test_activation_and_grad():
+    z = np.linspace(-6.05, 5.95, 101)  # no sample at the kinks z = +-3
+    assert np.min(np.abs(np.abs(z) - 3.0)) > 1e-3
     h = 1e-6
-    fd = (M.gelu(z + h) - M.gelu(z - h)) / (2 * h)
-    np.testing.assert_allclose(M.gelu_grad(z), fd, rtol=1e-7, atol=1e-9)
-    # known values: gelu(0) = 0, gelu(x) -> x for large x
-    assert M.gelu(0.0) == 0.0 and abs(M.gelu(10.0) - 10.0) < 1e-12
+    fd = (M.act(z + h) - M.act(z - h)) / (2 * h)
+    np.testing.assert_allclose(M.act_grad(z), fd, rtol=1e-7, atol=1e-9)
+    # closed forms: 0 at and below -3, identity at and above 3, z (z + 3) / 6 between
+    assert M.act(0.0) == 0.0 and M.act(-3.0) == 0.0 and M.act(3.0) == 3.0
+    assert M.act(-5.0) == 0.0 and M.act(10.0) == 10.0
+    assert abs(M.act(1.0) - 2.0 / 3.0) < 1e-15 and abs(M.act(-1.5) + 0.375) < 1e-15
 
 
 def test_backward_matches_finite_differences():
