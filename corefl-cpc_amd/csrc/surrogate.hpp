@@ -45,6 +45,9 @@ struct SurrogateGate {
 struct ElementMatrix {
   int ne;
   double E[4][kMaxSpec];
+  // inverse of the projection's normal matrix E E^T (rows e >= ne: identity) when no species is held at
+  // zero, the common case
+  double M0inv[4][4];
 };
 
 // C = A B with A(m,k) = A[m*sam + k*sak], B(k,n) = B[k*sbk + n*sbn] in BF16; FP32 accumulation.

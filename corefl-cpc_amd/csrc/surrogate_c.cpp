@@ -194,6 +194,35 @@ void surrogate_substep(corefl_chem *c, corefl_surrogate *s, const ChemBatch &b, 
   em.ne = mech.ne;
   for (int e = 0; e < mech.ne; ++e)
     for (int k = 0; k < mech.ns; ++k) em.E[e][k] = mech.comp[k][e] / mech.mw[k];
+  // (E E^T)^-1 by Gauss-Jordan elimination with partial pivoting
+  double A[4][8] = {};
+  for (int e = 0; e < 4; ++e) {
+    for (int f = 0; f < 4; ++f) {
+      double v = e == f ? 1.0 : 0.0;
+      if (e < em.ne && f < em.ne) {
+        v = 0;
+        for (int k = 0; k < mech.ns; ++k) v += em.E[e][k] * em.E[f][k];
+      }
+      A[e][f] = v;
+    }
+    A[e][4 + e] = 1.0;
+  }
+  for (int k = 0; k < 4; ++k) {
+    int p = k;
+    for (int r = k + 1; r < 4; ++r)
+      if (std::fabs(A[r][k]) > std::fabs(A[p][k])) p = r;
+    if (!(std::fabs(A[p][k]) > 0)) throw std::runtime_error("surrogate projection: singular element matrix");
+    for (int c = 0; c < 8; ++c) std::swap(A[k][c], A[p][c]);
+    const double inv = 1.0 / A[k][k];
+    for (int c = 0; c < 8; ++c) A[k][c] *= inv;
+    for (int r = 0; r < 4; ++r) {
+      if (r == k) continue;
+      const double f = A[r][k];
+      for (int c = 0; c < 8; ++c) A[r][c] -= f * A[k][c];
+    }
+  }
+  for (int e = 0; e < 4; ++e)
+    for (int f = 0; f < 4; ++f) em.M0inv[e][f] = A[e][4 + f];
   // dt of the launch: only used when the batch has neither per-cell nor device-side lengths
   const double dt = s->step_dt;
   const SurrogateWork w{c->sfeat, c->sout, c->sflag};
@@ -201,9 +230,9 @@ void surrogate_substep(corefl_chem *c, corefl_surrogate *s, const ChemBatch &b, 
   ChemBatch fb = b;
   fb.list = c->list;
   fb.n_list = c->list_count;
-  // The gated cells are the stiff ones (igniting, or outside the training range): the one-lane queue for
-  // four attempts, then the cooperative kernel for the long chains.
-  const ChemLaunch fallback{3, 64, 4 * s->cus, 4};
+  // The gated cells are the stiff ones (igniting, or outside the training range) and few: the cooperative
+  // kernel (lowest latency per step attempt) at two workgroups of four waves per CU.
+  const ChemLaunch fallback{2, 256, 2 * s->cus, 0};
   HIPCHK(launch_chem_integrate(c->mech_dev, c->mech.ns, fb, dt, rtol, atol, max_attempts, fallback, c->counter, st));
 }
 

@@ -212,23 +212,34 @@ __device__ __forceinline__ bool chem_features(const FusedArgs &a, long c, float 
   const ChemBatch &b = a.b;
   const ChemMech &m = *a.mech;
   const double rho = b.rho[c], T = b.T[c], dt = cell_dt(b, c, a.dt);
+  const double irho = 1.0 / rho;
   double imw = 0.0;
   double y[9];
 #pragma unroll
   for (int k = 0; k < 9; ++k) {
-    y[k] = b.rhoY[(long)k * b.n + c] / rho;
+    y[k] = b.rhoY[(long)k * b.n + c] * irho;
     imw += y[k] * m.imw[k];
   }
   const double p = rho * chemdev::kRu * imw * T;
   const SurrogateScaling &s = a.sc;
-  f[0] = (float)((T - s.in_mu[0]) / s.in_sig[0]);
-  f[1] = (float)((log(p) - s.in_mu[1]) / s.in_sig[1]);
+  // The features are rounded to BF16 (8 significant bits) for the network: FP32 arithmetic with the
+  // hardware log2 / exp2 (1 ulp) is exact to far below that.
+  f[0] = ((float)T - s.in_mu[0]) / s.in_sig[0];
+  f[1] = (__builtin_amdgcn_logf((float)p) * 0.6931471805599453f - s.in_mu[1]) / s.in_sig[1];
+  const float ap = (float)s.y_power;
 #pragma unroll
   for (int k = 0; k < 9; ++k) {
-    const double yf = s.y_power == 1.0 ? y[k] : (y[k] > 0 ? exp(s.y_power * log(y[k])) : 0.0);
-    f[2 + k] = (float)((yf - s.in_mu[2 + k]) / s.in_sig[2 + k]);
+    float yf;
+    if (s.y_power == 1.0) {
+      yf = (float)y[k];
+    } else {
+      // y^a = 2^(a log2 y); y scaled by 2^64 first so that mass fractions below the FP32 normal range keep
+      // their logarithm
+      yf = y[k] > 0 ? __builtin_amdgcn_exp2f(ap * (__builtin_amdgcn_logf((float)(y[k] * 0x1p64)) - 64.0f)) : 0.0f;
+    }
+    f[2 + k] = (yf - s.in_mu[2 + k]) / s.in_sig[2 + k];
   }
-  f[11] = (float)((log(dt) - s.in_mu[11]) / s.in_sig[11]);
+  f[11] = (__builtin_amdgcn_logf((float)dt) * 0.6931471805599453f - s.in_mu[11]) / s.in_sig[11];
   const SurrogateGate &g = a.gate;
   return T >= g.t_min && T <= g.t_max && p >= g.p_min && p <= g.p_max && dt >= g.dt_min && dt <= g.dt_max;
 }
@@ -257,12 +268,14 @@ __device__ __forceinline__ bool spd_solve4(double (&M)[4][4], double (&r)[4]) {
   return true;
 }
 
-// Network output q -> rate (inverse of the signed-logarithm target transform).
+// Network output q -> rate (inverse of the signed-logarithm target transform).  FP32: the network's
+// outputs carry BF16-level error, and r0 (e^|v| - 1) is taken with the hardware exp2 (its absolute error
+// near v = 0, r0 * 1e-7, is far below the rates that matter).
 __device__ __forceinline__ double rate_of(const SurrogateScaling &s, int q, float o) {
-  const double v = (double)o * s.out_sig[q] + s.out_mu[q];
-  if (!(s.out_r0[q] > 0)) return v;
-  const double m = s.out_r0[q] * expm1(fabs(v));
-  return v < 0 ? -m : m;
+  const float v = o * (float)s.out_sig[q] + (float)s.out_mu[q];
+  if (!(s.out_r0[q] > 0)) return (double)v;
+  const float m = (float)s.out_r0[q] * (__builtin_amdgcn_exp2f(fabsf(v) * 1.4426950408889634f) - 1.0f);
+  return (double)(v < 0 ? -m : m);
 }
 
 // Network outputs o[10] of cell c -> projected mass fractions and energy-consistent temperature; writes the
@@ -271,10 +284,11 @@ __device__ void chem_finish(const FusedArgs &a, long c, const float *o, bool in_
   const ChemBatch &b = a.b;
   const ChemMech &m = *a.mech;
   const double rho = b.rho[c], T = b.T[c], dt = cell_dt(b, c, a.dt);
+  const double irho = 1.0 / rho;
   double y0[9], dy[9];
 #pragma unroll
   for (int k = 0; k < 9; ++k) {
-    y0[k] = b.rhoY[(long)k * b.n + c] / rho;
+    y0[k] = b.rhoY[(long)k * b.n + c] * irho;
     dy[k] = dt * rate_of(a.sc, k, o[k]);
   }
   const double t_net = T + dt * rate_of(a.sc, 9, o[9]);
@@ -284,25 +298,41 @@ __device__ void chem_finish(const FusedArgs &a, long c, const float *o, bool in_
   bool ok = in_range;
   double yn[9];
   for (int pass = 0; pass < 4 && ok; ++pass) {
-    double res[4], M[4][4];
+    double res[4];
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       double r = 0.0;
 #pragma unroll
       for (int k = 0; k < 9; ++k) r += a.em.E[e][k] * ((clamp >> k) & 1u ? -y0[k] : dy[k]);
       res[e] = r;
-#pragma unroll
-      for (int f = 0; f < 4; ++f) {
-        double s = 0.0;
-#pragma unroll
-        for (int k = 0; k < 9; ++k)
-          if (!((clamp >> k) & 1u)) s += a.em.E[e][k] * a.em.E[f][k];
-        M[e][f] = (e >= a.em.ne || f >= a.em.ne) ? (e == f ? 1.0 : 0.0) : s;
-      }
     }
-    if (!spd_solve4(M, res)) {
-      ok = false;
-      break;
+    if (clamp == 0) {
+      // no species held at zero: the normal matrix is the mechanism's, inverted on the host
+      double x[4];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        x[e] = 0.0;
+#pragma unroll
+        for (int f = 0; f < 4; ++f) x[e] += a.em.M0inv[e][f] * res[f];
+      }
+#pragma unroll
+      for (int e = 0; e < 4; ++e) res[e] = x[e];
+    } else {
+      double M[4][4];
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+#pragma unroll
+        for (int f = 0; f < 4; ++f) {
+          double s = 0.0;
+#pragma unroll
+          for (int k = 0; k < 9; ++k)
+            if (!((clamp >> k) & 1u)) s += a.em.E[e][k] * a.em.E[f][k];
+          M[e][f] = (e >= a.em.ne || f >= a.em.ne) ? (e == f ? 1.0 : 0.0) : s;
+        }
+      if (!spd_solve4(M, res)) {
+        ok = false;
+        break;
+      }
     }
     unsigned neg = 0;
 #pragma unroll

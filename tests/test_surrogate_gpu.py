@@ -146,7 +146,7 @@ def test_closed_gate_falls_back_to_exact(net):
     nf = s.chem_step(ch, 1e-7)
     got = ch.get_cells()
     ch.set_cells(rho, T, rho * Y)
-    ch.set_launch(3, 64, 64, 4)  # the fallback's launch: the same operations, so the same bits
+    ch.set_launch(2, 256, 16)  # the fallback's launch: the same operations, so the same bits
     ch.integrate(1e-7)
     ref = ch.get_cells()
     assert nf == T.size
@@ -166,7 +166,7 @@ def test_partial_gate_mixes_paths(net):
     assert nf == int(hot.sum())
     assert np.all(g["steps"][hot] > 0) and np.all(g["steps"][~hot] == 0)
     ch.set_cells(rho, T, rho * Y)
-    ch.set_launch(3, 64, 64, 4)
+    ch.set_launch(2, 256, 16)
     ch.integrate(1e-7)
     ref = ch.get_cells()
     np.testing.assert_array_equal(g["T"][hot], ref["T"][hot])
