@@ -32,6 +32,11 @@ STEP_BYTES = 560.0
 # flux kernel, per interface: conservative state (40 B) + 3 metric components and J (32 B) read, 5 fluxes
 # (40 B) written
 FLUX_BYTES = 112.0
+# flux kernel, per interface: FP64 operations (FMA counted twice) of WENO7-characteristic + AUSM+, as counted
+# by SQ_INSTS_VALU_{FMA,MUL,ADD}_F64 on the 256^3 workload (65.3 GFLOP per launch over 257 x 257 x 258
+# interfaces; profiles/flow_tgv256_fp64_valu.md)
+FLUX_FLOP = 3830.0
+FP64_PEAK_TFLOPS = 78.6
 
 
 def measured_traffic(kernel_symbol, n):
@@ -248,6 +253,10 @@ def main():
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(symbol, n), "kernel": dom,
                 "algorithmic_bytes": FLUX_BYTES * faces}
+    # the flux kernels are bound by FP64 arithmetic, not by HBM: their FP64 VALU rate against the peak
+    tflops = FLUX_FLOP * faces / (flux[dom] * 1e-3) / 1e12
+    fp64 = {"kernel": dom, "achieved": round(tflops, 2), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(tflops / FP64_PEAK_TFLOPS, 4), "flop_per_interface": FLUX_FLOP}
 
     S.close()
     reactive = None if args.no_reactive else reactive_bench()
@@ -268,6 +277,7 @@ def main():
         "step_hbm_frac": round(step_frac, 4),
         "kernel_ms": {k: round(v, 4) for k, v in ms.items()},
         "roofline": roofline,
+        "fp64_valu": fp64,
         "cpu_baseline": cpu,
         "reactive": reactive,
         "surrogate": surrogate,
