@@ -355,7 +355,8 @@ __device__ __forceinline__ void row_attempt(const ChemMech &m, const SpeciesLds 
   double yall[NS], Jrow[NS];
   row_gather<NS>(s.y, yall);
   double w = row_source<NS, true>(m, S, X, a, gl, s.T, s.y, yall, Jrow);
-  const double hs = fmin(s.h, dt - s.time);
+  const double rem = dt - s.time;
+  const double hs = s.h * kStretch >= rem ? rem : s.h;
 #pragma unroll
   for (int n = 0; n < NS; ++n) Jrow[n] = -hs * G * Jrow[n];
 #pragma unroll
@@ -389,6 +390,7 @@ __device__ __forceinline__ void row_attempt(const ChemMech &m, const SpeciesLds 
   double hstar = hs;
   if (s.steps > 0) hstar *= fmin(5.0, fmax(0.2, 0.8 * pow(s.err_last, 0.1) * pow(err + 1e-20, -0.175)));
   else if (!(err <= 1)) hstar *= 0.2;
+  if (err <= 1 && hs < s.h) hstar = fmax(hstar, s.h);
   s.h = hstar;
   ++s.attempts;
   if (err <= 1) {

@@ -104,7 +104,10 @@ __device__ __forceinline__ void rok4e_attempt(const ChemMech &m, Cell<NS> &s, do
   double w[NS], J[NS * NS], k1[NS], k2[NS], k3[NS], k4[NS], ys[NS], yk[NS];
   int piv[NS];
   source<NS, true>(m, s.T, s.y, w, J);
-  const double hs = fmin(s.h, dt - s.time);
+  // The step ends the interval when it would leave less than 5% of itself (no sliver step, whose size
+  // would otherwise become the proposal carried to the next sub-step).
+  const double rem = dt - s.time;
+  const double hs = s.h * kStretch >= rem ? rem : s.h;
 #pragma unroll
   for (int q = 0; q < NS * NS; ++q) J[q] = -hs * G * J[q];
 #pragma unroll
@@ -153,6 +156,8 @@ __device__ __forceinline__ void rok4e_attempt(const ChemMech &m, Cell<NS> &s, do
   double hstar = hs;
   if (s.steps > 0) hstar *= fmin(5.0, fmax(0.2, 0.8 * pow(s.err_last, 0.1) * pow(err + 1e-20, -0.175)));
   else if (!(err <= 1)) hstar *= 0.2; // a NaN first-step error also cuts the step (the reference would repeat it)
+  // a step cut short by the end of the interval does not shrink the proposal
+  if (err <= 1 && hs < s.h) hstar = fmax(hstar, s.h);
   s.h = hstar;
   ++s.attempts;
   if (err <= 1) {

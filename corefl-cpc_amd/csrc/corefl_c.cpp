@@ -758,6 +758,11 @@ int corefl_chemistry(corefl_ctx *ctx, int block, corefl_chem **out) {
     HIPCHK(hipMemsetAsync(ctx->chem->b.h, 0, sizeof(double) * b.dev.n, ctx->chem->stream));
     HIPCHK(hipStreamSynchronize(ctx->chem->stream));
     ctx->chem->have_dtc = false;
+    // the caller may retune the chemistry launch: the next graph replay captures the step afresh
+    if (ctx->graph_exec) {
+      (void)hipGraphExecDestroy(ctx->graph_exec);
+      ctx->graph_exec = nullptr;
+    }
     *out = ctx->chem;
     return 0;
   } catch (const std::exception &e) {

@@ -13,5 +13,8 @@ constexpr double G41 = 0.992356412977094 / G, G42 = -1.390032613873701 / G, G43 
 constexpr double B1 = 0.194335256262729, B2 = 0.483167813989227, B4 = 0.322496929748044;
 constexpr double E1 = -0.217819895945721 - B1, E2 = 1.03130847478467 - B2, E3 = 0.186511421161047, E4 = -B4;
 
+// A step whose size h satisfies kStretch h >= (time left in the interval) is taken to the interval's end.
+constexpr double kStretch = 1.05;
+
 } // namespace rok4e
 } // namespace corefl

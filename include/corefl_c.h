@@ -117,7 +117,7 @@ int corefl_get_real(const corefl_ctx *ctx, const char *name, double *out);
  * chemistry sub-step the batched ROK4E of corefl_chem_integrate; keys chemistry_rtol, chemistry_atol).
  * The borrowed chemistry context of a mixture setup, its batch filled with the interior cells of `block`
  * (the state before the next step), for corefl_chem_autotune / corefl_chem_set_launch, which the split
- * sub-steps then use.  Owned by ctx. */
+ * sub-steps then use (the next graph replay of corefl_advance captures the step afresh).  Owned by ctx. */
 int corefl_chemistry(corefl_ctx *ctx, int block, corefl_chem **out);
 /* Split chemistry sub-steps through corefl_chem_surrogate_step's path (network, projection, gate, exact
  * fallback) instead of the exact integrator; NULL restores the exact sub-steps.  s is not owned and must

@@ -248,7 +248,10 @@ int orc_chem_rok4e(const orc_mech *m, double rho, double E, double *rhoY, double
   (void)nr;
   while (time < t_end && step + rej < max_attempts) {
     orc_chem_source(m, *T, rhoY, q1, q2, wd, w);
-    const double hs = fmin(*h, t_end - time);
+    /* a step that would leave less than 5% of itself to t_end ends the interval (step stretching; the
+     * reference takes min(h, t_end - time), whose sliver last step then becomes the carried proposal) */
+    const double rem = t_end - time;
+    const double hs = *h * 1.05 >= rem ? rem : *h;
     orc_chem_jacobian(m, rhoY, q1, q2, jac);
     for (int a = 0; a < ns; ++a)
       for (int n = 0; n < ns; ++n) lhs[a * ns + n] = (a == n ? 1.0 : 0.0) - hs * G * jac[a * ns + n];
@@ -289,6 +292,7 @@ int orc_chem_rok4e(const orc_mech *m, double rho, double E, double *rhoY, double
       hstar *= fmin(5.0, fmax(0.2, 0.8 * pow(err_last, 0.1) * pow(err + 1e-20, -0.175)));
     else if (!(err <= 1))
       hstar *= 0.2;
+    if (err <= 1 && hs < *h) hstar = fmax(hstar, *h); /* a step cut short by t_end keeps the proposal */
     *h = hstar;
     if (err <= 1) {
       for (int l = 0; l < ns; ++l) {
