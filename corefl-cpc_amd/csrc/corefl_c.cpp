@@ -346,9 +346,16 @@ void build(corefl_ctx *ctx, std::vector<corefl::HostBlock> blocks) {
     if (ctx->chem_max_attempts < 1) throw std::runtime_error("chemistry_max_attempts must be at least 1");
     corefl::chem_ensure_device(ctx->chem);
   } else {
-    if (rec != 4 && rec != 5) throw std::runtime_error("reconstruction must be 4 (WENO5) or 5 (WENO7) on this path");
-    if (inv < 1 || inv > 4)
-      throw std::runtime_error("inviscid_scheme must be 1 (LF), 2 (Roe), 3 (AUSM+) or 4 (HLLC)");
+    const bool fs = inv == 51 || inv == 52 || inv == 71 || inv == 72;
+    if (!fs && rec != 4 && rec != 5)
+      throw std::runtime_error("reconstruction must be 4 (WENO5) or 5 (WENO7) on this path");
+    if (!fs && (inv < 1 || inv > 4))
+      throw std::runtime_error("inviscid_scheme must be 1 (LF), 2 (Roe), 3 (AUSM+), 4 (HLLC), or 51/52/71/72 (WENO5/WENO7 "
+                               "flux splitting, component-wise/characteristic)");
+    // Parameter.cpp:236-255: a positive shock-sensor threshold switches these schemes to the hybrid
+    // upwind/WENO scheme, which is not on this path
+    if (fs && P.get_real("shockSensor_threshold") > 1e-10)
+      throw std::runtime_error("shockSensor_threshold > 0 (hybrid UD/WENO scheme) is not on this path");
     if (vo != 0 && vo != 4 && vo != 8) throw std::runtime_error("viscous_order must be 0, 4 or 8 on this path");
     if (temporal != 3) throw std::runtime_error("temporal_scheme must be 3 (SSP-RK3)");
   }
@@ -372,7 +379,7 @@ void build(corefl_ctx *ctx, std::vector<corefl::HostBlock> blocks) {
   P.set_real("v_inf", v_inf);
   P.set_real("weno_eps", 1e-40 * rho_inf * v_inf * rho_inf * v_inf);
   ctx->scheme = {inv, P.get_int("reconstruction"), vo, P.get_real("weno_eps"), P.get_real("entropy_fix_factor"),
-                 P.get_real("prandtl_number"), P.get_real("cfl")};
+                 P.get_real("prandtl_number"), P.get_real("cfl"), v_inf};
   ctx->fixed_dt = P.get_bool("fixed_time_step");
   ctx->t_end = P.get_real("total_simulation_time");
 

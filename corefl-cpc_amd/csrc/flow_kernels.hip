@@ -432,6 +432,166 @@ __global__ __launch_bounds__(256) void k_inviscid(DevBlock b, Scheme sc, double 
   for (int l = 0; l < 5; ++l) F[id + l * sz] = fc[l];
 }
 
+// ---------------------------------------------------------------- LLF flux-splitting WENO (air)
+// inviscid_scheme 51/71: component-wise WENO5/WENO7 of the locally split fluxes (WENO.cu:890-990);
+// 52/72: characteristic WENO of the fluxes split with the largest spectral radius on the stencil
+// (WENO.cu:10-244 with the stencil of WENO.cu:1096-1136).  One thread per interface t+1/2 along DIR; the
+// stencil nodes are t-R+1 .. t+R (m = 0 .. 2R-1).  The shock sensor is off (its threshold defaults to 0),
+// so the nonlinear weights always apply.  F receives the interface flux at node t.
+template<int DIR, int R, bool CH>
+__global__ __launch_bounds__(256) void k_fsweno(DevBlock b, Scheme sc, double *F, TileGrid tg) {
+  Tile t;
+  if (!xcd_tile(tg, t)) return;
+  int i = t.x * 64 + threadIdx.x, j = t.y * 4 + threadIdx.y, k = t.z;
+  if constexpr (DIR == 0) i -= 1;
+  else if constexpr (DIR == 1) j -= 1;
+  else k -= 1;
+  if (i >= b.mx || j >= b.my) return;
+  const long st = DIR == 0 ? 1 : (DIR == 1 ? b.d2 : b.d1);
+  const long id = nid(b, i, j, k);
+  const long sz = b.sz;
+  constexpr int N = 2 * R;
+  constexpr double gm1 = kGamma - 1;
+
+  if constexpr (!CH) {
+    // split fluxes F+- = (J/2)(k.F +- lambda q), lambda = |Uk| + |k| c of the node itself; fp holds
+    // nodes 0..N-2 (the plus stencil), fm nodes 1..N-1 (the minus stencil)
+    double fp[N - 1][5], fm[N - 1][5];
+#pragma unroll
+    for (int m = 0; m < N; ++m) {
+      const long im = id + (m - R + 1) * st;
+      const double q0 = b.cv[im], q1 = b.cv[im + sz], q2 = b.cv[im + 2 * sz], q3 = b.cv[im + 3 * sz],
+                   q4 = b.cv[im + 4 * sz];
+      const double k0 = b.met[im + (3 * DIR) * sz], k1 = b.met[im + (3 * DIR + 1) * sz],
+                   k2 = b.met[im + (3 * DIR + 2) * sz];
+      const double ri = 1.0 / q0;
+      const double Uk = (q1 * k0 + q2 * k1 + q3 * k2) * ri;
+      const double p = gm1 * (q4 - 0.5 * (q1 * q1 + q2 * q2 + q3 * q3) * ri);
+      const double lam = fabs(Uk) + sqrt((k0 * k0 + k1 * k1 + k2 * k2) * (kGamma * p * ri));
+      const double hj = 0.5 * b.jac[im];
+      const double pk = p * hj;
+      if (m < N - 1) {
+        const double s = hj * (Uk + lam);
+        fp[m][0] = s * q0;
+        fp[m][1] = fma(s, q1, pk * k0);
+        fp[m][2] = fma(s, q2, pk * k1);
+        fp[m][3] = fma(s, q3, pk * k2);
+        fp[m][4] = fma(s, q4, pk * Uk);
+      }
+      if (m > 0) {
+        const double s = hj * (Uk - lam);
+        fm[m - 1][0] = s * q0;
+        fm[m - 1][1] = fma(s, q1, pk * k0);
+        fm[m - 1][2] = fma(s, q2, pk * k1);
+        fm[m - 1][3] = fma(s, q3, pk * k2);
+        fm[m - 1][4] = fma(s, q4, pk * Uk);
+      }
+    }
+    const double e0 = sc.weno_eps, e1 = e0 * sc.v_ref * sc.v_ref, e4 = e1 * sc.v_ref * sc.v_ref;
+#pragma unroll
+    for (int l = 0; l < 5; ++l) {
+      const double eps = l == 0 ? e0 : (l == 4 ? e4 : e1);
+      double f;
+      if constexpr (R == 4) {
+        const double a[7] = {fp[0][l], fp[1][l], fp[2][l], fp[3][l], fp[4][l], fp[5][l], fp[6][l]};
+        const double c[7] = {fm[6][l], fm[5][l], fm[4][l], fm[3][l], fm[2][l], fm[1][l], fm[0][l]};
+        f = weno7_half(a, eps) + weno7_half(c, eps);
+      } else {
+        f = weno5_half(fp[0][l], fp[1][l], fp[2][l], fp[3][l], fp[4][l], eps) +
+            weno5_half(fm[4][l], fm[3][l], fm[2][l], fm[1][l], fm[0][l], eps);
+      }
+      F[id + l * sz] = f;
+    }
+  } else {
+    // node states q, physical fluxes through the face fl = k.F(q), half Jacobians, largest radius
+    double q[N][5], fl[N][5], hj[N];
+    double lmax = 0, pa = 0, pb = 0;
+#pragma unroll
+    for (int m = 0; m < N; ++m) {
+      const long im = id + (m - R + 1) * st;
+#pragma unroll
+      for (int l = 0; l < 5; ++l) q[m][l] = b.cv[im + l * sz];
+      const double k0 = b.met[im + (3 * DIR) * sz], k1 = b.met[im + (3 * DIR + 1) * sz],
+                   k2 = b.met[im + (3 * DIR + 2) * sz];
+      const double ri = 1.0 / q[m][0];
+      const double Uk = (q[m][1] * k0 + q[m][2] * k1 + q[m][3] * k2) * ri;
+      const double p = gm1 * (q[m][4] - 0.5 * (q[m][1] * q[m][1] + q[m][2] * q[m][2] + q[m][3] * q[m][3]) * ri);
+      lmax = fmax(lmax, fabs(Uk) + sqrt((k0 * k0 + k1 * k1 + k2 * k2) * (kGamma * p * ri)));
+      fl[m][0] = q[m][0] * Uk;
+      fl[m][1] = fma(q[m][1], Uk, p * k0);
+      fl[m][2] = fma(q[m][2], Uk, p * k1);
+      fl[m][3] = fma(q[m][3], Uk, p * k2);
+      fl[m][4] = (q[m][4] + p) * Uk;
+      hj[m] = 0.5 * b.jac[im];
+      if (m == R - 1) pa = p;
+      if (m == R) pb = p;
+    }
+    // Roe state of the interface nodes; sound speed from the Roe-weighted p/rho (WENO.cu:17-30)
+    const double *qa = q[R - 1], *qb = q[R];
+    const double sab = sqrt(qa[0] * qb[0]);
+    const double rlc = 1 / (qa[0] + sab), rrc = 1 / (sab + qb[0]);
+    const double um = rlc * qa[1] + rrc * qb[1], vm = rlc * qa[2] + rrc * qb[2], wm = rlc * qa[3] + rrc * qb[3];
+    const double cm = sqrt(kGamma * (rlc * pa + rrc * pb));
+    double kx = 0.5 * (b.met[id + (3 * DIR) * sz] + b.met[id + st + (3 * DIR) * sz]);
+    double ky = 0.5 * (b.met[id + (3 * DIR + 1) * sz] + b.met[id + st + (3 * DIR + 1) * sz]);
+    double kz = 0.5 * (b.met[id + (3 * DIR + 2) * sz] + b.met[id + st + (3 * DIR + 2) * sz]);
+    {
+      const double r = 1.0 / sqrt(kx * kx + ky * ky + kz * kz);
+      kx *= r; ky *= r; kz *= r;
+    }
+    double nx = 0, ny = 0, nz = 0;
+    if (fabs(kx) > 0.8) {
+      nx = -kz; nz = kx;
+    } else {
+      ny = kz; nz = -ky;
+    }
+    {
+      const double r = 1.0 / sqrt(nx * nx + ny * ny + nz * nz);
+      nx *= r; ny *= r; nz *= r;
+    }
+    double qx = ky * nz - kz * ny, qy = kz * nx - kx * nz, qz = kx * ny - ky * nx;
+    {
+      const double r = 1.0 / sqrt(qx * qx + qy * qy + qz * qz);
+      qx *= r; qy *= r; qz *= r;
+    }
+    const double Ub = kx * um + ky * vm + kz * wm;
+    const double Un = nx * um + ny * vm + nz * wm, Uq = qx * um + qy * vm + qz * wm;
+    const double ic2 = 1.0 / (cm * cm), al = 0.5 * gm1 * (um * um + vm * vm + wm * wm);
+    const double Lm[5][5] = {
+        {(al + Ub * cm) * ic2 * 0.5, -(gm1 * um + kx * cm) * ic2 * 0.5, -(gm1 * vm + ky * cm) * ic2 * 0.5,
+         -(gm1 * wm + kz * cm) * ic2 * 0.5, gm1 * ic2 * 0.5},
+        {-Un, nx, ny, nz, 0.0},
+        {-Uq, qx, qy, qz, 0.0},
+        {1 - al * ic2, gm1 * um * ic2, gm1 * vm * ic2, gm1 * wm * ic2, -gm1 * ic2},
+        {(al - Ub * cm) * ic2 * 0.5, -(gm1 * um - kx * cm) * ic2 * 0.5, -(gm1 * vm - ky * cm) * ic2 * 0.5,
+         -(gm1 * wm - kz * cm) * ic2 * 0.5, gm1 * ic2 * 0.5}};
+    const double eps = sc.weno_eps;
+    double fch[5];
+#pragma unroll
+    for (int l = 0; l < 5; ++l) {
+      const double *L = Lm[l];
+      double vp[N - 1], vn[N - 1];
+#pragma unroll
+      for (int m = 0; m < N; ++m) {
+        const double LF = L[0] * fl[m][0] + L[1] * fl[m][1] + L[2] * fl[m][2] + L[3] * fl[m][3] + L[4] * fl[m][4];
+        const double LQ = L[0] * q[m][0] + L[1] * q[m][1] + L[2] * q[m][2] + L[3] * q[m][3] + L[4] * q[m][4];
+        if (m < N - 1) vp[m] = hj[m] * (LF + lmax * LQ);
+        if (m > 0) vn[N - 1 - m] = hj[m] * (LF - lmax * LQ); // stored mirrored
+      }
+      if constexpr (R == 4) fch[l] = weno7_half(vp, eps) + weno7_half(vn, eps);
+      else fch[l] = weno5_half(vp[0], vp[1], vp[2], vp[3], vp[4], eps) + weno5_half(vn[0], vn[1], vn[2], vn[3], vn[4], eps);
+    }
+    // back-projection with the Roe-averaged enthalpy (WENO.cu:229-243)
+    const double s1 = fch[0] + fch[3] + fch[4], s3 = fch[0] - fch[4];
+    const double H = rlc * (qa[4] + pa) + rrc * (qb[4] + pb);
+    F[id] = s1;
+    F[id + sz] = um * s1 - kx * cm * s3 + nx * fch[1] + qx * fch[2];
+    F[id + 2 * sz] = vm * s1 - ky * cm * s3 + ny * fch[1] + qy * fch[2];
+    F[id + 3 * sz] = wm * s1 - kz * cm * s3 + nz * fch[1] + qz * fch[2];
+    F[id + 4 * sz] = H * s1 - Ub * cm * s3 + Un * fch[1] + Uq * fch[2] - cm * cm / gm1 * fch[3];
+  }
+}
+
 // ---------------------------------------------------------------- viscous terms
 // Collocated first derivative along one index (ViscousScheme.cu:129-307 for ORDER 8; ORDER 4 uses the
 // reference's fourth-order rows: the i==2 row as the interior stencil and the i==0/1 rows as closures).
@@ -822,6 +982,19 @@ static void dispatch_rs(const DevBlock &b, const Scheme &sc, dim3 grid, double *
   }
 }
 
+template<int DIR>
+static void dispatch_fs(const DevBlock &b, const Scheme &sc, dim3 grid, double *F, hipStream_t s) {
+  const dim3 tpb(64, 4, 1);
+  const TileGrid tg{(int)grid.x, (int)grid.y, (int)grid.z, DIR};
+  const unsigned nb = tile_blocks(tg);
+  switch (sc.inviscid_scheme) {
+    case 51: k_fsweno<DIR, 3, false><<<nb, tpb, 0, s>>>(b, sc, F, tg); break;
+    case 52: k_fsweno<DIR, 3, true><<<nb, tpb, 0, s>>>(b, sc, F, tg); break;
+    case 71: k_fsweno<DIR, 4, false><<<nb, tpb, 0, s>>>(b, sc, F, tg); break;
+    default: k_fsweno<DIR, 4, true><<<nb, tpb, 0, s>>>(b, sc, F, tg); break;
+  }
+}
+
 hipError_t launch_inviscid_flux(const DevBlock &b, const Scheme &sc, int dir, hipStream_t s) {
   dim3 grid;
   double *F;
@@ -834,6 +1007,12 @@ hipError_t launch_inviscid_flux(const DevBlock &b, const Scheme &sc, int dir, hi
   } else {
     grid = dim3((b.mx + 63) / 64, (b.my + 3) / 4, b.mz + 1);
     F = b.fz;
+  }
+  if (sc.inviscid_scheme > 50) {
+    if (dir == 0) dispatch_fs<0>(b, sc, grid, F, s);
+    else if (dir == 1) dispatch_fs<1>(b, sc, grid, F, s);
+    else dispatch_fs<2>(b, sc, grid, F, s);
+    return hipGetLastError();
   }
   const bool w7 = sc.reconstruction == 5;
   if (dir == 0) w7 ? dispatch_rs<0, 4>(b, sc, grid, F, s) : dispatch_rs<0, 3>(b, sc, grid, F, s);

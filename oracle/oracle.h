@@ -26,13 +26,15 @@ extern "C" {
 
 /* Scheme selection and constants the path reads from DParameter (DParameter.cuh:13-207). */
 typedef struct {
-  int inviscid_scheme;   /* 1 LF, 2 Roe, 3 AUSM+, 4 HLLC (Parameter.cpp:218-260)                 */
+  int inviscid_scheme;   /* 1 LF, 2 Roe, 3 AUSM+, 4 HLLC; 51/52/71/72 LLF-split WENO5/7-cp/ch      */
+                         /* (Parameter.cpp:218-305)                                                */
   int reconstruction;    /* 4 WENO5 / 5 WENO7 characteristic reconstruction (new pairing)         */
   int viscous_order;     /* 0 inviscid, 4 (new), 8 collocated (ViscousScheme.cu:7-73)            */
   double prandtl;        /* prandtl_number, default 0.72                                          */
   double cfl;            /* cfl                                                                   */
   double entropy_fix_factor; /* default 0.125 (Parameter.cpp:380)                                 */
   double weno_eps;       /* eps_weno * weno_eps_scale = 1e-40 (rho_inf v_inf)^2 (WENO.cu:8,884)   */
+  double v_ref;          /* v_inf: scales the component-wise WENO epsilon (WENO.cu:946-949)       */
 } orc_params;
 
 /* ---- point functions (pure; parity pinned by tests/test_oracle_points.py) ---- */
@@ -60,6 +62,14 @@ void orc_riemann(int scheme, const double *pl, const double *pr, const double *m
  * interface lies between m = n/2-1 and n/2).  Returns 1 if the positivity fallback fired.      */
 int orc_interface_flux(const orc_params *p, const double (*q)[5], int n, const double *ml,
                        const double *mr, double jl, double jr, double efl, double efr, double *fc);
+
+/* Interface flux of the reference's LLF flux-splitting WENO for air: params.inviscid_scheme 51/71
+ * component-wise WENO5/WENO7 (WENO.cu:890-990), 52/72 characteristic (WENO.cu:10-244, 1090-1145).
+ * q[m][5], met[m][3] (metric row of the sweep direction) and jac[m] for the stencil nodes
+ * m = 0..n-1 (n = 6 WENO5, 8 WENO7); the interface lies between m = n/2-1 and n/2.  The shock
+ * sensor is off (shockSensor_threshold = 0, its default), so the nonlinear weights always apply. */
+void orc_fsweno_flux(const orc_params *p, const double (*q)[5], const double (*met)[3], const double *jac, int n,
+                     double *fc);
 
 /* Sutherland's law, Transport.cu:6-9. */
 double orc_sutherland(double T);

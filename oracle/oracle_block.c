@@ -523,7 +523,8 @@ static void entropy_fix(orc_block *b) {
 /* Inviscid flux of one direction and its difference into dq (the role of the WENO/pv kernels plus
  * compute_derivative_x/y/z, WENO.cu:1544-1571). */
 static void inviscid_dir(orc_block *b, int dir) {
-  const int r = b->p.reconstruction == 5 ? 4 : 3; /* stencil half width: n = 2r points */
+  const int fs = b->p.inviscid_scheme >= 51; /* LLF flux-splitting WENO (WENO.cu:830-990) */
+  const int r = fs ? (b->p.inviscid_scheme >= 71 ? 4 : 3) : (b->p.reconstruction == 5 ? 4 : 3); /* n = 2r points */
   const int n = 2 * r;
   const long s = b->sz;
   const long st = dir == 0 ? 1 : (dir == 1 ? b->d2 : b->d1);
@@ -544,6 +545,16 @@ static void inviscid_dir(orc_block *b, int dir) {
         double q[8][5];
         for (int mm = 0; mm < n; ++mm)
           for (int l = 0; l < 5; ++l) q[mm][l] = b->cv[id + (mm - r + 1) * st + l * s];
+        if (fs) {
+          double met[8][3], jac[8];
+          for (int mm = 0; mm < n; ++mm) {
+            const long im = id + (mm - r + 1) * st;
+            for (int c = 0; c < 3; ++c) met[mm][c] = b->met[im + (3 * dir + c) * s];
+            jac[mm] = b->jac[im];
+          }
+          orc_fsweno_flux(&b->p, (const double(*)[5])q, (const double(*)[3])met, jac, n, Fl + 5 * (t + 1));
+          continue;
+        }
         double ml[3], mr[3];
         for (int c = 0; c < 3; ++c) {
           ml[c] = b->met[id + (3 * dir + c) * s];

@@ -497,6 +497,147 @@ int orc_interface_flux(const orc_params *p, const double (*q)[5], int n, const d
   return fallback;
 }
 
+/* Pressure, temperature and the point quantities both LLF-split WENO schemes build their stencils
+ * from (WENO.cu:890-917 component-wise, WENO.cu:1096-1136 characteristic): Uk with the unscaled
+ * metric, cGradK = |grad k| sqrt(gamma R T) with T = p / (rho R) as the primitive update stores it. */
+typedef struct {
+  double p, Uk, cgk;
+} split_point;
+
+static split_point split_point_of(const double *q, const double *k) {
+  split_point s;
+  double pv[6];
+  cons_to_pv(q, pv);
+  const double R = kRu / kMwAir;
+  const double T = pv[4] / (pv[0] * R);
+  s.p = pv[4];
+  s.Uk = (q[1] * k[0] + q[2] * k[1] + q[3] * k[2]) / q[0];
+  s.cgk = sqrt(k[0] * k[0] + k[1] * k[1] + k[2] * k[2]) * sqrt(kGamma * R * T);
+  return s;
+}
+
+/* Component-wise LLF flux splitting (schemes 51/71): WENO.cu:890-990 for the air model. Each node splits
+ * its own flux with its own spectral radius |Uk| + cGradK; each conservative variable is then
+ * reconstructed by WENO5/WENO7 with the epsilon scaled by v_ref^0, v_ref^2, v_ref^4. */
+static void fsweno_componentwise(const orc_params *p, const double (*q)[5], const double (*met)[3],
+                                 const double *jac, int n, double *fc) {
+  double fp[8][5], fm[8][5];
+  for (int m = 0; m < n; ++m) {
+    const split_point s = split_point_of(q[m], met[m]);
+    const double lam = fabs(s.Uk) + s.cgk;
+    const double sp = 0.5 * jac[m] * (s.Uk + lam), sm = 0.5 * jac[m] * (s.Uk - lam);
+    const double pk = s.p * (0.5 * jac[m]);
+    fp[m][0] = sp * q[m][0];
+    fm[m][0] = sm * q[m][0];
+    for (int c = 0; c < 3; ++c) {
+      fp[m][1 + c] = fma(sp, q[m][1 + c], pk * met[m][c]);
+      fm[m][1 + c] = fma(sm, q[m][1 + c], pk * met[m][c]);
+    }
+    fp[m][4] = fma(sp, q[m][4], pk * s.Uk);
+    fm[m][4] = fma(sm, q[m][4], pk * s.Uk);
+  }
+  const double e1 = p->weno_eps * p->v_ref * p->v_ref;
+  const double eps[5] = {p->weno_eps, e1, e1, e1, e1 * p->v_ref * p->v_ref};
+  for (int l = 0; l < 5; ++l) {
+    double vp[7], vm[7];
+    for (int m = 0; m < n - 1; ++m) {
+      vp[m] = fp[m][l];
+      vm[m] = fm[m + 1][l];
+    }
+    fc[l] = n == 8 ? orc_weno7(vp, vm, eps[l], 1) : orc_weno5(vp, vm, eps[l], 1);
+  }
+}
+
+/* Characteristic LLF flux splitting for air (schemes 52/72): WENO.cu:1090-1145 builds the stencil,
+ * weno_ch_air_x (WENO.cu:10-244) projects with the left eigenvectors of the Roe state, splits with the
+ * largest spectral radius on the stencil, reconstructs each field and projects back. */
+static void fsweno_characteristic(const orc_params *p, const double (*q)[5], const double (*met)[3],
+                                  const double *jac, int n, double *fc) {
+  const int c = n / 2 - 1;
+  split_point s[8];
+  double F[8][5]; /* F[m][0..3] = Uk, rho u Uk + p kx, rho v Uk + p ky, rho w Uk + p kz; F[m][4] = p */
+  double max_lambda = 0;
+  for (int m = 0; m < n; ++m) {
+    s[m] = split_point_of(q[m], met[m]);
+    F[m][0] = s[m].Uk;
+    for (int d = 0; d < 3; ++d) F[m][1 + d] = fma(q[m][1 + d], s[m].Uk, s[m].p * met[m][d]);
+    F[m][4] = s[m].p;
+    const double lam = fabs(s[m].Uk) + s[m].cgk;
+    if (lam > max_lambda) max_lambda = lam;
+  }
+  /* Roe state of the two interface nodes; the sound speed from the Roe-weighted p/rho (WENO.cu:21-30) */
+  const double R = kRu / kMwAir, gm1 = kGamma - 1;
+  const double t1 = sqrt(q[c][0] * q[c + 1][0]);
+  const double rlc = 1 / (q[c][0] + t1), rrc = 1 / (t1 + q[c + 1][0]);
+  const double um = rlc * q[c][1] + rrc * q[c + 1][1];
+  const double vm = rlc * q[c][2] + rrc * q[c + 1][2];
+  const double wm = rlc * q[c][3] + rrc * q[c + 1][3];
+  const double Tm = (rlc * F[c][4] + rrc * F[c + 1][4]) / R;
+  const double cm = sqrt(kGamma * R * Tm);
+  /* frame (WENO.cu:33-64) */
+  double kx = 0.5 * (met[c][0] + met[c + 1][0]), ky = 0.5 * (met[c][1] + met[c + 1][1]);
+  double kz = 0.5 * (met[c][2] + met[c + 1][2]);
+  double r = 1 / sqrt(kx * kx + ky * ky + kz * kz);
+  kx *= r;
+  ky *= r;
+  kz *= r;
+  const double Uk_bar = kx * um + ky * vm + kz * wm;
+  double nx = 0, ny = 0, nz = 0;
+  if (fabs(kx) > 0.8) {
+    nx = -kz;
+    nz = kx;
+  } else {
+    ny = kz;
+    nz = -ky;
+  }
+  r = 1 / sqrt(nx * nx + ny * ny + nz * nz);
+  nx *= r;
+  ny *= r;
+  nz *= r;
+  double qx = ky * nz - kz * ny, qy = kz * nx - kx * nz, qz = kx * ny - ky * nx;
+  r = 1 / sqrt(qx * qx + qy * qy + qz * qz);
+  qx *= r;
+  qy *= r;
+  qz *= r;
+  const double Un = nx * um + ny * vm + nz * wm, Uq = qx * um + qy * vm + qz * wm;
+  const double ic2 = 1.0 / (cm * cm), al = 0.5 * gm1 * (um * um + vm * vm + wm * wm);
+  const double Lm[5][5] = {
+      {(al + Uk_bar * cm) * ic2 * 0.5, -(gm1 * um + kx * cm) * ic2 * 0.5, -(gm1 * vm + ky * cm) * ic2 * 0.5,
+       -(gm1 * wm + kz * cm) * ic2 * 0.5, gm1 * ic2 * 0.5},
+      {-Un, nx, ny, nz, 0},
+      {-Uq, qx, qy, qz, 0},
+      {1 - al * ic2, gm1 * um * ic2, gm1 * vm * ic2, gm1 * wm * ic2, -gm1 * ic2},
+      {(al - Uk_bar * cm) * ic2 * 0.5, -(gm1 * um - kx * cm) * ic2 * 0.5, -(gm1 * vm - ky * cm) * ic2 * 0.5,
+       -(gm1 * wm - kz * cm) * ic2 * 0.5, gm1 * ic2 * 0.5}};
+  double fch[5];
+  for (int l = 0; l < 5; ++l) {
+    const double *L = Lm[l];
+    double vp[7], vmn[7];
+    for (int m = 0; m < n; ++m) {
+      const double LF = L[0] * (q[m][0] * F[m][0]) + L[1] * F[m][1] + L[2] * F[m][2] + L[3] * F[m][3] +
+                        L[4] * ((q[m][4] + F[m][4]) * F[m][0]);
+      const double LQ = L[0] * q[m][0] + L[1] * q[m][1] + L[2] * q[m][2] + L[3] * q[m][3] + L[4] * q[m][4];
+      if (m < n - 1) vp[m] = 0.5 * jac[m] * (LF + max_lambda * LQ);
+      if (m > 0) vmn[m - 1] = 0.5 * jac[m] * (LF - max_lambda * LQ);
+    }
+    fch[l] = n == 8 ? orc_weno7(vp, vmn, p->weno_eps, 1) : orc_weno5(vp, vmn, p->weno_eps, 1);
+  }
+  /* back-projection (WENO.cu:229-243) with the Roe-averaged enthalpy */
+  const double s1 = fch[0] + fch[3] + fch[4], s3 = fch[0] - fch[4];
+  fc[0] = s1;
+  fc[1] = um * s1 - kx * cm * s3 + nx * fch[1] + qx * fch[2];
+  fc[2] = vm * s1 - ky * cm * s3 + ny * fch[1] + qy * fch[2];
+  fc[3] = wm * s1 - kz * cm * s3 + nz * fch[1] + qz * fch[2];
+  const double H = rlc * (q[c][4] + F[c][4]) + rrc * (q[c + 1][4] + F[c + 1][4]);
+  fc[4] = H * s1 - Uk_bar * cm * s3 + Un * fch[1] + Uq * fch[2] - cm * cm / gm1 * fch[3];
+}
+
+void orc_fsweno_flux(const orc_params *p, const double (*q)[5], const double (*met)[3], const double *jac, int n,
+                     double *fc) {
+  if (p->inviscid_scheme % 10 == 1) fsweno_componentwise(p, q, met, jac, n, fc);
+  else fsweno_characteristic(p, q, met, jac, n, fc);
+}
+
 /* Transport.cu:6-9 */
 double orc_sutherland(double T) {
   return 1.7894e-5 * pow(T / 288.16, 1.5) * (288.16 + 110) / (T + 110);

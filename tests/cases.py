@@ -89,7 +89,7 @@ struct init_cond_0 {{
     faces = box_faces(n, my, 1, 6)
     return dict(setup=setup, grid=(X, Y, Z), scale=1.0, faces=faces, boundary=boundary_text(faces),
                 scheme=(inviscid, recon, 0), prandtl=0.72, cfl=cfl, t_end=t_end,
-                eps=1e-40 * (1.0 * math.sqrt(GAMMA)) ** 2)
+                eps=1e-40 * (1.0 * math.sqrt(GAMMA)) ** 2, v_ref=math.sqrt(GAMMA))
 
 
 def tgv_length(reynolds, mach=0.1, T0=300.0, p0=101325.0):
@@ -144,7 +144,7 @@ struct taylorGreen {{
     faces = box_faces(n, n, n, 8)
     return dict(setup=setup, grid=(X, Y, Z), scale=L, faces=faces,
                 boundary=boundary_text(faces), scheme=(inviscid, recon, visc), prandtl=0.72, cfl=cfl,
-                L=L, rho0=rho0, V0=V0, eps=1e-40 * (rho0 * V0) ** 2)
+                L=L, rho0=rho0, V0=V0, eps=1e-40 * (rho0 * V0) ** 2, v_ref=V0)
 
 
 def physical_grid(case):

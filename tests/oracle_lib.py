@@ -15,7 +15,8 @@ LIB = os.path.join(ROOT, "oracle", "_build", "liboracle.so")
 class OrcParams(ctypes.Structure):
     _fields_ = [("inviscid_scheme", ctypes.c_int), ("reconstruction", ctypes.c_int),
                 ("viscous_order", ctypes.c_int), ("prandtl", ctypes.c_double), ("cfl", ctypes.c_double),
-                ("entropy_fix_factor", ctypes.c_double), ("weno_eps", ctypes.c_double)]
+                ("entropy_fix_factor", ctypes.c_double), ("weno_eps", ctypes.c_double),
+                ("v_ref", ctypes.c_double)]
 
 
 _lib = None
@@ -37,6 +38,7 @@ def load():
     lib.orc_riemann.argtypes = [I, P, P, P, P, D, D, D, D, I, P]
     lib.orc_interface_flux.restype = I
     lib.orc_interface_flux.argtypes = [ctypes.POINTER(OrcParams), P, I, P, P, D, D, D, D, P]
+    lib.orc_fsweno_flux.argtypes = [ctypes.POINTER(OrcParams), P, P, P, I, P]
     lib.orc_sutherland.restype = D
     lib.orc_sutherland.argtypes = [D]
     lib.orc_metrics.restype = I
@@ -86,8 +88,8 @@ def riemann(scheme, pl, pr, ml, mr, jl=1.0, jr=1.0, efl=0.0, efr=0.0):
     return fc
 
 
-def params(inviscid=3, recon=5, visc=0, prandtl=0.72, cfl=0.5, efix=0.125, eps=1e-40):
-    return OrcParams(inviscid, recon, visc, prandtl, cfl, efix, eps)
+def params(inviscid=3, recon=5, visc=0, prandtl=0.72, cfl=0.5, efix=0.125, eps=1e-40, v_ref=1.0):
+    return OrcParams(inviscid, recon, visc, prandtl, cfl, efix, eps, v_ref)
 
 
 def interface_flux(p, q, ml, mr, jl=1.0, jr=1.0, efl=0.0, efr=0.0):
@@ -96,6 +98,14 @@ def interface_flux(p, q, ml, mr, jl=1.0, jr=1.0, efl=0.0, efr=0.0):
     fc = np.zeros(5)
     fb = load().orc_interface_flux(ctypes.byref(p), _p(q), q.shape[0], _p(ml), _p(mr), jl, jr, efl, efr, _p(fc))
     return fc, fb
+
+
+def fsweno_flux(p, q, met, jac):
+    """LLF flux-splitting WENO flux of one interface; q (n,5), met (n,3), jac (n,)."""
+    q, met, jac = (np.ascontiguousarray(a, float) for a in (q, met, jac))
+    fc = np.zeros(5)
+    load().orc_fsweno_flux(ctypes.byref(p), _p(q), _p(met), _p(jac), q.shape[0], _p(fc))
+    return fc
 
 
 def sutherland(T):

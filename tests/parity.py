@@ -17,7 +17,7 @@ def make_pair(case):
     X, Y, Z = cases.physical_grid(case)
     inv, recon, visc = case["scheme"]
     p = oracle_lib.params(inviscid=inv, recon=recon, visc=visc, prandtl=case["prandtl"], cfl=case["cfl"],
-                          eps=S.get_real("weno_eps"))
+                          eps=S.get_real("weno_eps"), v_ref=S.get_real("v_inf"))
     B = oracle_lib.Block(p, X, Y, Z, ngg, case["faces"])
     B.set_bv(S.get(corefl.BV))
     return S, B

@@ -3,6 +3,8 @@ import ctypes
 import os
 import re
 
+import pytest
+
 import corefl
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -31,3 +33,16 @@ def test_bad_input_reports_error():
     rc = lib.corefl_create(b"/nonexistent/input", ctypes.byref(ctx))
     assert rc != 0
     assert b"setup.txt" in lib.corefl_last_error()
+
+
+def test_flux_splitting_weno_scheme_validation():
+    """Schemes 51/52/71/72 are accepted; a positive shock-sensor threshold (the reference's hybrid
+    upwind/WENO switch, Parameter.cpp:236-255) and unknown scheme numbers fail before any device call."""
+    import cases
+    c = cases.sod(16, recon=4, inviscid=72)
+    bad = [(c["setup"].replace("int inviscid_scheme = 72", "int inviscid_scheme = 72\nreal shockSensor_threshold = 0.1"),
+            "shockSensor_threshold"),
+           (c["setup"].replace("int inviscid_scheme = 72", "int inviscid_scheme = 73"), "51/52/71/72")]
+    for setup, msg in bad:
+        with pytest.raises(corefl.CoreflError, match=msg):
+            corefl.Solver(setup_text=setup, blocks=[c["grid"]], boundary_text=c["boundary"])

@@ -13,8 +13,11 @@ from parity import interior, make_pair, make_solver, rel_err
 
 pytestmark = pytest.mark.gpu
 
-SOD = [(4, 4), (5, 3), (4, 2), (5, 1), (5, 4), (4, 3)]           # (reconstruction, inviscid_scheme)
-TGV = [(5, 3, 4), (4, 2, 8), (5, 4, 8), (4, 1, 4)]               # (reconstruction, inviscid, viscous_order)
+# (reconstruction, inviscid_scheme); 51/52/71/72 are the LLF flux-splitting WENO schemes, which ignore
+# the reconstruction setting
+SOD = [(4, 4), (5, 3), (4, 2), (5, 1), (5, 4), (4, 3), (4, 51), (4, 52), (4, 71), (4, 72)]
+# (reconstruction, inviscid, viscous_order)
+TGV = [(5, 3, 4), (4, 2, 8), (5, 4, 8), (4, 1, 4), (4, 51, 4), (4, 52, 8), (4, 71, 0), (4, 72, 8)]
 
 
 def _sod(recon, inv):
@@ -108,10 +111,12 @@ def test_graph_replay_matches_eager():
     assert S1.time() == S2.time() > 0
 
 
-def test_sod_to_final_time():
-    """Sod (400 nodes, WENO5 + HLLC, CFL 0.5) to t = 0.2 against the exact solution and the oracle."""
+@pytest.mark.parametrize("inv", [4, 51, 72])
+def test_sod_to_final_time(inv):
+    """Sod (400 nodes, CFL 0.5; WENO5 + HLLC, or the LLF flux-splitting WENO5 component-wise / WENO7
+    characteristic) to t = 0.2 against the exact solution and the oracle."""
     import oracle_lib
-    case = cases.sod(400, recon=4, inviscid=4)
+    case = cases.sod(400, recon=4, inviscid=inv)
     S, B = make_pair(case)
     g, shape = S.get_int("ngg"), case["grid"][0].shape
     t, n = 0.0, 0
@@ -136,4 +141,8 @@ def test_sod_to_final_time():
     bo = interior(B.get(1), 6, shape, g)
     d = np.mean(np.abs(bv[0] - bo[0]))
     print("mean |rho_gpu - rho_oracle|", d)
-    assert d < 1e-9
+    # Rounding differences grow over the 351 steps where the WENO weights act on smoothness indicators
+    # that are rounding noise against eps = 1e-40 (rho_inf v_inf)^2 (in the constant states) and at the
+    # shock; the flux-splitting schemes, whose dissipation follows the local |Uk| + c, amplify them more.
+    # Both bounds sit five or more orders below the scheme's own error against the exact solution.
+    assert d < (1e-9 if inv < 50 else 1e-7)
