@@ -37,6 +37,9 @@ FLUX_BYTES = 112.0
 # interfaces; profiles/flow_tgv256_fp64_valu.md)
 FLUX_FLOP = 3830.0
 FP64_PEAK_TFLOPS = 78.6
+SCHEME_NAMES = {1: "WENO7-ch + LF", 2: "WENO7-ch + Roe", 3: "WENO7-ch + AUSM+", 4: "WENO7-ch + HLLC",
+                51: "WENO5-cp LLF flux splitting", 52: "WENO5-ch LLF flux splitting",
+                71: "WENO7-cp LLF flux splitting", 72: "WENO7-ch LLF flux splitting"}
 
 
 def measured_traffic(kernel_symbol, n):
@@ -192,6 +195,9 @@ def main():
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=257, help="nodes per direction (n-1 unique)")
+    ap.add_argument("--inviscid", type=int, default=3,
+                    help="inviscid_scheme of the Taylor-Green workload: 3 (AUSM+ after characteristic WENO7, the "
+                         "default) or 51/52/71/72 (LLF flux-splitting WENO5/WENO7, component-wise/characteristic)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-reps", type=int, default=3)
     ap.add_argument("--no-reactive", action="store_true")
@@ -227,7 +233,8 @@ def main():
         return
     n = args.n
     log(f"bench: building TGV wavy grid {n}^3")
-    case = cases.tgv(n, reynolds=1600.0, recon=5, inviscid=3, visc=4, cfl=0.5)
+    inv = args.inviscid
+    case = cases.tgv(n, reynolds=1600.0, recon=5, inviscid=inv, visc=4, cfl=0.5)
     t0 = time.perf_counter()
     S = corefl.Solver(setup_text=case["setup"], blocks=[case["grid"]], boundary_text=case["boundary"])
     del case["grid"]
@@ -253,14 +260,18 @@ def main():
     dom = max(flux, key=flux.get)
     faces = {"flux_x": (n + 1) * n * n, "flux_y": n * (n + 1) * n, "flux_z": n * n * (n + 1)}[dom]
     achieved = FLUX_BYTES * faces / (flux[dom] * 1e-3) / 1e9
-    symbol = {"flux_x": "k_inviscid<0, 4, 3>", "flux_y": "k_inviscid<1, 4, 3>", "flux_z": "k_inviscid<2, 4, 3>"}[dom]
+    d = "xyz".index(dom[-1])
+    symbol = (f"k_inviscid<{d}, 4, {inv}>" if inv < 50 else
+              f"k_fsweno<{d}, {4 if inv > 70 else 3}, {'true' if inv % 10 == 2 else 'false'}>")
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(symbol, n), "kernel": dom,
                 "algorithmic_bytes": FLUX_BYTES * faces}
     # the flux kernels are bound by FP64 arithmetic, not by HBM: their FP64 VALU rate against the peak
-    tflops = FLUX_FLOP * faces / (flux[dom] * 1e-3) / 1e12
-    fp64 = {"kernel": dom, "achieved": round(tflops, 2), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "frac": round(tflops / FP64_PEAK_TFLOPS, 4), "flop_per_interface": FLUX_FLOP}
+    fp64 = None
+    if inv == 3:
+        tflops = FLUX_FLOP * faces / (flux[dom] * 1e-3) / 1e12
+        fp64 = {"kernel": dom, "achieved": round(tflops, 2), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(tflops / FP64_PEAK_TFLOPS, 4), "flop_per_interface": FLUX_FLOP}
 
     S.close()
     reactive = None if args.no_reactive else reactive_bench()
@@ -275,7 +286,7 @@ def main():
         "metric": METRIC, "value": round(value, 2), "unit": "Mcell-steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(el / args.steps * 1e3, 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"taylor-green wavy-grid {n - 1}^3 unique cells ({n}^3 nodes), WENO7-ch + AUSM+, "
+        "config": {"workload": f"taylor-green wavy-grid {n - 1}^3 unique cells ({n}^3 nodes), {SCHEME_NAMES[inv]}, "
                                "4th-order viscous, SSP-RK3, CFL 0.5 device step, hipGraph replay",
                    "cells": unique, "reynolds": 1600, "mach": 0.1},
         "step_hbm_frac": round(step_frac, 4),

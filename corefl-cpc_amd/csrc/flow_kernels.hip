@@ -439,7 +439,7 @@ __global__ __launch_bounds__(256) void k_inviscid(DevBlock b, Scheme sc, double 
 // stencil nodes are t-R+1 .. t+R (m = 0 .. 2R-1).  The shock sensor is off (its threshold defaults to 0),
 // so the nonlinear weights always apply.  F receives the interface flux at node t.
 template<int DIR, int R, bool CH>
-__global__ __launch_bounds__(256) void k_fsweno(DevBlock b, Scheme sc, double *F, TileGrid tg) {
+__global__ __launch_bounds__(256, 2) void k_fsweno(DevBlock b, Scheme sc, double *F, TileGrid tg) {
   Tile t;
   if (!xcd_tile(tg, t)) return;
   int i = t.x * 64 + threadIdx.x, j = t.y * 4 + threadIdx.y, k = t.z;
@@ -503,38 +503,36 @@ __global__ __launch_bounds__(256) void k_fsweno(DevBlock b, Scheme sc, double *F
       F[id + l * sz] = f;
     }
   } else {
-    // node states q, physical fluxes through the face fl = k.F(q), half Jacobians, largest radius
-    double q[N][5], fl[N][5], hj[N];
+    // Two passes over the stencil keep the register footprint within two waves per SIMD: the first finds
+    // the largest spectral radius, the second projects each node's flux and state and splits them.
     double lmax = 0, pa = 0, pb = 0;
 #pragma unroll
     for (int m = 0; m < N; ++m) {
       const long im = id + (m - R + 1) * st;
-#pragma unroll
-      for (int l = 0; l < 5; ++l) q[m][l] = b.cv[im + l * sz];
+      const double q0 = b.cv[im], q1 = b.cv[im + sz], q2 = b.cv[im + 2 * sz], q3 = b.cv[im + 3 * sz],
+                   q4 = b.cv[im + 4 * sz];
       const double k0 = b.met[im + (3 * DIR) * sz], k1 = b.met[im + (3 * DIR + 1) * sz],
                    k2 = b.met[im + (3 * DIR + 2) * sz];
-      const double ri = 1.0 / q[m][0];
-      const double Uk = (q[m][1] * k0 + q[m][2] * k1 + q[m][3] * k2) * ri;
-      const double p = gm1 * (q[m][4] - 0.5 * (q[m][1] * q[m][1] + q[m][2] * q[m][2] + q[m][3] * q[m][3]) * ri);
+      const double ri = 1.0 / q0;
+      const double Uk = (q1 * k0 + q2 * k1 + q3 * k2) * ri;
+      const double p = gm1 * (q4 - 0.5 * (q1 * q1 + q2 * q2 + q3 * q3) * ri);
       lmax = fmax(lmax, fabs(Uk) + sqrt((k0 * k0 + k1 * k1 + k2 * k2) * (kGamma * p * ri)));
-      fl[m][0] = q[m][0] * Uk;
-      fl[m][1] = fma(q[m][1], Uk, p * k0);
-      fl[m][2] = fma(q[m][2], Uk, p * k1);
-      fl[m][3] = fma(q[m][3], Uk, p * k2);
-      fl[m][4] = (q[m][4] + p) * Uk;
-      hj[m] = 0.5 * b.jac[im];
       if (m == R - 1) pa = p;
       if (m == R) pb = p;
     }
     // Roe state of the interface nodes; sound speed from the Roe-weighted p/rho (WENO.cu:17-30)
-    const double *qa = q[R - 1], *qb = q[R];
-    const double sab = sqrt(qa[0] * qb[0]);
-    const double rlc = 1 / (qa[0] + sab), rrc = 1 / (sab + qb[0]);
-    const double um = rlc * qa[1] + rrc * qb[1], vm = rlc * qa[2] + rrc * qb[2], wm = rlc * qa[3] + rrc * qb[3];
+    const long ia = id, ib = id + st;
+    const double ra = b.cv[ia], rb = b.cv[ib];
+    const double sab = sqrt(ra * rb);
+    const double rlc = 1 / (ra + sab), rrc = 1 / (sab + rb);
+    const double um = rlc * b.cv[ia + sz] + rrc * b.cv[ib + sz];
+    const double vm = rlc * b.cv[ia + 2 * sz] + rrc * b.cv[ib + 2 * sz];
+    const double wm = rlc * b.cv[ia + 3 * sz] + rrc * b.cv[ib + 3 * sz];
+    const double H = rlc * (b.cv[ia + 4 * sz] + pa) + rrc * (b.cv[ib + 4 * sz] + pb);
     const double cm = sqrt(kGamma * (rlc * pa + rrc * pb));
-    double kx = 0.5 * (b.met[id + (3 * DIR) * sz] + b.met[id + st + (3 * DIR) * sz]);
-    double ky = 0.5 * (b.met[id + (3 * DIR + 1) * sz] + b.met[id + st + (3 * DIR + 1) * sz]);
-    double kz = 0.5 * (b.met[id + (3 * DIR + 2) * sz] + b.met[id + st + (3 * DIR + 2) * sz]);
+    double kx = 0.5 * (b.met[ia + (3 * DIR) * sz] + b.met[ib + (3 * DIR) * sz]);
+    double ky = 0.5 * (b.met[ia + (3 * DIR + 1) * sz] + b.met[ib + (3 * DIR + 1) * sz]);
+    double kz = 0.5 * (b.met[ia + (3 * DIR + 2) * sz] + b.met[ib + (3 * DIR + 2) * sz]);
     {
       const double r = 1.0 / sqrt(kx * kx + ky * ky + kz * kz);
       kx *= r; ky *= r; kz *= r;
@@ -556,34 +554,50 @@ __global__ __launch_bounds__(256) void k_fsweno(DevBlock b, Scheme sc, double *F
     }
     const double Ub = kx * um + ky * vm + kz * wm;
     const double Un = nx * um + ny * vm + nz * wm, Uq = qx * um + qy * vm + qz * wm;
-    const double ic2 = 1.0 / (cm * cm), al = 0.5 * gm1 * (um * um + vm * vm + wm * wm);
-    const double Lm[5][5] = {
-        {(al + Ub * cm) * ic2 * 0.5, -(gm1 * um + kx * cm) * ic2 * 0.5, -(gm1 * vm + ky * cm) * ic2 * 0.5,
-         -(gm1 * wm + kz * cm) * ic2 * 0.5, gm1 * ic2 * 0.5},
-        {-Un, nx, ny, nz, 0.0},
-        {-Uq, qx, qy, qz, 0.0},
-        {1 - al * ic2, gm1 * um * ic2, gm1 * vm * ic2, gm1 * wm * ic2, -gm1 * ic2},
-        {(al - Ub * cm) * ic2 * 0.5, -(gm1 * um - kx * cm) * ic2 * 0.5, -(gm1 * vm - ky * cm) * ic2 * 0.5,
-         -(gm1 * wm - kz * cm) * ic2 * 0.5, gm1 * ic2 * 0.5}};
+    const double hic2 = 0.5 / (cm * cm), al = 0.5 * gm1 * (um * um + vm * vm + wm * wm);
+    // Left eigenvectors of WENO.cu:73-124 applied to x: rows 0 and 4 are A +- B, row 3 is x0 - 2A, with
+    // A = (al x0 - gm1 (u . x1:3) + gm1 x4) / (2 c^2) and B = c (Ub x0 - k . x1:3) / (2 c^2).
+    auto project = [&](const double *x, double *w) {
+      const double A = hic2 * (al * x[0] - gm1 * (um * x[1] + vm * x[2] + wm * x[3]) + gm1 * x[4]);
+      const double B = hic2 * cm * (Ub * x[0] - (kx * x[1] + ky * x[2] + kz * x[3]));
+      w[0] = A + B;
+      w[1] = nx * x[1] + ny * x[2] + nz * x[3] - Un * x[0];
+      w[2] = qx * x[1] + qy * x[2] + qz * x[3] - Uq * x[0];
+      w[3] = x[0] - 2 * A;
+      w[4] = A - B;
+    };
+    double vp[5][N - 1], vn[5][N - 1]; // plus stencil nodes 0..N-2; minus stencil nodes N-1..1 (mirrored)
+#pragma unroll
+    for (int m = 0; m < N; ++m) {
+      const long im = id + (m - R + 1) * st;
+      const double q[5] = {b.cv[im], b.cv[im + sz], b.cv[im + 2 * sz], b.cv[im + 3 * sz], b.cv[im + 4 * sz]};
+      const double k0 = b.met[im + (3 * DIR) * sz], k1 = b.met[im + (3 * DIR + 1) * sz],
+                   k2 = b.met[im + (3 * DIR + 2) * sz];
+      const double ri = 1.0 / q[0];
+      const double Uk = (q[1] * k0 + q[2] * k1 + q[3] * k2) * ri;
+      const double p = gm1 * (q[4] - 0.5 * (q[1] * q[1] + q[2] * q[2] + q[3] * q[3]) * ri);
+      const double fl[5] = {q[0] * Uk, fma(q[1], Uk, p * k0), fma(q[2], Uk, p * k1), fma(q[3], Uk, p * k2),
+                            (q[4] + p) * Uk};
+      const double hj = 0.5 * b.jac[im];
+      double LF[5], LQ[5];
+      project(fl, LF);
+      project(q, LQ);
+#pragma unroll
+      for (int l = 0; l < 5; ++l) {
+        if (m < N - 1) vp[l][m] = hj * (LF[l] + lmax * LQ[l]);
+        if (m > 0) vn[l][N - 1 - m] = hj * (LF[l] - lmax * LQ[l]);
+      }
+    }
     const double eps = sc.weno_eps;
     double fch[5];
 #pragma unroll
     for (int l = 0; l < 5; ++l) {
-      const double *L = Lm[l];
-      double vp[N - 1], vn[N - 1];
-#pragma unroll
-      for (int m = 0; m < N; ++m) {
-        const double LF = L[0] * fl[m][0] + L[1] * fl[m][1] + L[2] * fl[m][2] + L[3] * fl[m][3] + L[4] * fl[m][4];
-        const double LQ = L[0] * q[m][0] + L[1] * q[m][1] + L[2] * q[m][2] + L[3] * q[m][3] + L[4] * q[m][4];
-        if (m < N - 1) vp[m] = hj[m] * (LF + lmax * LQ);
-        if (m > 0) vn[N - 1 - m] = hj[m] * (LF - lmax * LQ); // stored mirrored
-      }
-      if constexpr (R == 4) fch[l] = weno7_half(vp, eps) + weno7_half(vn, eps);
-      else fch[l] = weno5_half(vp[0], vp[1], vp[2], vp[3], vp[4], eps) + weno5_half(vn[0], vn[1], vn[2], vn[3], vn[4], eps);
+      if constexpr (R == 4) fch[l] = weno7_half(vp[l], eps) + weno7_half(vn[l], eps);
+      else fch[l] = weno5_half(vp[l][0], vp[l][1], vp[l][2], vp[l][3], vp[l][4], eps) +
+                    weno5_half(vn[l][0], vn[l][1], vn[l][2], vn[l][3], vn[l][4], eps);
     }
     // back-projection with the Roe-averaged enthalpy (WENO.cu:229-243)
     const double s1 = fch[0] + fch[3] + fch[4], s3 = fch[0] - fch[4];
-    const double H = rlc * (qa[4] + pa) + rrc * (qb[4] + pb);
     F[id] = s1;
     F[id + sz] = um * s1 - kx * cm * s3 + nx * fch[1] + qx * fch[2];
     F[id + 2 * sz] = vm * s1 - ky * cm * s3 + ny * fch[1] + qy * fch[2];
