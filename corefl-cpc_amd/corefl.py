@@ -59,7 +59,7 @@ def load_library(path=LIB_PATH):
     P, I, D = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
     lib.corefl_create.argtypes = [ctypes.c_char_p, ctypes.POINTER(P)]
     lib.corefl_create_mem.argtypes = [ctypes.c_char_p, I, ctypes.POINTER(I), ctypes.POINTER(P), ctypes.c_char_p,
-                                      ctypes.POINTER(P)]
+                                      ctypes.c_char_p, ctypes.POINTER(P)]
     lib.corefl_destroy.argtypes = [P]
     lib.corefl_destroy.restype = None
     lib.corefl_last_error.restype = ctypes.c_char_p
@@ -128,7 +128,7 @@ def _check(rc):
 class Solver:
     """One COREFL context: input contract in, device state and SSP-RK3 steps out."""
 
-    def __init__(self, input_dir=None, *, setup_text=None, blocks=None, boundary_text=None):
+    def __init__(self, input_dir=None, *, setup_text=None, blocks=None, boundary_text=None, inner_text=None):
         lib = load_library()
         self._lib = lib
         self._ctx = ctypes.c_void_p()
@@ -146,7 +146,7 @@ class Solver:
                     keep.append(flat)
                     ptrs[3 * b + q] = flat.ctypes.data
             _check(lib.corefl_create_mem(setup_text.encode(), len(blocks), dims, ptrs, boundary_text.encode(),
-                                         ctypes.byref(self._ctx)))
+                                         inner_text.encode() if inner_text else None, ctypes.byref(self._ctx)))
 
     def close(self):
         if self._ctx:

@@ -104,6 +104,8 @@ struct Block {
   corefl::MixArrays mix{}; // species arrays (ns = 0: single-species air)
   double *hstep = nullptr; // per-cell chemistry step size carried between sub-steps
   std::vector<corefl::DevFace> faces; // in boundary-file order
+  std::vector<corefl::DevInner> inner; // block-to-block faces, in inner-file order
+  std::vector<int> inner_target;       // their partner blocks
   long part_offset = 0;
   int part_count = 0;
 };
@@ -159,6 +161,27 @@ struct corefl_ctx {
         }
   }
 
+  // data_communication (DataCommunication.cu:6-34): every block-to-block face in block and file order, one
+  // launch each, as the reference averages coincident nodes face after face.
+  void communicate(hipStream_t s) {
+    for (auto &b : blocks)
+      for (size_t l = 0; l < b.inner.size(); ++l) {
+        Block &t = blocks[b.inner_target[l]];
+        if (ns) HIPCHK(corefl::launch_mix_inner_exchange(b.dev, b.mix, t.dev, t.mix, b.inner[l], s));
+        else HIPCHK(corefl::launch_inner_exchange(b.dev, t.dev, b.inner[l], s));
+      }
+  }
+
+  // exchange_value task 1 (DataCommunication.cu:408-456): block-to-block faces, then periodic faces.
+  void viscous_exchange(hipStream_t s) {
+    for (auto &b : blocks)
+      for (size_t l = 0; l < b.inner.size(); ++l)
+        HIPCHK(corefl::launch_inner_viscous_exchange(b.dev, blocks[b.inner_target[l]].dev, b.inner[l], s));
+    for (auto &b : blocks)
+      for (const auto &f : b.faces)
+        if (f.type == 8) HIPCHK(corefl::launch_viscous_exchange(b.dev, f, s));
+  }
+
   // Exact chemistry over scale * dt (dt from the device) in every interior cell of every block
   // (constant density and internal energy), then boundary conditions.
   void chem_substep(double scale, hipStream_t s) {
@@ -177,17 +200,15 @@ struct corefl_ctx {
       HIPCHK(corefl::launch_mix_chem_scatter(b.dev, b.mix, chem->b, s));
       apply_bcs(b, s);
     }
+    communicate(s);
   }
 
   // Residual of one stage for all blocks; stage < 0 writes dq only.
   void stage(int rk, hipStream_t s) {
     const bool rhs_only = rk < 0;
     if (scheme.viscous_order > 0) {
-      for (auto &b : blocks) {
-        HIPCHK(corefl::launch_viscous_flux(b.dev, scheme, s));
-        for (const auto &f : b.faces)
-          if (f.type == 8) HIPCHK(corefl::launch_viscous_exchange(b.dev, f, s));
-      }
+      for (auto &b : blocks) HIPCHK(corefl::launch_viscous_flux(b.dev, scheme, s));
+      viscous_exchange(s);
     }
     for (auto &b : blocks) {
       if (ns) {
@@ -204,6 +225,7 @@ struct corefl_ctx {
       HIPCHK(corefl::launch_update(b.dev, scheme, rhs_only ? 0 : rk, dt_dev, rhs_only, s));
       if (!rhs_only) apply_bcs(b, s);
     }
+    if (!rhs_only) communicate(s);
   }
 
   void enqueue_dt(hipStream_t s, int clip) {
@@ -386,6 +408,34 @@ void build(corefl_ctx *ctx, std::vector<corefl::HostBlock> blocks) {
   HIPCHK(hipStreamCreate(&ctx->stream));
   ctx->dt_dev = ctx->alloc(1);
   ctx->time_dev = ctx->alloc(1);
+  // Block-to-block faces: the coincident layer on a side of the block, the in-face ranges inside it, and every
+  // partner node inside the target block (the exchange kernels index both blocks without further checks).
+  for (size_t bi = 0; bi < blocks.size(); ++bi) {
+    const auto &hb = blocks[bi];
+    const int ext[3] = {hb.mx, hb.my, hb.mz};
+    for (const auto &f : hb.inner) {
+      if (f.target_block < 0 || f.target_block >= (int)blocks.size() || f.target_block == (int)bi)
+        throw std::runtime_error("a block-to-block face must join two different blocks");
+      const auto &T = blocks[f.target_block];
+      const int text[3] = {T.mx, T.my, T.mz};
+      if (f.face_layer != (f.direction < 0 ? 0 : ext[f.face] - 1))
+        throw std::runtime_error("block-to-block face is not on a block side");
+      for (int d = 0; d < 3; ++d) {
+        if (d == f.face) continue;
+        const int lo = std::min(f.start[d], f.end[d]), hi = std::max(f.start[d], f.end[d]);
+        if (lo < 0 || hi >= ext[d]) throw std::runtime_error("block-to-block face range outside the block");
+      }
+      for (int c = 0; c < 8; ++c) {
+        int dd[3];
+        for (int q = 0; q < 3; ++q) dd[q] = f.loop_dir[q] * ((c >> q & 1) ? f.n_point[q] - 1 : 0);
+        for (int q = 0; q < 3; ++q) {
+          const int t = f.target_start[q] + f.target_loop_dir[q] * dd[f.src_tar[q]];
+          if (f.src_tar[q] < 0 || f.src_tar[q] > 2 || t < 0 || t >= text[q])
+            throw std::runtime_error("block-to-block face maps outside its target block");
+        }
+      }
+    }
+  }
   for (auto &hb : blocks) {
     // Every kernel indexes at most ngg nodes beyond the interior; reject blocks and faces that would
     // make any of those accesses leave the allocation.
@@ -452,6 +502,25 @@ void build(corefl_ctx *ctx, std::vector<corefl::HostBlock> blocks) {
       df.type = f.type; df.face = f.face; df.direction = f.direction;
       B.faces.push_back(df);
     }
+    for (const auto &f : h.inner) {
+      corefl::DevInner di{};
+      for (int q = 0; q < 3; ++q) {
+        di.start[q] = f.start[q];
+        di.loop_dir[q] = f.loop_dir[q];
+        di.n_point[q] = f.n_point[q];
+        di.target_start[q] = f.target_start[q];
+        di.target_loop_dir[q] = f.target_loop_dir[q];
+        di.src_tar[q] = f.src_tar[q];
+      }
+      di.face = f.face;
+      di.face_layer = f.face_layer;
+      for (int q = 0; q < 3; ++q) {
+        di.vmap[f.src_tar[q]] = q;
+        di.vsign[f.src_tar[q]] = f.target_loop_dir[q];
+      }
+      B.inner.push_back(di);
+      B.inner_target.push_back(f.target_block);
+    }
     B.part_offset = ctx->n_parts;
     B.part_count = corefl::dt_partial_count(d);
     ctx->n_parts += B.part_count;
@@ -476,6 +545,7 @@ void build(corefl_ctx *ctx, std::vector<corefl::HostBlock> blocks) {
     }
     ctx->apply_bcs(B, ctx->stream);
   }
+  ctx->communicate(ctx->stream);
   ctx->host_dt = P.get_real("dt");
   HIPCHK(hipMemcpyAsync(ctx->dt_dev, &ctx->host_dt, sizeof(double), hipMemcpyHostToDevice, ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -492,12 +562,17 @@ int fail(const std::exception &e) {
   return -1;
 }
 
+// Boundary and block-to-block faces, ghost grid (mirrored, then the neighbours' nodes across block-to-block
+// faces: Mesh.cpp:1013-1083), metrics.
 std::vector<corefl::HostBlock> prepare_blocks(corefl::Parameter &P, std::vector<corefl::HostBlock> blocks,
-                                              const std::string &boundary_text) {
+                                              const std::string &boundary_text, const std::string &inner_text) {
   corefl::parse_boundaries(boundary_text, blocks, P.get_int("periodic_label"));
+  if (!inner_text.empty()) corefl::parse_inner_faces(inner_text, blocks);
+  for (auto &b : blocks) corefl::mirror_ghost_grid(b);
+  corefl::fill_inner_ghost_grid(blocks);
   for (auto &b : blocks) {
     std::string err;
-    if (!corefl::compute_metrics(b, err)) throw std::runtime_error(err);
+    if (!corefl::compute_metrics(b, err, false)) throw std::runtime_error(err);
   }
   return blocks;
 }
@@ -522,7 +597,10 @@ int corefl_create(const char *input_dir, corefl_ctx **out) {
     auto &P = ctx->param;
     if (!P.get_int("gridIsBinary")) throw std::runtime_error("only binary grids (gridIsBinary = 1) are read");
     auto blocks = corefl::read_grid_binary(dir + "/grid/grid   0.dat", P.get_int("ngg"), P.get_real("gridScale"));
-    blocks = prepare_blocks(P, std::move(blocks), read_file(dir + "/boundary_condition/boundary   0.txt"));
+    // inner%4d.txt lists the block-to-block faces; a single block needs none
+    std::string inner_text;
+    if (std::ifstream(dir + "/boundary_condition/inner   0.txt")) inner_text = read_file(dir + "/boundary_condition/inner   0.txt");
+    blocks = prepare_blocks(P, std::move(blocks), read_file(dir + "/boundary_condition/boundary   0.txt"), inner_text);
     build(ctx.get(), std::move(blocks));
     *out = ctx.release();
     return 0;
@@ -532,7 +610,7 @@ int corefl_create(const char *input_dir, corefl_ctx **out) {
 }
 
 int corefl_create_mem(const char *setup_text, int n_block, const int *dims, const double *const *xyz,
-                      const char *boundary_text, corefl_ctx **out) {
+                      const char *boundary_text, const char *inner_text, corefl_ctx **out) {
   try {
     auto ctx = std::make_unique<corefl_ctx>();
     ctx->param.parse(setup_text);
@@ -542,7 +620,7 @@ int corefl_create_mem(const char *setup_text, int n_block, const int *dims, cons
     for (int b = 0; b < n_block; ++b)
       blocks.push_back(corefl::make_block(dims[3 * b], dims[3 * b + 1], dims[3 * b + 2], P.get_int("ngg"),
                                           xyz[3 * b], xyz[3 * b + 1], xyz[3 * b + 2], P.get_real("gridScale")));
-    blocks = prepare_blocks(P, std::move(blocks), boundary_text);
+    blocks = prepare_blocks(P, std::move(blocks), boundary_text, inner_text ? inner_text : "");
     build(ctx.get(), std::move(blocks));
     *out = ctx.release();
     return 0;
@@ -611,6 +689,7 @@ int corefl_set_state(corefl_ctx *ctx, int block, const double *bv_host, long lon
       HIPCHK(corefl::launch_cv_from_bv_all(B.dev, ctx->stream));
     }
     ctx->apply_bcs(B, ctx->stream);
+    ctx->communicate(ctx->stream);
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return 0;
   } catch (const std::exception &e) {
@@ -731,10 +810,7 @@ int corefl_profile_kernels(corefl_ctx *ctx, int reps, double *ms_out, int n_out)
     for (auto &b : ctx->blocks) {
       if (sc.viscous_order > 0) {
         timed(0, [&] { HIPCHK(corefl::launch_viscous_flux(b.dev, sc, s)); });
-        timed(1, [&] {
-          for (const auto &f : b.faces)
-            if (f.type == 8) HIPCHK(corefl::launch_viscous_exchange(b.dev, f, s));
-        });
+        timed(1, [&] { ctx->viscous_exchange(s); });
       }
       if (sc.inviscid_scheme == 2) HIPCHK(corefl::launch_entropy_fix(b.dev, sc, s));
       timed(2, [&] { HIPCHK(corefl::launch_inviscid_flux(b.dev, sc, 0, s)); });

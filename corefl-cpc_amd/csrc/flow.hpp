@@ -22,6 +22,15 @@ struct DevFace {
   int type, face, direction;
 };
 
+// A block-to-block face on the device (see InnerFace in mesh.hpp).
+struct DevInner {
+  int start[3], loop_dir[3], n_point[3];
+  int target_start[3], target_loop_dir[3], src_tar[3];
+  int face, face_layer;
+  // the partner's index direction vmap[s] runs along this block's direction s, with sign vsign[s]
+  int vmap[3], vsign[3];
+};
+
 struct Scheme {
   int inviscid_scheme; // 1 LF, 2 Roe, 3 AUSM+, 4 HLLC; 51/52/71/72 LLF-split WENO5/7, component-wise/characteristic
   int reconstruction;  // 4 WENO5, 5 WENO7 (characteristic) for schemes 1-4
@@ -35,6 +44,10 @@ hipError_t launch_cv_from_bv_all(const DevBlock &b, hipStream_t s);
 hipError_t launch_boundary(const DevBlock &b, const DevFace &f, hipStream_t s);
 hipError_t launch_viscous_flux(const DevBlock &b, const Scheme &sc, hipStream_t s);
 hipError_t launch_viscous_exchange(const DevBlock &b, const DevFace &f, hipStream_t s);
+// inner_communication<Air> / inner_exchange task 1 (DataCommunication.cu:36-103, 458-495) for one face of b
+// whose partner block is t.
+hipError_t launch_inner_exchange(const DevBlock &b, const DevBlock &t, const DevInner &f, hipStream_t s);
+hipError_t launch_inner_viscous_exchange(const DevBlock &b, const DevBlock &t, const DevInner &f, hipStream_t s);
 hipError_t launch_entropy_fix(const DevBlock &b, const Scheme &sc, hipStream_t s);
 hipError_t launch_inviscid_flux(const DevBlock &b, const Scheme &sc, int dir, hipStream_t s);
 // Residual + SSP-RK3 stage update; rhs_only writes dq instead of updating.  dt is read from *dt_dev.

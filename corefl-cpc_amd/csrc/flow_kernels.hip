@@ -880,6 +880,79 @@ __global__ __launch_bounds__(256) void k_boundary(DevBlock b, DevFace f) {
   cons_from_prim(b, ig);
 }
 
+// One node of a block-to-block face: its index in b and its partner's index in the target block.
+__device__ __forceinline__ bool inner_item(const DevInner &f, int *idx, int *it) {
+  const long n0 = f.n_point[0], n1 = f.n_point[1], n2 = f.n_point[2];
+  const long item = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (item >= n0 * n1 * n2) return false;
+  const int n[3] = {(int)(item % n0), (int)((item / n0) % n1), (int)(item / (n0 * n1))};
+  int d[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    d[c] = f.loop_dir[c] * n[c];
+    idx[c] = f.start[c] + d[c];
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) it[c] = f.target_start[c] + f.target_loop_dir[c] * d[f.src_tar[c]];
+  return true;
+}
+
+// inner_communication<Air> (DataCommunication.cu:36-103): the coincident layer takes the average of both
+// blocks (or the partner's state where only this side moves, as at a wall corner), the ghost layers the
+// partner's interior state; cv follows bv.
+__global__ __launch_bounds__(256) void k_inner_exchange(DevBlock b, DevBlock t, DevInner f) {
+  int idx[3], it[3];
+  if (!inner_item(f, idx, it)) return;
+  const long id = nid(b, idx[0], idx[1], idx[2]), jt = nid(t, it[0], it[1], it[2]);
+  const long sz = b.sz, tz = t.sz;
+  if (idx[f.face] == f.face_layer) {
+    const double us = sqrt(b.bv[id + sz] * b.bv[id + sz] + b.bv[id + 2 * sz] * b.bv[id + 2 * sz] +
+                           b.bv[id + 3 * sz] * b.bv[id + 3 * sz]);
+    if (us > 1e-15) {
+      const double ut = sqrt(t.bv[jt + tz] * t.bv[jt + tz] + t.bv[jt + 2 * tz] * t.bv[jt + 2 * tz] +
+                             t.bv[jt + 3 * tz] * t.bv[jt + 3 * tz]);
+      if (ut < 1e-15) {
+        for (int l = 0; l < 6; ++l) b.bv[id + l * sz] = t.bv[jt + l * tz];
+      } else {
+        for (int l = 0; l < 6; ++l) {
+          const double a = 0.5 * (t.bv[jt + l * tz] + b.bv[id + l * sz]);
+          b.bv[id + l * sz] = a;
+          t.bv[jt + l * tz] = a;
+        }
+      }
+    }
+  } else {
+    for (int l = 0; l < 6; ++l) b.bv[id + l * sz] = t.bv[jt + l * tz];
+  }
+  cons_from_prim(b, id);
+}
+
+// inner_exchange task 1 (DataCommunication.cu:458-495): collocated viscous fluxes (components 1..4).  These
+// are contravariant fluxes along each block's own index directions, so the partner's flux along direction
+// vmap[a], signed by vsign[a], is this block's flux along a (the reference copies direction a to direction a,
+// which is the same thing when both blocks' index directions agree).
+__global__ __launch_bounds__(256) void k_inner_viscous_exchange(DevBlock b, DevBlock t, DevInner f) {
+  int idx[3], it[3];
+  if (!inner_item(f, idx, it)) return;
+  const long id = nid(b, idx[0], idx[1], idx[2]), jt = nid(t, it[0], it[1], it[2]);
+  const long sz = b.sz, tz = t.sz;
+  double *A[3] = {b.fv, b.gv, b.hv}, *T[3] = {t.fv, t.gv, t.hv};
+  const bool coincident = idx[f.face] == f.face_layer;
+  for (int a = 0; a < 3; ++a) {
+    double *P = T[f.vmap[a]];
+    const double sg = f.vsign[a];
+    for (int l = 1; l < 5; ++l) {
+      if (coincident) {
+        const double v = 0.5 * (A[a][id + l * sz] + sg * P[jt + l * tz]);
+        A[a][id + l * sz] = v;
+        P[jt + l * tz] = sg * v;
+      } else {
+        A[a][id + l * sz] = sg * P[jt + l * tz];
+      }
+    }
+  }
+}
+
 __global__ void k_cv_from_bv(DevBlock b) {
   const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (id >= b.sz) return;
@@ -974,6 +1047,21 @@ hipError_t launch_viscous_flux(const DevBlock &b, const Scheme &sc, hipStream_t 
 
 hipError_t launch_viscous_exchange(const DevBlock &b, const DevFace &f, hipStream_t s) {
   k_viscous_exchange<<<face_blocks(b, f), 256, 0, s>>>(b, f);
+  return hipGetLastError();
+}
+
+static unsigned inner_blocks(const DevInner &f) {
+  const long n = (long)f.n_point[0] * f.n_point[1] * f.n_point[2];
+  return (unsigned)((n + 255) / 256);
+}
+
+hipError_t launch_inner_exchange(const DevBlock &b, const DevBlock &t, const DevInner &f, hipStream_t s) {
+  k_inner_exchange<<<inner_blocks(f), 256, 0, s>>>(b, t, f);
+  return hipGetLastError();
+}
+
+hipError_t launch_inner_viscous_exchange(const DevBlock &b, const DevBlock &t, const DevInner &f, hipStream_t s) {
+  k_inner_viscous_exchange<<<inner_blocks(f), 256, 0, s>>>(b, t, f);
   return hipGetLastError();
 }
 

@@ -25,6 +25,9 @@ struct MixArrays {
 
 hipError_t launch_mix_cv_from_bv_all(const DevBlock &b, const MixArrays &x, hipStream_t s);
 hipError_t launch_mix_boundary(const DevBlock &b, const MixArrays &x, const DevFace &f, hipStream_t s);
+// inner_communication<Mixture> (DataCommunication.cu:36-103) for one face of b whose partner block is t.
+hipError_t launch_mix_inner_exchange(const DevBlock &b, const MixArrays &x, const DevBlock &t, const MixArrays &y,
+                                     const DevInner &f, hipStream_t s);
 hipError_t launch_mix_entropy_fix(const DevBlock &b, const MixArrays &x, const Scheme &sc, hipStream_t s);
 // reconstruction 1 first order, 2 MUSCL, 3 NND2 (minmod), 4 / 5 component-wise WENO5 / WENO7;
 // inviscid_scheme 2 Roe, 3 AUSM+, 4 HLLC.

@@ -50,6 +50,51 @@ __global__ void k_mix_cv_from_bv(DevBlock b, MixArrays x) {
   mix_cons(b, x, id);
 }
 
+// inner_communication<Mixture> (DataCommunication.cu:36-103): as the single-species face exchange, with the
+// mass fractions exchanged alongside bv and cv (and the acoustic speed) recomputed from both.
+__global__ __launch_bounds__(256) void k_mix_inner_exchange(DevBlock b, MixArrays x, DevBlock t, MixArrays y,
+                                                            DevInner f) {
+  const long n0 = f.n_point[0], n1 = f.n_point[1], n2 = f.n_point[2];
+  const long item = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (item >= n0 * n1 * n2) return;
+  const int n[3] = {(int)(item % n0), (int)((item / n0) % n1), (int)(item / (n0 * n1))};
+  int d[3], idx[3], it[3];
+  for (int c = 0; c < 3; ++c) {
+    d[c] = f.loop_dir[c] * n[c];
+    idx[c] = f.start[c] + d[c];
+  }
+  for (int c = 0; c < 3; ++c) it[c] = f.target_start[c] + f.target_loop_dir[c] * d[f.src_tar[c]];
+  const long id = nid(b, idx[0], idx[1], idx[2]), jt = nid(t, it[0], it[1], it[2]);
+  const long sz = b.sz, tz = t.sz;
+  bool copy = idx[f.face] != f.face_layer, average = false;
+  if (!copy) {
+    const double us = sqrt(b.bv[id + sz] * b.bv[id + sz] + b.bv[id + 2 * sz] * b.bv[id + 2 * sz] +
+                           b.bv[id + 3 * sz] * b.bv[id + 3 * sz]);
+    if (us > 1e-15) {
+      const double ut = sqrt(t.bv[jt + tz] * t.bv[jt + tz] + t.bv[jt + 2 * tz] * t.bv[jt + 2 * tz] +
+                             t.bv[jt + 3 * tz] * t.bv[jt + 3 * tz]);
+      copy = ut < 1e-15;
+      average = !copy;
+    }
+  }
+  if (average) {
+    for (int l = 0; l < 6; ++l) {
+      const double a = 0.5 * (t.bv[jt + l * tz] + b.bv[id + l * sz]);
+      b.bv[id + l * sz] = a;
+      t.bv[jt + l * tz] = a;
+    }
+    for (int l = 0; l < NS; ++l) {
+      const double a = 0.5 * (y.sv[jt + l * tz] + x.sv[id + l * sz]);
+      x.sv[id + l * sz] = a;
+      y.sv[jt + l * tz] = a;
+    }
+  } else if (copy) {
+    for (int l = 0; l < 6; ++l) b.bv[id + l * sz] = t.bv[jt + l * tz];
+    for (int l = 0; l < NS; ++l) x.sv[id + l * sz] = y.sv[jt + l * tz];
+  }
+  mix_cons(b, x, id);
+}
+
 // apply_outflow / apply_periodic with the species (BoundCond.cu:2065-2188, 3033-3083); work items as in
 // the single-species fill: one (face node, ghost layer) pair each.
 __global__ __launch_bounds__(256) void k_mix_boundary(DevBlock b, MixArrays x, DevFace f) {
@@ -647,6 +692,14 @@ void dispatch_rec(const DevBlock &b, const MixArrays &x, const Scheme &sc, dim3 
 hipError_t launch_mix_cv_from_bv_all(const DevBlock &b, const MixArrays &x, hipStream_t s) {
   if (x.ns != NS) return hipErrorNotSupported;
   k_mix_cv_from_bv<<<(unsigned)((b.sz + 255) / 256), 256, 0, s>>>(b, x);
+  return hipGetLastError();
+}
+
+hipError_t launch_mix_inner_exchange(const DevBlock &b, const MixArrays &x, const DevBlock &t, const MixArrays &y,
+                                     const DevInner &f, hipStream_t s) {
+  if (x.ns != NS || y.ns != NS) return hipErrorNotSupported;
+  const long n = (long)f.n_point[0] * f.n_point[1] * f.n_point[2];
+  k_mix_inner_exchange<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(b, x, t, y, f);
   return hipGetLastError();
 }
 

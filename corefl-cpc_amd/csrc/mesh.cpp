@@ -2,6 +2,7 @@
 #include "mesh.hpp"
 
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <sstream>
@@ -169,8 +170,157 @@ void parse_boundaries(const std::string &text, std::vector<HostBlock> &blocks, i
   }
 }
 
-bool compute_metrics(HostBlock &b, std::string &err) {
-  mirror_ghosts(b);
+void mirror_ghost_grid(HostBlock &b) { mirror_ghosts(b); }
+
+namespace {
+int sgn(int v) { return (v > 0) - (v < 0); }
+
+// InnerFace::register_boundary (Mesh.cpp:39-173): s, t = (x1, x2, y1, y2, z1, z2) of this block and of the
+// target, 1-based; the normal direction has equal ends, the first in-face direction is marked negative in 3D.
+InnerFace register_inner(const int *s, const int *t, int target_id, int dim, int ngg) {
+  InnerFace f{};
+  int rs[3] = {s[0], s[2], s[4]}, re[3] = {s[1], s[3], s[5]};
+  int ts[3] = {t[0], t[2], t[4]}, te[3] = {t[1], t[3], t[5]};
+  f.target_block = target_id - 1;
+  f.face = -1;
+  f.direction = 1;
+  for (int d = 0; d < dim; ++d)
+    if (rs[d] == re[d]) {
+      f.face = d;
+      if (rs[d] == 1) f.direction = -1;
+      --rs[d];
+      --re[d];
+      break;
+    }
+  f.target_face = -1;
+  f.target_direction = 1;
+  for (int d = 0; d < dim; ++d)
+    if (ts[d] == te[d]) {
+      f.target_face = d;
+      if (ts[d] == 1) f.target_direction = -1;
+      --ts[d];
+      --te[d];
+      break;
+    }
+  if (f.face < 0 || f.target_face < 0) throw std::runtime_error("inner face without a constant index direction");
+  int st[3] = {0, 0, 0}; // signed, 1-based source direction of each target direction
+  st[f.target_face] = -(f.direction * f.target_direction) * (f.face + 1);
+  auto pair = [&](int a, int ta) {
+    st[ta] = sgn((te[ta] - ts[ta]) * (re[a] - rs[a])) * (a + 1);
+    if (st[ta] == 0) throw std::runtime_error("inner face with a single-node in-face range");
+  };
+  if (dim == 2) {
+    const int a = 1 - f.face, ta = 1 - f.target_face;
+    rs[a] = std::abs(rs[a]) - 1;
+    re[a] = std::abs(re[a]) - 1;
+    ts[ta] = std::abs(ts[ta]) - 1;
+    te[ta] = std::abs(te[ta]) - 1;
+    pair(a, ta);
+    st[2] = 3;
+    rs[2] = re[2] = ts[2] = te[2] = 0;
+  } else {
+    int a = -1, ta = -1;
+    for (int d = 0; d < 3 && a < 0; ++d)
+      if (rs[d] < 0) a = d;
+    for (int d = 0; d < 3 && ta < 0; ++d)
+      if (ts[d] < 0) ta = d;
+    if (a < 0 || ta < 0 || a == f.face || ta == f.target_face)
+      throw std::runtime_error("inner face: the first in-face direction must be marked negative on both sides");
+    rs[a] = -rs[a] - 1;
+    re[a] = -re[a] - 1;
+    ts[ta] = -ts[ta] - 1;
+    te[ta] = -te[ta] - 1;
+    pair(a, ta);
+    const int b = 3 - f.face - a, tb = 3 - f.target_face - ta;
+    --rs[b];
+    --re[b];
+    --ts[tb];
+    --te[tb];
+    pair(b, tb);
+  }
+  // the normal range becomes the coincident layer plus the ngg ghost layers
+  f.face_layer = f.direction > 0 ? rs[f.face] : re[f.face];
+  if (f.direction < 0) {
+    rs[f.face] = -ngg;
+    re[f.face] = 0;
+  } else {
+    rs[f.face] = re[f.face];
+    re[f.face] = re[f.face] + ngg;
+  }
+  const int sg = sgn(st[f.target_face]);
+  if (f.target_direction > 0) ts[f.target_face] = sg > 0 ? te[f.target_face] - ngg : te[f.target_face];
+  else ts[f.target_face] = sg > 0 ? 0 : ngg;
+  for (int d = 0; d < 3; ++d) {
+    f.start[d] = rs[d];
+    f.end[d] = re[d];
+    f.target_start[d] = ts[d];
+    f.loop_dir[d] = rs[d] <= re[d] ? 1 : -1;
+    f.target_loop_dir[d] = sgn(st[d]);
+    f.src_tar[d] = std::abs(st[d]) - 1;
+    f.n_point[d] = std::abs(rs[d] - re[d]) + 1;
+  }
+  return f;
+}
+} // namespace
+
+void parse_inner_faces(const std::string &text, std::vector<HostBlock> &blocks) {
+  std::istringstream in(text);
+  std::string line;
+  if (!std::getline(in, line)) return; // total count (unused)
+  std::vector<int> n_face(blocks.size());
+  for (auto &n : n_face)
+    if (!(in >> n) || n < 0) throw std::runtime_error("inner face file: bad face count");
+  std::getline(in, line); // rest of the count line
+  std::getline(in, line); // column header
+  for (size_t b = 0; b < blocks.size(); ++b)
+    for (int l = 0; l < n_face[b]; ++l) {
+      int s[7], t[7];
+      for (int *r : {s, t}) {
+        if (!std::getline(in, line)) throw std::runtime_error("inner face file: truncated");
+        std::istringstream ls(line);
+        for (int q = 0; q < 7; ++q)
+          if (!(ls >> r[q])) throw std::runtime_error("inner face file: bad record");
+      }
+      if (t[6] < 1 || t[6] > (int)blocks.size()) throw std::runtime_error("inner face file: bad target block");
+      blocks[b].inner.push_back(register_inner(s, t, t[6], blocks[b].dim, blocks[b].ngg));
+    }
+}
+
+void fill_inner_ghost_grid(std::vector<HostBlock> &blocks) {
+  for (auto &B : blocks) {
+    const int G = B.ngg + 1;
+    for (const auto &f : B.inner) {
+      HostBlock &T = blocks[f.target_block];
+      Coord bx{B.x, B.mx, B.my, B.mz, G}, by{B.y, B.mx, B.my, B.mz, G}, bz{B.z, B.mx, B.my, B.mz, G};
+      Coord tx{T.x, T.mx, T.my, T.mz, G}, ty{T.y, T.mx, T.my, T.mz, G}, tz{T.z, T.mx, T.my, T.mz, G};
+      // the ngg + 1 ghost layers beyond the coincident layer, from the target's interior (init_inner_ghost_grid)
+      int lo[3] = {f.start[0], f.start[1], f.start[2]}, tl[3] = {f.target_start[0], f.target_start[1],
+                                                                 f.target_start[2]};
+      lo[f.face] += f.direction < 0 ? -1 : 1;
+      tl[f.target_face] += f.target_direction > 0 ? -1 : 1;
+      const int T_ext[3] = {T.mx, T.my, T.mz};
+      for (int n2 = 0; n2 < f.n_point[2]; ++n2)
+        for (int n1 = 0; n1 < f.n_point[1]; ++n1)
+          for (int n0 = 0; n0 < f.n_point[0]; ++n0) {
+            const int d[3] = {f.loop_dir[0] * n0, f.loop_dir[1] * n1, f.loop_dir[2] * n2};
+            int a[3], t[3];
+            for (int c = 0; c < 3; ++c) a[c] = lo[c] + d[c];
+            const int B_ext[3] = {B.mx, B.my, B.mz};
+            for (int c = 0; c < 3; ++c)
+              if (a[c] < -G || a[c] >= B_ext[c] + G) throw std::runtime_error("inner face outside the ghost grid");
+            for (int c = 0; c < 3; ++c) t[c] = tl[c] + f.target_loop_dir[c] * d[f.src_tar[c]];
+            for (int c = 0; c < 3; ++c)
+              if (t[c] < 0 || t[c] >= T_ext[c]) throw std::runtime_error("inner face maps outside the target block");
+            bx(a[0], a[1], a[2]) = tx(t[0], t[1], t[2]);
+            by(a[0], a[1], a[2]) = ty(t[0], t[1], t[2]);
+            bz(a[0], a[1], a[2]) = tz(t[0], t[1], t[2]);
+          }
+    }
+  }
+}
+
+bool compute_metrics(HostBlock &b, std::string &err, bool mirror) {
+  if (mirror) mirror_ghosts(b);
   const int g = b.ngg, G = g + 1;
   const long sz = b.size();
   b.jac.assign(sz, 0.0);

@@ -52,16 +52,19 @@ enum {
 
 /* Replaces Parameter(&argc,&argv) + Mesh(parameter) + Driver<mix>(parameter, mesh)
  * (src/Parameter.cpp:9-22, src/Mesh.cpp:619-682, src/Driver.cu:13-124): reads
- * <input_dir>/setup.txt, <input_dir>/grid/grid   0.dat, <input_dir>/boundary_condition/boundary   0.txt,
+ * <input_dir>/setup.txt, <input_dir>/grid/grid   0.dat, <input_dir>/boundary_condition/boundary   0.txt
+ * and, when present, <input_dir>/boundary_condition/inner   0.txt (block-to-block faces; the ghost layers
+ * across them are refreshed after every stage as data_communication does, src/DataCommunication.cu:6-103),
  * computes metrics, allocates device memory and sets the initial condition
  * (src/Initialize.cu:51-95) followed by initialize_computation (src/Driver.cu:126-211). */
 int corefl_create(const char *input_dir, corefl_ctx **out);
 
-/* Same as corefl_create with the three inputs in memory: setup_text is the content of setup.txt;
+/* Same as corefl_create with the inputs in memory: setup_text is the content of setup.txt;
  * dims = n_block*(mx,my,mz); xyz[3*b+0..2] = x, y, z of block b (i fastest, mx*my*mz each);
- * boundary_text is the content of boundary%4d.txt. */
+ * boundary_text is the content of boundary%4d.txt; inner_text the content of inner%4d.txt (the
+ * block-to-block faces read by Mesh::read_inner_interface, src/Mesh.cpp:927-960), or NULL for none. */
 int corefl_create_mem(const char *setup_text, int n_block, const int *dims, const double *const *xyz,
-                      const char *boundary_text, corefl_ctx **out);
+                      const char *boundary_text, const char *inner_text, corefl_ctx **out);
 
 void corefl_destroy(corefl_ctx *ctx);
 const char *corefl_last_error(void);

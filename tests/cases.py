@@ -147,6 +147,76 @@ struct taylorGreen {{
                 L=L, rho0=rho0, V0=V0, eps=1e-40 * (rho0 * V0) ** 2, v_ref=V0)
 
 
+def inner_text(records):
+    """inner%4d.txt content: records[b] lists (own range + id, target range + id) pairs of block b, each a
+    7-tuple in the grid tool's convention (1-based; the constant index is the face normal; in 3D the
+    first in-face direction is given with negative values)."""
+    lines = [f"{sum(len(r) for r in records):9d}", "".join(f"{len(r):9d}\t" for r in records),
+             "iMin  iMax  jMin  jMax  kMin  kMax   id"]
+    for r in records:
+        for own, target in r:
+            lines.append("".join(f"{v:9d}" for v in own))
+            lines.append("".join(f"{v:9d}" for v in target))
+    return "\n".join(lines) + "\n"
+
+
+def tgv_blocks(n=13, split=6, layout="aligned", recon=5, inviscid=3, visc=4, dt=None):
+    """The Taylor-Green initial field on the uniform grid, outflow on the x sides and periodic in y and z, as
+    one block and as two blocks joined at global i = split (that node layer belongs to both) by a
+    block-to-block face.  layout: "aligned" (block 2 continues block 1's axes), "flipped" (block 2's i and j
+    run backwards), "rotated" (block 2's i runs along -y and its j along +x).  Returns (single, multi, place),
+    place(arr2) mapping block 2's interior array (ncomp, mz, my, mx) onto global (ncomp, k, j, i) indices."""
+    case = tgv(n, recon=recon, inviscid=inviscid, visc=visc, amp=0.0, fixed_dt=dt)
+    setup = case["setup"].replace("""array string boundary_conditions {
+    periodic
+}""", """array string boundary_conditions {
+    outflow periodic
+}
+struct outflow {
+    string type = outflow
+    int label = 6
+}""")
+    X, Y, Z = case["grid"]
+    faces = [[0, 0, 0, n - 1, 0, n - 1, 6], [n - 1, n - 1, 0, n - 1, 0, n - 1, 6],
+             [0, n - 1, 0, 0, 0, n - 1, 8], [0, n - 1, n - 1, n - 1, 0, n - 1, 8],
+             [0, n - 1, 0, n - 1, 0, 0, 8], [0, n - 1, 0, n - 1, n - 1, n - 1, 8]]
+    single = dict(case, setup=setup, faces=faces, boundary=boundary_text(faces))
+    a = split + 1  # block 1: global i = 0 .. split
+    m = n - split  # block 2 extent along x
+    g1 = tuple(c[:a] for c in (X, Y, Z))
+    f1 = [[0, 0, 0, n - 1, 0, n - 1, 6], [0, split, 0, 0, 0, n - 1, 8], [0, split, n - 1, n - 1, 0, n - 1, 8],
+          [0, split, 0, n - 1, 0, 0, 8], [0, split, 0, n - 1, n - 1, n - 1, 8]]
+    face1 = (a, a, -1, -n, 1, n)
+    if layout == "aligned":
+        g2 = tuple(c[split:] for c in (X, Y, Z))
+        f2 = [[m - 1, m - 1, 0, n - 1, 0, n - 1, 6], [0, m - 1, 0, 0, 0, n - 1, 8],
+              [0, m - 1, n - 1, n - 1, 0, n - 1, 8], [0, m - 1, 0, n - 1, 0, 0, 8],
+              [0, m - 1, 0, n - 1, n - 1, n - 1, 8]]
+        face2 = (1, 1, -1, -n, 1, n)
+        place = lambda v: v  # noqa: E731
+    elif layout == "flipped":
+        g2 = tuple(c[split:][::-1, ::-1, :].copy() for c in (X, Y, Z))
+        f2 = [[0, 0, 0, n - 1, 0, n - 1, 6], [0, m - 1, 0, 0, 0, n - 1, 8],
+              [0, m - 1, n - 1, n - 1, 0, n - 1, 8], [0, m - 1, 0, n - 1, 0, 0, 8],
+              [0, m - 1, 0, n - 1, n - 1, n - 1, 8]]
+        face2 = (m, m, -n, -1, 1, n)
+        place = lambda v: v[:, :, ::-1, ::-1]  # noqa: E731
+    elif layout == "rotated":
+        # block 2 node (i', j', k) is global (split + j', n - 1 - i', k)
+        g2 = tuple(np.transpose(c[split:], (1, 0, 2))[::-1, :, :].copy() for c in (X, Y, Z))
+        f2 = [[0, 0, 0, m - 1, 0, n - 1, 8], [n - 1, n - 1, 0, m - 1, 0, n - 1, 8],
+              [0, n - 1, m - 1, m - 1, 0, n - 1, 6], [0, n - 1, 0, m - 1, 0, 0, 8],
+              [0, n - 1, 0, m - 1, n - 1, n - 1, 8]]
+        face2 = (-n, -1, 1, 1, 1, n)
+        place = lambda v: np.transpose(v[:, :, :, ::-1], (0, 1, 3, 2))  # noqa: E731
+    else:
+        raise ValueError(layout)
+    inner = inner_text([[(face1 + (1,), face2 + (2,))], [(face2 + (2,), face1 + (1,))]])
+    multi = dict(single, grids=[g1, g2], boundary=boundary_text(f1) + boundary_text(f2), inner=inner,
+                 split=split)
+    return single, multi, place
+
+
 def physical_grid(case):
     """Coordinates after gridScale, as Mesh::scale applies it (Mesh.cpp:657)."""
     s = case["scale"]
