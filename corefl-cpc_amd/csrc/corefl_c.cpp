@@ -172,6 +172,16 @@ struct corefl_ctx {
       }
   }
 
+  // A state set block by block is exchanged across the block-to-block faces before the next computation,
+  // once every block holds its new state (the reference communicates once after initialising all blocks,
+  // Driver.cu:160-190).
+  bool exchange_pending = false;
+  void flush_exchange(hipStream_t s) {
+    if (!exchange_pending) return;
+    communicate(s);
+    exchange_pending = false;
+  }
+
   // exchange_value task 1 (DataCommunication.cu:408-456): block-to-block faces, then periodic faces.
   void viscous_exchange(hipStream_t s) {
     for (auto &b : blocks)
@@ -689,7 +699,7 @@ int corefl_set_state(corefl_ctx *ctx, int block, const double *bv_host, long lon
       HIPCHK(corefl::launch_cv_from_bv_all(B.dev, ctx->stream));
     }
     ctx->apply_bcs(B, ctx->stream);
-    ctx->communicate(ctx->stream);
+    ctx->exchange_pending = true;
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return 0;
   } catch (const std::exception &e) {
@@ -699,6 +709,7 @@ int corefl_set_state(corefl_ctx *ctx, int block, const double *bv_host, long lon
 
 int corefl_rhs(corefl_ctx *ctx, void *stream) {
   try {
+    ctx->flush_exchange(ctx->pick(stream));
     ctx->stage(-1, ctx->pick(stream));
     return 0;
   } catch (const std::exception &e) {
@@ -708,6 +719,7 @@ int corefl_rhs(corefl_ctx *ctx, void *stream) {
 
 int corefl_compute_dt(corefl_ctx *ctx, double *dt_out) {
   try {
+    ctx->flush_exchange(ctx->stream);
     ctx->enqueue_dt(ctx->stream, 0);
     HIPCHK(hipMemcpyAsync(dt_out, ctx->dt_dev, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -720,6 +732,7 @@ int corefl_compute_dt(corefl_ctx *ctx, double *dt_out) {
 int corefl_rk3_step(corefl_ctx *ctx, double *dt_inout, void *stream) {
   try {
     hipStream_t s = ctx->pick(stream);
+    ctx->flush_exchange(s);
     const bool given = dt_inout && *dt_inout > 0;
     if (given) HIPCHK(hipMemcpyAsync(ctx->dt_dev, dt_inout, sizeof(double), hipMemcpyHostToDevice, s));
     else if (ctx->fixed_dt) {
@@ -740,6 +753,7 @@ int corefl_rk3_step(corefl_ctx *ctx, double *dt_inout, void *stream) {
 int corefl_advance(corefl_ctx *ctx, int n_steps, int use_graph, void *stream) {
   try {
     hipStream_t s = ctx->pick(stream);
+    ctx->flush_exchange(s); // outside any capture
     if (ctx->fixed_dt) {
       ctx->host_dt = ctx->param.get_real("dt");
       HIPCHK(hipMemcpyAsync(ctx->dt_dev, &ctx->host_dt, sizeof(double), hipMemcpyHostToDevice, s));
@@ -793,6 +807,7 @@ int corefl_profile_kernels(corefl_ctx *ctx, int reps, double *ms_out, int n_out)
   try {
     if (reps < 1 || n_out < 1) throw std::runtime_error("reps and n_out must be positive");
     hipStream_t s = ctx->stream;
+    ctx->flush_exchange(s);
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));

@@ -76,7 +76,8 @@ int corefl_block_view_get(corefl_ctx *ctx, int block, corefl_block_view *view);
 int corefl_get_array(corefl_ctx *ctx, int block, int which, double *host, long long n);
 /* Host-to-device primitive state (ghosts included): bv (6*sz) followed, for mixtures, by the species mass
  * fractions sv (n_spec*sz); recomputes cv on all nodes and applies the block's boundary conditions
- * (src/FieldOperation.cuh:39-64, src/BoundCond.cu:3489-3619). */
+ * (src/FieldOperation.cuh:39-64, src/BoundCond.cu:3489-3619).  The exchange across block-to-block faces
+ * (src/DataCommunication.cu:6-103) runs before the next computation, once every block has been set. */
 int corefl_set_state(corefl_ctx *ctx, int block, const double *bv_host, long long n);
 
 /* One right-hand-side evaluation dq on every block: viscous flux (src/ViscousScheme.cu:7-73) and

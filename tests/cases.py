@@ -162,11 +162,16 @@ def inner_text(records):
 
 def tgv_blocks(n=13, split=6, layout="aligned", recon=5, inviscid=3, visc=4, dt=None):
     """The Taylor-Green initial field on the uniform grid, outflow on the x sides and periodic in y and z, as
-    one block and as two blocks joined at global i = split (that node layer belongs to both) by a
-    block-to-block face.  layout: "aligned" (block 2 continues block 1's axes), "flipped" (block 2's i and j
-    run backwards), "rotated" (block 2's i runs along -y and its j along +x).  Returns (single, multi, place),
-    place(arr2) mapping block 2's interior array (ncomp, mz, my, mx) onto global (ncomp, k, j, i) indices."""
-    case = tgv(n, recon=recon, inviscid=inviscid, visc=visc, amp=0.0, fixed_dt=dt)
+    one block and as two blocks joined at global i = split (see two_blocks)."""
+    return two_blocks(tgv(n, recon=recon, inviscid=inviscid, visc=visc, amp=0.0, fixed_dt=dt), n, split, layout)
+
+
+def two_blocks(case, n, split, layout="aligned"):
+    """A periodic n^3 box case made outflow on the x sides, as one block and as two blocks joined at global
+    i = split (that node layer belongs to both) by a block-to-block face.  layout: "aligned" (block 2
+    continues block 1's axes), "flipped" (block 2's i and j run backwards), "rotated" (block 2's i runs along
+    -y and its j along +x).  Returns (single, multi, place), place(arr2) mapping block 2's interior array
+    (ncomp, mz, my, mx) onto global (ncomp, k, j, i) indices."""
     setup = case["setup"].replace("""array string boundary_conditions {
     periodic
 }""", """array string boundary_conditions {
