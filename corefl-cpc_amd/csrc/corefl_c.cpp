@@ -184,6 +184,7 @@ struct corefl_ctx {
 
   // exchange_value task 1 (DataCommunication.cu:408-456): block-to-block faces, then periodic faces.
   void viscous_exchange(hipStream_t s) {
+    if (scheme.viscous_order == 2) return; // face fluxes come from the exchanged state itself
     for (auto &b : blocks)
       for (size_t l = 0; l < b.inner.size(); ++l)
         HIPCHK(corefl::launch_inner_viscous_exchange(b.dev, blocks[b.inner_target[l]].dev, b.inner[l], s));
@@ -388,7 +389,8 @@ void build(corefl_ctx *ctx, std::vector<corefl::HostBlock> blocks) {
     // upwind/WENO scheme, which is not on this path
     if (fs && P.get_real("shockSensor_threshold") > 1e-10)
       throw std::runtime_error("shockSensor_threshold > 0 (hybrid UD/WENO scheme) is not on this path");
-    if (vo != 0 && vo != 4 && vo != 8) throw std::runtime_error("viscous_order must be 0, 4 or 8 on this path");
+    if (vo != 0 && vo != 2 && vo != 4 && vo != 8)
+      throw std::runtime_error("viscous_order must be 0, 2, 4 or 8 on this path");
     if (temporal != 3) throw std::runtime_error("temporal_scheme must be 3 (SSP-RK3)");
   }
   const corefl::ChemMech *mech = ctx->chem ? &ctx->chem->mech : nullptr;
@@ -454,7 +456,7 @@ void build(corefl_ctx *ctx, std::vector<corefl::HostBlock> blocks) {
       throw std::runtime_error("block extents must be >= 3 (mz = 1 for 2D blocks)");
     for (int d = 0; d < hb.dim; ++d)
       if (ext[d] < ngg + 2) throw std::runtime_error("block extent smaller than ngg + 2");
-    if (vo > 0)
+    if (vo > 2)
       for (int d = 0; d < hb.dim; ++d)
         if (ext[d] < 5) throw std::runtime_error("viscous closures need at least 5 nodes per direction");
     for (auto &f : hb.faces) {

@@ -640,6 +640,67 @@ __device__ __forceinline__ double sutherland(double T) {
   return 1.7894e-5 * pow(T / 288.16, 1.5) * (288.16 + 110) / (T + 110); // Transport.cu:6-9
 }
 
+// compute_fv/gv/hv_2nd_order<Air> (ViscousScheme.cu:614-1010): viscous flux on the face between node t and
+// t+1 along DIR, stored at node t (components 1..4), for t = -1 .. m-1.  Metrics and metric * Jacobian are
+// the two nodes' averages; the derivative along DIR is the nodes' difference, the transverse ones the
+// average of the two nodes' central differences (none along z on 2D blocks); viscosity the nodes' average.
+template<int DIR>
+__global__ __launch_bounds__(256) void k_viscous_face(DevBlock b, Scheme sc, TileGrid tg) {
+  Tile t;
+  if (!xcd_tile(tg, t)) return;
+  int i = t.x * 64 + threadIdx.x, j = t.y * 4 + threadIdx.y, k = t.z;
+  if constexpr (DIR == 0) i -= 1;
+  else if constexpr (DIR == 1) j -= 1;
+  else k -= 1;
+  if (i >= b.mx || j >= b.my) return;
+  const long sz = b.sz;
+  const long st[3] = {1, b.d2, b.d1};
+  const long id = nid(b, i, j, k), ie = id + st[DIR];
+  const double *M = b.met;
+  double m[9];
+#pragma unroll
+  for (int c = 0; c < 9; ++c) m[c] = 0.5 * (M[id + c * sz] + M[ie + c * sz]);
+  constexpr int comp[4] = {1, 2, 3, 5};
+  double g[4][3];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const double *f = b.bv + comp[q] * sz;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      if (a == DIR) g[q][a] = f[ie] - f[id];
+      else if (a == 2 && b.dim == 2) g[q][a] = 0;
+      else g[q][a] = 0.25 * (f[id + st[a]] - f[id - st[a]] + f[ie + st[a]] - f[ie - st[a]]);
+    }
+  }
+  double d[4][3];
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) d[q][c] = g[q][0] * m[c] + g[q][1] * m[3 + c] + g[q][2] * m[6 + c];
+  const double mu = 0.5 * (sutherland(b.bv[id + 5 * sz]) + sutherland(b.bv[ie + 5 * sz]));
+  const double txx = mu * (4 * d[0][0] - 2 * d[1][1] - 2 * d[2][2]) / 3.0;
+  const double tyy = mu * (4 * d[1][1] - 2 * d[0][0] - 2 * d[2][2]) / 3.0;
+  const double tzz = mu * (4 * d[2][2] - 2 * d[0][0] - 2 * d[1][1]) / 3.0;
+  const double txy = mu * (d[0][1] + d[1][0]), txz = mu * (d[0][2] + d[2][0]), tyz = mu * (d[1][2] + d[2][1]);
+  double kj[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+    kj[c] = 0.5 * (M[id + (3 * DIR + c) * sz] * b.jac[id] + M[ie + (3 * DIR + c) * sz] * b.jac[ie]);
+  const double f1 = kj[0] * txx + kj[1] * txy + kj[2] * txz;
+  const double f2 = kj[0] * txy + kj[1] * tyy + kj[2] * tyz;
+  const double f3 = kj[0] * txz + kj[1] * tyz + kj[2] * tzz;
+  const double um = 0.5 * (b.bv[id + sz] + b.bv[ie + sz]);
+  const double vm = 0.5 * (b.bv[id + 2 * sz] + b.bv[ie + 2 * sz]);
+  const double wm = 0.5 * (b.bv[id + 3 * sz] + b.bv[ie + 3 * sz]);
+  constexpr double cp = kGamma * kRair / (kGamma - 1);
+  const double kc = mu / sc.prandtl * cp;
+  double *F = DIR == 0 ? b.fv : (DIR == 1 ? b.gv : b.hv);
+  F[id + sz] = f1;
+  F[id + 2 * sz] = f2;
+  F[id + 3 * sz] = f3;
+  F[id + 4 * sz] = um * f1 + vm * f2 + wm * f3 + kc * (kj[0] * d[3][0] + kj[1] * d[3][1] + kj[2] * d[3][2]);
+}
+
 // compute_viscous_flux_collocated<Air, ORDER> (ViscousScheme.cu:309-408): node fluxes fv/gv/hv[1..4].
 template<int ORDER>
 __global__ __launch_bounds__(256) void k_viscous_flux(DevBlock b, Scheme sc, TileGrid tg) {
@@ -794,7 +855,16 @@ __global__ __launch_bounds__(256) void k_update(DevBlock b, int rk, const double
   const long id = nid(b, i, j, k), sz = b.sz;
   double r[5];
   r[0] = 0;
-  if constexpr (ORDER > 0) {
+  if constexpr (ORDER == 2) {
+    // face fluxes (compute_dFv_dx / dGv_dy / dHv_dz, ViscousScheme.cu:75-128)
+#pragma unroll
+    for (int l = 1; l < 5; ++l) {
+      double s = b.fv[id + l * sz] - b.fv[id - 1 + l * sz];
+      s += b.gv[id + l * sz] - b.gv[id - b.d2 + l * sz];
+      if (b.dim == 3) s += b.hv[id + l * sz] - b.hv[id - b.d1 + l * sz];
+      r[l] = s;
+    }
+  } else if constexpr (ORDER > 0) {
     const double J = b.jac[id];
 #pragma unroll
     for (int l = 1; l < 5; ++l) {
@@ -1040,6 +1110,18 @@ hipError_t launch_boundary(const DevBlock &b, const DevFace &f, hipStream_t s) {
 hipError_t launch_viscous_flux(const DevBlock &b, const Scheme &sc, hipStream_t s) {
   const dim3 tpb(64, 4, 1);
   const TileGrid tg = node_tiles(b, 0);
+  if (sc.viscous_order == 2) {
+    // one launch per face direction, over the faces -1/2 .. m-1/2 along it
+    const TileGrid tx{(b.mx + 1 + 63) / 64, (b.my + 3) / 4, b.mz, 0};
+    const TileGrid ty{(b.mx + 63) / 64, (b.my + 1 + 3) / 4, b.mz, 1};
+    k_viscous_face<0><<<tile_blocks(tx), tpb, 0, s>>>(b, sc, tx);
+    k_viscous_face<1><<<tile_blocks(ty), tpb, 0, s>>>(b, sc, ty);
+    if (b.dim == 3) {
+      const TileGrid tz{(b.mx + 63) / 64, (b.my + 3) / 4, b.mz + 1, 2};
+      k_viscous_face<2><<<tile_blocks(tz), tpb, 0, s>>>(b, sc, tz);
+    }
+    return hipGetLastError();
+  }
   if (sc.viscous_order == 8) k_viscous_flux<8><<<tile_blocks(tg), tpb, 0, s>>>(b, sc, tg);
   else k_viscous_flux<4><<<tile_blocks(tg), tpb, 0, s>>>(b, sc, tg);
   return hipGetLastError();
@@ -1132,10 +1214,12 @@ hipError_t launch_update(const DevBlock &b, const Scheme &sc, int rk, const doub
   if (rhs_only) {
     if (o == 8) k_update<8, true><<<grid, tpb, 0, s>>>(b, rk, dt_dev, tg);
     else if (o == 4) k_update<4, true><<<grid, tpb, 0, s>>>(b, rk, dt_dev, tg);
+    else if (o == 2) k_update<2, true><<<grid, tpb, 0, s>>>(b, rk, dt_dev, tg);
     else k_update<0, true><<<grid, tpb, 0, s>>>(b, rk, dt_dev, tg);
   } else {
     if (o == 8) k_update<8, false><<<grid, tpb, 0, s>>>(b, rk, dt_dev, tg);
     else if (o == 4) k_update<4, false><<<grid, tpb, 0, s>>>(b, rk, dt_dev, tg);
+    else if (o == 2) k_update<2, false><<<grid, tpb, 0, s>>>(b, rk, dt_dev, tg);
     else k_update<0, false><<<grid, tpb, 0, s>>>(b, rk, dt_dev, tg);
   }
   return hipGetLastError();

@@ -29,7 +29,7 @@ typedef struct {
   int inviscid_scheme;   /* 1 LF, 2 Roe, 3 AUSM+, 4 HLLC; 51/52/71/72 LLF-split WENO5/7-cp/ch      */
                          /* (Parameter.cpp:218-305)                                                */
   int reconstruction;    /* 4 WENO5 / 5 WENO7 characteristic reconstruction (new pairing)         */
-  int viscous_order;     /* 0 inviscid, 4 (new), 8 collocated (ViscousScheme.cu:7-73)            */
+  int viscous_order;     /* 0 inviscid, 2 face-centred, 4 (new), 8 collocated (ViscousScheme.cu:7-73) */
   double prandtl;        /* prandtl_number, default 0.72                                          */
   double cfl;            /* cfl                                                                   */
   double entropy_fix_factor; /* default 0.125 (Parameter.cpp:380)                                 */

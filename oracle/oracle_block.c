@@ -493,6 +493,77 @@ static void viscous(orc_block *b) {
       }
 }
 
+/* Second-order face-centred viscous fluxes, compute_fv/gv/hv_2nd_order<Air> (ViscousScheme.cu:614-1010):
+ * on the face between nodes t and t+1 along dir, metrics and metric*Jacobian are the two nodes' averages,
+ * the derivative along dir is the difference of the two nodes, the transverse ones the average of the two
+ * nodes' central differences; the viscosity is the nodes' average.  Stored at node t (face t+1/2), for
+ * t = -1 .. m-1, then differenced into dq (compute_dFv_dx etc., ViscousScheme.cu:75-128).  2D blocks have
+ * no transverse z differences. */
+static void viscous_faces(orc_block *b, int dir, double *F) {
+  const long s = b->sz;
+  const double cp = kGamma * (kRu / kMwAir) / (kGamma - 1);
+  const long st[3] = {1, b->d2, b->d1};
+  const int ext[3] = {b->mx, b->my, b->mz};
+  int lo[3] = {0, 0, 0};
+  lo[dir] = -1;
+#pragma omp parallel for collapse(2)
+  for (int k = lo[2]; k < ext[2]; ++k)
+    for (int j = lo[1]; j < ext[1]; ++j)
+      for (int i = lo[0]; i < ext[0]; ++i) {
+        const long id = ID(b, i, j, k), ie = id + st[dir];
+        const double *M = b->met;
+        double m[9];
+        for (int c = 0; c < 9; ++c) m[c] = 0.5 * (M[id + c * s] + M[ie + c * s]);
+        const int comp[4] = {1, 2, 3, 5};
+        double g[4][3];
+        for (int q = 0; q < 4; ++q) {
+          const double *f = b->bv + comp[q] * s;
+          for (int a = 0; a < 3; ++a) {
+            if (a == dir) g[q][a] = f[ie] - f[id];
+            else if (a == 2 && b->dim == 2) g[q][a] = 0;
+            else g[q][a] = 0.25 * (f[id + st[a]] - f[id - st[a]] + f[ie + st[a]] - f[ie - st[a]]);
+          }
+        }
+        double d[4][3]; /* d/dx, d/dy, d/dz of u, v, w, T */
+        for (int q = 0; q < 4; ++q)
+          for (int c = 0; c < 3; ++c) d[q][c] = g[q][0] * m[c] + g[q][1] * m[3 + c] + g[q][2] * m[6 + c];
+        const double mu = 0.5 * (b->mul[id] + b->mul[ie]);
+        const double txx = mu * (4 * d[0][0] - 2 * d[1][1] - 2 * d[2][2]) / 3.0;
+        const double tyy = mu * (4 * d[1][1] - 2 * d[0][0] - 2 * d[2][2]) / 3.0;
+        const double tzz = mu * (4 * d[2][2] - 2 * d[0][0] - 2 * d[1][1]) / 3.0;
+        const double txy = mu * (d[0][1] + d[1][0]), txz = mu * (d[0][2] + d[2][0]), tyz = mu * (d[1][2] + d[2][1]);
+        double kj[3];
+        for (int c = 0; c < 3; ++c)
+          kj[c] = 0.5 * (M[id + (3 * dir + c) * s] * b->jac[id] + M[ie + (3 * dir + c) * s] * b->jac[ie]);
+        const double f1 = kj[0] * txx + kj[1] * txy + kj[2] * txz;
+        const double f2 = kj[0] * txy + kj[1] * tyy + kj[2] * tyz;
+        const double f3 = kj[0] * txz + kj[1] * tyz + kj[2] * tzz;
+        const double um = 0.5 * (b->bv[id + s] + b->bv[ie + s]);
+        const double vm = 0.5 * (b->bv[id + 2 * s] + b->bv[ie + 2 * s]);
+        const double wm = 0.5 * (b->bv[id + 3 * s] + b->bv[ie + 3 * s]);
+        const double kc = mu / b->p.prandtl * cp;
+        F[id + s] = f1;
+        F[id + 2 * s] = f2;
+        F[id + 3 * s] = f3;
+        F[id + 4 * s] = um * f1 + vm * f2 + wm * f3 + kc * (kj[0] * d[3][0] + kj[1] * d[3][1] + kj[2] * d[3][2]);
+      }
+}
+
+static void viscous2(orc_block *b) {
+  double *F[3] = {b->fv, b->gv, b->hv};
+  const long st[3] = {1, b->d2, b->d1};
+  for (int dir = 0; dir < b->dim; ++dir) {
+    viscous_faces(b, dir, F[dir]);
+#pragma omp parallel for collapse(2)
+    for (int k = 0; k < b->mz; ++k)
+      for (int j = 0; j < b->my; ++j)
+        for (int i = 0; i < b->mx; ++i) {
+          const long id = ID(b, i, j, k), iq = IDQ(b, i, j, k);
+          for (int l = 1; l < 5; ++l) b->dq[iq + l * b->n] += F[dir][id + l * b->sz] - F[dir][id - st[dir] + l * b->sz];
+        }
+  }
+}
+
 /* compute_entropy_fix_delta<Air> (InviscidScheme.cu:269-304), over the interior plus one layer */
 static void entropy_fix(orc_block *b) {
   const long s = b->sz;
@@ -577,7 +648,8 @@ static void inviscid_dir(orc_block *b, int dir) {
 
 void orc_block_rhs(orc_block *b) {
   memset(b->dq, 0, sizeof(double) * 5 * b->n);
-  if (b->p.viscous_order > 0) viscous(b);
+  if (b->p.viscous_order == 2) viscous2(b);
+  else if (b->p.viscous_order > 0) viscous(b);
   if (b->p.inviscid_scheme == 2) entropy_fix(b);
   inviscid_dir(b, 0);
   inviscid_dir(b, 1);

@@ -19,7 +19,7 @@ from parity import interior, rel_err
 pytestmark = pytest.mark.gpu
 
 N, SPLIT = 13, 6
-SCHEMES = [(5, 3, 4), (4, 72, 8)]  # (reconstruction, inviscid_scheme, viscous_order)
+SCHEMES = [(5, 3, 4), (4, 72, 8), (5, 4, 2)]  # (reconstruction, inviscid_scheme, viscous_order)
 
 
 def _pair(layout, scheme, dt=None):
@@ -55,6 +55,12 @@ def test_two_blocks_reproduce_one(layout, scheme):
     S1, S2, place = _pair(layout, scheme)
     assert S2.n_blocks == 2
     tol = 1e-11
+    # The face-centred viscous scheme (order 2) reads the ghost nodes where the block-to-block face meets the
+    # periodic faces (transverse differences on the block's last face).  Neither exchange fills those edge
+    # nodes (the reference leaves them out on purpose, Mesh.cpp:119-169), so they keep their initial values
+    # in the two-block run while the single block refreshes them as periodic ghosts: the runs agree on the
+    # first residual and drift apart slowly afterwards.
+    tol_steps = 1e-8 if scheme[2] == 2 else tol
     # initial state (ghosts filled across the block-to-block face) and the residual on it
     assert np.all(rel_err(_glued(S2, place, corefl.CV, 1e-14), _interior(S1, corefl.CV)) <= 1e-14)
     S1.rhs()
@@ -71,7 +77,7 @@ def test_two_blocks_reproduce_one(layout, scheme):
         S2.rk3_step(d1)
     err = rel_err(_glued(S2, place, corefl.CV, tol), _interior(S1, corefl.CV))
     print("3-step rel err", err)
-    assert np.all(err <= tol), err
+    assert np.all(err <= tol_steps), err
     S1.close()
     S2.close()
 

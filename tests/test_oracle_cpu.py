@@ -266,7 +266,7 @@ def tgv_ke_rate(n, visc, reynolds):
     return rate / ke, L, V0, rho0
 
 
-@pytest.mark.parametrize("visc", [4, 8])
+@pytest.mark.parametrize("visc", [2, 4, 8])
 def test_tgv_viscous_decay_rate(visc):
     """d(KE)/dt / KE from the viscous terms equals -6 nu / L^2 for the Taylor-Green field at t = 0."""
     n, Re = 25, 100.0
@@ -275,3 +275,79 @@ def test_tgv_viscous_decay_rate(visc):
     nu = cases.sutherland(300.0) / rho0
     expected = -6 * nu / L ** 2
     assert (r_vis - r_inv) == pytest.approx(expected, rel=0.02)
+
+
+@pytest.mark.parametrize("visc", [2, 4, 8])
+def test_viscous_terms_exact_for_linear_velocity(visc):
+    """On a skewed affine grid a linear velocity field at uniform T has a uniform stress tau, so the viscous
+    momentum residual vanishes and the energy residual is J (tau : grad u): exact for every viscous order
+    (the face scheme's transverse averages and the collocated derivatives are exact for linear data)."""
+    n, g = 11, 4
+    A = np.array([[1.0, 0.2, 0.1], [0.1, 1.0, 0.3], [0.05, 0.1, 1.0]]) * 1e-3
+    s = np.arange(n, dtype=float)
+    xi, eta, zeta = np.meshgrid(s, s, s, indexing="ij")
+    X = A[0, 0] * xi + A[0, 1] * eta + A[0, 2] * zeta
+    Y = A[1, 0] * xi + A[1, 1] * eta + A[1, 2] * zeta
+    Z = A[2, 0] * xi + A[2, 1] * eta + A[2, 2] * zeta
+    G = np.array([[3.0, -1.0, 2.0], [0.5, -2.0, 1.0], [1.5, 0.7, 1.0]]) * 1e3  # du_i/dx_j
+    faces = cases.box_faces(n, n, n, 6)
+
+    def residual(order):
+        B = O.Block(O.params(inviscid=3, recon=5, visc=order, eps=1e-30), X, Y, Z, g, faces)
+        sp = np.arange(-g, n + g, dtype=float)
+        a, b, c = np.meshgrid(sp, sp, sp, indexing="ij")
+        x = A[0, 0] * a + A[0, 1] * b + A[0, 2] * c
+        y = A[1, 0] * a + A[1, 1] * b + A[1, 2] * c
+        z = A[2, 0] * a + A[2, 1] * b + A[2, 2] * c
+        bv = np.zeros((6,) + a.shape)
+        bv[0] = 1.2
+        for q in range(3):
+            bv[1 + q] = 20.0 + G[q, 0] * x + G[q, 1] * y + G[q, 2] * z
+        bv[4] = 1e5
+        bv[5] = 300.0
+        B.set_bv(bv.transpose(0, 3, 2, 1).ravel())
+        return B.rhs().reshape(5, n, n, n), B.jac.reshape((n + 2 * g,) * 3)[g:-g, g:-g, g:-g]
+
+    r_v, jac = residual(visc)
+    r_0, _ = residual(0)
+    d = (r_v - r_0)[:, 2:-2, 2:-2, 2:-2]  # (l, k, j, i); nodes whose stencils stay inside the block
+    mu = cases.sutherland(300.0)
+    tau = mu * (G + G.T - 2.0 / 3.0 * np.trace(G) * np.eye(3))
+    expected = np.sum(tau * G) * jac[2:-2, 2:-2, 2:-2]
+    scale = np.abs(expected).max()
+    assert np.abs(d[1:4]).max() <= 1e-9 * scale
+    np.testing.assert_allclose(d[4], expected, rtol=1e-9, atol=1e-9 * scale)
+
+
+@pytest.mark.parametrize("visc", [2, 4, 8])
+def test_heat_conduction_linear_temperature(visc):
+    """At rest with a linear temperature field on a skewed affine grid, the viscous momentum residual vanishes
+    and the energy residual approaches J d/dx.(k(T) grad T) = J k'(T) |grad T|^2 (k = mu(T) cp / Pr, Sutherland
+    mu): second order in the face spacing for every viscous order (the conductivity varies along the field)."""
+    n, g = 11, 4
+    A = np.array([[1.0, 0.2, 0.1], [0.1, 1.0, 0.3], [0.05, 0.1, 1.0]]) * 1e-3
+    s = np.arange(n, dtype=float)
+    xi, eta, zeta = np.meshgrid(s, s, s, indexing="ij")
+    X = A[0, 0] * xi + A[0, 1] * eta + A[0, 2] * zeta
+    Y = A[1, 0] * xi + A[1, 1] * eta + A[1, 2] * zeta
+    Z = A[2, 0] * xi + A[2, 1] * eta + A[2, 2] * zeta
+    gT = np.array([2.0, -1.0, 1.5]) * 1e3  # K/m
+    cp = cases.GAMMA * cases.R_AIR / (cases.GAMMA - 1)
+    B = O.Block(O.params(inviscid=3, recon=5, visc=visc, eps=1e-30), X, Y, Z, g, cases.box_faces(n, n, n, 6))
+    sp = np.arange(-g, n + g, dtype=float)
+    a, b, c = np.meshgrid(sp, sp, sp, indexing="ij")
+    T = 300.0 + gT[0] * (A[0, 0] * a + A[0, 1] * b + A[0, 2] * c) + gT[1] * (A[1, 0] * a + A[1, 1] * b + A[1, 2] * c) \
+        + gT[2] * (A[2, 0] * a + A[2, 1] * b + A[2, 2] * c)
+    bv = np.zeros((6,) + a.shape)
+    bv[4] = 1e5
+    bv[5] = T
+    bv[0] = bv[4] / (cases.R_AIR * T)
+    B.set_bv(bv.transpose(0, 3, 2, 1).ravel())
+    r = B.rhs().reshape(5, n, n, n)[:, 2:-2, 2:-2, 2:-2]
+    jac = B.jac.reshape((n + 2 * g,) * 3)[g:-g, g:-g, g:-g][2:-2, 2:-2, 2:-2]
+    Ti = T[g:-g, g:-g, g:-g].transpose(2, 1, 0)[2:-2, 2:-2, 2:-2]
+    dT = 1e-3
+    dk = (cases.sutherland(Ti + dT) - cases.sutherland(Ti - dT)) / (2 * dT) * cp / 0.72
+    expected = jac * dk * (gT @ gT)
+    assert np.abs(r[1:4]).max() <= 1e-9 * np.abs(expected).max()
+    np.testing.assert_allclose(r[4], expected, rtol=2e-4)

@@ -17,7 +17,8 @@ pytestmark = pytest.mark.gpu
 # the reconstruction setting
 SOD = [(4, 4), (5, 3), (4, 2), (5, 1), (5, 4), (4, 3), (4, 51), (4, 52), (4, 71), (4, 72)]
 # (reconstruction, inviscid, viscous_order)
-TGV = [(5, 3, 4), (4, 2, 8), (5, 4, 8), (4, 1, 4), (4, 51, 4), (4, 52, 8), (4, 71, 0), (4, 72, 8)]
+TGV = [(5, 3, 4), (4, 2, 8), (5, 4, 8), (4, 1, 4), (4, 51, 4), (4, 52, 8), (4, 71, 0), (4, 72, 8),
+       (5, 3, 2), (4, 72, 2)]
 
 
 def _sod(recon, inv):
