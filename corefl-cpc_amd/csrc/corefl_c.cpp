@@ -128,6 +128,7 @@ struct corefl_ctx {
   // mixtures
   int ns = 0;
   bool strang = false;          // temporal_scheme 5: chemistry dt/2, SSP-RK3, chemistry dt/2
+  int wu_method = -1;           // temporal_scheme 4 (Wu's splitting): chem_advance_method 0 or 3
   corefl_chem *chem = nullptr;  // mechanism, device tables and the cell batch of the chemistry sub-steps
   corefl_surrogate *surrogate = nullptr; // not owned; when set, sub-steps use it with the exact fallback
   double chem_rtol = 1e-6, chem_atol = 1e-8;
@@ -247,8 +248,41 @@ struct corefl_ctx {
     HIPCHK(corefl::launch_dt_reduce(parts, n_parts, dt_dev, time_dev, t_end, clip, s));
   }
 
+  void mix_fluxes(Block &b, hipStream_t s) {
+    if (scheme.inviscid_scheme == 2) HIPCHK(corefl::launch_mix_entropy_fix(b.dev, b.mix, scheme, s));
+    for (int d = 0; d < b.dev.dim; ++d) HIPCHK(corefl::launch_mix_inviscid_flux(b.dev, b.mix, scheme, d, s));
+  }
+
+  // wu_splitting, one step (StrangSplitting.cu:683-861): the transport residual Cn at U^n (compute_cn) kept in
+  // dq; the per-cell reaction sub-step with Cn frozen (wu_reaction_subStep); then three SSP-RK3 stages over
+  // dt/2 of R(U) - Cn (wu_RK_subStep).  Boundary conditions and block exchanges after each part.
+  void wu_step(hipStream_t s) {
+    for (auto &b : blocks) {
+      mix_fluxes(b, s);
+      HIPCHK(corefl::launch_mix_update(b.dev, b.mix, 0, dt_dev, true, s));
+    }
+    for (auto &b : blocks) {
+      HIPCHK(corefl::launch_mix_wu_reaction(b.dev, b.mix, dt_dev, wu_method, s));
+      apply_bcs(b, s);
+    }
+    communicate(s);
+    for (int rk = 0; rk < 3; ++rk) {
+      for (auto &b : blocks) {
+        mix_fluxes(b, s);
+        HIPCHK(corefl::launch_mix_wu_update(b.dev, b.mix, rk, dt_dev, s));
+        apply_bcs(b, s);
+      }
+      communicate(s);
+    }
+  }
+
   void enqueue_step(hipStream_t s, bool compute_dt) {
     if (compute_dt) enqueue_dt(s, 1);
+    if (wu_method >= 0) {
+      wu_step(s);
+      HIPCHK(corefl::launch_time_advance(time_dev, dt_dev, s));
+      return;
+    }
     if (strang) chem_substep(0.5, s);
     for (int rk = 0; rk < 3; ++rk) stage(rk, s);
     if (strang) chem_substep(0.5, s);
@@ -365,8 +399,17 @@ void build(corefl_ctx *ctx, std::vector<corefl::HostBlock> blocks) {
                                "(component-wise WENO5)");
     if (inv < 2 || inv > 4) throw std::runtime_error("with species, inviscid_scheme must be 2 (Roe), 3 (AUSM+) or 4 (HLLC)");
     if (vo != 0) throw std::runtime_error("viscous mixtures are not on this path (viscous_order must be 0)");
-    if (reaction && temporal != 3 && temporal != 5)
-      throw std::runtime_error("reacting flow: temporal_scheme 3 (explicit source) or 5 (Strang splitting)");
+    if (reaction && temporal != 3 && temporal != 4 && temporal != 5)
+      throw std::runtime_error("reacting flow: temporal_scheme 3 (explicit source), 4 (Wu's splitting) or 5 (Strang "
+                               "splitting)");
+    if (reaction && temporal == 4) {
+      // wu_zeroDReaction (StrangSplitting.cu:43-225): 0 forward Euler, 3 SSP-RK3 stages; 1 (EPI) and 2 (HCST,
+      // hydrogen only) are not on this path
+      const int cam = P.get_int("chem_advance_method");
+      if (cam != 0 && cam != 3)
+        throw std::runtime_error("with temporal_scheme 4, chem_advance_method must be 0 (explicit) or 3 (SSP-RK3)");
+      ctx->wu_method = cam;
+    }
     if (reaction && temporal == 3 && P.get_int("chemSrcMethod") != 0)
       throw std::runtime_error("only chemSrcMethod = 0 (explicit source) is on this path with temporal_scheme 3");
     if (!reaction && temporal != 3) throw std::runtime_error("temporal_scheme must be 3 (SSP-RK3)");
@@ -492,6 +535,7 @@ void build(corefl_ctx *ctx, std::vector<corefl::HostBlock> blocks) {
       B.mix.snd = ctx->alloc(sz);
       B.mix.mech = ctx->chem->mech_dev;
       B.mix.explicit_source = reaction && temporal == 3;
+      if (ctx->wu_method >= 0) B.mix.tscale = ctx->alloc(d.n); // reaction time scale, zero at the start
       if (B.mix.explicit_source) {
         B.mix.tscale = ctx->alloc(d.n);
         const std::vector<double> big(d.n, 1e+6);

@@ -34,6 +34,11 @@ hipError_t launch_mix_entropy_fix(const DevBlock &b, const MixArrays &x, const S
 hipError_t launch_mix_inviscid_flux(const DevBlock &b, const MixArrays &x, const Scheme &sc, int dir, hipStream_t s);
 hipError_t launch_mix_update(const DevBlock &b, const MixArrays &x, int rk, const double *dt_dev, bool rhs_only,
                              hipStream_t s);
+// Wu's splitting (temporal_scheme 4): the reaction sub-step with the frozen transport residual held in b.dq,
+// and the correction stage over dt/2 with R(U) - b.dq (StrangSplitting.cu:373-421, 504-634).
+hipError_t launch_mix_wu_reaction(const DevBlock &b, const MixArrays &x, const double *dt_dev, int method,
+                                  hipStream_t s);
+hipError_t launch_mix_wu_update(const DevBlock &b, const MixArrays &x, int rk, const double *dt_dev, hipStream_t s);
 hipError_t launch_mix_local_dt(const DevBlock &b, const MixArrays &x, const Scheme &sc, double *part, hipStream_t s);
 // Interior cells -> chemistry batch (rho, e, T, rho Y; n = mx*my*mz cells, i fastest) and back.
 hipError_t launch_mix_chem_gather(const DevBlock &b, const MixArrays &x, const ChemBatch &c, hipStream_t s);

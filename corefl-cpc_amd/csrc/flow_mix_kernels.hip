@@ -494,8 +494,51 @@ __constant__ double kRkC[3] = {1.0, 0.25, 2.0 / 3.0};
 
 // dq = -sum_dir (F(i+1/2) - F(i-1/2)) and update_cv_and_bv_rk<Mixture> (RK.cuh:224-280) with
 // compute_temperature_and_pressure (FieldOperation.cu:4-45).  Stage 0 saves U^n into qn.
+// Species clipping and renormalisation of q, then the Newton temperature at the internal energy of q
+// (update_cv_and_bv_rk<Mixture> tail, RK.cuh:224-280; compute_temperature_and_pressure,
+// FieldOperation.cu:4-45); returns T, y the mass fractions, R the mixture gas constant.
+__device__ __forceinline__ double mix_state(const ChemMech &m, double *q, double t0, double *y, double &R) {
+  const double di = 1.0 / q[0];
+  const double u = q[1] * di, v = q[2] * di, w = q[3] * di;
+  double sum = 0;
+#pragma unroll
+  for (int l = 0; l < NS; ++l) {
+    if (q[5 + l] < 0) q[5 + l] = 0;
+    else sum += q[5 + l];
+  }
+  sum = 1.0 / sum;
+  const double f = q[0] * sum;
+  R = 0;
+#pragma unroll
+  for (int l = 0; l < NS; ++l) {
+    y[l] = q[5 + l] * sum;
+    q[5 + l] *= f;
+    R += y[l] * kRu * m.imw[l];
+  }
+  // Newton iteration for T at fixed e, |1 - T1/T| < 1e-5, at most 1000 iterations
+  const double e = q[4] * di - 0.5 * (u * u + v * v + w * w);
+  double t = t0, err = 1;
+  for (int it = 0; it < 1000 && err > 1e-5; ++it) {
+    double h[NS], cp[NS];
+    chemdev::h_cp<NS>(m, t, h, cp);
+    double cpt = 0, hh = 0;
+#pragma unroll
+    for (int l = 0; l < NS; ++l) {
+      cpt += cp[l] * y[l];
+      hh += h[l] * y[l];
+    }
+    const double t1 = t - (hh - R * t - e) / (cpt - R);
+    err = fabs(1 - t1 / t);
+    t = t1;
+  }
+  return t;
+}
+
+// cn: the frozen transport residual of Wu's splitting (update_cv_and_bv_rk_wu, StrangSplitting.cu:504-562:
+// the stage advances with R(U) - cn over dts * dt), or nullptr for the plain stage (dts = 1).
 template<bool RHS_ONLY>
-__global__ __launch_bounds__(256) void k_mix_update(DevBlock b, MixArrays x, int rk, const double *__restrict__ dtp) {
+__global__ __launch_bounds__(256) void k_mix_update(DevBlock b, MixArrays x, int rk, const double *__restrict__ dtp,
+                                                    const double *__restrict__ cn, double dts) {
   const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
   if (i >= b.mx || j >= b.my) return;
   const long id = nid(b, i, j, k), sz = b.sz;
@@ -529,7 +572,11 @@ __global__ __launch_bounds__(256) void k_mix_update(DevBlock b, MixArrays x, int
     for (int l = 0; l < NV; ++l) b.dq[iq + l * b.n] = r[l];
     return;
   }
-  const double dj = dtp[0] / b.jac[id];
+  if (cn) {
+#pragma unroll
+    for (int l = 0; l < NV; ++l) r[l] -= cn[iq + l * b.n];
+  }
+  const double dj = dtp[0] * dts / b.jac[id];
   double q[NV];
 #pragma unroll
   for (int l = 0; l < NV; ++l) {
@@ -544,41 +591,11 @@ __global__ __launch_bounds__(256) void k_mix_update(DevBlock b, MixArrays x, int
   if (b.mz == 1) q[3] = 0;
   const double di = 1.0 / q[0];
   const double u = q[1] * di, v = q[2] * di, w = q[3] * di;
-  double sum = 0;
-#pragma unroll
-  for (int l = 0; l < NS; ++l) {
-    if (q[5 + l] < 0) q[5 + l] = 0;
-    else sum += q[5 + l];
-  }
-  sum = 1.0 / sum;
-  const double f = q[0] * sum;
-  double y[NS];
+  double y[NS], R;
   const ChemMech &m = *x.mech;
-  double R = 0;
-#pragma unroll
-  for (int l = 0; l < NS; ++l) {
-    y[l] = q[5 + l] * sum;
-    q[5 + l] *= f;
-    R += y[l] * kRu * m.imw[l];
-  }
+  const double t = mix_state(m, q, b.bv[id + 5 * sz], y, R);
 #pragma unroll
   for (int l = 0; l < NV; ++l) b.cv[id + l * sz] = q[l];
-  // Newton iteration for T at fixed e, |1 - T1/T| < 1e-5, at most 1000 iterations
-  const double e = q[4] * di - 0.5 * (u * u + v * v + w * w);
-  double t = b.bv[id + 5 * sz], err = 1;
-  for (int it = 0; it < 1000 && err > 1e-5; ++it) {
-    double h[NS], cp[NS];
-    chemdev::h_cp<NS>(m, t, h, cp);
-    double cpt = 0, hh = 0;
-#pragma unroll
-    for (int l = 0; l < NS; ++l) {
-      cpt += cp[l] * y[l];
-      hh += h[l] * y[l];
-    }
-    const double t1 = t - (hh - R * t - e) / (cpt - R);
-    err = fabs(1 - t1 / t);
-    t = t1;
-  }
   b.bv[id] = q[0];
   b.bv[id + sz] = u;
   b.bv[id + 2 * sz] = v;
@@ -724,11 +741,102 @@ hipError_t launch_mix_inviscid_flux(const DevBlock &b, const MixArrays &x, const
   return hipGetLastError();
 }
 
+// Wu's splitting reaction sub-step (wu_reaction_step, StrangSplitting.cu:373-421; wu_zeroDReaction, 43-225):
+// per interior cell, n_sub = dt / tau steps of dt / (n_sub + 1) and a last one to dt, tau the cell's reaction
+// time scale from its previous sub-step (0.2 dt at the start); each advances the conservative state by the
+// chemical source of its start state and the frozen transport residual cn (the dq array), by forward Euler
+// (method 0) or by three SSP-RK3 stages with that same source (method 3), then clips and renormalises the
+// species and solves for T.  n_sub is capped at 10^6.
+__global__ __launch_bounds__(256) void k_mix_wu_reaction(DevBlock b, MixArrays x, const double *__restrict__ dtp,
+                                                         int method) {
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+  if (i >= b.mx || j >= b.my) return;
+  const long id = nid(b, i, j, k), sz = b.sz;
+  const long iq = ((long)k * b.my + j) * b.mx + i;
+  const ChemMech &m = *x.mech;
+  const double dt = dtp[0];
+  const double iJ = 1.0 / b.jac[id];
+  double cn[NV], q[NV], y[NS];
+#pragma unroll
+  for (int l = 0; l < NV; ++l) {
+    cn[l] = b.dq[iq + l * b.n];
+    q[l] = b.cv[id + l * sz];
+  }
+  double t = b.bv[id + 5 * sz], R = 0;
+  double ts = x.tscale[iq];
+  const double tau = ts < 1e-15 ? 0.2 * dt : ts;
+  const double ratio = dt / tau;
+  const int n_sub = ratio > 1e6 ? 1000000 : (int)ratio;
+  const double h0 = dt / (n_sub + 1);
+  for (int n = 0; n <= n_sub; ++n) {
+    const double h = n < n_sub ? h0 : dt - h0 * n_sub;
+    if (n == n_sub && !(fabs(h) > 1e-12)) break;
+    double w[NS], wd[NS];
+    chemdev::source<NS, false>(m, t, q + 5, w, nullptr, wd);
+    if (method == 3) {
+      double qn[NV];
+#pragma unroll
+      for (int l = 0; l < NV; ++l) qn[l] = q[l];
+      for (int st = 0; st < 3; ++st) {
+#pragma unroll
+        for (int l = 0; l < 5; ++l) q[l] = kRkA[st] * qn[l] + kRkB[st] * q[l] + kRkC[st] * h * iJ * cn[l];
+        ts = 1e+6;
+#pragma unroll
+        for (int l = 0; l < NS; ++l) {
+          if (q[5 + l] > 1e-25 && fabs(wd[l]) > 1e-25) ts = fmin(ts, fabs(q[5 + l] / wd[l]));
+          q[5 + l] = kRkA[st] * qn[5 + l] + kRkB[st] * q[5 + l] + kRkC[st] * h * (w[l] + cn[5 + l] * iJ);
+        }
+        t = mix_state(m, q, t, y, R);
+      }
+    } else {
+#pragma unroll
+      for (int l = 0; l < 5; ++l) q[l] += cn[l] * h * iJ;
+      ts = 1e+6;
+#pragma unroll
+      for (int l = 0; l < NS; ++l) {
+        if (q[5 + l] > 1e-25 && fabs(wd[l]) > 1e-25) ts = fmin(ts, fabs(q[5 + l] / wd[l]));
+        q[5 + l] += (w[l] + cn[5 + l] * iJ) * h;
+      }
+      t = mix_state(m, q, t, y, R);
+    }
+  }
+  x.tscale[iq] = ts;
+#pragma unroll
+  for (int l = 0; l < NV; ++l) b.cv[id + l * sz] = q[l];
+  const double di = 1.0 / q[0];
+  b.bv[id] = q[0];
+  b.bv[id + sz] = q[1] * di;
+  b.bv[id + 2 * sz] = q[2] * di;
+  b.bv[id + 3 * sz] = q[3] * di;
+  b.bv[id + 4 * sz] = q[0] * t * R;
+  b.bv[id + 5 * sz] = t;
+#pragma unroll
+  for (int l = 0; l < NS; ++l) x.sv[id + l * sz] = y[l];
+  double hh[NS], cp[NS], cpt = 0;
+  chemdev::h_cp<NS>(m, t, hh, cp);
+#pragma unroll
+  for (int l = 0; l < NS; ++l) cpt += cp[l] * y[l];
+  x.snd[id] = sqrt(cpt / (cpt - R) * R * t);
+}
+
+hipError_t launch_mix_wu_reaction(const DevBlock &b, const MixArrays &x, const double *dt_dev, int method,
+                                  hipStream_t s) {
+  if (x.ns != NS) return hipErrorNotSupported;
+  k_mix_wu_reaction<<<node_grid(b), dim3(64, 4, 1), 0, s>>>(b, x, dt_dev, method);
+  return hipGetLastError();
+}
+
+hipError_t launch_mix_wu_update(const DevBlock &b, const MixArrays &x, int rk, const double *dt_dev, hipStream_t s) {
+  if (x.ns != NS) return hipErrorNotSupported;
+  k_mix_update<false><<<node_grid(b), dim3(64, 4, 1), 0, s>>>(b, x, rk, dt_dev, b.dq, 0.5);
+  return hipGetLastError();
+}
+
 hipError_t launch_mix_update(const DevBlock &b, const MixArrays &x, int rk, const double *dt_dev, bool rhs_only,
                              hipStream_t s) {
   if (x.ns != NS) return hipErrorNotSupported;
-  if (rhs_only) k_mix_update<true><<<node_grid(b), dim3(64, 4, 1), 0, s>>>(b, x, rk, dt_dev);
-  else k_mix_update<false><<<node_grid(b), dim3(64, 4, 1), 0, s>>>(b, x, rk, dt_dev);
+  if (rhs_only) k_mix_update<true><<<node_grid(b), dim3(64, 4, 1), 0, s>>>(b, x, rk, dt_dev, nullptr, 1.0);
+  else k_mix_update<false><<<node_grid(b), dim3(64, 4, 1), 0, s>>>(b, x, rk, dt_dev, nullptr, 1.0);
   return hipGetLastError();
 }
 

@@ -29,6 +29,7 @@ Parameter::Parameter() {
   ints["species"] = 0;
   ints["reaction"] = 0;
   ints["chemSrcMethod"] = 0;
+  ints["chem_advance_method"] = 0; // Wu's splitting reaction sub-steps (Parameter.cpp:431)
   strings["mechanism_file"] = "chemistry/mechanism.inp";
   strings["therm_file"] = "chemistry/therm.dat";
   reals["chemistry_rtol"] = 1e-6; // ROK4E tolerances of the split chemistry sub-steps (new)

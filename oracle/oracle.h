@@ -166,6 +166,8 @@ double orc_mix_cfl_dt(orc_mix *b);
 void orc_mix_rk3_step(orc_mix *b, double dt);
 int orc_mix_chem_substep(orc_mix *b, double dt, double rtol, double atol, int max_attempts);
 int orc_mix_strang_step(orc_mix *b, double dt, double rtol, double atol, int max_attempts);
+/* Wu's splitting (temporal_scheme 4), one step: chem_advance_method 0 (forward Euler) or 3 (SSP-RK3 stages). */
+void orc_mix_wu_step(orc_mix *b, double dt, int method);
 /* One interface flux of the mixture Riemann solver params->inviscid_scheme from primitive states
  * (rho,u,v,w,p,Y_1..Y_ns), after the reference post-processing for E and gamma. */
 void orc_mix_point_flux(const orc_params *p, const orc_mech *m, const double *pl, const double *pr,

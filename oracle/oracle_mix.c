@@ -34,6 +34,7 @@ struct orc_mix {
   int n_face;
   int explicit_source; /* chemical source in the residual (reaction = 1, temporal_scheme 3) */
   double *cv, *qn, *bv, *sv, *c, *efd, *dq, *jac, *met, *hstep, *tscale;
+  double *cn, *wu_ts; /* Wu's splitting: frozen transport residual, reaction time scale (starts at zero) */
 };
 
 #define ID(b, i, j, k) ((long)((k) + (b)->g) * (b)->d1 + (long)((j) + (b)->g) * (b)->d2 + ((i) + (b)->g))
@@ -82,6 +83,8 @@ orc_mix *orc_mix_create(const orc_params *p, const orc_mech *m, int mx, int my, 
   b->efd = calloc((size_t)b->sz, sizeof(double));
   b->dq = calloc((size_t)b->nv * b->n, sizeof(double));
   b->hstep = calloc((size_t)b->n, sizeof(double));
+  b->cn = calloc((size_t)b->nv * b->n, sizeof(double));
+  b->wu_ts = calloc((size_t)b->n, sizeof(double));
   b->tscale = malloc((size_t)b->n * sizeof(double));
   for (long q = 0; q < b->n; ++q) b->tscale[q] = 1e+6;
   b->jac = malloc((size_t)b->sz * sizeof(double));
@@ -94,7 +97,7 @@ orc_mix *orc_mix_create(const orc_params *p, const orc_mech *m, int mx, int my, 
 void orc_mix_destroy(orc_mix *b) {
   if (!b) return;
   free(b->cv); free(b->qn); free(b->bv); free(b->sv); free(b->c); free(b->efd); free(b->dq); free(b->hstep);
-  free(b->tscale);
+  free(b->tscale); free(b->cn); free(b->wu_ts);
   free(b->jac); free(b->met);
   free(b);
 }
@@ -752,6 +755,107 @@ int orc_mix_strang_step(orc_mix *b, double dt, double rtol, double atol, int max
   orc_mix_rk3_step(b, dt);
   f += orc_mix_chem_substep(b, 0.5 * dt, rtol, atol, max_attempts);
   return f;
+}
+
+/* Species clipping and renormalisation, then temperature and pressure, of node id after cv changed
+ * (the tail of wu_zeroDReaction, StrangSplitting.cu:108-132 and 196-224). */
+static void wu_node_state(orc_mix *b, long id) {
+  const long s = b->sz;
+  double *cv = b->cv, *bv = b->bv;
+  bv[id] = cv[id];
+  const double di = 1.0 / cv[id];
+  bv[id + s] = cv[id + s] * di;
+  bv[id + 2 * s] = cv[id + 2 * s] * di;
+  bv[id + 3 * s] = cv[id + 3 * s] * di;
+  double sum = 0;
+  for (int l = 0; l < b->ns; ++l) {
+    if (cv[id + (5 + l) * s] < 0) cv[id + (5 + l) * s] = 0;
+    else sum += cv[id + (5 + l) * s];
+  }
+  sum = 1.0 / sum;
+  const double f = cv[id] * sum;
+  for (int l = 0; l < b->ns; ++l) {
+    b->sv[id + l * s] = cv[id + (5 + l) * s] * sum;
+    cv[id + (5 + l) * s] *= f;
+  }
+  temperature_pressure(b, id);
+}
+
+/* wu_zeroDReaction (StrangSplitting.cu:43-225) over h at node id: the chemical source at the current state
+ * and the frozen transport cn advance cv by forward Euler (chem_advance_method 0) or by the three SSP-RK3
+ * stages with that same source (method 3, whose rates are not re-evaluated between stages); returns the
+ * reaction time scale min_k rho Y_k / omega_d,k. */
+static double wu_zero_d(orc_mix *b, long id, const double *cn, double h, int method) {
+  const long s = b->sz;
+  const int ns = b->ns, nv = b->nv;
+  double *cv = b->cv;
+  const double iJ = 1.0 / b->jac[id];
+  double qn[NV], ry[MS], q1[ORC_MAX_REAC], q2[ORC_MAX_REAC], wd[MS], w[MS];
+  for (int l = 0; l < nv; ++l) qn[l] = cv[id + l * s];
+  for (int l = 0; l < ns; ++l) ry[l] = cv[id + (5 + l) * s];
+  orc_chem_source(&b->m, b->bv[id + 5 * s], ry, q1, q2, wd, w);
+  double ts = 1e+6;
+  if (method == 3) {
+    for (int st = 0; st < 3; ++st) {
+      for (int l = 0; l < 5; ++l)
+        cv[id + l * s] = rk_a[st] * qn[l] + rk_b[st] * cv[id + l * s] + rk_c[st] * h * iJ * cn[l];
+      ts = 1e+6;
+      for (int l = 0; l < ns; ++l) {
+        const double y = cv[id + (5 + l) * s];
+        if (y > 1e-25 && fabs(wd[l]) > 1e-25) ts = fmin(ts, fabs(y / wd[l]));
+        cv[id + (5 + l) * s] = rk_a[st] * qn[5 + l] + rk_b[st] * y + rk_c[st] * h * (w[l] + cn[5 + l] * iJ);
+      }
+      wu_node_state(b, id);
+    }
+    return ts;
+  }
+  for (int l = 0; l < 5; ++l) cv[id + l * s] += cn[l] * h * iJ;
+  for (int l = 0; l < ns; ++l) {
+    const double y = cv[id + (5 + l) * s];
+    if (y > 1e-25 && fabs(wd[l]) > 1e-25) ts = fmin(ts, fabs(y / wd[l]));
+    cv[id + (5 + l) * s] = y + (w[l] + cn[5 + l] * iJ) * h;
+  }
+  wu_node_state(b, id);
+  return ts;
+}
+
+/* Wu's splitting, one step of size dt (StrangSplitting.cu:683-861): the transport residual Cn at U^n
+ * (compute_cn, 15-41); per cell n_sub + 1 zero-dimensional reaction steps with Cn frozen, n_sub from the cell's
+ * reaction time scale (wu_reaction_step, 373-421); boundary conditions; then three SSP-RK3 stages over dt/2 of
+ * the transport change R(U) - Cn (wu_RK_subStep and update_cv_and_bv_rk_wu, 504-634). */
+void orc_mix_wu_step(orc_mix *b, double dt, int method) {
+  const int nv = b->nv;
+  const int es = b->explicit_source;
+  b->explicit_source = 0;
+  orc_mix_rhs(b);
+  memcpy(b->cn, b->dq, sizeof(double) * nv * b->n);
+#pragma omp parallel for collapse(2) schedule(dynamic)
+  for (int k = 0; k < b->mz; ++k)
+    for (int j = 0; j < b->my; ++j)
+      for (int i = 0; i < b->mx; ++i) {
+        const long id = ID(b, i, j, k), iq = IDQ(b, i, j, k);
+        double cn[NV];
+        for (int l = 0; l < nv; ++l) cn[l] = b->cn[iq + l * b->n];
+        double tau = b->wu_ts[iq];
+        if (tau < 1e-15) tau = 0.2 * dt;
+        const double ratio = dt / tau;
+        const int n_sub = ratio > 1e6 ? 1000000 : (int)ratio;
+        double h = dt / (n_sub + 1);
+        for (int n = 0; n < n_sub; ++n) b->wu_ts[iq] = wu_zero_d(b, id, cn, h, method);
+        h = dt - h * n_sub;
+        if (fabs(h) > 1e-12) b->wu_ts[iq] = wu_zero_d(b, id, cn, h, method);
+      }
+  apply_bcs(b);
+  all_properties(b);
+  memcpy(b->qn, b->cv, sizeof(double) * nv * b->sz);
+  for (int rk = 0; rk < 3; ++rk) {
+    orc_mix_rhs(b);
+    for (long q = 0; q < (long)nv * b->n; ++q) b->dq[q] -= b->cn[q];
+    update(b, 0.5 * dt, rk);
+    apply_bcs(b);
+    all_properties(b);
+  }
+  b->explicit_source = es;
 }
 
 /* Interface flux of the mixture Riemann solvers for given primitive states (rho,u,v,w,p,Y_1..Y_ns). */

@@ -281,6 +281,7 @@ def _mix_protos(lib):
     lib.orc_mix_chem_substep.argtypes = [P, D, D, D, I]
     lib.orc_mix_strang_step.restype = I
     lib.orc_mix_strang_step.argtypes = [P, D, D, D, I]
+    lib.orc_mix_wu_step.argtypes = [P, D, I]
     lib.orc_mix_point_flux.argtypes = [ctypes.POINTER(OrcParams), ctypes.POINTER(OrcMech), P, P, P, P, D, D, D, D,
                                        P]
 
@@ -361,3 +362,6 @@ class MixBlock:
 
     def strang_step(self, dt, rtol=1e-6, atol=1e-8, max_attempts=100000):
         return load().orc_mix_strang_step(self._h, dt, rtol, atol, max_attempts)
+
+    def wu_step(self, dt, method=0):
+        load().orc_mix_wu_step(self._h, dt, method)
