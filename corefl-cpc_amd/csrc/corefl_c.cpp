@@ -394,10 +394,17 @@ void build(corefl_ctx *ctx, std::vector<corefl::HostBlock> blocks) {
     ctx->chem->mech = corefl::parse_mechanism(mech_text, therm_text);
     ctx->ns = ctx->chem->mech.ns;
     if (ctx->ns != 9) throw std::runtime_error("the mixture kernels are instantiated for 9 species");
-    if (rec < 1 || rec > 4)
+    const bool fs = inv == 51 || inv == 52 || inv == 71 || inv == 72;
+    if (!fs && (rec < 1 || rec > 4))
       throw std::runtime_error("with species, reconstruction must be 1 (first order), 2 (MUSCL), 3 (NND2) or 4 "
                                "(component-wise WENO5)");
-    if (inv < 2 || inv > 4) throw std::runtime_error("with species, inviscid_scheme must be 2 (Roe), 3 (AUSM+) or 4 (HLLC)");
+    if (!fs && (inv < 2 || inv > 4))
+      throw std::runtime_error("with species, inviscid_scheme must be 2 (Roe), 3 (AUSM+), 4 (HLLC) or 51/52/71/72 "
+                               "(WENO5/WENO7 flux splitting)");
+    if (fs && P.get_real("shockSensor_threshold") > 1e-10)
+      throw std::runtime_error("shockSensor_threshold > 0 (hybrid UD/WENO scheme) is not on this path");
+    if (fs && P.get_bool("positive_preserving"))
+      throw std::runtime_error("positive_preserving (the species-flux limiter of WENO.cu) is not on this path");
     if (vo != 0) throw std::runtime_error("viscous mixtures are not on this path (viscous_order must be 0)");
     if (reaction && temporal != 3 && temporal != 4 && temporal != 5)
       throw std::runtime_error("reacting flow: temporal_scheme 3 (explicit source), 4 (Wu's splitting) or 5 (Strang "

@@ -5,10 +5,14 @@
 #include <cmath>
 
 #include "chem_device.hpp"
+#include "weno.hpp"
 
 namespace corefl {
 
 namespace {
+
+using wenodev::weno5_half;
+using wenodev::weno7_half;
 
 constexpr int NS = 9, NV = 5 + NS;
 using chemdev::kRu;
@@ -694,8 +698,193 @@ void dispatch_rs(const DevBlock &b, const MixArrays &x, const Scheme &sc, dim3 g
   }
 }
 
+// LLF flux-splitting WENO with species (WENO.cu:890-990 component-wise 51/71, WENO.cu:1146-1470
+// characteristic 52/72): one thread per interface t+1/2 along DIR, stencil nodes t-R+1 .. t+R.  The spectral
+// radius uses the stored acoustic speed.  The characteristic variant projects the five flow fields with the
+// air eigenvectors at the mixture's Roe state plus alpha_l rho Y_l terms, and each species field as
+// rho Y_l - Y_l,Roe rho; the flow fields are split and reconstructed in one pass over the stencil, the species
+// one at a time, to stay within the register file.
+template<int DIR, int R, bool CH>
+__global__ __launch_bounds__(256) void k_mix_fsweno(DevBlock b, MixArrays x, Scheme sc, double *F) {
+  int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+  if constexpr (DIR == 0) i -= 1;
+  else if constexpr (DIR == 1) j -= 1;
+  else k -= 1;
+  if (i >= b.mx || j >= b.my) return;
+  const long st = DIR == 0 ? 1 : (DIR == 1 ? b.d2 : b.d1);
+  const long id = nid(b, i, j, k), sz = b.sz;
+  constexpr int N = 2 * R;
+  auto weno = [](const double *vp, const double *vn, double eps) {
+    if constexpr (R == 4) return weno7_half(vp, eps) + weno7_half(vn, eps);
+    else return weno5_half(vp[0], vp[1], vp[2], vp[3], vp[4], eps) + weno5_half(vn[0], vn[1], vn[2], vn[3], vn[4], eps);
+  };
+  // node quantities: U_k (unscaled metric), radius, half Jacobian
+  double Uk[N], lam[N], hj[N];
+  double lmax = 0;
+#pragma unroll
+  for (int m = 0; m < N; ++m) {
+    const long im = id + (m - R + 1) * st;
+    const double k0 = b.met[im + (3 * DIR) * sz], k1 = b.met[im + (3 * DIR + 1) * sz],
+                 k2 = b.met[im + (3 * DIR + 2) * sz];
+    Uk[m] = (b.cv[im + sz] * k0 + b.cv[im + 2 * sz] * k1 + b.cv[im + 3 * sz] * k2) / b.cv[im];
+    lam[m] = fabs(Uk[m]) + sqrt(k0 * k0 + k1 * k1 + k2 * k2) * x.snd[im];
+    lmax = fmax(lmax, lam[m]);
+    hj[m] = 0.5 * b.jac[im];
+  }
+  if constexpr (!CH) {
+    const double e0 = sc.weno_eps, e1 = e0 * sc.v_ref * sc.v_ref, e4 = e1 * sc.v_ref * sc.v_ref;
+    for (int l = 0; l < NV; ++l) {
+      double vp[N - 1], vn[N - 1];
+#pragma unroll
+      for (int m = 0; m < N; ++m) {
+        const long im = id + (m - R + 1) * st;
+        const double ql = b.cv[im + l * sz];
+        const double sp = hj[m] * (Uk[m] + lam[m]), sm = hj[m] * (Uk[m] - lam[m]);
+        double fp = sp * ql, fm = sm * ql;
+        if (l >= 1 && l <= 4) {
+          const double pk = b.bv[im + 4 * sz] * hj[m];
+          const double kl = l == 4 ? Uk[m] : b.met[im + (3 * DIR + l - 1) * sz];
+          fp = fma(sp, ql, pk * kl);
+          fm = fma(sm, ql, pk * kl);
+        }
+        if (m < N - 1) vp[m] = fp;
+        if (m > 0) vn[N - 1 - m] = fm;
+      }
+      F[id + l * sz] = weno(vp, vn, l == 0 || l >= 5 ? e0 : (l == 4 ? e4 : e1));
+    }
+  } else {
+    const ChemMech &mc = *x.mech;
+    // Roe state of the two interface nodes (WENO.cu:1146-1178)
+    const long ia = id, ib = id + st;
+    const double ra = b.cv[ia], rb = b.cv[ib];
+    const double sab = sqrt(ra * rb);
+    const double rlc = 1 / (ra + sab), rrc = 1 / (sab + rb);
+    const double um = rlc * b.cv[ia + sz] + rrc * b.cv[ib + sz];
+    const double vm = rlc * b.cv[ia + 2 * sz] + rrc * b.cv[ib + 2 * sz];
+    const double wm = rlc * b.cv[ia + 3 * sz] + rrc * b.cv[ib + 3 * sz];
+    const double pa = b.bv[ia + 4 * sz], pb = b.bv[ib + 4 * sz];
+    double svm[NS], al[NS];
+    double Rm = 0;
+#pragma unroll
+    for (int l = 0; l < NS; ++l) {
+      svm[l] = rlc * b.cv[ia + (5 + l) * sz] + rrc * b.cv[ib + (5 + l) * sz];
+      Rm += svm[l] * (kRu * mc.imw[l]);
+    }
+    const double T = (rlc * pa + rrc * pb) / Rm;
+    double hh[NS], cpl[NS];
+    chemdev::h_cp<NS>(mc, T, hh, cpl);
+    double cp = 0;
+#pragma unroll
+    for (int l = 0; l < NS; ++l) cp += svm[l] * cpl[l];
+    const double gam = cp / (cp - Rm);
+    const double cm = sqrt(gam * Rm * T);
+    const double gm1 = gam - 1;
+    double kx = 0.5 * (b.met[ia + (3 * DIR) * sz] + b.met[ib + (3 * DIR) * sz]);
+    double ky = 0.5 * (b.met[ia + (3 * DIR + 1) * sz] + b.met[ib + (3 * DIR + 1) * sz]);
+    double kz = 0.5 * (b.met[ia + (3 * DIR + 2) * sz] + b.met[ib + (3 * DIR + 2) * sz]);
+    {
+      const double r = 1.0 / sqrt(kx * kx + ky * ky + kz * kz);
+      kx *= r; ky *= r; kz *= r;
+    }
+    double nx = 0, ny = 0, nz = 0;
+    if (fabs(kx) > 0.8) {
+      nx = -kz; nz = kx;
+    } else {
+      ny = kz; nz = -ky;
+    }
+    {
+      const double r = 1.0 / sqrt(nx * nx + ny * ny + nz * nz);
+      nx *= r; ny *= r; nz *= r;
+    }
+    double qx = ky * nz - kz * ny, qy = kz * nx - kx * nz, qz = kx * ny - ky * nx;
+    {
+      const double r = 1.0 / sqrt(qx * qx + qy * qy + qz * qz);
+      qx *= r; qy *= r; qz *= r;
+    }
+    const double Ub = kx * um + ky * vm + kz * wm;
+    const double Un = nx * um + ny * vm + nz * wm, Uq = qx * um + qy * vm + qz * wm;
+    const double ic2 = 1.0 / (cm * cm), hic2 = 0.5 * ic2, a = 0.5 * gm1 * (um * um + vm * vm + wm * wm);
+#pragma unroll
+    for (int l = 0; l < NS; ++l) al[l] = (gam * (kRu * mc.imw[l]) * T - gm1 * hh[l]) * ic2;
+    // flow fields: rows 0 and 4 are A +- B, row 3 is x0 - 2A (the L of WENO.cu:1253-1291 in factored form);
+    // the species terms enter rows 0/4 with +1/2 and row 3 with -1 times sum_l alpha_l x_(5+l)
+    double vp[5][N - 1], vn[5][N - 1];
+#pragma unroll
+    for (int m = 0; m < N; ++m) {
+      const long im = id + (m - R + 1) * st;
+      double q[5];
+#pragma unroll
+      for (int l = 0; l < 5; ++l) q[l] = b.cv[im + l * sz];
+      double sb = 0;
+#pragma unroll
+      for (int l = 0; l < NS; ++l) sb = fma(al[l], b.cv[im + (5 + l) * sz], sb);
+      const double p = b.bv[im + 4 * sz];
+      const double k0 = b.met[im + (3 * DIR) * sz], k1 = b.met[im + (3 * DIR + 1) * sz],
+                   k2 = b.met[im + (3 * DIR + 2) * sz];
+      const double u = Uk[m];
+      const double fl[5] = {q[0] * u, fma(q[1], u, p * k0), fma(q[2], u, p * k1), fma(q[3], u, p * k2),
+                            (q[4] + p) * u};
+      double LF[5], LQ[5];
+      auto project = [&](const double *v, double s5, double *w) {
+        const double A = hic2 * (a * v[0] - gm1 * (um * v[1] + vm * v[2] + wm * v[3]) + gm1 * v[4]);
+        const double B = hic2 * cm * (Ub * v[0] - (kx * v[1] + ky * v[2] + kz * v[3]));
+        w[0] = A + B + 0.5 * s5;
+        w[1] = nx * v[1] + ny * v[2] + nz * v[3] - Un * v[0];
+        w[2] = qx * v[1] + qy * v[2] + qz * v[3] - Uq * v[0];
+        w[3] = v[0] - 2 * A - s5;
+        w[4] = A - B + 0.5 * s5;
+      };
+      project(fl, sb * u, LF);
+      project(q, sb, LQ);
+#pragma unroll
+      for (int l = 0; l < 5; ++l) {
+        if (m < N - 1) vp[l][m] = hj[m] * (LF[l] + lmax * LQ[l]);
+        if (m > 0) vn[l][N - 1 - m] = hj[m] * (LF[l] - lmax * LQ[l]);
+      }
+    }
+    const double eps = sc.weno_eps;
+    double fch[5];
+#pragma unroll
+    for (int l = 0; l < 5; ++l) fch[l] = weno(vp[l], vn[l], eps);
+    const double s1 = fch[0] + fch[3] + fch[4], s3 = fch[0] - fch[4];
+    double sa = 0;
+    for (int l = 0; l < NS; ++l) {
+      double wp[N - 1], wn[N - 1];
+#pragma unroll
+      for (int m = 0; m < N; ++m) {
+        const long im = id + (m - R + 1) * st;
+        const double LQ = b.cv[im + (5 + l) * sz] - svm[l] * b.cv[im];
+        const double LF = Uk[m] * LQ;
+        if (m < N - 1) wp[m] = hj[m] * (LF + lmax * LQ);
+        if (m > 0) wn[N - 1 - m] = hj[m] * (LF - lmax * LQ);
+      }
+      const double f = weno(wp, wn, eps);
+      F[id + (5 + l) * sz] = svm[l] * s1 + f;
+      sa += al[l] * f;
+    }
+    F[id] = s1;
+    F[id + sz] = um * s1 - kx * cm * s3 + nx * fch[1] + qx * fch[2];
+    F[id + 2 * sz] = vm * s1 - ky * cm * s3 + ny * fch[1] + qy * fch[2];
+    F[id + 3 * sz] = wm * s1 - kz * cm * s3 + nz * fch[1] + qz * fch[2];
+    const double H = rlc * (b.cv[ia + 4 * sz] + pa) + rrc * (b.cv[ib + 4 * sz] + pb);
+    F[id + 4 * sz] = H * s1 - Ub * cm * s3 + Un * fch[1] + Uq * fch[2] - cm * cm / gm1 * fch[3] - sa * cm * cm / gm1;
+  }
+}
+
+template<int DIR>
+void dispatch_fs(const DevBlock &b, const MixArrays &x, const Scheme &sc, dim3 grid, double *F, hipStream_t s) {
+  const dim3 tpb(64, 4, 1);
+  switch (sc.inviscid_scheme) {
+    case 51: k_mix_fsweno<DIR, 3, false><<<grid, tpb, 0, s>>>(b, x, sc, F); break;
+    case 52: k_mix_fsweno<DIR, 3, true><<<grid, tpb, 0, s>>>(b, x, sc, F); break;
+    case 71: k_mix_fsweno<DIR, 4, false><<<grid, tpb, 0, s>>>(b, x, sc, F); break;
+    default: k_mix_fsweno<DIR, 4, true><<<grid, tpb, 0, s>>>(b, x, sc, F); break;
+  }
+}
+
 template<int DIR>
 void dispatch_rec(const DevBlock &b, const MixArrays &x, const Scheme &sc, dim3 grid, double *F, hipStream_t s) {
+  if (sc.inviscid_scheme > 50) return dispatch_fs<DIR>(b, x, sc, grid, F, s);
   switch (sc.reconstruction) {
     case 1: dispatch_rs<DIR, 1>(b, x, sc, grid, F, s); break;
     case 2: dispatch_rs<DIR, 2>(b, x, sc, grid, F, s); break;

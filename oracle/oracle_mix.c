@@ -520,9 +520,163 @@ static void hllc(const orc_mix *b, const double *pl, const double *pr, const dou
   for (int l = 5; l < nv; ++l) fc[l] = QC * q[l];
 }
 
+/* LLF flux-splitting WENO with species, compute_convective_term_weno_x<Mixture> (WENO.cu:890-990 component-wise
+ * 51/71; WENO.cu:1146-1470 characteristic 52/72) for the interface between stencil nodes n/2-1 and n/2.  Per node
+ * m: the conservative state q (nv), the sweep's metric k, the Jacobian, pressure and acoustic speed.  The
+ * characteristic variant extends the air eigenvectors with the species: the Roe state carries Roe-averaged mass
+ * fractions, the sound speed comes from the mixture gamma at the Roe-weighted temperature, the rows of the
+ * five flow fields gain alpha_l rho Y_l terms (alpha_l = (gamma R_l T - (gamma - 1) h_l) / c^2), and each species
+ * field is rho Y_l - Y_l,Roe rho. */
+static void fsweno_mix(const orc_mix *b, double (*q)[NV], double (*k)[3], const double *jac, const double *pp,
+                       const double *snd, int n, double *fc) {
+  const int nv = b->nv, ns = b->ns;
+  const double eps = b->p.weno_eps;
+  double Uk[8], lam[8];
+  double lmax = 0;
+  for (int m = 0; m < n; ++m) {
+    Uk[m] = (q[m][1] * k[m][0] + q[m][2] * k[m][1] + q[m][3] * k[m][2]) / q[m][0];
+    lam[m] = fabs(Uk[m]) + sqrt(k[m][0] * k[m][0] + k[m][1] * k[m][1] + k[m][2] * k[m][2]) * snd[m];
+    if (lam[m] > lmax) lmax = lam[m];
+  }
+  if (b->p.inviscid_scheme % 10 == 1) {
+    const double e1 = eps * b->p.v_ref * b->p.v_ref;
+    for (int l = 0; l < nv; ++l) {
+      double vp[7], vm[7];
+      for (int m = 0; m < n; ++m) {
+        const double hj = 0.5 * jac[m];
+        const double sp = hj * (Uk[m] + lam[m]), sm = hj * (Uk[m] - lam[m]), pk = pp[m] * hj;
+        double fp, fm;
+        if (l == 0 || l >= 5) {
+          fp = sp * q[m][l];
+          fm = sm * q[m][l];
+        } else if (l < 4) {
+          fp = fma(sp, q[m][l], pk * k[m][l - 1]);
+          fm = fma(sm, q[m][l], pk * k[m][l - 1]);
+        } else {
+          fp = fma(sp, q[m][4], pk * Uk[m]);
+          fm = fma(sm, q[m][4], pk * Uk[m]);
+        }
+        if (m < n - 1) vp[m] = fp;
+        if (m > 0) vm[m - 1] = fm;
+      }
+      const double el = (l >= 1 && l <= 3) ? e1 : (l == 4 ? e1 * b->p.v_ref * b->p.v_ref : eps);
+      fc[l] = n == 8 ? orc_weno7(vp, vm, el, 1) : orc_weno5(vp, vm, el, 1);
+    }
+    return;
+  }
+  const int c = n / 2 - 1;
+  const double t1 = sqrt(q[c][0] * q[c + 1][0]);
+  const double rlc = 1 / (q[c][0] + t1), rrc = 1 / (t1 + q[c + 1][0]);
+  const double um = rlc * q[c][1] + rrc * q[c + 1][1];
+  const double vm = rlc * q[c][2] + rrc * q[c + 1][2];
+  const double wm = rlc * q[c][3] + rrc * q[c + 1][3];
+  double svm[MS], Rl[MS];
+  double R = 0;
+  for (int l = 0; l < ns; ++l) {
+    svm[l] = rlc * q[c][5 + l] + rrc * q[c + 1][5 + l];
+    Rl[l] = kRu / b->m.mw[l];
+    R += svm[l] * Rl[l];
+  }
+  const double T = (rlc * pp[c] + rrc * pp[c + 1]) / R;
+  double h[MS], cpl[MS];
+  orc_chem_h_cp(&b->m, T, h, cpl);
+  double cp = 0;
+  for (int l = 0; l < ns; ++l) cp += svm[l] * cpl[l];
+  const double gam = cp / (cp - R);
+  const double cm = sqrt(gam * R * T);
+  const double gm1 = gam - 1;
+  double kx = 0.5 * (k[c][0] + k[c + 1][0]), ky = 0.5 * (k[c][1] + k[c + 1][1]), kz = 0.5 * (k[c][2] + k[c + 1][2]);
+  double r = 1 / sqrt(kx * kx + ky * ky + kz * kz);
+  kx *= r;
+  ky *= r;
+  kz *= r;
+  const double Ub = kx * um + ky * vm + kz * wm;
+  double nx = 0, ny = 0, nz = 0;
+  if (fabs(kx) > 0.8) {
+    nx = -kz;
+    nz = kx;
+  } else {
+    ny = kz;
+    nz = -ky;
+  }
+  r = 1 / sqrt(nx * nx + ny * ny + nz * nz);
+  nx *= r;
+  ny *= r;
+  nz *= r;
+  double qx = ky * nz - kz * ny, qy = kz * nx - kx * nz, qz = kx * ny - ky * nx;
+  r = 1 / sqrt(qx * qx + qy * qy + qz * qz);
+  qx *= r;
+  qy *= r;
+  qz *= r;
+  const double Un = nx * um + ny * vm + nz * wm, Uq = qx * um + qy * vm + qz * wm;
+  const double ic2 = 1.0 / (cm * cm);
+  double al[MS];
+  for (int l = 0; l < ns; ++l) al[l] = (gam * Rl[l] * T - gm1 * h[l]) * ic2;
+  const double a = 0.5 * gm1 * (um * um + vm * vm + wm * wm);
+  double sbq[8], sbf[8];
+  for (int m = 0; m < n; ++m) {
+    double sp = 0, sm = 0;
+    for (int l = 0; l < ns; ++l) {
+      sp = fma(al[l], q[m][5 + l], sp);
+      sm = fma(al[l], q[m][5 + l] * Uk[m], sm);
+    }
+    sbq[m] = sp;
+    sbf[m] = sm;
+  }
+  const double Lm[5][5] = {
+      {(a + Ub * cm) * ic2 * 0.5, -(gm1 * um + kx * cm) * ic2 * 0.5, -(gm1 * vm + ky * cm) * ic2 * 0.5,
+       -(gm1 * wm + kz * cm) * ic2 * 0.5, gm1 * ic2 * 0.5},
+      {-Un, nx, ny, nz, 0},
+      {-Uq, qx, qy, qz, 0},
+      {1 - a * ic2, gm1 * um * ic2, gm1 * vm * ic2, gm1 * wm * ic2, -gm1 * ic2},
+      {(a - Ub * cm) * ic2 * 0.5, -(gm1 * um - kx * cm) * ic2 * 0.5, -(gm1 * vm - ky * cm) * ic2 * 0.5,
+       -(gm1 * wm - kz * cm) * ic2 * 0.5, gm1 * ic2 * 0.5}};
+  const double tb[5] = {0.5, 0, 0, -1, 0.5};
+  double fch[NV];
+  for (int l = 0; l < 5; ++l) {
+    const double *L = Lm[l];
+    double vp[7], vmn[7];
+    for (int m = 0; m < n; ++m) {
+      const double F1 = fma(q[m][1], Uk[m], pp[m] * k[m][0]), F2 = fma(q[m][2], Uk[m], pp[m] * k[m][1]);
+      const double F3 = fma(q[m][3], Uk[m], pp[m] * k[m][2]);
+      const double LF = L[0] * (q[m][0] * Uk[m]) + L[1] * F1 + L[2] * F2 + L[3] * F3 +
+                        L[4] * ((q[m][4] + pp[m]) * Uk[m]) + tb[l] * sbf[m];
+      const double LQ = L[0] * q[m][0] + L[1] * q[m][1] + L[2] * q[m][2] + L[3] * q[m][3] + L[4] * q[m][4] +
+                        tb[l] * sbq[m];
+      if (m < n - 1) vp[m] = 0.5 * jac[m] * (LF + lmax * LQ);
+      if (m > 0) vmn[m - 1] = 0.5 * jac[m] * (LF - lmax * LQ);
+    }
+    fch[l] = n == 8 ? orc_weno7(vp, vmn, eps, 1) : orc_weno5(vp, vmn, eps, 1);
+  }
+  for (int l = 0; l < ns; ++l) {
+    double vp[7], vmn[7];
+    for (int m = 0; m < n; ++m) {
+      const double LQ = q[m][5 + l] - svm[l] * q[m][0];
+      const double LF = Uk[m] * LQ;
+      if (m < n - 1) vp[m] = 0.5 * jac[m] * (LF + lmax * LQ);
+      if (m > 0) vmn[m - 1] = 0.5 * jac[m] * (LF - lmax * LQ);
+    }
+    fch[5 + l] = n == 8 ? orc_weno7(vp, vmn, eps, 1) : orc_weno5(vp, vmn, eps, 1);
+  }
+  const double s1 = fch[0] + fch[3] + fch[4], s3 = fch[0] - fch[4];
+  fc[0] = s1;
+  fc[1] = um * s1 - kx * cm * s3 + nx * fch[1] + qx * fch[2];
+  fc[2] = vm * s1 - ky * cm * s3 + ny * fch[1] + qy * fch[2];
+  fc[3] = wm * s1 - kz * cm * s3 + nz * fch[1] + qz * fch[2];
+  const double H = rlc * (q[c][4] + pp[c]) + rrc * (q[c + 1][4] + pp[c + 1]);
+  fc[4] = H * s1 - Ub * cm * s3 + Un * fch[1] + Uq * fch[2] - cm * cm / gm1 * fch[3];
+  double sa = 0;
+  for (int l = 0; l < ns; ++l) {
+    fc[5 + l] = svm[l] * s1 + fch[5 + l];
+    sa += al[l] * fch[5 + l];
+  }
+  fc[4] -= sa * cm * cm / gm1;
+}
+
 static void inviscid_dir(orc_mix *b, int dir) {
+  const int fs = b->p.inviscid_scheme > 50;
   const int rec = b->p.reconstruction;
-  const int r = rec == 5 ? 4 : (rec == 4 ? 3 : 2); /* stencil half width */
+  const int r = fs ? (b->p.inviscid_scheme > 70 ? 4 : 3) : (rec == 5 ? 4 : (rec == 4 ? 3 : 2)); /* half width */
   const int n = 2 * r, nv = b->nv, ns = b->ns;
   const long s = b->sz;
   const long st = dir == 0 ? 1 : (dir == 1 ? b->d2 : b->d1);
@@ -539,6 +693,19 @@ static void inviscid_dir(orc_mix *b, int dir) {
         idx[oa] = A;
         idx[ob] = B;
         const long id = ID(b, idx[0], idx[1], idx[2]);
+        if (fs) {
+          double q[8][NV], kk[8][3], jj[8], pp[8], cc[8];
+          for (int mm = 0; mm < n; ++mm) {
+            const long im = id + (mm - r + 1) * st;
+            for (int l = 0; l < nv; ++l) q[mm][l] = b->cv[im + l * s];
+            for (int c = 0; c < 3; ++c) kk[mm][c] = b->met[im + (3 * dir + c) * s];
+            jj[mm] = b->jac[im];
+            pp[mm] = b->bv[im + 4 * s];
+            cc[mm] = b->c[im];
+          }
+          fsweno_mix(b, q, kk, jj, pp, cc, n, Fl + (long)nv * (t + 1));
+          continue;
+        }
         double pv[8][NV];
         for (int q = 0; q < n; ++q) {
           const long iq = id + (q - r + 1) * st;

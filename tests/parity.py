@@ -55,7 +55,8 @@ def make_mix_pair(case, seed=3, **state_kw):
     ngg = S.get_int("ngg")
     X, Y, Z = cases.physical_grid(case)
     inv, recon, _ = case["scheme"]
-    p = oracle_lib.params(inviscid=inv, recon=recon, visc=0, cfl=case["cfl"], eps=S.get_real("weno_eps"))
+    p = oracle_lib.params(inviscid=inv, recon=recon, visc=0, cfl=case["cfl"], eps=S.get_real("weno_eps"),
+                          v_ref=S.get_real("v_inf"))
     m = oracle_lib.mech_struct(C.parse(open(CC.MECH).read(), open(CC.THERM).read()))
     B = oracle_lib.MixBlock(p, m, X, Y, Z, ngg, case["faces"])
     bv, sv = cases.reactive_state(case["n"], ngg, seed=seed, **state_kw)

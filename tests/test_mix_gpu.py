@@ -13,7 +13,8 @@ from parity import interior, make_mix_pair, rel_err
 pytestmark = pytest.mark.gpu
 
 N = 13
-PAIRS = [(4, 2), (2, 4), (3, 3), (1, 2), (4, 4), (2, 2)]  # (reconstruction, inviscid_scheme)
+# (reconstruction, inviscid_scheme); 51/52/71/72: the LLF flux-splitting WENO schemes (reconstruction ignored)
+PAIRS = [(4, 2), (2, 4), (3, 3), (1, 2), (4, 4), (2, 2), (4, 51), (4, 52), (4, 71), (4, 72)]
 
 
 def _case(recon, inv, temporal=5, reaction=1):
@@ -53,7 +54,7 @@ def test_explicit_source_rhs_and_dt():
     assert abs(dt - dto) <= 1e-13 * dto
 
 
-@pytest.mark.parametrize("recon,inv", [(4, 2), (2, 4)])
+@pytest.mark.parametrize("recon,inv", [(4, 2), (2, 4), (4, 72), (4, 51)])
 def test_rk3_step_non_reacting(recon, inv):
     S, B = make_mix_pair(_case(recon, inv, temporal=3, reaction=0))
     dt = B.cfl_dt()
@@ -96,11 +97,11 @@ def test_strang_step_matches_oracle():
     assert np.max(cv[5 + 7] / cv[0]) > 1e-4
 
 
-@pytest.mark.parametrize("method", [0, 3])
-def test_wu_splitting_step_matches_oracle(method):
+@pytest.mark.parametrize("method,inv", [(0, 2), (3, 2), (0, 72)])
+def test_wu_splitting_step_matches_oracle(method, inv):
     """Wu's splitting (temporal_scheme 4, chem_advance_method 0 or 3) against the oracle over two steps, with
     slow hot kernels (the scheme's explicit chemistry sub-steps are only stable for time scales above them)."""
-    case = _case(4, 2, temporal=4, reaction=1)
+    case = _case(4, inv, temporal=4, reaction=1)
     if method:
         case["setup"] += f"int chem_advance_method = {method}\n"
     S, B = make_mix_pair(case, peak=(1150.0, 1250.0))
