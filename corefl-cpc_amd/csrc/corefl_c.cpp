@@ -403,8 +403,7 @@ void build(corefl_ctx *ctx, std::vector<corefl::HostBlock> blocks) {
                                "(WENO5/WENO7 flux splitting)");
     if (fs && P.get_real("shockSensor_threshold") > 1e-10)
       throw std::runtime_error("shockSensor_threshold > 0 (hybrid UD/WENO scheme) is not on this path");
-    if (fs && P.get_bool("positive_preserving"))
-      throw std::runtime_error("positive_preserving (the species-flux limiter of WENO.cu) is not on this path");
+
     if (vo != 0) throw std::runtime_error("viscous mixtures are not on this path (viscous_order must be 0)");
     if (reaction && temporal != 3 && temporal != 4 && temporal != 5)
       throw std::runtime_error("reacting flow: temporal_scheme 3 (explicit source), 4 (Wu's splitting) or 5 (Strang "
@@ -463,7 +462,8 @@ void build(corefl_ctx *ctx, std::vector<corefl::HostBlock> blocks) {
   P.set_real("v_inf", v_inf);
   P.set_real("weno_eps", 1e-40 * rho_inf * v_inf * rho_inf * v_inf);
   ctx->scheme = {inv, P.get_int("reconstruction"), vo, P.get_real("weno_eps"), P.get_real("entropy_fix_factor"),
-                 P.get_real("prandtl_number"), P.get_real("cfl"), v_inf};
+                 P.get_real("prandtl_number"), P.get_real("cfl"), v_inf, P.get_bool("positive_preserving") ? 1 : 0,
+                 rho_inf};
   ctx->fixed_dt = P.get_bool("fixed_time_step");
   ctx->t_end = P.get_real("total_simulation_time");
 
@@ -541,6 +541,7 @@ void build(corefl_ctx *ctx, std::vector<corefl::HostBlock> blocks) {
       B.mix.sv = ctx->alloc(ctx->ns * sz);
       B.mix.snd = ctx->alloc(sz);
       B.mix.mech = ctx->chem->mech_dev;
+      B.mix.dt = ctx->dt_dev;
       B.mix.explicit_source = reaction && temporal == 3;
       if (ctx->wu_method >= 0) B.mix.tscale = ctx->alloc(d.n); // reaction time scale, zero at the start
       if (B.mix.explicit_source) {

@@ -37,6 +37,8 @@ struct Scheme {
   int viscous_order;   // 0, 2 (face-centred), 4, 8 (collocated)
   double weno_eps, entropy_fix_factor, prandtl, cfl;
   double v_ref;        // v_inf: the component-wise WENO epsilon is weno_eps v_ref^0, ^2, ^4 (WENO.cu:946-949)
+  int positive_preserving; // species-flux positivity limiter of the flux-splitting schemes (mixtures)
+  double rho_ref;          // rho_inf: the limiter's threshold scale
 };
 
 // Enqueue on `s`.  Each returns hipSuccess or the launch error.

@@ -21,6 +21,7 @@ struct MixArrays {
   const ChemMech *mech; // device tables
   int explicit_source;  // add the chemical source to the residual (reaction = 1 with temporal_scheme 3)
   double *tscale;       // reaction time scale per interior node (n), written with the explicit source
+  const double *dt;     // the step size on the device (read by the positivity limiter)
 };
 
 hipError_t launch_mix_cv_from_bv_all(const DevBlock &b, const MixArrays &x, hipStream_t s);

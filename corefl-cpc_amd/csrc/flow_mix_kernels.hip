@@ -719,7 +719,7 @@ __global__ __launch_bounds__(256) void k_mix_fsweno(DevBlock b, MixArrays x, Sch
     else return weno5_half(vp[0], vp[1], vp[2], vp[3], vp[4], eps) + weno5_half(vn[0], vn[1], vn[2], vn[3], vn[4], eps);
   };
   // node quantities: U_k (unscaled metric), radius, half Jacobian
-  double Uk[N], lam[N], hj[N];
+  double Uk[N], lam[N], hj[N], cg[N];
   double lmax = 0;
 #pragma unroll
   for (int m = 0; m < N; ++m) {
@@ -727,10 +727,12 @@ __global__ __launch_bounds__(256) void k_mix_fsweno(DevBlock b, MixArrays x, Sch
     const double k0 = b.met[im + (3 * DIR) * sz], k1 = b.met[im + (3 * DIR + 1) * sz],
                  k2 = b.met[im + (3 * DIR + 2) * sz];
     Uk[m] = (b.cv[im + sz] * k0 + b.cv[im + 2 * sz] * k1 + b.cv[im + 3 * sz] * k2) / b.cv[im];
-    lam[m] = fabs(Uk[m]) + sqrt(k0 * k0 + k1 * k1 + k2 * k2) * x.snd[im];
+    cg[m] = sqrt(k0 * k0 + k1 * k1 + k2 * k2) * x.snd[im];
+    lam[m] = fabs(Uk[m]) + cg[m];
     lmax = fmax(lmax, lam[m]);
     hj[m] = 0.5 * b.jac[im];
   }
+  double fs[NS]; // species fluxes, limited below before they are stored
   if constexpr (!CH) {
     const double e0 = sc.weno_eps, e1 = e0 * sc.v_ref * sc.v_ref, e4 = e1 * sc.v_ref * sc.v_ref;
     for (int l = 0; l < NV; ++l) {
@@ -750,7 +752,9 @@ __global__ __launch_bounds__(256) void k_mix_fsweno(DevBlock b, MixArrays x, Sch
         if (m < N - 1) vp[m] = fp;
         if (m > 0) vn[N - 1 - m] = fm;
       }
-      F[id + l * sz] = weno(vp, vn, l == 0 || l >= 5 ? e0 : (l == 4 ? e4 : e1));
+      const double f = weno(vp, vn, l == 0 || l >= 5 ? e0 : (l == 4 ? e4 : e1));
+      if (l < 5) F[id + l * sz] = f;
+      else fs[l - 5] = f;
     }
   } else {
     const ChemMech &mc = *x.mech;
@@ -859,7 +863,7 @@ __global__ __launch_bounds__(256) void k_mix_fsweno(DevBlock b, MixArrays x, Sch
         if (m > 0) wn[N - 1 - m] = hj[m] * (LF - lmax * LQ);
       }
       const double f = weno(wp, wn, eps);
-      F[id + (5 + l) * sz] = svm[l] * s1 + f;
+      fs[l] = svm[l] * s1 + f;
       sa += al[l] * f;
     }
     F[id] = s1;
@@ -869,6 +873,44 @@ __global__ __launch_bounds__(256) void k_mix_fsweno(DevBlock b, MixArrays x, Sch
     const double H = rlc * (b.cv[ia + 4 * sz] + pa) + rrc * (b.cv[ib + 4 * sz] + pb);
     F[id + 4 * sz] = H * s1 - Ub * cm * s3 + Un * fch[1] + Uq * fch[2] - cm * cm / gm1 * fch[3] - sa * cm * cm / gm1;
   }
+  // positivity limiter of the species fluxes (WENO.cu:991-1087 component-wise, 1471-1540 characteristic): where
+  // alpha J rho Y_L / 2 - dt f or alpha J rho Y_R / 2 + dt f would be negative (alpha = 1/3 in 3D, 1/2 in 2D),
+  // f is blended towards a first-order split flux of the two interface nodes, from the first offending
+  // species on; that flux splits with |U_k| + c|k| (component-wise) or U_k + c|k| (characteristic)
+  if (sc.positive_preserving) {
+    const double dt = x.dt[0];
+    const double alpha = b.dim == 3 ? 1.0 / 3.0 : 0.5;
+    const long ia = id, ib = id + st;
+    const double jl = 2 * hj[R - 1], jr = 2 * hj[R];
+    const double aL = 0.5 * alpha * jl, aR = 0.5 * alpha * jr;
+    int start = NS;
+    for (int l = NS - 1; l >= 0; --l) {
+      const double up = aL * b.cv[ia + (5 + l) * sz] - dt * fs[l];
+      const double um = aR * b.cv[ib + (5 + l) * sz] + dt * fs[l];
+      if (up < 0 || um < 0) start = l;
+    }
+    const double sl = CH ? cg[R - 1] : lam[R - 1];
+    const double sr = CH ? cg[R] : lam[R];
+    for (int l = start; l < NS; ++l) {
+      const double yl = b.cv[ia + (5 + l) * sz], yr = b.cv[ib + (5 + l) * sz];
+      const double fp = 0.5 * jl * (Uk[R - 1] + sl) * yl, fm = 0.5 * jr * (Uk[R] - sr) * yr;
+      const double f1 = 0.5 * (fp + Uk[R - 1] * jl * yl) + 0.5 * (fm - Uk[R] * jr * yr);
+      double tp = 1, tm = 1;
+      const double up = aL * yl - dt * fs[l];
+      if (up < 0) {
+        const double up1 = aL * yl - dt * f1;
+        if (fabs(up - up1) > 1e-20 * sc.rho_ref) tp = fmin(fmax((0 - up1) / (up - up1), 0.0), 1.0);
+      }
+      const double um = aR * yr + dt * fs[l];
+      if (um < 0) {
+        const double um1 = aR * yr + dt * f1;
+        if (fabs(um - um1) > 1e-20 * sc.rho_ref) tm = fmin(fmax((0 - um1) / (um - um1), 0.0), 1.0);
+      }
+      fs[l] = fmin(tp, tm) * (fs[l] - f1) + f1;
+    }
+  }
+#pragma unroll
+  for (int l = 0; l < NS; ++l) F[id + (5 + l) * sz] = fs[l];
 }
 
 template<int DIR>

@@ -35,6 +35,8 @@ typedef struct {
   double entropy_fix_factor; /* default 0.125 (Parameter.cpp:380)                                 */
   double weno_eps;       /* eps_weno * weno_eps_scale = 1e-40 (rho_inf v_inf)^2 (WENO.cu:8,884)   */
   double v_ref;          /* v_inf: scales the component-wise WENO epsilon (WENO.cu:946-949)       */
+  int positive_preserving; /* species-flux limiter of the flux-splitting schemes (WENO.cu:991-1087)  */
+  double rho_ref;        /* rho_inf: the limiter's threshold scale                                  */
 } orc_params;
 
 /* ---- point functions (pure; parity pinned by tests/test_oracle_points.py) ---- */
@@ -164,6 +166,8 @@ void orc_mix_set_explicit_source(orc_mix *b, int on);
 void orc_mix_rhs(orc_mix *b);
 double orc_mix_cfl_dt(orc_mix *b);
 void orc_mix_rk3_step(orc_mix *b, double dt);
+/* Step size the positivity limiter uses in the next orc_mix_rhs (the step functions set it themselves). */
+void orc_mix_set_step(orc_mix *b, double dt);
 int orc_mix_chem_substep(orc_mix *b, double dt, double rtol, double atol, int max_attempts);
 int orc_mix_strang_step(orc_mix *b, double dt, double rtol, double atol, int max_attempts);
 /* Wu's splitting (temporal_scheme 4), one step: chem_advance_method 0 (forward Euler) or 3 (SSP-RK3 stages). */

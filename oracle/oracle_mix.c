@@ -35,6 +35,7 @@ struct orc_mix {
   int explicit_source; /* chemical source in the residual (reaction = 1, temporal_scheme 3) */
   double *cv, *qn, *bv, *sv, *c, *efd, *dq, *jac, *met, *hstep, *tscale;
   double *cn, *wu_ts; /* Wu's splitting: frozen transport residual, reaction time scale (starts at zero) */
+  double pp_dt;       /* step size seen by the positivity limiter (the global dt of the current step) */
 };
 
 #define ID(b, i, j, k) ((long)((k) + (b)->g) * (b)->d1 + (long)((j) + (b)->g) * (b)->d2 + ((i) + (b)->g))
@@ -673,6 +674,50 @@ static void fsweno_mix(const orc_mix *b, double (*q)[NV], double (*k)[3], const 
   fc[4] -= sa * cm * cm / gm1;
 }
 
+/* Positivity limiter of the species fluxes (WENO.cu:991-1087 component-wise, 1471-1540 characteristic): where
+ * alpha J rho Y_L / 2 - dt f or alpha J rho Y_R / 2 + dt f would be negative (alpha = 1/3 in 3D, 1/2 in 2D), the
+ * species flux f is blended towards a first-order split flux f1 of the two interface nodes, by the largest
+ * factor that keeps both non-negative; from the first offending species on.  f1 splits with |U_k| + c|k| in
+ * the component-wise schemes and with U_k + c|k| in the characteristic ones, as the reference does. */
+static void positivity_limit(const orc_mix *b, double (*q)[NV], double (*k)[3], const double *jac,
+                             const double *snd, int n, double *fc) {
+  const int ns = b->ns, c = n / 2 - 1;
+  const double dt = b->pp_dt;
+  const double alpha = b->dim == 3 ? 1.0 / 3.0 : 0.5;
+  const double aL = 0.5 * alpha * jac[c], aR = 0.5 * alpha * jac[c + 1];
+  int start = -1;
+  for (int l = 0; l < ns && start < 0; ++l) {
+    const double up = aL * q[c][5 + l] - dt * fc[5 + l];
+    const double um = aR * q[c + 1][5 + l] + dt * fc[5 + l];
+    if (up < 0 || um < 0) start = l;
+  }
+  if (start < 0) return;
+  double U[2], s[2];
+  for (int e = 0; e < 2; ++e) {
+    const double *kk = k[c + e], *qq = q[c + e];
+    U[e] = (kk[0] * qq[1] + kk[1] * qq[2] + kk[2] * qq[3]) / qq[0];
+    const double cg = sqrt(kk[0] * kk[0] + kk[1] * kk[1] + kk[2] * kk[2]) * snd[c + e];
+    s[e] = b->p.inviscid_scheme % 10 == 1 ? fabs(U[e]) + cg : cg;
+  }
+  for (int l = start; l < ns; ++l) {
+    const double yl = q[c][5 + l], yr = q[c + 1][5 + l];
+    const double fp = 0.5 * jac[c] * (U[0] + s[0]) * yl, fm = 0.5 * jac[c + 1] * (U[1] - s[1]) * yr;
+    const double f1 = 0.5 * (fp + U[0] * jac[c] * yl) + 0.5 * (fm - U[1] * jac[c + 1] * yr);
+    double tp = 1, tm = 1;
+    const double up = aL * yl - dt * fc[5 + l];
+    if (up < 0) {
+      const double up1 = aL * yl - dt * f1;
+      if (fabs(up - up1) > 1e-20 * b->p.rho_ref) tp = fmin(fmax((0 - up1) / (up - up1), 0.0), 1.0);
+    }
+    const double um = aR * yr + dt * fc[5 + l];
+    if (um < 0) {
+      const double um1 = aR * yr + dt * f1;
+      if (fabs(um - um1) > 1e-20 * b->p.rho_ref) tm = fmin(fmax((0 - um1) / (um - um1), 0.0), 1.0);
+    }
+    fc[5 + l] = fmin(tp, tm) * (fc[5 + l] - f1) + f1;
+  }
+}
+
 static void inviscid_dir(orc_mix *b, int dir) {
   const int fs = b->p.inviscid_scheme > 50;
   const int rec = b->p.reconstruction;
@@ -704,6 +749,7 @@ static void inviscid_dir(orc_mix *b, int dir) {
             cc[mm] = b->c[im];
           }
           fsweno_mix(b, q, kk, jj, pp, cc, n, Fl + (long)nv * (t + 1));
+          if (b->p.positive_preserving) positivity_limit(b, q, kk, jj, cc, n, Fl + (long)nv * (t + 1));
           continue;
         }
         double pv[8][NV];
@@ -869,7 +915,10 @@ double orc_mix_cfl_dt(orc_mix *b) {
   return dtmin;
 }
 
+void orc_mix_set_step(orc_mix *b, double dt) { b->pp_dt = dt; }
+
 void orc_mix_rk3_step(orc_mix *b, double dt) {
+  b->pp_dt = dt;
   memcpy(b->qn, b->cv, sizeof(double) * b->nv * b->sz);
   for (int rk = 0; rk < 3; ++rk) {
     orc_mix_rhs(b);
@@ -994,6 +1043,7 @@ void orc_mix_wu_step(orc_mix *b, double dt, int method) {
   const int nv = b->nv;
   const int es = b->explicit_source;
   b->explicit_source = 0;
+  b->pp_dt = dt;
   orc_mix_rhs(b);
   memcpy(b->cn, b->dq, sizeof(double) * nv * b->n);
 #pragma omp parallel for collapse(2) schedule(dynamic)

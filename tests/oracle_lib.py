@@ -16,7 +16,7 @@ class OrcParams(ctypes.Structure):
     _fields_ = [("inviscid_scheme", ctypes.c_int), ("reconstruction", ctypes.c_int),
                 ("viscous_order", ctypes.c_int), ("prandtl", ctypes.c_double), ("cfl", ctypes.c_double),
                 ("entropy_fix_factor", ctypes.c_double), ("weno_eps", ctypes.c_double),
-                ("v_ref", ctypes.c_double)]
+                ("v_ref", ctypes.c_double), ("positive_preserving", ctypes.c_int), ("rho_ref", ctypes.c_double)]
 
 
 _lib = None
@@ -88,8 +88,8 @@ def riemann(scheme, pl, pr, ml, mr, jl=1.0, jr=1.0, efl=0.0, efr=0.0):
     return fc
 
 
-def params(inviscid=3, recon=5, visc=0, prandtl=0.72, cfl=0.5, efix=0.125, eps=1e-40, v_ref=1.0):
-    return OrcParams(inviscid, recon, visc, prandtl, cfl, efix, eps, v_ref)
+def params(inviscid=3, recon=5, visc=0, prandtl=0.72, cfl=0.5, efix=0.125, eps=1e-40, v_ref=1.0, pp=0, rho_ref=1.0):
+    return OrcParams(inviscid, recon, visc, prandtl, cfl, efix, eps, v_ref, pp, rho_ref)
 
 
 def interface_flux(p, q, ml, mr, jl=1.0, jr=1.0, efl=0.0, efr=0.0):
@@ -282,6 +282,7 @@ def _mix_protos(lib):
     lib.orc_mix_strang_step.restype = I
     lib.orc_mix_strang_step.argtypes = [P, D, D, D, I]
     lib.orc_mix_wu_step.argtypes = [P, D, I]
+    lib.orc_mix_set_step.argtypes = [P, D]
     lib.orc_mix_point_flux.argtypes = [ctypes.POINTER(OrcParams), ctypes.POINTER(OrcMech), P, P, P, P, D, D, D, D,
                                        P]
 
@@ -362,6 +363,9 @@ class MixBlock:
 
     def strang_step(self, dt, rtol=1e-6, atol=1e-8, max_attempts=100000):
         return load().orc_mix_strang_step(self._h, dt, rtol, atol, max_attempts)
+
+    def set_step(self, dt):
+        load().orc_mix_set_step(self._h, dt)
 
     def wu_step(self, dt, method=0):
         load().orc_mix_wu_step(self._h, dt, method)
