@@ -38,6 +38,14 @@ struct ChemMech {
   int comp[kMaxSpec][kMaxElem];  // element composition
   char species[kMaxSpec][16];
   char elements[kMaxElem][4];
+  // mixture-averaged transport from tran.dat (Species::read_tran, ChemData.cpp:740-831); has_transport = 1
+  // once read: geometry (0 atom, 1 linear, 2 nonlinear), 1 / (eps / k_B), the Chapman-Enskog viscosity
+  // coefficient, Z_rot(298) F(298); Wilke factors (W_j / W_i)^(1/4) and 1 / sqrt(8 (1 + W_i / W_j)); binary
+  // diffusion coefficient D_jk p sqrt(T)^-3 Omega_D and k_B / eps_jk
+  int has_transport;
+  int geometry[kMaxSpec];
+  double lj_inv[kMaxSpec], vis_coeff[kMaxSpec], zrot[kMaxSpec];
+  double wdw[kMaxSpec][kMaxSpec], sqw[kMaxSpec][kMaxSpec], bdc[kMaxSpec][kMaxSpec], kbe[kMaxSpec][kMaxSpec];
 };
 
 // h_k and cp_k per unit mass at t, linear extrapolation outside [t_low, t_high] (Thermo.cu:470-520).
@@ -57,5 +65,7 @@ inline void host_h_cp(const ChemMech &m, double t, double *h, double *cp) {
 
 // Throws std::runtime_error with a message on malformed input or table overflow.
 ChemMech parse_mechanism(const std::string &mech_text, const std::string &therm_text);
+// Read tran.dat content into m's transport tables (Species::read_tran, ChemData.cpp:740-831).
+void parse_transport(const std::string &tran_text, ChemMech &m);
 
 } // namespace corefl

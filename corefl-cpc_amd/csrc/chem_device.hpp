@@ -209,5 +209,84 @@ template<int NS> __device__ double update_t(const ChemMech &m, double t, const d
   return t;
 }
 
+// Collision integral of binary diffusion at reduced temperature t (Transport.cu:188-190).
+__device__ __forceinline__ double omega_d(double t) { return 1.0548 * pow(t, -0.15504) + pow(t + 0.55909, -2.1705); }
+
+// compute_transport_property (Transport.cu:43-150) at temperature t and mass fractions y (cp: species cp per
+// unit mass at t): Chapman-Enskog species viscosities mixed by Wilke's rule; species conductivities from
+// Eucken's relation with the rotational-relaxation correction of linear and nonlinear molecules, mixed with
+// the 1.065 factor; mixture-averaged rho D_k from the binary coefficients.
+template<int NS>
+__device__ inline void transport(const ChemMech &m, double t, const double *y, const double *cp, double &mu,
+                                 double &lambda, double *rhoD) {
+  constexpr double pi = 3.14159265358979323846, Ru = 8314.462618;
+  double x[NS], vs[NS];
+  double imw = 0;
+#pragma unroll
+  for (int l = 0; l < NS; ++l) imw += y[l] * m.imw[l];
+  const double mwt = 1.0 / imw;
+  const double st = sqrt(t), t32 = t * st;
+#pragma unroll
+  for (int l = 0; l < NS; ++l) {
+    x[l] = y[l] * mwt * m.imw[l];
+    const double td = t * m.lj_inv[l];
+    vs[l] = m.vis_coeff[l] * st / (1.147 * pow(td, -0.145) + pow(td + 0.5, -2.0));
+  }
+  mu = 0;
+  lambda = 0;
+#pragma unroll
+  for (int a = 0; a < NS; ++a) {
+    const double R = Ru * m.imw[a];
+    double lam = 15 * 0.25 * vs[a] * R;
+    const int geom = m.geometry[a];
+    if (geom == 1 || geom == 2) {
+      const double tr = t * m.kbe[a][a], tri = 1 / tr;
+      const double q = sqrt(tri * pi) * pi;
+      const double ft = 1 + 0.5 * q + (2 + 0.25 * pi * pi) * tri + q * tri;
+      const double rd = m.bdc[a][a] * t32 / (omega_d(tr) * R * t);
+      if (geom == 1) {
+        const double ab = (2.5 * vs[a] - rd) / (pi * m.zrot[a] / ft * vs[a] + 10.0 / 3 * vs[a] + 2.0 * rd);
+        lam += -5 * ab * vs[a] * R + rd * cp[a] + rd * (2 * ab - 2.5) * R;
+      } else {
+        const double ab = (2.5 * vs[a] - rd) / (pi * m.zrot[a] / ft * vs[a] + 5 * vs[a] + 2.0 * rd);
+        lam += -7.5 * ab * vs[a] * R + rd * cp[a] + rd * (3 * ab - 2.5) * R;
+      }
+    }
+    double den = 0;
+    const double sa = sqrt(vs[a]);
+#pragma unroll
+    for (int b = 0; b < NS; ++b) {
+      double phi = 1.0;
+      if (a != b) {
+        const double num = 1.0 + sa / sqrt(vs[b]) * m.wdw[a][b];
+        phi = num * num * m.sqw[a][b];
+      }
+      den += phi * x[b];
+    }
+    mu += vs[a] * x[a] / den;
+    lambda += lam * x[a] / (1.065 * den - 0.065 * x[a]);
+  }
+  constexpr double eps = 1e-12;
+  double num[NS], dsum[NS];
+  double sum = 0;
+#pragma unroll
+  for (int l = 0; l < NS; ++l) {
+    num[l] = (x[l] + eps) * m.mw[l];
+    sum += num[l];
+    dsum[l] = 0;
+  }
+#pragma unroll
+  for (int l = 0; l < NS; ++l)
+#pragma unroll
+    for (int n = l + 1; n < NS; ++n) {
+      const double inv = omega_d(t * m.kbe[l][n]) / (m.bdc[l][n] * t32);
+      dsum[l] += (x[n] + eps) * inv;
+      dsum[n] += (x[l] + eps) * inv;
+    }
+  const double f = 1.0 / (t * Ru);
+#pragma unroll
+  for (int l = 0; l < NS; ++l) rhoD[l] = (sum - num[l]) * f / dsum[l];
+}
+
 } // namespace chemdev
 } // namespace corefl

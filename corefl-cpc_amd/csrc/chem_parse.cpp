@@ -336,4 +336,66 @@ ChemMech parse_mechanism(const std::string &mech_text, const std::string &therm_
   return m;
 }
 
+void parse_transport(const std::string &tran_text, ChemMech &m) {
+  const int ns = m.ns;
+  std::vector<double> eps(ns, 0), sig(ns, 0), dip(ns, 0), pol(ns, 0);
+  std::vector<bool> seen(ns, false);
+  int n_read = 0;
+  std::istringstream in(tran_text);
+  std::string line;
+  while (std::getline(in, line)) {
+    for (auto &c : line) c = (char)std::toupper((unsigned char)c);
+    if (line.empty() || line[0] == '!') continue;
+    std::istringstream ls(line);
+    std::string key;
+    if (!(ls >> key)) continue;
+    if (key.rfind("END", 0) == 0 || key.rfind("REAC", 0) == 0) break;
+    int k = -1;
+    for (int q = 0; q < ns; ++q)
+      if (key == m.species[q]) k = q;
+    if (k < 0 || seen[k]) continue; // unknown species, or a second record of a species read already
+    double z = 0;
+    if (!(ls >> m.geometry[k] >> eps[k] >> sig[k] >> dip[k] >> pol[k] >> z))
+      throw std::runtime_error("transport file: bad record for " + key);
+    m.lj_inv[k] = 1.0 / eps[k];
+    m.vis_coeff[k] = 2.6693e-6 * std::sqrt(m.mw[k]) / (sig[k] * sig[k]);
+    const double pi = 3.14159265358979323846, tri = eps[k] / 298;
+    const double f298 = 1 + 0.5 * pi * pi * pi * std::sqrt(tri) + (2 + 0.25 * pi * pi) * tri +
+                        std::sqrt(pi * pi * pi) * std::sqrt(tri * tri * tri);
+    m.zrot[k] = z * f298;
+    seen[k] = true;
+    ++n_read;
+  }
+  if (n_read < ns) throw std::runtime_error("transport file: not every species has transport data");
+  // reduced dipole moment (Debye, Angstrom, K) and the polar / non-polar correction of the pair potential
+  const double kb = 1.380649e-23;
+  auto mu_star = [&](int q) {
+    if (dip[q] == 0) return 0.0;
+    const double conv = 1e-18 * std::sqrt(1e-7) * std::pow(1e+8, 1.5) / std::sqrt(kb);
+    return conv * dip[q] / std::sqrt(eps[q] * sig[q] * sig[q] * sig[q]);
+  };
+  for (int i = 0; i < ns; ++i)
+    for (int j = 0; j < ns; ++j) {
+      m.wdw[i][j] = std::pow(m.mw[j] / m.mw[i], 0.25);
+      m.sqw[i][j] = 1.0 / std::sqrt(8 * (1 + m.mw[i] / m.mw[j]));
+    }
+  for (int j = 0; j < ns; ++j) {
+    m.bdc[j][j] = 0.0188324 / (std::sqrt(m.mw[j] * 0.5) * sig[j] * sig[j]);
+    m.kbe[j][j] = 1.0 / eps[j];
+    for (int k = j + 1; k < ns; ++k) {
+      const double w = m.mw[j] * m.mw[k] / (m.mw[j] + m.mw[k]);
+      double xi = 1;
+      if ((dip[j] != 0) != (dip[k] != 0)) {
+        const int p = dip[j] != 0 ? j : k, n = dip[j] != 0 ? k : j;
+        const double an = pol[n] / std::pow(sig[n], 3), mp = mu_star(p);
+        xi = 1.0 + 0.25 * an * mp * mp * std::sqrt(eps[p] / eps[n]);
+      }
+      const double sjk = 0.5 * (sig[j] + sig[k]) * std::pow(xi, -1.0 / 6.0);
+      m.bdc[j][k] = m.bdc[k][j] = 0.0188324 / (std::sqrt(w) * sjk * sjk);
+      m.kbe[j][k] = m.kbe[k][j] = 1.0 / (xi * xi * std::sqrt(eps[j] * eps[k]));
+    }
+  }
+  m.has_transport = 1;
+}
+
 } // namespace corefl

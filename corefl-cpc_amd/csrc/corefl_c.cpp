@@ -219,7 +219,10 @@ struct corefl_ctx {
   void stage(int rk, hipStream_t s) {
     const bool rhs_only = rk < 0;
     if (scheme.viscous_order > 0) {
-      for (auto &b : blocks) HIPCHK(corefl::launch_viscous_flux(b.dev, scheme, s));
+      for (auto &b : blocks) {
+        if (ns) HIPCHK(corefl::launch_mix_viscous(b.dev, b.mix, scheme, s));
+        else HIPCHK(corefl::launch_viscous_flux(b.dev, scheme, s));
+      }
       viscous_exchange(s);
     }
     for (auto &b : blocks) {
@@ -242,6 +245,7 @@ struct corefl_ctx {
 
   void enqueue_dt(hipStream_t s, int clip) {
     for (auto &b : blocks) {
+      if (ns && b.mix.mul) HIPCHK(corefl::launch_mix_transport(b.dev, b.mix, s));
       if (ns) HIPCHK(corefl::launch_mix_local_dt(b.dev, b.mix, scheme, parts + b.part_offset, s));
       else HIPCHK(corefl::launch_local_dt(b.dev, scheme, parts + b.part_offset, s));
     }
@@ -249,6 +253,7 @@ struct corefl_ctx {
   }
 
   void mix_fluxes(Block &b, hipStream_t s) {
+    if (scheme.viscous_order > 0) HIPCHK(corefl::launch_mix_viscous(b.dev, b.mix, scheme, s));
     if (scheme.inviscid_scheme == 2) HIPCHK(corefl::launch_mix_entropy_fix(b.dev, b.mix, scheme, s));
     for (int d = 0; d < b.dev.dim; ++d) HIPCHK(corefl::launch_mix_inviscid_flux(b.dev, b.mix, scheme, d, s));
   }
@@ -404,7 +409,13 @@ void build(corefl_ctx *ctx, std::vector<corefl::HostBlock> blocks) {
     if (fs && P.get_real("shockSensor_threshold") > 1e-10)
       throw std::runtime_error("shockSensor_threshold > 0 (hybrid UD/WENO scheme) is not on this path");
 
-    if (vo != 0) throw std::runtime_error("viscous mixtures are not on this path (viscous_order must be 0)");
+    if (vo != 0 && vo != 2) throw std::runtime_error("with species, viscous_order must be 0 or 2 (face-centred)");
+    if (vo == 2 && P.get_bool("gradPInDiffusionFlux"))
+      throw std::runtime_error("gradPInDiffusionFlux (pressure-driven species diffusion) is not on this path");
+    if (vo == 2) {
+      // Species::read_tran (ChemData.cpp:90-98, 740-831)
+      corefl::parse_transport(read_file(path(P.get_string("transport_file"))), ctx->chem->mech);
+    }
     if (reaction && temporal != 3 && temporal != 4 && temporal != 5)
       throw std::runtime_error("reacting flow: temporal_scheme 3 (explicit source), 4 (Wu's splitting) or 5 (Strang "
                                "splitting)");
@@ -552,9 +563,14 @@ void build(corefl_ctx *ctx, std::vector<corefl::HostBlock> blocks) {
       if (ctx->strang) B.hstep = ctx->alloc(d.n);
     }
     if (vo > 0) {
-      d.fv = ctx->alloc(5 * sz);
-      d.gv = ctx->alloc(5 * sz);
-      d.hv = ctx->alloc(5 * sz);
+      d.fv = ctx->alloc(nv * sz);
+      d.gv = ctx->alloc(nv * sz);
+      d.hv = ctx->alloc(nv * sz);
+      if (ctx->ns) {
+        B.mix.mul = ctx->alloc(sz);
+        B.mix.cond = ctx->alloc(sz);
+        B.mix.rhod = ctx->alloc(ctx->ns * sz);
+      }
     }
     if (inv == 2) d.efd = ctx->alloc(sz);
     d.dq = ctx->alloc(nv * d.n);

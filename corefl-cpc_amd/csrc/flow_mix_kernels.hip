@@ -498,6 +498,108 @@ __constant__ double kRkC[3] = {1.0, 0.25, 2.0 / 3.0};
 
 // dq = -sum_dir (F(i+1/2) - F(i-1/2)) and update_cv_and_bv_rk<Mixture> (RK.cuh:224-280) with
 // compute_temperature_and_pressure (FieldOperation.cu:4-45).  Stage 0 saves U^n into qn.
+// Transport properties of every node (update_physical_properties<Mixture> with compute_transport_property).
+__global__ __launch_bounds__(256) void k_mix_transport(DevBlock b, MixArrays x) {
+  const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= b.sz) return;
+  const long sz = b.sz;
+  const double t = b.bv[id + 5 * sz];
+  if (!(t > 0)) return; // unused corner nodes hold no state
+  double y[NS], h[NS], cp[NS], rd[NS], mu, lam;
+#pragma unroll
+  for (int l = 0; l < NS; ++l) y[l] = x.sv[id + l * sz];
+  chemdev::h_cp<NS>(*x.mech, t, h, cp);
+  chemdev::transport<NS>(*x.mech, t, y, cp, mu, lam, rd);
+  x.mul[id] = mu;
+  x.cond[id] = lam;
+#pragma unroll
+  for (int l = 0; l < NS; ++l) x.rhod[id + l * sz] = rd[l];
+}
+
+// compute_fv/gv/hv_2nd_order<Mixture> (ViscousScheme.cu:614-778 and the gv/hv analogues): the face flux of the
+// air scheme with the nodes' average viscosity and conductivity, plus species diffusion with the correction
+// velocity (mixture-averaged rho D_k, mass fractions and their gradients at the face) and its enthalpy flux;
+// stored at node t for the face t+1/2 along DIR, t = -1 .. m-1, components 1..4 and 5..5+NS-1.
+template<int DIR>
+__global__ __launch_bounds__(256) void k_mix_viscous_face(DevBlock b, MixArrays x, Scheme sc) {
+  int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+  if constexpr (DIR == 0) i -= 1;
+  else if constexpr (DIR == 1) j -= 1;
+  else k -= 1;
+  if (i >= b.mx || j >= b.my) return;
+  const long sz = b.sz;
+  const long st[3] = {1, b.d2, b.d1};
+  const long id = nid(b, i, j, k), ie = id + st[DIR];
+  const double *M = b.met;
+  double m[9];
+#pragma unroll
+  for (int c = 0; c < 9; ++c) m[c] = 0.5 * (M[id + c * sz] + M[ie + c * sz]);
+  auto grad = [&](const double *f, double *gx) { // d f / d(x, y, z) at the face
+    double g[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      if (a == DIR) g[a] = f[ie] - f[id];
+      else if (a == 2 && b.dim == 2) g[a] = 0;
+      else g[a] = 0.25 * (f[id + st[a]] - f[id - st[a]] + f[ie + st[a]] - f[ie - st[a]]);
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) gx[c] = g[0] * m[c] + g[1] * m[3 + c] + g[2] * m[6 + c];
+  };
+  double du[3], dv[3], dw[3], dT[3];
+  grad(b.bv + sz, du);
+  grad(b.bv + 2 * sz, dv);
+  grad(b.bv + 3 * sz, dw);
+  grad(b.bv + 5 * sz, dT);
+  const double mu = 0.5 * (x.mul[id] + x.mul[ie]);
+  const double txx = mu * (4 * du[0] - 2 * dv[1] - 2 * dw[2]) / 3.0;
+  const double tyy = mu * (4 * dv[1] - 2 * du[0] - 2 * dw[2]) / 3.0;
+  const double tzz = mu * (4 * dw[2] - 2 * du[0] - 2 * dv[1]) / 3.0;
+  const double txy = mu * (du[1] + dv[0]), txz = mu * (du[2] + dw[0]), tyz = mu * (dv[2] + dw[1]);
+  double kj[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+    kj[c] = 0.5 * (M[id + (3 * DIR + c) * sz] * b.jac[id] + M[ie + (3 * DIR + c) * sz] * b.jac[ie]);
+  const double f1 = kj[0] * txx + kj[1] * txy + kj[2] * txz;
+  const double f2 = kj[0] * txy + kj[1] * tyy + kj[2] * tyz;
+  const double f3 = kj[0] * txz + kj[1] * tyz + kj[2] * tzz;
+  const double um = 0.5 * (b.bv[id + sz] + b.bv[ie + sz]);
+  const double vm = 0.5 * (b.bv[id + 2 * sz] + b.bv[ie + 2 * sz]);
+  const double wm = 0.5 * (b.bv[id + 3 * sz] + b.bv[ie + 3 * sz]);
+  const double lam = 0.5 * (x.cond[id] + x.cond[ie]);
+  double e = um * f1 + vm * f2 + wm * f3 + lam * (kj[0] * dT[0] + kj[1] * dT[1] + kj[2] * dT[2]);
+  // species diffusion (ViscousScheme.cu:697-778; no pressure-gradient term)
+  const ChemMech &mc = *x.mech;
+  double yk[NS], dk[NS], gy[NS];
+  double sgw = 0, srdy = 0, cvt = 0, imw = 0;
+#pragma unroll
+  for (int l = 0; l < NS; ++l) {
+    yk[l] = 0.5 * (x.sv[id + l * sz] + x.sv[ie + l * sz]);
+    dk[l] = 0.5 * (x.rhod[id + l * sz] + x.rhod[ie + l * sz]);
+    double gY[3];
+    grad(x.sv + l * sz, gY);
+    gy[l] = kj[0] * gY[0] + kj[1] * gY[1] + kj[2] * gY[2];
+    cvt += dk[l] * gy[l];
+    sgw += gy[l] * mc.imw[l];
+    imw += yk[l] * mc.imw[l];
+    srdy += dk[l] * yk[l];
+  }
+  const double mwt = 1.0 / imw;
+  cvt -= mwt * srdy * sgw;
+  double hh[NS], cp[NS];
+  chemdev::h_cp<NS>(mc, 0.5 * (b.bv[id + 5 * sz] + b.bv[ie + 5 * sz]), hh, cp);
+  double *F = DIR == 0 ? b.fv : (DIR == 1 ? b.gv : b.hv);
+#pragma unroll
+  for (int l = 0; l < NS; ++l) {
+    const double jd = dk[l] * (gy[l] - mwt * yk[l] * sgw) - yk[l] * cvt;
+    F[id + (5 + l) * sz] = jd;
+    e += hh[l] * jd;
+  }
+  F[id + sz] = f1;
+  F[id + 2 * sz] = f2;
+  F[id + 3 * sz] = f3;
+  F[id + 4 * sz] = e;
+}
+
 // Species clipping and renormalisation of q, then the Newton temperature at the internal energy of q
 // (update_cv_and_bv_rk<Mixture> tail, RK.cuh:224-280; compute_temperature_and_pressure,
 // FieldOperation.cu:4-45); returns T, y the mass fractions, R the mixture gas constant.
@@ -554,6 +656,15 @@ __global__ __launch_bounds__(256) void k_mix_update(DevBlock b, MixArrays x, int
     r[l] = s;
   }
   const long iq = ((long)k * b.my + j) * b.mx + i;
+  if (x.mul) { // face-centred viscous fluxes (compute_dFv_dx etc., ViscousScheme.cu:75-128)
+#pragma unroll
+    for (int l = 1; l < NV; ++l) {
+      double v = b.fv[id + l * sz] - b.fv[id - 1 + l * sz];
+      v += b.gv[id + l * sz] - b.gv[id - b.d2 + l * sz];
+      if (b.dim == 3) v += b.hv[id + l * sz] - b.hv[id - b.d1 + l * sz];
+      r[l] += v;
+    }
+  }
   if (x.explicit_source) {
     // finite_rate_chemistry with chemSrcMethod 0 (FiniteRateChem.cu:1388-1464): dq += J omega and the
     // reaction time scale min_k rho Y_k / omega_d,k
@@ -631,6 +742,23 @@ __global__ __launch_bounds__(256) void k_mix_local_dt(DevBlock b, MixArrays x, S
     double s = fmax(fabs(u * M[0] + v * M[sz] + w * M[2 * sz]) + c * gx,
                     fabs(u * M[3 * sz] + v * M[4 * sz] + w * M[5 * sz]) + c * ge);
     if (b.dim == 3) s = fmax(s, fabs(u * M[6 * sz] + v * M[7 * sz] + w * M[8 * sz]) + c * gz);
+    if (x.mul) { // viscous spectral radius (TimeAdvanceFunc.cuh:66-98)
+      double ml = fmax(gx, ge);
+      if (b.dim == 3) ml = fmax(ml, gz);
+      double h[NS], cp[NS], cpt = 0, rmax = 0, R = 0;
+      chemdev::h_cp<NS>(*x.mech, b.bv[id + 5 * sz], h, cp);
+#pragma unroll
+      for (int l = 0; l < NS; ++l) {
+        cpt += cp[l] * x.sv[id + l * sz];
+        R += x.sv[id + l * sz] * kRu * x.mech->imw[l];
+        rmax = fmax(rmax, x.rhod[id + l * sz]);
+      }
+      const double ir = 1.0 / b.bv[id], gam = cpt / (cpt - R);
+      double dv = x.mul[id] * ir;
+      dv = fmax(dv, x.cond[id] * ir * gam / cpt);
+      dv = fmax(dv, rmax * ir);
+      s = fmax(s, ml * ml * dv);
+    }
     dt = sc.cfl / s;
     if (x.explicit_source) {
       const double rdt = sc.cfl * x.tscale[((long)k * b.my + j) * b.mx + i];
@@ -1048,6 +1176,22 @@ __global__ __launch_bounds__(256) void k_mix_wu_reaction(DevBlock b, MixArrays x
 #pragma unroll
   for (int l = 0; l < NS; ++l) cpt += cp[l] * y[l];
   x.snd[id] = sqrt(cpt / (cpt - R) * R * t);
+}
+
+hipError_t launch_mix_transport(const DevBlock &b, const MixArrays &x, hipStream_t s) {
+  if (x.ns != NS) return hipErrorNotSupported;
+  k_mix_transport<<<(unsigned)((b.sz + 255) / 256), 256, 0, s>>>(b, x);
+  return hipGetLastError();
+}
+
+hipError_t launch_mix_viscous(const DevBlock &b, const MixArrays &x, const Scheme &sc, hipStream_t s) {
+  if (x.ns != NS) return hipErrorNotSupported;
+  k_mix_transport<<<(unsigned)((b.sz + 255) / 256), 256, 0, s>>>(b, x);
+  const dim3 tpb(64, 4, 1);
+  k_mix_viscous_face<0><<<dim3((b.mx + 1 + 63) / 64, (b.my + 3) / 4, b.mz), tpb, 0, s>>>(b, x, sc);
+  k_mix_viscous_face<1><<<dim3((b.mx + 63) / 64, (b.my + 1 + 3) / 4, b.mz), tpb, 0, s>>>(b, x, sc);
+  if (b.dim == 3) k_mix_viscous_face<2><<<dim3((b.mx + 63) / 64, (b.my + 3) / 4, b.mz + 1), tpb, 0, s>>>(b, x, sc);
+  return hipGetLastError();
 }
 
 hipError_t launch_mix_wu_reaction(const DevBlock &b, const MixArrays &x, const double *dt_dev, int method,

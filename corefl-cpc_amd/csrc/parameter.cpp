@@ -32,6 +32,8 @@ Parameter::Parameter() {
   ints["chem_advance_method"] = 0; // Wu's splitting reaction sub-steps (Parameter.cpp:431)
   strings["mechanism_file"] = "chemistry/mechanism.inp";
   strings["therm_file"] = "chemistry/therm.dat";
+  strings["transport_file"] = "chemistry/tran.dat"; // Parameter.cpp:401
+  bools["gradPInDiffusionFlux"] = false;            // Parameter.cpp:382
   reals["chemistry_rtol"] = 1e-6; // ROK4E tolerances of the split chemistry sub-steps (new)
   reals["chemistry_atol"] = 1e-8;
   ints["chemistry_max_attempts"] = 100000; // per-cell step-attempt limit of a split chemistry sub-step (new)

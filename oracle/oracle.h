@@ -170,6 +170,19 @@ void orc_mix_rk3_step(orc_mix *b, double dt);
 void orc_mix_set_step(orc_mix *b, double dt);
 int orc_mix_chem_substep(orc_mix *b, double dt, double rtol, double atol, int max_attempts);
 int orc_mix_strang_step(orc_mix *b, double dt, double rtol, double atol, int max_attempts);
+/* Transport data of tran.dat per species (geometry 0/1/2, eps/k_B in K, sigma in Angstrom, dipole moment in
+ * Debye, polarizability in Angstrom^3, Z_rot at 298 K). */
+typedef struct {
+  int geometry[ORC_MAX_SPEC];
+  double eps_kb[ORC_MAX_SPEC], sigma[ORC_MAX_SPEC], dipole[ORC_MAX_SPEC], polar[ORC_MAX_SPEC], zrot[ORC_MAX_SPEC];
+} orc_tran;
+/* Mixture viscosity, conductivity and mixture-averaged rho D_k at T and mass fractions Y
+ * (compute_transport_property, Transport.cu:43-150, with the coefficients of Species::read_tran,
+ * ChemData.cpp:740-831, derived here from the raw data). */
+void orc_transport(const orc_mech *m, const orc_tran *t, double T, const double *Y, double *mu, double *lambda,
+                   double *rhoD);
+/* Viscous mixtures (viscous_order 2): the transport data the face-centred viscous fluxes use. */
+void orc_mix_set_transport(orc_mix *b, const orc_tran *t);
 /* Wu's splitting (temporal_scheme 4), one step: chem_advance_method 0 (forward Euler) or 3 (SSP-RK3 stages). */
 void orc_mix_wu_step(orc_mix *b, double dt, int method);
 /* One interface flux of the mixture Riemann solver params->inviscid_scheme from primitive states

@@ -36,6 +36,9 @@ struct orc_mix {
   double *cv, *qn, *bv, *sv, *c, *efd, *dq, *jac, *met, *hstep, *tscale;
   double *cn, *wu_ts; /* Wu's splitting: frozen transport residual, reaction time scale (starts at zero) */
   double pp_dt;       /* step size seen by the positivity limiter (the global dt of the current step) */
+  int viscous;        /* viscous_order 2 with transport data set */
+  orc_tran tran;
+  double *mul, *cond, *rhod, *fvis[3];
 };
 
 #define ID(b, i, j, k) ((long)((k) + (b)->g) * (b)->d1 + (long)((j) + (b)->g) * (b)->d2 + ((i) + (b)->g))
@@ -99,6 +102,8 @@ void orc_mix_destroy(orc_mix *b) {
   if (!b) return;
   free(b->cv); free(b->qn); free(b->bv); free(b->sv); free(b->c); free(b->efd); free(b->dq); free(b->hstep);
   free(b->tscale); free(b->cn); free(b->wu_ts);
+  free(b->mul); free(b->cond); free(b->rhod);
+  for (int d = 0; d < 3; ++d) free(b->fvis[d]);
   free(b->jac); free(b->met);
   free(b);
 }
@@ -812,8 +817,210 @@ static void explicit_source(orc_mix *b) {
 
 void orc_mix_set_explicit_source(orc_mix *b, int on) { b->explicit_source = on; }
 
+static double omega_d(double t) { return 1.0548 * pow(t, -0.15504) + pow(t + 0.55909, -2.1705); }
+
+void orc_transport(const orc_mech *m, const orc_tran *tr, double T, const double *Y, double *mu_out,
+                   double *lambda_out, double *rhoD) {
+  const int ns = m->ns;
+  const double pi = 3.14159265358979323846, kb = 1.380649e-23;
+  /* coefficients (ChemData.cpp:779-831) */
+  double vc[MS], wdw[MS][MS], sqw[MS][MS], bdc[MS][MS], kbe[MS][MS], zr[MS];
+  for (int k = 0; k < ns; ++k) {
+    vc[k] = 2.6693e-6 * sqrt(m->mw[k]) / (tr->sigma[k] * tr->sigma[k]);
+    const double ti = tr->eps_kb[k] / 298;
+    zr[k] = tr->zrot[k] * (1 + 0.5 * pi * pi * pi * sqrt(ti) + (2 + 0.25 * pi * pi) * ti + sqrt(pi * pi * pi) * sqrt(ti * ti * ti));
+    for (int j = 0; j < ns; ++j) {
+      wdw[k][j] = pow(m->mw[j] / m->mw[k], 0.25);
+      sqw[k][j] = 1.0 / sqrt(8 * (1 + m->mw[k] / m->mw[j]));
+    }
+  }
+  for (int j = 0; j < ns; ++j) {
+    bdc[j][j] = 0.0188324 / (sqrt(m->mw[j] * 0.5) * tr->sigma[j] * tr->sigma[j]);
+    kbe[j][j] = 1.0 / tr->eps_kb[j];
+    for (int k = j + 1; k < ns; ++k) {
+      const double W = m->mw[j] * m->mw[k] / (m->mw[j] + m->mw[k]);
+      double xi = 1;
+      if ((tr->dipole[j] != 0) != (tr->dipole[k] != 0)) {
+        const int p = tr->dipole[j] != 0 ? j : k, n = tr->dipole[j] != 0 ? k : j;
+        const double conv = 1e-18 * sqrt(1e-7) * pow(1e+8, 1.5) / sqrt(kb);
+        const double mp = conv * tr->dipole[p] / sqrt(tr->eps_kb[p] * pow(tr->sigma[p], 3));
+        xi = 1.0 + 0.25 * tr->polar[n] / pow(tr->sigma[n], 3) * mp * mp * sqrt(tr->eps_kb[p] / tr->eps_kb[n]);
+      }
+      const double sg = 0.5 * (tr->sigma[j] + tr->sigma[k]) * pow(xi, -1.0 / 6.0);
+      bdc[j][k] = bdc[k][j] = 0.0188324 / (sqrt(W) * sg * sg);
+      kbe[j][k] = kbe[k][j] = 1.0 / (xi * xi * sqrt(tr->eps_kb[j] * tr->eps_kb[k]));
+    }
+  }
+  /* properties (Transport.cu:62-150) */
+  double h[MS], cp[MS], X[MS], vs[MS];
+  orc_chem_h_cp(m, T, h, cp);
+  double imw = 0;
+  for (int l = 0; l < ns; ++l) imw += Y[l] / m->mw[l];
+  const double mwt = 1.0 / imw, st = sqrt(T), t32 = T * st;
+  for (int l = 0; l < ns; ++l) {
+    X[l] = Y[l] * mwt / m->mw[l];
+    const double td = T / tr->eps_kb[l];
+    vs[l] = vc[l] * st / (1.147 * pow(td, -0.145) + pow(td + 0.5, -2));
+  }
+  double mu = 0, lambda = 0;
+  for (int a = 0; a < ns; ++a) {
+    const double R = kRu / m->mw[a];
+    double lam = 15 * 0.25 * vs[a] * R;
+    if (tr->geometry[a] == 1 || tr->geometry[a] == 2) {
+      const double trd = T * kbe[a][a], tri = 1 / trd, q = sqrt(tri * pi) * pi;
+      const double ft = 1 + 0.5 * q + (2 + 0.25 * pi * pi) * tri + q * tri;
+      const double rd = bdc[a][a] * t32 / (omega_d(trd) * R * T);
+      if (tr->geometry[a] == 1) {
+        const double ab = (2.5 * vs[a] - rd) / (pi * zr[a] / ft * vs[a] + 10.0 / 3 * vs[a] + 2.0 * rd);
+        lam += -5 * ab * vs[a] * R + rd * cp[a] + rd * (2 * ab - 2.5) * R;
+      } else {
+        const double ab = (2.5 * vs[a] - rd) / (pi * zr[a] / ft * vs[a] + 5 * vs[a] + 2.0 * rd);
+        lam += -7.5 * ab * vs[a] * R + rd * cp[a] + rd * (3 * ab - 2.5) * R;
+      }
+    }
+    double den = 0;
+    for (int c = 0; c < ns; ++c) {
+      double phi = 1.0;
+      if (c != a) {
+        const double num = 1.0 + sqrt(vs[a] / vs[c]) * wdw[a][c];
+        phi = num * num * sqw[a][c];
+      }
+      den += phi * X[c];
+    }
+    mu += vs[a] * X[a] / den;
+    lambda += lam * X[a] / (1.065 * den - 0.065 * X[a]);
+  }
+  *mu_out = mu;
+  *lambda_out = lambda;
+  const double eps = 1e-12;
+  double num[MS], dsum[MS], sum = 0;
+  for (int l = 0; l < ns; ++l) {
+    num[l] = (X[l] + eps) * m->mw[l];
+    sum += num[l];
+    dsum[l] = 0;
+  }
+  for (int l = 0; l < ns; ++l)
+    for (int n = l + 1; n < ns; ++n) {
+      const double inv = omega_d(T * kbe[l][n]) / (bdc[l][n] * t32);
+      dsum[l] += (X[n] + eps) * inv;
+      dsum[n] += (X[l] + eps) * inv;
+    }
+  for (int l = 0; l < ns; ++l) rhoD[l] = (sum - num[l]) / (T * kRu) / dsum[l];
+}
+
+void orc_mix_set_transport(orc_mix *b, const orc_tran *t) {
+  b->tran = *t;
+  b->viscous = b->p.viscous_order == 2;
+  if (!b->viscous || b->mul) return;
+  b->mul = calloc((size_t)b->sz, sizeof(double));
+  b->cond = calloc((size_t)b->sz, sizeof(double));
+  b->rhod = calloc((size_t)b->ns * b->sz, sizeof(double));
+  for (int d = 0; d < 3; ++d) b->fvis[d] = calloc((size_t)b->nv * b->sz, sizeof(double));
+}
+
+/* compute_fv/gv/hv_2nd_order<Mixture> (ViscousScheme.cu:614-778): face t+1/2 along dir stored at node t */
+static void mix_viscous_faces(orc_mix *b, int dir) {
+  const long s = b->sz;
+  const int ns = b->ns;
+  const long st[3] = {1, b->d2, b->d1};
+  const int ext[3] = {b->mx, b->my, b->mz};
+  int lo[3] = {0, 0, 0};
+  lo[dir] = -1;
+  double *F = b->fvis[dir];
+#pragma omp parallel for collapse(2)
+  for (int k = lo[2]; k < ext[2]; ++k)
+    for (int j = lo[1]; j < ext[1]; ++j)
+      for (int i = lo[0]; i < ext[0]; ++i) {
+        const long id = ID(b, i, j, k), ie = id + st[dir];
+        const double *M = b->met;
+        double m9[9];
+        for (int c = 0; c < 9; ++c) m9[c] = 0.5 * (M[id + c * s] + M[ie + c * s]);
+        double gq[4 + MS][3];
+        for (int q = 0; q < 4 + ns; ++q) {
+          const double *f = q < 3 ? b->bv + (1 + q) * s : (q == 3 ? b->bv + 5 * s : b->sv + (q - 4) * s);
+          double g[3];
+          for (int a = 0; a < 3; ++a) {
+            if (a == dir) g[a] = f[ie] - f[id];
+            else if (a == 2 && b->dim == 2) g[a] = 0;
+            else g[a] = 0.25 * (f[id + st[a]] - f[id - st[a]] + f[ie + st[a]] - f[ie - st[a]]);
+          }
+          for (int c = 0; c < 3; ++c) gq[q][c] = g[0] * m9[c] + g[1] * m9[3 + c] + g[2] * m9[6 + c];
+        }
+        const double mu = 0.5 * (b->mul[id] + b->mul[ie]);
+        const double txx = mu * (4 * gq[0][0] - 2 * gq[1][1] - 2 * gq[2][2]) / 3.0;
+        const double tyy = mu * (4 * gq[1][1] - 2 * gq[0][0] - 2 * gq[2][2]) / 3.0;
+        const double tzz = mu * (4 * gq[2][2] - 2 * gq[0][0] - 2 * gq[1][1]) / 3.0;
+        const double txy = mu * (gq[0][1] + gq[1][0]), txz = mu * (gq[0][2] + gq[2][0]), tyz = mu * (gq[1][2] + gq[2][1]);
+        double kj[3];
+        for (int c = 0; c < 3; ++c)
+          kj[c] = 0.5 * (M[id + (3 * dir + c) * s] * b->jac[id] + M[ie + (3 * dir + c) * s] * b->jac[ie]);
+        const double f1 = kj[0] * txx + kj[1] * txy + kj[2] * txz;
+        const double f2 = kj[0] * txy + kj[1] * tyy + kj[2] * tyz;
+        const double f3 = kj[0] * txz + kj[1] * tyz + kj[2] * tzz;
+        const double um = 0.5 * (b->bv[id + s] + b->bv[ie + s]);
+        const double vm = 0.5 * (b->bv[id + 2 * s] + b->bv[ie + 2 * s]);
+        const double wm = 0.5 * (b->bv[id + 3 * s] + b->bv[ie + 3 * s]);
+        const double lam = 0.5 * (b->cond[id] + b->cond[ie]);
+        double e = um * f1 + vm * f2 + wm * f3 + lam * (kj[0] * gq[3][0] + kj[1] * gq[3][1] + kj[2] * gq[3][2]);
+        double yk[MS], dk[MS], gy[MS], sgw = 0, srdy = 0, cvt = 0, imw = 0;
+        for (int l = 0; l < ns; ++l) {
+          yk[l] = 0.5 * (b->sv[id + l * s] + b->sv[ie + l * s]);
+          dk[l] = 0.5 * (b->rhod[id + l * s] + b->rhod[ie + l * s]);
+          gy[l] = kj[0] * gq[4 + l][0] + kj[1] * gq[4 + l][1] + kj[2] * gq[4 + l][2];
+          cvt += dk[l] * gy[l];
+          sgw += gy[l] / b->m.mw[l];
+          imw += yk[l] / b->m.mw[l];
+          srdy += dk[l] * yk[l];
+        }
+        const double mwt = 1.0 / imw;
+        cvt -= mwt * srdy * sgw;
+        double hh[MS], cp[MS];
+        orc_chem_h_cp(&b->m, 0.5 * (b->bv[id + 5 * s] + b->bv[ie + 5 * s]), hh, cp);
+        for (int l = 0; l < ns; ++l) {
+          const double jd = dk[l] * (gy[l] - mwt * yk[l] * sgw) - yk[l] * cvt;
+          F[id + (5 + l) * s] = jd;
+          e += hh[l] * jd;
+        }
+        F[id + s] = f1;
+        F[id + 2 * s] = f2;
+        F[id + 3 * s] = f3;
+        F[id + 4 * s] = e;
+      }
+}
+
+static void mix_transport(orc_mix *b) {
+  const long s = b->sz;
+#pragma omp parallel for
+  for (long id = 0; id < s; ++id) {
+    const double T = b->bv[id + 5 * s];
+    if (!(T > 0)) continue;
+    double Y[MS], rd[MS];
+    for (int l = 0; l < b->ns; ++l) Y[l] = b->sv[id + l * s];
+    orc_transport(&b->m, &b->tran, T, Y, &b->mul[id], &b->cond[id], rd);
+    for (int l = 0; l < b->ns; ++l) b->rhod[id + l * s] = rd[l];
+  }
+}
+
+static void mix_viscous(orc_mix *b) {
+  const long s = b->sz;
+  mix_transport(b);
+  const long st[3] = {1, b->d2, b->d1};
+  for (int dir = 0; dir < b->dim; ++dir) {
+    mix_viscous_faces(b, dir);
+    const double *F = b->fvis[dir];
+#pragma omp parallel for collapse(2)
+    for (int k = 0; k < b->mz; ++k)
+      for (int j = 0; j < b->my; ++j)
+        for (int i = 0; i < b->mx; ++i) {
+          const long id = ID(b, i, j, k), iq = IDQ(b, i, j, k);
+          for (int l = 1; l < b->nv; ++l) b->dq[iq + l * b->n] += F[id + l * s] - F[id - st[dir] + l * s];
+        }
+  }
+}
+
 void orc_mix_rhs(orc_mix *b) {
   memset(b->dq, 0, sizeof(double) * b->nv * b->n);
+  if (b->viscous) mix_viscous(b);
   if (b->p.inviscid_scheme == 2) entropy_fix(b);
   inviscid_dir(b, 0);
   inviscid_dir(b, 1);
@@ -886,6 +1093,7 @@ static void update(orc_mix *b, double dt, int rk) {
 
 double orc_mix_cfl_dt(orc_mix *b) {
   const long s = b->sz;
+  if (b->viscous) mix_transport(b); /* the properties of the current state (update_physical_properties) */
   double dtmin = 1e+6;
 #pragma omp parallel for collapse(2) reduction(min : dtmin)
   for (int k = 0; k < b->mz; ++k)
@@ -905,6 +1113,22 @@ double orc_mix_cfl_dt(orc_mix *b) {
         const double c = b->c[id];
         double sr = fmax(fabs(U) + c * gx, fabs(V) + c * ge);
         if (b->dim == 3) sr = fmax(sr, fabs(W) + c * gz);
+        if (b->viscous) { /* viscous spectral radius, TimeAdvanceFunc.cuh:66-98 */
+          double ml = fmax(gx, ge);
+          if (b->dim == 3) ml = fmax(ml, gz);
+          double h[MS], cpl[MS], cpt = 0, R = 0, rmax = 0;
+          orc_chem_h_cp(&b->m, bv[id + 5 * s], h, cpl);
+          for (int l = 0; l < b->ns; ++l) {
+            cpt += cpl[l] * b->sv[id + l * s];
+            R += b->sv[id + l * s] * kRu / b->m.mw[l];
+            rmax = fmax(rmax, b->rhod[id + l * s]);
+          }
+          const double ir = 1.0 / bv[id], gam = cpt / (cpt - R);
+          double dv = b->mul[id] * ir;
+          dv = fmax(dv, b->cond[id] * ir * gam / cpt);
+          dv = fmax(dv, rmax * ir);
+          sr = fmax(sr, ml * ml * dv);
+        }
         double dt = b->p.cfl / sr;
         if (b->explicit_source) { /* TimeAdvanceFunc.cuh:104-111 */
           const double rdt = b->p.cfl * b->tscale[IDQ(b, i, j, k)];

@@ -248,9 +248,10 @@ def write_input_dir(case, path):
 
 
 def reactive_box(n=17, recon=4, inviscid=2, temporal=5, reaction=1, cfl=0.5, length=1e-2, amp=0.0,
-                 fixed_dt=None):
+                 fixed_dt=None, visc=0):
     """Periodic box of H2/air (Li 2004 mechanism), side `length` metres, n nodes per direction with
-    coincident end nodes; the initial field comes from reactive_state() through set_state."""
+    coincident end nodes; the initial field comes from reactive_state() through set_state.  visc=2 adds the
+    mixture's viscous fluxes (transport data of tests/golden/tran.dat)."""
     import chem_cases as CC
     X, Y, Z = wavy_grid(n, amp)
     scale = length / (2 * np.pi)
@@ -271,7 +272,8 @@ int temporal_scheme = {temporal}
 real cfl = {cfl}
 int reconstruction = {recon}
 int inviscid_scheme = {inviscid}
-int viscous_order = 0
+int viscous_order = {visc}
+string transport_file = {CC.TRAN_FILE}
 array string boundary_conditions {{
     periodic
 }}
@@ -292,7 +294,7 @@ struct fresh {{
 """
     faces = box_faces(n, n, n, 8)
     return dict(setup=setup, grid=(X, Y, Z), scale=scale, faces=faces, boundary=boundary_text(faces),
-                scheme=(inviscid, recon, 0), cfl=cfl, n=n)
+                scheme=(inviscid, recon, visc), cfl=cfl, n=n)
 
 
 def reactive_state(n, ngg, seed=3, n_kernels=2, peak=(1800.0, 2400.0), radius=(2.0, 4.0), u_rms=20.0,

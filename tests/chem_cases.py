@@ -7,6 +7,7 @@ import numpy as np
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 MECH = os.path.join(GOLDEN, "h2_li2004.inp")
 THERM = os.path.join(GOLDEN, "therm_h2o2n2.dat")
+TRAN_FILE = os.path.join(GOLDEN, "tran.dat")  # transport data of the H2/O2/N2 case (example/constantVolumeReactor/input/chemistry/tran.dat)
 RU, PATM = 8314.462618, 101325.0
 MW = np.array([2.016, 1.008, 31.998, 15.999, 17.007, 33.006, 34.014, 18.015, 28.014])
 # stoichiometric H2/air mole fractions of the reference reactor example

@@ -212,9 +212,48 @@ def mech_struct(d):
     return m
 
 
+class OrcTran(ctypes.Structure):
+    _fields_ = [("geometry", ctypes.c_int * MAXS), ("eps_kb", ctypes.c_double * MAXS),
+                ("sigma", ctypes.c_double * MAXS), ("dipole", ctypes.c_double * MAXS),
+                ("polar", ctypes.c_double * MAXS), ("zrot", ctypes.c_double * MAXS)]
+
+
+def tran_struct(species, tran_text):
+    """OrcTran for the species list from tran.dat content: the first record of each species (upper case),
+    up to END / REACTIONS, as Species::read_tran reads it (ChemData.cpp:740-790)."""
+    t = OrcTran()
+    seen = set()
+    for line in tran_text.upper().splitlines():
+        if not line.strip() or line.startswith("!"):
+            continue
+        tok = line.split()
+        if tok[0].startswith("END") or tok[0].startswith("REAC"):
+            break
+        if tok[0] not in species or tok[0] in seen:
+            continue
+        k = species.index(tok[0])
+        t.geometry[k] = int(tok[1])
+        t.eps_kb[k], t.sigma[k], t.dipole[k], t.polar[k], t.zrot[k] = (float(v) for v in tok[2:7])
+        seen.add(tok[0])
+    assert len(seen) == len(species), "transport data missing"
+    return t
+
+
+def transport(m, t, T, Y):
+    """(mu, lambda, rho D_k) of the oracle at temperature T and mass fractions Y."""
+    Y = np.ascontiguousarray(Y, float)
+    mu, lam = ctypes.c_double(), ctypes.c_double()
+    rd = np.zeros(m.ns)
+    lib = load()
+    _chem_protos(lib)
+    lib.orc_transport(ctypes.byref(m), ctypes.byref(t), T, _p(Y), ctypes.byref(mu), ctypes.byref(lam), _p(rd))
+    return mu.value, lam.value, rd
+
+
 def _chem_protos(lib):
     D, I, P = ctypes.c_double, ctypes.c_int, ctypes.c_void_p
     M = ctypes.POINTER(OrcMech)
+    lib.orc_transport.argtypes = [M, ctypes.POINTER(OrcTran), D, P, ctypes.POINTER(D), ctypes.POINTER(D), P]
     lib.orc_chem_h_cp.argtypes = [M, D, P, P]
     lib.orc_chem_source.argtypes = [M, D, P, P, P, P, P]
     lib.orc_chem_jacobian.argtypes = [M, P, P, P, P]
@@ -282,6 +321,7 @@ def _mix_protos(lib):
     lib.orc_mix_strang_step.restype = I
     lib.orc_mix_strang_step.argtypes = [P, D, D, D, I]
     lib.orc_mix_wu_step.argtypes = [P, D, I]
+    lib.orc_mix_set_transport.argtypes = [P, ctypes.POINTER(OrcTran)]
     lib.orc_mix_set_step.argtypes = [P, D]
     lib.orc_mix_point_flux.argtypes = [ctypes.POINTER(OrcParams), ctypes.POINTER(OrcMech), P, P, P, P, D, D, D, D,
                                        P]
@@ -363,6 +403,10 @@ class MixBlock:
 
     def strang_step(self, dt, rtol=1e-6, atol=1e-8, max_attempts=100000):
         return load().orc_mix_strang_step(self._h, dt, rtol, atol, max_attempts)
+
+    def set_transport(self, t):
+        self._tran = t
+        load().orc_mix_set_transport(self._h, ctypes.byref(t))
 
     def set_step(self, dt):
         load().orc_mix_set_step(self._h, dt)

@@ -54,11 +54,14 @@ def make_mix_pair(case, seed=3, **state_kw):
     S = make_solver(case)
     ngg = S.get_int("ngg")
     X, Y, Z = cases.physical_grid(case)
-    inv, recon, _ = case["scheme"]
-    p = oracle_lib.params(inviscid=inv, recon=recon, visc=0, cfl=case["cfl"], eps=S.get_real("weno_eps"),
+    inv, recon, visc = case["scheme"]
+    p = oracle_lib.params(inviscid=inv, recon=recon, visc=visc, cfl=case["cfl"], eps=S.get_real("weno_eps"),
                           v_ref=S.get_real("v_inf"))
-    m = oracle_lib.mech_struct(C.parse(open(CC.MECH).read(), open(CC.THERM).read()))
+    d = C.parse(open(CC.MECH).read(), open(CC.THERM).read())
+    m = oracle_lib.mech_struct(d)
     B = oracle_lib.MixBlock(p, m, X, Y, Z, ngg, case["faces"])
+    if visc:
+        B.set_transport(oracle_lib.tran_struct([s.upper() for s in d["species"]], open(CC.TRAN_FILE).read()))
     bv, sv = cases.reactive_state(case["n"], ngg, seed=seed, **state_kw)
     S.set_state(bv, sv=sv)
     B.set_state(bv.ravel(), sv.ravel())

@@ -198,3 +198,40 @@ def test_positivity_limiter_matches_oracle(inv):
     err = rel_err(dq, dqo)
     print("limited rhs rel err", err, flush=True)
     assert np.max(err) < 1e-12
+
+
+@pytest.mark.parametrize("recon,inv", [(4, 2), (2, 4), (4, 72)])
+def test_viscous_mixture_rhs_dt_and_step(recon, inv):
+    """viscous_order 2 for the mixture (Chapman-Enskog/Wilke properties from tran.dat, species diffusion with
+    the correction velocity): residual, CFL step with its viscous term, and two SSP-RK3 steps."""
+    case = cases.reactive_box(N, recon=recon, inviscid=inv, temporal=3, reaction=0, visc=2)
+    S, B = make_mix_pair(case)
+    print("viscous mixture case", recon, inv, flush=True)
+    S.rhs()
+    dq = S.get(corefl.DQ).reshape(14, -1)
+    dqo = B.rhs().reshape(14, -1)
+    err = rel_err(dq, dqo)
+    print("viscous rhs rel err", err, flush=True)
+    assert np.max(err) < 1e-12
+    shape, g = (N, N, N), S.get_int("ngg")
+    for _ in range(2):
+        dt = B.cfl_dt()
+        assert abs(S.compute_dt() - dt) <= 1e-13 * dt
+        S.rk3_step(dt)
+        B.step(dt)
+    err = rel_err(interior(S.get(corefl.CV), 14, shape, g), interior(B.get(0), 14, shape, g))
+    print("viscous step rel err", err, flush=True)
+    assert np.max(err) < 1e-10
+
+
+def test_viscous_reacting_strang_step():
+    case = cases.reactive_box(N, recon=4, inviscid=2, temporal=5, reaction=1, visc=2)
+    S, B = make_mix_pair(case, peak=(2300.0, 2400.0))
+    shape, g = (N, N, N), S.get_int("ngg")
+    dt = B.cfl_dt()
+    assert abs(S.compute_dt() - dt) <= 1e-13 * dt
+    S.rk3_step(dt)
+    B.strang_step(dt)
+    err = rel_err(interior(S.get(corefl.CV), 14, shape, g), interior(B.get(0), 14, shape, g))
+    print("viscous strang rel err", err, flush=True)
+    assert np.max(err) < 1e-10
