@@ -4,6 +4,7 @@
 //
 // Thread mapping: threadIdx.x always walks i (the unit-stride index of the DZone layout), so every
 // global load and store is coalesced along a 64-lane wavefront whatever the sweep direction.
+#include <cstdlib>
 #include "flow.hpp"
 
 #include <cmath>
@@ -580,6 +581,28 @@ __device__ __forceinline__ double dd(const double *f, long id, long st, int i, i
 #undef FF
 }
 
+// Interior row of dd<ORDER> on a window w[0 .. ORDER] of consecutive values centred on w[ORDER / 2].
+template<int ORDER>
+__device__ __forceinline__ double dd_mid(const double *w) {
+  constexpr int H = ORDER / 2;
+  if constexpr (ORDER == 8)
+    return 0.8 * (w[H + 1] - w[H - 1]) + -0.2 * (w[H + 2] - w[H - 2]) + 4.0 / 105 * (w[H + 3] - w[H - 3]) +
+           -1.0 / 280 * (w[H + 4] - w[H - 4]);
+  else
+    return -1.0 / 12 * (w[H + 2] - w[H - 2]) + 2.0 / 3 * (w[H + 1] - w[H - 1]);
+}
+
+// Whether dd<ORDER> at index i of n takes a one-sided boundary row.
+template<int ORDER>
+__device__ __forceinline__ bool dd_closure(int i, int n, int lb, int rb) {
+  return (lb && i < ORDER / 2) || (rb && i >= n - ORDER / 2);
+}
+
+// z-marching node kernels (k_viscous_flux, k_update): each thread walks its (i, j) column through kz
+// consecutive planes and keeps the values its z stencil shares between neighbouring planes in registers, so
+// every plane of the z-differenced arrays is read from memory once per column instead of once per stencil
+// point.  kz trades that reuse against the number of waves in flight (COREFL_KZ overrides the default).
+
 __device__ __forceinline__ double sutherland(double T) {
   return 1.7894e-5 * pow(T / 288.16, 1.5) * (288.16 + 110) / (T + 110); // Transport.cu:6-9
 }
@@ -647,56 +670,78 @@ __global__ __launch_bounds__(256) void k_viscous_face(DevBlock b, Scheme sc, Til
 
 // compute_viscous_flux_collocated<Air, ORDER> (ViscousScheme.cu:309-408): node fluxes fv/gv/hv[1..4].
 template<int ORDER>
-__global__ __launch_bounds__(256) void k_viscous_flux(DevBlock b, Scheme sc, TileGrid tg) {
+__global__ __launch_bounds__(256) void k_viscous_flux(DevBlock b, Scheme sc, TileGrid tg, int kz) {
   Tile t;
   if (!xcd_tile(tg, t)) return;
-  const int i = t.x * 64 + threadIdx.x, j = t.y * 4 + threadIdx.y, k = t.z;
+  const int i = t.x * 64 + threadIdx.x, j = t.y * 4 + threadIdx.y;
   if (i >= b.mx || j >= b.my) return;
-  const long id = nid(b, i, j, k), sz = b.sz;
-  double g[4][3];
+  const int k0 = t.z * kz, k1 = min(k0 + kz, b.mz);
+  const long sz = b.sz, d1 = b.d1, id0 = nid(b, i, j, k0);
+  const bool z3 = b.dim == 3;
+  constexpr int H = ORDER / 2, W = ORDER + 1;
   const int comp[4] = {1, 2, 3, 5};
+  double wz[4][W]; // u, v, w, T on planes k - H .. k + H
+  if (z3) {
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    const double *f = b.bv + comp[q] * sz;
-    g[q][0] = dd<ORDER>(f, id, 1, i, b.mx, b.bt[0], b.bt[1]);
-    g[q][1] = dd<ORDER>(f, id, b.d2, j, b.my, b.bt[2], b.bt[3]);
-    g[q][2] = b.dim == 3 ? dd<ORDER>(f, id, b.d1, k, b.mz, b.bt[4], b.bt[5]) : 0.0;
-  }
-  const double *M = b.met + id;
-  const double xx = M[0], xy = M[sz], xz = M[2 * sz], ex = M[3 * sz], ey = M[4 * sz], ez = M[5 * sz];
-  const double zx = M[6 * sz], zy = M[7 * sz], zz = M[8 * sz];
-  double d[4][3]; // physical gradients of u, v, w, T
+    for (int q = 0; q < 4; ++q)
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    d[q][0] = g[q][0] * xx + g[q][1] * ex + g[q][2] * zx;
-    d[q][1] = g[q][0] * xy + g[q][1] * ey + g[q][2] * zy;
-    d[q][2] = g[q][0] * xz + g[q][1] * ez + g[q][2] * zz;
+      for (int m = 1; m < W; ++m) wz[q][m] = b.bv[id0 + (m - 1 - H) * d1 + comp[q] * sz];
   }
-  const double T = b.bv[id + 5 * sz];
-  const double mu = sutherland(T);
-  const double txx = mu * (4 * d[0][0] - 2 * d[1][1] - 2 * d[2][2]) / 3.0;
-  const double tyy = mu * (4 * d[1][1] - 2 * d[0][0] - 2 * d[2][2]) / 3.0;
-  const double tzz = mu * (4 * d[2][2] - 2 * d[0][0] - 2 * d[1][1]) / 3.0;
-  const double txy = mu * (d[0][1] + d[1][0]), txz = mu * (d[0][2] + d[2][0]), tyz = mu * (d[1][2] + d[2][1]);
-  constexpr double cp = kGamma * kRair / (kGamma - 1);
-  const double kc = mu / sc.prandtl * cp;
-  const double u = b.bv[id + sz], v = b.bv[id + 2 * sz], w = b.bv[id + 3 * sz];
-  const double Ex = u * txx + v * txy + w * txz + kc * d[3][0];
-  const double Ey = u * txy + v * tyy + w * tyz + kc * d[3][1];
-  const double Ez = u * txz + v * tyz + w * tzz + kc * d[3][2];
-  double *fv = b.fv + id, *gv = b.gv + id, *hv = b.hv + id;
-  fv[sz] = xx * txx + xy * txy + xz * txz;
-  fv[2 * sz] = xx * txy + xy * tyy + xz * tyz;
-  fv[3 * sz] = xx * txz + xy * tyz + xz * tzz;
-  fv[4 * sz] = xx * Ex + xy * Ey + xz * Ez;
-  gv[sz] = ex * txx + ey * txy + ez * txz;
-  gv[2 * sz] = ex * txy + ey * tyy + ez * tyz;
-  gv[3 * sz] = ex * txz + ey * tyz + ez * tzz;
-  gv[4 * sz] = ex * Ex + ey * Ey + ez * Ez;
-  hv[sz] = zx * txx + zy * txy + zz * txz;
-  hv[2 * sz] = zx * txy + zy * tyy + zz * tyz;
-  hv[3 * sz] = zx * txz + zy * tyz + zz * tzz;
-  hv[4 * sz] = zx * Ex + zy * Ey + zz * Ez;
+  for (int k = k0; k < k1; ++k) {
+    const long id = id0 + (long)(k - k0) * d1;
+    if (z3) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+#pragma unroll
+        for (int m = 0; m < W - 1; ++m) wz[q][m] = wz[q][m + 1];
+        wz[q][W - 1] = b.bv[id + H * d1 + comp[q] * sz];
+      }
+    }
+    const bool zc = z3 && dd_closure<ORDER>(k, b.mz, b.bt[4], b.bt[5]);
+    double g[4][3];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const double *f = b.bv + comp[q] * sz;
+      g[q][0] = dd<ORDER>(f, id, 1, i, b.mx, b.bt[0], b.bt[1]);
+      g[q][1] = dd<ORDER>(f, id, b.d2, j, b.my, b.bt[2], b.bt[3]);
+      g[q][2] = !z3 ? 0.0 : zc ? dd<ORDER>(f, id, d1, k, b.mz, b.bt[4], b.bt[5]) : dd_mid<ORDER>(wz[q]);
+    }
+    const double *M = b.met + id;
+    const double xx = M[0], xy = M[sz], xz = M[2 * sz], ex = M[3 * sz], ey = M[4 * sz], ez = M[5 * sz];
+    const double zx = M[6 * sz], zy = M[7 * sz], zz = M[8 * sz];
+    double d[4][3]; // physical gradients of u, v, w, T
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      d[q][0] = g[q][0] * xx + g[q][1] * ex + g[q][2] * zx;
+      d[q][1] = g[q][0] * xy + g[q][1] * ey + g[q][2] * zy;
+      d[q][2] = g[q][0] * xz + g[q][1] * ez + g[q][2] * zz;
+    }
+    const double u = z3 ? wz[0][H] : b.bv[id + sz], v = z3 ? wz[1][H] : b.bv[id + 2 * sz];
+    const double w = z3 ? wz[2][H] : b.bv[id + 3 * sz], T = z3 ? wz[3][H] : b.bv[id + 5 * sz];
+    const double mu = sutherland(T);
+    const double txx = mu * (4 * d[0][0] - 2 * d[1][1] - 2 * d[2][2]) / 3.0;
+    const double tyy = mu * (4 * d[1][1] - 2 * d[0][0] - 2 * d[2][2]) / 3.0;
+    const double tzz = mu * (4 * d[2][2] - 2 * d[0][0] - 2 * d[1][1]) / 3.0;
+    const double txy = mu * (d[0][1] + d[1][0]), txz = mu * (d[0][2] + d[2][0]), tyz = mu * (d[1][2] + d[2][1]);
+    constexpr double cp = kGamma * kRair / (kGamma - 1);
+    const double kc = mu / sc.prandtl * cp;
+    const double Ex = u * txx + v * txy + w * txz + kc * d[3][0];
+    const double Ey = u * txy + v * tyy + w * tyz + kc * d[3][1];
+    const double Ez = u * txz + v * tyz + w * tzz + kc * d[3][2];
+    double *fv = b.fv + id, *gv = b.gv + id, *hv = b.hv + id;
+    fv[sz] = xx * txx + xy * txy + xz * txz;
+    fv[2 * sz] = xx * txy + xy * tyy + xz * tyz;
+    fv[3 * sz] = xx * txz + xy * tyz + xz * tzz;
+    fv[4 * sz] = xx * Ex + xy * Ey + xz * Ez;
+    gv[sz] = ex * txx + ey * txy + ez * txz;
+    gv[2 * sz] = ex * txy + ey * tyy + ez * tyz;
+    gv[3 * sz] = ex * txz + ey * tyz + ez * tzz;
+    gv[4 * sz] = ex * Ex + ey * Ey + ez * Ez;
+    hv[sz] = zx * txx + zy * txy + zz * txz;
+    hv[2 * sz] = zx * txy + zy * tyy + zz * tyz;
+    hv[3 * sz] = zx * txz + zy * tyz + zz * tzz;
+    hv[4 * sz] = zx * Ex + zy * Ey + zz * Ez;
+  }
 }
 
 // One work item = one (face node, ghost layer g) pair.  On i faces the ghost layers of a node are
@@ -791,76 +836,113 @@ __constant__ double kRkC[3] = {1.0, 0.25, 2.0 / 3.0};
 // dq = J (dFv/dxi + dGv/deta + dHv/dzeta) - sum_dir (F(i+1/2) - F(i-1/2)), then
 // update_cv_and_bv_rk<Air> (RK.cuh:224-280).  Stage 0 also saves U^n into qn.
 template<int ORDER, bool RHS_ONLY>
-__global__ __launch_bounds__(256) void k_update(DevBlock b, int rk, const double *__restrict__ dtp, TileGrid tg) {
+__global__ __launch_bounds__(256) void k_update(DevBlock b, int rk, const double *__restrict__ dtp, TileGrid tg,
+                                                int kz) {
   Tile t;
   if (!xcd_tile(tg, t)) return;
-  const int i = t.x * 64 + threadIdx.x, j = t.y * 4 + threadIdx.y, k = t.z;
+  const int i = t.x * 64 + threadIdx.x, j = t.y * 4 + threadIdx.y;
   if (i >= b.mx || j >= b.my) return;
-  const long id = nid(b, i, j, k), sz = b.sz;
-  double r[5];
-  r[0] = 0;
-  if constexpr (ORDER == 2) {
-    // face fluxes (compute_dFv_dx / dGv_dy / dHv_dz, ViscousScheme.cu:75-128)
+  const int k0 = t.z * kz, k1 = min(k0 + kz, b.mz);
+  const long sz = b.sz, d1 = b.d1, id0 = nid(b, i, j, k0);
+  const bool z3 = b.dim == 3;
+  // hv (components 1..4) on planes k - HB .. k + HA: the face flux below k and at k (ORDER 2), or the
+  // collocated stencil (ORDER 4, 8)
+  constexpr int HB = ORDER == 2 ? 1 : ORDER / 2, HA = ORDER == 2 ? 0 : ORDER / 2, W = HB + HA + 1;
+  double wz[4][W];
+  double fzm[5]; // fz on plane k - 1
+  if (z3) {
+    if constexpr (ORDER > 0) {
 #pragma unroll
-    for (int l = 1; l < 5; ++l) {
-      double s = b.fv[id + l * sz] - b.fv[id - 1 + l * sz];
-      s += b.gv[id + l * sz] - b.gv[id - b.d2 + l * sz];
-      if (b.dim == 3) s += b.hv[id + l * sz] - b.hv[id - b.d1 + l * sz];
-      r[l] = s;
+      for (int l = 0; l < 4; ++l)
+#pragma unroll
+        for (int m = 1; m < W; ++m) wz[l][m] = b.hv[id0 + (m - 1 - HB) * d1 + (l + 1) * sz];
     }
-  } else if constexpr (ORDER > 0) {
-    const double J = b.jac[id];
 #pragma unroll
-    for (int l = 1; l < 5; ++l) {
-      double s = dd<ORDER>(b.fv + l * sz, id, 1, i, b.mx, b.bt[0], b.bt[1]) +
-                 dd<ORDER>(b.gv + l * sz, id, b.d2, j, b.my, b.bt[2], b.bt[3]);
-      if (b.dim == 3) s += dd<ORDER>(b.hv + l * sz, id, b.d1, k, b.mz, b.bt[4], b.bt[5]);
-      r[l] = s * J;
+    for (int l = 0; l < 5; ++l) fzm[l] = b.fz[id0 - d1 + l * sz];
+  }
+  const double dt = RHS_ONLY ? 0.0 : dtp[0];
+  for (int k = k0; k < k1; ++k) {
+    const long id = id0 + (long)(k - k0) * d1;
+    if constexpr (ORDER > 0) {
+      if (z3) {
+#pragma unroll
+        for (int l = 0; l < 4; ++l) {
+#pragma unroll
+          for (int m = 0; m < W - 1; ++m) wz[l][m] = wz[l][m + 1];
+          wz[l][W - 1] = b.hv[id + HA * d1 + (l + 1) * sz];
+        }
+      }
     }
-  } else {
+    double r[5];
+    r[0] = 0;
+    if constexpr (ORDER == 2) {
+      // face fluxes (compute_dFv_dx / dGv_dy / dHv_dz, ViscousScheme.cu:75-128)
 #pragma unroll
-    for (int l = 1; l < 5; ++l) r[l] = 0;
-  }
+      for (int l = 1; l < 5; ++l) {
+        double s = b.fv[id + l * sz] - b.fv[id - 1 + l * sz];
+        s += b.gv[id + l * sz] - b.gv[id - b.d2 + l * sz];
+        if (z3) s += wz[l - 1][1] - wz[l - 1][0];
+        r[l] = s;
+      }
+    } else if constexpr (ORDER > 0) {
+      const double J = b.jac[id];
+      const bool zc = z3 && dd_closure<ORDER>(k, b.mz, b.bt[4], b.bt[5]);
 #pragma unroll
-  for (int l = 0; l < 5; ++l) r[l] -= b.fx[id + l * sz] - b.fx[id - 1 + l * sz];
+      for (int l = 1; l < 5; ++l) {
+        double s = dd<ORDER>(b.fv + l * sz, id, 1, i, b.mx, b.bt[0], b.bt[1]) +
+                   dd<ORDER>(b.gv + l * sz, id, b.d2, j, b.my, b.bt[2], b.bt[3]);
+        if (z3) s += zc ? dd<ORDER>(b.hv + l * sz, id, d1, k, b.mz, b.bt[4], b.bt[5]) : dd_mid<ORDER>(wz[l - 1]);
+        r[l] = s * J;
+      }
+    } else {
 #pragma unroll
-  for (int l = 0; l < 5; ++l) r[l] -= b.fy[id + l * sz] - b.fy[id - b.d2 + l * sz];
-  if (b.dim == 3) {
-#pragma unroll
-    for (int l = 0; l < 5; ++l) r[l] -= b.fz[id + l * sz] - b.fz[id - b.d1 + l * sz];
-  }
-  if constexpr (RHS_ONLY) {
-    const long iq = ((long)k * b.my + j) * b.mx + i;
-#pragma unroll
-    for (int l = 0; l < 5; ++l) b.dq[iq + l * b.n] = r[l];
-    return;
-  }
-  const double dj = dtp[0] / b.jac[id];
-  double q[5];
-  if (rk == 0) {
-#pragma unroll
-    for (int l = 0; l < 5; ++l) {
-      const double u0 = b.cv[id + l * sz];
-      b.qn[id + l * sz] = u0;
-      q[l] = kRkA[0] * u0 + kRkB[0] * u0 + kRkC[0] * dj * r[l];
+      for (int l = 1; l < 5; ++l) r[l] = 0;
     }
-  } else {
 #pragma unroll
-    for (int l = 0; l < 5; ++l)
-      q[l] = kRkA[rk] * b.qn[id + l * sz] + kRkB[rk] * b.cv[id + l * sz] + kRkC[rk] * dj * r[l];
+    for (int l = 0; l < 5; ++l) r[l] -= b.fx[id + l * sz] - b.fx[id - 1 + l * sz];
+#pragma unroll
+    for (int l = 0; l < 5; ++l) r[l] -= b.fy[id + l * sz] - b.fy[id - b.d2 + l * sz];
+    if (z3) {
+#pragma unroll
+      for (int l = 0; l < 5; ++l) {
+        const double f = b.fz[id + l * sz];
+        r[l] -= f - fzm[l];
+        fzm[l] = f;
+      }
+    }
+    if constexpr (RHS_ONLY) {
+      const long iq = ((long)k * b.my + j) * b.mx + i;
+#pragma unroll
+      for (int l = 0; l < 5; ++l) b.dq[iq + l * b.n] = r[l];
+      continue;
+    }
+    const double dj = dt / b.jac[id];
+    double q[5];
+    if (rk == 0) {
+#pragma unroll
+      for (int l = 0; l < 5; ++l) {
+        const double u0 = b.cv[id + l * sz];
+        b.qn[id + l * sz] = u0;
+        q[l] = kRkA[0] * u0 + kRkB[0] * u0 + kRkC[0] * dj * r[l];
+      }
+    } else {
+#pragma unroll
+      for (int l = 0; l < 5; ++l)
+        q[l] = kRkA[rk] * b.qn[id + l * sz] + kRkB[rk] * b.cv[id + l * sz] + kRkC[rk] * dj * r[l];
+    }
+    if (b.mz == 1) q[3] = 0;
+#pragma unroll
+    for (int l = 0; l < 5; ++l) b.cv[id + l * sz] = q[l];
+    const double di = 1.0 / q[0];
+    const double u = q[1] * di, v = q[2] * di, w = q[3] * di;
+    const double p = (kGamma - 1) * (q[4] - 0.5 * q[0] * (u * u + v * v + w * w));
+    b.bv[id] = q[0];
+    b.bv[id + sz] = u;
+    b.bv[id + 2 * sz] = v;
+    b.bv[id + 3 * sz] = w;
+    b.bv[id + 4 * sz] = p;
+    b.bv[id + 5 * sz] = p * kMwAir * di / kRu;
   }
-  if (b.mz == 1) q[3] = 0;
-#pragma unroll
-  for (int l = 0; l < 5; ++l) b.cv[id + l * sz] = q[l];
-  const double di = 1.0 / q[0];
-  const double u = q[1] * di, v = q[2] * di, w = q[3] * di;
-  const double p = (kGamma - 1) * (q[4] - 0.5 * q[0] * (u * u + v * v + w * w));
-  b.bv[id] = q[0];
-  b.bv[id + sz] = u;
-  b.bv[id + 2 * sz] = v;
-  b.bv[id + 3 * sz] = w;
-  b.bv[id + 4 * sz] = p;
-  b.bv[id + 5 * sz] = p * kMwAir * di / kRu;
 }
 
 // ---------------------------------------------------------------- boundary conditions
@@ -1030,8 +1112,19 @@ __global__ void k_time_advance(double *time, const double *dt) { time[0] += dt[0
 
 dim3 node_grid(const DevBlock &b) { return dim3((b.mx + 63) / 64, (b.my + 3) / 4, b.mz); }
 
-TileGrid node_tiles(const DevBlock &b, int fast) {
-  return TileGrid{(b.mx + 63) / 64, (b.my + 3) / 4, b.mz, fast};
+// Planes per column of the z-marching kernels.
+int march_kz() {
+  static const int kz = [] {
+    const char *e = std::getenv("COREFL_KZ");
+    const int v = e ? std::atoi(e) : 8;
+    return v < 1 ? 1 : v > 256 ? 256 : v;
+  }();
+  return kz;
+}
+
+// Tiles of the z-marching kernels: 64 x 4 columns of kz planes each.
+TileGrid march_tiles(const DevBlock &b, int kz) {
+  return TileGrid{(b.mx + 63) / 64, (b.my + 3) / 4, (b.mz + kz - 1) / kz, 0};
 }
 
 } // namespace
@@ -1053,7 +1146,6 @@ hipError_t launch_boundary(const DevBlock &b, const DevFace &f, hipStream_t s) {
 
 hipError_t launch_viscous_flux(const DevBlock &b, const Scheme &sc, hipStream_t s) {
   const dim3 tpb(64, 4, 1);
-  const TileGrid tg = node_tiles(b, 0);
   if (sc.viscous_order == 2) {
     // one launch per face direction, over the faces -1/2 .. m-1/2 along it
     const TileGrid tx{(b.mx + 1 + 63) / 64, (b.my + 3) / 4, b.mz, 0};
@@ -1066,8 +1158,10 @@ hipError_t launch_viscous_flux(const DevBlock &b, const Scheme &sc, hipStream_t 
     }
     return hipGetLastError();
   }
-  if (sc.viscous_order == 8) k_viscous_flux<8><<<tile_blocks(tg), tpb, 0, s>>>(b, sc, tg);
-  else k_viscous_flux<4><<<tile_blocks(tg), tpb, 0, s>>>(b, sc, tg);
+  const int kz = march_kz();
+  const TileGrid tm = march_tiles(b, kz);
+  if (sc.viscous_order == 8) k_viscous_flux<8><<<tile_blocks(tm), tpb, 0, s>>>(b, sc, tm, kz);
+  else k_viscous_flux<4><<<tile_blocks(tm), tpb, 0, s>>>(b, sc, tm, kz);
   return hipGetLastError();
 }
 
@@ -1152,19 +1246,20 @@ hipError_t launch_inviscid_flux(const DevBlock &b, const Scheme &sc, int dir, hi
 hipError_t launch_update(const DevBlock &b, const Scheme &sc, int rk, const double *dt_dev, bool rhs_only,
                          hipStream_t s) {
   const dim3 tpb(64, 4, 1);
-  const TileGrid tg = node_tiles(b, 0);
+  const int kz = march_kz();
+  const TileGrid tg = march_tiles(b, kz);
   const unsigned grid = tile_blocks(tg);
   const int o = sc.viscous_order;
   if (rhs_only) {
-    if (o == 8) k_update<8, true><<<grid, tpb, 0, s>>>(b, rk, dt_dev, tg);
-    else if (o == 4) k_update<4, true><<<grid, tpb, 0, s>>>(b, rk, dt_dev, tg);
-    else if (o == 2) k_update<2, true><<<grid, tpb, 0, s>>>(b, rk, dt_dev, tg);
-    else k_update<0, true><<<grid, tpb, 0, s>>>(b, rk, dt_dev, tg);
+    if (o == 8) k_update<8, true><<<grid, tpb, 0, s>>>(b, rk, dt_dev, tg, kz);
+    else if (o == 4) k_update<4, true><<<grid, tpb, 0, s>>>(b, rk, dt_dev, tg, kz);
+    else if (o == 2) k_update<2, true><<<grid, tpb, 0, s>>>(b, rk, dt_dev, tg, kz);
+    else k_update<0, true><<<grid, tpb, 0, s>>>(b, rk, dt_dev, tg, kz);
   } else {
-    if (o == 8) k_update<8, false><<<grid, tpb, 0, s>>>(b, rk, dt_dev, tg);
-    else if (o == 4) k_update<4, false><<<grid, tpb, 0, s>>>(b, rk, dt_dev, tg);
-    else if (o == 2) k_update<2, false><<<grid, tpb, 0, s>>>(b, rk, dt_dev, tg);
-    else k_update<0, false><<<grid, tpb, 0, s>>>(b, rk, dt_dev, tg);
+    if (o == 8) k_update<8, false><<<grid, tpb, 0, s>>>(b, rk, dt_dev, tg, kz);
+    else if (o == 4) k_update<4, false><<<grid, tpb, 0, s>>>(b, rk, dt_dev, tg, kz);
+    else if (o == 2) k_update<2, false><<<grid, tpb, 0, s>>>(b, rk, dt_dev, tg, kz);
+    else k_update<0, false><<<grid, tpb, 0, s>>>(b, rk, dt_dev, tg, kz);
   }
   return hipGetLastError();
 }
