@@ -235,3 +235,18 @@ def test_viscous_reacting_strang_step():
     err = rel_err(interior(S.get(corefl.CV), 14, shape, g), interior(B.get(0), 14, shape, g))
     print("viscous strang rel err", err, flush=True)
     assert np.max(err) < 1e-10
+
+
+def test_viscous_wu_splitting_step():
+    """Wu's splitting with the mixture's viscous fluxes in the frozen transport residual Cn and in the
+    half-step correction."""
+    case = cases.reactive_box(N, recon=4, inviscid=2, temporal=4, reaction=1, visc=2)
+    S, B = make_mix_pair(case, peak=(1150.0, 1250.0))
+    shape, g = (N, N, N), S.get_int("ngg")
+    for _ in range(2):
+        dt = B.cfl_dt()
+        S.rk3_step(dt)
+        B.wu_step(dt, 0)
+    err = rel_err(interior(S.get(corefl.CV), 14, shape, g), interior(B.get(0), 14, shape, g))
+    print("viscous wu rel err", err, flush=True)
+    assert np.max(err) < 1e-10

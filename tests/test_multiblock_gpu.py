@@ -97,11 +97,17 @@ def test_two_blocks_graph_replay():
     S2.close()
 
 
-def test_two_blocks_reacting_mixture():
+@pytest.mark.parametrize("visc", [0, 2])
+def test_two_blocks_reacting_mixture(visc):
     """The reacting H2/air box (9 species, WENO5 + Roe, Strang-split exact chemistry), outflow in x, as one
-    block and as two aligned blocks: species and state exchanged across the block-to-block face."""
+    block and as two aligned blocks: species and state exchanged across the block-to-block face.  The first
+    residual agrees to rounding in both cases.  With the mixture's viscous fluxes (visc 2) the face-centred
+    scheme then reads the edge ghosts that neither exchange fills (as for air above); next to the hot
+    kernels' steep temperature and species gradients those stale values move the two-step state by a few
+    1e-7 (relative), so the steps are compared at 1e-6 there."""
     n, split = 13, 6
-    box = cases.reactive_box(n, recon=4, inviscid=2, temporal=5, reaction=1)
+    box = cases.reactive_box(n, recon=4, inviscid=2, temporal=5, reaction=1, visc=visc)
+    tol = 1e-6 if visc else 1e-10
     single, multi, place = cases.two_blocks(box, n, split, "aligned")
     S1 = corefl.Solver(setup_text=single["setup"], blocks=[single["grid"]], boundary_text=single["boundary"])
     S2 = corefl.Solver(setup_text=multi["setup"], blocks=multi["grids"], boundary_text=multi["boundary"],
@@ -113,6 +119,18 @@ def test_two_blocks_reacting_mixture():
     bv4, sv4 = bv.reshape(6, N, N, N), sv.reshape(9, N, N, N)
     for b, xs in enumerate((slice(0, split + 1 + 2 * g), slice(split, n + 2 * g))):
         S2.set_state(np.ascontiguousarray(bv4[..., xs]).ravel(), b=b, sv=np.ascontiguousarray(sv4[..., xs]).ravel())
+    # first residual: every ghost node of both runs holds the continued field, so they agree to rounding
+    S1.rhs()
+    S2.rhs()
+
+    def mix_dq(S, b):
+        v = S.view(b)
+        return S.get(corefl.DQ, b).reshape(14, v.mz, v.my, v.mx)
+
+    d1, d2 = mix_dq(S2, 0), place(mix_dq(S2, 1))
+    err = rel_err(np.concatenate([d1[..., :split], d2], axis=-1), mix_dq(S1, 0))
+    print("mixture rhs rel err", err)
+    assert np.all(err <= 1e-11), err
     dt = S1.compute_dt()
     assert S2.compute_dt() == pytest.approx(dt, rel=1e-13)
     for _ in range(2):
@@ -124,9 +142,9 @@ def test_two_blocks_reacting_mixture():
         return interior(S.get(corefl.CV, b), 14, (v.mx, v.my, v.mz), g)
 
     a1, a2 = mix_interior(S2, 0), place(mix_interior(S2, 1))
-    assert np.all(rel_err(a2[..., 0], a1[..., split]) <= 1e-10)
+    assert np.all(rel_err(a2[..., 0], a1[..., split]) <= tol)
     err = rel_err(np.concatenate([a1, a2[..., 1:]], axis=-1), mix_interior(S1, 0))
     print("mixture 2-step rel err", err)
-    assert np.all(err <= 1e-10), err
+    assert np.all(err <= tol), err
     S1.close()
     S2.close()
