@@ -123,7 +123,8 @@ def reactive_bench(n=129, steps=10, warmup=2):
 
 
 def surrogate_bench(n_states=2000000, train_steps=30000, n_eval=256, dt=5e-8,
-                    gates=((0.5, 1e30), (0.5, 20.0), (0.5, 5.0), (0.25, 2.0), (1.0, 10.0))):
+                    gates=((0.5, 1e30), (0.5, 20.0), (0.5, 5.0), (0.5, 2.0), (0.35, 3.0), (0.35, 2.0), (0.25, 3.0),
+                           (0.25, 2.0), (1.0, 10.0))):
     """Train the surrogate on exact-integrator states, then time one chemistry sub-step of n_eval^3 cells by
     surrogate (with fallback, for each temperature gate) and by the exact integrator (autotuned launch)."""
     import chem_cases as CC
@@ -151,7 +152,7 @@ def surrogate_bench(n_states=2000000, train_steps=30000, n_eval=256, dt=5e-8,
     ch.set_cells(rho, T1, rho * Y1)
     ch.autotune(dt, reps=1)
 
-    def timed(fn, reps=3):
+    def timed(fn, reps=5):
         best, res = 1e30, None
         for _ in range(reps):
             ch.set_cells(rho, T1, rho * Y1)
