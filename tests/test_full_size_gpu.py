@@ -1,0 +1,45 @@
+"""The benchmark workload itself against the oracle: Taylor-Green on the wavy 257^3-node grid (256^3 unique
+cells), WENO7-characteristic + AUSM+, 4th-order viscous, SSP-RK3 (needs an MI355X; the oracle's OpenMP
+restatement takes a few seconds per residual at this size on the box's host cores).
+
+The state after one step agrees to 1e-12 relative per conserved variable, as in the small parity cases
+(measured: under 1e-14), and the CFL step to 1e-13.  The residual itself is judged at 1e-9 of its own
+largest value (measured: 1.2e-10 for density, 3e-11 for the others): at Mach 0.1 the flow is nearly
+solenoidal, so the density residual is a small difference of much larger interface fluxes, and on smooth
+data the WENO weights amplify rounding (their smoothness indicators are differences of nearly equal
+values).  The two sides perform the same operations with different association and contraction, so their
+fluxes differ in the last bits, and on this fine grid that difference is a larger fraction of the
+residual than on the 12^3 parity grid, where 1e-12 holds.  Its effect on the state, dt/J times the
+residual, is the step bound above.
+"""
+import numpy as np
+import pytest
+
+import cases
+import corefl
+from parity import interior, make_pair, rel_err
+
+pytestmark = pytest.mark.gpu
+
+N = 257
+
+
+def test_benchmark_workload_rhs_and_step():
+    print("case: taylor-green", N, "^3 nodes, WENO7-ch + AUSM+, viscous order 4", flush=True)
+    case = cases.tgv(N, reynolds=1600.0, recon=5, inviscid=3, visc=4, cfl=0.5)
+    S, B = make_pair(case)
+    g, shape = S.get_int("ngg"), case["grid"][0].shape
+    del case["grid"]
+    print("pair ready", flush=True)
+    S.rhs()
+    err = rel_err(S.get(corefl.DQ).reshape(5, -1), B.rhs().reshape(5, -1))
+    print("rhs rel err", err, flush=True)
+    assert np.all(err <= 1e-9), err
+    dt = B.cfl_dt()
+    assert S.compute_dt() == pytest.approx(dt, rel=1e-13)
+    S.rk3_step(dt)
+    B.step(dt)
+    err = rel_err(interior(S.get(corefl.CV), 5, shape, g), interior(B.get(0), 5, shape, g))
+    print("step rel err", err, flush=True)
+    assert np.all(err <= 1e-12), err
+    S.close()
