@@ -42,24 +42,40 @@ SCHEME_NAMES = {1: "WENO7-ch + LF", 2: "WENO7-ch + Roe", 3: "WENO7-ch + AUSM+", 
                 71: "WENO7-cp LLF flux splitting", 72: "WENO7-ch LLF flux splitting"}
 
 
-def measured_traffic(kernel_symbol, n):
-    """HBM bytes per launch of one kernel from the committed rocprofv3 counter summary (same workload only)."""
-    path = os.path.join(ROOT, "profiles", "flow_tgv256_hbm_traffic.md")
-    if n != 257 or not os.path.exists(path):
-        return None
-    for line in open(path):
-        cells = [c.strip() for c in line.split("|")]
-        if len(cells) > 7 and cells[1] == f"`{kernel_symbol}`":
-            return float(cells[7]) * 1e9
-    return None
-
-
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(seconds=12.0, n=65):
-    """The oracle (OpenMP C restatement) on a bounded sample of the same workload."""
+def host_info():
+    """nproc, the CPU model name and its clock, as the host reports them."""
+    model, mhz = None, None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name") and model is None:
+                model = line.split(":", 1)[1].strip()
+            elif line.startswith("cpu MHz") and mhz is None:
+                mhz = float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    max_mhz = None
+    try:
+        import subprocess
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("CPU max MHz"):
+                max_mhz = float(line.split(":", 1)[1])
+            elif line.startswith("Model name") and model is None:
+                model = line.split(":", 1)[1].strip()
+    except Exception:
+        pass
+    return {"nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)), "model": model,
+            "mhz_now": mhz, "mhz_max": max_mhz}
+
+
+def cpu_baseline(seconds=10.0, seconds_1core=6.0, n=65, cells_full=256 ** 3, steps_full=100):
+    """The oracle (OpenMP C restatement of the reference's scheme routines) on a bounded sample of the same
+    workload: the same scheme, grid family and initial field at n^3 nodes, timed with all host threads and
+    with one.  Mcell-steps/s is a per-cell rate; the full workload's time at that rate is stated beside it."""
     import cases
     import oracle_lib
     from parity import make_pair
@@ -68,21 +84,36 @@ def cpu_baseline(seconds=12.0, n=65):
     S, B = make_pair(case)
     S.close()
     unique = (n - 1) ** 3
-    B.step(B.cfl_dt())  # warm-up
-    steps, t0 = 0, time.perf_counter()
-    while True:
-        B.step(B.cfl_dt())
-        steps += 1
-        el = time.perf_counter() - t0
-        if el >= seconds or steps >= 50:
-            break
-    return {"value": unique * steps / el / 1e6, "unit": "Mcell-steps/s", "cores": oracle_lib.load().orc_num_threads(),
-            "kind": "port",
-            "sample": f"oracle (C, OpenMP) on the same scheme at {n}^3 nodes ({n - 1}^3 unique), {steps} steps, "
-                      f"{el:.1f} s"}
+    lib = oracle_lib.load()
+    threads = lib.orc_num_threads()
+
+    def timed(limit, max_steps):
+        B.step(B.cfl_dt())  # warm-up
+        steps, t0 = 0, time.perf_counter()
+        while True:
+            B.step(B.cfl_dt())
+            steps += 1
+            el = time.perf_counter() - t0
+            if el >= limit or steps >= max_steps:
+                return steps, el
+
+    steps, el = timed(seconds, 50)
+    rate = unique * steps / el / 1e6
+    lib.orc_set_threads(1)
+    s1, e1 = timed(seconds_1core, 10)
+    lib.orc_set_threads(threads)
+    rate1 = unique * s1 / e1 / 1e6
+    return {"value": round(rate, 4), "unit": "Mcell-steps/s", "cores": threads, "kind": "port",
+            "sample": f"oracle (C restatement of the reference scheme routines, OpenMP) on the same scheme and "
+                      f"field at {n}^3 nodes ({n - 1}^3 unique), {steps} steps in {el:.1f} s on {threads} threads; "
+                      f"one thread: {s1} steps in {e1:.1f} s",
+            "value_1core": round(rate1, 5),
+            "full_workload_s": round(cells_full * steps_full / (rate * 1e6), 1),
+            "full_workload": f"{cells_full} cells x {steps_full} steps at this per-cell rate",
+            "host": host_info()}
 
 
-def reactive_bench(n=129, steps=10, warmup=2):
+def reactive_bench(n=129, steps=50, warmup=5):
     """Strang-split reacting box: n^3 nodes ((n-1)^3 unique cells)."""
     import cases
     import corefl
@@ -202,6 +233,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-reps", type=int, default=3)
     ap.add_argument("--no-reactive", action="store_true")
+    ap.add_argument("--reactive-steps", type=int, default=50, help="timed steps of the reacting box (50 as specified)")
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--only", choices=("reactive", "surrogate"),
                     help="run one secondary section alone and print its JSON object (profiling runs)")
@@ -227,7 +259,7 @@ def main():
     import corefl
 
     if args.only == "reactive":
-        print(json.dumps(reactive_bench()), flush=True)
+        print(json.dumps(reactive_bench(steps=args.reactive_steps)), flush=True)
         return
     if args.only == "surrogate":
         print(json.dumps(surrogate_bench(n_states=args.surrogate_states, train_steps=args.train_steps)), flush=True)
@@ -265,7 +297,7 @@ def main():
     symbol = (f"k_inviscid<{d}, 4, {inv}>" if inv < 50 else
               f"k_fsweno<{d}, {4 if inv > 70 else 3}, {'true' if inv % 10 == 2 else 'false'}>")
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(symbol, n), "kernel": dom,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom, "symbol": symbol,
                 "algorithmic_bytes": FLUX_BYTES * faces}
     # the flux kernels are bound by FP64 arithmetic, not by HBM: their FP64 VALU rate against the peak
     fp64 = None
@@ -275,7 +307,7 @@ def main():
                 "frac": round(tflops / FP64_PEAK_TFLOPS, 4), "flop_per_interface": FLUX_FLOP}
 
     S.close()
-    reactive = None if args.no_reactive else reactive_bench()
+    reactive = None if args.no_reactive else reactive_bench(steps=args.reactive_steps)
     surrogate = None if args.no_surrogate else surrogate_bench(n_states=args.surrogate_states,
                                                                train_steps=args.train_steps)
     cpu = None

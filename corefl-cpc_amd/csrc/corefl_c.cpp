@@ -152,6 +152,14 @@ struct corefl_ctx {
 
   hipStream_t pick(void *s) const { return s ? static_cast<hipStream_t>(s) : stream; }
 
+  // Forget the captured step (its executable graph and the graph it was instantiated from).
+  void drop_graph() {
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    graph_exec = nullptr;
+    graph = nullptr;
+  }
+
   void apply_bcs(Block &b, hipStream_t s) {
     // DBoundCond::apply_boundary_conditions order (BoundCond.cu:3489-3619): outflow, then periodic.
     for (int type : {6, 8})
@@ -675,8 +683,10 @@ int corefl_create(const char *input_dir, corefl_ctx **out) {
     ctx->param.parse(read_file(dir + "/setup.txt"));
     ctx->param.deduce();
     auto &P = ctx->param;
-    if (!P.get_int("gridIsBinary")) throw std::runtime_error("only binary grids (gridIsBinary = 1) are read");
-    auto blocks = corefl::read_grid_binary(dir + "/grid/grid   0.dat", P.get_int("ngg"), P.get_real("gridScale"));
+    // Mesh::read_grid (Mesh.cpp:692-788): grid%4d.dat (binary) or grid%4d.grd (ASCII)
+    auto blocks = P.get_int("gridIsBinary")
+                      ? corefl::read_grid_binary(dir + "/grid/grid   0.dat", P.get_int("ngg"), P.get_real("gridScale"))
+                      : corefl::read_grid_ascii(dir + "/grid/grid   0.grd", P.get_int("ngg"), P.get_real("gridScale"));
     // inner%4d.txt lists the block-to-block faces; a single block needs none
     std::string inner_text;
     if (std::ifstream(dir + "/boundary_condition/inner   0.txt")) inner_text = read_file(dir + "/boundary_condition/inner   0.txt");
@@ -927,10 +937,7 @@ int corefl_chemistry(corefl_ctx *ctx, int block, corefl_chem **out) {
     HIPCHK(hipStreamSynchronize(ctx->chem->stream));
     ctx->chem->have_dtc = false;
     // the caller may retune the chemistry launch: the next graph replay captures the step afresh
-    if (ctx->graph_exec) {
-      (void)hipGraphExecDestroy(ctx->graph_exec);
-      ctx->graph_exec = nullptr;
-    }
+    ctx->drop_graph();
     *out = ctx->chem;
     return 0;
   } catch (const std::exception &e) {
@@ -943,14 +950,7 @@ int corefl_set_chemistry_surrogate(corefl_ctx *ctx, corefl_surrogate *s) {
     if (!ctx) throw std::runtime_error("null context");
     if (s && !ctx->strang) throw std::runtime_error("the surrogate replaces split chemistry sub-steps (temporal_scheme 5)");
     ctx->surrogate = s;
-    if (ctx->graph_exec) { // the captured step embeds the old choice
-      (void)hipGraphExecDestroy(ctx->graph_exec);
-      ctx->graph_exec = nullptr;
-    }
-    if (ctx->graph) {
-      (void)hipGraphDestroy(ctx->graph);
-      ctx->graph = nullptr;
-    }
+    ctx->drop_graph(); // the captured step embeds the old choice
     return 0;
   } catch (const std::exception &e) {
     return fail(e);

@@ -132,6 +132,36 @@ std::vector<HostBlock> read_grid_binary(const std::string &path, int ngg, double
   return out;
 }
 
+std::vector<HostBlock> read_grid_ascii(const std::string &path, int ngg, double grid_scale) {
+  // grid%4d.grd (Mesh.cpp:739-788): the block count on the first line, one "mx my mz" line per block, then per
+  // block all x, all y and all z values (i fastest) separated by white space
+  std::ifstream f(path);
+  if (!f) throw std::runtime_error("cannot open grid file " + path);
+  std::string line;
+  if (!std::getline(f, line)) throw std::runtime_error("empty grid file " + path);
+  const int nb = std::stoi(line);
+  if (nb <= 0 || nb > 100000) throw std::runtime_error("bad block count in " + path);
+  std::vector<int> dims(3 * nb);
+  for (int b = 0; b < nb; ++b) {
+    if (!std::getline(f, line)) throw std::runtime_error("grid file truncated: " + path);
+    std::istringstream ls(line);
+    ls >> dims[3 * b] >> dims[3 * b + 1] >> dims[3 * b + 2];
+    if (!ls || dims[3 * b] < 1 || dims[3 * b + 1] < 1 || dims[3 * b + 2] < 1)
+      throw std::runtime_error("bad block extents in " + path);
+  }
+  std::vector<HostBlock> out;
+  for (int b = 0; b < nb; ++b) {
+    const size_t n = (size_t)dims[3 * b] * dims[3 * b + 1] * dims[3 * b + 2];
+    std::vector<double> x(n), y(n), z(n);
+    for (auto *a : {&x, &y, &z})
+      for (size_t q = 0; q < n; ++q)
+        if (!(f >> (*a)[q])) throw std::runtime_error("grid file truncated: " + path);
+    out.push_back(make_block(dims[3 * b], dims[3 * b + 1], dims[3 * b + 2], ngg, x.data(), y.data(), z.data(),
+                             grid_scale));
+  }
+  return out;
+}
+
 void parse_boundaries(const std::string &text, std::vector<HostBlock> &blocks, int periodic_label) {
   std::istringstream in(text);
   for (auto &b : blocks) {

@@ -50,6 +50,8 @@ struct HostBlock {
 
 // Parse grid%4d.dat content (int32 n_block; n_block*(mx,my,mz); x,y,z float64 per block, i fastest).
 std::vector<HostBlock> read_grid_binary(const std::string &path, int ngg, double grid_scale);
+// Parse grid%4d.grd (ASCII: block count, one "mx my mz" line per block, then x, y, z of each block, i fastest).
+std::vector<HostBlock> read_grid_ascii(const std::string &path, int ngg, double grid_scale);
 // Build blocks from in-memory coordinates (same ordering as the binary file).
 HostBlock make_block(int mx, int my, int mz, int ngg, const double *x, const double *y, const double *z,
                      double grid_scale);

@@ -45,6 +45,11 @@ typedef struct {
 double orc_weno5(const double *vp, const double *vm, double eps, int if_shock);
 /* WENO.cu:3116-3222 (Z-type tau7 weights). */
 double orc_weno7(const double *vp, const double *vm, double eps, int if_shock);
+/* The upwind-biased halves the two functions above sum (fPlus of v; fMinus is the same function of the
+ * mirrored stencil), with their smoothness indicators beta and unnormalised weights alpha (either may be
+ * NULL): WENO5 3 sub-stencils of v[0..4], WENO7 4 sub-stencils of v[0..6]. */
+double orc_weno5_half(const double *v, double eps, int if_shock, double *beta, double *alpha);
+double orc_weno7_half(const double *v, double eps, int if_shock, double *beta, double *alpha);
 
 /* Left/right characteristic matrices at an interface from the Roe average of the two
  * conservative states ql, qr (5 each) and the (unnormalised) averaged metric k[3].
@@ -111,6 +116,7 @@ double orc_block_cfl_dt(orc_block *b);
 void orc_block_rk3_step(orc_block *b, double dt);
 /* Number of OpenMP threads the oracle uses (for the CPU baseline report). */
 int orc_num_threads(void);
+void orc_set_threads(int n); /* OpenMP threads of the following oracle calls */
 
 /* ---- chemistry of one constant-volume cell ---- */
 
@@ -181,6 +187,9 @@ typedef struct {
  * ChemData.cpp:740-831, derived here from the raw data). */
 void orc_transport(const orc_mech *m, const orc_tran *t, double T, const double *Y, double *mu, double *lambda,
                    double *rhoD);
+/* Pure-species values at T of the function above: out[0] viscosity, out[1] conductivity, out[2] self-diffusion
+ * rho D_kk (the coefficients of Transport.cu:67-105 for species k alone). */
+void orc_transport_species(const orc_mech *m, const orc_tran *t, double T, int k, double *out);
 /* Viscous mixtures (viscous_order 2): the transport data the face-centred viscous fluxes use. */
 void orc_mix_set_transport(orc_mix *b, const orc_tran *t);
 /* Wu's splitting (temporal_scheme 4), one step: chem_advance_method 0 (forward Euler) or 3 (SSP-RK3 stages). */
@@ -189,6 +198,12 @@ void orc_mix_wu_step(orc_mix *b, double dt, int method);
  * (rho,u,v,w,p,Y_1..Y_ns), after the reference post-processing for E and gamma. */
 void orc_mix_point_flux(const orc_params *p, const orc_mech *m, const double *pl, const double *pr,
                         const double *ml, const double *mr, double jl, double jr, double efl, double efr, double *fc);
+
+/* The species positivity limiter of the flux-splitting schemes (WENO.cu:991-1087) on one interface: stencil
+ * states q[m][5 + ns] (conservative) with metric k[m][3] along the sweep, jac[m] and acoustic speed snd[m],
+ * m = 0..n-1 with the interface between n/2-1 and n/2; fc (5 + ns) is limited in place with step dt. */
+void orc_mix_positivity_point(const orc_params *p, int ns, int dim, double dt, const double *q, const double *k,
+                              const double *jac, const double *snd, int n, double *fc);
 
 /* ---- exact solution ---- */
 

@@ -723,6 +723,23 @@ static void positivity_limit(const orc_mix *b, double (*q)[NV], double (*k)[3], 
   }
 }
 
+void orc_mix_positivity_point(const orc_params *p, int ns, int dim, double dt, const double *q, const double *k,
+                              const double *jac, const double *snd, int n, double *fc) {
+  orc_mix b;
+  memset(&b, 0, sizeof b);
+  b.p = *p;
+  b.ns = ns;
+  b.nv = 5 + ns;
+  b.dim = dim;
+  b.pp_dt = dt;
+  double qq[8][NV], kk[8][3];
+  for (int m = 0; m < n; ++m) {
+    for (int l = 0; l < 5 + ns; ++l) qq[m][l] = q[m * (5 + ns) + l];
+    for (int c = 0; c < 3; ++c) kk[m][c] = k[3 * m + c];
+  }
+  positivity_limit(&b, qq, kk, jac, snd, n, fc);
+}
+
 static void inviscid_dir(orc_mix *b, int dir) {
   const int fs = b->p.inviscid_scheme > 50;
   const int rec = b->p.reconstruction;
@@ -819,24 +836,26 @@ void orc_mix_set_explicit_source(orc_mix *b, int on) { b->explicit_source = on; 
 
 static double omega_d(double t) { return 1.0548 * pow(t, -0.15504) + pow(t + 0.55909, -2.1705); }
 
-void orc_transport(const orc_mech *m, const orc_tran *tr, double T, const double *Y, double *mu_out,
-                   double *lambda_out, double *rhoD) {
+/* Pair coefficients of Species::read_tran (ChemData.cpp:779-831). */
+typedef struct {
+  double vc[MS], wdw[MS][MS], sqw[MS][MS], bdc[MS][MS], kbe[MS][MS], zr[MS];
+} tran_coef;
+
+static void tran_coeffs(const orc_mech *m, const orc_tran *tr, tran_coef *c) {
   const int ns = m->ns;
   const double pi = 3.14159265358979323846, kb = 1.380649e-23;
-  /* coefficients (ChemData.cpp:779-831) */
-  double vc[MS], wdw[MS][MS], sqw[MS][MS], bdc[MS][MS], kbe[MS][MS], zr[MS];
   for (int k = 0; k < ns; ++k) {
-    vc[k] = 2.6693e-6 * sqrt(m->mw[k]) / (tr->sigma[k] * tr->sigma[k]);
+    c->vc[k] = 2.6693e-6 * sqrt(m->mw[k]) / (tr->sigma[k] * tr->sigma[k]);
     const double ti = tr->eps_kb[k] / 298;
-    zr[k] = tr->zrot[k] * (1 + 0.5 * pi * pi * pi * sqrt(ti) + (2 + 0.25 * pi * pi) * ti + sqrt(pi * pi * pi) * sqrt(ti * ti * ti));
+    c->zr[k] = tr->zrot[k] * (1 + 0.5 * pi * pi * pi * sqrt(ti) + (2 + 0.25 * pi * pi) * ti + sqrt(pi * pi * pi) * sqrt(ti * ti * ti));
     for (int j = 0; j < ns; ++j) {
-      wdw[k][j] = pow(m->mw[j] / m->mw[k], 0.25);
-      sqw[k][j] = 1.0 / sqrt(8 * (1 + m->mw[k] / m->mw[j]));
+      c->wdw[k][j] = pow(m->mw[j] / m->mw[k], 0.25);
+      c->sqw[k][j] = 1.0 / sqrt(8 * (1 + m->mw[k] / m->mw[j]));
     }
   }
   for (int j = 0; j < ns; ++j) {
-    bdc[j][j] = 0.0188324 / (sqrt(m->mw[j] * 0.5) * tr->sigma[j] * tr->sigma[j]);
-    kbe[j][j] = 1.0 / tr->eps_kb[j];
+    c->bdc[j][j] = 0.0188324 / (sqrt(m->mw[j] * 0.5) * tr->sigma[j] * tr->sigma[j]);
+    c->kbe[j][j] = 1.0 / tr->eps_kb[j];
     for (int k = j + 1; k < ns; ++k) {
       const double W = m->mw[j] * m->mw[k] / (m->mw[j] + m->mw[k]);
       double xi = 1;
@@ -847,48 +866,75 @@ void orc_transport(const orc_mech *m, const orc_tran *tr, double T, const double
         xi = 1.0 + 0.25 * tr->polar[n] / pow(tr->sigma[n], 3) * mp * mp * sqrt(tr->eps_kb[p] / tr->eps_kb[n]);
       }
       const double sg = 0.5 * (tr->sigma[j] + tr->sigma[k]) * pow(xi, -1.0 / 6.0);
-      bdc[j][k] = bdc[k][j] = 0.0188324 / (sqrt(W) * sg * sg);
-      kbe[j][k] = kbe[k][j] = 1.0 / (xi * xi * sqrt(tr->eps_kb[j] * tr->eps_kb[k]));
+      c->bdc[j][k] = c->bdc[k][j] = 0.0188324 / (sqrt(W) * sg * sg);
+      c->kbe[j][k] = c->kbe[k][j] = 1.0 / (xi * xi * sqrt(tr->eps_kb[j] * tr->eps_kb[k]));
     }
   }
+}
+
+/* Pure-species viscosity vs, conductivity lam (Eucken with the rotational-relaxation correction for linear and
+ * non-linear molecules) and self-diffusion rho D_aa rd at T (Transport.cu:67-105). */
+static void species_transport(const orc_mech *m, const orc_tran *tr, const tran_coef *c, double T, const double *cp,
+                              int a, double *vs, double *lam, double *rd_out) {
+  const double pi = 3.14159265358979323846;
+  const double st = sqrt(T), t32 = T * st;
+  const double td = T / tr->eps_kb[a];
+  *vs = c->vc[a] * st / (1.147 * pow(td, -0.145) + pow(td + 0.5, -2));
+  const double R = kRu / m->mw[a];
+  const double trd = T * c->kbe[a][a];
+  const double rd = c->bdc[a][a] * t32 / (omega_d(trd) * R * T);
+  double l = 15 * 0.25 * *vs * R;
+  if (tr->geometry[a] == 1 || tr->geometry[a] == 2) {
+    const double tri = 1 / trd, q = sqrt(tri * pi) * pi;
+    const double ft = 1 + 0.5 * q + (2 + 0.25 * pi * pi) * tri + q * tri;
+    if (tr->geometry[a] == 1) {
+      const double ab = (2.5 * *vs - rd) / (pi * c->zr[a] / ft * *vs + 10.0 / 3 * *vs + 2.0 * rd);
+      l += -5 * ab * *vs * R + rd * cp[a] + rd * (2 * ab - 2.5) * R;
+    } else {
+      const double ab = (2.5 * *vs - rd) / (pi * c->zr[a] / ft * *vs + 5 * *vs + 2.0 * rd);
+      l += -7.5 * ab * *vs * R + rd * cp[a] + rd * (3 * ab - 2.5) * R;
+    }
+  }
+  *lam = l;
+  if (rd_out) *rd_out = rd;
+}
+
+void orc_transport_species(const orc_mech *m, const orc_tran *tr, double T, int k, double *out) {
+  tran_coef c;
+  tran_coeffs(m, tr, &c);
+  double h[MS], cp[MS];
+  orc_chem_h_cp(m, T, h, cp);
+  species_transport(m, tr, &c, T, cp, k, &out[0], &out[1], &out[2]);
+}
+
+void orc_transport(const orc_mech *m, const orc_tran *tr, double T, const double *Y, double *mu_out,
+                   double *lambda_out, double *rhoD) {
+  const int ns = m->ns;
+  tran_coef c;
+  tran_coeffs(m, tr, &c);
   /* properties (Transport.cu:62-150) */
-  double h[MS], cp[MS], X[MS], vs[MS];
+  double h[MS], cp[MS], X[MS], vs[MS], lam[MS];
   orc_chem_h_cp(m, T, h, cp);
   double imw = 0;
   for (int l = 0; l < ns; ++l) imw += Y[l] / m->mw[l];
   const double mwt = 1.0 / imw, st = sqrt(T), t32 = T * st;
   for (int l = 0; l < ns; ++l) {
     X[l] = Y[l] * mwt / m->mw[l];
-    const double td = T / tr->eps_kb[l];
-    vs[l] = vc[l] * st / (1.147 * pow(td, -0.145) + pow(td + 0.5, -2));
+    species_transport(m, tr, &c, T, cp, l, &vs[l], &lam[l], NULL);
   }
   double mu = 0, lambda = 0;
   for (int a = 0; a < ns; ++a) {
-    const double R = kRu / m->mw[a];
-    double lam = 15 * 0.25 * vs[a] * R;
-    if (tr->geometry[a] == 1 || tr->geometry[a] == 2) {
-      const double trd = T * kbe[a][a], tri = 1 / trd, q = sqrt(tri * pi) * pi;
-      const double ft = 1 + 0.5 * q + (2 + 0.25 * pi * pi) * tri + q * tri;
-      const double rd = bdc[a][a] * t32 / (omega_d(trd) * R * T);
-      if (tr->geometry[a] == 1) {
-        const double ab = (2.5 * vs[a] - rd) / (pi * zr[a] / ft * vs[a] + 10.0 / 3 * vs[a] + 2.0 * rd);
-        lam += -5 * ab * vs[a] * R + rd * cp[a] + rd * (2 * ab - 2.5) * R;
-      } else {
-        const double ab = (2.5 * vs[a] - rd) / (pi * zr[a] / ft * vs[a] + 5 * vs[a] + 2.0 * rd);
-        lam += -7.5 * ab * vs[a] * R + rd * cp[a] + rd * (3 * ab - 2.5) * R;
-      }
-    }
     double den = 0;
-    for (int c = 0; c < ns; ++c) {
+    for (int b = 0; b < ns; ++b) {
       double phi = 1.0;
-      if (c != a) {
-        const double num = 1.0 + sqrt(vs[a] / vs[c]) * wdw[a][c];
-        phi = num * num * sqw[a][c];
+      if (b != a) {
+        const double num = 1.0 + sqrt(vs[a] / vs[b]) * c.wdw[a][b];
+        phi = num * num * c.sqw[a][b];
       }
-      den += phi * X[c];
+      den += phi * X[b];
     }
     mu += vs[a] * X[a] / den;
-    lambda += lam * X[a] / (1.065 * den - 0.065 * X[a]);
+    lambda += lam[a] * X[a] / (1.065 * den - 0.065 * X[a]);
   }
   *mu_out = mu;
   *lambda_out = lambda;
@@ -901,7 +947,7 @@ void orc_transport(const orc_mech *m, const orc_tran *tr, double T, const double
   }
   for (int l = 0; l < ns; ++l)
     for (int n = l + 1; n < ns; ++n) {
-      const double inv = omega_d(T * kbe[l][n]) / (bdc[l][n] * t32);
+      const double inv = omega_d(T * c.kbe[l][n]) / (c.bdc[l][n] * t32);
       dsum[l] += (X[n] + eps) * inv;
       dsum[n] += (X[l] + eps) * inv;
     }

@@ -11,49 +11,35 @@ static const double kGamma = 1.4;                   /* gamma_air, Constants.h:21
 static const double kRu = 8314.462618;              /* R_u, Constants.h:17 */
 static const double kMwAir = 28.9647;               /* mw_air, Constants.h:26 */
 
-/* WENO.cu:3059-3114 */
-double orc_weno5(const double *vp, const double *vm, double eps, int if_shock) {
-  const double one6th = 1.0 / 6;
-  if (if_shock) {
-    double v0 = one6th * (2 * vp[2] + 5 * vp[3] - vp[4]);
-    double v1 = one6th * (-vp[1] + 5 * vp[2] + 2 * vp[3]);
-    double v2 = one6th * (2 * vp[0] - 7 * vp[1] + 11 * vp[2]);
-    const double t12 = 13.0 / 12;
-    double b0 = t12 * (vp[2] + vp[4] - 2 * vp[3]) * (vp[2] + vp[4] - 2 * vp[3]) +
-                0.25 * (3 * vp[2] - 4 * vp[3] + vp[4]) * (3 * vp[2] - 4 * vp[3] + vp[4]);
-    double b1 = t12 * (vp[1] + vp[3] - 2 * vp[2]) * (vp[1] + vp[3] - 2 * vp[2]) +
-                0.25 * (vp[1] - vp[3]) * (vp[1] - vp[3]);
-    double b2 = t12 * (vp[0] + vp[2] - 2 * vp[1]) * (vp[0] + vp[2] - 2 * vp[1]) +
-                0.25 * (vp[0] - 4 * vp[1] + 3 * vp[2]) * (vp[0] - 4 * vp[1] + 3 * vp[2]);
-    double a0 = 0.3 / ((eps + b0) * (eps + b0));
-    double a1 = 0.6 / ((eps + b1) * (eps + b1));
-    double a2 = 0.1 / ((eps + b2) * (eps + b2));
-    const double fp = (a0 * v0 + a1 * v1 + a2 * v2) / (a0 + a1 + a2);
-
-    v0 = one6th * (11 * vm[2] - 7 * vm[3] + 2 * vm[4]);
-    v1 = one6th * (2 * vm[1] + 5 * vm[2] - vm[3]);
-    v2 = one6th * (-vm[0] + 5 * vm[1] + 2 * vm[2]);
-    b0 = t12 * (vm[2] + vm[4] - 2 * vm[3]) * (vm[2] + vm[4] - 2 * vm[3]) +
-         0.25 * (3 * vm[2] - 4 * vm[3] + vm[4]) * (3 * vm[2] - 4 * vm[3] + vm[4]);
-    b1 = t12 * (vm[1] + vm[3] - 2 * vm[2]) * (vm[1] + vm[3] - 2 * vm[2]) +
-         0.25 * (vm[1] - vm[3]) * (vm[1] - vm[3]);
-    b2 = t12 * (vm[0] + vm[2] - 2 * vm[1]) * (vm[0] + vm[2] - 2 * vm[1]) +
-         0.25 * (vm[0] - 4 * vm[1] + 3 * vm[2]) * (vm[0] - 4 * vm[1] + 3 * vm[2]);
-    a0 = 0.1 / ((eps + b0) * (eps + b0));
+/* One upwind-biased half of WENO5 (the fPlus part of WENO.cu:3059-3114): candidate values q[0..2] at the
+ * right face of v[2] from v[0..4], Jiang-Shu smoothness indicators beta[0..2] and unnormalised weights
+ * alpha[0..2] (linear weights when if_shock is 0).  Returns sum(alpha q) / sum(alpha).  The downwind half
+ * (fMinus) is the same function of the mirrored stencil. */
+double orc_weno5_half(const double *v, double eps, int if_shock, double *beta, double *alpha) {
+  const double one6th = 1.0 / 6, t12 = 13.0 / 12;
+  const double q0 = one6th * (2 * v[2] + 5 * v[3] - v[4]);
+  const double q1 = one6th * (-v[1] + 5 * v[2] + 2 * v[3]);
+  const double q2 = one6th * (2 * v[0] - 7 * v[1] + 11 * v[2]);
+  const double b0 = t12 * (v[2] + v[4] - 2 * v[3]) * (v[2] + v[4] - 2 * v[3]) +
+                    0.25 * (3 * v[2] - 4 * v[3] + v[4]) * (3 * v[2] - 4 * v[3] + v[4]);
+  const double b1 = t12 * (v[1] + v[3] - 2 * v[2]) * (v[1] + v[3] - 2 * v[2]) + 0.25 * (v[1] - v[3]) * (v[1] - v[3]);
+  const double b2 = t12 * (v[0] + v[2] - 2 * v[1]) * (v[0] + v[2] - 2 * v[1]) +
+                    0.25 * (v[0] - 4 * v[1] + 3 * v[2]) * (v[0] - 4 * v[1] + 3 * v[2]);
+  double a0 = 0.3, a1 = 0.6, a2 = 0.1;
+  if (if_shock) { /* Jiang-Shu weights (WENO.cu:3077-3079; the Z form there is commented out) */
+    a0 = 0.3 / ((eps + b0) * (eps + b0));
     a1 = 0.6 / ((eps + b1) * (eps + b1));
-    a2 = 0.3 / ((eps + b2) * (eps + b2));
-    const double fm = (a0 * v0 + a1 * v1 + a2 * v2) / (a0 + a1 + a2);
-    return fp + fm;
+    a2 = 0.1 / ((eps + b2) * (eps + b2));
   }
-  double v0 = one6th * (2 * vp[2] + 5 * vp[3] - vp[4]);
-  double v1 = one6th * (-vp[1] + 5 * vp[2] + 2 * vp[3]);
-  double v2 = one6th * (2 * vp[0] - 7 * vp[1] + 11 * vp[2]);
-  const double fp = 0.3 * v0 + 0.6 * v1 + 0.1 * v2;
-  v0 = one6th * (11 * vm[2] - 7 * vm[3] + 2 * vm[4]);
-  v1 = one6th * (2 * vm[1] + 5 * vm[2] - vm[3]);
-  v2 = one6th * (-vm[0] + 5 * vm[1] + 2 * vm[2]);
-  const double fm = 0.1 * v0 + 0.6 * v1 + 0.3 * v2;
-  return fp + fm;
+  if (beta) { beta[0] = b0; beta[1] = b1; beta[2] = b2; }
+  if (alpha) { alpha[0] = a0; alpha[1] = a1; alpha[2] = a2; }
+  return (a0 * q0 + a1 * q1 + a2 * q2) / (a0 + a1 + a2);
+}
+
+/* WENO.cu:3059-3114: fPlus from vp, fMinus from the mirrored vm. */
+double orc_weno5(const double *vp, const double *vm, double eps, int if_shock) {
+  const double m[5] = {vm[4], vm[3], vm[2], vm[1], vm[0]};
+  return orc_weno5_half(vp, eps, if_shock, NULL, NULL) + orc_weno5_half(m, eps, if_shock, NULL, NULL);
 }
 
 /* smoothness indicator of one 4-point sub-stencil in the WENO.cu:3120-3140 form */
@@ -62,77 +48,49 @@ static double beta7(double s1, double s2, double s3) {
   return s1 * s1 + d12 * s2 * s2 + d13 * s3 * s3 + d14 * s1 * s3;
 }
 
-/* WENO.cu:3116-3222 */
-double orc_weno7(const double *vp, const double *vm, double eps, int if_shock) {
+/* One upwind-biased half of WENO7 (the fPlus part of WENO.cu:3116-3222): candidates at the right face of v[3]
+ * from v[0..6], smoothness indicators beta[0..3], Z-type weights alpha_k = d_k (1 + tau^2 / (eps + beta_k)^2)
+ * with tau = beta_0 + 3 beta_1 - 3 beta_2 - beta_3 (WENO.cu:3146-3152; linear weights when if_shock is 0). */
+double orc_weno7_half(const double *v, double eps, int if_shock, double *beta, double *alpha) {
   const double one6th = 1.0 / 6, one12th = 1.0 / 12;
   const double c0 = 1.0 / 35, c1 = 12.0 / 35, c2 = 18.0 / 35, c3 = 4.0 / 35;
+  double s1 = one6th * (-2 * v[0] + 9 * v[1] - 18 * v[2] + 11 * v[3]);
+  double s2 = -v[0] + 4 * v[1] - 5 * v[2] + 2 * v[3];
+  double s3 = -v[0] + 3 * v[1] - 3 * v[2] + v[3];
+  const double b0 = beta7(s1, s2, s3);
+  s1 = one6th * (v[1] - 6 * v[2] + 3 * v[3] + 2 * v[4]);
+  s2 = v[2] - 2 * v[3] + v[4];
+  s3 = -v[1] + 3 * v[2] - 3 * v[3] + v[4];
+  const double b1 = beta7(s1, s2, s3);
+  s1 = one6th * (-2 * v[2] - 3 * v[3] + 6 * v[4] - v[5]);
+  s3 = -v[2] + 3 * v[3] - 3 * v[4] + v[5];
+  const double b2 = beta7(s1, s2, s3); /* s2 is shared with b1, as in the reference */
+  s1 = one6th * (-11 * v[3] + 18 * v[4] - 9 * v[5] + 2 * v[6]);
+  s2 = 2 * v[3] - 5 * v[4] + 4 * v[5] - v[6];
+  s3 = -v[3] + 3 * v[4] - 3 * v[5] + v[6];
+  const double b3 = beta7(s1, s2, s3);
+  double a0 = c0, a1 = c1, a2 = c2, a3 = c3;
   if (if_shock) {
-    double s1 = one6th * (-2 * vp[0] + 9 * vp[1] - 18 * vp[2] + 11 * vp[3]);
-    double s2 = -vp[0] + 4 * vp[1] - 5 * vp[2] + 2 * vp[3];
-    double s3 = -vp[0] + 3 * vp[1] - 3 * vp[2] + vp[3];
-    double b0 = beta7(s1, s2, s3);
-    s1 = one6th * (vp[1] - 6 * vp[2] + 3 * vp[3] + 2 * vp[4]);
-    s2 = vp[2] - 2 * vp[3] + vp[4];
-    s3 = -vp[1] + 3 * vp[2] - 3 * vp[3] + vp[4];
-    double b1 = beta7(s1, s2, s3);
-    s1 = one6th * (-2 * vp[2] - 3 * vp[3] + 6 * vp[4] - vp[5]);
-    s3 = -vp[2] + 3 * vp[3] - 3 * vp[4] + vp[5];
-    double b2 = beta7(s1, s2, s3); /* s2 is shared with b1, as in the reference */
-    s1 = one6th * (-11 * vp[3] + 18 * vp[4] - 9 * vp[5] + 2 * vp[6]);
-    s2 = 2 * vp[3] - 5 * vp[4] + 4 * vp[5] - vp[6];
-    s3 = -vp[3] + 3 * vp[4] - 3 * vp[5] + vp[6];
-    double b3 = beta7(s1, s2, s3);
     double tau = b0 + 3 * b1 - 3 * b2 - b3;
-    tau *= tau;
-    double a0 = c0 + c0 * tau / ((eps + b0) * (eps + b0));
-    double a1 = c1 + c1 * tau / ((eps + b1) * (eps + b1));
-    double a2 = c2 + c2 * tau / ((eps + b2) * (eps + b2));
-    double a3 = c3 + c3 * tau / ((eps + b3) * (eps + b3));
-    double v0 = -3 * vp[0] + 13 * vp[1] - 23 * vp[2] + 25 * vp[3];
-    double v1 = vp[1] - 5 * vp[2] + 13 * vp[3] + 3 * vp[4];
-    double v2 = -vp[2] + 7 * vp[3] + 7 * vp[4] - vp[5];
-    double v3 = 3 * vp[3] + 13 * vp[4] - 5 * vp[5] + vp[6];
-    const double fp = one12th * (a0 * v0 + a1 * v1 + a2 * v2 + a3 * v3) / (a0 + a1 + a2 + a3);
-
-    s1 = one6th * (-2 * vm[6] + 9 * vm[5] - 18 * vm[4] + 11 * vm[3]);
-    s2 = -vm[6] + 4 * vm[5] - 5 * vm[4] + 2 * vm[3];
-    s3 = -vm[6] + 3 * vm[5] - 3 * vm[4] + vm[3];
-    b0 = beta7(s1, s2, s3);
-    s1 = one6th * (vm[5] - 6 * vm[4] + 3 * vm[3] + 2 * vm[2]);
-    s2 = vm[4] - 2 * vm[3] + vm[2];
-    s3 = -vm[5] + 3 * vm[4] - 3 * vm[3] + vm[2];
-    b1 = beta7(s1, s2, s3);
-    s1 = one6th * (-2 * vm[4] - 3 * vm[3] + 6 * vm[2] - vm[1]);
-    s3 = -vm[4] + 3 * vm[3] - 3 * vm[2] + vm[1];
-    b2 = beta7(s1, s2, s3);
-    s1 = one6th * (-11 * vm[3] + 18 * vm[2] - 9 * vm[1] + 2 * vm[0]);
-    s2 = 2 * vm[3] - 5 * vm[2] + 4 * vm[1] - vm[0];
-    s3 = -vm[3] + 3 * vm[2] - 3 * vm[1] + vm[0];
-    b3 = beta7(s1, s2, s3);
-    tau = b0 + 3 * b1 - 3 * b2 - b3;
     tau *= tau;
     a0 = c0 + c0 * tau / ((eps + b0) * (eps + b0));
     a1 = c1 + c1 * tau / ((eps + b1) * (eps + b1));
     a2 = c2 + c2 * tau / ((eps + b2) * (eps + b2));
     a3 = c3 + c3 * tau / ((eps + b3) * (eps + b3));
-    v0 = -3 * vm[6] + 13 * vm[5] - 23 * vm[4] + 25 * vm[3];
-    v1 = vm[5] - 5 * vm[4] + 13 * vm[3] + 3 * vm[2];
-    v2 = -vm[4] + 7 * vm[3] + 7 * vm[2] - vm[1];
-    v3 = 3 * vm[3] + 13 * vm[2] - 5 * vm[1] + vm[0];
-    const double fm = one12th * (a0 * v0 + a1 * v1 + a2 * v2 + a3 * v3) / (a0 + a1 + a2 + a3);
-    return fp + fm;
   }
-  double v3 = 3 * vp[3] + 13 * vp[4] - 5 * vp[5] + vp[6];
-  double v2 = -vp[2] + 7 * vp[3] + 7 * vp[4] - vp[5];
-  double v1 = vp[1] - 5 * vp[2] + 13 * vp[3] + 3 * vp[4];
-  double v0 = -3 * vp[0] + 13 * vp[1] - 23 * vp[2] + 25 * vp[3];
-  const double fp = one12th * (c0 * v0 + c1 * v1 + c2 * v2 + c3 * v3);
-  v0 = -3 * vm[6] + 13 * vm[5] - 23 * vm[4] + 25 * vm[3];
-  v1 = vm[5] - 5 * vm[4] + 13 * vm[3] + 3 * vm[2];
-  v2 = -vm[4] + 7 * vm[3] + 7 * vm[2] - vm[1];
-  v3 = 3 * vm[3] + 13 * vm[2] - 5 * vm[1] + vm[0];
-  const double fm = one12th * (c0 * v0 + c1 * v1 + c2 * v2 + c3 * v3);
-  return fp + fm;
+  if (beta) { beta[0] = b0; beta[1] = b1; beta[2] = b2; beta[3] = b3; }
+  if (alpha) { alpha[0] = a0; alpha[1] = a1; alpha[2] = a2; alpha[3] = a3; }
+  const double q0 = -3 * v[0] + 13 * v[1] - 23 * v[2] + 25 * v[3];
+  const double q1 = v[1] - 5 * v[2] + 13 * v[3] + 3 * v[4];
+  const double q2 = -v[2] + 7 * v[3] + 7 * v[4] - v[5];
+  const double q3 = 3 * v[3] + 13 * v[4] - 5 * v[5] + v[6];
+  return one12th * (a0 * q0 + a1 * q1 + a2 * q2 + a3 * q3) / (a0 + a1 + a2 + a3);
+}
+
+/* WENO.cu:3116-3222: fPlus from vp, fMinus from the mirrored vm. */
+double orc_weno7(const double *vp, const double *vm, double eps, int if_shock) {
+  const double m[7] = {vm[6], vm[5], vm[4], vm[3], vm[2], vm[1], vm[0]};
+  return orc_weno7_half(vp, eps, if_shock, NULL, NULL) + orc_weno7_half(m, eps, if_shock, NULL, NULL);
 }
 
 /* Roe-averaged interface state and the eigenvector frame (WENO.cu:22-62). */

@@ -34,6 +34,9 @@ def load():
     lib.orc_weno5.argtypes = [P, P, D, I]
     lib.orc_weno7.restype = D
     lib.orc_weno7.argtypes = [P, P, D, I]
+    for name in ("orc_weno5_half", "orc_weno7_half"):
+        getattr(lib, name).restype = D
+        getattr(lib, name).argtypes = [P, D, I, P, P]
     lib.orc_char_matrices.argtypes = [P, P, P, P, P]
     lib.orc_riemann.argtypes = [I, P, P, P, P, D, D, D, D, I, P]
     lib.orc_interface_flux.restype = I
@@ -54,6 +57,7 @@ def load():
     lib.orc_block_cfl_dt.argtypes = [P]
     lib.orc_block_rk3_step.argtypes = [P, D]
     lib.orc_num_threads.restype = I
+    lib.orc_set_threads.argtypes = [I]
     lib.orc_exact_riemann.restype = D
     lib.orc_exact_riemann.argtypes = [D, D, D, D, D, D, D, D, D, I, P, P, P, P]
     _lib = lib
@@ -67,6 +71,16 @@ def _p(a):
 def weno5(vp, vm, eps, shock=1):
     vp, vm = np.ascontiguousarray(vp, float), np.ascontiguousarray(vm, float)
     return load().orc_weno5(_p(vp), _p(vm), eps, shock)
+
+
+def weno_half(v, eps, shock=1):
+    """One upwind-biased WENO5 (5 values) or WENO7 (7 values) half: (value, beta, alpha)."""
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    k = 3 if v.size == 5 else 4
+    beta, alpha = np.empty(k), np.empty(k)
+    fn = load().orc_weno5_half if v.size == 5 else load().orc_weno7_half
+    val = fn(_p(v), eps, shock, beta.ctypes.data, alpha.ctypes.data)
+    return val, beta, alpha
 
 
 def weno7(vp, vm, eps, shock=1):
@@ -250,6 +264,16 @@ def transport(m, t, T, Y):
     return mu.value, lam.value, rd
 
 
+def transport_species(m, t, T, k):
+    """Pure-species (viscosity, conductivity, self-diffusion rho D_kk) of species k at T."""
+    out = np.zeros(3)
+    lib = load()
+    lib.orc_transport_species.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_int,
+                                          ctypes.c_void_p]
+    lib.orc_transport_species(ctypes.byref(m), ctypes.byref(t), T, k, _p(out))
+    return out
+
+
 def _chem_protos(lib):
     D, I, P = ctypes.c_double, ctypes.c_int, ctypes.c_void_p
     M = ctypes.POINTER(OrcMech)
@@ -304,6 +328,7 @@ def chem_rok4e(m, rho, E, rhoY, T, t_end, rtol=1e-6, atol=1e-12, h=1e-8, max_att
 # ---------------------------------------------------------------- mixture block
 def _mix_protos(lib):
     D, I, P = ctypes.c_double, ctypes.c_int, ctypes.c_void_p
+    lib.orc_mix_positivity_point.argtypes = [ctypes.POINTER(OrcParams), I, I, D, P, P, P, P, I, P]
     _chem_protos(lib)
     lib.orc_mix_create.restype = P
     lib.orc_mix_create.argtypes = [ctypes.POINTER(OrcParams), ctypes.POINTER(OrcMech), I, I, I, I, P, I, P, P]
@@ -325,6 +350,21 @@ def _mix_protos(lib):
     lib.orc_mix_set_step.argtypes = [P, D]
     lib.orc_mix_point_flux.argtypes = [ctypes.POINTER(OrcParams), ctypes.POINTER(OrcMech), P, P, P, P, D, D, D, D,
                                        P]
+
+
+def mix_positivity_point(p, dt, q, k, jac, snd, fc, dim=3):
+    """Species positivity limiter on one interface (stencil q (n, 5+ns), k (n, 3), jac (n,), snd (n,)); returns
+    the limited copy of fc (5+ns,)."""
+    lib = load()
+    _mix_protos(lib)
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    k = np.ascontiguousarray(k, dtype=np.float64)
+    jac = np.ascontiguousarray(jac, dtype=np.float64)
+    snd = np.ascontiguousarray(snd, dtype=np.float64)
+    out = np.array(fc, dtype=np.float64)
+    lib.orc_mix_positivity_point(ctypes.byref(p), q.shape[1] - 5, dim, dt, _p(q), _p(k), _p(jac), _p(snd),
+                                 q.shape[0], _p(out))
+    return out
 
 
 def constant_gamma_mech(n_species=1, mw=28.9647, cp_over_r=3.5):
