@@ -17,9 +17,11 @@ CV, BV, DQ, SV, JAC, METRIC, FV, GV, HV = 0, 1, 2, 3, 4, 5, 6, 7, 8
 # exported symbols, in the order include/corefl_c.h declares them
 SYMBOLS = (
     "corefl_create", "corefl_create_mem", "corefl_destroy", "corefl_last_error", "corefl_n_blocks",
-    "corefl_block_view_get", "corefl_get_array", "corefl_set_state", "corefl_rhs", "corefl_compute_dt",
+    "corefl_block_view_get", "corefl_get_array", "corefl_set_state", "corefl_rhs", "corefl_set_dt",
+    "corefl_compute_dt",
     "corefl_rk3_step", "corefl_advance", "corefl_get_time", "corefl_synchronize", "corefl_get_int",
     "corefl_profile_kernels", "corefl_get_real", "corefl_chemistry", "corefl_set_chemistry_surrogate",
+    "corefl_chem_substep",
     "corefl_chem_create", "corefl_chem_create_mem", "corefl_chem_destroy", "corefl_chem_info", "corefl_chem_name",
     "corefl_chem_table", "corefl_chem_set_cells", "corefl_chem_set_cell_dt", "corefl_chem_get_cells", "corefl_chem_view_get",
     "corefl_chem_source", "corefl_chem_integrate", "corefl_chem_set_launch", "corefl_chem_get_launch",
@@ -67,7 +69,9 @@ def load_library(path=LIB_PATH):
     lib.corefl_block_view_get.argtypes = [P, I, ctypes.POINTER(BlockView)]
     lib.corefl_get_array.argtypes = [P, I, I, P, ctypes.c_longlong]
     lib.corefl_set_state.argtypes = [P, I, P, ctypes.c_longlong]
-    lib.corefl_rhs.argtypes = [P, P]
+    lib.corefl_rhs.argtypes = [P, I, P]
+    lib.corefl_set_dt.argtypes = [P, D]
+    lib.corefl_chem_substep.argtypes = [P, D, I, P]
     lib.corefl_compute_dt.argtypes = [P, ctypes.POINTER(D)]
     lib.corefl_rk3_step.argtypes = [P, ctypes.POINTER(D), P]
     lib.corefl_advance.argtypes = [P, I, I, P]
@@ -194,9 +198,17 @@ class Solver:
         self._surrogate = surrogate
         _check(self._lib.corefl_set_chemistry_surrogate(self._ctx, surrogate._ctx if surrogate else None))
 
-    def rhs(self):
-        _check(self._lib.corefl_rhs(self._ctx, None))
+    def rhs(self, stage=-1):
+        """stage -1: residual dq of the current state; 0, 1, 2: that SSP-RK3 stage (residual and update)."""
+        _check(self._lib.corefl_rhs(self._ctx, int(stage), None))
         self.sync()
+
+    def set_dt(self, dt):
+        _check(self._lib.corefl_set_dt(self._ctx, float(dt)))
+
+    def chem_substep(self, dt, mode=0):
+        """One chemistry sub-step of length dt: mode 0 exact, 1 surrogate with gate and exact fallback."""
+        _check(self._lib.corefl_chem_substep(self._ctx, float(dt), int(mode), None))
 
     def compute_dt(self):
         d = ctypes.c_double()

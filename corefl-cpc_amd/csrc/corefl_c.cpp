@@ -122,6 +122,8 @@ struct corefl_ctx {
   bool fixed_dt = false;
   double t_end = 1e10;
   double host_dt = 0; // source of host-to-device copies of the step size
+  double *sub_dt_dev = nullptr; // length of a caller's chemistry sub-step (corefl_chem_substep)
+  double sub_dt = 0;
   hipGraphExec_t graph_exec = nullptr;
   hipGraph_t graph = nullptr;
   std::vector<void *> allocations;
@@ -202,17 +204,19 @@ struct corefl_ctx {
         if (f.type == 8) HIPCHK(corefl::launch_viscous_exchange(b.dev, f, s));
   }
 
-  // Exact chemistry over scale * dt (dt from the device) in every interior cell of every block
-  // (constant density and internal energy), then boundary conditions.
-  void chem_substep(double scale, hipStream_t s) {
+  // Chemistry over scale * (*dtp) (a device scalar) in every interior cell of every block (constant density
+  // and internal energy), exact or through the surrogate with its gate and exact fallback, then boundary
+  // conditions and the block-to-block exchange.
+  void chem_substep(double scale, hipStream_t s, const double *dtp = nullptr, int use_surrogate = -1) {
+    const bool surr = use_surrogate < 0 ? surrogate != nullptr : use_surrogate != 0;
     for (auto &b : blocks) {
       corefl::chem_ensure_batch(chem, b.dev.n);
       HIPCHK(corefl::launch_mix_chem_gather(b.dev, b.mix, chem->b, s));
       corefl::ChemBatch cb = chem->b;
       cb.h = b.hstep; // this block's carried step sizes
-      cb.dt_dev = dt_dev;
+      cb.dt_dev = dtp ? dtp : dt_dev;
       cb.dt_scale = scale;
-      if (surrogate)
+      if (surr)
         corefl::surrogate_substep(chem, surrogate, cb, chem_rtol, chem_atol, chem_max_attempts, s);
       else
         HIPCHK(corefl::launch_chem_integrate(chem->mech_dev, ns, cb, 0.0, chem_rtol, chem_atol, chem_max_attempts,
@@ -489,6 +493,7 @@ void build(corefl_ctx *ctx, std::vector<corefl::HostBlock> blocks) {
   HIPCHK(hipStreamCreate(&ctx->stream));
   ctx->dt_dev = ctx->alloc(1);
   ctx->time_dev = ctx->alloc(1);
+  ctx->sub_dt_dev = ctx->alloc(1);
   // Block-to-block faces: the coincident layer on a side of the block, the in-face ranges inside it, and every
   // partner node inside the target block (the exchange kernels index both blocks without further checks).
   for (size_t bi = 0; bi < blocks.size(); ++bi) {
@@ -787,10 +792,47 @@ int corefl_set_state(corefl_ctx *ctx, int block, const double *bv_host, long lon
   }
 }
 
-int corefl_rhs(corefl_ctx *ctx, void *stream) {
+int corefl_rhs(corefl_ctx *ctx, int stage, void *stream) {
   try {
-    ctx->flush_exchange(ctx->pick(stream));
-    ctx->stage(-1, ctx->pick(stream));
+    if (!ctx) throw std::runtime_error("null context");
+    if (stage > 2) throw std::runtime_error("stage must be -1 (residual only) or an SSP-RK3 stage 0, 1, 2");
+    if (stage >= 0 && ctx->wu_method >= 0)
+      throw std::runtime_error("Wu's splitting advances whole steps only (corefl_rk3_step / corefl_advance)");
+    hipStream_t s = ctx->pick(stream);
+    ctx->flush_exchange(s);
+    ctx->stage(stage < 0 ? -1 : stage, s);
+    return 0;
+  } catch (const std::exception &e) {
+    return fail(e);
+  }
+}
+
+int corefl_set_dt(corefl_ctx *ctx, double dt) {
+  try {
+    if (!ctx || !(dt > 0)) throw std::runtime_error("corefl_set_dt: need a context and dt > 0");
+    ctx->host_dt = dt;
+    HIPCHK(hipMemcpyAsync(ctx->dt_dev, &ctx->host_dt, sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return 0;
+  } catch (const std::exception &e) {
+    return fail(e);
+  }
+}
+
+int corefl_chem_substep(corefl_ctx *ctx, double dt, int mode, void *stream) {
+  try {
+    if (!ctx) throw std::runtime_error("null context");
+    if (!ctx->chem || !ctx->strang)
+      throw std::runtime_error("chemistry sub-steps need a reacting mixture with temporal_scheme 5");
+    if (!(dt > 0)) throw std::runtime_error("dt must be positive");
+    if (mode != 0 && mode != 1) throw std::runtime_error("mode must be 0 (exact) or 1 (surrogate with gate and fallback)");
+    if (mode == 1 && !ctx->surrogate) throw std::runtime_error("mode 1 needs corefl_set_chemistry_surrogate first");
+    hipStream_t s = ctx->pick(stream);
+    ctx->flush_exchange(s);
+    ctx->sub_dt = dt;
+    HIPCHK(hipMemcpyAsync(ctx->sub_dt_dev, &ctx->sub_dt, sizeof(double), hipMemcpyHostToDevice, s));
+    ctx->chem_substep(1.0, s, ctx->sub_dt_dev, mode);
+    HIPCHK(hipStreamSynchronize(s));
     return 0;
   } catch (const std::exception &e) {
     return fail(e);

@@ -80,9 +80,18 @@ int corefl_get_array(corefl_ctx *ctx, int block, int which, double *host, long l
  * (src/DataCommunication.cu:6-103) runs before the next computation, once every block has been set. */
 int corefl_set_state(corefl_ctx *ctx, int block, const double *bv_host, long long n);
 
-/* One right-hand-side evaluation dq on every block: viscous flux (src/ViscousScheme.cu:7-73) and
- * inviscid flux (src/InviscidScheme.cu:12-26) of the current state; dq is left in device memory. */
-int corefl_rhs(corefl_ctx *ctx, void *stream);
+/* Right-hand side of the current state on every block: viscous flux (src/ViscousScheme.cu:7-73) and
+ * inviscid flux (src/InviscidScheme.cu:12-26).  stage = -1 writes the residual dq and leaves the state
+ * unchanged.  stage = 0, 1, 2 runs that SSP-RK3 stage as the reference's RK3 loop does (src/RK.cuh:137-192):
+ * the residual, then update_cv_and_bv_rk with the stage's coefficients (src/RK.cuh:224-280; stage 0 also
+ * saves U^n), boundary conditions and the block-to-block exchange, with the step size held on the device
+ * (corefl_compute_dt, corefl_set_dt, the setup's fixed dt or the last corefl_rk3_step).  Stages of Wu's
+ * splitting are not exposed (error). */
+int corefl_rhs(corefl_ctx *ctx, int stage, void *stream);
+
+/* Step size held on the device for the following stages (the host side of DParameter::dt that
+ * global_time_step writes, src/TimeAdvanceFunc.cu:102-127). */
+int corefl_set_dt(corefl_ctx *ctx, double dt);
 
 /* CFL time step: local_time_step + global_time_step (src/TimeAdvanceFunc.cuh:28-115,
  * src/TimeAdvanceFunc.cu:102-127), reduced on the device; copies the value to *dt_out. */
@@ -127,6 +136,13 @@ int corefl_chemistry(corefl_ctx *ctx, int block, corefl_chem **out);
  * fallback) instead of the exact integrator; NULL restores the exact sub-steps.  s is not owned and must
  * outlive its use.  No reference counterpart. */
 int corefl_set_chemistry_surrogate(corefl_ctx *ctx, corefl_surrogate *s);
+/* One chemistry sub-step of length dt over every interior cell of every block at constant density and
+ * internal energy, as the split step's half steps run it, then boundary conditions and block exchanges.
+ * mode 0: the batched exact ROK4E; mode 1: the surrogate set by corefl_set_chemistry_surrogate (network,
+ * projection, gate, exact fallback; a gate wide open sends every cell through the network).  Needs
+ * temporal_scheme 5.  Replaces the reaction part of the reference's split step (wu_reaction_step,
+ * src/StrangSplitting.cu:373-421, which integrates each cell with its own sub-steps). */
+int corefl_chem_substep(corefl_ctx *ctx, double dt, int mode, void *stream);
 
 /* ---------------------------------------------------------------------------------------------------
  * Finite-rate chemistry of cell batches: CHEMKIN mechanism + NASA-7 thermo, the chemical source, and the

@@ -250,3 +250,27 @@ def test_viscous_wu_splitting_step():
     err = rel_err(interior(S.get(corefl.CV), 14, shape, g), interior(B.get(0), 14, shape, g))
     print("viscous wu rel err", err, flush=True)
     assert np.max(err) < 1e-10
+
+
+def test_chem_substep_entry_matches_oracle():
+    """corefl_chem_substep (mode 0, exact) on every interior cell against the oracle's chemistry sub-step, then a
+    split step's three RK stages driven through corefl_rhs(stage) against the oracle's SSP-RK3 step."""
+    S, B = make_mix_pair(_case(4, 2, temporal=5, reaction=1), peak=(2300.0, 2400.0))
+    shape, g = (N, N, N), S.get_int("ngg")
+    dt = B.cfl_dt()
+    S.chem_substep(0.5 * dt, mode=0)
+    B.chem_substep(0.5 * dt)
+    cv, cvo = interior(S.get(corefl.CV), 14, shape, g), interior(B.get(0), 14, shape, g)
+    err = rel_err(cv, cvo)
+    print("chem sub-step rel err", err, flush=True)
+    assert np.max(err) < 1e-12
+    assert np.max(cv[5 + 7]) > 0  # the fresh mixture holds no water: the sub-step reacted
+    S.set_dt(dt)
+    for stage in range(3):
+        S.rhs(stage)
+    B.step(dt)
+    err = rel_err(interior(S.get(corefl.CV), 14, shape, g), interior(B.get(0), 14, shape, g))
+    print("staged RK3 rel err", err, flush=True)
+    assert np.max(err) < 1e-12
+    with pytest.raises(corefl.CoreflError, match="mode"):
+        S.chem_substep(dt, mode=2)
