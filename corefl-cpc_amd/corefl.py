@@ -19,7 +19,7 @@ SYMBOLS = (
     "corefl_create", "corefl_create_mem", "corefl_destroy", "corefl_last_error", "corefl_n_blocks",
     "corefl_block_view_get", "corefl_get_array", "corefl_set_state", "corefl_rhs", "corefl_set_dt",
     "corefl_compute_dt",
-    "corefl_rk3_step", "corefl_advance", "corefl_get_time", "corefl_synchronize", "corefl_get_int",
+    "corefl_rk3_step", "corefl_advance", "corefl_residual", "corefl_limit_flow", "corefl_get_time", "corefl_synchronize", "corefl_get_int",
     "corefl_profile_kernels", "corefl_get_real", "corefl_chemistry", "corefl_set_chemistry_surrogate",
     "corefl_chem_substep",
     "corefl_chem_create", "corefl_chem_create_mem", "corefl_chem_destroy", "corefl_chem_info", "corefl_chem_name",
@@ -75,6 +75,8 @@ def load_library(path=LIB_PATH):
     lib.corefl_compute_dt.argtypes = [P, ctypes.POINTER(D)]
     lib.corefl_rk3_step.argtypes = [P, ctypes.POINTER(D), P]
     lib.corefl_advance.argtypes = [P, I, I, P]
+    lib.corefl_residual.argtypes = [P, ctypes.POINTER(D), ctypes.POINTER(I), ctypes.POINTER(I)]
+    lib.corefl_limit_flow.argtypes = [P, P]
     lib.corefl_get_time.argtypes = [P, ctypes.POINTER(D)]
     lib.corefl_synchronize.argtypes = [P]
     lib.corefl_get_int.argtypes = [P, ctypes.c_char_p, ctypes.POINTER(I)]
@@ -222,6 +224,16 @@ class Solver:
 
     def advance(self, n_steps, graph=True):
         _check(self._lib.corefl_advance(self._ctx, int(n_steps), 1 if graph else 0, None))
+
+    def residual(self):
+        """(res[4] of the last monitored step: rho, |V|, p, T; NaN flag; number of monitored steps)."""
+        r = (ctypes.c_double * 4)()
+        nan, cnt = ctypes.c_int(), ctypes.c_int()
+        _check(self._lib.corefl_residual(self._ctx, r, ctypes.byref(nan), ctypes.byref(cnt)))
+        return np.array(list(r)), bool(nan.value), cnt.value
+
+    def limit_flow(self):
+        _check(self._lib.corefl_limit_flow(self._ctx, None))
 
     def time(self):
         d = ctypes.c_double()

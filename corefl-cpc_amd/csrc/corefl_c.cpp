@@ -18,6 +18,7 @@
 #include "flow.hpp"
 #include "flow_mix.hpp"
 #include "mesh.hpp"
+#include "monitor.hpp"
 #include "parameter.hpp"
 
 namespace {
@@ -108,6 +109,9 @@ struct Block {
   std::vector<int> inner_target;       // their partner blocks
   long part_offset = 0;
   int part_count = 0;
+  double *last = nullptr;          // rho, |V|, p, T at the start of a monitored step (4 * n)
+  unsigned char *lflag = nullptr;  // limit_flow flags (n)
+  long res_offset = 0;             // this block's residual partials
 };
 
 } // namespace
@@ -126,6 +130,18 @@ struct corefl_ctx {
   double sub_dt = 0;
   hipGraphExec_t graph_exec = nullptr;
   hipGraph_t graph = nullptr;
+  // step monitor (compute_residual, Residual.cu:8-121) and limit_flow (TimeAdvanceFunc.cuh:117-220)
+  int output_screen = 0;            // residual on step 1 and every output_screen steps; 0: never
+  bool limit_flow = false;
+  corefl::FlowLimits limits{};
+  corefl::MonitorState *mon = nullptr;      // device
+  corefl::MonitorState *mon_host = nullptr; // pinned copy read after a monitored step
+  double *res_parts = nullptr;
+  long n_res_parts = 0;
+  double n_nodes_total = 0;
+  long step_count = 0;              // steps taken (the reference's step counter)
+  hipGraphExec_t graph_exec_mon = nullptr; // the captured monitored step
+  hipGraph_t graph_mon = nullptr;
   std::vector<void *> allocations;
   // mixtures
   int ns = 0;
@@ -146,8 +162,8 @@ struct corefl_ctx {
   }
   ~corefl_ctx() {
     delete chem;
-    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
-    if (graph) (void)hipGraphDestroy(graph);
+    drop_graph();
+    if (mon_host) (void)hipHostFree(mon_host);
     for (void *p : allocations) (void)hipFree(p);
     if (stream) (void)hipStreamDestroy(stream);
   }
@@ -158,8 +174,21 @@ struct corefl_ctx {
   void drop_graph() {
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     if (graph) (void)hipGraphDestroy(graph);
+    if (graph_exec_mon) (void)hipGraphExecDestroy(graph_exec_mon);
+    if (graph_mon) (void)hipGraphDestroy(graph_mon);
     graph_exec = nullptr;
     graph = nullptr;
+    graph_exec_mon = nullptr;
+    graph_mon = nullptr;
+  }
+
+  // The reference stores the state at steps that are multiples of output_screen and computes the residual
+  // after those steps and after step 1 (RK.cuh:92-96, 196-205).
+  bool monitored(long step) const { return output_screen > 0 && (step == 1 || step % output_screen == 0); }
+  bool stores_last(long step) const { return output_screen > 0 && step % output_screen == 0; }
+
+  void limit(Block &b, hipStream_t s) {
+    if (limit_flow) HIPCHK(corefl::launch_limit_flow(b.dev, b.mix, limits, b.lflag, s));
   }
 
   void apply_bcs(Block &b, hipStream_t s) {
@@ -196,12 +225,13 @@ struct corefl_ctx {
   // exchange_value task 1 (DataCommunication.cu:408-456): block-to-block faces, then periodic faces.
   void viscous_exchange(hipStream_t s) {
     if (scheme.viscous_order == 2) return; // face fluxes come from the exchanged state itself
+    const int nv = 5 + ns;
     for (auto &b : blocks)
       for (size_t l = 0; l < b.inner.size(); ++l)
-        HIPCHK(corefl::launch_inner_viscous_exchange(b.dev, blocks[b.inner_target[l]].dev, b.inner[l], s));
+        HIPCHK(corefl::launch_inner_viscous_exchange(b.dev, blocks[b.inner_target[l]].dev, b.inner[l], s, nv));
     for (auto &b : blocks)
       for (const auto &f : b.faces)
-        if (f.type == 8) HIPCHK(corefl::launch_viscous_exchange(b.dev, f, s));
+        if (f.type == 8) HIPCHK(corefl::launch_viscous_exchange(b.dev, f, s, nv));
   }
 
   // Chemistry over scale * (*dtp) (a device scalar) in every interior cell of every block (constant density
@@ -242,7 +272,10 @@ struct corefl_ctx {
         if (scheme.inviscid_scheme == 2) HIPCHK(corefl::launch_mix_entropy_fix(b.dev, b.mix, scheme, s));
         for (int d = 0; d < b.dev.dim; ++d) HIPCHK(corefl::launch_mix_inviscid_flux(b.dev, b.mix, scheme, d, s));
         HIPCHK(corefl::launch_mix_update(b.dev, b.mix, rhs_only ? 0 : rk, dt_dev, rhs_only, s));
-        if (!rhs_only) apply_bcs(b, s);
+        if (!rhs_only) {
+          limit(b, s);
+          apply_bcs(b, s);
+        }
         continue;
       }
       if (scheme.inviscid_scheme == 2) HIPCHK(corefl::launch_entropy_fix(b.dev, scheme, s));
@@ -250,7 +283,10 @@ struct corefl_ctx {
       HIPCHK(corefl::launch_inviscid_flux(b.dev, scheme, 1, s));
       if (b.dev.dim == 3) HIPCHK(corefl::launch_inviscid_flux(b.dev, scheme, 2, s));
       HIPCHK(corefl::launch_update(b.dev, scheme, rhs_only ? 0 : rk, dt_dev, rhs_only, s));
-      if (!rhs_only) apply_bcs(b, s);
+      if (!rhs_only) {
+        limit(b, s);
+        apply_bcs(b, s);
+      }
     }
     if (!rhs_only) communicate(s);
   }
@@ -264,46 +300,63 @@ struct corefl_ctx {
     HIPCHK(corefl::launch_dt_reduce(parts, n_parts, dt_dev, time_dev, t_end, clip, s));
   }
 
-  void mix_fluxes(Block &b, hipStream_t s) {
-    if (scheme.viscous_order > 0) HIPCHK(corefl::launch_mix_viscous(b.dev, b.mix, scheme, s));
-    if (scheme.inviscid_scheme == 2) HIPCHK(corefl::launch_mix_entropy_fix(b.dev, b.mix, scheme, s));
-    for (int d = 0; d < b.dev.dim; ++d) HIPCHK(corefl::launch_mix_inviscid_flux(b.dev, b.mix, scheme, d, s));
+  // Viscous fluxes of every block (and their exchange for the collocated scheme), then per block the inviscid
+  // fluxes and `update`.
+  template<class F> void mix_fluxes(hipStream_t s, F &&update) {
+    if (scheme.viscous_order > 0) {
+      for (auto &b : blocks) HIPCHK(corefl::launch_mix_viscous(b.dev, b.mix, scheme, s));
+      viscous_exchange(s);
+    }
+    for (auto &b : blocks) {
+      if (scheme.inviscid_scheme == 2) HIPCHK(corefl::launch_mix_entropy_fix(b.dev, b.mix, scheme, s));
+      for (int d = 0; d < b.dev.dim; ++d) HIPCHK(corefl::launch_mix_inviscid_flux(b.dev, b.mix, scheme, d, s));
+      update(b);
+    }
   }
 
   // wu_splitting, one step (StrangSplitting.cu:683-861): the transport residual Cn at U^n (compute_cn) kept in
   // dq; the per-cell reaction sub-step with Cn frozen (wu_reaction_subStep); then three SSP-RK3 stages over
   // dt/2 of R(U) - Cn (wu_RK_subStep).  Boundary conditions and block exchanges after each part.
   void wu_step(hipStream_t s) {
-    for (auto &b : blocks) {
-      mix_fluxes(b, s);
-      HIPCHK(corefl::launch_mix_update(b.dev, b.mix, 0, dt_dev, true, s));
-    }
+    mix_fluxes(s, [&](Block &b) { HIPCHK(corefl::launch_mix_update(b.dev, b.mix, 0, dt_dev, true, s)); });
     for (auto &b : blocks) {
       HIPCHK(corefl::launch_mix_wu_reaction(b.dev, b.mix, dt_dev, wu_method, s));
       apply_bcs(b, s);
     }
     communicate(s);
     for (int rk = 0; rk < 3; ++rk) {
-      for (auto &b : blocks) {
-        mix_fluxes(b, s);
+      mix_fluxes(s, [&](Block &b) {
         HIPCHK(corefl::launch_mix_wu_update(b.dev, b.mix, rk, dt_dev, s));
         apply_bcs(b, s);
-      }
+      });
       communicate(s);
     }
   }
 
-  void enqueue_step(hipStream_t s, bool compute_dt) {
+  // One step; store: save rho, |V|, p, T first; residual: the monitor's reduction after the step.
+  void enqueue_step(hipStream_t s, bool compute_dt, bool store = false, bool residual = false) {
+    if (store)
+      for (auto &b : blocks) HIPCHK(corefl::launch_store_last(b.dev, b.last, s));
     if (compute_dt) enqueue_dt(s, 1);
     if (wu_method >= 0) {
       wu_step(s);
-      HIPCHK(corefl::launch_time_advance(time_dev, dt_dev, s));
-      return;
+    } else {
+      if (strang) chem_substep(0.5, s);
+      for (int rk = 0; rk < 3; ++rk) stage(rk, s);
+      if (strang) chem_substep(0.5, s);
     }
-    if (strang) chem_substep(0.5, s);
-    for (int rk = 0; rk < 3; ++rk) stage(rk, s);
-    if (strang) chem_substep(0.5, s);
     HIPCHK(corefl::launch_time_advance(time_dev, dt_dev, s));
+    if (residual) {
+      for (auto &b : blocks) HIPCHK(corefl::launch_residual_partial(b.dev, b.last, res_parts + 4 * b.res_offset, s));
+      HIPCHK(corefl::launch_residual_finish(res_parts, n_res_parts, n_nodes_total, mon, s));
+    }
+  }
+
+  // After a monitored step: wait for it and stop on a NaN residual, as compute_residual does (Residual.cu:100-118).
+  void check_monitor(hipStream_t s, long step) {
+    HIPCHK(hipMemcpyAsync(mon_host, mon, sizeof(corefl::MonitorState), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (mon_host->nan) throw std::runtime_error("NaN occurred in step " + std::to_string(step) + ". Stop simulation.");
   }
 };
 
@@ -421,10 +474,11 @@ void build(corefl_ctx *ctx, std::vector<corefl::HostBlock> blocks) {
     if (fs && P.get_real("shockSensor_threshold") > 1e-10)
       throw std::runtime_error("shockSensor_threshold > 0 (hybrid UD/WENO scheme) is not on this path");
 
-    if (vo != 0 && vo != 2) throw std::runtime_error("with species, viscous_order must be 0 or 2 (face-centred)");
-    if (vo == 2 && P.get_bool("gradPInDiffusionFlux"))
+    if (vo != 0 && vo != 2 && vo != 8)
+      throw std::runtime_error("with species, viscous_order must be 0, 2 (face-centred) or 8 (collocated)");
+    if (vo > 0 && P.get_bool("gradPInDiffusionFlux"))
       throw std::runtime_error("gradPInDiffusionFlux (pressure-driven species diffusion) is not on this path");
-    if (vo == 2) {
+    if (vo > 0) {
       // Species::read_tran (ChemData.cpp:90-98, 740-831)
       corefl::parse_transport(read_file(path(P.get_string("transport_file"))), ctx->chem->mech);
     }
@@ -469,26 +523,48 @@ void build(corefl_ctx *ctx, std::vector<corefl::HostBlock> blocks) {
   const int ngg = P.get_int("ngg");
   P.set_int("dimension", blocks.at(0).dim);
   // reference state and the WENO epsilon scale (BoundCond.cpp:276-295, DParameter.cu:25-28, WENO.cu:8)
-  double rho_inf, v_inf;
+  double rho_inf, v_inf, p_inf, c_inf;
   if (P.get_string("canonical_problem") == "taylorGreen") {
     const auto &s = P.get_struct("taylorGreen");
     const double T0 = corefl::struct_real(s, "temperature", 300.0), p0 = corefl::struct_real(s, "pressure", 101325.0);
     const double R = kRu / kMwAir;
     rho_inf = p0 / (R * T0);
-    v_inf = corefl::struct_real(s, "mach", 0.1) * std::sqrt(kGamma * R * T0);
+    c_inf = std::sqrt(kGamma * R * T0);
+    v_inf = corefl::struct_real(s, "mach", 0.1) * c_inf;
+    p_inf = p0;
   } else {
     const FlowState r = inflow_state(P.get_struct(P.get_string("reference_state")), mech);
     rho_inf = r.rho;
+    p_inf = r.p;
+    c_inf = r.c;
     v_inf = std::abs(std::sqrt(r.u * r.u + r.v * r.v + r.w * r.w)) < 1 ? r.c : std::sqrt(r.u * r.u + r.v * r.v + r.w * r.w);
   }
   P.set_real("rho_inf", rho_inf);
   P.set_real("v_inf", v_inf);
+  P.set_real("p_inf", p_inf);
+  P.set_real("speed_of_sound", c_inf);
   P.set_real("weno_eps", 1e-40 * rho_inf * v_inf * rho_inf * v_inf);
   ctx->scheme = {inv, P.get_int("reconstruction"), vo, P.get_real("weno_eps"), P.get_real("entropy_fix_factor"),
                  P.get_real("prandtl_number"), P.get_real("cfl"), v_inf, P.get_bool("positive_preserving") ? 1 : 0,
                  rho_inf};
   ctx->fixed_dt = P.get_bool("fixed_time_step");
   ctx->t_end = P.get_real("total_simulation_time");
+
+  ctx->output_screen = P.get_int("output_screen");
+  ctx->limit_flow = P.get_bool("limit_flow");
+  {
+    // limit_flow bounds (DParameter.cu:332-349)
+    auto &L = ctx->limits;
+    L.ll[0] = 1e-6 * rho_inf;
+    L.ul[0] = 1e+3 * rho_inf;
+    const double vref = std::abs(v_inf) < 1 ? c_inf : v_inf;
+    for (int l = 1; l < 4; ++l) {
+      L.ll[l] = -1000 * vref;
+      L.ul[l] = 1000 * vref;
+    }
+    L.ll[4] = 1e-6 * p_inf;
+    L.ul[4] = 1e+3 * p_inf;
+  }
 
   HIPCHK(hipStreamCreate(&ctx->stream));
   ctx->dt_dev = ctx->alloc(1);
@@ -580,6 +656,7 @@ void build(corefl_ctx *ctx, std::vector<corefl::HostBlock> blocks) {
       d.gv = ctx->alloc(nv * sz);
       d.hv = ctx->alloc(nv * sz);
       if (ctx->ns) {
+        B.mix.visc_order = vo;
         B.mix.mul = ctx->alloc(sz);
         B.mix.cond = ctx->alloc(sz);
         B.mix.rhod = ctx->alloc(ctx->ns * sz);
@@ -617,9 +694,22 @@ void build(corefl_ctx *ctx, std::vector<corefl::HostBlock> blocks) {
     B.part_offset = ctx->n_parts;
     B.part_count = corefl::dt_partial_count(d);
     ctx->n_parts += B.part_count;
+    if (ctx->output_screen > 0) {
+      B.last = ctx->alloc(4 * d.n); // zero: the reference's first residual (step 1) measures against zero
+      B.res_offset = ctx->n_res_parts;
+      ctx->n_res_parts += corefl::residual_partial_count(d);
+      ctx->n_nodes_total += (double)d.n;
+    }
+    if (ctx->limit_flow) B.lflag = reinterpret_cast<unsigned char *>(ctx->alloc((d.n + 7) / 8));
     ctx->blocks.push_back(std::move(B));
   }
   ctx->parts = ctx->alloc(ctx->n_parts);
+  if (ctx->output_screen > 0) {
+    ctx->res_parts = ctx->alloc(4 * ctx->n_res_parts);
+    ctx->mon = reinterpret_cast<corefl::MonitorState *>(ctx->alloc((sizeof(corefl::MonitorState) + 7) / 8));
+    HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&ctx->mon_host), sizeof(corefl::MonitorState), hipHostMallocDefault));
+    std::memset(ctx->mon_host, 0, sizeof(corefl::MonitorState));
+  }
   if (ctx->strang) { // the chemistry batch is allocated here, never inside a captured step
     long n_max = 0;
     for (auto &B : ctx->blocks) n_max = std::max(n_max, B.dev.n);
@@ -861,11 +951,14 @@ int corefl_rk3_step(corefl_ctx *ctx, double *dt_inout, void *stream) {
       ctx->host_dt = ctx->param.get_real("dt");
       HIPCHK(hipMemcpyAsync(ctx->dt_dev, &ctx->host_dt, sizeof(double), hipMemcpyHostToDevice, s));
     }
-    ctx->enqueue_step(s, !given && !ctx->fixed_dt);
+    const long step = ++ctx->step_count;
+    const bool mon = ctx->monitored(step);
+    ctx->enqueue_step(s, !given && !ctx->fixed_dt, ctx->stores_last(step), mon);
     if (dt_inout && !given) {
       HIPCHK(hipMemcpyAsync(dt_inout, ctx->dt_dev, sizeof(double), hipMemcpyDeviceToHost, s));
     }
     HIPCHK(hipStreamSynchronize(s));
+    if (mon) ctx->check_monitor(s, step);
     return 0;
   } catch (const std::exception &e) {
     return fail(e);
@@ -881,24 +974,74 @@ int corefl_advance(corefl_ctx *ctx, int n_steps, int use_graph, void *stream) {
       HIPCHK(hipMemcpyAsync(ctx->dt_dev, &ctx->host_dt, sizeof(double), hipMemcpyHostToDevice, s));
       HIPCHK(hipStreamSynchronize(s));
     }
-    if (!use_graph) {
-      for (int n = 0; n < n_steps; ++n) ctx->enqueue_step(s, !ctx->fixed_dt);
-      return 0;
-    }
-    if (!ctx->graph_exec) {
+    // Two captured steps: the plain one and, when the monitor is on, the monitored one (state saved before,
+    // residual reduced after); the monitor only ever runs on step 1 and on multiples of output_screen.
+    auto capture = [&](hipGraph_t &g, hipGraphExec_t &ge, bool store, bool residual) {
+      if (ge) return;
       HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
       try {
-        ctx->enqueue_step(s, !ctx->fixed_dt);
+        ctx->enqueue_step(s, !ctx->fixed_dt, store, residual);
       } catch (...) {
-        hipGraph_t g = nullptr;
-        (void)hipStreamEndCapture(s, &g);
-        if (g) (void)hipGraphDestroy(g);
+        hipGraph_t tmp = nullptr;
+        (void)hipStreamEndCapture(s, &tmp);
+        if (tmp) (void)hipGraphDestroy(tmp);
         throw;
       }
-      HIPCHK(hipStreamEndCapture(s, &ctx->graph));
-      HIPCHK(hipGraphInstantiate(&ctx->graph_exec, ctx->graph, nullptr, nullptr, 0));
+      HIPCHK(hipStreamEndCapture(s, &g));
+      HIPCHK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+    };
+    for (int n = 0; n < n_steps; ++n) {
+      const long step = ++ctx->step_count;
+      const bool mon = ctx->monitored(step), store = ctx->stores_last(step);
+      if (!use_graph) {
+        ctx->enqueue_step(s, !ctx->fixed_dt, store, mon);
+      } else if (mon) {
+        // step 1 measures against the zero initial store unless it is also a multiple of output_screen
+        if (!store) {
+          ctx->enqueue_step(s, !ctx->fixed_dt, false, true);
+        } else {
+          capture(ctx->graph_mon, ctx->graph_exec_mon, true, true);
+          HIPCHK(hipGraphLaunch(ctx->graph_exec_mon, s));
+        }
+      } else {
+        capture(ctx->graph, ctx->graph_exec, false, false);
+        HIPCHK(hipGraphLaunch(ctx->graph_exec, s));
+      }
+      if (mon) ctx->check_monitor(s, step);
     }
-    for (int n = 0; n < n_steps; ++n) HIPCHK(hipGraphLaunch(ctx->graph_exec, s));
+    return 0;
+  } catch (const std::exception &e) {
+    return fail(e);
+  }
+}
+
+int corefl_residual(corefl_ctx *ctx, double *res, int *nan_flag, int *n_monitored) {
+  try {
+    if (!ctx || !res) throw std::runtime_error("corefl_residual: null argument");
+    if (ctx->output_screen <= 0) throw std::runtime_error("the residual monitor is off (output_screen = 0)");
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipMemcpy(ctx->mon_host, ctx->mon, sizeof(corefl::MonitorState), hipMemcpyDeviceToHost));
+    for (int l = 0; l < 4; ++l) res[l] = ctx->mon_host->res[l];
+    if (nan_flag) *nan_flag = ctx->mon_host->nan;
+    if (n_monitored) *n_monitored = ctx->mon_host->count;
+    return 0;
+  } catch (const std::exception &e) {
+    return fail(e);
+  }
+}
+
+int corefl_limit_flow(corefl_ctx *ctx, void *stream) {
+  try {
+    if (!ctx) throw std::runtime_error("null context");
+    if (!ctx->limit_flow) throw std::runtime_error("limit_flow is off in this setup");
+    hipStream_t s = ctx->pick(stream);
+    ctx->flush_exchange(s);
+    for (auto &b : ctx->blocks) {
+      ctx->limit(b, s);
+      ctx->apply_bcs(b, s);
+    }
+    ctx->communicate(s);
+    HIPCHK(hipStreamSynchronize(s));
     return 0;
   } catch (const std::exception &e) {
     return fail(e);

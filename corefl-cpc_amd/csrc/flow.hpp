@@ -45,11 +45,13 @@ struct Scheme {
 hipError_t launch_cv_from_bv_all(const DevBlock &b, hipStream_t s);
 hipError_t launch_boundary(const DevBlock &b, const DevFace &f, hipStream_t s);
 hipError_t launch_viscous_flux(const DevBlock &b, const Scheme &sc, hipStream_t s);
-hipError_t launch_viscous_exchange(const DevBlock &b, const DevFace &f, hipStream_t s);
+// Ghost layers of the collocated viscous fluxes across a periodic face, components 1 .. nv-1.
+hipError_t launch_viscous_exchange(const DevBlock &b, const DevFace &f, hipStream_t s, int nv = 5);
 // inner_communication<Air> / inner_exchange task 1 (DataCommunication.cu:36-103, 458-495) for one face of b
 // whose partner block is t.
 hipError_t launch_inner_exchange(const DevBlock &b, const DevBlock &t, const DevInner &f, hipStream_t s);
-hipError_t launch_inner_viscous_exchange(const DevBlock &b, const DevBlock &t, const DevInner &f, hipStream_t s);
+hipError_t launch_inner_viscous_exchange(const DevBlock &b, const DevBlock &t, const DevInner &f, hipStream_t s,
+                                         int nv = 5);
 hipError_t launch_entropy_fix(const DevBlock &b, const Scheme &sc, hipStream_t s);
 hipError_t launch_inviscid_flux(const DevBlock &b, const Scheme &sc, int dir, hipStream_t s);
 // Residual + SSP-RK3 stage update; rhs_only writes dq instead of updating.  dt is read from *dt_dev.

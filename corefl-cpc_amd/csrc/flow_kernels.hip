@@ -9,6 +9,7 @@
 
 #include <cmath>
 
+#include "stencil.hpp"
 #include "weno.hpp"
 
 namespace corefl {
@@ -552,51 +553,9 @@ __global__ __launch_bounds__(256, 2) void k_fsweno(DevBlock b, Scheme sc, double
 }
 
 // ---------------------------------------------------------------- viscous terms
-// Collocated first derivative along one index (ViscousScheme.cu:129-307 for ORDER 8; ORDER 4 uses the
-// reference's fourth-order rows: the i==2 row as the interior stencil and the i==0/1 rows as closures).
-template<int ORDER>
-__device__ __forceinline__ double dd(const double *f, long id, long st, int i, int n, int lb, int rb) {
-#define FF(o) f[id + (o) * st]
-  if (lb && i < ORDER / 2) {
-    if (i == 0) return -25.0 / 12 * FF(0) + 4.0 * FF(1) - 3 * FF(2) + 4.0 / 3 * FF(3) - 0.25 * FF(4);
-    if (i == 1) return -0.25 * FF(-1) - 5.0 / 6 * FF(0) + 1.5 * FF(1) - 0.5 * FF(2) + 1.0 / 12 * FF(3);
-    if (ORDER == 8) {
-      if (i == 2) return -1.0 / 12 * (FF(2) - FF(-2)) + 2.0 / 3 * (FF(1) - FF(-1));
-      return 0.75 * (FF(1) - FF(-1)) - 0.15 * (FF(2) - FF(-2)) + 1.0 / 60 * (FF(3) - FF(-3));
-    }
-  }
-  if (rb && i >= n - ORDER / 2) {
-    if (i == n - 1) return 25.0 / 12 * FF(0) - 4.0 * FF(-1) + 3 * FF(-2) - 4.0 / 3 * FF(-3) + 0.25 * FF(-4);
-    if (i == n - 2) return 0.25 * FF(1) + 5.0 / 6 * FF(0) - 1.5 * FF(-1) + 0.5 * FF(-2) - 1.0 / 12 * FF(-3);
-    if (ORDER == 8) {
-      if (i == n - 3) return -1.0 / 12 * (FF(2) - FF(-2)) + 2.0 / 3 * (FF(1) - FF(-1));
-      return 0.75 * (FF(1) - FF(-1)) - 0.15 * (FF(2) - FF(-2)) + 1.0 / 60 * (FF(3) - FF(-3));
-    }
-  }
-  if constexpr (ORDER == 8)
-    return 0.8 * (FF(1) - FF(-1)) + -0.2 * (FF(2) - FF(-2)) + 4.0 / 105 * (FF(3) - FF(-3)) +
-           -1.0 / 280 * (FF(4) - FF(-4));
-  else
-    return -1.0 / 12 * (FF(2) - FF(-2)) + 2.0 / 3 * (FF(1) - FF(-1));
-#undef FF
-}
-
-// Interior row of dd<ORDER> on a window w[0 .. ORDER] of consecutive values centred on w[ORDER / 2].
-template<int ORDER>
-__device__ __forceinline__ double dd_mid(const double *w) {
-  constexpr int H = ORDER / 2;
-  if constexpr (ORDER == 8)
-    return 0.8 * (w[H + 1] - w[H - 1]) + -0.2 * (w[H + 2] - w[H - 2]) + 4.0 / 105 * (w[H + 3] - w[H - 3]) +
-           -1.0 / 280 * (w[H + 4] - w[H - 4]);
-  else
-    return -1.0 / 12 * (w[H + 2] - w[H - 2]) + 2.0 / 3 * (w[H + 1] - w[H - 1]);
-}
-
-// Whether dd<ORDER> at index i of n takes a one-sided boundary row.
-template<int ORDER>
-__device__ __forceinline__ bool dd_closure(int i, int n, int lb, int rb) {
-  return (lb && i < ORDER / 2) || (rb && i >= n - ORDER / 2);
-}
+using stencil::dd;
+using stencil::dd_closure;
+using stencil::dd_mid;
 
 // z-marching node kernels (k_viscous_flux, k_update): each thread walks its (i, j) column through kz
 // consecutive planes and keeps the values its z stencil shares between neighbouring planes in registers, so
@@ -772,8 +731,8 @@ __device__ __forceinline__ bool face_item(const DevBlock &b, const DevFace &f, F
 }
 
 // periodic_exchange task 1 (DataCommunication.cu:628-683): average coincident face nodes, copy ghosts of
-// the collocated viscous fluxes (components 1..4; component 0 of the viscous fluxes is never used).
-__global__ __launch_bounds__(256) void k_viscous_exchange(DevBlock b, DevFace f) {
+// the collocated viscous fluxes (components 1..nv-1; component 0 of the viscous fluxes is never used).
+__global__ __launch_bounds__(256) void k_viscous_exchange(DevBlock b, DevFace f, int nv) {
   FaceItem it;
   if (!face_item(b, f, it)) return;
   int o[3] = {it.i, it.j, it.k};
@@ -785,7 +744,7 @@ __global__ __launch_bounds__(256) void k_viscous_exchange(DevBlock b, DevFace f)
   double *A[3] = {b.fv, b.gv, b.hv};
   if (it.g == 1) {
     const long id = nid(b, it.i, it.j, it.k), io = nid(b, o[0], o[1], o[2]);
-    for (int l = 1; l < 5; ++l)
+    for (int l = 1; l < nv; ++l)
       for (int a = 0; a < 3; ++a) {
         const double av = 0.5 * (A[a][id + l * sz] + A[a][io + l * sz]);
         A[a][id + l * sz] = av;
@@ -795,7 +754,7 @@ __global__ __launch_bounds__(256) void k_viscous_exchange(DevBlock b, DevFace f)
   const int g = it.g;
   const long ig = nid(b, it.i + g * dv[0], it.j + g * dv[1], it.k + g * dv[2]);
   const long is = nid(b, o[0] + g * dv[0], o[1] + g * dv[1], o[2] + g * dv[2]);
-  for (int l = 1; l < 5; ++l)
+  for (int l = 1; l < nv; ++l)
     for (int a = 0; a < 3; ++a) A[a][ig + l * sz] = A[a][is + l * sz];
 }
 
@@ -1023,11 +982,11 @@ __global__ __launch_bounds__(256) void k_inner_exchange(DevBlock b, DevBlock t, 
   cons_from_prim(b, id);
 }
 
-// inner_exchange task 1 (DataCommunication.cu:458-495): collocated viscous fluxes (components 1..4).  These
+// inner_exchange task 1 (DataCommunication.cu:458-495): collocated viscous fluxes (components 1..nv-1).  These
 // are contravariant fluxes along each block's own index directions, so the partner's flux along direction
 // vmap[a], signed by vsign[a], is this block's flux along a (the reference copies direction a to direction a,
 // which is the same thing when both blocks' index directions agree).
-__global__ __launch_bounds__(256) void k_inner_viscous_exchange(DevBlock b, DevBlock t, DevInner f) {
+__global__ __launch_bounds__(256) void k_inner_viscous_exchange(DevBlock b, DevBlock t, DevInner f, int nv) {
   int idx[3], it[3];
   if (!inner_item(f, idx, it)) return;
   const long id = nid(b, idx[0], idx[1], idx[2]), jt = nid(t, it[0], it[1], it[2]);
@@ -1037,7 +996,7 @@ __global__ __launch_bounds__(256) void k_inner_viscous_exchange(DevBlock b, DevB
   for (int a = 0; a < 3; ++a) {
     double *P = T[f.vmap[a]];
     const double sg = f.vsign[a];
-    for (int l = 1; l < 5; ++l) {
+    for (int l = 1; l < nv; ++l) {
       if (coincident) {
         const double v = 0.5 * (A[a][id + l * sz] + sg * P[jt + l * tz]);
         A[a][id + l * sz] = v;
@@ -1165,8 +1124,8 @@ hipError_t launch_viscous_flux(const DevBlock &b, const Scheme &sc, hipStream_t 
   return hipGetLastError();
 }
 
-hipError_t launch_viscous_exchange(const DevBlock &b, const DevFace &f, hipStream_t s) {
-  k_viscous_exchange<<<face_blocks(b, f), 256, 0, s>>>(b, f);
+hipError_t launch_viscous_exchange(const DevBlock &b, const DevFace &f, hipStream_t s, int nv) {
+  k_viscous_exchange<<<face_blocks(b, f), 256, 0, s>>>(b, f, nv);
   return hipGetLastError();
 }
 
@@ -1180,8 +1139,9 @@ hipError_t launch_inner_exchange(const DevBlock &b, const DevBlock &t, const Dev
   return hipGetLastError();
 }
 
-hipError_t launch_inner_viscous_exchange(const DevBlock &b, const DevBlock &t, const DevInner &f, hipStream_t s) {
-  k_inner_viscous_exchange<<<inner_blocks(f), 256, 0, s>>>(b, t, f);
+hipError_t launch_inner_viscous_exchange(const DevBlock &b, const DevBlock &t, const DevInner &f, hipStream_t s,
+                                         int nv) {
+  k_inner_viscous_exchange<<<inner_blocks(f), 256, 0, s>>>(b, t, f, nv);
   return hipGetLastError();
 }
 

@@ -22,8 +22,9 @@ struct MixArrays {
   int explicit_source;  // add the chemical source to the residual (reaction = 1 with temporal_scheme 3)
   double *tscale;       // reaction time scale per interior node (n), written with the explicit source
   const double *dt;     // the step size on the device (read by the positivity limiter)
-  // viscous mixtures (viscous_order 2): node transport properties (viscosity, conductivity: sz; rho D_k: ns * sz)
+  // viscous mixtures: node transport properties (viscosity, conductivity: sz; rho D_k: ns * sz)
   double *mul, *cond, *rhod;
+  int visc_order; // 0 inviscid, 2 face-centred fluxes, 8 collocated node fluxes (exchanged, then differenced)
 };
 
 hipError_t launch_mix_cv_from_bv_all(const DevBlock &b, const MixArrays &x, hipStream_t s);
@@ -43,8 +44,10 @@ hipError_t launch_mix_wu_reaction(const DevBlock &b, const MixArrays &x, const d
                                   hipStream_t s);
 hipError_t launch_mix_wu_update(const DevBlock &b, const MixArrays &x, int rk, const double *dt_dev, hipStream_t s);
 // Viscous mixtures: transport properties on every node (compute_transport_property, Transport.cu:43-150), then
-// the face-centred viscous fluxes with species diffusion (compute_fv/gv/hv_2nd_order<Mixture>,
-// ViscousScheme.cu:614-1010) into b.fv / b.gv / b.hv (nv components).
+// for viscous_order 2 the face-centred viscous fluxes with species diffusion (compute_fv/gv/hv_2nd_order<Mixture>,
+// ViscousScheme.cu:614-1010), for viscous_order 8 the collocated node fluxes (compute_viscous_flux_collocated +
+// _scalar, ViscousScheme.cu:309-566; their ghost layers come from the viscous-flux exchange) into b.fv / b.gv /
+// b.hv (nv components).
 hipError_t launch_mix_viscous(const DevBlock &b, const MixArrays &x, const Scheme &sc, hipStream_t s);
 hipError_t launch_mix_transport(const DevBlock &b, const MixArrays &x, hipStream_t s);
 hipError_t launch_mix_local_dt(const DevBlock &b, const MixArrays &x, const Scheme &sc, double *part, hipStream_t s);

@@ -5,6 +5,7 @@
 #include <cmath>
 
 #include "chem_device.hpp"
+#include "stencil.hpp"
 #include "weno.hpp"
 
 namespace corefl {
@@ -16,6 +17,7 @@ using wenodev::weno7_half;
 
 constexpr int NS = 9, NV = 5 + NS;
 using chemdev::kRu;
+using stencil::dd;
 
 __device__ __forceinline__ long nid(const DevBlock &b, int i, int j, int k) {
   return (long)(k + b.g) * b.d1 + (long)(j + b.g) * b.d2 + (i + b.g);
@@ -516,6 +518,94 @@ __global__ __launch_bounds__(256) void k_mix_transport(DevBlock b, MixArrays x) 
   for (int l = 0; l < NS; ++l) x.rhod[id + l * sz] = rd[l];
 }
 
+// compute_viscous_flux_collocated<Mixture, ORDER> + compute_viscous_flux_collocated_scalar<ORDER>
+// (ViscousScheme.cu:309-566, gradPInDiffusionFlux off): node fluxes fv/gv/hv, components 1..3 the stress
+// projected on each direction's metric row, 4 the work of the stress plus conduction (node conductivity) plus
+// the enthalpy carried by species diffusion, 5.. the mixture-averaged diffusion fluxes with the correction
+// velocity.  Gradients by the collocated ORDER-th central difference with one-sided rows at physical faces.
+template<int ORDER>
+__global__ __launch_bounds__(256) void k_mix_viscous_node(DevBlock b, MixArrays x) {
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+  if (i >= b.mx || j >= b.my) return;
+  const long id = nid(b, i, j, k), sz = b.sz;
+  const bool z3 = b.dim == 3;
+  const double *M = b.met + id;
+  const double xx = M[0], xy = M[sz], xz = M[2 * sz], ex = M[3 * sz], ey = M[4 * sz], ez = M[5 * sz];
+  const double zx = M[6 * sz], zy = M[7 * sz], zz = M[8 * sz];
+  // physical gradient of f (chain rule through the metric)
+  auto grad = [&](const double *f, double &gx, double &gy, double &gz) {
+    const double a = dd<ORDER>(f, id, 1, i, b.mx, b.bt[0], b.bt[1]);
+    const double e = dd<ORDER>(f, id, b.d2, j, b.my, b.bt[2], b.bt[3]);
+    const double c = z3 ? dd<ORDER>(f, id, b.d1, k, b.mz, b.bt[4], b.bt[5]) : 0.0;
+    gx = a * xx + e * ex + c * zx;
+    gy = a * xy + e * ey + c * zy;
+    gz = a * xz + e * ez + c * zz;
+  };
+  double ux, uy, uz, vx, vy, vz, wx, wy, wz, tx, ty, tz;
+  grad(b.bv + sz, ux, uy, uz);
+  grad(b.bv + 2 * sz, vx, vy, vz);
+  grad(b.bv + 3 * sz, wx, wy, wz);
+  grad(b.bv + 5 * sz, tx, ty, tz);
+  const double mu = x.mul[id];
+  const double txx = mu * (4 * ux - 2 * vy - 2 * wz) / 3.0;
+  const double tyy = mu * (4 * vy - 2 * ux - 2 * wz) / 3.0;
+  const double tzz = mu * (4 * wz - 2 * ux - 2 * vy) / 3.0;
+  const double txy = mu * (uy + vx), txz = mu * (uz + wx), tyz = mu * (vz + wy);
+  double *fv = b.fv + id, *gv = b.gv + id, *hv = b.hv + id;
+  fv[sz] = xx * txx + xy * txy + xz * txz;
+  fv[2 * sz] = xx * txy + xy * tyy + xz * tyz;
+  fv[3 * sz] = xx * txz + xy * tyz + xz * tzz;
+  gv[sz] = ex * txx + ey * txy + ez * txz;
+  gv[2 * sz] = ex * txy + ey * tyy + ez * tyz;
+  gv[3 * sz] = ex * txz + ey * tyz + ez * tzz;
+  hv[sz] = zx * txx + zy * txy + zz * txz;
+  hv[2 * sz] = zx * txy + zy * tyy + zz * tyz;
+  hv[3 * sz] = zx * txz + zy * tyz + zz * tzz;
+  const double lam = x.cond[id];
+  const double u = b.bv[id + sz], v = b.bv[id + 2 * sz], w = b.bv[id + 3 * sz];
+  const double Ex = u * txx + v * txy + w * txz + lam * tx;
+  const double Ey = u * txy + v * tyy + w * tyz + lam * ty;
+  const double Ez = u * txz + v * tyz + w * tzz + lam * tz;
+  double e4[3] = {xx * Ex + xy * Ey + xz * Ez, ex * Ex + ey * Ey + ez * Ez, zx * Ex + zy * Ey + zz * Ez};
+  const ChemMech &m = *x.mech;
+  double yk[NS], dk[NS], drv[NS][3];
+  double cvt[3] = {0, 0, 0}, sgw[3] = {0, 0, 0}, imw = 0, srdy = 0;
+#pragma unroll
+  for (int l = 0; l < NS; ++l) {
+    yk[l] = x.sv[id + l * sz];
+    dk[l] = x.rhod[id + l * sz];
+    double gx, gy, gz;
+    grad(x.sv + l * sz, gx, gy, gz);
+    drv[l][0] = xx * gx + xy * gy + xz * gz;
+    drv[l][1] = ex * gx + ey * gy + ez * gz;
+    drv[l][2] = zx * gx + zy * gy + zz * gz;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      cvt[a] += dk[l] * drv[l][a];
+      sgw[a] += drv[l][a] * m.imw[l];
+    }
+    imw += yk[l] * m.imw[l];
+    srdy += dk[l] * yk[l];
+  }
+  const double mwt = 1.0 / imw;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) cvt[a] -= mwt * srdy * sgw[a];
+  double h[NS], cp[NS];
+  chemdev::h_cp<NS>(m, b.bv[id + 5 * sz], h, cp);
+  double *F[3] = {fv, gv, hv};
+#pragma unroll
+  for (int l = 0; l < NS; ++l)
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const double jd = dk[l] * (drv[l][a] - mwt * yk[l] * sgw[a]) - yk[l] * cvt[a];
+      F[a][(5 + l) * sz] = jd;
+      e4[a] += h[l] * jd;
+    }
+  fv[4 * sz] = e4[0];
+  gv[4 * sz] = e4[1];
+  hv[4 * sz] = e4[2];
+}
+
 // compute_fv/gv/hv_2nd_order<Mixture> (ViscousScheme.cu:614-778 and the gv/hv analogues): the face flux of the
 // air scheme with the nodes' average viscosity and conductivity, plus species diffusion with the correction
 // velocity (mixture-averaged rho D_k, mass fractions and their gradients at the face) and its enthalpy flux;
@@ -656,13 +746,22 @@ __global__ __launch_bounds__(256) void k_mix_update(DevBlock b, MixArrays x, int
     r[l] = s;
   }
   const long iq = ((long)k * b.my + j) * b.mx + i;
-  if (x.mul) { // face-centred viscous fluxes (compute_dFv_dx etc., ViscousScheme.cu:75-128)
+  if (x.visc_order == 2) { // face-centred viscous fluxes (compute_dFv_dx etc., ViscousScheme.cu:75-128)
 #pragma unroll
     for (int l = 1; l < NV; ++l) {
       double v = b.fv[id + l * sz] - b.fv[id - 1 + l * sz];
       v += b.gv[id + l * sz] - b.gv[id - b.d2 + l * sz];
       if (b.dim == 3) v += b.hv[id + l * sz] - b.hv[id - b.d1 + l * sz];
       r[l] += v;
+    }
+  } else if (x.visc_order == 8) { // compute_viscous_flux_derivative<8> (ViscousScheme.cu:568-612)
+    const double J = b.jac[id];
+#pragma unroll
+    for (int l = 1; l < NV; ++l) {
+      double v = dd<8>(b.fv + l * sz, id, 1, i, b.mx, b.bt[0], b.bt[1]) +
+                 dd<8>(b.gv + l * sz, id, b.d2, j, b.my, b.bt[2], b.bt[3]);
+      if (b.dim == 3) v += dd<8>(b.hv + l * sz, id, b.d1, k, b.mz, b.bt[4], b.bt[5]);
+      r[l] += v * J;
     }
   }
   if (x.explicit_source) {
@@ -742,7 +841,7 @@ __global__ __launch_bounds__(256) void k_mix_local_dt(DevBlock b, MixArrays x, S
     double s = fmax(fabs(u * M[0] + v * M[sz] + w * M[2 * sz]) + c * gx,
                     fabs(u * M[3 * sz] + v * M[4 * sz] + w * M[5 * sz]) + c * ge);
     if (b.dim == 3) s = fmax(s, fabs(u * M[6 * sz] + v * M[7 * sz] + w * M[8 * sz]) + c * gz);
-    if (x.mul) { // viscous spectral radius (TimeAdvanceFunc.cuh:66-98)
+    if (x.visc_order) { // viscous spectral radius (TimeAdvanceFunc.cuh:66-98)
       double ml = fmax(gx, ge);
       if (b.dim == 3) ml = fmax(ml, gz);
       double h[NS], cp[NS], cpt = 0, rmax = 0, R = 0;
@@ -1188,6 +1287,11 @@ hipError_t launch_mix_viscous(const DevBlock &b, const MixArrays &x, const Schem
   if (x.ns != NS) return hipErrorNotSupported;
   k_mix_transport<<<(unsigned)((b.sz + 255) / 256), 256, 0, s>>>(b, x);
   const dim3 tpb(64, 4, 1);
+  if (sc.viscous_order == 8) {
+    k_mix_viscous_node<8><<<dim3((b.mx + 63) / 64, (b.my + 3) / 4, b.mz), tpb, 0, s>>>(b, x);
+    return hipGetLastError();
+  }
+  if (sc.viscous_order != 2) return hipErrorNotSupported;
   k_mix_viscous_face<0><<<dim3((b.mx + 1 + 63) / 64, (b.my + 3) / 4, b.mz), tpb, 0, s>>>(b, x, sc);
   k_mix_viscous_face<1><<<dim3((b.mx + 63) / 64, (b.my + 1 + 3) / 4, b.mz), tpb, 0, s>>>(b, x, sc);
   if (b.dim == 3) k_mix_viscous_face<2><<<dim3((b.mx + 63) / 64, (b.my + 3) / 4, b.mz + 1), tpb, 0, s>>>(b, x, sc);

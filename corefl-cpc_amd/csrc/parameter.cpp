@@ -44,6 +44,9 @@ Parameter::Parameter() {
   reals["prandtl_number"] = 0.72;
   reals["schmidt_number"] = 0.5;
   bools["parallel"] = false;
+  // residual monitor: the reference requires output_screen in setup.txt; 0 (here the default) turns it off
+  ints["output_screen"] = 0;
+  bools["limit_flow"] = false; // Parameter.cpp:490
 }
 
 static bool starts_with(const std::string &s, const char *p) { return s.rfind(p, 0) == 0; }

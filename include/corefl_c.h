@@ -106,6 +106,25 @@ int corefl_rk3_step(corefl_ctx *ctx, double *dt_inout, void *stream);
  * hipGraph on first use and replays it. */
 int corefl_advance(corefl_ctx *ctx, int n_steps, int use_graph, void *stream);
 
+/* Residual monitor (compute_residual, src/Residual.cu:8-121), on when setup.txt gives output_screen > 0: the
+ * steps taken by corefl_rk3_step / corefl_advance are counted as the reference's RK3 loop counts them; before
+ * every step that is a multiple of output_screen rho, |V|, p and T are saved (store_last_step,
+ * src/TimeAdvanceFunc.cu:6-19), and after that step and after step 1 the RMS change of each over all interior
+ * nodes is reduced on the device (src/TimeAdvanceFunc.cu:84-100, src/Residual.cu:123-170), divided by the first
+ * such value (res_scale, floored at 1e-20), and a NaN flag is set when any of them is NaN.  Those steps end with
+ * a stream synchronisation, as the reference's residual copy does, and a NaN residual makes corefl_rk3_step /
+ * corefl_advance return an error ("NaN occurred in step N", src/Residual.cu:100-118).  This call gives the last
+ * monitored step's res[0..3] (rho, |V|, p, T), its NaN flag and the number of monitored steps so far. */
+int corefl_residual(corefl_ctx *ctx, double *res, int *nan_flag, int *n_monitored);
+
+/* limit_flow (src/TimeAdvanceFunc.cuh:117-220), which the stages also run after every update when setup.txt sets
+ * limit_flow = 1: every interior node whose rho, u, v, w or p is NaN or outside the bounds of
+ * src/DParameter.cu:332-349 (rho and p within 1e-6..1e3 of the reference state's, velocities within 1000 v_ref)
+ * is set to the mean of the in-bounds nodes of its 3x3x3 neighbourhood inside the block (its clipped values when
+ * there are none), T from the equation of state and cv from bv; then boundary conditions and block exchanges.
+ * Which neighbours count as in bounds is decided before any node is changed. */
+int corefl_limit_flow(corefl_ctx *ctx, void *stream);
+
 /* Simulated physical time (device accumulator) and a stream synchronisation. */
 int corefl_get_time(corefl_ctx *ctx, double *t);
 int corefl_synchronize(corefl_ctx *ctx);

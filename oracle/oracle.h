@@ -78,6 +78,11 @@ int orc_interface_flux(const orc_params *p, const double (*q)[5], int n, const d
 void orc_fsweno_flux(const orc_params *p, const double (*q)[5], const double (*met)[3], const double *jac, int n,
                      double *fc);
 
+/* Collocated first derivative of f along a direction of stride st at index i of extent n: order 8 the central
+ * stencil of d_dXi<8> (ViscousScheme.cu:129-307) with its one-sided rows where left / right mark a physical
+ * face; order 4 its fourth-order rows.  Pinned by polynomial exactness (tests/test_oracle_cpu.py). */
+double orc_deriv(const double *f, long id, long st, int i, int n, int left, int right, int order);
+
 /* Sutherland's law, Transport.cu:6-9. */
 double orc_sutherland(double T);
 
@@ -204,6 +209,22 @@ void orc_mix_point_flux(const orc_params *p, const orc_mech *m, const double *pl
  * m = 0..n-1 with the interface between n/2-1 and n/2; fc (5 + ns) is limited in place with step dt. */
 void orc_mix_positivity_point(const orc_params *p, int ns, int dim, double dt, const double *q, const double *k,
                               const double *jac, const double *snd, int n, double *fc);
+
+/* ---- step monitor and limit_flow (oracle_monitor.c) ---- */
+
+/* store_last_step (TimeAdvanceFunc.cu:6-19): rho, |V|, p, T of the interior nodes into last (4 * mx*my*mz). */
+void orc_store_last(int mx, int my, int mz, int ngg, const double *bv, double *last);
+/* Sums over the interior of the squared changes of rho, |V|, p, T against last (compute_square_of_dbv +
+ * reduction_of_dv_squared, TimeAdvanceFunc.cu:84-100, Residual.cu:8-60); the residual is sqrt(sum / n) over
+ * all blocks' nodes, divided by the first monitored step's value.  Pinned by closed forms
+ * (tests/test_monitor_oracle_cpu.py). */
+void orc_residual_sums(int mx, int my, int mz, int ngg, const double *bv, const double *last, double *sums);
+/* limit_flow (TimeAdvanceFunc.cuh:117-220) on bv (6*sz, in place): returns the number of nodes changed.  Rgas:
+ * per-node mixture gas constant (sz values) or NULL for air.  In-bounds status is decided before any change (the
+ * reference kernel reads neighbours that other threads may be repairing).  Pinned by closed forms
+ * (tests/test_monitor_oracle_cpu.py). */
+int orc_limit_flow(int mx, int my, int mz, int ngg, double *bv, const double *ll, const double *ul,
+                   const double *Rgas);
 
 /* ---- exact solution ---- */
 

@@ -398,7 +398,7 @@ static void exchange_viscous_flux(orc_block *b) {
 
 /* d/dxi of f along a direction with stride st at index i of extent n (ViscousScheme.cu:129-177 for
  * ORDER 8; ORDER 4 uses the reference's 4th-order closures as its interior and near-boundary rows). */
-static double deriv(const double *f, long id, long st, int i, int n, int left, int right, int order) {
+double orc_deriv(const double *f, long id, long st, int i, int n, int left, int right, int order) {
 #define F(o) f[id + (long)(o) * st]
   if (order == 8) {
     if (left) {
@@ -442,9 +442,9 @@ static void viscous(orc_block *b) {
         const int comp[4] = {1, 2, 3, 5};
         for (int q = 0; q < 4; ++q) {
           const double *f = b->bv + comp[q] * s;
-          dv[q][0] = deriv(f, id, 1, i, b->mx, b->bt[0], b->bt[1], order);
-          dv[q][1] = deriv(f, id, b->d2, j, b->my, b->bt[2], b->bt[3], order);
-          dv[q][2] = b->dim == 3 ? deriv(f, id, b->d1, k, b->mz, b->bt[4], b->bt[5], order) : 0.0;
+          dv[q][0] = orc_deriv(f, id, 1, i, b->mx, b->bt[0], b->bt[1], order);
+          dv[q][1] = orc_deriv(f, id, b->d2, j, b->my, b->bt[2], b->bt[3], order);
+          dv[q][2] = b->dim == 3 ? orc_deriv(f, id, b->d1, k, b->mz, b->bt[4], b->bt[5], order) : 0.0;
         }
         const double *M = b->met;
         const double xx = M[id], xy = M[id + s], xz = M[id + 2 * s];
@@ -493,9 +493,9 @@ static void viscous(orc_block *b) {
         const long id = ID(b, i, j, k), iq = IDQ(b, i, j, k);
         const double J = b->jac[id];
         for (int l = 1; l < 5; ++l) {
-          double d = deriv(b->fv + l * s, id, 1, i, b->mx, b->bt[0], b->bt[1], order) +
-                     deriv(b->gv + l * s, id, b->d2, j, b->my, b->bt[2], b->bt[3], order);
-          d += b->dim == 3 ? deriv(b->hv + l * s, id, b->d1, k, b->mz, b->bt[4], b->bt[5], order) : 0.0;
+          double d = orc_deriv(b->fv + l * s, id, 1, i, b->mx, b->bt[0], b->bt[1], order) +
+                     orc_deriv(b->gv + l * s, id, b->d2, j, b->my, b->bt[2], b->bt[3], order);
+          d += b->dim == 3 ? orc_deriv(b->hv + l * s, id, b->d1, k, b->mz, b->bt[4], b->bt[5], order) : 0.0;
           b->dq[iq + l * b->n] += d * J;
         }
       }

@@ -36,7 +36,8 @@ struct orc_mix {
   double *cv, *qn, *bv, *sv, *c, *efd, *dq, *jac, *met, *hstep, *tscale;
   double *cn, *wu_ts; /* Wu's splitting: frozen transport residual, reaction time scale (starts at zero) */
   double pp_dt;       /* step size seen by the positivity limiter (the global dt of the current step) */
-  int viscous;        /* viscous_order 2 with transport data set */
+  int viscous;        /* viscous_order 2 or 8 with transport data set */
+  int bt[6];          /* physical-boundary flags per side for the one-sided collocated differences */
   orc_tran tran;
   double *mul, *cond, *rhod, *fvis[3];
 };
@@ -78,6 +79,7 @@ orc_mix *orc_mix_create(const orc_params *p, const orc_mech *m, int mx, int my, 
       if (F->e[d] < F->s[d]) { const int t = F->e[d]; F->e[d] = F->s[d]; F->s[d] = t; }
     }
     if (b->dim == 2) { F->s[2] = 0; F->e[2] = 0; }
+    if (F->type != 8) b->bt[2 * F->face + (F->dir > 0 ? 1 : 0)] = F->type;
   }
   b->cv = calloc((size_t)b->nv * b->sz, sizeof(double));
   b->qn = calloc((size_t)b->nv * b->sz, sizeof(double));
@@ -956,7 +958,7 @@ void orc_transport(const orc_mech *m, const orc_tran *tr, double T, const double
 
 void orc_mix_set_transport(orc_mix *b, const orc_tran *t) {
   b->tran = *t;
-  b->viscous = b->p.viscous_order == 2;
+  b->viscous = b->p.viscous_order == 2 || b->p.viscous_order == 8;
   if (!b->viscous || b->mul) return;
   b->mul = calloc((size_t)b->sz, sizeof(double));
   b->cond = calloc((size_t)b->sz, sizeof(double));
@@ -1047,9 +1049,145 @@ static void mix_transport(orc_mix *b) {
   }
 }
 
+/* compute_viscous_flux_collocated<Mixture, 8> + compute_viscous_flux_collocated_scalar<8> (ViscousScheme.cu:309-566,
+ * gradPInDiffusionFlux off): node fluxes of every interior node into fvis[0..2] (components 1..nv-1). */
+static void mix_viscous_nodes(orc_mix *b) {
+  const long s = b->sz;
+  const int ns = b->ns, order = b->p.viscous_order;
+#pragma omp parallel for collapse(2)
+  for (int k = 0; k < b->mz; ++k)
+    for (int j = 0; j < b->my; ++j)
+      for (int i = 0; i < b->mx; ++i) {
+        const long id = ID(b, i, j, k);
+        const double *M = b->met;
+        const double xx = M[id], xy = M[id + s], xz = M[id + 2 * s];
+        const double ex = M[id + 3 * s], ey = M[id + 4 * s], ez = M[id + 5 * s];
+        const double zx = M[id + 6 * s], zy = M[id + 7 * s], zz = M[id + 8 * s];
+        double gp[4 + MS][3]; /* physical gradients of u, v, w, T, Y_l */
+        for (int q = 0; q < 4 + ns; ++q) {
+          const double *f = q < 3 ? b->bv + (1 + q) * s : (q == 3 ? b->bv + 5 * s : b->sv + (q - 4) * s);
+          const double a = orc_deriv(f, id, 1, i, b->mx, b->bt[0], b->bt[1], order);
+          const double e = orc_deriv(f, id, b->d2, j, b->my, b->bt[2], b->bt[3], order);
+          const double c = b->dim == 3 ? orc_deriv(f, id, b->d1, k, b->mz, b->bt[4], b->bt[5], order) : 0.0;
+          gp[q][0] = a * xx + e * ex + c * zx;
+          gp[q][1] = a * xy + e * ey + c * zy;
+          gp[q][2] = a * xz + e * ez + c * zz;
+        }
+        const double mu = b->mul[id];
+        const double txx = mu * (4 * gp[0][0] - 2 * gp[1][1] - 2 * gp[2][2]) / 3.0;
+        const double tyy = mu * (4 * gp[1][1] - 2 * gp[0][0] - 2 * gp[2][2]) / 3.0;
+        const double tzz = mu * (4 * gp[2][2] - 2 * gp[0][0] - 2 * gp[1][1]) / 3.0;
+        const double txy = mu * (gp[0][1] + gp[1][0]), txz = mu * (gp[0][2] + gp[2][0]);
+        const double tyz = mu * (gp[1][2] + gp[2][1]);
+        double *fv = b->fvis[0], *gv = b->fvis[1], *hv = b->fvis[2];
+        fv[id + s] = xx * txx + xy * txy + xz * txz;
+        fv[id + 2 * s] = xx * txy + xy * tyy + xz * tyz;
+        fv[id + 3 * s] = xx * txz + xy * tyz + xz * tzz;
+        gv[id + s] = ex * txx + ey * txy + ez * txz;
+        gv[id + 2 * s] = ex * txy + ey * tyy + ez * tyz;
+        gv[id + 3 * s] = ex * txz + ey * tyz + ez * tzz;
+        hv[id + s] = zx * txx + zy * txy + zz * txz;
+        hv[id + 2 * s] = zx * txy + zy * tyy + zz * tyz;
+        hv[id + 3 * s] = zx * txz + zy * tyz + zz * tzz;
+        const double lam = b->cond[id];
+        const double u = b->bv[id + s], v = b->bv[id + 2 * s], w = b->bv[id + 3 * s];
+        const double Ex = u * txx + v * txy + w * txz + lam * gp[3][0];
+        const double Ey = u * txy + v * tyy + w * tyz + lam * gp[3][1];
+        const double Ez = u * txz + v * tyz + w * tzz + lam * gp[3][2];
+        double e4[3] = {xx * Ex + xy * Ey + xz * Ez, ex * Ex + ey * Ey + ez * Ez, zx * Ex + zy * Ey + zz * Ez};
+        /* species diffusion: Term 1 (mass-fraction gradients), Term 2 (molar-mass gradient), correction velocity */
+        double yk[MS], dk[MS], drv[MS][3], cvt[3] = {0, 0, 0}, sgw[3] = {0, 0, 0}, imw = 0, srdy = 0;
+        for (int l = 0; l < ns; ++l) {
+          yk[l] = b->sv[id + l * s];
+          dk[l] = b->rhod[id + l * s];
+          const double *g = gp[4 + l];
+          drv[l][0] = xx * g[0] + xy * g[1] + xz * g[2];
+          drv[l][1] = ex * g[0] + ey * g[1] + ez * g[2];
+          drv[l][2] = zx * g[0] + zy * g[1] + zz * g[2];
+          for (int a = 0; a < 3; ++a) {
+            cvt[a] += dk[l] * drv[l][a];
+            sgw[a] += drv[l][a] / b->m.mw[l];
+          }
+          imw += yk[l] / b->m.mw[l];
+          srdy += dk[l] * yk[l];
+        }
+        const double mwt = 1.0 / imw;
+        for (int a = 0; a < 3; ++a) cvt[a] -= mwt * srdy * sgw[a];
+        double hh[MS], cp[MS];
+        orc_chem_h_cp(&b->m, b->bv[id + 5 * s], hh, cp);
+        double *F[3] = {fv, gv, hv};
+        for (int l = 0; l < ns; ++l)
+          for (int a = 0; a < 3; ++a) {
+            const double jd = dk[l] * (drv[l][a] - mwt * yk[l] * sgw[a]) - yk[l] * cvt[a];
+            F[a][id + (5 + l) * s] = jd;
+            e4[a] += hh[l] * jd;
+          }
+        fv[id + 4 * s] = e4[0];
+        gv[id + 4 * s] = e4[1];
+        hv[id + 4 * s] = e4[2];
+      }
+}
+
+/* periodic_exchange task 1 for the node viscous fluxes (DataCommunication.cu:628-683), face after face: average
+ * the coincident face nodes, copy the ghost layers from inside the opposite face. */
+static void mix_exchange_viscous(orc_mix *b) {
+  const int ext[3] = {b->mx, b->my, b->mz};
+  const long s = b->sz;
+  for (int f = 0; f < b->n_face; ++f) {
+    const mface_t *F = &b->faces[f];
+    if (F->type != 8) continue;
+    int d[3] = {0, 0, 0};
+    d[F->face] = F->dir;
+    for (int k = F->s[2]; k <= F->e[2]; ++k)
+      for (int j = F->s[1]; j <= F->e[1]; ++j)
+        for (int i = F->s[0]; i <= F->e[0]; ++i) {
+          int o[3] = {i, j, k};
+          o[F->face] = F->dir < 0 ? ext[F->face] - 1 : 0;
+          const long id = ID(b, i, j, k), io = ID(b, o[0], o[1], o[2]);
+          for (int l = 1; l < b->nv; ++l)
+            for (int a = 0; a < 3; ++a) {
+              const double av = 0.5 * (b->fvis[a][id + l * s] + b->fvis[a][io + l * s]);
+              b->fvis[a][id + l * s] = av;
+              b->fvis[a][io + l * s] = av;
+            }
+          for (int g = 1; g <= b->g; ++g) {
+            const long ig = ID(b, i + g * d[0], j + g * d[1], k + g * d[2]);
+            const long is = ID(b, o[0] + g * d[0], o[1] + g * d[1], o[2] + g * d[2]);
+            for (int l = 1; l < b->nv; ++l)
+              for (int a = 0; a < 3; ++a) b->fvis[a][ig + l * s] = b->fvis[a][is + l * s];
+          }
+        }
+  }
+}
+
+/* compute_viscous_flux_derivative<8> (ViscousScheme.cu:568-612): dq += J (dFv/dxi + dGv/deta + dHv/dzeta). */
+static void mix_viscous_collocated(orc_mix *b) {
+  const long s = b->sz;
+  const int order = b->p.viscous_order;
+  mix_viscous_nodes(b);
+  mix_exchange_viscous(b);
+#pragma omp parallel for collapse(2)
+  for (int k = 0; k < b->mz; ++k)
+    for (int j = 0; j < b->my; ++j)
+      for (int i = 0; i < b->mx; ++i) {
+        const long id = ID(b, i, j, k), iq = IDQ(b, i, j, k);
+        const double J = b->jac[id];
+        for (int l = 1; l < b->nv; ++l) {
+          double d = orc_deriv(b->fvis[0] + l * s, id, 1, i, b->mx, b->bt[0], b->bt[1], order) +
+                     orc_deriv(b->fvis[1] + l * s, id, b->d2, j, b->my, b->bt[2], b->bt[3], order);
+          if (b->dim == 3) d += orc_deriv(b->fvis[2] + l * s, id, b->d1, k, b->mz, b->bt[4], b->bt[5], order);
+          b->dq[iq + l * b->n] += d * J;
+        }
+      }
+}
+
 static void mix_viscous(orc_mix *b) {
   const long s = b->sz;
   mix_transport(b);
+  if (b->p.viscous_order == 8) {
+    mix_viscous_collocated(b);
+    return;
+  }
   const long st[3] = {1, b->d2, b->d1};
   for (int dir = 0; dir < b->dim; ++dir) {
     mix_viscous_faces(b, dir);

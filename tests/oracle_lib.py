@@ -453,3 +453,48 @@ class MixBlock:
 
     def wu_step(self, dt, method=0):
         load().orc_mix_wu_step(self._h, dt, method)
+
+
+def _monitor_protos(lib):
+    P, I = ctypes.c_void_p, ctypes.c_int
+    lib.orc_store_last.argtypes = [I, I, I, I, P, P]
+    lib.orc_residual_sums.argtypes = [I, I, I, I, P, P, P]
+    lib.orc_limit_flow.restype = I
+    lib.orc_limit_flow.argtypes = [I, I, I, I, P, P, P, P]
+
+
+def store_last(shape, ngg, bv):
+    """rho, |V|, p, T of the interior nodes (4, n) from bv (6*sz, ghosts included)."""
+    lib = load()
+    _monitor_protos(lib)
+    mx, my, mz = shape
+    bv = np.ascontiguousarray(bv, dtype=np.float64).ravel()
+    last = np.zeros(4 * mx * my * mz)
+    lib.orc_store_last(mx, my, mz, ngg, _p(bv), _p(last))
+    return last
+
+
+def residual_sums(shape, ngg, bv, last):
+    lib = load()
+    _monitor_protos(lib)
+    mx, my, mz = shape
+    bv = np.ascontiguousarray(bv, dtype=np.float64).ravel()
+    last = np.ascontiguousarray(last, dtype=np.float64).ravel()
+    sums = np.zeros(4)
+    lib.orc_residual_sums(mx, my, mz, ngg, _p(bv), _p(last), _p(sums))
+    return sums
+
+
+def limit_flow(shape, ngg, bv, ll, ul, rgas=None):
+    """limit_flow on a copy of bv (6*sz); rgas: per-node gas constant (sz) for mixtures.  Returns (bv, count)."""
+    lib = load()
+    _monitor_protos(lib)
+    mx, my, mz = shape
+    out = np.array(bv, dtype=np.float64).ravel().copy()
+    ll, ul = np.ascontiguousarray(ll, float), np.ascontiguousarray(ul, float)
+    rp = None
+    if rgas is not None:
+        rgas = np.ascontiguousarray(rgas, dtype=np.float64).ravel()
+        rp = _p(rgas)
+    n = lib.orc_limit_flow(mx, my, mz, ngg, _p(out), _p(ll), _p(ul), rp)
+    return out.reshape(6, -1), n
