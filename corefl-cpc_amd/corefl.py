@@ -25,7 +25,7 @@ SYMBOLS = (
     "corefl_chem_create", "corefl_chem_create_mem", "corefl_chem_destroy", "corefl_chem_info", "corefl_chem_name",
     "corefl_chem_table", "corefl_chem_set_cells", "corefl_chem_set_cell_dt", "corefl_chem_get_cells", "corefl_chem_view_get",
     "corefl_chem_source", "corefl_chem_integrate", "corefl_chem_set_launch", "corefl_chem_get_launch",
-    "corefl_chem_autotune", "corefl_chem_synchronize",
+    "corefl_chem_autotune", "corefl_chem_synchronize", "corefl_chem_integrate_noalloc",
     "corefl_surrogate_n_params", "corefl_surrogate_create", "corefl_surrogate_destroy", "corefl_surrogate_set_params",
     "corefl_surrogate_get_params", "corefl_surrogate_set_scaling", "corefl_surrogate_set_transform",
     "corefl_surrogate_set_gate",
@@ -102,6 +102,7 @@ def load_library(path=LIB_PATH):
     lib.corefl_chem_get_launch.argtypes = [P, PI, PI, PI, PI]
     lib.corefl_chem_autotune.argtypes = [P, D, D, D, I, I, ctypes.POINTER(D), I]
     lib.corefl_chem_synchronize.argtypes = [P]
+    lib.corefl_chem_integrate_noalloc.argtypes = [P, LL, P, P, P, P, D, D, D, I, P]
     F = ctypes.c_float
     lib.corefl_surrogate_n_params.restype = LL
     lib.corefl_surrogate_n_params.argtypes = []
@@ -391,6 +392,12 @@ class Chemistry:
                                                None))
         if sync:
             self.sync()
+
+    def integrate_noalloc(self, n, rho, e, T, rhoY, dt, rtol=1e-6, atol=1e-8, max_attempts=100000, stream=None):
+        """Exact sub-step of n cells in caller-owned device arrays (device addresses as ints), with no device
+        allocation; stream-ordered on `stream` (a HIP stream handle, None: the context's stream)."""
+        _check(self._lib.corefl_chem_integrate_noalloc(self._ctx, int(n), rho, e, T, rhoY, float(dt), float(rtol),
+                                                       float(atol), int(max_attempts), stream))
 
     def set_launch(self, mode, threads=64, blocks=1024, budget=0):
         """mode 0: one thread per cell; 1: work queue; 2: cooperative (16 lanes per cell); 3: work queue for

@@ -243,6 +243,18 @@ int corefl_chem_autotune(corefl_chem *chem, double dt, double rtol, double atol,
                          double *ms_out, int n_out);
 int corefl_chem_synchronize(corefl_chem *chem);
 
+/* Allocation-free variant of corefl_chem_integrate (no counterpart in the reference; for callers that cannot
+ * allocate device memory: code under HIP graph capture and tight serving loops).  One exact ROK4E sub-step of
+ * length dt > 0 for n cells held in the CALLER's device arrays: rho (n), e (n, specific internal energy, kept
+ * constant), T (n, in: initial guess, out: final) and rhoY (n_spec * n, [k * n + cell], in/out); each cell starts
+ * from the full sub-step, as after corefl_chem_set_cells, with the context's launch shape.  The integrator's
+ * working arrays live in one fixed device array of the library (36 MiB per GPU, held until the process ends), so
+ * n is at most 2^20 (larger n is an error) and the call performs no allocation.  The context must already hold
+ * its device tables (created by its first corefl_chem_set_cells or corefl_chem_integrate).  Every call on a GPU
+ * shares that array: calls that overlap on one GPU race.  Stream-ordered on `stream` (NULL: the context's). */
+int corefl_chem_integrate_noalloc(corefl_chem *chem, long long n, const double *rho, const double *e, double *T,
+                                  double *rhoY, double dt, double rtol, double atol, int max_attempts, void *stream);
+
 /* ---------------------------------------------------------------------------------------------------
  * Chemistry surrogate (no reference counterpart: a method change).  An MLP 12 -> 4 x 128 (hard swish,
  * z clamp(z/6 + 1/2, 0, 1)) -> 10 with BF16 weights and FP32 accumulation on the matrix cores maps the features
