@@ -506,8 +506,10 @@ void build(corefl_ctx *ctx, std::vector<corefl::HostBlock> blocks) {
     corefl::chem_ensure_device(ctx->chem);
   } else {
     const bool fs = inv == 51 || inv == 52 || inv == 71 || inv == 72;
-    if (!fs && rec != 4 && rec != 5)
-      throw std::runtime_error("reconstruction must be 4 (WENO5) or 5 (WENO7) on this path");
+    if (!fs && (rec < 1 || rec > 5))
+      throw std::runtime_error("reconstruction must be 1 (first order), 2 (MUSCL), 3 (NND2), 4 (WENO5) or 5 (WENO7)");
+    if (!fs && rec <= 3 && P.get_int("limiter") != 0 && P.get_int("limiter") != 1)
+      throw std::runtime_error("limiter must be 0 or 1 (minmod)");
     if (!fs && (inv < 1 || inv > 4))
       throw std::runtime_error("inviscid_scheme must be 1 (LF), 2 (Roe), 3 (AUSM+), 4 (HLLC), or 51/52/71/72 (WENO5/WENO7 "
                                "flux splitting, component-wise/characteristic)");

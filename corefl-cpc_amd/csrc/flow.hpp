@@ -33,7 +33,8 @@ struct DevInner {
 
 struct Scheme {
   int inviscid_scheme; // 1 LF, 2 Roe, 3 AUSM+, 4 HLLC; 51/52/71/72 LLF-split WENO5/7, component-wise/characteristic
-  int reconstruction;  // 4 WENO5, 5 WENO7 (characteristic) for schemes 1-4
+  int reconstruction;  // schemes 1-4: 1 first order, 2 MUSCL, 3 NND2 (primitive, minmod); 4 WENO5, 5 WENO7
+                       // (characteristic)
   int viscous_order;   // 0, 2 (face-centred), 4, 8 (collocated)
   double weno_eps, entropy_fix_factor, prandtl, cfl;
   double v_ref;        // v_inf: the component-wise WENO epsilon is weno_eps v_ref^0, ^2, ^4 (WENO.cu:946-949)

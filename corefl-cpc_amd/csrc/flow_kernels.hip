@@ -378,6 +378,67 @@ __global__ __launch_bounds__(256) void k_inviscid(DevBlock b, Scheme sc, double 
   for (int l = 0; l < 5; ++l) F[id + l * sz] = fc[l];
 }
 
+// ---------------------------------------------------------------- primitive reconstruction (air)
+// minmod limiter (Limiter.cuh:8-41; limiter 0 and 1 are both minmod).
+__device__ __forceinline__ double minmod(double a, double c) {
+  const double sa = (a > 0) - (a < 0), sc = (c > 0) - (c < 0);
+  return 0.5 * (sa + sc) * fmin(fabs(a), fabs(c));
+}
+
+// compute_convective_term_pv_1D<Air> and Roe_compute_inviscid_flux_1D<Air> (InviscidScheme.cu:61-169, 306-395):
+// one thread per interface t+1/2 along DIR.  rho, u, v, w, p of nodes t-1 .. t+2 reconstructed to the interface
+// (Reconstruction.cu:5-44: REC 1 first order, 2 MUSCL with kappa = 1/3, 3 NND2, both with minmod), the total
+// energy of each side from its reconstructed state (InviscidScheme.cu:212-217), then the Riemann solver RS.
+template<int DIR, int REC, int RS>
+__global__ __launch_bounds__(256) void k_inviscid_pv(DevBlock b, Scheme sc, double *F, TileGrid tg) {
+  Tile t;
+  if (!xcd_tile(tg, t)) return;
+  int i = t.x * 64 + threadIdx.x, j = t.y * 4 + threadIdx.y, k = t.z;
+  if constexpr (DIR == 0) i -= 1;
+  else if constexpr (DIR == 1) j -= 1;
+  else k -= 1;
+  if (i >= b.mx || j >= b.my) return;
+  const long st = DIR == 0 ? 1 : (DIR == 1 ? b.d2 : b.d1);
+  const long id = nid(b, i, j, k), sz = b.sz;
+  double PL[6], PR[6];
+#pragma unroll
+  for (int l = 0; l < 5; ++l) {
+    const double *f = b.bv + l * sz + id;
+    const double v1 = f[0], v2 = f[st];
+    if constexpr (REC == 1) {
+      PL[l] = v1;
+      PR[l] = v2;
+    } else {
+      const double v0 = f[-st], v3 = f[2 * st];
+      const double d0 = v1 - v0, d1 = v2 - v1, d2 = v3 - v2;
+      if constexpr (REC == 2) {
+        constexpr double kappa = 1.0 / 3.0;
+        PL[l] = v1 + 0.25 * ((1 - kappa) * minmod(d0, d1) + (1 + kappa) * minmod(d1, d0));
+        PR[l] = v2 - 0.25 * ((1 - kappa) * minmod(d2, d1) + (1 + kappa) * minmod(d1, d2));
+      } else {
+        PL[l] = v1 + 0.5 * minmod(d0, d1);
+        PR[l] = v2 - 0.5 * minmod(d1, d2);
+      }
+    }
+  }
+  PL[5] = 0.5 * (PL[1] * PL[1] + PL[2] * PL[2] + PL[3] * PL[3]) * PL[0] + PL[4] / (kGamma - 1);
+  PR[5] = 0.5 * (PR[1] * PR[1] + PR[2] * PR[2] + PR[3] * PR[3]) * PR[0] + PR[4] / (kGamma - 1);
+  double ml[3], mr[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    ml[c] = b.met[id + (3 * DIR + c) * sz];
+    mr[c] = b.met[id + st + (3 * DIR + c) * sz];
+  }
+  const double jl = b.jac[id], jr = b.jac[id + st];
+  double fc[5];
+  if constexpr (RS == 1) flux_lf(PL, PR, ml, mr, jl, jr, fc);
+  else if constexpr (RS == 2) flux_roe(PL, PR, ml, mr, jl, jr, b.efd[id], b.efd[id + st], fc);
+  else if constexpr (RS == 4) flux_hllc(PL, PR, ml, mr, jl, jr, fc);
+  else flux_ausm(PL, PR, ml, mr, jl, jr, fc);
+#pragma unroll
+  for (int l = 0; l < 5; ++l) F[id + l * sz] = fc[l];
+}
+
 // ---------------------------------------------------------------- LLF flux-splitting WENO (air)
 // inviscid_scheme 51/71: component-wise WENO5/WENO7 of the locally split fluxes (WENO.cu:890-990);
 // 52/72: characteristic WENO of the fluxes split with the largest spectral radius on the stencil
@@ -1164,6 +1225,26 @@ static void dispatch_rs(const DevBlock &b, const Scheme &sc, dim3 grid, double *
   }
 }
 
+template<int DIR, int REC>
+static void dispatch_pv(const DevBlock &b, const Scheme &sc, dim3 grid, double *F, hipStream_t s) {
+  const dim3 tpb(64, 4, 1);
+  const TileGrid tg{(int)grid.x, (int)grid.y, (int)grid.z, DIR};
+  const unsigned nb = tile_blocks(tg);
+  switch (sc.inviscid_scheme) {
+    case 1: k_inviscid_pv<DIR, REC, 1><<<nb, tpb, 0, s>>>(b, sc, F, tg); break;
+    case 2: k_inviscid_pv<DIR, REC, 2><<<nb, tpb, 0, s>>>(b, sc, F, tg); break;
+    case 4: k_inviscid_pv<DIR, REC, 4><<<nb, tpb, 0, s>>>(b, sc, F, tg); break;
+    default: k_inviscid_pv<DIR, REC, 3><<<nb, tpb, 0, s>>>(b, sc, F, tg); break;
+  }
+}
+
+template<int DIR>
+static void dispatch_pv_rec(const DevBlock &b, const Scheme &sc, dim3 grid, double *F, hipStream_t s) {
+  if (sc.reconstruction == 1) dispatch_pv<DIR, 1>(b, sc, grid, F, s);
+  else if (sc.reconstruction == 2) dispatch_pv<DIR, 2>(b, sc, grid, F, s);
+  else dispatch_pv<DIR, 3>(b, sc, grid, F, s);
+}
+
 template<int DIR>
 static void dispatch_fs(const DevBlock &b, const Scheme &sc, dim3 grid, double *F, hipStream_t s) {
   const dim3 tpb(64, 4, 1);
@@ -1194,6 +1275,12 @@ hipError_t launch_inviscid_flux(const DevBlock &b, const Scheme &sc, int dir, hi
     if (dir == 0) dispatch_fs<0>(b, sc, grid, F, s);
     else if (dir == 1) dispatch_fs<1>(b, sc, grid, F, s);
     else dispatch_fs<2>(b, sc, grid, F, s);
+    return hipGetLastError();
+  }
+  if (sc.reconstruction >= 1 && sc.reconstruction <= 3) {
+    if (dir == 0) dispatch_pv_rec<0>(b, sc, grid, F, s);
+    else if (dir == 1) dispatch_pv_rec<1>(b, sc, grid, F, s);
+    else dispatch_pv_rec<2>(b, sc, grid, F, s);
     return hipGetLastError();
   }
   const bool w7 = sc.reconstruction == 5;

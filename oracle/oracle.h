@@ -70,6 +70,13 @@ void orc_riemann(int scheme, const double *pl, const double *pr, const double *m
 int orc_interface_flux(const orc_params *p, const double (*q)[5], int n, const double *ml,
                        const double *mr, double jl, double jr, double efl, double efr, double *fc);
 
+/* Primitive-variable interface flux for air: params.reconstruction 1 first order, 2 MUSCL (kappa 1/3), 3 NND2 (minmod,
+ * Reconstruction.cu:5-44) of rho, u, v, w, p at the nodes pv[0..3] (interface between 1 and 2), total energy per side,
+ * then params.inviscid_scheme 1-4 (InviscidScheme.cu:61-219, 306-395).  Pinned by closed forms
+ * (tests/test_pv_oracle_cpu.py). */
+void orc_interface_flux_pv(const orc_params *p, const double (*pv)[5], const double *ml, const double *mr, double jl,
+                           double jr, double efl, double efr, double *fc);
+
 /* Interface flux of the reference's LLF flux-splitting WENO for air: params.inviscid_scheme 51/71
  * component-wise WENO5/WENO7 (WENO.cu:890-990), 52/72 characteristic (WENO.cu:10-244, 1090-1145).
  * q[m][5], met[m][3] (metric row of the sweep direction) and jac[m] for the stencil nodes

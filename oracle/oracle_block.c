@@ -603,7 +603,8 @@ static void entropy_fix(orc_block *b) {
  * compute_derivative_x/y/z, WENO.cu:1544-1571). */
 static void inviscid_dir(orc_block *b, int dir) {
   const int fs = b->p.inviscid_scheme >= 51; /* LLF flux-splitting WENO (WENO.cu:830-990) */
-  const int r = fs ? (b->p.inviscid_scheme >= 71 ? 4 : 3) : (b->p.reconstruction == 5 ? 4 : 3); /* n = 2r points */
+  const int pvr = !fs && b->p.reconstruction <= 3; /* primitive reconstruction (InviscidScheme.cu:61-219) */
+  const int r = fs ? (b->p.inviscid_scheme >= 71 ? 4 : 3) : (pvr ? 2 : (b->p.reconstruction == 5 ? 4 : 3));
   const int n = 2 * r;
   const long s = b->sz;
   const long st = dir == 0 ? 1 : (dir == 1 ? b->d2 : b->d1);
@@ -638,6 +639,14 @@ static void inviscid_dir(orc_block *b, int dir) {
         for (int c = 0; c < 3; ++c) {
           ml[c] = b->met[id + (3 * dir + c) * s];
           mr[c] = b->met[id + st + (3 * dir + c) * s];
+        }
+        if (pvr) {
+          double pv[4][5];
+          for (int mm = 0; mm < 4; ++mm)
+            for (int l = 0; l < 5; ++l) pv[mm][l] = b->bv[id + (mm - 1) * st + l * s];
+          orc_interface_flux_pv(&b->p, (const double(*)[5])pv, ml, mr, b->jac[id], b->jac[id + st], b->efd[id],
+                                b->efd[id + st], Fl + 5 * (t + 1));
+          continue;
         }
         orc_interface_flux(&b->p, (const double(*)[5])q, n, ml, mr, b->jac[id], b->jac[id + st], b->efd[id],
                            b->efd[id + st], Fl + 5 * (t + 1));

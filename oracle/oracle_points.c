@@ -455,6 +455,43 @@ int orc_interface_flux(const orc_params *p, const double (*q)[5], int n, const d
   return fallback;
 }
 
+/* minmod (Limiter.cuh:8-41; gxl::sgn(0) = 1 there, which changes nothing: min(|a|, |b|) = 0 when a or b is 0). */
+static double minmod(double a, double b) {
+  const double sa = a < 0 ? -1.0 : 1.0, sb = b < 0 ? -1.0 : 1.0;
+  return 0.5 * (sa + sb) * fmin(fabs(a), fabs(b));
+}
+
+/* Primitive-variable interface flux for air (reconstruction 1/2/3 with inviscid_scheme 1-4): rho, u, v, w, p of the
+ * four nodes pv[0..3] (the interface between pv[1] and pv[2]) reconstructed as first_order_reconstruct,
+ * MUSCL_reconstruct (kappa = 1/3) or NND2_reconstruct (Reconstruction.cu:5-44) with minmod, the total energy of each
+ * side from its own state (reconstruction<Air>, InviscidScheme.cu:212-217), then the Riemann solver
+ * (compute_convective_term_pv_1D and Roe_compute_inviscid_flux_1D, InviscidScheme.cu:61-169, 306-395). */
+void orc_interface_flux_pv(const orc_params *p, const double (*pv)[5], const double *ml, const double *mr, double jl,
+                           double jr, double efl, double efr, double *fc) {
+  double pl[6], pr[6];
+  for (int l = 0; l < 5; ++l) {
+    if (p->reconstruction == 1) {
+      pl[l] = pv[1][l];
+      pr[l] = pv[2][l];
+      continue;
+    }
+    const double d0 = pv[1][l] - pv[0][l], d1 = pv[2][l] - pv[1][l], d2 = pv[3][l] - pv[2][l];
+    if (p->reconstruction == 2) {
+      const double kappa = 1.0 / 3.0;
+      pl[l] = pv[1][l] + 0.25 * ((1 - kappa) * minmod(d0, d1) + (1 + kappa) * minmod(d1, d0));
+      pr[l] = pv[2][l] - 0.25 * ((1 - kappa) * minmod(d2, d1) + (1 + kappa) * minmod(d1, d2));
+    } else {
+      pl[l] = pv[1][l] + 0.5 * minmod(d0, d1);
+      pr[l] = pv[2][l] - 0.5 * minmod(d1, d2);
+    }
+  }
+  const double el = 0.5 * (pl[1] * pl[1] + pl[2] * pl[2] + pl[3] * pl[3]);
+  const double er = 0.5 * (pr[1] * pr[1] + pr[2] * pr[2] + pr[3] * pr[3]);
+  pl[5] = el * pl[0] + pl[4] / (kGamma - 1);
+  pr[5] = er * pr[0] + pr[4] / (kGamma - 1);
+  orc_riemann(p->inviscid_scheme, pl, pr, ml, mr, jl, jr, efl, efr, 3, fc);
+}
+
 /* Pressure, temperature and the point quantities both LLF-split WENO schemes build their stencils
  * from (WENO.cu:890-917 component-wise, WENO.cu:1096-1136 characteristic): Uk with the unscaled
  * metric, cGradK = |grad k| sqrt(gamma R T) with T = p / (rho R) as the primitive update stores it. */

@@ -498,3 +498,15 @@ def limit_flow(shape, ngg, bv, ll, ul, rgas=None):
         rp = _p(rgas)
     n = lib.orc_limit_flow(mx, my, mz, ngg, _p(out), _p(ll), _p(ul), rp)
     return out.reshape(6, -1), n
+
+
+def interface_flux_pv(p, pv, ml, mr, jl=1.0, jr=1.0, efl=0.0, efr=0.0):
+    """Primitive-reconstruction interface flux (reconstruction 1/2/3) from rho, u, v, w, p of 4 nodes (4, 5)."""
+    lib = load()
+    P, D = ctypes.c_void_p, ctypes.c_double
+    lib.orc_interface_flux_pv.argtypes = [ctypes.POINTER(OrcParams), P, P, P, D, D, D, D, P]
+    pv = np.ascontiguousarray(pv, float)
+    ml, mr = np.ascontiguousarray(ml, float), np.ascontiguousarray(mr, float)
+    fc = np.zeros(5)
+    lib.orc_interface_flux_pv(ctypes.byref(p), _p(pv), _p(ml), _p(mr), jl, jr, efl, efr, _p(fc))
+    return fc
