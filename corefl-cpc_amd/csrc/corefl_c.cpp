@@ -13,6 +13,7 @@
 #include <string>
 #include <vector>
 
+#include "bc.hpp"
 #include "chem.hpp"
 #include "chem_ctx.hpp"
 #include "flow.hpp"
@@ -99,6 +100,79 @@ FlowState inflow_state(const corefl::Struct &s, const corefl::ChemMech *mech = n
   return {rho, u, v, w, p, T, c, mu, Y};
 }
 
+bool is_mixing_layer(const corefl::Parameter &P) {
+  std::string cp = P.get_string("canonical_problem");
+  for (auto &ch : cp) ch = (char)std::toupper((unsigned char)ch);
+  return cp == "MIXINGLAYER"; // Parameter.cpp:617-625: problem_type 1
+}
+
+// The two streams of a mixing layer (get_mixing_layer_info, MixingLayer.cu:9-130): temperature, pressure and
+// mass fractions of the structs upper_stream / lower_stream; their Mach numbers given, or derived from the
+// convective Mach number ma_c with velocity_ratio or one given Mach number.  Velocity along x only.
+struct MixingLayer {
+  FlowState upper, lower;
+  double delta_omega, delta_u, uc;
+};
+
+MixingLayer mixing_layer(const corefl::Parameter &P, const corefl::ChemMech *mech) {
+  using namespace corefl;
+  const auto &up = P.get_struct("upper_stream"), &lo = P.get_struct("lower_stream");
+  auto stream = [&](const Struct &s, double &c) {
+    FlowState f{};
+    f.T = struct_real(s, "temperature", -1);
+    f.p = struct_real(s, "pressure", -1);
+    if (!(f.T > 0) || !(f.p > 0)) throw std::runtime_error("mixing layer streams need temperature and pressure");
+    double imw = 1.0 / kMwAir, gamma = kGamma;
+    if (mech) {
+      f.Y.assign(mech->ns, 0.0);
+      imw = 0;
+      double h[kMaxSpec], cp[kMaxSpec], cpt = 0;
+      host_h_cp(*mech, f.T, h, cp);
+      for (int k = 0; k < mech->ns; ++k) {
+        f.Y[k] = struct_real(s, mech->species[k], 0.0);
+        imw += f.Y[k] / mech->mw[k];
+        cpt += cp[k] * f.Y[k];
+      }
+      if (!(imw > 0)) throw std::runtime_error("mixing layer streams of a mixture need species mass fractions");
+      gamma = cpt / (cpt - kRu * imw);
+    }
+    c = std::sqrt(gamma * kRu * imw * f.T);
+    f.rho = f.p / (kRu * imw * f.T);
+    f.c = c;
+    return f;
+  };
+  double cu = 0, cl = 0;
+  MixingLayer ml{stream(up, cu), stream(lo, cl), P.get_real("delta_omega"), 0, 0};
+  const double ma_c = P.get_real("ma_c"), vr = P.get_real("velocity_ratio");
+  const bool upper_faster = P.get_bool("upper_faster");
+  double mu = struct_real(up, "mach", -1), ml_ = struct_real(lo, "mach", -1);
+  if (vr > 0) {
+    double slow, fast;
+    if (vr > 1) {
+      slow = ma_c * (cu + cl) / (vr - 1);
+      fast = vr * slow;
+    } else {
+      fast = ma_c * (cu + cl) / (1 - vr);
+      slow = fast * vr;
+    }
+    mu = (upper_faster ? fast : slow) / cu;
+    ml_ = (upper_faster ? slow : fast) / cl;
+  } else if (mu < 0 && ml_ < 0) {
+    throw std::runtime_error("mixing layer: give a Mach number of one stream, or velocity_ratio with ma_c");
+  } else if (mu > 0 && ml_ < 0) {
+    ml_ = upper_faster ? (mu - ma_c) * cu / cl - ma_c : (ma_c + mu) * cu / cl + ma_c;
+  } else if (mu < 0) {
+    mu = upper_faster ? (ma_c + ml_) * cl / cu + ma_c : (ml_ - ma_c) * cl / cu - ma_c;
+  }
+  ml.upper.u = mu * cu;
+  ml.lower.u = ml_ * cl;
+  ml.upper.mu = ml.lower.mu = 0;
+  // Driver.cu:286-293: convective velocity and velocity difference
+  ml.uc = (ml.upper.u * cl + ml.lower.u * cu) / (cu + cl);
+  ml.delta_u = std::abs(ml.upper.u - ml.lower.u);
+  return ml;
+}
+
 struct Block {
   corefl::HostBlock host;
   corefl::DevBlock dev{};
@@ -109,6 +183,8 @@ struct Block {
   std::vector<int> inner_target;       // their partner blocks
   long part_offset = 0;
   int part_count = 0;
+  std::vector<corefl::DevInflow> inflow; // inflow data per face (used for faces of type 5)
+  double *sponge_sigma = nullptr, *sponge_mean = nullptr;
   double *last = nullptr;          // rho, |V|, p, T at the start of a monitored step (4 * n)
   unsigned char *lflag = nullptr;  // limit_flow flags (n)
   long res_offset = 0;             // this block's residual partials
@@ -140,6 +216,8 @@ struct corefl_ctx {
   long n_res_parts = 0;
   double n_nodes_total = 0;
   long step_count = 0;              // steps taken (the reference's step counter)
+  unsigned long long *sponge_iter = nullptr; // steps averaged into the sponge layer's mean (device)
+  int n_inflow = 0;                           // inflow faces so far (each draws its own noise stream)
   hipGraphExec_t graph_exec_mon = nullptr; // the captured monitored step
   hipGraph_t graph_mon = nullptr;
   std::vector<void *> allocations;
@@ -192,13 +270,16 @@ struct corefl_ctx {
   }
 
   void apply_bcs(Block &b, hipStream_t s) {
-    // DBoundCond::apply_boundary_conditions order (BoundCond.cu:3489-3619): outflow, then periodic.
-    for (int type : {6, 8})
-      for (const auto &f : b.faces)
-        if (f.type == type) {
-          if (ns) HIPCHK(corefl::launch_mix_boundary(b.dev, b.mix, f, s));
-          else HIPCHK(corefl::launch_boundary(b.dev, f, s));
-        }
+    // DBoundCond::apply_boundary_conditions order (BoundCond.cu:3489-3619): outflow, inflow, symmetry, periodic.
+    for (int type : {6, 5, 3, 8})
+      for (size_t q = 0; q < b.faces.size(); ++q) {
+        const auto &f = b.faces[q];
+        if (f.type != type) continue;
+        if (type == 5) HIPCHK(corefl::launch_inflow(b.dev, b.mix, b.inflow[q], s));
+        else if (type == 3) HIPCHK(corefl::launch_symmetry(b.dev, b.mix, f, s));
+        else if (ns) HIPCHK(corefl::launch_mix_boundary(b.dev, b.mix, f, s));
+        else HIPCHK(corefl::launch_boundary(b.dev, f, s));
+      }
   }
 
   // data_communication (DataCommunication.cu:6-34): every block-to-block face in block and file order, one
@@ -337,6 +418,10 @@ struct corefl_ctx {
   void enqueue_step(hipStream_t s, bool compute_dt, bool store = false, bool residual = false) {
     if (store)
       for (auto &b : blocks) HIPCHK(corefl::launch_store_last(b.dev, b.last, s));
+    if (sponge_iter) // the sponge layer's running mean at the start of the step (RK.cuh:105-110)
+      for (size_t q = 0; q < blocks.size(); ++q)
+        HIPCHK(corefl::launch_sponge_mean(blocks[q].dev, 5 + ns, blocks[q].sponge_mean, sponge_iter,
+                                          q + 1 == blocks.size(), s));
     if (compute_dt) enqueue_dt(s, 1);
     if (wu_method >= 0) {
       wu_step(s);
@@ -412,6 +497,21 @@ std::vector<double> initial_bv(const corefl::Parameter &P, const corefl::HostBlo
         }
     return bv;
   }
+  if (is_mixing_layer(P)) {
+    // initialize_mixing_layer_with_info (Initialize.cu:970-1035): tanh profile of u across y = 0, every other
+    // variable the upper stream's for y >= 0 and the lower stream's below, on every node including ghosts
+    const MixingLayer ml = mixing_layer(P, mech);
+    for (int k = -g; k < hb.mz + g; ++k)
+      for (int j = -g; j < hb.my + g; ++j)
+        for (int i = -g; i < hb.mx + g; ++i) {
+          const double y = coord(hb.y, i, j, k);
+          FlowState st = y >= 0 ? ml.upper : ml.lower;
+          st.u = 0.5 * (ml.upper.u + ml.lower.u) + 0.5 * (ml.upper.u - ml.lower.u) * std::tanh(2 * y / ml.delta_omega);
+          st.v = st.w = 0;
+          put(hb.idx(i, j, k), st);
+        }
+    return bv;
+  }
   const int groups = P.get_int("groups_init");
   std::vector<FlowState> states{inflow_state(P.get_struct(P.get_string("default_init")), mech)};
   std::vector<double> boxes;
@@ -436,6 +536,142 @@ std::vector<double> initial_bv(const corefl::Parameter &P, const corefl::HostBlo
         put(hb.idx(i, j, k), states[pick]);
       }
   return bv;
+}
+
+// The struct of the boundary condition with this label and type (BoundCond.cpp registers boundary_conditions by
+// label).
+const corefl::Struct &bc_struct(const corefl::Parameter &P, int label, const char *type) {
+  for (const auto &n : P.get_string_array("boundary_conditions")) {
+    if (!P.has_struct(n)) continue;
+    const auto &st = P.get_struct(n);
+    if (corefl::struct_string(st, "type", "") == type && corefl::struct_int(st, "label", -1) == label) return st;
+  }
+  throw std::runtime_error(std::string("no ") + type + " struct with label " + std::to_string(label) +
+                           " in boundary_conditions");
+}
+
+// Inflow face data (Inflow::Inflow, BoundCond.cpp:9-330, and the values apply_inflow assigns): the state of every
+// face node (the mixing layer's tanh profile of the two streams, or the struct's constant state), the noise level
+// per node for fluctuation_type 1, and the subsonic options.
+corefl::DevInflow make_inflow(corefl_ctx *ctx, const corefl::Parameter &P, const corefl::HostBlock &h,
+                              const corefl::Face &f, const corefl::ChemMech *mech) {
+  using namespace corefl;
+  const Struct &st = bc_struct(P, f.type, "inflow");
+  DevInflow d{};
+  for (int q = 0; q < 3; ++q) { d.start[q] = f.start[q]; d.end[q] = f.end[q]; }
+  d.face = f.face;
+  d.direction = f.direction;
+  const bool ml = is_mixing_layer(P); // inflow_type 2 (BoundCond.cpp:13-16)
+  if (struct_int(st, "inflow_type", 0) == 1 && !ml)
+    throw std::runtime_error("profile inflow (inflow_type 1) is not on this path");
+  const int sub = struct_int(st, "inflow_sub", 0), fluct = struct_int(st, "fluctuation_type", 0);
+  if (fluct != 0 && fluct != 1) throw std::runtime_error("inflow fluctuation_type must be 0 or 1 (white noise)");
+  const double intensity = struct_real(st, "fluctuation_intensity", 0.0);
+  MixingLayer mlay{};
+  FlowState cst{};
+  if (ml) mlay = mixing_layer(P, mech);
+  else cst = inflow_state(st, mech);
+  const FlowState &own = ml ? mlay.upper : cst;
+  double imw = 1.0 / kMwAir;
+  if (mech) {
+    imw = 0;
+    for (int k = 0; k < mech->ns; ++k) imw += own.Y[k] / mech->mw[k];
+  }
+  d.mw = 1.0 / imw;
+  d.sub_node = !ml && sub;
+  d.sub_ghost = sub;
+  d.fluct = fluct == 1 ? (ml ? 2 : 1) : 0;
+  const int ns = mech ? mech->ns : 0;
+  const long n0 = f.end[0] - f.start[0] + 1, n1 = f.end[1] - f.start[1] + 1, n2 = f.end[2] - f.start[2] + 1;
+  const long npt = n0 * n1 * n2;
+  std::vector<double> prof((6 + ns) * npt), rms(npt, 0.0);
+  const int G = h.ngg + 1;
+  const long c2 = h.mx + 2L * G, c1 = (h.my + 2L * G) * c2;
+  const double vel = std::sqrt(cst.u * cst.u + cst.v * cst.v + cst.w * cst.w);
+  for (long node = 0; node < npt; ++node) {
+    const int i = f.start[0] + (int)(node % n0), j = f.start[1] + (int)((node / n0) % n1),
+              k = f.start[2] + (int)(node / (n0 * n1));
+    FlowState s0 = cst;
+    if (ml) {
+      const double y = h.y[(k + G) * c1 + (j + G) * c2 + (i + G)];
+      s0 = y >= 0 ? mlay.upper : mlay.lower;
+      s0.u = 0.5 * (mlay.upper.u + mlay.lower.u) + 0.5 * (mlay.upper.u - mlay.lower.u) * std::tanh(2 * y / mlay.delta_omega);
+      s0.v = s0.w = 0;
+      rms[node] = intensity * s0.u * std::exp(-y * y);
+    } else {
+      rms[node] = intensity * vel;
+    }
+    const double q[6] = {s0.rho, s0.u, s0.v, s0.w, s0.p, s0.T};
+    for (int c = 0; c < 6; ++c) prof[c * npt + node] = q[c];
+    for (int l = 0; l < ns; ++l) prof[(6 + l) * npt + node] = s0.Y[l];
+  }
+  double *pd = ctx->alloc(prof.size()), *rd = ctx->alloc(npt);
+  HIPCHK(hipMemcpy(pd, prof.data(), prof.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(rd, rms.data(), rms.size() * sizeof(double), hipMemcpyHostToDevice));
+  d.prof = pd;
+  d.rms = rd;
+  d.calls = reinterpret_cast<unsigned long long *>(ctx->alloc(1));
+  d.seed = (unsigned long long)P.get_int("fluctuation_seed") * 0x100000001b3ULL + (unsigned long long)ctx->n_inflow++;
+  return d;
+}
+
+// Sponge layer (the reference's design, SpongeLayer.cu:48-172 and DParameter.cu:360-395, whose code is commented
+// out): sigma = sum over the active sides of sigma_side S(xi), S(xi) = 6 xi^5 - 15 xi^4 + 10 xi^3, xi the
+// normalised depth into the layer (clipped at 1), sigma_side = spongeCoefficient v_char / layer thickness; the
+// residual gets -J sigma (U - U_mean) with U_mean the running mean of U over the steps.
+void setup_sponge(corefl_ctx *ctx, const corefl::Parameter &P, Block &B, const corefl::ChemMech *mech) {
+  using corefl::HostBlock;
+  const HostBlock &h = B.host;
+  if (P.get_int("sponge_function") != 0) throw std::runtime_error("sponge_function must be 0");
+  double v_char = P.get_real("v_inf");
+  if (is_mixing_layer(P)) {
+    const MixingLayer ml = mixing_layer(P, mech);
+    v_char = P.get_int("characteristic_velocity_ml") == 0 ? ml.uc : ml.delta_u;
+  }
+  const double coef = P.get_real("spongeCoefficient");
+  struct Side {
+    int axis, mode;
+    double a, b;
+  };
+  std::vector<Side> sides;
+  const char *axes[3] = {"X", "Y", "Z"};
+  for (int d = 0; d < 3; ++d) {
+    const int mode = P.get_int(std::string("sponge") + axes[d]);
+    if (mode == 1 || mode == 3)
+      sides.push_back({d, -1, P.get_real(std::string("sponge") + axes[d] + "MinusStart"),
+                       P.get_real(std::string("sponge") + axes[d] + "MinusEnd")});
+    if (mode == 2 || mode == 3)
+      sides.push_back({d, 1, P.get_real(std::string("sponge") + axes[d] + "PlusStart"),
+                       P.get_real(std::string("sponge") + axes[d] + "PlusEnd")});
+  }
+  for (const auto &sd : sides)
+    if (!(std::abs(sd.b - sd.a) > 0)) throw std::runtime_error("sponge layer of zero thickness");
+  const long n = (long)h.mx * h.my * h.mz;
+  std::vector<double> sigma(n, 0.0);
+  const int G = h.ngg + 1;
+  const long c2 = h.mx + 2L * G, c1 = (h.my + 2L * G) * c2;
+  for (int k = 0; k < h.mz; ++k)
+    for (int j = 0; j < h.my; ++j)
+      for (int i = 0; i < h.mx; ++i) {
+        const long gi = (k + G) * c1 + (j + G) * c2 + (i + G);
+        const double xyz[3] = {h.x[gi], h.y[gi], h.z[gi]};
+        double sg = 0;
+        for (const auto &sd : sides) {
+          const double q = xyz[sd.axis];
+          if (sd.mode < 0 ? !(q < sd.a) : !(q > sd.a)) continue;
+          double xi = (q - sd.a) / (sd.b - sd.a);
+          if (xi > 1) xi = 1;
+          const double x3 = xi * xi * xi;
+          sg += coef * v_char / std::abs(sd.b - sd.a) * (6 * x3 * xi * xi - 15 * x3 * xi + 10 * x3);
+        }
+        sigma[((long)k * h.my + j) * h.mx + i] = sg;
+      }
+  B.sponge_sigma = ctx->alloc(n);
+  HIPCHK(hipMemcpy(B.sponge_sigma, sigma.data(), n * sizeof(double), hipMemcpyHostToDevice));
+  B.sponge_mean = ctx->alloc((size_t)(5 + ctx->ns) * n);
+  B.dev.sponge_sigma = B.sponge_sigma;
+  B.dev.sponge_mean = B.sponge_mean;
+  if (!ctx->sponge_iter) ctx->sponge_iter = reinterpret_cast<unsigned long long *>(ctx->alloc(1));
 }
 
 void build(corefl_ctx *ctx, std::vector<corefl::HostBlock> blocks) {
@@ -534,6 +770,14 @@ void build(corefl_ctx *ctx, std::vector<corefl::HostBlock> blocks) {
     c_inf = std::sqrt(kGamma * R * T0);
     v_inf = corefl::struct_real(s, "mach", 0.1) * c_inf;
     p_inf = p0;
+  } else if (is_mixing_layer(P)) {
+    // Inflow::Inflow for a mixing layer (BoundCond.cpp:30-60, 265-275): the upper stream, and v_inf =
+    // |velocity - v_lower| with v_lower the lower stream's (zero) v component, as written
+    const MixingLayer ml = mixing_layer(P, mech);
+    rho_inf = ml.upper.rho;
+    p_inf = ml.upper.p;
+    c_inf = ml.upper.c;
+    v_inf = std::abs(std::abs(ml.upper.u) - 0.0);
   } else {
     const FlowState r = inflow_state(P.get_struct(P.get_string("reference_state")), mech);
     rho_inf = r.rho;
@@ -612,8 +856,9 @@ void build(corefl_ctx *ctx, std::vector<corefl::HostBlock> blocks) {
       for (int d = 0; d < hb.dim; ++d)
         if (ext[d] < 5) throw std::runtime_error("viscous closures need at least 5 nodes per direction");
     for (auto &f : hb.faces) {
-      if (f.type != 6 && f.type != 8)
-        throw std::runtime_error("boundary type " + std::to_string(f.type) + " is not supported on this path");
+      if (f.type != 3 && f.type != 5 && f.type != 6 && f.type != 8)
+        throw std::runtime_error("boundary type " + std::to_string(f.type) + " is not supported on this path "
+                                 "(3 symmetry, 5 inflow, 6 outflow, 8 periodic)");
       for (int d = 0; d < 3; ++d)
         if (f.start[d] < 0 || f.end[d] < f.start[d] || f.end[d] >= ext[d])
           throw std::runtime_error("boundary face range outside the block");
@@ -673,7 +918,9 @@ void build(corefl_ctx *ctx, std::vector<corefl::HostBlock> blocks) {
       for (int q = 0; q < 3; ++q) { df.start[q] = f.start[q]; df.end[q] = f.end[q]; }
       df.type = f.type; df.face = f.face; df.direction = f.direction;
       B.faces.push_back(df);
+      B.inflow.push_back(f.type == 5 ? make_inflow(ctx, P, h, f, mech) : corefl::DevInflow{});
     }
+    if (P.get_bool("sponge_layer")) setup_sponge(ctx, P, B, mech);
     for (const auto &f : h.inner) {
       corefl::DevInner di{};
       for (int q = 0; q < 3; ++q) {

@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cmath>
+
 namespace corefl {
 
 // One block's device arrays in the DZone layout (Field.h:12-110): f(i,j,k,l) = p[idx(i,j,k) + l*sz].
@@ -15,6 +17,8 @@ struct DevBlock {
   double *efd;                      // Roe entropy-fix delta (sz)
   double *dq;                       // residual (5*n), written only by the RHS-only path
   int bt[6];                        // physical-boundary flags il, ir, jl, jr, kl, kr
+  // sponge layer (null when off): damping rate sigma per interior node (n) and the running mean of cv (nv * n)
+  const double *sponge_sigma, *sponge_mean;
 };
 
 struct DevFace {
@@ -29,6 +33,21 @@ struct DevInner {
   int face, face_layer;
   // the partner's index direction vmap[s] runs along this block's direction s, with sign vsign[s]
   int vmap[3], vsign[3];
+};
+
+// An inflow face (apply_inflow, BoundCond.cu:2190-2705) with its values per face node precomputed on the host:
+// the constant inflow state or the mixing layer's tanh profile of the two streams.
+struct DevInflow {
+  int start[3], end[3];
+  int face, direction;
+  int sub_node;  // constant inflow with inflow_sub: characteristic subsonic boundary value
+  int sub_ghost; // inflow_sub: ghost layers extrapolated from the boundary value, with floors on rho and p
+  int fluct;     // 0 none; 1 white noise on u (constant inflow); 2 white noise on u, v and w (mixing layer)
+  double mw;     // molar mass of the struct's own stream (temperature of subsonic values, mixtures)
+  const double *prof; // per face node (6 + ns) values rho u v w p T Y..., [q * n_face_nodes + node]
+  const double *rms;  // per face node standard deviation of the noise (fluct > 0)
+  unsigned long long seed;
+  unsigned long long *calls; // device count of applications: every application draws fresh noise
 };
 
 struct Scheme {
@@ -65,5 +84,20 @@ hipError_t launch_local_dt(const DevBlock &b, const Scheme &sc, double *part, hi
 hipError_t launch_dt_reduce(const double *part, long n, double *dt_dev, const double *time_dev, double t_end,
                             int clip, hipStream_t s);
 hipError_t launch_time_advance(double *time_dev, const double *dt_dev, hipStream_t s);
+
+// Normal deviate of the inflow noise (host and device): a counter-based draw from (seed, application, node,
+// component) through the SplitMix64 finaliser and Box-Muller.
+__host__ __device__ inline unsigned long long noise_mix(unsigned long long z) {
+  z += 0x9e3779b97f4a7c15ULL;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  return z ^ (z >> 31);
+}
+__host__ __device__ inline double noise_normal(unsigned long long seed, unsigned long long call, long node, int comp) {
+  const unsigned long long key = noise_mix(noise_mix(noise_mix(seed) + call) + (unsigned long long)node);
+  const double u1 = ((double)(noise_mix(key + 2ULL * comp) >> 11) + 0.5) * 0x1p-53;
+  const double u2 = ((double)(noise_mix(key + 2ULL * comp + 1) >> 11) + 0.5) * 0x1p-53;
+  return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+}
 
 } // namespace corefl

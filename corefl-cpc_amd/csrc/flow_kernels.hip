@@ -930,6 +930,13 @@ __global__ __launch_bounds__(256) void k_update(DevBlock b, int rk, const double
         fzm[l] = f;
       }
     }
+    if (b.sponge_sigma) {
+      // sponge layer: relaxation towards the running mean, -sigma (U - U_mean) per unit volume
+      const long iq = ((long)k * b.my + j) * b.mx + i;
+      const double sg = b.sponge_sigma[iq] * b.jac[id];
+#pragma unroll
+      for (int l = 0; l < 5; ++l) r[l] -= sg * (b.cv[id + l * sz] - b.sponge_mean[iq + l * b.n]);
+    }
     if constexpr (RHS_ONLY) {
       const long iq = ((long)k * b.my + j) * b.mx + i;
 #pragma unroll

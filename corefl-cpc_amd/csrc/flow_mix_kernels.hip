@@ -5,6 +5,7 @@
 #include <cmath>
 
 #include "chem_device.hpp"
+#include "mix_state.hpp"
 #include "stencil.hpp"
 #include "weno.hpp"
 
@@ -23,32 +24,7 @@ __device__ __forceinline__ long nid(const DevBlock &b, int i, int j, int k) {
   return (long)(k + b.g) * b.d1 + (long)(j + b.g) * b.d2 + (i + b.g);
 }
 
-// compute_cv_from_bv_1_point<Mixture> + compute_total_energy (FieldOperation.cuh:16-84) and the acoustic
-// speed of update_physical_properties<Mixture> (FieldOperation.cuh:86-120) at node id.
-__device__ void mix_cons(const DevBlock &b, const MixArrays &x, long id) {
-  const long sz = b.sz;
-  const ChemMech &m = *x.mech;
-  const double rho = b.bv[id], u = b.bv[id + sz], v = b.bv[id + 2 * sz], w = b.bv[id + 3 * sz];
-  const double p = b.bv[id + 4 * sz], T = b.bv[id + 5 * sz];
-  double h[NS], cp[NS];
-  chemdev::h_cp<NS>(m, T, h, cp);
-  double E = 0.5 * (u * u + v * v + w * w), imw = 0, cpt = 0;
-#pragma unroll
-  for (int l = 0; l < NS; ++l) {
-    const double y = x.sv[id + l * sz];
-    E += h[l] * y;
-    imw += y * m.imw[l];
-    cpt += y * cp[l];
-    b.cv[id + (5 + l) * sz] = rho * y;
-  }
-  b.cv[id] = rho;
-  b.cv[id + sz] = rho * u;
-  b.cv[id + 2 * sz] = rho * v;
-  b.cv[id + 3 * sz] = rho * w;
-  b.cv[id + 4 * sz] = E * rho - p;
-  const double R = kRu * imw;
-  x.snd[id] = sqrt(cpt / (cpt - R) * R * T);
-}
+using mixdev::mix_cons;
 
 __global__ void k_mix_cv_from_bv(DevBlock b, MixArrays x) {
   const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -780,6 +756,12 @@ __global__ __launch_bounds__(256) void k_mix_update(DevBlock b, MixArrays x, int
       if (x.sv[id + l * sz] > 1e-25 && fabs(wd[l]) > 1e-25) ts = fmin(ts, fabs(ry[l] / wd[l]));
     }
     x.tscale[iq] = ts;
+  }
+  if (b.sponge_sigma) {
+    // sponge layer: relaxation towards the running mean, -sigma (U - U_mean) per unit volume
+    const double sg = b.sponge_sigma[iq] * b.jac[id];
+#pragma unroll
+    for (int l = 0; l < NV; ++l) r[l] -= sg * (b.cv[id + l * sz] - b.sponge_mean[iq + l * b.n]);
   }
   if constexpr (RHS_ONLY) {
 #pragma unroll

@@ -47,6 +47,24 @@ Parameter::Parameter() {
   // residual monitor: the reference requires output_screen in setup.txt; 0 (here the default) turns it off
   ints["output_screen"] = 0;
   bools["limit_flow"] = false; // Parameter.cpp:490
+  // mixing layer (canonical_problem mixingLayer; MixingLayer.cu:9-193, Parameter.cpp:486-487)
+  reals["velocity_ratio"] = -1;
+  reals["ma_c"] = 0;
+  bools["upper_faster"] = true;
+  reals["delta_omega"] = 1e-3;
+  ints["characteristic_velocity_ml"] = 1;
+  ints["fluctuation_seed"] = 0; // seed of the inflow faces' white noise (new: the reference seeds curand from the clock)
+  // sponge layer (Parameter.cpp:467-484; SpongeLayer.cu)
+  bools["sponge_layer"] = false;
+  ints["sponge_function"] = 0;
+  ints["spongeX"] = 0;
+  ints["spongeY"] = 0;
+  ints["spongeZ"] = 0;
+  reals["spongeCoefficient"] = 1.0;
+  for (const char *k : {"spongeXMinusStart", "spongeXMinusEnd", "spongeXPlusStart", "spongeXPlusEnd", "spongeYMinusStart",
+                        "spongeYMinusEnd", "spongeYPlusStart", "spongeYPlusEnd", "spongeZMinusStart", "spongeZMinusEnd",
+                        "spongeZPlusStart", "spongeZPlusEnd"})
+    reals[k] = 0;
 }
 
 static bool starts_with(const std::string &s, const char *p) { return s.rfind(p, 0) == 0; }

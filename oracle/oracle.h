@@ -217,6 +217,38 @@ void orc_mix_point_flux(const orc_params *p, const orc_mech *m, const double *pl
 void orc_mix_positivity_point(const orc_params *p, int ns, int dim, double dt, const double *q, const double *k,
                               const double *jac, const double *snd, int n, double *fc);
 
+/* ---- inflow and symmetry faces, sponge layer (oracle_bc.c) ---- */
+
+/* Normal deviate of the inflow noise from (seed, application, face node, component): SplitMix64 finaliser and
+ * Box-Muller.  This build's own counter-based generator (the reference draws clock-seeded curand numbers). */
+double orc_noise_normal(unsigned long long seed, unsigned long long call, long node, int comp);
+/* apply_inflow (BoundCond.cu:2190-2705) on face nodes s..e (normal `face`, side dir): prof (6 + ns values per face
+ * node: rho u v w p T Y..., [q * npt + node]) is the state without noise; sub_node: constant inflow with inflow_sub
+ * (characteristic subsonic value, BoundCond.cu:2505-2553); sub_ghost: ghosts extrapolated 2 b - interior with floors;
+ * fluct 1: noise on u (rms per node, BoundCond.cu:2562-2597), 2: on u, v, w (mixing layer, 2479-2503); mw: molar mass
+ * of the subsonic temperatures (mixtures); snd: acoustic speed array (NULL for air).  cv is left to the caller.
+ * Pinned by closed forms (tests/test_bc_oracle_cpu.py). */
+void orc_bc_inflow(int mx, int my, int mz, int ngg, const int *s, const int *e, int face, int dir, double *bv,
+                   double *sv, int ns, const double *snd, const double *met, const double *prof, const double *rms,
+                   int sub_node, int sub_ghost, int fluct, double mw, unsigned long long seed,
+                   unsigned long long call);
+/* apply_symmetry (BoundCond.cu:2002-2063).  Pinned by closed forms (tests/test_bc_oracle_cpu.py). */
+void orc_bc_symmetry(int mx, int my, int mz, int ngg, const int *s, const int *e, int face, int dir, double *bv,
+                     double *sv, int ns, const double *met);
+/* The sponge layer's ramp 6 xi^5 - 15 xi^4 + 10 xi^3 (SpongeLayer.cu:164-172). */
+double orc_sponge_ramp(double xi);
+/* Inflow data of face `face` of an air block (faces of type 5): prof 6 values per face node, rms per node; the
+ * noise of the block's applications counts from first_call. */
+void orc_block_set_inflow(orc_block *b, int face, const double *prof, const double *rms, int sub_node, int sub_ghost,
+                          int fluct, double mw, unsigned long long seed, unsigned long long first_call);
+/* The same for a mixture block (prof: 6 + ns values per face node). */
+void orc_mix_set_inflow(orc_mix *b, int face, const double *prof, const double *rms, int sub_node, int sub_ghost,
+                        int fluct, double mw, unsigned long long seed, unsigned long long first_call);
+void orc_mix_set_sponge(orc_mix *b, const double *sigma);
+/* Sponge layer of an air block: damping rate per interior node; the residual gets -J sigma (U - U_mean), the mean
+ * of U over the steps taken, updated at the start of every step. */
+void orc_block_set_sponge(orc_block *b, const double *sigma);
+
 /* ---- step monitor and limit_flow (oracle_monitor.c) ---- */
 
 /* store_last_step (TimeAdvanceFunc.cu:6-19): rho, |V|, p, T of the interior nodes into last (4 * mx*my*mz). */

@@ -225,14 +225,25 @@ typedef struct {
   int s[3], e[3];
 } face_t;
 
+/* Inflow face data (orc_block_set_inflow) */
+typedef struct {
+  double *prof, *rms;
+  int sub_node, sub_ghost, fluct;
+  double mw;
+  unsigned long long seed, call;
+} inflow_t;
+
 struct orc_block {
   orc_params p;
   int mx, my, mz, g, dim;
   long d2, d1, sz, n;
   face_t faces[64];
+  inflow_t inflow[64];
   int n_face;
   int bt[6]; /* physical-boundary flags per side for the one-sided viscous closures */
   double *cv, *bv, *qn, *dq, *mul, *jac, *met, *fv, *gv, *hv, *efd;
+  double *sponge_sigma, *sponge_mean; /* sponge layer: damping rate (n), running mean of cv (5 * n) */
+  unsigned long long sponge_iter;
 };
 
 #define ID(b, i, j, k) ((long)((k) + (b)->g) * (b)->d1 + (long)((j) + (b)->g) * (b)->d2 + ((i) + (b)->g))
@@ -290,6 +301,12 @@ orc_block *orc_block_create(const orc_params *p, int mx, int my, int mz, int ngg
 
 void orc_block_destroy(orc_block *b) {
   if (!b) return;
+  for (int f = 0; f < 64; ++f) {
+    free(b->inflow[f].prof);
+    free(b->inflow[f].rms);
+  }
+  free(b->sponge_sigma);
+  free(b->sponge_mean);
   free(b->cv); free(b->qn); free(b->bv); free(b->fv); free(b->gv); free(b->hv);
   free(b->mul); free(b->efd); free(b->dq); free(b->jac); free(b->met);
   free(b);
@@ -357,12 +374,90 @@ static void apply_periodic(orc_block *b, const face_t *F) {
       }
 }
 
-/* DBoundCond::apply_boundary_conditions order: outflow faces, then periodic faces (BoundCond.cu:3489-3619) */
+/* cv of the face nodes (from 0 when the face nodes are included) and of their ghost layers */
+static void face_cv(orc_block *b, const face_t *F, int from) {
+  int d[3] = {0, 0, 0};
+  d[F->face] = F->dir;
+  for (int k = F->s[2]; k <= F->e[2]; ++k)
+    for (int j = F->s[1]; j <= F->e[1]; ++j)
+      for (int i = F->s[0]; i <= F->e[0]; ++i)
+        for (int g = from; g <= b->g; ++g) cv_from_bv(b, ID(b, i + g * d[0], j + g * d[1], k + g * d[2]));
+}
+
+/* DBoundCond::apply_boundary_conditions order (BoundCond.cu:3489-3619): outflow, inflow, symmetry, periodic */
 static void apply_bcs(orc_block *b) {
   for (int f = 0; f < b->n_face; ++f)
     if (b->faces[f].type == 6) apply_outflow(b, &b->faces[f]);
+  for (int f = 0; f < b->n_face; ++f) {
+    const face_t *F = &b->faces[f];
+    inflow_t *I = &b->inflow[f];
+    if (F->type != 5 || !I->prof) continue;
+    orc_bc_inflow(b->mx, b->my, b->mz, b->g, F->s, F->e, F->face, F->dir, b->bv, NULL, 0, NULL, b->met, I->prof,
+                  I->rms, I->sub_node, I->sub_ghost, I->fluct, I->mw, I->seed, I->call);
+    face_cv(b, F, 0);
+    if (I->fluct) ++I->call;
+  }
+  for (int f = 0; f < b->n_face; ++f) {
+    const face_t *F = &b->faces[f];
+    if (F->type != 3) continue;
+    orc_bc_symmetry(b->mx, b->my, b->mz, b->g, F->s, F->e, F->face, F->dir, b->bv, NULL, 0, b->met);
+    face_cv(b, F, 1);
+  }
   for (int f = 0; f < b->n_face; ++f)
     if (b->faces[f].type == 8) apply_periodic(b, &b->faces[f]);
+}
+
+void orc_block_set_inflow(orc_block *b, int face, const double *prof, const double *rms, int sub_node, int sub_ghost,
+                          int fluct, double mw, unsigned long long seed, unsigned long long first_call) {
+  const face_t *F = &b->faces[face];
+  const long npt = (long)(F->e[0] - F->s[0] + 1) * (F->e[1] - F->s[1] + 1) * (F->e[2] - F->s[2] + 1);
+  inflow_t *I = &b->inflow[face];
+  free(I->prof);
+  free(I->rms);
+  I->prof = malloc(sizeof(double) * 6 * (size_t)npt);
+  I->rms = malloc(sizeof(double) * (size_t)npt);
+  memcpy(I->prof, prof, sizeof(double) * 6 * (size_t)npt);
+  memcpy(I->rms, rms, sizeof(double) * (size_t)npt);
+  I->sub_node = sub_node;
+  I->sub_ghost = sub_ghost;
+  I->fluct = fluct;
+  I->mw = mw;
+  I->seed = seed;
+  I->call = first_call;
+}
+
+void orc_block_set_sponge(orc_block *b, const double *sigma) {
+  free(b->sponge_sigma);
+  free(b->sponge_mean);
+  b->sponge_sigma = malloc(sizeof(double) * (size_t)b->n);
+  memcpy(b->sponge_sigma, sigma, sizeof(double) * (size_t)b->n);
+  b->sponge_mean = calloc((size_t)5 * b->n, sizeof(double));
+  b->sponge_iter = 0;
+}
+
+/* The sponge layer's relaxation -J sigma (U - U_mean) added to the residual, and its running mean. */
+static void sponge_source(orc_block *b) {
+  if (!b->sponge_sigma) return;
+  for (int k = 0; k < b->mz; ++k)
+    for (int j = 0; j < b->my; ++j)
+      for (int i = 0; i < b->mx; ++i) {
+        const long id = ID(b, i, j, k), iq = IDQ(b, i, j, k);
+        const double sg = b->sponge_sigma[iq] * b->jac[id];
+        for (int l = 0; l < 5; ++l) b->dq[iq + l * b->n] -= sg * (b->cv[id + l * b->sz] - b->sponge_mean[iq + l * b->n]);
+      }
+}
+
+static void sponge_mean(orc_block *b) {
+  if (!b->sponge_sigma) return;
+  const double n = (double)b->sponge_iter, inv = 1.0 / (n + 1);
+  for (int k = 0; k < b->mz; ++k)
+    for (int j = 0; j < b->my; ++j)
+      for (int i = 0; i < b->mx; ++i) {
+        const long id = ID(b, i, j, k), iq = IDQ(b, i, j, k);
+        for (int l = 0; l < 5; ++l)
+          b->sponge_mean[iq + l * b->n] = (b->sponge_mean[iq + l * b->n] * n + b->cv[id + l * b->sz]) * inv;
+      }
+  ++b->sponge_iter;
 }
 
 /* periodic_exchange task 1 for the viscous fluxes (DataCommunication.cu:628-683) */
@@ -671,6 +766,7 @@ void orc_block_rhs(orc_block *b) {
   inviscid_dir(b, 0);
   inviscid_dir(b, 1);
   if (b->dim == 3) inviscid_dir(b, 2);
+  sponge_source(b);
 }
 
 static const double rk_a[3] = {1.0, 0.75, 1.0 / 3.0};
@@ -769,6 +865,7 @@ double orc_block_cfl_dt(orc_block *b) {
 
 /* One SSP-RK3 step (RK.cuh:100-192) for a single block with only outflow/periodic faces. */
 void orc_block_rk3_step(orc_block *b, double dt) {
+  sponge_mean(b);
   memcpy(b->qn, b->cv, sizeof(double) * 5 * b->sz);
   for (int rk = 0; rk < 3; ++rk) {
     orc_block_rhs(b);

@@ -36,6 +36,14 @@ struct orc_mix {
   double *cv, *qn, *bv, *sv, *c, *efd, *dq, *jac, *met, *hstep, *tscale;
   double *cn, *wu_ts; /* Wu's splitting: frozen transport residual, reaction time scale (starts at zero) */
   double pp_dt;       /* step size seen by the positivity limiter (the global dt of the current step) */
+  struct {
+    double *prof, *rms;
+    int sub_node, sub_ghost, fluct;
+    double mw;
+    unsigned long long seed, call;
+  } inflow[64];       /* inflow faces (orc_mix_set_inflow) */
+  double *sponge_sigma, *sponge_mean; /* sponge layer: damping rate (n), running mean of cv (nv * n) */
+  unsigned long long sponge_iter;
   int viscous;        /* viscous_order 2 or 8 with transport data set */
   int bt[6];          /* physical-boundary flags per side for the one-sided collocated differences */
   orc_tran tran;
@@ -102,6 +110,12 @@ orc_mix *orc_mix_create(const orc_params *p, const orc_mech *m, int mx, int my, 
 
 void orc_mix_destroy(orc_mix *b) {
   if (!b) return;
+  for (int f = 0; f < 64; ++f) {
+    free(b->inflow[f].prof);
+    free(b->inflow[f].rms);
+  }
+  free(b->sponge_sigma);
+  free(b->sponge_mean);
   free(b->cv); free(b->qn); free(b->bv); free(b->sv); free(b->c); free(b->efd); free(b->dq); free(b->hstep);
   free(b->tscale); free(b->cn); free(b->wu_ts);
   free(b->mul); free(b->cond); free(b->rhod);
@@ -202,11 +216,96 @@ static void apply_periodic(orc_mix *b, const mface_t *F) {
       }
 }
 
+/* cv and acoustic speed of the face nodes (from 0 when the face nodes are included) and of their ghost layers */
+static void face_cv(orc_mix *b, const mface_t *F, int from) {
+  int d[3] = {0, 0, 0};
+  d[F->face] = F->dir;
+  for (int k = F->s[2]; k <= F->e[2]; ++k)
+    for (int j = F->s[1]; j <= F->e[1]; ++j)
+      for (int i = F->s[0]; i <= F->e[0]; ++i)
+        for (int g = from; g <= b->g; ++g) {
+          const long id = ID(b, i + g * d[0], j + g * d[1], k + g * d[2]);
+          cv_from_bv(b, id);
+          properties(b, id);
+        }
+}
+
+/* DBoundCond::apply_boundary_conditions order (BoundCond.cu:3489-3619): outflow, inflow, symmetry, periodic */
 static void apply_bcs(orc_mix *b) {
   for (int f = 0; f < b->n_face; ++f)
     if (b->faces[f].type == 6) apply_outflow(b, &b->faces[f]);
+  for (int f = 0; f < b->n_face; ++f) {
+    const mface_t *F = &b->faces[f];
+    if (F->type != 5 || !b->inflow[f].prof) continue;
+    /* the subsonic value reads the acoustic speed of the current state */
+    if (b->inflow[f].sub_node) all_properties(b);
+    orc_bc_inflow(b->mx, b->my, b->mz, b->g, F->s, F->e, F->face, F->dir, b->bv, b->sv, b->ns, b->c, b->met,
+                  b->inflow[f].prof, b->inflow[f].rms, b->inflow[f].sub_node, b->inflow[f].sub_ghost,
+                  b->inflow[f].fluct, b->inflow[f].mw, b->inflow[f].seed, b->inflow[f].call);
+    face_cv(b, F, 0);
+    if (b->inflow[f].fluct) ++b->inflow[f].call;
+  }
+  for (int f = 0; f < b->n_face; ++f) {
+    const mface_t *F = &b->faces[f];
+    if (F->type != 3) continue;
+    orc_bc_symmetry(b->mx, b->my, b->mz, b->g, F->s, F->e, F->face, F->dir, b->bv, b->sv, b->ns, b->met);
+    face_cv(b, F, 1);
+  }
   for (int f = 0; f < b->n_face; ++f)
     if (b->faces[f].type == 8) apply_periodic(b, &b->faces[f]);
+}
+
+void orc_mix_set_inflow(orc_mix *b, int face, const double *prof, const double *rms, int sub_node, int sub_ghost,
+                        int fluct, double mw, unsigned long long seed, unsigned long long first_call) {
+  const mface_t *F = &b->faces[face];
+  const long npt = (long)(F->e[0] - F->s[0] + 1) * (F->e[1] - F->s[1] + 1) * (F->e[2] - F->s[2] + 1);
+  free(b->inflow[face].prof);
+  free(b->inflow[face].rms);
+  b->inflow[face].prof = malloc(sizeof(double) * (6 + b->ns) * (size_t)npt);
+  b->inflow[face].rms = malloc(sizeof(double) * (size_t)npt);
+  memcpy(b->inflow[face].prof, prof, sizeof(double) * (6 + b->ns) * (size_t)npt);
+  memcpy(b->inflow[face].rms, rms, sizeof(double) * (size_t)npt);
+  b->inflow[face].sub_node = sub_node;
+  b->inflow[face].sub_ghost = sub_ghost;
+  b->inflow[face].fluct = fluct;
+  b->inflow[face].mw = mw;
+  b->inflow[face].seed = seed;
+  b->inflow[face].call = first_call;
+}
+
+void orc_mix_set_sponge(orc_mix *b, const double *sigma) {
+  free(b->sponge_sigma);
+  free(b->sponge_mean);
+  b->sponge_sigma = malloc(sizeof(double) * (size_t)b->n);
+  memcpy(b->sponge_sigma, sigma, sizeof(double) * (size_t)b->n);
+  b->sponge_mean = calloc((size_t)b->nv * b->n, sizeof(double));
+  b->sponge_iter = 0;
+}
+
+/* The sponge layer's relaxation -J sigma (U - U_mean) added to the residual, and its running mean. */
+static void sponge_source(orc_mix *b) {
+  if (!b->sponge_sigma) return;
+  for (int k = 0; k < b->mz; ++k)
+    for (int j = 0; j < b->my; ++j)
+      for (int i = 0; i < b->mx; ++i) {
+        const long id = ID(b, i, j, k), iq = IDQ(b, i, j, k);
+        const double sg = b->sponge_sigma[iq] * b->jac[id];
+        for (int l = 0; l < b->nv; ++l)
+          b->dq[iq + l * b->n] -= sg * (b->cv[id + l * b->sz] - b->sponge_mean[iq + l * b->n]);
+      }
+}
+
+static void sponge_mean(orc_mix *b) {
+  if (!b->sponge_sigma) return;
+  const double n = (double)b->sponge_iter, inv = 1.0 / (n + 1);
+  for (int k = 0; k < b->mz; ++k)
+    for (int j = 0; j < b->my; ++j)
+      for (int i = 0; i < b->mx; ++i) {
+        const long id = ID(b, i, j, k), iq = IDQ(b, i, j, k);
+        for (int l = 0; l < b->nv; ++l)
+          b->sponge_mean[iq + l * b->n] = (b->sponge_mean[iq + l * b->n] * n + b->cv[id + l * b->sz]) * inv;
+      }
+  ++b->sponge_iter;
 }
 
 void orc_mix_set_state(orc_mix *b, const double *bv, const double *sv) {
@@ -1210,6 +1309,7 @@ void orc_mix_rhs(orc_mix *b) {
   inviscid_dir(b, 1);
   if (b->dim == 3) inviscid_dir(b, 2);
   if (b->explicit_source) explicit_source(b);
+  sponge_source(b);
 }
 
 static const double rk_a[3] = {1.0, 0.75, 1.0 / 3.0};
@@ -1325,7 +1425,14 @@ double orc_mix_cfl_dt(orc_mix *b) {
 
 void orc_mix_set_step(orc_mix *b, double dt) { b->pp_dt = dt; }
 
+static void rk3_stages(orc_mix *b, double dt);
+
 void orc_mix_rk3_step(orc_mix *b, double dt) {
+  sponge_mean(b);
+  rk3_stages(b, dt);
+}
+
+static void rk3_stages(orc_mix *b, double dt) {
   b->pp_dt = dt;
   memcpy(b->qn, b->cv, sizeof(double) * b->nv * b->sz);
   for (int rk = 0; rk < 3; ++rk) {
@@ -1375,8 +1482,9 @@ int orc_mix_chem_substep(orc_mix *b, double dt, double rtol, double atol, int ma
 
 /* Strang splitting: chemistry dt/2, one SSP-RK3 flow step dt, chemistry dt/2. */
 int orc_mix_strang_step(orc_mix *b, double dt, double rtol, double atol, int max_attempts) {
+  sponge_mean(b);
   int f = orc_mix_chem_substep(b, 0.5 * dt, rtol, atol, max_attempts);
-  orc_mix_rk3_step(b, dt);
+  rk3_stages(b, dt);
   f += orc_mix_chem_substep(b, 0.5 * dt, rtol, atol, max_attempts);
   return f;
 }
@@ -1448,6 +1556,7 @@ static double wu_zero_d(orc_mix *b, long id, const double *cn, double h, int met
  * reaction time scale (wu_reaction_step, 373-421); boundary conditions; then three SSP-RK3 stages over dt/2 of
  * the transport change R(U) - Cn (wu_RK_subStep and update_cv_and_bv_rk_wu, 504-634). */
 void orc_mix_wu_step(orc_mix *b, double dt, int method) {
+  sponge_mean(b);
   const int nv = b->nv;
   const int es = b->explicit_source;
   b->explicit_source = 0;

@@ -336,3 +336,167 @@ def reactive_state(n, ngg, seed=3, n_kernels=2, peak=(1800.0, 2400.0), radius=(2
     rho = CC.density(p0, T.ravel(), Y)
     bv = np.stack([rho, vel[0].ravel(), vel[1].ravel(), vel[2].ravel(), np.full(T.size, p0), T.ravel()])
     return bv, Y
+
+
+def channel(mx=24, my=12, mz=5, mach=0.5, inflow_sub=1, fluct=0, intensity=0.0, recon=4, inviscid=4, visc=2,
+            cfl=0.5, length=0.02):
+    """Air channel on a Cartesian grid: inflow (label 5) at i = 0, outflow (6) at i = mx - 1, symmetry (3) on
+    both j faces, periodic (8) in z.  The interior starts with a denser slug in the middle third."""
+    x = np.linspace(0.0, length, mx)
+    y = np.linspace(-0.25 * length, 0.25 * length, my)
+    z = np.linspace(0.0, 0.2 * length, mz)
+    X, Y, Z = np.meshgrid(x, y, z, indexing="ij")
+    setup = f"""// air channel: inflow, outflow, symmetry walls, periodic span
+real gridScale = 1
+int gridIsBinary = 1
+bool steady = 0
+int temporal_scheme = 3
+bool fixed_time_step = 0
+real cfl = {cfl}
+int reconstruction = {recon}
+int inviscid_scheme = {inviscid}
+int viscous_order = {visc}
+array string boundary_conditions {{
+    inlet outlet wall periodic
+}}
+struct inlet {{
+    string type = inflow
+    int label = 5
+    real mach = {mach}
+    real pressure = 101325
+    real temperature = 300
+    int inflow_sub = {inflow_sub}
+    int fluctuation_type = {fluct}
+    real fluctuation_intensity = {intensity}
+}}
+struct outlet {{
+    string type = outflow
+    int label = 6
+}}
+struct wall {{
+    string type = symmetry
+    int label = 3
+}}
+struct periodic {{
+    string type = periodic
+    int label = 8
+}}
+struct slug {{
+    string type = inflow
+    int label = 5
+    real mach = {0.6 * mach}
+    real pressure = 121590
+    real temperature = 280
+}}
+string default_init = inlet
+string reference_state = inlet
+int groups_init = 2
+struct init_cond_0 {{
+    real x0 = {length / 3!r}
+    real x1 = {2 * length / 3!r}
+    real y0 = -1
+    real y1 = 1
+    real z0 = -1
+    real z1 = 1
+    string name = slug
+}}
+int fluctuation_seed = 11
+"""
+    faces = [[0, 0, 0, my - 1, 0, mz - 1, 5], [mx - 1, mx - 1, 0, my - 1, 0, mz - 1, 6],
+             [0, mx - 1, 0, 0, 0, mz - 1, 3], [0, mx - 1, my - 1, my - 1, 0, mz - 1, 3],
+             [0, mx - 1, 0, my - 1, 0, 0, 8], [0, mx - 1, 0, my - 1, mz - 1, mz - 1, 8]]
+    return dict(setup=setup, grid=(X, Y, Z), scale=1.0, faces=faces, boundary=boundary_text(faces),
+                scheme=(inviscid, recon, visc), prandtl=0.72, cfl=cfl, seed=11, length=length)
+
+
+# The two streams of the reacting mixing layer: H2/N2 (Y_H2 = 0.2) at 600 K and 300 m/s over air at 1200 K and 600 m/s
+MIXING_UPPER = dict(T=600.0, p=101325.0, U=300.0, Y={"H2": 0.2, "N2": 0.8})
+MIXING_LOWER = dict(T=1200.0, p=101325.0, U=600.0, Y={"O2": 0.233, "N2": 0.767})
+
+
+def stream_mach(stream, cp_of_T):
+    """Mach number of a stream of speed U: gamma from the mixture cp at T (cp_of_T: mass fractions -> cp)."""
+    import chem_cases as CC
+    names = ["H2", "H", "O2", "O", "OH", "HO2", "H2O2", "H2O", "N2"]
+    Y = np.array([stream["Y"].get(s, 0.0) for s in names])
+    imw = np.sum(Y / CC.MW)
+    R = CC.RU * imw
+    cp = cp_of_T(Y, stream["T"])
+    c = np.sqrt(cp / (cp - R) * R * stream["T"])
+    return stream["U"] / c
+
+
+def mixing_layer(mx=24, my=16, mz=5, length=0.02, height=0.01, delta_omega=1e-3, intensity=0.02, recon=4,
+                 inviscid=4, visc=2, reaction=1, temporal=5, sponge=True, mach=(None, None), cfl=0.5, seed=5):
+    """Reacting H2-N2 / air mixing layer on a Cartesian block: mixingLayer initial field and inflow (label 5, white
+    noise), outflow (6) at the exit and on both j faces, periodic span, and a sponge layer over the last quarter of
+    x.  mach: the Mach numbers of the two streams (the 300 and 600 m/s speeds divided by each stream's sound
+    speed; stream_mach computes them)."""
+    import chem_cases as CC
+    x = np.linspace(0.0, length, mx)
+    y = np.linspace(-0.5 * height, 0.5 * height, my)
+    z = np.linspace(0.0, 0.25 * height, mz)
+    X, Y, Z = np.meshgrid(x, y, z, indexing="ij")
+
+    def struct(name, st, m):
+        comp = "\n".join(f"    real {k} = {v!r}" for k, v in st["Y"].items())
+        return f"""struct {name} {{
+    real temperature = {st['T']!r}
+    real pressure = {st['p']!r}
+    real mach = {float(m)!r}
+    real mixture_fraction = 0
+{comp}
+}}"""
+    spon = f"""bool sponge_layer = 1
+int spongeX = 2
+real spongeXPlusStart = {0.75 * length!r}
+real spongeXPlusEnd = {length!r}
+real spongeCoefficient = 1.0
+""" if sponge else ""
+    setup = f"""// reacting mixing layer: H2/N2 over air
+string canonical_problem = mixingLayer
+real gridScale = 1
+int gridIsBinary = 1
+bool steady = 0
+int species = 1
+int reaction = {reaction}
+string mechanism_file = {CC.MECH}
+string therm_file = {CC.THERM}
+string transport_file = {CC.TRAN_FILE}
+int temporal_scheme = {temporal}
+bool fixed_time_step = 0
+real cfl = {cfl}
+int reconstruction = {recon}
+int inviscid_scheme = {inviscid}
+int viscous_order = {visc}
+real delta_omega = {delta_omega!r}
+real ma_c = 0
+{struct('upper_stream', MIXING_UPPER, mach[0])}
+{struct('lower_stream', MIXING_LOWER, mach[1])}
+array string boundary_conditions {{
+    inlet outlet periodic
+}}
+struct inlet {{
+    string type = inflow
+    int label = 5
+    int fluctuation_type = 1
+    real fluctuation_intensity = {intensity!r}
+}}
+struct outlet {{
+    string type = outflow
+    int label = 6
+}}
+struct periodic {{
+    string type = periodic
+    int label = 8
+}}
+string default_init = inlet
+string reference_state = inlet
+int fluctuation_seed = {seed}
+{spon}"""
+    faces = [[0, 0, 0, my - 1, 0, mz - 1, 5], [mx - 1, mx - 1, 0, my - 1, 0, mz - 1, 6],
+             [0, mx - 1, 0, 0, 0, mz - 1, 6], [0, mx - 1, my - 1, my - 1, 0, mz - 1, 6],
+             [0, mx - 1, 0, my - 1, 0, 0, 8], [0, mx - 1, 0, my - 1, mz - 1, mz - 1, 8]]
+    return dict(setup=setup, grid=(X, Y, Z), scale=1.0, faces=faces, boundary=boundary_text(faces),
+                scheme=(inviscid, recon, visc), cfl=cfl, seed=seed, length=length, delta_omega=delta_omega,
+                intensity=intensity)

@@ -510,3 +510,77 @@ def interface_flux_pv(p, pv, ml, mr, jl=1.0, jr=1.0, efl=0.0, efr=0.0):
     fc = np.zeros(5)
     lib.orc_interface_flux_pv(ctypes.byref(p), _p(pv), _p(ml), _p(mr), jl, jr, efl, efr, _p(fc))
     return fc
+
+
+def _bc_protos(lib):
+    P, I, D, U = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_ulonglong
+    lib.orc_noise_normal.restype = D
+    lib.orc_noise_normal.argtypes = [U, U, ctypes.c_long, I]
+    lib.orc_bc_inflow.argtypes = [I, I, I, I, P, P, I, I, P, P, I, P, P, P, P, I, I, I, D, U, U]
+    lib.orc_bc_symmetry.argtypes = [I, I, I, I, P, P, I, I, P, P, I, P]
+    lib.orc_sponge_ramp.restype = D
+    lib.orc_sponge_ramp.argtypes = [D]
+    for name in ("orc_block_set_inflow", "orc_mix_set_inflow"):
+        getattr(lib, name).argtypes = [P, I, P, P, I, I, I, D, U, U]
+    lib.orc_block_set_sponge.argtypes = [P, P]
+    lib.orc_mix_set_sponge.argtypes = [P, P]
+
+
+def noise_normal(seed, call, node, comp):
+    lib = load()
+    _bc_protos(lib)
+    return lib.orc_noise_normal(seed, call, node, comp)
+
+
+def sponge_ramp(xi):
+    lib = load()
+    _bc_protos(lib)
+    return lib.orc_sponge_ramp(xi)
+
+
+def bc_inflow(shape, ngg, s, e, face, direction, bv, met, prof, rms, sub_node=0, sub_ghost=0, fluct=0, mw=28.9647,
+              seed=0, call=0, sv=None, snd=None):
+    """apply_inflow on copies of bv (and sv): returns (bv, sv)."""
+    lib = load()
+    _bc_protos(lib)
+    mx, my, mz = shape
+    bv = np.array(bv, dtype=np.float64).ravel().copy()
+    sva = None if sv is None else np.array(sv, dtype=np.float64).ravel().copy()
+    ns = 0 if sv is None else sva.size // (bv.size // 6)
+    s3, e3 = np.ascontiguousarray(s, np.int32), np.ascontiguousarray(e, np.int32)
+    met = np.ascontiguousarray(met, float)
+    prof, rms = np.ascontiguousarray(prof, float), np.ascontiguousarray(rms, float)
+    sndp = None if snd is None else _p(np.ascontiguousarray(snd, float))
+    lib.orc_bc_inflow(mx, my, mz, ngg, _p(s3), _p(e3), face, direction, _p(bv), None if sva is None else _p(sva), ns,
+                      sndp, _p(met), _p(prof), _p(rms), sub_node, sub_ghost, fluct, mw, seed, call)
+    return bv.reshape(6, -1), sva
+
+
+def bc_symmetry(shape, ngg, s, e, face, direction, bv, met):
+    lib = load()
+    _bc_protos(lib)
+    mx, my, mz = shape
+    bv = np.array(bv, dtype=np.float64).ravel().copy()
+    s3, e3 = np.ascontiguousarray(s, np.int32), np.ascontiguousarray(e, np.int32)
+    lib.orc_bc_symmetry(mx, my, mz, ngg, _p(s3), _p(e3), face, direction, _p(bv), None, 0,
+                        _p(np.ascontiguousarray(met, float)))
+    return bv.reshape(6, -1)
+
+
+def _set_inflow(fn, h, face, prof, rms, sub_node, sub_ghost, fluct, mw, seed, first_call):
+    prof, rms = np.ascontiguousarray(prof, float), np.ascontiguousarray(rms, float)
+    fn(h, face, _p(prof), _p(rms), sub_node, sub_ghost, fluct, mw, seed, first_call)
+
+
+def block_set_inflow(B, face, prof, rms, sub_node=0, sub_ghost=0, fluct=0, mw=28.9647, seed=0, first_call=0):
+    lib = load()
+    _bc_protos(lib)
+    fn = lib.orc_mix_set_inflow if isinstance(B, MixBlock) else lib.orc_block_set_inflow
+    _set_inflow(fn, B._h, face, prof, rms, sub_node, sub_ghost, fluct, mw, seed, first_call)
+
+
+def block_set_sponge(B, sigma):
+    lib = load()
+    _bc_protos(lib)
+    sigma = np.ascontiguousarray(sigma, float)
+    (lib.orc_mix_set_sponge if isinstance(B, MixBlock) else lib.orc_block_set_sponge)(B._h, _p(sigma))
