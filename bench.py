@@ -5,12 +5,15 @@ Sutherland viscosity, Pr 0.72, characteristic WENO7 + AUSM+, 4th-order collocate
 CFL 0.5 step computed on the device each step, FP64.  Each step is replayed from one hipGraph.
 Prints one JSON line.
 
-Two further objects ride on the same line (skip with --no-reactive / --no-surrogate):
+Further objects ride on the same line (skip with --no-reactive / --no-surrogate / --no-mixing):
   reactive   - the reacting H2/air periodic box (128^3 unique cells, 9 species, WENO5 + Roe, Strang-split
                batched exact chemistry with the autotuned launch shape), Mcell-steps/s;
   surrogate  - the chemistry surrogate trained on the GPU on exact-integrator states, then one chemistry
                sub-step of 256^3 cells by surrogate (with fallback) and by the exact integrator: speedup,
-               max |dY|, max |dT| against the exact FP64 result, fallback fraction.
+               max |dY|, max |dT| against the exact FP64 result, fallback fraction;
+  mixing_layer - the reacting H2-N2/air mixing layer on 8 blocks of 384x192x96 in one context (WENO5 + HLLC,
+               viscous, Strang chemistry, inflow noise, sponge outflow): Mcell-steps/s with exact chemistry and
+               with the surrogate + fallback in the sub-steps, and one sub-step of a block both ways.
 """
 import argparse
 import json
@@ -221,6 +224,118 @@ def surrogate_bench(n_states=2000000, train_steps=30000, n_eval=256, dt=5e-8,
     return out
 
 
+def stream_machs(ch, streams, names):
+    """Mach numbers of the mixing layer's streams (speed U over the stream's sound speed, gamma from the mixture
+    cp of the NASA-7 tables the chemistry context parsed)."""
+    mw, lo, hi, tmid = ch.table("mw"), ch.table("lo"), ch.table("hi"), ch.table("t_mid")
+    out = []
+    for st in streams:
+        Y = np.array([st["Y"].get(s, 0.0) for s in names])
+        T = st["T"]
+        a = np.where((T < tmid)[:, None], lo, hi)
+        cp_r = a[:, 0] + a[:, 1] * T + a[:, 2] * T ** 2 + a[:, 3] * T ** 3 + a[:, 4] * T ** 4
+        R = 8314.462618 * np.sum(Y / mw)
+        cp = float(np.sum(Y * cp_r * 8314.462618 / mw))
+        out.append(st["U"] / np.sqrt(cp / (cp - R) * R * T))
+    return out
+
+
+def mixing_layer_bench(n_blocks=8, shape=(384, 192, 96), steps=3, warmup=1, n_states=1000000, train_steps=15000):
+    """The reacting H2-N2 / air mixing layer on n_blocks blocks joined in a row (WENO5 + HLLC, viscous, Strang-split
+    chemistry, sponge outflow, seeded inflow noise), SSP-RK3 steps replayed from a hipGraph: Mcell-steps/s with the
+    batched exact chemistry and with the surrogate (gate + exact fallback) in the sub-steps, and one chemistry
+    sub-step of a block's cells both ways against the exact FP64 result."""
+    import cases
+    import chem_cases as CC
+    import corefl
+    import surrogate_train as ST
+    names = ["H2", "H", "O2", "O", "OH", "HO2", "H2O2", "H2O", "N2"]
+    ch0 = corefl.Chemistry(CC.MECH, CC.THERM)
+    machs = stream_machs(ch0, (cases.MIXING_UPPER, cases.MIXING_LOWER), names)
+    mx, my, mz = shape
+    t0 = time.perf_counter()
+    case = cases.mixing_layer_blocks(n_blocks=n_blocks, mx=mx, my=my, mz=mz, mach=machs, recon=4, inviscid=4, visc=2,
+                                     reaction=1, temporal=5)
+    S = corefl.Solver(setup_text=case["setup"], blocks=case["grids"], boundary_text=case["boundary"],
+                      inner_text=case["inner"])
+    del case["grids"]
+    cells = case["cells"]
+    log(f"bench: mixing layer {n_blocks} x {mx}x{my}x{mz} ready in {time.perf_counter() - t0:.1f} s")
+    S.advance(warmup, graph=True)
+    S.sync()
+
+    def timed_steps():
+        S.advance(1, graph=True)  # (re)capture, untimed
+        S.sync()
+        t1 = time.perf_counter()
+        S.advance(steps, graph=True)
+        S.sync()
+        return time.perf_counter() - t1
+
+    el_exact = timed_steps()
+    log(f"bench: mixing layer exact chemistry {el_exact / steps * 1e3:.1f} ms/step")
+    # the surrogate, trained on the exact integrator's states of the two streams' mixtures
+    t1 = time.perf_counter()
+    ya = np.array([cases.MIXING_UPPER["Y"].get(s, 0.0) for s in names])
+    yb = np.array([cases.MIXING_LOWER["Y"].get(s, 0.0) for s in names])
+    data = ST.build_mixing_dataset(ch0, n_states, 5, ya, yb, cases.MIXING_UPPER["T"], cases.MIXING_LOWER["T"])
+    sur = corefl.Surrogate(8192)
+    _, loss = ST.train(sur, data, ch0.table("mw"), steps=train_steps)
+    del data
+    ST.set_gate(sur, dT_max=0.35, dT_step=3.0)
+    t_train = time.perf_counter() - t1
+    log(f"bench: mixing-layer surrogate trained in {t_train:.1f} s, loss {np.mean(loss[-100:]):.3e}")
+    # one chemistry sub-step of the middle block's cells: surrogate with fallback against the exact sub-step
+    dt = S.compute_dt()
+    ch = S.chemistry(n_blocks // 2)
+    g = ch.get_cells()
+    rho = ch.get_cells()["rhoY"].sum(axis=0)
+    T0, Y0 = g["T"], g["rhoY"] / rho
+    del g
+
+    def best_of(fn, reps=3):
+        best = 1e30
+        for _ in range(reps):
+            ch.set_cells(rho, T0, rho * Y0)
+            ch.sync()
+            t2 = time.perf_counter()
+            fn()
+            ch.sync()
+            best = min(best, time.perf_counter() - t2)
+        return best * 1e3
+
+    ms_exact = best_of(lambda: ch.integrate(0.5 * dt))
+    x = ch.get_cells()
+    nf = [0]
+    ms_sur = best_of(lambda: nf.__setitem__(0, sur.chem_step(ch, 0.5 * dt)))
+    sg = ch.get_cells()
+    sub = {"cells": int(T0.size), "dt_s": 0.5 * dt, "exact_ms": round(ms_exact, 3), "surrogate_ms": round(ms_sur, 3),
+           "speedup": round(ms_exact / ms_sur, 3),
+           "max_abs_dY": float(np.max(np.abs(sg["rhoY"] - x["rhoY"]) / rho)),
+           "max_abs_dT_K": float(np.max(np.abs(sg["T"] - x["T"]))), "fallback_frac": round(nf[0] / T0.size, 6)}
+    del x, sg
+    log("bench: mixing-layer chemistry sub-step " + json.dumps(sub))
+    S.set_chemistry_surrogate(sur)
+    el_sur = timed_steps()
+    bv = S.get(corefl.BV, n_blocks // 2)
+    finite = bool(np.isfinite(bv).all())
+    T = bv[5 * bv.size // 6:]
+    out = {"workload": f"reacting H2-N2/air mixing layer, {n_blocks} blocks of {mx}x{my}x{mz} joined in a row "
+                       f"({cells} nodes), 9 species / 19 reactions, WENO5 + HLLC, 2nd-order viscous with mixture "
+                       "transport, Strang-split chemistry, inflow noise 2%, sponge outflow, hipGraph replay",
+           "cells": cells, "steps": steps,
+           "exact": {"value": round(cells * steps / el_exact / 1e6, 3), "unit": "Mcell-steps/s",
+                     "ms_per_step": round(el_exact / steps * 1e3, 2)},
+           "surrogate": {"value": round(cells * steps / el_sur / 1e6, 3), "unit": "Mcell-steps/s",
+                         "ms_per_step": round(el_sur / steps * 1e3, 2)},
+           "step_speedup": round(el_exact / el_sur, 3), "chem_substep": sub, "train_states": n_states,
+           "train_steps": train_steps, "train_s": round(t_train, 1), "T_max_mid_block": round(float(np.max(T)), 1),
+           "finite": finite}
+    log("bench: mixing layer " + json.dumps(out))
+    S.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -235,7 +350,11 @@ def main():
     ap.add_argument("--no-reactive", action="store_true")
     ap.add_argument("--reactive-steps", type=int, default=50, help="timed steps of the reacting box (50 as specified)")
     ap.add_argument("--no-surrogate", action="store_true")
-    ap.add_argument("--only", choices=("reactive", "surrogate"),
+    ap.add_argument("--no-mixing", action="store_true", help="skip the multiblock reacting mixing layer")
+    ap.add_argument("--mixing-blocks", type=int, default=8)
+    ap.add_argument("--mixing-shape", type=int, nargs=3, default=(384, 192, 96))
+    ap.add_argument("--mixing-steps", type=int, default=3)
+    ap.add_argument("--only", choices=("reactive", "surrogate", "mixing"),
                     help="run one secondary section alone and print its JSON object (profiling runs)")
     ap.add_argument("--train-steps", type=int, default=30000)
     ap.add_argument("--surrogate-states", type=int, default=2000000)
@@ -260,6 +379,10 @@ def main():
 
     if args.only == "reactive":
         print(json.dumps(reactive_bench(steps=args.reactive_steps)), flush=True)
+        return
+    if args.only == "mixing":
+        print(json.dumps(mixing_layer_bench(args.mixing_blocks, tuple(args.mixing_shape), args.mixing_steps)),
+              flush=True)
         return
     if args.only == "surrogate":
         print(json.dumps(surrogate_bench(n_states=args.surrogate_states, train_steps=args.train_steps)), flush=True)
@@ -310,6 +433,8 @@ def main():
     reactive = None if args.no_reactive else reactive_bench(steps=args.reactive_steps)
     surrogate = None if args.no_surrogate else surrogate_bench(n_states=args.surrogate_states,
                                                                train_steps=args.train_steps)
+    mixing = None if args.no_mixing else mixing_layer_bench(args.mixing_blocks, tuple(args.mixing_shape),
+                                                             args.mixing_steps)
     cpu = None
     if not args.no_cpu_baseline:
         log("bench: cpu baseline (oracle)")
@@ -329,6 +454,7 @@ def main():
         "cpu_baseline": cpu,
         "reactive": reactive,
         "surrogate": surrogate,
+        "mixing_layer": mixing,
     }
     print(json.dumps(line), flush=True)
 

@@ -123,6 +123,54 @@ def build_dataset(chem, n_states, seed, y_fresh, dt_lo=1e-9, dt_hi=1e-7, t_min=3
     return dict(rho=rho[ok], T=T[ok], Y=Y[:, ok], dt=dt[ok], dY=dY[:, ok], dT=dT[ok])
 
 
+def build_mixing_dataset(chem, n_states, seed, y_a, y_b, t_a, t_b, dt_lo=1e-9, dt_hi=1e-7, t_max=3200.0,
+                         p_lo=0.5 * PATM, p_hi=2 * PATM, n_reactors=8192, n_snap=48):
+    """n_states (rho, T, Y, dt) with exact labels for a mixing layer of two streams (mass fractions y_a, y_b at
+    temperatures t_a, t_b): half from constant-volume ignition of n_reactors mixtures z y_a + (1 - z) y_b (z
+    uniform) started at the mixed temperature z t_a + (1 - z) t_b plus up to 1200 K, recorded at n_snap
+    geometric times in [1e-7, 3e-4] s; half on the mixing line itself at T between the mixed temperature and
+    2600 K; p in [p_lo, p_hi], dt log-uniform in [dt_lo, dt_hi]."""
+    rng = np.random.default_rng(seed)
+    mw = chem.table("mw")
+    y_a, y_b = np.asarray(y_a, dtype=np.float64), np.asarray(y_b, dtype=np.float64)
+
+    def mixtures(n):
+        z = rng.uniform(0, 1, n)
+        return z * y_a[:, None] + (1 - z) * y_b[:, None], z * t_a + (1 - z) * t_b
+
+    Y0, Tm = mixtures(n_reactors)
+    T0 = Tm + rng.uniform(0, 1200.0, n_reactors)
+    p0 = rng.uniform(p_lo, p_hi, n_reactors)
+    rho0 = p0 / (RU * T0 * mixture_imw(mw, Y0))
+    chem.set_cells(rho0, T0, rho0 * Y0)
+    times = np.concatenate([[0.0], np.geomspace(1e-7, 3e-4, n_snap)])
+    Ts, Ys = [T0.copy()], [Y0.copy()]
+    for j in range(n_snap):
+        chem.integrate(times[j + 1] - times[j])
+        g = chem.get_cells()
+        Ts.append(g["T"].copy())
+        Ys.append(g["rhoY"] / rho0)
+    r_T, r_Y, r_rho = np.concatenate(Ts), np.concatenate(Ys, axis=1), np.tile(rho0, n_snap + 1)
+    n_a = n_states // 2
+    pick = rng.choice(r_T.size, n_a, replace=r_T.size < n_a)
+    Y_a, T_a = r_Y[:, pick], r_T[pick]
+    p_a = r_rho[pick] * RU * mixture_imw(mw, Y_a) * T_a
+    n_b = n_states - n_a
+    Y_b, Tm_b = mixtures(n_b)
+    T_b = Tm_b + rng.uniform(0, 1, n_b) * (2600.0 - Tm_b)
+    p_b = rng.uniform(p_lo, p_hi, n_b)
+    Y = np.concatenate([Y_a, Y_b], axis=1)
+    T = np.concatenate([T_a, T_b])
+    p = np.clip(np.concatenate([p_a, p_b]), p_lo, p_hi)
+    rho = p / (RU * T * mixture_imw(mw, Y))
+    dt = np.exp(rng.uniform(np.log(dt_lo), np.log(dt_hi), T.size))
+    keep = T <= t_max
+    rho, T, Y, dt = rho[keep], T[keep], Y[:, keep], dt[keep]
+    dY, dT, steps = label(chem, rho, T, Y, dt)
+    ok = steps > 0
+    return dict(rho=rho[ok], T=T[ok], Y=Y[:, ok], dt=dt[ok], dY=dY[:, ok], dT=dT[ok])
+
+
 def targets(data):
     """Rates (n, 10): dY_k/dt and dT/dt."""
     return np.column_stack([(data["dY"] / data["dt"]).T, data["dT"] / data["dt"]])

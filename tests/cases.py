@@ -500,3 +500,40 @@ int fluctuation_seed = {seed}
     return dict(setup=setup, grid=(X, Y, Z), scale=1.0, faces=faces, boundary=boundary_text(faces),
                 scheme=(inviscid, recon, visc), cfl=cfl, seed=seed, length=length, delta_omega=delta_omega,
                 intensity=intensity)
+
+
+def mixing_layer_blocks(n_blocks=8, mx=384, my=192, mz=96, length=0.08, height=0.01, sponge_frac=0.1, **kw):
+    """The reacting mixing layer as n_blocks blocks of mx x my x mz nodes in a row along x, consecutive blocks
+    joined by block-to-block faces (the node layer at a join belongs to both).  Inflow at the first block's
+    i = 0, outflow at the last block's exit and on every j face, periodic span; a sponge layer over the last
+    sponge_frac of the length.  Keyword arguments go to mixing_layer (streams, scheme, chemistry)."""
+    nx = n_blocks * (mx - 1) + 1
+    base = mixing_layer(mx=5, my=5, mz=5, length=length, height=height, sponge=False, **kw)  # the setup text
+    setup = base["setup"] + f"""bool sponge_layer = 1
+int spongeX = 2
+real spongeXPlusStart = {(1 - sponge_frac) * length!r}
+real spongeXPlusEnd = {length!r}
+real spongeCoefficient = 1.0
+"""
+    x = np.linspace(0.0, length, nx)
+    y = np.linspace(-0.5 * height, 0.5 * height, my)
+    z = np.linspace(0.0, 0.25 * height, mz)
+    grids, boundary, records = [], "", []
+    for b in range(n_blocks):
+        xs = x[b * (mx - 1): b * (mx - 1) + mx]
+        grids.append(tuple(np.meshgrid(xs, y, z, indexing="ij")))
+        faces = [[0, mx - 1, 0, 0, 0, mz - 1, 6], [0, mx - 1, my - 1, my - 1, 0, mz - 1, 6],
+                 [0, mx - 1, 0, my - 1, 0, 0, 8], [0, mx - 1, 0, my - 1, mz - 1, mz - 1, 8]]
+        if b == 0:
+            faces.insert(0, [0, 0, 0, my - 1, 0, mz - 1, 5])
+        if b == n_blocks - 1:
+            faces.insert(1, [mx - 1, mx - 1, 0, my - 1, 0, mz - 1, 6])
+        boundary += boundary_text(faces)
+        rec = []
+        if b > 0:
+            rec.append(((1, 1, -1, -my, 1, mz, b + 1), (mx, mx, -1, -my, 1, mz, b)))
+        if b < n_blocks - 1:
+            rec.append(((mx, mx, -1, -my, 1, mz, b + 1), (1, 1, -1, -my, 1, mz, b + 2)))
+        records.append(rec)
+    return dict(base, setup=setup, grids=grids, boundary=boundary, inner=inner_text(records), n_blocks=n_blocks,
+                block_shape=(mx, my, mz), cells=n_blocks * mx * my * mz)
