@@ -72,13 +72,14 @@ __device__ __forceinline__ double int_pow(double x, int k) {
 }
 
 // Chemical source omega [kg/(m^3 s)] of one cell (FiniteRateChem.cu:1466-1605).  With JAC, also the
-// Jacobian d omega / d(rho Y) with the rates of progress frozen (ConstVolumeReactor.cpp:1168-1187).
+// Jacobian d omega / d(rho Y) with the rates of progress frozen (ConstVolumeReactor.cpp:1168-1187); columns of
+// species with rho Y at or below floor are zero (1e-30 in the reactor, 1e-25 in Wu's EPI, StrangSplitting.cu:164).
 // Stoichiometry is walked through each reaction's species lists (a few entries) rather than all species;
 // the Jacobian keeps static indices (register-resident) and skips rows and columns through the reaction's
 // species bit masks.
 template<int NS, bool JAC>
 __device__ __forceinline__ void source(const ChemMech &m, double t, const double *rhoY, double *w, double *J,
-                                       double *wd = nullptr) {
+                                       double *wd = nullptr, double floor = 1e-30) {
   double c[NS], g[NS], cr[NS], de[NS];
 #pragma unroll
   for (int l = 0; l < NS; ++l) {
@@ -179,10 +180,79 @@ __device__ __forceinline__ void source(const ChemMech &m, double t, const double
   if constexpr (JAC) {
 #pragma unroll
     for (int n = 0; n < NS; ++n) {
-      const double inv = rhoY[n] > 1e-30 ? 1.0 / rhoY[n] : 0.0;
+      const double inv = rhoY[n] > floor ? 1.0 / rhoY[n] : 0.0;
 #pragma unroll
       for (int a = 0; a < NS; ++a) J[a * NS + n] = J[a * NS + n] * inv * (1e+3 * m.mw[a]);
     }
+  }
+}
+
+// Doolittle LU with partial pivoting, row-major (ConstVolumeReactor.cpp:1231-1262), and its solve.
+template<int NS> __device__ __forceinline__ void lu(double *a, int *piv) {
+#pragma unroll
+  for (int k = 0; k < NS; ++k) {
+    // first row of largest |a[r][k]|; the running maximum keeps every array index static, so the matrix
+    // stays in registers
+    int ik = k;
+    double amax = 0;
+#pragma unroll
+    for (int r = k; r < NS; ++r) {
+#pragma unroll
+      for (int t = 0; t < k; ++t) a[r * NS + k] -= a[r * NS + t] * a[t * NS + k];
+      const double v = fabs(a[r * NS + k]);
+      if (r == k) {
+        amax = v;
+      } else if (v > amax) {
+        amax = v;
+        ik = r;
+      }
+    }
+    piv[k] = ik;
+    // row exchange through value selects: every index stays static and no branch is formed (a branch per
+    // candidate row lets the optimiser merge them into stores through a chosen address, which sends the
+    // whole matrix to scratch memory)
+#pragma unroll
+    for (int r = k + 1; r < NS; ++r) {
+      const bool sw = r == ik;
+#pragma unroll
+      for (int t = 0; t < NS; ++t) {
+        const double x = a[r * NS + t], y = a[k * NS + t];
+        a[r * NS + t] = sw ? y : x;
+        a[k * NS + t] = sw ? x : y;
+      }
+    }
+#pragma unroll
+    for (int p = k + 1; p < NS; ++p)
+#pragma unroll
+      for (int t = 0; t < k; ++t) a[k * NS + p] -= a[k * NS + t] * a[t * NS + p];
+    const double inv = 1.0 / a[k * NS + k];
+#pragma unroll
+    for (int r = k + 1; r < NS; ++r) a[r * NS + k] *= inv;
+    a[k * NS + k] = inv; // the solve multiplies by the reciprocal pivot
+  }
+}
+
+template<int NS> __device__ __forceinline__ void lu_solve(const double *a, const int *piv, double *b) {
+#pragma unroll
+  for (int r = 0; r < NS; ++r) {
+#pragma unroll
+    for (int q = r + 1; q < NS; ++q) {
+      const bool sw = piv[r] == q;
+      const double x = b[q], y = b[r];
+      b[q] = sw ? y : x;
+      b[r] = sw ? x : y;
+    }
+  }
+#pragma unroll
+  for (int r = 1; r < NS; ++r)
+#pragma unroll
+    for (int t = 0; t < r; ++t) b[r] -= a[r * NS + t] * b[t];
+  b[NS - 1] *= a[NS * NS - 1];
+#pragma unroll
+  for (int r = NS - 2; r >= 0; --r) {
+#pragma unroll
+    for (int t = r + 1; t < NS; ++t) b[r] -= a[r * NS + t] * b[t];
+    b[r] *= a[r * NS + r];
   }
 }
 

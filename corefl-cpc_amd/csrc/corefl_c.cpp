@@ -224,7 +224,7 @@ struct corefl_ctx {
   // mixtures
   int ns = 0;
   bool strang = false;          // temporal_scheme 5: chemistry dt/2, SSP-RK3, chemistry dt/2
-  int wu_method = -1;           // temporal_scheme 4 (Wu's splitting): chem_advance_method 0 or 3
+  int wu_method = -1;           // temporal_scheme 4 (Wu's splitting): chem_advance_method 0, 1 or 3
   corefl_chem *chem = nullptr;  // mechanism, device tables and the cell batch of the chemistry sub-steps
   corefl_surrogate *surrogate = nullptr; // not owned; when set, sub-steps use it with the exact fallback
   double chem_rtol = 1e-6, chem_atol = 1e-8;
@@ -722,15 +722,22 @@ void build(corefl_ctx *ctx, std::vector<corefl::HostBlock> blocks) {
       throw std::runtime_error("reacting flow: temporal_scheme 3 (explicit source), 4 (Wu's splitting) or 5 (Strang "
                                "splitting)");
     if (reaction && temporal == 4) {
-      // wu_zeroDReaction (StrangSplitting.cu:43-225): 0 forward Euler, 3 SSP-RK3 stages; 1 (EPI) and 2 (HCST,
-      // hydrogen only) are not on this path
+      // wu_zeroDReaction (StrangSplitting.cu:43-225): 0 forward Euler, 1 EPI, 3 SSP-RK3 stages; 2 (HCST) has an
+      // empty branch for generic mechanisms (StrangSplitting.cu:155-156), so it advances as 0.  The hard-coded
+      // hydrogen mechanisms (hardCodedMech > 0, StrangSplitting.cu:226-371, 479-481) are not on this path.
       const int cam = P.get_int("chem_advance_method");
-      if (cam != 0 && cam != 3)
-        throw std::runtime_error("with temporal_scheme 4, chem_advance_method must be 0 (explicit) or 3 (SSP-RK3)");
-      ctx->wu_method = cam;
+      if (cam < 0 || cam > 3)
+        throw std::runtime_error("with temporal_scheme 4, chem_advance_method must be 0 (explicit), 1 (EPI), 2 (HCST) "
+                                 "or 3 (SSP-RK3)");
+      if (P.get_int("hardCodedMech") != 0)
+        throw std::runtime_error("hardCodedMech > 0 (hard-coded hydrogen mechanisms) is not supported; use the "
+                                 "mechanism file");
+      ctx->wu_method = cam == 2 ? 0 : cam;
     }
-    if (reaction && temporal == 3 && P.get_int("chemSrcMethod") != 0)
-      throw std::runtime_error("only chemSrcMethod = 0 (explicit source) is on this path with temporal_scheme 3");
+    // chemSrcMethod 1 and 2 make finite_rate_chemistry also store the source Jacobian (FiniteRateChem.cu:1450-1462),
+    // which only the implicit solvers read; SSP-RK3 adds the source explicitly whatever its value (RK.cuh:145-156)
+    if (reaction && temporal == 3 && (P.get_int("chemSrcMethod") < 0 || P.get_int("chemSrcMethod") > 2))
+      throw std::runtime_error("chemSrcMethod must be 0 (explicit), 1 (EPI) or 2 (diagonal approximation)");
     if (!reaction && temporal != 3) throw std::runtime_error("temporal_scheme must be 3 (SSP-RK3)");
     ctx->strang = reaction && temporal == 5;
     P.set_int("n_spec", ctx->ns);

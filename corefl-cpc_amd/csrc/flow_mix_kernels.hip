@@ -1185,10 +1185,11 @@ hipError_t launch_mix_inviscid_flux(const DevBlock &b, const MixArrays &x, const
 // per interior cell, n_sub = dt / tau steps of dt / (n_sub + 1) and a last one to dt, tau the cell's reaction
 // time scale from its previous sub-step (0.2 dt at the start); each advances the conservative state by the
 // chemical source of its start state and the frozen transport residual cn (the dq array), by forward Euler
-// (method 0) or by three SSP-RK3 stages with that same source (method 3), then clips and renormalises the
-// species and solves for T.  n_sub is capped at 10^6.
-__global__ __launch_bounds__(256) void k_mix_wu_reaction(DevBlock b, MixArrays x, const double *__restrict__ dtp,
-                                                         int method) {
+// (METHOD 0), by forward Euler of the EPI source (I - h J)^-1 omega with J the Jacobian of omega in rho Y at
+// frozen rates of progress (METHOD 1, StrangSplitting.cu:157-181), or by three SSP-RK3 stages with that same
+// source (METHOD 3), then clips and renormalises the species and solves for T.  n_sub is capped at 10^6.
+template<int METHOD>
+__global__ __launch_bounds__(256) void k_mix_wu_reaction(DevBlock b, MixArrays x, const double *__restrict__ dtp) {
   const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
   if (i >= b.mx || j >= b.my) return;
   const long id = nid(b, i, j, k), sz = b.sz;
@@ -1212,8 +1213,20 @@ __global__ __launch_bounds__(256) void k_mix_wu_reaction(DevBlock b, MixArrays x
     const double h = n < n_sub ? h0 : dt - h0 * n_sub;
     if (n == n_sub && !(fabs(h) > 1e-12)) break;
     double w[NS], wd[NS];
-    chemdev::source<NS, false>(m, t, q + 5, w, nullptr, wd);
-    if (method == 3) {
+    if constexpr (METHOD == 1) {
+      double a[NS * NS];
+      int piv[NS];
+      chemdev::source<NS, true>(m, t, q + 5, w, a, wd, 1e-25);
+#pragma unroll
+      for (int r = 0; r < NS; ++r)
+#pragma unroll
+        for (int c = 0; c < NS; ++c) a[r * NS + c] = (r == c ? 1.0 : 0.0) - h * a[r * NS + c];
+      chemdev::lu<NS>(a, piv);
+      chemdev::lu_solve<NS>(a, piv, w);
+    } else {
+      chemdev::source<NS, false>(m, t, q + 5, w, nullptr, wd);
+    }
+    if constexpr (METHOD == 3) {
       double qn[NV];
 #pragma unroll
       for (int l = 0; l < NV; ++l) qn[l] = q[l];
@@ -1283,7 +1296,9 @@ hipError_t launch_mix_viscous(const DevBlock &b, const MixArrays &x, const Schem
 hipError_t launch_mix_wu_reaction(const DevBlock &b, const MixArrays &x, const double *dt_dev, int method,
                                   hipStream_t s) {
   if (x.ns != NS) return hipErrorNotSupported;
-  k_mix_wu_reaction<<<node_grid(b), dim3(64, 4, 1), 0, s>>>(b, x, dt_dev, method);
+  if (method == 3) k_mix_wu_reaction<3><<<node_grid(b), dim3(64, 4, 1), 0, s>>>(b, x, dt_dev);
+  else if (method == 1) k_mix_wu_reaction<1><<<node_grid(b), dim3(64, 4, 1), 0, s>>>(b, x, dt_dev);
+  else k_mix_wu_reaction<0><<<node_grid(b), dim3(64, 4, 1), 0, s>>>(b, x, dt_dev);
   return hipGetLastError();
 }
 

@@ -160,6 +160,10 @@ void orc_chem_source(const orc_mech *m, double t, const double *rhoY, double *q1
                      double *omega);
 /* d omega / d(rho Y) (ConstVolumeReactor.cpp:1168-1187). */
 void orc_chem_jacobian(const orc_mech *m, const double *rhoY, const double *q1, const double *q2, double *jac);
+/* Doolittle LU with partial pivoting, row-major, and its solve (ConstVolumeReactor.cpp:1231-1285,
+ * FiniteRateChem.cu:1769-1822). */
+void orc_lu_decomp(double *a, int n, int *piv);
+void orc_lu_solve(const double *a, double *b, int n, const int *piv);
 /* T from internal energy by Newton (ConstVolumeReactor.cpp:970-995). */
 double orc_chem_update_t(const orc_mech *m, double t0, const double *yk, double E);
 /* Adaptive ROK4E at constant rho and E (ConstVolumeReactor.cpp:173-497); -1 if max_attempts attempts

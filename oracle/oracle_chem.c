@@ -174,7 +174,7 @@ double orc_chem_update_t(const orc_mech *m, double t0, const double *yk, double 
 }
 
 /* Doolittle LU with partial pivoting and the matching solve (ConstVolumeReactor.cpp:1231-1285). */
-static void lu_decomp(double *a, int n, int *piv) {
+void orc_lu_decomp(double *a, int n, int *piv) {
   for (int k = 0; k < n; ++k) {
     int ik = k;
     for (int r = k; r < n; ++r) {
@@ -194,7 +194,7 @@ static void lu_decomp(double *a, int n, int *piv) {
   }
 }
 
-static void lu_solve(const double *a, double *b, int n, const int *piv) {
+void orc_lu_solve(const double *a, double *b, int n, const int *piv) {
   for (int r = 0; r < n; ++r)
     if (piv[r] != r) {
       const double s = b[piv[r]];
@@ -255,9 +255,9 @@ int orc_chem_rok4e(const orc_mech *m, double rho, double E, double *rhoY, double
     orc_chem_jacobian(m, rhoY, q1, q2, jac);
     for (int a = 0; a < ns; ++a)
       for (int n = 0; n < ns; ++n) lhs[a * ns + n] = (a == n ? 1.0 : 0.0) - hs * G * jac[a * ns + n];
-    lu_decomp(lhs, ns, piv);
+    orc_lu_decomp(lhs, ns, piv);
     for (int l = 0; l < ns; ++l) k1[l] = w[l];
-    lu_solve(lhs, k1, ns, piv);
+    orc_lu_solve(lhs, k1, ns, piv);
     for (int l = 0; l < ns; ++l) {
       ys[l] = rhoY[l] + hs * A21 * k1[l];
       yk[l] = ys[l] / rho;
@@ -265,7 +265,7 @@ int orc_chem_rok4e(const orc_mech *m, double rho, double E, double *rhoY, double
     double Ts = orc_chem_update_t(m, *T, yk, E);
     orc_chem_source(m, Ts, ys, q1, q2, wd, w);
     for (int l = 0; l < ns; ++l) k2[l] = w[l] + G21 * k1[l];
-    lu_solve(lhs, k2, ns, piv);
+    orc_lu_solve(lhs, k2, ns, piv);
     for (int l = 0; l < ns; ++l) k2[l] -= G21 * k1[l];
     for (int l = 0; l < ns; ++l) {
       ys[l] = rhoY[l] + hs * (A31 * k1[l] + A32 * k2[l]);
@@ -274,10 +274,10 @@ int orc_chem_rok4e(const orc_mech *m, double rho, double E, double *rhoY, double
     Ts = orc_chem_update_t(m, Ts, yk, E);
     orc_chem_source(m, Ts, ys, q1, q2, wd, w);
     for (int l = 0; l < ns; ++l) k3[l] = w[l] + G31 * k1[l] + G32 * k2[l];
-    lu_solve(lhs, k3, ns, piv);
+    orc_lu_solve(lhs, k3, ns, piv);
     for (int l = 0; l < ns; ++l) k3[l] -= G31 * k1[l] + G32 * k2[l];
     for (int l = 0; l < ns; ++l) k4[l] = w[l] + G41 * k1[l] + G42 * k2[l] + G43 * k3[l];
-    lu_solve(lhs, k4, ns, piv);
+    orc_lu_solve(lhs, k4, ns, piv);
     for (int l = 0; l < ns; ++l) k4[l] -= G41 * k1[l] + G42 * k2[l] + G43 * k3[l];
     double err = 0;
     for (int l = 0; l < ns; ++l) {

@@ -1514,18 +1514,37 @@ static void wu_node_state(orc_mix *b, long id) {
 }
 
 /* wu_zeroDReaction (StrangSplitting.cu:43-225) over h at node id: the chemical source at the current state
- * and the frozen transport cn advance cv by forward Euler (chem_advance_method 0) or by the three SSP-RK3
- * stages with that same source (method 3, whose rates are not re-evaluated between stages); returns the
- * reaction time scale min_k rho Y_k / omega_d,k. */
+ * and the frozen transport cn advance cv by forward Euler (chem_advance_method 0), by forward Euler of the EPI
+ * source (I - h J)^-1 omega (method 1, StrangSplitting.cu:157-181: J_mn = 1e3 MW_m / (rho Y_n)
+ * sum_r (nu''_rm - nu'_rm)(nu'_rn q1_r - nu''_rn q2_r), rho Y_n recovered from the concentrations, columns with
+ * rho Y_n <= 1e-25 zero) or by the three SSP-RK3 stages with that same source (method 3, whose rates are not
+ * re-evaluated between stages); returns the reaction time scale min_k rho Y_k / omega_d,k. */
 static double wu_zero_d(orc_mix *b, long id, const double *cn, double h, int method) {
   const long s = b->sz;
   const int ns = b->ns, nv = b->nv;
   double *cv = b->cv;
   const double iJ = 1.0 / b->jac[id];
-  double qn[NV], ry[MS], q1[ORC_MAX_REAC], q2[ORC_MAX_REAC], wd[MS], w[MS];
+  double qn[NV], ry[MS] = {0}, q1[ORC_MAX_REAC], q2[ORC_MAX_REAC], wd[MS], w[MS];
   for (int l = 0; l < nv; ++l) qn[l] = cv[id + l * s];
   for (int l = 0; l < ns; ++l) ry[l] = cv[id + (5 + l) * s];
   orc_chem_source(&b->m, b->bv[id + 5 * s], ry, q1, q2, wd, w);
+  if (method == 1) {
+    const orc_mech *m = &b->m;
+    double a[MS * MS];
+    int piv[MS];
+    for (int r = 0; r < ns; ++r)
+      for (int c = 0; c < ns; ++c) {
+        const double rhoY = ry[c] * (1.0 / m->mw[c]) * 1e-3 * 1e+3 / (1.0 / m->mw[c]);
+        double zz = 0;
+        if (rhoY > 1e-25) {
+          for (int q = 0; q < m->nr; ++q) zz += (m->nb[q][r] - m->nf[q][r]) * (m->nf[q][c] * q1[q] - m->nb[q][c] * q2[q]);
+          zz /= rhoY;
+        }
+        a[r * ns + c] = (r == c ? 1.0 : 0.0) - h * (zz * 1e+3 * m->mw[r]);
+      }
+    orc_lu_decomp(a, ns, piv);
+    orc_lu_solve(a, w, ns, piv);
+  }
   double ts = 1e+6;
   if (method == 3) {
     for (int st = 0; st < 3; ++st) {

@@ -67,8 +67,12 @@ def test_rk3_step_non_reacting(recon, inv):
     assert np.max(err) < 1e-12
 
 
-def test_explicit_source_step():
+@pytest.mark.parametrize("src_method", [0, 1, 2])
+def test_explicit_source_step(src_method):
+    """SSP-RK3 with the chemical source added at every stage; chemSrcMethod 1 and 2 only make the reference store
+    a source Jacobian for its implicit solvers, so the step is the explicit one for all three."""
     case = _case(4, 2, temporal=3, reaction=1)
+    case["setup"] += f"int chemSrcMethod = {src_method}\n"
     S, B = make_mix_pair(case)
     B.set_explicit_source(True)
     shape, g = (N, N, N), S.get_int("ngg")
@@ -97,10 +101,11 @@ def test_strang_step_matches_oracle():
     assert np.max(cv[5 + 7] / cv[0]) > 1e-4
 
 
-@pytest.mark.parametrize("method,inv", [(0, 2), (3, 2), (0, 72)])
+@pytest.mark.parametrize("method,inv", [(0, 2), (1, 2), (2, 2), (3, 2), (0, 72), (1, 72)])
 def test_wu_splitting_step_matches_oracle(method, inv):
-    """Wu's splitting (temporal_scheme 4, chem_advance_method 0 or 3) against the oracle over two steps, with
-    slow hot kernels (the scheme's explicit chemistry sub-steps are only stable for time scales above them)."""
+    """Wu's splitting (temporal_scheme 4, chem_advance_method 0 explicit, 1 EPI, 2 HCST, which is explicit for a
+    mechanism read from file, 3 SSP-RK3) against the oracle over two steps, with slow hot kernels (the scheme's
+    explicit chemistry sub-steps are only stable for time scales above them)."""
     case = _case(4, inv, temporal=4, reaction=1)
     if method:
         case["setup"] += f"int chem_advance_method = {method}\n"

@@ -327,3 +327,49 @@ def test_positivity_limiter_acts_on_species_fluxes_only(inviscid):
     d = np.abs(res[1] - res[0])
     assert np.max(d[:5]) == 0.0  # flow fields untouched
     assert np.max(d[5:]) > 0.0  # the limiter acted on some species flux
+
+
+def _linear_network(kf1, kb1, k2):
+    """A <=> B (rates kf1, kb1 from explicit REV parameters), B -> C (k2): first-order kinetics between three
+    species of one thermo (constant cp/R = 3.5, air's molar mass), so the temperature is fixed by the internal
+    energy and the source is linear, d(rho Y)/dt = K rho Y."""
+    d = O.constant_gamma_mech(3)
+    d.update(nr=2, type=np.array([1, 0], np.int32), rev=np.array([1, 0], np.int32), A=np.array([kf1, k2]),
+             b=np.zeros(2), Ea=np.zeros(2), A2=np.array([kb1, 0.0]), b2=np.zeros(2), Ea2=np.zeros(2),
+             order=np.zeros(2), troe=np.zeros((2, 4)), nf=np.array([[1.0, 0, 0], [0, 1.0, 0]]),
+             nb=np.array([[0, 1.0, 0], [0, 0, 1.0]]), tb=np.zeros((2, 3)))
+    K = np.array([[-kf1, kb1, 0], [kf1, -kb1 - k2, 0], [0, k2, 0]])
+    return O.mech_struct(d), K
+
+
+def _uniform_network_block(m, n=7):
+    X, Y, Z = cases.wavy_grid(n, amp=0.0)
+    blk = O.MixBlock(params(2, 4), m, X, Y, Z, 3, cases.box_faces(n, n, n, 8))
+    bv = np.zeros((6, blk.sz))
+    bv[1], bv[2], bv[3], bv[4], bv[5] = 30.0, -10.0, 5.0, 101325.0, 900.0
+    bv[0] = 101325.0 / (RU / 28.9647 * 900.0)
+    Y0 = np.array([0.6, 0.3, 0.1])
+    blk.set_state(bv.ravel(), np.repeat(Y0[:, None], blk.sz, axis=1).ravel())
+    return blk, bv[0, 0] * Y0
+
+
+@pytest.mark.parametrize("method", [0, 1, 2])
+def test_wu_linear_network_closed_form(method):
+    """On a uniform periodic box the transport residual vanishes and the Wu step is the cell's reaction sub-steps
+    alone: n_sub = int(dt / 0.2 dt) steps of dt / (n_sub + 1) and one to dt (StrangSplitting.cu:373-421).  For
+    linear kinetics an EPI sub-step (chem_advance_method 1) is backward Euler, rho Y <- (I - h K)^-1 rho Y, since
+    the frozen-rate Jacobian is K itself; explicit (0, and 2, whose branch is empty for generic mechanisms) is
+    rho Y <- (I + h K) rho Y.  A transposed or sign-flipped Jacobian, or a wrong h, breaks the first form."""
+    m, K = _linear_network(4.0e4, 1.5e4, 2.5e4)
+    blk, ry0 = _uniform_network_block(m)
+    dt = 5e-5  # h K up to about 1.3: explicit and implicit differ at O(1)
+    blk.wu_step(dt, method)
+    n_sub = int(dt / (0.2 * dt))
+    h0 = dt / (n_sub + 1)
+    want = ry0.copy()
+    for h in [h0] * n_sub + [dt - h0 * n_sub]:
+        want = np.linalg.solve(np.eye(3) - h * K, want) if method == 1 else want + h * K @ want
+    got = _unique_interior(blk, 0, 8, 7).reshape(8, -1)[5:]
+    np.testing.assert_allclose(got, np.repeat(want[:, None], got.shape[1], axis=1), rtol=1e-12, atol=1e-14 * ry0.sum())
+    T = _unique_interior(blk, 1, 6, 7).reshape(6, -1)[5]
+    np.testing.assert_allclose(T, 900.0, rtol=1e-13)
