@@ -126,6 +126,19 @@ void orc_block_rhs(orc_block *b);
 double orc_block_cfl_dt(orc_block *b);
 /* One SSP-RK3 step with time step dt (RK.cuh:100-192 for one block). */
 void orc_block_rk3_step(orc_block *b, double dt);
+/* Blocks joined by block-to-block faces, paired by coordinates (x, y, z: interior nodes, i fastest).  A link
+ * joins the whole side (dir 0/1/2, side -1 low or +1 high) of block a to block t; NULL when a face node has no
+ * partner at its place. */
+typedef struct orc_link orc_link;
+orc_link *orc_link_find(orc_block **B, int a, const double *xa, const double *ya, const double *za, int dir, int side,
+                        int t, const double *xt, const double *yt, const double *zt);
+void orc_link_destroy(orc_link *L);
+/* The link's direction map and node pairs (padded indices; layer 1 on the coincident layer); returns the count. */
+long orc_link_info(const orc_link *L, int *vmap, int *vsign, long *ia, long *it, char *layer);
+void orc_multi_exchange(orc_block **B, int nb, orc_link **L, int nl);
+void orc_multi_rhs(orc_block **B, int nb, orc_link **L, int nl);
+double orc_multi_cfl_dt(orc_block **B, int nb);
+void orc_multi_rk3_step(orc_block **B, int nb, orc_link **L, int nl, double dt);
 /* Number of OpenMP threads the oracle uses (for the CPU baseline report). */
 int orc_num_threads(void);
 void orc_set_threads(int n); /* OpenMP threads of the following oracle calls */

@@ -522,9 +522,8 @@ double orc_deriv(const double *f, long id, long st, int i, int n, int left, int 
 #undef F
 }
 
-/* compute_viscous_flux_collocated<Air,ORDER> (ViscousScheme.cu:309-408), periodic exchange, and
- * compute_viscous_flux_derivative<ORDER> (ViscousScheme.cu:568-612). */
-static void viscous(orc_block *b) {
+/* compute_viscous_flux_collocated<Air,ORDER> (ViscousScheme.cu:309-408): stress and heat-flux node fluxes */
+static void viscous_nodes(orc_block *b) {
   const int order = b->p.viscous_order;
   const long s = b->sz;
   const double cp = kGamma * (kRu / kMwAir) / (kGamma - 1);
@@ -580,7 +579,13 @@ static void viscous(orc_block *b) {
         b->gv[id + 4 * s] = ex * Ex + ey * Ey + ez * Ez;
         b->hv[id + 4 * s] = zx * Ex + zy * Ey + zz * Ez;
       }
-  exchange_viscous_flux(b);
+}
+
+/* compute_viscous_flux_derivative<ORDER> (ViscousScheme.cu:568-612): d(fv)/dxi + d(gv)/deta + d(hv)/dzeta of
+ * the node fluxes, ghosts included, into dq */
+static void viscous_diff(orc_block *b) {
+  const int order = b->p.viscous_order;
+  const long s = b->sz;
 #pragma omp parallel for collapse(2)
   for (int k = 0; k < b->mz; ++k)
     for (int j = 0; j < b->my; ++j)
@@ -594,6 +599,14 @@ static void viscous(orc_block *b) {
           b->dq[iq + l * b->n] += d * J;
         }
       }
+}
+
+/* compute_viscous_flux with collocated node fluxes (ViscousScheme.cu:7-73) on one block: node fluxes, the
+ * periodic exchange of their ghosts (exchange_value task 1, DataCommunication.cu:408-456), differences */
+static void viscous(orc_block *b) {
+  viscous_nodes(b);
+  exchange_viscous_flux(b);
+  viscous_diff(b);
 }
 
 /* Second-order face-centred viscous fluxes, compute_fv/gv/hv_2nd_order<Air> (ViscousScheme.cu:614-1010):
@@ -872,5 +885,246 @@ void orc_block_rk3_step(orc_block *b, double dt) {
     update(b, dt, rk);
     apply_bcs(b);
     update_properties(b);
+  }
+}
+
+/* ---- Several blocks joined by block-to-block faces ----
+ * data_communication / inner_communication (DataCommunication.cu:6-103) and exchange_value task 1
+ * (DataCommunication.cu:408-495) restated with a pairing of nodes found from the coordinates alone: every node of
+ * block a's face layer is matched with the node of block t at the same place, the ghost layers beyond it with t's
+ * nodes walked away from a, and each index direction of a with the direction of t along the same grid line.  The
+ * reference derives the same pairs from the inner file's signed index ranges (InnerFace::register_boundary,
+ * Mesh.cpp:38-170); matching by position checks that index algebra instead of repeating it.  A link covers the
+ * whole side of block a (no corner ghosts, as the reference leaves them out). */
+struct orc_link {
+  int a, t;
+  long n;
+  long *ia, *it; /* padded node index in a, in t */
+  char *layer;   /* 1 on the coincident layer */
+  int vmap[3], vsign[3];
+};
+
+static double dist3(const double *x, const double *y, const double *z, long p, double px, double py, double pz) {
+  const double dx = x[p] - px, dy = y[p] - py, dz = z[p] - pz;
+  return sqrt(dx * dx + dy * dy + dz * dz);
+}
+
+orc_link *orc_link_find(orc_block **B, int a, const double *xa, const double *ya, const double *za, int dir, int side,
+                        int t, const double *xt, const double *yt, const double *zt) {
+  const orc_block *A = B[a], *T = B[t];
+  const int ea[3] = {A->mx, A->my, A->mz}, et[3] = {T->mx, T->my, T->mz};
+#define PA(i, j, k) ((long)(i) + (long)A->mx * ((j) + (long)A->my * (k)))
+#define PT(i, j, k) ((long)(i) + (long)T->mx * ((j) + (long)T->my * (k)))
+  const int u = dir == 0 ? 1 : 0, v = dir == 2 ? 1 : 2; /* the two in-face directions of a */
+  const int layer = side > 0 ? ea[dir] - 1 : 0;
+  const long nface = (long)ea[u] * ea[v];
+  orc_link *L = calloc(1, sizeof(orc_link));
+  L->a = a;
+  L->t = t;
+  L->ia = malloc(sizeof(long) * nface * (A->g + 1));
+  L->it = malloc(sizeof(long) * nface * (A->g + 1));
+  L->layer = malloc((size_t)nface * (A->g + 1));
+  for (long f = 0; f < nface; ++f) {
+    int n[3];
+    n[dir] = layer;
+    n[u] = (int)(f % ea[u]);
+    n[v] = (int)(f / ea[u]);
+    const long p = PA(n[0], n[1], n[2]);
+    int m[3] = {n[0], n[1], n[2]};
+    m[dir] -= side; /* a's interior neighbour: the spacing across the face */
+    const long pin = PA(m[0], m[1], m[2]);
+    const double h = dist3(xa, ya, za, pin, xa[p], ya[p], za[p]);
+    long best = -1;
+    double bd = 1e300;
+    for (long q = 0; q < (long)T->mx * T->my * T->mz; ++q) {
+      const double d = dist3(xt, yt, zt, q, xa[p], ya[p], za[p]);
+      if (d < bd) { bd = d; best = q; }
+    }
+    if (bd > 1e-6 * h) { orc_link_destroy(L); return NULL; }
+    const int Q[3] = {(int)(best % T->mx), (int)((best / T->mx) % T->my), (int)(best / ((long)T->mx * T->my))};
+    /* the direction of t that continues a's grid line across the face: its first node lies nearest the
+     * reflection of a's interior neighbour through the face node */
+    const double rx = 2 * xa[p] - xa[pin], ry = 2 * ya[p] - ya[pin], rz = 2 * za[p] - za[pin];
+    int oc = -1, os = 0;
+    double od = 1e300;
+    for (int c = 0; c < 3; ++c)
+      for (int sg = -1; sg <= 1; sg += 2) {
+        int r[3] = {Q[0], Q[1], Q[2]};
+        r[c] += sg;
+        if (r[c] < 0 || r[c] >= et[c]) continue;
+        const double d = dist3(xt, yt, zt, PT(r[0], r[1], r[2]), rx, ry, rz);
+        if (d < od) { od = d; oc = c; os = sg; }
+      }
+    if (oc < 0 || od > 0.5 * h) { orc_link_destroy(L); return NULL; }
+    for (int g = 0; g <= A->g; ++g) {
+      int r[3] = {Q[0], Q[1], Q[2]}, w[3] = {n[0], n[1], n[2]};
+      r[oc] += g * os;
+      w[dir] += g * side;
+      if (r[oc] < 0 || r[oc] >= et[oc]) { orc_link_destroy(L); return NULL; }
+      L->ia[L->n] = ID(A, w[0], w[1], w[2]);
+      L->it[L->n] = ID(T, r[0], r[1], r[2]);
+      L->layer[L->n] = g == 0;
+      ++L->n;
+    }
+    if (f == nface / 2) { /* the direction map, from a node in the middle of the face */
+      L->vmap[dir] = oc;
+      L->vsign[dir] = os * side;
+      for (int d = 0; d < 3; ++d) {
+        if (d == dir) continue;
+        if (A->dim == 2 && d == 2) { L->vmap[2] = 2; L->vsign[2] = 1; continue; }
+        int e[3] = {n[0], n[1], n[2]};
+        const int up = n[d] + 1 < ea[d];
+        e[d] += up ? 1 : -1;
+        const long pe = PA(e[0], e[1], e[2]);
+        const double sd = up ? 1.0 : -1.0;
+        const double Dx = sd * (xa[pe] - xa[p]), Dy = sd * (ya[pe] - ya[p]), Dz = sd * (za[pe] - za[p]);
+        double bc = -2;
+        for (int c = 0; c < 3; ++c)
+          for (int sg = -1; sg <= 1; sg += 2) {
+            int r[3] = {Q[0], Q[1], Q[2]};
+            r[c] += sg;
+            if (c == oc || r[c] < 0 || r[c] >= et[c]) continue;
+            const long pr = PT(r[0], r[1], r[2]);
+            const double Ex = xt[pr] - xt[best], Ey = yt[pr] - yt[best], Ez = zt[pr] - zt[best];
+            const double cs = (Dx * Ex + Dy * Ey + Dz * Ez) /
+                              (sqrt(Dx * Dx + Dy * Dy + Dz * Dz) * sqrt(Ex * Ex + Ey * Ey + Ez * Ez));
+            if (cs > bc) { bc = cs; L->vmap[d] = c; L->vsign[d] = sg; }
+          }
+      }
+    }
+  }
+#undef PA
+#undef PT
+  return L;
+}
+
+void orc_link_destroy(orc_link *L) {
+  if (!L) return;
+  free(L->ia);
+  free(L->it);
+  free(L->layer);
+  free(L);
+}
+
+long orc_link_info(const orc_link *L, int *vmap, int *vsign, long *ia, long *it, char *layer) {
+  for (int d = 0; d < 3; ++d) {
+    vmap[d] = L->vmap[d];
+    vsign[d] = L->vsign[d];
+  }
+  if (ia) memcpy(ia, L->ia, sizeof(long) * L->n);
+  if (it) memcpy(it, L->it, sizeof(long) * L->n);
+  if (layer) memcpy(layer, L->layer, (size_t)L->n);
+  return L->n;
+}
+
+/* inner_communication<Air> (DataCommunication.cu:36-103): the coincident layer takes the two blocks' average (the
+ * partner's state where a's velocity is nonzero and the partner's is zero; nothing where a's is zero), the ghost
+ * layers the partner's state; cv follows bv on a's nodes. */
+static void link_state(orc_block **B, const orc_link *L) {
+  orc_block *A = B[L->a], *T = B[L->t];
+  const long sa = A->sz, st = T->sz;
+  for (long q = 0; q < L->n; ++q) {
+    const long i = L->ia[q], j = L->it[q];
+    if (L->layer[q]) {
+      const double ua = sqrt(A->bv[i + sa] * A->bv[i + sa] + A->bv[i + 2 * sa] * A->bv[i + 2 * sa] +
+                             A->bv[i + 3 * sa] * A->bv[i + 3 * sa]);
+      if (ua > 1e-15) {
+        const double ut = sqrt(T->bv[j + st] * T->bv[j + st] + T->bv[j + 2 * st] * T->bv[j + 2 * st] +
+                               T->bv[j + 3 * st] * T->bv[j + 3 * st]);
+        for (int l = 0; l < 6; ++l) {
+          if (ut < 1e-15) {
+            A->bv[i + l * sa] = T->bv[j + l * st];
+          } else {
+            const double m = 0.5 * (T->bv[j + l * st] + A->bv[i + l * sa]);
+            A->bv[i + l * sa] = m;
+            T->bv[j + l * st] = m;
+          }
+        }
+      }
+    } else {
+      for (int l = 0; l < 6; ++l) A->bv[i + l * sa] = T->bv[j + l * st];
+    }
+    cv_from_bv(A, i);
+  }
+}
+
+/* exchange_value task 1 (DataCommunication.cu:458-495) for the collocated viscous fluxes, each of a's index
+ * directions taking the partner's flux along the matching direction with its sign (the same arrays when the two
+ * blocks' directions agree, as in the reference). */
+static void link_viscous(orc_block **B, const orc_link *L) {
+  orc_block *A = B[L->a], *T = B[L->t];
+  double *FA[3] = {A->fv, A->gv, A->hv}, *FT[3] = {T->fv, T->gv, T->hv};
+  for (long q = 0; q < L->n; ++q) {
+    const long i = L->ia[q], j = L->it[q];
+    for (int d = 0; d < 3; ++d) {
+      double *P = FT[L->vmap[d]];
+      const double sg = L->vsign[d];
+      for (int l = 1; l < 5; ++l) {
+        if (L->layer[q]) {
+          const double m = 0.5 * (FA[d][i + l * A->sz] + sg * P[j + l * T->sz]);
+          FA[d][i + l * A->sz] = m;
+          P[j + l * T->sz] = sg * m;
+        } else {
+          FA[d][i + l * A->sz] = sg * P[j + l * T->sz];
+        }
+      }
+    }
+  }
+}
+
+/* data_communication over the links in order, then the physical properties of every block (RK.cuh:174-190) */
+void orc_multi_exchange(orc_block **B, int nb, orc_link **L, int nl) {
+  for (int l = 0; l < nl; ++l) link_state(B, L[l]);
+  for (int q = 0; q < nb; ++q) update_properties(B[q]);
+}
+
+/* The residual of every block: viscous node fluxes of all blocks, their exchange across the links and then the
+ * periodic faces (exchange_value, DataCommunication.cu:408-456), differences; inviscid fluxes (RK.cuh:137-157). */
+void orc_multi_rhs(orc_block **B, int nb, orc_link **L, int nl) {
+  int collocated = 0;
+  for (int q = 0; q < nb; ++q) {
+    orc_block *b = B[q];
+    memset(b->dq, 0, sizeof(double) * 5 * b->n);
+    if (b->p.viscous_order == 2) viscous2(b);
+    else if (b->p.viscous_order > 0) {
+      viscous_nodes(b);
+      collocated = 1;
+    }
+  }
+  if (collocated) {
+    for (int l = 0; l < nl; ++l) link_viscous(B, L[l]);
+    for (int q = 0; q < nb; ++q) exchange_viscous_flux(B[q]);
+    for (int q = 0; q < nb; ++q) viscous_diff(B[q]);
+  }
+  for (int q = 0; q < nb; ++q) {
+    orc_block *b = B[q];
+    if (b->p.inviscid_scheme == 2) entropy_fix(b);
+    inviscid_dir(b, 0);
+    inviscid_dir(b, 1);
+    if (b->dim == 3) inviscid_dir(b, 2);
+    sponge_source(b);
+  }
+}
+
+double orc_multi_cfl_dt(orc_block **B, int nb) {
+  double dt = 1e+6;
+  for (int q = 0; q < nb; ++q) dt = fmin(dt, orc_block_cfl_dt(B[q]));
+  return dt;
+}
+
+/* One SSP-RK3 step of all blocks (RK.cuh:100-192): per stage every block's update and boundary conditions, then
+ * the block-to-block exchange and the properties. */
+void orc_multi_rk3_step(orc_block **B, int nb, orc_link **L, int nl, double dt) {
+  for (int q = 0; q < nb; ++q) {
+    sponge_mean(B[q]);
+    memcpy(B[q]->qn, B[q]->cv, sizeof(double) * 5 * B[q]->sz);
+  }
+  for (int rk = 0; rk < 3; ++rk) {
+    orc_multi_rhs(B, nb, L, nl);
+    for (int q = 0; q < nb; ++q) {
+      update(B[q], dt, rk);
+      apply_bcs(B[q]);
+    }
+    orc_multi_exchange(B, nb, L, nl);
   }
 }

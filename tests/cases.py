@@ -217,8 +217,11 @@ struct outflow {
     else:
         raise ValueError(layout)
     inner = inner_text([[(face1 + (1,), face2 + (2,))], [(face2 + (2,), face1 + (1,))]])
+    # the joined sides as (block, direction, side, partner block), for the oracle's links
+    d2 = next(d for d in range(3) if face2[2 * d] == face2[2 * d + 1])
+    links = [(0, 0, 1, 1), (1, d2, -1 if face2[2 * d2] == 1 else 1, 0)]
     multi = dict(single, grids=[g1, g2], boundary=boundary_text(f1) + boundary_text(f2), inner=inner,
-                 split=split)
+                 split=split, faces=[f1, f2], links=links)
     return single, multi, place
 
 

@@ -56,6 +56,16 @@ def load():
     lib.orc_block_cfl_dt.restype = D
     lib.orc_block_cfl_dt.argtypes = [P]
     lib.orc_block_rk3_step.argtypes = [P, D]
+    lib.orc_link_find.restype = P
+    lib.orc_link_find.argtypes = [P, I, P, P, P, I, I, I, P, P, P]
+    lib.orc_link_destroy.argtypes = [P]
+    lib.orc_link_info.restype = ctypes.c_long
+    lib.orc_link_info.argtypes = [P, P, P, P, P, P]
+    for name in ("orc_multi_exchange", "orc_multi_rhs"):
+        getattr(lib, name).argtypes = [P, I, P, I]
+    lib.orc_multi_cfl_dt.restype = D
+    lib.orc_multi_cfl_dt.argtypes = [P, I]
+    lib.orc_multi_rk3_step.argtypes = [P, I, P, I, D]
     lib.orc_num_threads.restype = I
     lib.orc_set_threads.argtypes = [I]
     lib.orc_exact_riemann.restype = D
@@ -146,6 +156,8 @@ class Block:
         self.ngg = ngg
         self.sz = (self.mx + 2 * ngg) * (self.my + 2 * ngg) * (self.mz + 2 * ngg)
         self.jac, self.metric = metrics(x, y, z, ngg)
+        # interior coordinates, i fastest (the layout block-to-block links match nodes in)
+        self.coords = [np.ascontiguousarray(np.transpose(np.asarray(c, float), (2, 1, 0))).ravel() for c in (x, y, z)]
         f = np.ascontiguousarray(np.asarray(faces, dtype=np.int32).ravel())
         self._p = p
         self._h = load().orc_block_create(ctypes.byref(p), self.mx, self.my, self.mz, ngg, _p(f), len(faces),
@@ -182,6 +194,52 @@ class Block:
 
     def step(self, dt):
         load().orc_block_rk3_step(self._h, dt)
+
+
+class MultiBlock:
+    """Oracle blocks joined by block-to-block faces.  links: (a, dir, side, t) rows, block a's whole side dir
+    (0 i, 1 j, 2 k) at side -1 (low) or +1 (high) joined to block t; the node pairs and the direction map come
+    from the coordinates (oracle_block.c, orc_link_find).  Each block's state is set through its own Block
+    (set_bv); exchange() then fills the ghosts across the links."""
+
+    def __init__(self, blocks, links):
+        lib = load()
+        self.blocks = blocks
+        self._arr = (ctypes.c_void_p * len(blocks))(*[b._h for b in blocks])
+        hs = []
+        for a, d, side, t in links:
+            A, T = blocks[a], blocks[t]
+            h = lib.orc_link_find(self._arr, a, *[_p(c) for c in A.coords], d, side, t, *[_p(c) for c in T.coords])
+            if not h:
+                raise ValueError(f"no partner by position for a node of block {a}'s side {d}/{side}")
+            hs.append(h)
+        self._links = (ctypes.c_void_p * max(1, len(hs)))(*hs)
+        self._hs = hs
+
+    def __del__(self):
+        for h in getattr(self, "_hs", []):
+            load().orc_link_destroy(h)
+        self._hs = []
+
+    def link_info(self, q):
+        vmap, vsign = (ctypes.c_int * 3)(), (ctypes.c_int * 3)()
+        n = load().orc_link_info(self._hs[q], vmap, vsign, None, None, None)
+        ia, it, layer = np.zeros(n, np.int64), np.zeros(n, np.int64), np.zeros(n, np.int8)
+        load().orc_link_info(self._hs[q], vmap, vsign, _p(ia), _p(it), _p(layer))
+        return list(vmap), list(vsign), ia, it, layer
+
+    def exchange(self):
+        load().orc_multi_exchange(self._arr, len(self.blocks), self._links, len(self._hs))
+
+    def rhs(self):
+        load().orc_multi_rhs(self._arr, len(self.blocks), self._links, len(self._hs))
+        return [b.get(2) for b in self.blocks]
+
+    def cfl_dt(self):
+        return load().orc_multi_cfl_dt(self._arr, len(self.blocks))
+
+    def step(self, dt):
+        load().orc_multi_rk3_step(self._arr, len(self.blocks), self._links, len(self._hs), dt)
 
 
 def exact_riemann(left, right, gamma, x0, t, x):
