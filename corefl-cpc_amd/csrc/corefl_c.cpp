@@ -1002,16 +1002,20 @@ int fail(const std::exception &e) {
 }
 
 // Boundary and block-to-block faces, ghost grid (mirrored, then the neighbours' nodes across block-to-block
-// faces: Mesh.cpp:1013-1083), metrics.
+// faces: Mesh.cpp:1013-1083), metrics (second-order, or symmetric-conservative with symmetric_conservative_metric).
 std::vector<corefl::HostBlock> prepare_blocks(corefl::Parameter &P, std::vector<corefl::HostBlock> blocks,
                                               const std::string &boundary_text, const std::string &inner_text) {
   corefl::parse_boundaries(boundary_text, blocks, P.get_int("periodic_label"));
   if (!inner_text.empty()) corefl::parse_inner_faces(inner_text, blocks);
   for (auto &b : blocks) corefl::mirror_ghost_grid(b);
   corefl::fill_inner_ghost_grid(blocks);
+  const int scm = P.get_int("symmetric_conservative_metric");
   for (auto &b : blocks) {
     std::string err;
+    // the standard metrics check the interior Jacobian; the symmetric-conservative ones then replace them
+    // (compute_jac_metric, Mesh.cu:136-141)
     if (!corefl::compute_metrics(b, err, false)) throw std::runtime_error(err);
+    if (scm != 0 && !corefl::sc_metrics_device(b, scm, err)) throw std::runtime_error(err);
   }
   return blocks;
 }

@@ -69,5 +69,8 @@ void fill_inner_ghost_grid(std::vector<HostBlock> &blocks);
 // Metrics and Jacobian (mirror = true builds the mirrored ghost grid first); returns false on a
 // non-positive interior Jacobian.
 bool compute_metrics(HostBlock &b, std::string &err, bool mirror = true);
+// Symmetric-conservative metrics of order 2, 4 or 6 (Mesh.cu:10-250) on the device from the block's ghost grid,
+// overwriting jac and metric; false (with err) on a non-positive Jacobian at any node, a 2D block or a HIP error.
+bool sc_metrics_device(HostBlock &b, int order, std::string &err);
 
 } // namespace corefl

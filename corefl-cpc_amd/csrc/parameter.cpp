@@ -31,6 +31,7 @@ Parameter::Parameter() {
   ints["chemSrcMethod"] = 0;
   ints["chem_advance_method"] = 0; // Wu's splitting reaction sub-steps (Parameter.cpp:431)
   ints["hardCodedMech"] = 0;        // hard-coded hydrogen mechanisms (Parameter.cpp:399)
+  ints["symmetric_conservative_metric"] = 0; // 2, 4, 6: symmetric-conservative metrics (Parameter.cpp:351)
   strings["mechanism_file"] = "chemistry/mechanism.inp";
   strings["therm_file"] = "chemistry/therm.dat";
   strings["transport_file"] = "chemistry/tran.dat"; // Parameter.cpp:401

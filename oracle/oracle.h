@@ -139,6 +139,10 @@ void orc_multi_exchange(orc_block **B, int nb, orc_link **L, int nl);
 void orc_multi_rhs(orc_block **B, int nb, orc_link **L, int nl);
 double orc_multi_cfl_dt(orc_block **B, int nb);
 void orc_multi_rk3_step(orc_block **B, int nb, orc_link **L, int nl, double dt);
+/* Symmetric-conservative metrics of order 2, 4 or 6 (Mesh.cu:10-250) on a 3D block, same layout and ghost grid as
+ * orc_metrics; returns -1 on a non-positive Jacobian, -2 for a 2D block. */
+int orc_sc_metrics(int mx, int my, int mz, int ngg, const double *x, const double *y, const double *z, int order,
+                   double *jac, double *metric);
 /* Number of OpenMP threads the oracle uses (for the CPU baseline report). */
 int orc_num_threads(void);
 void orc_set_threads(int n); /* OpenMP threads of the following oracle calls */

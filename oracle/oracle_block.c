@@ -218,6 +218,127 @@ int orc_metrics(int mx, int my, int mz, int ngg, const double *x, const double *
   return rc;
 }
 
+/* ---------------- symmetric-conservative metrics (Mesh.cu:10-141, 136-250) ---------------- */
+
+/* First derivative at index i of the degree-p Lagrange interpolant through p + 1 consecutive nodes, centred where
+ * they fit in [-ng, n - 1 + ng] and shifted to one side near its ends (ggxl::Derivatives, gxl_lib/Math.cuh:6-101).
+ * f(l) reads the node l along the direction. */
+static double lag_deriv(const double *f0, long stride, int i, int n, int ng, int p) {
+  int s = i - p / 2;
+  if (i <= -ng + p / 2) s = -ng;
+  else if (i >= n - 1 + ng - p / 2) s = n - 1 + ng - p;
+  double D = 0;
+  for (int l = s; l <= s + p; ++l) {
+    double den = 1, num = 0;
+    for (int a = s; a <= s + p; ++a)
+      if (a != l) den *= (double)(l - a);
+    for (int m = s; m <= s + p; ++m) {
+      if (m == l) continue;
+      double prod = 1;
+      for (int a = s; a <= s + p; ++a)
+        if (a != l && a != m) prod *= (double)(i - a);
+      num += prod;
+    }
+    D += num / den * f0[(long)(l - i) * stride];
+  }
+  return D;
+}
+
+/* symmetric_conservative_metric = order (2, 4, 6) on a 3D block: the 18 products x_a y - y_a x of coordinates and
+ * their order-`order` derivatives (compute_xyz123), the three combinations D123 (compute_D123), then the Jacobian
+ * (sum of the D123 derivatives) / 6 and each metric as half a difference of product derivatives over it, both with
+ * sixth-order derivatives (Mesh.cu:215-243).  Every node of the ngg-ghost layout, on the same mirrored ghost grid as
+ * orc_metrics; returns -1 if a Jacobian is not positive. */
+int orc_sc_metrics(int mx, int my, int mz, int ngg, const double *x, const double *y, const double *z, int order,
+                   double *jac, double *metric) {
+  if (mz == 1) return -2;
+  arr3 X, Y, Z;
+  arr3_init(&X, mx, my, mz, ngg + 1);
+  arr3_init(&Y, mx, my, mz, ngg + 1);
+  arr3_init(&Z, mx, my, mz, ngg + 1);
+  for (int k = 0; k < mz; ++k)
+    for (int j = 0; j < my; ++j)
+      for (int i = 0; i < mx; ++i) {
+        const long s = ((long)k * my + j) * mx + i;
+        *A3(&X, i, j, k) = x[s];
+        *A3(&Y, i, j, k) = y[s];
+        *A3(&Z, i, j, k) = z[s];
+      }
+  ghost_grid(&X, &Y, &Z, ngg, 3);
+  arr3 P[18], D[3];
+  for (int q = 0; q < 18; ++q) arr3_init(&P[q], mx, my, mz, ngg);
+  for (int q = 0; q < 3; ++q) arr3_init(&D[q], mx, my, mz, ngg);
+  const int ext[3] = {mx, my, mz};
+  /* derivative of array a along direction d (0 xi, 1 eta, 2 zeta) at (i, j, k), order p */
+#define DER(a, d, p)                                                                                                 \
+  lag_deriv(A3((a), i, j, k), (d) == 0 ? 1 : ((d) == 1 ? (a)->d2 : (a)->d1), (d) == 0 ? i : ((d) == 1 ? j : k),        \
+            ext[d], ngg, (p))
+#pragma omp parallel for collapse(2)
+  for (int k = -ngg; k < mz + ngg; ++k)
+    for (int j = -ngg; j < my + ngg; ++j)
+      for (int i = -ngg; i < mx + ngg; ++i) {
+        const double xv = *A3(&X, i, j, k), yv = *A3(&Y, i, j, k), zv = *A3(&Z, i, j, k);
+        double dx[3], dy[3], dz[3];
+        for (int d = 0; d < 3; ++d) {
+          /* the coordinate arrays carry ngg + 1 layers; the stencils use ngg of them, as the reference does */
+          dx[d] = DER(&X, d, order);
+          dy[d] = DER(&Y, d, order);
+          dz[d] = DER(&Z, d, order);
+        }
+        const double v[18] = {dx[0] * yv - dy[0] * xv, dy[1] * xv - dx[1] * yv, dy[0] * zv - dz[0] * yv,
+                              dz[1] * yv - dy[1] * zv, dz[0] * xv - dx[0] * zv, dx[1] * zv - dz[1] * xv,
+                              dx[2] * yv - dy[2] * xv, dy[0] * xv - dx[0] * yv, dy[2] * zv - dz[2] * yv,
+                              dz[0] * yv - dy[0] * zv, dz[2] * xv - dx[2] * zv, dx[0] * zv - dz[0] * xv,
+                              dx[1] * yv - dy[1] * xv, dy[2] * xv - dx[2] * yv, dy[1] * zv - dz[1] * yv,
+                              dz[2] * yv - dy[2] * zv, dz[1] * xv - dx[1] * zv, dx[2] * zv - dz[2] * xv};
+        for (int q = 0; q < 18; ++q) *A3(&P[q], i, j, k) = v[q];
+      }
+#pragma omp parallel for collapse(2)
+  for (int k = -ngg; k < mz + ngg; ++k)
+    for (int j = -ngg; j < my + ngg; ++j)
+      for (int i = -ngg; i < mx + ngg; ++i) {
+        const double xv = *A3(&X, i, j, k), yv = *A3(&Y, i, j, k), zv = *A3(&Z, i, j, k);
+        /* D123 components 2, 1, 0 of compute_D123 */
+        *A3(&D[2], i, j, k) = (DER(&P[0], 1, order) + DER(&P[1], 0, order)) * zv +
+                              (DER(&P[2], 1, order) + DER(&P[3], 0, order)) * xv +
+                              (DER(&P[4], 1, order) + DER(&P[5], 0, order)) * yv;
+        *A3(&D[1], i, j, k) = (DER(&P[6], 0, order) + DER(&P[7], 2, order)) * zv +
+                              (DER(&P[8], 0, order) + DER(&P[9], 2, order)) * xv +
+                              (DER(&P[10], 0, order) + DER(&P[11], 2, order)) * yv;
+        *A3(&D[0], i, j, k) = (DER(&P[12], 2, order) + DER(&P[13], 1, order)) * zv +
+                              (DER(&P[14], 2, order) + DER(&P[15], 1, order)) * xv +
+                              (DER(&P[16], 2, order) + DER(&P[17], 1, order)) * yv;
+      }
+  const long d2 = mx + 2 * ngg, d1 = (long)(my + 2 * ngg) * d2, sz = d1 * (mz + 2 * ngg);
+  int rc = 0;
+#pragma omp parallel for collapse(2) reduction(min : rc)
+  for (int k = -ngg; k < mz + ngg; ++k)
+    for (int j = -ngg; j < my + ngg; ++j)
+      for (int i = -ngg; i < mx + ngg; ++i) {
+        const double J = (DER(&D[2], 2, 6) + DER(&D[1], 1, 6) + DER(&D[0], 0, 6)) / 6.0;
+        if (!(J > 0)) rc = -1;
+        const long id = (long)(k + ngg) * d1 + (long)(j + ngg) * d2 + (i + ngg);
+        jac[id] = J;
+        const double h = 0.5 / J;
+        metric[id + 0 * sz] = (DER(&P[14], 2, 6) - DER(&P[8], 1, 6)) * h;
+        metric[id + 1 * sz] = (DER(&P[16], 2, 6) - DER(&P[10], 1, 6)) * h;
+        metric[id + 2 * sz] = (DER(&P[12], 2, 6) - DER(&P[6], 1, 6)) * h;
+        metric[id + 3 * sz] = (DER(&P[8], 0, 6) - DER(&P[2], 2, 6)) * h;
+        metric[id + 4 * sz] = (DER(&P[10], 0, 6) - DER(&P[4], 2, 6)) * h;
+        metric[id + 5 * sz] = (DER(&P[6], 0, 6) - DER(&P[0], 2, 6)) * h;
+        metric[id + 6 * sz] = (DER(&P[2], 1, 6) - DER(&P[14], 0, 6)) * h;
+        metric[id + 7 * sz] = (DER(&P[4], 1, 6) - DER(&P[16], 0, 6)) * h;
+        metric[id + 8 * sz] = (DER(&P[0], 1, 6) - DER(&P[12], 0, 6)) * h;
+      }
+#undef DER
+  for (int q = 0; q < 18; ++q) free(P[q].v);
+  for (int q = 0; q < 3; ++q) free(D[q].v);
+  free(X.v);
+  free(Y.v);
+  free(Z.v);
+  return rc;
+}
+
 /* ---------------- block state and step ---------------- */
 
 typedef struct {

@@ -46,6 +46,8 @@ def load():
     lib.orc_sutherland.argtypes = [D]
     lib.orc_metrics.restype = I
     lib.orc_metrics.argtypes = [I, I, I, I, P, P, P, P, P]
+    lib.orc_sc_metrics.restype = I
+    lib.orc_sc_metrics.argtypes = [I, I, I, I, P, P, P, I, P, P]
     lib.orc_block_create.restype = P
     lib.orc_block_create.argtypes = [ctypes.POINTER(OrcParams), I, I, I, I, P, I, P, P]
     lib.orc_block_destroy.argtypes = [P]
@@ -148,14 +150,28 @@ def metrics(x, y, z, ngg):
     return jac, met
 
 
+def sc_metrics(x, y, z, ngg, order):
+    """Symmetric-conservative metrics of the given order (orc_sc_metrics): jac (sz,) and metric (9*sz,)."""
+    mx, my, mz = x.shape
+    fx, fy, fz = (np.ascontiguousarray(np.asarray(a, float).ravel(order="F")) for a in (x, y, z))
+    sz = (mx + 2 * ngg) * (my + 2 * ngg) * (mz + 2 * ngg)
+    jac, met = np.zeros(sz), np.zeros(9 * sz)
+    rc = load().orc_sc_metrics(mx, my, mz, ngg, _p(fx), _p(fy), _p(fz), order, _p(jac), _p(met))
+    if rc != 0:
+        raise ValueError("non-positive Jacobian" if rc == -1 else "2D block")
+    return jac, met
+
+
 class Block:
     """One oracle block: metrics from the raw grid, faces as in boundary%4d.txt."""
 
-    def __init__(self, p, x, y, z, ngg, faces):
+    def __init__(self, p, x, y, z, ngg, faces, jac_metric=None):
         self.mx, self.my, self.mz = x.shape
         self.ngg = ngg
         self.sz = (self.mx + 2 * ngg) * (self.my + 2 * ngg) * (self.mz + 2 * ngg)
-        self.jac, self.metric = metrics(x, y, z, ngg)
+        # the second-order metrics of the grid unless given (as (jac, metric) in the same layout)
+        self.jac, self.metric = metrics(x, y, z, ngg) if jac_metric is None else (
+            np.ascontiguousarray(jac_metric[0], float), np.ascontiguousarray(jac_metric[1], float))
         # interior coordinates, i fastest (the layout block-to-block links match nodes in)
         self.coords = [np.ascontiguousarray(np.transpose(np.asarray(c, float), (2, 1, 0))).ravel() for c in (x, y, z)]
         f = np.ascontiguousarray(np.asarray(faces, dtype=np.int32).ravel())
