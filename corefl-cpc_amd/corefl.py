@@ -20,6 +20,7 @@ SYMBOLS = (
     "corefl_block_view_get", "corefl_get_array", "corefl_set_state", "corefl_rhs", "corefl_set_dt",
     "corefl_compute_dt",
     "corefl_rk3_step", "corefl_advance", "corefl_residual", "corefl_limit_flow", "corefl_get_time", "corefl_synchronize", "corefl_get_int",
+    "corefl_write_flowfield", "corefl_read_flowfield",
     "corefl_profile_kernels", "corefl_get_real", "corefl_chemistry", "corefl_set_chemistry_surrogate",
     "corefl_chem_substep",
     "corefl_chem_create", "corefl_chem_create_mem", "corefl_chem_destroy", "corefl_chem_info", "corefl_chem_name",
@@ -79,6 +80,8 @@ def load_library(path=LIB_PATH):
     lib.corefl_limit_flow.argtypes = [P, P]
     lib.corefl_get_time.argtypes = [P, ctypes.POINTER(D)]
     lib.corefl_synchronize.argtypes = [P]
+    lib.corefl_write_flowfield.argtypes = [P, ctypes.c_char_p]
+    lib.corefl_read_flowfield.argtypes = [P, ctypes.c_char_p]
     lib.corefl_get_int.argtypes = [P, ctypes.c_char_p, ctypes.POINTER(I)]
     lib.corefl_get_real.argtypes = [P, ctypes.c_char_p, ctypes.POINTER(D)]
     lib.corefl_profile_kernels.argtypes = [P, I, ctypes.POINTER(D), I]
@@ -243,6 +246,14 @@ class Solver:
 
     def sync(self):
         _check(self._lib.corefl_synchronize(self._ctx))
+
+    def write_flowfield(self, path):
+        """Write the state and time as the reference's flowfield.plt (Tecplot binary, FieldIO.h:89-430, 709-825)."""
+        _check(self._lib.corefl_write_flowfield(self._ctx, os.fsencode(path)))
+
+    def read_flowfield(self, path):
+        """Continue from a flowfield.plt of the same blocks (read_flowfield, Initialize.cu:97-345)."""
+        _check(self._lib.corefl_read_flowfield(self._ctx, os.fsencode(path)))
 
     KERNELS = ("viscous_flux", "viscous_exchange", "flux_x", "flux_y", "flux_z", "update", "boundary", "cfl_dt")
 

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cctype>
 #include <cmath>
 #include <cstring>
 #include <fstream>
@@ -188,6 +189,7 @@ struct Block {
   double *last = nullptr;          // rho, |V|, p, T at the start of a monitored step (4 * n)
   unsigned char *lflag = nullptr;  // limit_flow flags (n)
   long res_offset = 0;             // this block's residual partials
+  std::vector<double> coords[3];   // x, y, z of the block's nodes (i fastest), for flowfield output
 };
 
 } // namespace
@@ -988,12 +990,257 @@ void build(corefl_ctx *ctx, std::vector<corefl::HostBlock> blocks) {
   ctx->host_dt = P.get_real("dt");
   HIPCHK(hipMemcpyAsync(ctx->dt_dev, &ctx->host_dt, sizeof(double), hipMemcpyHostToDevice, ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
-  // free the host copies of the coordinates and metrics
+  // keep the block nodes' coordinates for flowfield output; free the padded host copies
   for (auto &B : ctx->blocks) {
+    const auto &H = B.host;
+    const long G = H.ngg + 1, p2 = H.mx + 2 * G, p1 = (H.my + 2 * G) * p2;
+    for (int c = 0; c < 3; ++c) {
+      const auto &src = c == 0 ? H.x : (c == 1 ? H.y : H.z);
+      auto &dst = B.coords[c];
+      dst.resize((size_t)H.mx * H.my * H.mz);
+      size_t q = 0;
+      for (int k = 0; k < H.mz; ++k)
+        for (int j = 0; j < H.my; ++j)
+          for (int i = 0; i < H.mx; ++i) dst[q++] = src[(k + G) * p1 + (j + G) * p2 + (i + G)];
+    }
     std::vector<double>().swap(B.host.x);
     std::vector<double>().swap(B.host.y);
     std::vector<double>().swap(B.host.z);
   }
+}
+
+// ---------------------------------------------------------------- flowfield files
+// The reference's restart file, output/flowfield.plt: a Tecplot binary (#!TDV112) file of one ordered zone per
+// block with every variable in double precision at the nodes (FieldIO.h:89-430, 709-825).  Variables: x, y, z,
+// density, u, v, w, pressure, temperature, the species mass fractions under their mechanism names, mach
+// (Parameter.cpp:750-858 for a laminar run); no ghost layers (the reference writes with ngg_output = 0).
+struct PltWriter {
+  std::ofstream f;
+  void i32(int32_t v) { f.write(reinterpret_cast<const char *>(&v), 4); }
+  void f32(float v) { f.write(reinterpret_cast<const char *>(&v), 4); }
+  void f64(double v) { f.write(reinterpret_cast<const char *>(&v), 8); }
+  // a string as one int32 per character and a terminating 0 (gxl::write_str, gxl_lib/MyString.cpp:151-163)
+  void str(const std::string &t) {
+    for (char c : t) i32(static_cast<int32_t>(c));
+    i32(0);
+  }
+};
+
+struct PltReader {
+  std::ifstream f;
+  std::string path;
+  template<typename T> T get() {
+    T v{};
+    f.read(reinterpret_cast<char *>(&v), sizeof(T));
+    if (!f) throw std::runtime_error("flowfield file " + path + " ends early");
+    return v;
+  }
+  std::string str() {
+    std::string t;
+    for (int32_t c = get<int32_t>(); c != 0; c = get<int32_t>()) t.push_back(static_cast<char>(c));
+    return t;
+  }
+  void skip(long long n) { f.seekg(n, std::ios::cur); }
+};
+
+std::vector<std::string> flowfield_names(const corefl_ctx *ctx) {
+  std::vector<std::string> v{"x", "y", "z", "density", "u", "v", "w", "pressure", "temperature"};
+  for (int l = 0; l < ctx->ns; ++l) v.emplace_back(ctx->chem->mech.species[l]);
+  v.emplace_back("mach");
+  return v;
+}
+
+std::string upper(std::string t) {
+  for (auto &c : t) c = static_cast<char>(std::toupper(static_cast<unsigned char>(c)));
+  return t;
+}
+
+void write_flowfield(corefl_ctx *ctx, const std::string &path) {
+  ctx->flush_exchange(ctx->stream);
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  double time = 0;
+  HIPCHK(hipMemcpy(&time, ctx->time_dev, sizeof(double), hipMemcpyDeviceToHost));
+  const auto names = flowfield_names(ctx);
+  const int nv = static_cast<int>(names.size()), ns = ctx->ns;
+  PltWriter w;
+  w.f.open(path, std::ios::binary | std::ios::trunc);
+  if (!w.f) throw std::runtime_error("cannot write " + path);
+  // header (FieldIO.h:108-218)
+  w.f.write("#!TDV112", 8);
+  w.i32(1);
+  w.i32(0);
+  w.str("Solution file");
+  w.i32(nv);
+  for (const auto &n : names) w.str(n);
+  for (size_t b = 0; b < ctx->blocks.size(); ++b) {
+    const auto &H = ctx->blocks[b].host;
+    w.f32(299.0f);
+    w.str("zone " + std::to_string(b));
+    w.i32(-1);
+    w.i32(-2);
+    w.f64(time);
+    w.i32(-1);
+    w.i32(0);
+    w.i32(0);
+    w.i32(0);
+    w.i32(0);
+    w.i32(H.mx);
+    w.i32(H.my);
+    w.i32(H.mz);
+    w.i32(0);
+  }
+  w.f32(357.0f);
+  // data section (FieldIO.h:243-430)
+  for (auto &B : ctx->blocks) {
+    const auto &H = B.host;
+    const long sz = B.dev.sz, n = (long)H.mx * H.my * H.mz;
+    std::vector<double> bv(6 * sz), sv(ns * sz), snd(ns ? sz : 0);
+    HIPCHK(hipMemcpy(bv.data(), B.dev.bv, 6 * sz * sizeof(double), hipMemcpyDeviceToHost));
+    if (ns) {
+      HIPCHK(hipMemcpy(sv.data(), B.mix.sv, ns * sz * sizeof(double), hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpy(snd.data(), B.mix.snd, sz * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    std::vector<std::vector<double>> var(nv, std::vector<double>(n));
+    long q = 0;
+    for (int k = 0; k < H.mz; ++k)
+      for (int j = 0; j < H.my; ++j)
+        for (int i = 0; i < H.mx; ++i, ++q) {
+          const long id = H.idx(i, j, k);
+          for (int c = 0; c < 3; ++c) var[c][q] = B.coords[c][q];
+          for (int l = 0; l < 6; ++l) var[3 + l][q] = bv[l * sz + id];
+          for (int l = 0; l < ns; ++l) var[9 + l][q] = sv[l * sz + id];
+          const double u = bv[sz + id], v = bv[2 * sz + id], ww = bv[3 * sz + id];
+          // speed of sound: the mixture's, or the perfect gas's (update_physical_properties, FieldOperation.cuh)
+          const double c = ns ? snd[id] : std::sqrt(1.4 * 8314.462618 / 28.9647 * bv[5 * sz + id]);
+          var[nv - 1][q] = std::sqrt(u * u + v * v + ww * ww) / c;
+        }
+    w.f32(299.0f);
+    for (int l = 0; l < nv; ++l) w.i32(2);
+    w.i32(0);
+    w.i32(0);
+    w.i32(-1);
+    for (int l = 0; l < nv; ++l) {
+      const auto mm = std::minmax_element(var[l].begin(), var[l].end());
+      w.f64(*mm.first);
+      w.f64(*mm.second);
+    }
+    for (int l = 0; l < nv; ++l) w.f.write(reinterpret_cast<const char *>(var[l].data()), n * sizeof(double));
+  }
+  if (!w.f) throw std::runtime_error("writing " + path + " failed");
+}
+
+// read_flowfield (Initialize.cu:97-345) for a file whose zones are the current blocks, in order or permuted
+// (read_flowfield_with_same_block): density, u, v, w, pressure, temperature and the species found by name
+// (identify_variable_labels, Initialize.cu:346-405) replace the current state, with the file's ghost layers where
+// it has them; variables not found keep their current values; then cv, boundary conditions and block-to-block
+// exchanges follow as after any state upload, and the time is the file's solution time.
+void read_flowfield(corefl_ctx *ctx, const std::string &path) {
+  PltReader r;
+  r.path = path;
+  r.f.open(path, std::ios::binary);
+  if (!r.f) throw std::runtime_error("cannot open flowfield file " + path);
+  char magic[8];
+  r.f.read(magic, 8);
+  if (!r.f || std::string(magic, 8) != "#!TDV112") throw std::runtime_error(path + " is not a #!TDV112 file");
+  r.skip(8);
+  r.str();
+  const int nv = r.get<int32_t>();
+  if (nv < 3 || nv > 10000) throw std::runtime_error(path + ": bad variable count");
+  std::vector<std::string> names(nv);
+  for (auto &n : names) n = r.str();
+  // labels: 0..5 basic variables, 6 + l species l, -1 ignored
+  std::vector<int> label(nv, -1);
+  for (int v = 3; v < nv; ++v) {
+    const std::string n = upper(names[v]);
+    if (n == "DENSITY" || n == "ROE" || n == "RHO") label[v] = 0;
+    else if (n == "U") label[v] = 1;
+    else if (n == "V") label[v] = 2;
+    else if (n == "W") label[v] = 3;
+    else if (n == "P" || n == "PRESSURE") label[v] = 4;
+    else if (n == "T" || n == "TEMPERATURE") label[v] = 5;
+    else
+      for (int l = 0; l < ctx->ns; ++l)
+        if (n == upper(ctx->chem->mech.species[l])) label[v] = 6 + l;
+  }
+  std::vector<int> zx, zy, zz;
+  double time = 0;
+  float marker = r.get<float>();
+  while (std::fabs(marker - 299.0f) < 1e-3f) {
+    r.str();
+    r.skip(8);
+    time = r.get<double>();
+    r.skip(20);
+    zx.push_back(r.get<int32_t>());
+    zy.push_back(r.get<int32_t>());
+    zz.push_back(r.get<int32_t>());
+    r.skip(4);
+    marker = r.get<float>();
+  }
+  if (std::fabs(marker - 357.0f) > 1e-3f) throw std::runtime_error(path + ": no end-of-header marker");
+  const int nz = static_cast<int>(zx.size()), nb = static_cast<int>(ctx->blocks.size());
+  if (nz != nb) throw std::runtime_error(path + ": " + std::to_string(nz) + " zones for " + std::to_string(nb) +
+                                         " blocks (reading onto a different mesh is not supported)");
+  // zone z of the file feeds block order[z]: the same block where the sizes match in order, else the unique block
+  // with the zone's node counts (the reference's same-block search)
+  auto fits = [&](int z, int b) {
+    const auto &H = ctx->blocks[b].host;
+    const int g = (zx[z] - H.mx) / 2;
+    return g >= 0 && g <= H.ngg && zx[z] == H.mx + 2 * g && zy[z] == H.my + 2 * g && zz[z] == H.mz + 2 * g;
+  };
+  std::vector<int> order(nz, -1);
+  bool same = true;
+  for (int z = 0; z < nz && same; ++z) same = fits(z, z);
+  if (same) {
+    for (int z = 0; z < nz; ++z) order[z] = z;
+  } else {
+    for (int b = 0; b < nb; ++b) {
+      int hit = 0;
+      for (int z = 0; z < nz; ++z)
+        if (fits(z, b)) {
+          ++hit;
+          order[z] = b;
+        }
+      if (hit != 1) throw std::runtime_error(path + ": the zones do not match the blocks one to one");
+    }
+  }
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  const int ns = ctx->ns;
+  for (int z = 0; z < nz; ++z) {
+    auto &B = ctx->blocks[order[z]];
+    const auto &H = B.host;
+    const long sz = B.dev.sz, N = (long)zx[z] * zy[z] * zz[z];
+    const int g = (zx[z] - H.mx) / 2;
+    std::vector<double> bv(6 * sz), sv(ns * sz), buf(N);
+    HIPCHK(hipMemcpy(bv.data(), B.dev.bv, 6 * sz * sizeof(double), hipMemcpyDeviceToHost));
+    if (ns) HIPCHK(hipMemcpy(sv.data(), B.mix.sv, ns * sz * sizeof(double), hipMemcpyDeviceToHost));
+    r.skip(4 + 4LL * nv + 12 + 16LL * nv + 3 * N * 8);
+    for (int v = 3; v < nv; ++v) {
+      if (label[v] < 0) {
+        r.skip(N * 8);
+        continue;
+      }
+      r.f.read(reinterpret_cast<char *>(buf.data()), N * sizeof(double));
+      if (!r.f) throw std::runtime_error("flowfield file " + path + " ends early");
+      double *dst = label[v] < 6 ? bv.data() + label[v] * sz : sv.data() + (label[v] - 6) * sz;
+      long q = 0;
+      for (int k = -g; k < H.mz + g; ++k)
+        for (int j = -g; j < H.my + g; ++j)
+          for (int i = -g; i < H.mx + g; ++i) dst[H.idx(i, j, k)] = buf[q++];
+    }
+    HIPCHK(hipMemcpy(B.dev.bv, bv.data(), 6 * sz * sizeof(double), hipMemcpyHostToDevice));
+    if (ns) {
+      HIPCHK(hipMemcpy(B.mix.sv, sv.data(), ns * sz * sizeof(double), hipMemcpyHostToDevice));
+      HIPCHK(corefl::launch_mix_cv_from_bv_all(B.dev, B.mix, ctx->stream));
+    } else {
+      HIPCHK(corefl::launch_cv_from_bv_all(B.dev, ctx->stream));
+    }
+    ctx->apply_bcs(B, ctx->stream);
+  }
+  ctx->exchange_pending = true;
+  ctx->flush_exchange(ctx->stream);
+  HIPCHK(hipMemcpyAsync(ctx->time_dev, &time, sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  ctx->drop_graph();
 }
 
 int fail(const std::exception &e) {
@@ -1047,6 +1294,9 @@ int corefl_create(const char *input_dir, corefl_ctx **out) {
     if (std::ifstream(dir + "/boundary_condition/inner   0.txt")) inner_text = read_file(dir + "/boundary_condition/inner   0.txt");
     blocks = prepare_blocks(P, std::move(blocks), read_file(dir + "/boundary_condition/boundary   0.txt"), inner_text);
     build(ctx.get(), std::move(blocks));
+    // initial = 1: continue from the run directory's output/flowfield.plt (Initialize.cu:36-41, 97-345); the
+    // input directory sits in the run directory
+    if (P.get_int("initial") == 1) read_flowfield(ctx.get(), dir + "/../output/flowfield.plt");
     *out = ctx.release();
     return 0;
   } catch (const std::exception &e) {
@@ -1312,6 +1562,26 @@ int corefl_get_time(corefl_ctx *ctx, double *t) {
   try {
     HIPCHK(hipStreamSynchronize(ctx->stream));
     HIPCHK(hipMemcpy(t, ctx->time_dev, sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+  } catch (const std::exception &e) {
+    return fail(e);
+  }
+}
+
+int corefl_write_flowfield(corefl_ctx *ctx, const char *path) {
+  try {
+    if (!ctx || !path) throw std::runtime_error("corefl_write_flowfield: null argument");
+    write_flowfield(ctx, path);
+    return 0;
+  } catch (const std::exception &e) {
+    return fail(e);
+  }
+}
+
+int corefl_read_flowfield(corefl_ctx *ctx, const char *path) {
+  try {
+    if (!ctx || !path) throw std::runtime_error("corefl_read_flowfield: null argument");
+    read_flowfield(ctx, path);
     return 0;
   } catch (const std::exception &e) {
     return fail(e);

@@ -48,6 +48,7 @@ Parameter::Parameter() {
   bools["parallel"] = false;
   // residual monitor: the reference requires output_screen in setup.txt; 0 (here the default) turns it off
   ints["output_screen"] = 0;
+  ints["initial"] = 0; // 1: continue from output/flowfield.plt (set in every example setup.txt, read in Initialize.cu:12)
   bools["limit_flow"] = false; // Parameter.cpp:490
   // mixing layer (canonical_problem mixingLayer; MixingLayer.cu:9-193, Parameter.cpp:486-487)
   reals["velocity_ratio"] = -1;

@@ -129,6 +129,19 @@ int corefl_limit_flow(corefl_ctx *ctx, void *stream);
 int corefl_get_time(corefl_ctx *ctx, double *t);
 int corefl_synchronize(corefl_ctx *ctx);
 
+/* The reference's restart file flowfield.plt (Tecplot binary #!TDV112, one ordered zone per block, all variables
+ * double precision at the nodes): replaces FieldIO::initialize + FieldIO::print_field (src/FieldIO.h:89-430,
+ * 709-825; variables x, y, z, density, u, v, w, pressure, temperature, species, mach as
+ * Parameter::get_variable_names gives them for a laminar run, src/Parameter.cpp:750-858; no ghost layers).  Writes
+ * the current state and simulated time to path. */
+int corefl_write_flowfield(corefl_ctx *ctx, const char *path);
+/* Replaces read_flowfield (src/Initialize.cu:97-345): the file's zones must be the context's blocks (in order or
+ * permuted, matched by size); density, u, v, w, pressure, temperature and species found by name replace the
+ * state (with the file's ghost layers where it has them), the time becomes the file's solution time, then cv,
+ * boundary conditions and block-to-block exchanges follow.  With "int initial = 1" in setup.txt, corefl_create
+ * reads <input_dir>/../output/flowfield.plt this way (src/Initialize.cu:36-41). */
+int corefl_read_flowfield(corefl_ctx *ctx, const char *path);
+
 /* Kernel timing on the context's stream with HIP events (no reference counterpart: the reference has
  * no tracing, src/gxl_lib/Time.cpp:4-20).  Runs `reps` launches of each kernel class of one stage on the
  * current state and writes the average milliseconds per launch into ms_out[0..n_out):
