@@ -37,10 +37,18 @@ typedef struct {
   double v_ref;          /* v_inf: scales the component-wise WENO epsilon (WENO.cu:946-949)       */
   int positive_preserving; /* species-flux limiter of the flux-splitting schemes (WENO.cu:991-1087)  */
   double rho_ref;        /* rho_inf: the limiter's threshold scale                                  */
+  double sensor_threshold; /* shockSensor_threshold: > 1e-10 switches 51/52/71/72 to the hybrid scheme,  */
+                           /* linear upwind weights where no node of the stencil exceeds it (WENO.cu:838-849) */
+  int shock_sensor;        /* 0 Ducros, 1 Jameson, 2 density/pressure jump (FieldOperation.cu:158-179)     */
+  double sensor_eps;       /* shockSensor_epsilon of the Ducros sensor                                    */
 } orc_params;
 
 /* ---- point functions (pure; parity pinned by tests/test_oracle_points.py) ---- */
 
+/* compute_shock_sensor (FieldOperation.cu:74-179) at every block node from bv (6*sz) and the metrics (9*sz);
+ * the ghost entries of sensor (sz) are left as they are (the reference never fills them). */
+void orc_shock_sensor(int type, double eps, int mx, int my, int mz, int ngg, const double *bv, const double *met,
+                      double *sensor);
 /* WENO.cu:3059-3114.  Returns fPlus(vp)+fMinus(vm); if_shock=0 gives the linear upwind weights.  */
 double orc_weno5(const double *vp, const double *vm, double eps, int if_shock);
 /* WENO.cu:3116-3222 (Z-type tau7 weights). */
@@ -84,6 +92,9 @@ void orc_interface_flux_pv(const orc_params *p, const double (*pv)[5], const dou
  * sensor is off (shockSensor_threshold = 0, its default), so the nonlinear weights always apply. */
 void orc_fsweno_flux(const orc_params *p, const double (*q)[5], const double (*met)[3], const double *jac, int n,
                      double *fc);
+/* The same with the hybrid scheme's switch: if_shock = 0 reconstructs with the linear upwind weights. */
+void orc_fsweno_flux_hybrid(const orc_params *p, const double (*q)[5], const double (*met)[3], const double *jac,
+                            int n, int if_shock, double *fc);
 
 /* Collocated first derivative of f along a direction of stride st at index i of extent n: order 8 the central
  * stencil of d_dXi<8> (ViscousScheme.cu:129-307) with its one-sided rows where left / right mark a physical

@@ -365,6 +365,7 @@ struct orc_block {
   double *cv, *bv, *qn, *dq, *mul, *jac, *met, *fv, *gv, *hv, *efd;
   double *sponge_sigma, *sponge_mean; /* sponge layer: damping rate (n), running mean of cv (5 * n) */
   unsigned long long sponge_iter;
+  double *sensor; /* shock sensor (sz; ghost entries stay 0) of the hybrid flux-splitting scheme */
 };
 
 #define ID(b, i, j, k) ((long)((k) + (b)->g) * (b)->d1 + (long)((j) + (b)->g) * (b)->d2 + ((i) + (b)->g))
@@ -413,6 +414,7 @@ orc_block *orc_block_create(const orc_params *p, int mx, int my, int mz, int ngg
   b->mul = calloc((size_t)b->sz, sizeof(double));
   b->efd = calloc((size_t)b->sz, sizeof(double));
   b->dq = calloc((size_t)5 * b->n, sizeof(double));
+  b->sensor = calloc((size_t)b->sz, sizeof(double));
   b->jac = malloc((size_t)b->sz * sizeof(double));
   b->met = malloc((size_t)9 * b->sz * sizeof(double));
   memcpy(b->jac, jac, (size_t)b->sz * sizeof(double));
@@ -429,7 +431,7 @@ void orc_block_destroy(orc_block *b) {
   free(b->sponge_sigma);
   free(b->sponge_mean);
   free(b->cv); free(b->qn); free(b->bv); free(b->fv); free(b->gv); free(b->hv);
-  free(b->mul); free(b->efd); free(b->dq); free(b->jac); free(b->met);
+  free(b->mul); free(b->efd); free(b->dq); free(b->jac); free(b->met); free(b->sensor);
   free(b);
 }
 
@@ -861,7 +863,15 @@ static void inviscid_dir(orc_block *b, int dir) {
             for (int c = 0; c < 3; ++c) met[mm][c] = b->met[im + (3 * dir + c) * s];
             jac[mm] = b->jac[im];
           }
-          orc_fsweno_flux(&b->p, (const double(*)[5])q, (const double(*)[3])met, jac, n, Fl + 5 * (t + 1));
+          /* the hybrid scheme's switch (WENO.cu:838-849): nodes t-ngg+1 .. t+ngg of the sensor */
+          int shock = 1;
+          if (b->p.sensor_threshold > 1e-10) {
+            shock = 0;
+            for (int ii = -b->g + 1; ii <= b->g; ++ii)
+              if (b->sensor[id + ii * st] > b->p.sensor_threshold) shock = 1;
+          }
+          orc_fsweno_flux_hybrid(&b->p, (const double(*)[5])q, (const double(*)[3])met, jac, n, shock,
+                                 Fl + 5 * (t + 1));
           continue;
         }
         double ml[3], mr[3];
@@ -892,7 +902,20 @@ static void inviscid_dir(orc_block *b, int dir) {
     }
 }
 
+/* compute_shock_sensor at the start of a step (RK.cuh:112-117), when the hybrid scheme is on */
+static void shock_sensor(orc_block *b) {
+  if (b->p.inviscid_scheme >= 51 && b->p.sensor_threshold > 1e-10)
+    orc_shock_sensor(b->p.shock_sensor, b->p.sensor_eps, b->mx, b->my, b->mz, b->g, b->bv, b->met, b->sensor);
+}
+
+static void rhs_core(orc_block *b);
+
 void orc_block_rhs(orc_block *b) {
+  shock_sensor(b);
+  rhs_core(b);
+}
+
+static void rhs_core(orc_block *b) {
   memset(b->dq, 0, sizeof(double) * 5 * b->n);
   if (b->p.viscous_order == 2) viscous2(b);
   else if (b->p.viscous_order > 0) viscous(b);
@@ -958,6 +981,7 @@ void orc_block_get(const orc_block *b, int which, double *out) {
     case 6: memcpy(out, b->fv, sizeof(double) * 5 * b->sz); break;
     case 7: memcpy(out, b->gv, sizeof(double) * 5 * b->sz); break;
     case 8: memcpy(out, b->hv, sizeof(double) * 5 * b->sz); break;
+    case 9: memcpy(out, b->sensor, sizeof(double) * b->sz); break;
     default: break;
   }
 }
@@ -1000,9 +1024,10 @@ double orc_block_cfl_dt(orc_block *b) {
 /* One SSP-RK3 step (RK.cuh:100-192) for a single block with only outflow/periodic faces. */
 void orc_block_rk3_step(orc_block *b, double dt) {
   sponge_mean(b);
+  shock_sensor(b);
   memcpy(b->qn, b->cv, sizeof(double) * 5 * b->sz);
   for (int rk = 0; rk < 3; ++rk) {
-    orc_block_rhs(b);
+    rhs_core(b);
     update(b, dt, rk);
     apply_bcs(b);
     update_properties(b);
@@ -1201,7 +1226,14 @@ void orc_multi_exchange(orc_block **B, int nb, orc_link **L, int nl) {
 
 /* The residual of every block: viscous node fluxes of all blocks, their exchange across the links and then the
  * periodic faces (exchange_value, DataCommunication.cu:408-456), differences; inviscid fluxes (RK.cuh:137-157). */
+static void multi_rhs_core(orc_block **B, int nb, orc_link **L, int nl);
+
 void orc_multi_rhs(orc_block **B, int nb, orc_link **L, int nl) {
+  for (int q = 0; q < nb; ++q) shock_sensor(B[q]);
+  multi_rhs_core(B, nb, L, nl);
+}
+
+static void multi_rhs_core(orc_block **B, int nb, orc_link **L, int nl) {
   int collocated = 0;
   for (int q = 0; q < nb; ++q) {
     orc_block *b = B[q];
@@ -1238,10 +1270,11 @@ double orc_multi_cfl_dt(orc_block **B, int nb) {
 void orc_multi_rk3_step(orc_block **B, int nb, orc_link **L, int nl, double dt) {
   for (int q = 0; q < nb; ++q) {
     sponge_mean(B[q]);
+    shock_sensor(B[q]);
     memcpy(B[q]->qn, B[q]->cv, sizeof(double) * 5 * B[q]->sz);
   }
   for (int rk = 0; rk < 3; ++rk) {
-    orc_multi_rhs(B, nb, L, nl);
+    multi_rhs_core(B, nb, L, nl);
     for (int q = 0; q < nb; ++q) {
       update(B[q], dt, rk);
       apply_bcs(B[q]);

@@ -44,6 +44,7 @@ struct orc_mix {
   } inflow[64];       /* inflow faces (orc_mix_set_inflow) */
   double *sponge_sigma, *sponge_mean; /* sponge layer: damping rate (n), running mean of cv (nv * n) */
   unsigned long long sponge_iter;
+  double *sensor; /* shock sensor (sz; ghost entries stay 0) of the hybrid flux-splitting scheme */
   int viscous;        /* viscous_order 2 or 8 with transport data set */
   int bt[6];          /* physical-boundary flags per side for the one-sided collocated differences */
   orc_tran tran;
@@ -95,6 +96,7 @@ orc_mix *orc_mix_create(const orc_params *p, const orc_mech *m, int mx, int my, 
   b->sv = calloc((size_t)b->ns * b->sz, sizeof(double));
   b->c = calloc((size_t)b->sz, sizeof(double));
   b->efd = calloc((size_t)b->sz, sizeof(double));
+  b->sensor = calloc((size_t)b->sz, sizeof(double));
   b->dq = calloc((size_t)b->nv * b->n, sizeof(double));
   b->hstep = calloc((size_t)b->n, sizeof(double));
   b->cn = calloc((size_t)b->nv * b->n, sizeof(double));
@@ -117,6 +119,7 @@ void orc_mix_destroy(orc_mix *b) {
   free(b->sponge_sigma);
   free(b->sponge_mean);
   free(b->cv); free(b->qn); free(b->bv); free(b->sv); free(b->c); free(b->efd); free(b->dq); free(b->hstep);
+  free(b->sensor);
   free(b->tscale); free(b->cn); free(b->wu_ts);
   free(b->mul); free(b->cond); free(b->rhod);
   for (int d = 0; d < 3; ++d) free(b->fvis[d]);
@@ -331,6 +334,7 @@ void orc_mix_get(const orc_mix *b, int which, double *out) {
     case 2: memcpy(out, b->dq, sizeof(double) * b->nv * b->n); break;
     case 3: memcpy(out, b->sv, sizeof(double) * b->ns * b->sz); break;
     case 4: memcpy(out, b->c, sizeof(double) * b->sz); break;
+    case 9: memcpy(out, b->sensor, sizeof(double) * b->sz); break;
     default: break;
   }
 }
@@ -635,7 +639,7 @@ static void hllc(const orc_mix *b, const double *pl, const double *pr, const dou
  * five flow fields gain alpha_l rho Y_l terms (alpha_l = (gamma R_l T - (gamma - 1) h_l) / c^2), and each species
  * field is rho Y_l - Y_l,Roe rho. */
 static void fsweno_mix(const orc_mix *b, double (*q)[NV], double (*k)[3], const double *jac, const double *pp,
-                       const double *snd, int n, double *fc) {
+                       const double *snd, int n, int shock, double *fc) {
   const int nv = b->nv, ns = b->ns;
   const double eps = b->p.weno_eps;
   double Uk[8], lam[8];
@@ -667,7 +671,7 @@ static void fsweno_mix(const orc_mix *b, double (*q)[NV], double (*k)[3], const 
         if (m > 0) vm[m - 1] = fm;
       }
       const double el = (l >= 1 && l <= 3) ? e1 : (l == 4 ? e1 * b->p.v_ref * b->p.v_ref : eps);
-      fc[l] = n == 8 ? orc_weno7(vp, vm, el, 1) : orc_weno5(vp, vm, el, 1);
+      fc[l] = n == 8 ? orc_weno7(vp, vm, el, shock) : orc_weno5(vp, vm, el, shock);
     }
     return;
   }
@@ -753,7 +757,7 @@ static void fsweno_mix(const orc_mix *b, double (*q)[NV], double (*k)[3], const 
       if (m < n - 1) vp[m] = 0.5 * jac[m] * (LF + lmax * LQ);
       if (m > 0) vmn[m - 1] = 0.5 * jac[m] * (LF - lmax * LQ);
     }
-    fch[l] = n == 8 ? orc_weno7(vp, vmn, eps, 1) : orc_weno5(vp, vmn, eps, 1);
+    fch[l] = n == 8 ? orc_weno7(vp, vmn, eps, shock) : orc_weno5(vp, vmn, eps, shock);
   }
   for (int l = 0; l < ns; ++l) {
     double vp[7], vmn[7];
@@ -763,7 +767,7 @@ static void fsweno_mix(const orc_mix *b, double (*q)[NV], double (*k)[3], const 
       if (m < n - 1) vp[m] = 0.5 * jac[m] * (LF + lmax * LQ);
       if (m > 0) vmn[m - 1] = 0.5 * jac[m] * (LF - lmax * LQ);
     }
-    fch[5 + l] = n == 8 ? orc_weno7(vp, vmn, eps, 1) : orc_weno5(vp, vmn, eps, 1);
+    fch[5 + l] = n == 8 ? orc_weno7(vp, vmn, eps, shock) : orc_weno5(vp, vmn, eps, shock);
   }
   const double s1 = fch[0] + fch[3] + fch[4], s3 = fch[0] - fch[4];
   fc[0] = s1;
@@ -871,7 +875,13 @@ static void inviscid_dir(orc_mix *b, int dir) {
             pp[mm] = b->bv[im + 4 * s];
             cc[mm] = b->c[im];
           }
-          fsweno_mix(b, q, kk, jj, pp, cc, n, Fl + (long)nv * (t + 1));
+          int shock = 1; /* the hybrid scheme's switch (WENO.cu:838-849) */
+          if (b->p.sensor_threshold > 1e-10) {
+            shock = 0;
+            for (int ii = -b->g + 1; ii <= b->g; ++ii)
+              if (b->sensor[id + ii * st] > b->p.sensor_threshold) shock = 1;
+          }
+          fsweno_mix(b, q, kk, jj, pp, cc, n, shock, Fl + (long)nv * (t + 1));
           if (b->p.positive_preserving) positivity_limit(b, q, kk, jj, cc, n, Fl + (long)nv * (t + 1));
           continue;
         }
@@ -1301,7 +1311,20 @@ static void mix_viscous(orc_mix *b) {
   }
 }
 
+/* compute_shock_sensor at the start of a step (RK.cuh:112-117; compute_cn, StrangSplitting.cu:19-24) */
+static void shock_sensor(orc_mix *b) {
+  if (b->p.inviscid_scheme > 50 && b->p.sensor_threshold > 1e-10)
+    orc_shock_sensor(b->p.shock_sensor, b->p.sensor_eps, b->mx, b->my, b->mz, b->g, b->bv, b->met, b->sensor);
+}
+
+static void rhs_core(orc_mix *b);
+
 void orc_mix_rhs(orc_mix *b) {
+  shock_sensor(b);
+  rhs_core(b);
+}
+
+static void rhs_core(orc_mix *b) {
   memset(b->dq, 0, sizeof(double) * b->nv * b->n);
   if (b->viscous) mix_viscous(b);
   if (b->p.inviscid_scheme == 2) entropy_fix(b);
@@ -1434,9 +1457,10 @@ void orc_mix_rk3_step(orc_mix *b, double dt) {
 
 static void rk3_stages(orc_mix *b, double dt) {
   b->pp_dt = dt;
+  shock_sensor(b);
   memcpy(b->qn, b->cv, sizeof(double) * b->nv * b->sz);
   for (int rk = 0; rk < 3; ++rk) {
-    orc_mix_rhs(b);
+    rhs_core(b);
     update(b, dt, rk);
     apply_bcs(b);
     all_properties(b);
@@ -1602,7 +1626,7 @@ void orc_mix_wu_step(orc_mix *b, double dt, int method) {
   all_properties(b);
   memcpy(b->qn, b->cv, sizeof(double) * nv * b->sz);
   for (int rk = 0; rk < 3; ++rk) {
-    orc_mix_rhs(b);
+    rhs_core(b); /* the sensor of compute_cn stays (StrangSplitting.cu:576 does not recompute it) */
     for (long q = 0; q < (long)nv * b->n; ++q) b->dq[q] -= b->cn[q];
     update(b, 0.5 * dt, rk);
     apply_bcs(b);

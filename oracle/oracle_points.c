@@ -33,6 +33,7 @@ double orc_weno5_half(const double *v, double eps, int if_shock, double *beta, d
   }
   if (beta) { beta[0] = b0; beta[1] = b1; beta[2] = b2; }
   if (alpha) { alpha[0] = a0; alpha[1] = a1; alpha[2] = a2; }
+  if (!if_shock) return 0.3 * q0 + 0.6 * q1 + 0.1 * q2; /* the linear upwind scheme UD5 (WENO.cu:3097-3100) */
   return (a0 * q0 + a1 * q1 + a2 * q2) / (a0 + a1 + a2);
 }
 
@@ -84,6 +85,7 @@ double orc_weno7_half(const double *v, double eps, int if_shock, double *beta, d
   const double q1 = v[1] - 5 * v[2] + 13 * v[3] + 3 * v[4];
   const double q2 = -v[2] + 7 * v[3] + 7 * v[4] - v[5];
   const double q3 = 3 * v[3] + 13 * v[4] - 5 * v[5] + v[6];
+  if (!if_shock) return one12th * (c0 * q0 + c1 * q1 + c2 * q2 + c3 * q3); /* UD7 (WENO.cu:3205-3212) */
   return one12th * (a0 * q0 + a1 * q1 + a2 * q2 + a3 * q3) / (a0 + a1 + a2 + a3);
 }
 
@@ -515,7 +517,7 @@ static split_point split_point_of(const double *q, const double *k) {
  * its own flux with its own spectral radius |Uk| + cGradK; each conservative variable is then
  * reconstructed by WENO5/WENO7 with the epsilon scaled by v_ref^0, v_ref^2, v_ref^4. */
 static void fsweno_componentwise(const orc_params *p, const double (*q)[5], const double (*met)[3],
-                                 const double *jac, int n, double *fc) {
+                                 const double *jac, int n, int if_shock, double *fc) {
   double fp[8][5], fm[8][5];
   for (int m = 0; m < n; ++m) {
     const split_point s = split_point_of(q[m], met[m]);
@@ -539,7 +541,7 @@ static void fsweno_componentwise(const orc_params *p, const double (*q)[5], cons
       vp[m] = fp[m][l];
       vm[m] = fm[m + 1][l];
     }
-    fc[l] = n == 8 ? orc_weno7(vp, vm, eps[l], 1) : orc_weno5(vp, vm, eps[l], 1);
+    fc[l] = n == 8 ? orc_weno7(vp, vm, eps[l], if_shock) : orc_weno5(vp, vm, eps[l], if_shock);
   }
 }
 
@@ -547,7 +549,7 @@ static void fsweno_componentwise(const orc_params *p, const double (*q)[5], cons
  * weno_ch_air_x (WENO.cu:10-244) projects with the left eigenvectors of the Roe state, splits with the
  * largest spectral radius on the stencil, reconstructs each field and projects back. */
 static void fsweno_characteristic(const orc_params *p, const double (*q)[5], const double (*met)[3],
-                                  const double *jac, int n, double *fc) {
+                                  const double *jac, int n, int if_shock, double *fc) {
   const int c = n / 2 - 1;
   split_point s[8];
   double F[8][5]; /* F[m][0..3] = Uk, rho u Uk + p kx, rho v Uk + p ky, rho w Uk + p kz; F[m][4] = p */
@@ -615,7 +617,7 @@ static void fsweno_characteristic(const orc_params *p, const double (*q)[5], con
       if (m < n - 1) vp[m] = 0.5 * jac[m] * (LF + max_lambda * LQ);
       if (m > 0) vmn[m - 1] = 0.5 * jac[m] * (LF - max_lambda * LQ);
     }
-    fch[l] = n == 8 ? orc_weno7(vp, vmn, p->weno_eps, 1) : orc_weno5(vp, vmn, p->weno_eps, 1);
+    fch[l] = n == 8 ? orc_weno7(vp, vmn, p->weno_eps, if_shock) : orc_weno5(vp, vmn, p->weno_eps, if_shock);
   }
   /* back-projection (WENO.cu:229-243) with the Roe-averaged enthalpy */
   const double s1 = fch[0] + fch[3] + fch[4], s3 = fch[0] - fch[4];
@@ -627,10 +629,66 @@ static void fsweno_characteristic(const orc_params *p, const double (*q)[5], con
   fc[4] = H * s1 - Uk_bar * cm * s3 + Un * fch[1] + Uq * fch[2] - cm * cm / gm1 * fch[3];
 }
 
+void orc_fsweno_flux_hybrid(const orc_params *p, const double (*q)[5], const double (*met)[3], const double *jac,
+                            int n, int if_shock, double *fc) {
+  if (p->inviscid_scheme % 10 == 1) fsweno_componentwise(p, q, met, jac, n, if_shock, fc);
+  else fsweno_characteristic(p, q, met, jac, n, if_shock, fc);
+}
+
 void orc_fsweno_flux(const orc_params *p, const double (*q)[5], const double (*met)[3], const double *jac, int n,
                      double *fc) {
-  if (p->inviscid_scheme % 10 == 1) fsweno_componentwise(p, q, met, jac, n, fc);
-  else fsweno_characteristic(p, q, met, jac, n, fc);
+  orc_fsweno_flux_hybrid(p, q, met, jac, n, 1, fc);
+}
+
+/* The shock sensors of compute_shock_sensor (FieldOperation.cu:74-179), node (i, j, k) of a block with ngg
+ * ghosts.  Ducros (0): 0 where the fourth-order central divergence of the velocity is positive, else
+ * div^2 / (div^2 + |curl|^2 + eps); Jameson (1): the sum over the three directions of
+ * |p_+ + p_- - 2p| / (p_+ + p_- + 2p); jump (2): the largest relative forward jump of density or pressure. */
+void orc_shock_sensor(int type, double eps, int mx, int my, int mz, int ngg, const double *bv, const double *met,
+                      double *sensor) {
+  const long d2 = mx + 2 * ngg, d1 = (long)(my + 2 * ngg) * d2, sz = d1 * (mz + 2 * ngg);
+#pragma omp parallel for collapse(2)
+  for (int k = 0; k < mz; ++k)
+    for (int j = 0; j < my; ++j)
+      for (int i = 0; i < mx; ++i) {
+        const long id = (long)(k + ngg) * d1 + (long)(j + ngg) * d2 + (i + ngg);
+        const long st[3] = {1, d2, d1};
+        double s = 1;
+        if (type == 1) {
+          const double p2 = 2 * bv[id + 4 * sz];
+          s = 0;
+          for (int d = 0; d < 3; ++d) {
+            const double pp = bv[id + st[d] + 4 * sz] + bv[id - st[d] + 4 * sz];
+            s += fabs(pp - p2) / (pp + p2);
+          }
+        } else if (type == 2) {
+          const double r = bv[id], p = bv[id + 4 * sz];
+          s = 0;
+          for (int d = 0; d < 3; ++d) s = fmax(s, fabs(bv[id + st[d]] - r) / r);
+          for (int d = 0; d < 3; ++d) s = fmax(s, fabs(bv[id + st[d] + 4 * sz] - p) / p);
+        } else {
+          double g[3][3]; /* g[c][d]: d(velocity c)/d(index d), fourth-order central */
+          for (int c = 0; c < 3; ++c)
+            for (int d = 0; d < 3; ++d) {
+              const double *f = bv + (1 + c) * sz + id;
+              g[c][d] = (1.0 / 12) * f[-2 * st[d]] - (2.0 / 3) * f[-st[d]] + (2.0 / 3) * f[st[d]] -
+                        (1.0 / 12) * f[2 * st[d]];
+            }
+          double D[3][3]; /* D[c][x]: d(velocity c)/d(x) with the metrics */
+          for (int c = 0; c < 3; ++c)
+            for (int x = 0; x < 3; ++x)
+              D[c][x] = g[c][0] * met[id + x * sz] + g[c][1] * met[id + (3 + x) * sz] + g[c][2] * met[id + (6 + x) * sz];
+          double div = D[0][0] + D[1][1] + D[2][2];
+          if (div > 0) {
+            s = 0;
+          } else {
+            div *= div;
+            const double cx = D[2][1] - D[1][2], cy = D[0][2] - D[2][0], cz = D[1][0] - D[0][1];
+            s = div / (div + cx * cx + cy * cy + cz * cz + eps);
+          }
+        }
+        sensor[id] = s;
+      }
 }
 
 /* Transport.cu:6-9 */

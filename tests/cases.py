@@ -92,6 +92,47 @@ struct init_cond_0 {{
                 eps=1e-40 * (1.0 * math.sqrt(GAMMA)) ** 2, v_ref=math.sqrt(GAMMA))
 
 
+def sod_box(n=101, m=6, inviscid=71, threshold=0.0, sensor=0, sensor_eps=1.0, cfl=0.5):
+    """The Sod shock tube along x on an n x m x m 3D block, outflow in x and periodic in y and z, with the shock
+    sensor keys of the hybrid flux-splitting scheme (shock_sensor, shockSensor_threshold, shockSensor_epsilon)."""
+    c = sod(n=n, my=m, recon=4, inviscid=inviscid, cfl=cfl)
+    x = np.linspace(0.0, 1.0, n)
+    dy = 1.0 / 64
+    y = (np.arange(m) - (m - 1) / 2) * dy
+    X, Y, Z = np.meshgrid(x, y, y, indexing="ij")
+    setup = c["setup"].replace("""array string boundary_conditions {
+    outflow
+}""", """array string boundary_conditions {
+    outflow periodic
+}
+struct periodic {
+    string type = periodic
+    int label = 8
+}""")
+    setup += (f"int shock_sensor = {sensor}\nreal shockSensor_threshold = {threshold!r}\n"
+              f"real shockSensor_epsilon = {sensor_eps!r}\n")
+    faces = [[0, 0, 0, m - 1, 0, m - 1, 6], [n - 1, n - 1, 0, m - 1, 0, m - 1, 6],
+             [0, n - 1, 0, 0, 0, m - 1, 8], [0, n - 1, m - 1, m - 1, 0, m - 1, 8],
+             [0, n - 1, 0, m - 1, 0, 0, 8], [0, n - 1, 0, m - 1, m - 1, m - 1, 8]]
+    return dict(c, setup=setup, grid=(X, Y, Z), faces=faces, boundary=boundary_text(faces),
+                scheme=(inviscid, 4, 0), sensor=(threshold, sensor, sensor_eps))
+
+
+def sod_box_state(n, m, g, smear=0.0):
+    """Sod's left and right states on the padded n x m x m block (jump at x = 0.5), bv as (6, sz); smear > 0
+    replaces the jump by a tanh of that width in x."""
+    N0, N1 = n + 2 * g, m + 2 * g
+    x = (np.arange(N0) - g) / (n - 1)
+    f = (x >= 0.5).astype(float) if smear <= 0 else 0.5 * (1 + np.tanh((x - 0.5) / smear))
+    rho = 1.0 + (0.125 - 1.0) * f
+    p = 1.0 + (0.1 - 1.0) * f
+    bv = np.zeros((6, N1, N1, N0))
+    bv[0] = rho[None, None, :]
+    bv[4] = p[None, None, :]
+    bv[5] = bv[4] / (bv[0] * R_AIR)
+    return bv.reshape(6, -1)
+
+
 def tgv_length(reynolds, mach=0.1, T0=300.0, p0=101325.0):
     rho0 = p0 / (R_AIR * T0)
     V0 = mach * math.sqrt(GAMMA * R_AIR * T0)

@@ -16,7 +16,8 @@ class OrcParams(ctypes.Structure):
     _fields_ = [("inviscid_scheme", ctypes.c_int), ("reconstruction", ctypes.c_int),
                 ("viscous_order", ctypes.c_int), ("prandtl", ctypes.c_double), ("cfl", ctypes.c_double),
                 ("entropy_fix_factor", ctypes.c_double), ("weno_eps", ctypes.c_double),
-                ("v_ref", ctypes.c_double), ("positive_preserving", ctypes.c_int), ("rho_ref", ctypes.c_double)]
+                ("v_ref", ctypes.c_double), ("positive_preserving", ctypes.c_int), ("rho_ref", ctypes.c_double),
+                ("sensor_threshold", ctypes.c_double), ("shock_sensor", ctypes.c_int), ("sensor_eps", ctypes.c_double)]
 
 
 _lib = None
@@ -46,6 +47,7 @@ def load():
     lib.orc_sutherland.argtypes = [D]
     lib.orc_metrics.restype = I
     lib.orc_metrics.argtypes = [I, I, I, I, P, P, P, P, P]
+    lib.orc_shock_sensor.argtypes = [I, D, I, I, I, I, P, P, P]
     lib.orc_sc_metrics.restype = I
     lib.orc_sc_metrics.argtypes = [I, I, I, I, P, P, P, I, P, P]
     lib.orc_block_create.restype = P
@@ -114,8 +116,10 @@ def riemann(scheme, pl, pr, ml, mr, jl=1.0, jr=1.0, efl=0.0, efr=0.0):
     return fc
 
 
-def params(inviscid=3, recon=5, visc=0, prandtl=0.72, cfl=0.5, efix=0.125, eps=1e-40, v_ref=1.0, pp=0, rho_ref=1.0):
-    return OrcParams(inviscid, recon, visc, prandtl, cfl, efix, eps, v_ref, pp, rho_ref)
+def params(inviscid=3, recon=5, visc=0, prandtl=0.72, cfl=0.5, efix=0.125, eps=1e-40, v_ref=1.0, pp=0, rho_ref=1.0,
+           sensor_threshold=0.0, shock_sensor=0, sensor_eps=-1.0):
+    return OrcParams(inviscid, recon, visc, prandtl, cfl, efix, eps, v_ref, pp, rho_ref, sensor_threshold,
+                     shock_sensor, sensor_eps)
 
 
 def interface_flux(p, q, ml, mr, jl=1.0, jr=1.0, efl=0.0, efr=0.0):
@@ -148,6 +152,15 @@ def metrics(x, y, z, ngg):
     if rc != 0:
         raise ValueError("non-positive interior Jacobian")
     return jac, met
+
+
+def shock_sensor(kind, eps, bv, met, shape, ngg):
+    """orc_shock_sensor: the sensor (sz,) of every block node from bv (6*sz) and the metrics (9*sz); ghosts 0."""
+    mx, my, mz = shape
+    bv, met = np.ascontiguousarray(bv, float), np.ascontiguousarray(met, float)
+    out = np.zeros((mx + 2 * ngg) * (my + 2 * ngg) * (mz + 2 * ngg))
+    load().orc_shock_sensor(kind, eps, mx, my, mz, ngg, _p(bv), _p(met), _p(out))
+    return out
 
 
 def sc_metrics(x, y, z, ngg, order):
@@ -196,7 +209,7 @@ class Block:
 
     def get(self, which):
         n = {0: 5 * self.sz, 1: 6 * self.sz, 2: 5 * self.mx * self.my * self.mz, 3: self.sz, 4: self.sz,
-             5: 9 * self.sz, 6: 5 * self.sz, 7: 5 * self.sz, 8: 5 * self.sz}[which]
+             5: 9 * self.sz, 6: 5 * self.sz, 7: 5 * self.sz, 8: 5 * self.sz, 9: self.sz}[which]
         out = np.zeros(n)
         load().orc_block_get(self._h, which, _p(out))
         return out
@@ -494,7 +507,7 @@ class MixBlock:
 
     def get(self, which):
         nv = 5 + self.ns
-        n = {0: nv * self.sz, 1: 6 * self.sz, 2: nv * self.n, 3: self.ns * self.sz, 4: self.sz}[which]
+        n = {0: nv * self.sz, 1: 6 * self.sz, 2: nv * self.n, 3: self.ns * self.sz, 4: self.sz, 9: self.sz}[which]
         out = np.zeros(n)
         load().orc_mix_get(self._h, which, _p(out))
         return out
