@@ -12,7 +12,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libcorefl_amd.so")
 
-CV, BV, DQ, SV, JAC, METRIC, FV, GV, HV = 0, 1, 2, 3, 4, 5, 6, 7, 8
+CV, BV, DQ, SV, JAC, METRIC, FV, GV, HV, SENSOR = 0, 1, 2, 3, 4, 5, 6, 7, 8, 9
 
 # exported symbols, in the order include/corefl_c.h declares them
 SYMBOLS = (
@@ -182,7 +182,7 @@ class Solver:
         v = self.view(b)
         n_int = v.mx * v.my * v.mz
         count = {CV: v.n_var * v.sz, BV: 6 * v.sz, DQ: v.n_var * n_int, SV: v.n_spec * v.sz, JAC: v.sz,
-                 METRIC: 9 * v.sz, FV: 5 * v.sz, GV: 5 * v.sz, HV: 5 * v.sz}[which]
+                 METRIC: 9 * v.sz, FV: 5 * v.sz, GV: 5 * v.sz, HV: 5 * v.sz, SENSOR: v.sz}[which]
         out = np.empty(count, dtype=np.float64)
         _check(self._lib.corefl_get_array(self._ctx, b, which, out.ctypes.data, count))
         return out

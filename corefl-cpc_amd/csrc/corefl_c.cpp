@@ -343,6 +343,9 @@ struct corefl_ctx {
   // Residual of one stage for all blocks; stage < 0 writes dq only.
   void stage(int rk, hipStream_t s) {
     const bool rhs_only = rk < 0;
+    // compute_shock_sensor once per step, before its first residual (RK.cuh:112-117)
+    if (rk <= 0)
+      for (auto &b : blocks) HIPCHK(corefl::launch_shock_sensor(b.dev, scheme, s));
     if (scheme.viscous_order > 0) {
       for (auto &b : blocks) {
         if (ns) HIPCHK(corefl::launch_mix_viscous(b.dev, b.mix, scheme, s));
@@ -401,6 +404,8 @@ struct corefl_ctx {
   // dq; the per-cell reaction sub-step with Cn frozen (wu_reaction_subStep); then three SSP-RK3 stages over
   // dt/2 of R(U) - Cn (wu_RK_subStep).  Boundary conditions and block exchanges after each part.
   void wu_step(hipStream_t s) {
+    // compute_cn computes the shock sensor (StrangSplitting.cu:19-24); the transport stages keep it
+    for (auto &b : blocks) HIPCHK(corefl::launch_shock_sensor(b.dev, scheme, s));
     mix_fluxes(s, [&](Block &b) { HIPCHK(corefl::launch_mix_update(b.dev, b.mix, 0, dt_dev, true, s)); });
     for (auto &b : blocks) {
       HIPCHK(corefl::launch_mix_wu_reaction(b.dev, b.mix, dt_dev, wu_method, s));
@@ -709,9 +714,6 @@ void build(corefl_ctx *ctx, std::vector<corefl::HostBlock> blocks) {
     if (!fs && (inv < 2 || inv > 4))
       throw std::runtime_error("with species, inviscid_scheme must be 2 (Roe), 3 (AUSM+), 4 (HLLC) or 51/52/71/72 "
                                "(WENO5/WENO7 flux splitting)");
-    if (fs && P.get_real("shockSensor_threshold") > 1e-10)
-      throw std::runtime_error("shockSensor_threshold > 0 (hybrid UD/WENO scheme) is not on this path");
-
     if (vo != 0 && vo != 2 && vo != 8)
       throw std::runtime_error("with species, viscous_order must be 0, 2 (face-centred) or 8 (collocated)");
     if (vo > 0 && P.get_bool("gradPInDiffusionFlux"))
@@ -758,10 +760,6 @@ void build(corefl_ctx *ctx, std::vector<corefl::HostBlock> blocks) {
     if (!fs && (inv < 1 || inv > 4))
       throw std::runtime_error("inviscid_scheme must be 1 (LF), 2 (Roe), 3 (AUSM+), 4 (HLLC), or 51/52/71/72 (WENO5/WENO7 "
                                "flux splitting, component-wise/characteristic)");
-    // Parameter.cpp:236-255: a positive shock-sensor threshold switches these schemes to the hybrid
-    // upwind/WENO scheme, which is not on this path
-    if (fs && P.get_real("shockSensor_threshold") > 1e-10)
-      throw std::runtime_error("shockSensor_threshold > 0 (hybrid UD/WENO scheme) is not on this path");
     if (vo != 0 && vo != 2 && vo != 4 && vo != 8)
       throw std::runtime_error("viscous_order must be 0, 2, 4 or 8 on this path");
     if (temporal != 3) throw std::runtime_error("temporal_scheme must be 3 (SSP-RK3)");
@@ -799,9 +797,15 @@ void build(corefl_ctx *ctx, std::vector<corefl::HostBlock> blocks) {
   P.set_real("p_inf", p_inf);
   P.set_real("speed_of_sound", c_inf);
   P.set_real("weno_eps", 1e-40 * rho_inf * v_inf * rho_inf * v_inf);
+  // hybrid flux-splitting scheme (Parameter.cpp:236-255; DParameter.cu:48-50): schemes 51/52/71/72 with a
+  // positive shockSensor_threshold
+  const bool fsw = inv == 51 || inv == 52 || inv == 71 || inv == 72;
+  const double sensor_thr = fsw && P.get_real("shockSensor_threshold") > 1e-10 ? P.get_real("shockSensor_threshold") : 0;
+  if (sensor_thr > 0 && (P.get_int("shock_sensor") < 0 || P.get_int("shock_sensor") > 2))
+    throw std::runtime_error("shock_sensor must be 0 (Ducros), 1 (Jameson) or 2 (density/pressure jump)");
   ctx->scheme = {inv, P.get_int("reconstruction"), vo, P.get_real("weno_eps"), P.get_real("entropy_fix_factor"),
                  P.get_real("prandtl_number"), P.get_real("cfl"), v_inf, P.get_bool("positive_preserving") ? 1 : 0,
-                 rho_inf};
+                 rho_inf, sensor_thr, P.get_int("shock_sensor"), P.get_real("shockSensor_epsilon")};
   ctx->fixed_dt = P.get_bool("fixed_time_step");
   ctx->t_end = P.get_real("total_simulation_time");
 
@@ -919,6 +923,7 @@ void build(corefl_ctx *ctx, std::vector<corefl::HostBlock> blocks) {
       }
     }
     if (inv == 2) d.efd = ctx->alloc(sz);
+    if (ctx->scheme.sensor_threshold > 0) d.sensor = ctx->alloc(sz); // zeroed; the ghost entries stay 0
     d.dq = ctx->alloc(nv * d.n);
     HIPCHK(hipMemcpy(d.jac, h.jac.data(), sz * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(d.met, h.metric.data(), 9 * sz * sizeof(double), hipMemcpyHostToDevice));
@@ -1358,6 +1363,7 @@ int corefl_get_array(corefl_ctx *ctx, int block, int which, double *host, long l
       case COREFL_FV: src = d.fv; cnt = 5 * d.sz; break;
       case COREFL_GV: src = d.gv; cnt = 5 * d.sz; break;
       case COREFL_HV: src = d.hv; cnt = 5 * d.sz; break;
+      case COREFL_SENSOR: src = d.sensor; cnt = d.sz; break;
       default: throw std::runtime_error("unknown array selector");
     }
     if (!src) throw std::runtime_error("array not allocated for this scheme");

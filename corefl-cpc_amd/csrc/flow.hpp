@@ -19,6 +19,8 @@ struct DevBlock {
   int bt[6];                        // physical-boundary flags il, ir, jl, jr, kl, kr
   // sponge layer (null when off): damping rate sigma per interior node (n) and the running mean of cv (nv * n)
   const double *sponge_sigma, *sponge_mean;
+  // shock sensor of the hybrid flux-splitting scheme (null when off; sz, ghost entries 0)
+  double *sensor;
 };
 
 struct DevFace {
@@ -59,6 +61,11 @@ struct Scheme {
   double v_ref;        // v_inf: the component-wise WENO epsilon is weno_eps v_ref^0, ^2, ^4 (WENO.cu:946-949)
   int positive_preserving; // species-flux positivity limiter of the flux-splitting schemes (mixtures)
   double rho_ref;          // rho_inf: the limiter's threshold scale
+  // hybrid scheme (51/52/71/72 with shockSensor_threshold > 1e-10): linear upwind weights where no node of the
+  // interface's stencil has a sensor above the threshold (WENO.cu:838-849); the sensor kind and its epsilon
+  double sensor_threshold;
+  int shock_sensor;
+  double sensor_eps;
 };
 
 // Enqueue on `s`.  Each returns hipSuccess or the launch error.
@@ -73,6 +80,8 @@ hipError_t launch_inner_exchange(const DevBlock &b, const DevBlock &t, const Dev
 hipError_t launch_inner_viscous_exchange(const DevBlock &b, const DevBlock &t, const DevInner &f, hipStream_t s,
                                          int nv = 5);
 hipError_t launch_entropy_fix(const DevBlock &b, const Scheme &sc, hipStream_t s);
+// compute_shock_sensor (FieldOperation.cu:74-179) into b.sensor at the block's nodes; a no-op when b.sensor is null.
+hipError_t launch_shock_sensor(const DevBlock &b, const Scheme &sc, hipStream_t s);
 hipError_t launch_inviscid_flux(const DevBlock &b, const Scheme &sc, int dir, hipStream_t s);
 // Residual + SSP-RK3 stage update; rhs_only writes dq instead of updating.  dt is read from *dt_dev.
 hipError_t launch_update(const DevBlock &b, const Scheme &sc, int rk, const double *dt_dev, bool rhs_only,

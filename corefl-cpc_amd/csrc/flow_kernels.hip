@@ -443,8 +443,9 @@ __global__ __launch_bounds__(256) void k_inviscid_pv(DevBlock b, Scheme sc, doub
 // inviscid_scheme 51/71: component-wise WENO5/WENO7 of the locally split fluxes (WENO.cu:890-990);
 // 52/72: characteristic WENO of the fluxes split with the largest spectral radius on the stencil
 // (WENO.cu:10-244 with the stencil of WENO.cu:1096-1136).  One thread per interface t+1/2 along DIR; the
-// stencil nodes are t-R+1 .. t+R (m = 0 .. 2R-1).  The shock sensor is off (its threshold defaults to 0),
-// so the nonlinear weights always apply.  F receives the interface flux at node t.
+// stencil nodes are t-R+1 .. t+R (m = 0 .. 2R-1).  With the hybrid scheme on, interfaces whose nodes
+// t-ngg+1 .. t+ngg all have a shock sensor at or below the threshold take the linear upwind weights (UD5/UD7);
+// the others, and all of them when it is off, the nonlinear ones.  F receives the interface flux at node t.
 template<int DIR, int R, bool CH>
 __global__ __launch_bounds__(256, 2) void k_fsweno(DevBlock b, Scheme sc, double *F, TileGrid tg) {
   Tile t;
@@ -459,6 +460,7 @@ __global__ __launch_bounds__(256, 2) void k_fsweno(DevBlock b, Scheme sc, double
   const long sz = b.sz;
   constexpr int N = 2 * R;
   constexpr double gm1 = kGamma - 1;
+  const bool shock = wenodev::stencil_shock(b.sensor, sc.sensor_threshold, id, st, b.g);
 
   if constexpr (!CH) {
     // split fluxes F+- = (J/2)(k.F +- lambda q), lambda = |Uk| + |k| c of the node itself; fp holds
@@ -502,10 +504,13 @@ __global__ __launch_bounds__(256, 2) void k_fsweno(DevBlock b, Scheme sc, double
       if constexpr (R == 4) {
         const double a[7] = {fp[0][l], fp[1][l], fp[2][l], fp[3][l], fp[4][l], fp[5][l], fp[6][l]};
         const double c[7] = {fm[6][l], fm[5][l], fm[4][l], fm[3][l], fm[2][l], fm[1][l], fm[0][l]};
-        f = weno7_half(a, eps) + weno7_half(c, eps);
-      } else {
+        f = shock ? weno7_half(a, eps) + weno7_half(c, eps) : wenodev::ud7_half(a) + wenodev::ud7_half(c);
+      } else if (shock) {
         f = weno5_half(fp[0][l], fp[1][l], fp[2][l], fp[3][l], fp[4][l], eps) +
             weno5_half(fm[4][l], fm[3][l], fm[2][l], fm[1][l], fm[0][l], eps);
+      } else {
+        f = wenodev::ud5_half(fp[0][l], fp[1][l], fp[2][l], fp[3][l], fp[4][l]) +
+            wenodev::ud5_half(fm[4][l], fm[3][l], fm[2][l], fm[1][l], fm[0][l]);
       }
       F[id + l * sz] = f;
     }
@@ -599,9 +604,16 @@ __global__ __launch_bounds__(256, 2) void k_fsweno(DevBlock b, Scheme sc, double
     double fch[5];
 #pragma unroll
     for (int l = 0; l < 5; ++l) {
-      if constexpr (R == 4) fch[l] = weno7_half(vp[l], eps) + weno7_half(vn[l], eps);
-      else fch[l] = weno5_half(vp[l][0], vp[l][1], vp[l][2], vp[l][3], vp[l][4], eps) +
-                    weno5_half(vn[l][0], vn[l][1], vn[l][2], vn[l][3], vn[l][4], eps);
+      if constexpr (R == 4) {
+        fch[l] = shock ? weno7_half(vp[l], eps) + weno7_half(vn[l], eps)
+                       : wenodev::ud7_half(vp[l]) + wenodev::ud7_half(vn[l]);
+      } else if (shock) {
+        fch[l] = weno5_half(vp[l][0], vp[l][1], vp[l][2], vp[l][3], vp[l][4], eps) +
+                 weno5_half(vn[l][0], vn[l][1], vn[l][2], vn[l][3], vn[l][4], eps);
+      } else {
+        fch[l] = wenodev::ud5_half(vp[l][0], vp[l][1], vp[l][2], vp[l][3], vp[l][4]) +
+                 wenodev::ud5_half(vn[l][0], vn[l][1], vn[l][2], vn[l][3], vn[l][4]);
+      }
     }
     // back-projection with the Roe-averaged enthalpy (WENO.cu:229-243)
     const double s1 = fch[0] + fch[3] + fch[4], s3 = fch[0] - fch[4];
@@ -1154,6 +1166,57 @@ TileGrid march_tiles(const DevBlock &b, int kz) {
   return TileGrid{(b.mx + 63) / 64, (b.my + 3) / 4, (b.mz + kz - 1) / kz, 0};
 }
 
+// compute_shock_sensor (FieldOperation.cu:74-179) at the block's nodes, air and mixtures alike (it reads bv and the
+// metrics only): Ducros (0, the default): 0 where the fourth-order central divergence of the velocity is positive,
+// else div^2 / (div^2 + |curl|^2 + eps); Jameson (1): sum over directions of |p+ + p- - 2p| / (p+ + p- + 2p);
+// density/pressure jump (2): the largest relative forward jump of density or pressure.
+__global__ __launch_bounds__(256) void k_shock_sensor(DevBlock b, int type, double eps) {
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+  if (i >= b.mx || j >= b.my) return;
+  const long id = nid(b, i, j, k), sz = b.sz;
+  const long st[3] = {1, b.d2, b.d1};
+  double s;
+  if (type == 1) {
+    const double p2 = 2 * b.bv[id + 4 * sz];
+    s = 0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const double pp = b.bv[id + st[d] + 4 * sz] + b.bv[id - st[d] + 4 * sz];
+      s += fabs(pp - p2) / (pp + p2);
+    }
+  } else if (type == 2) {
+    const double r = b.bv[id], p = b.bv[id + 4 * sz];
+    s = 0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) s = fmax(s, fabs(b.bv[id + st[d]] - r) / r);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) s = fmax(s, fabs(b.bv[id + st[d] + 4 * sz] - p) / p);
+  } else {
+    double D[3][3]; // d(velocity c)/d(x_a)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      double g[3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const double *f = b.bv + (1 + c) * sz + id;
+        g[d] = (1.0 / 12) * f[-2 * st[d]] - (2.0 / 3) * f[-st[d]] + (2.0 / 3) * f[st[d]] - (1.0 / 12) * f[2 * st[d]];
+      }
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+        D[c][a] = g[0] * b.met[id + a * sz] + g[1] * b.met[id + (3 + a) * sz] + g[2] * b.met[id + (6 + a) * sz];
+    }
+    double div = D[0][0] + D[1][1] + D[2][2];
+    if (div > 0) {
+      s = 0;
+    } else {
+      div *= div;
+      const double cx = D[2][1] - D[1][2], cy = D[0][2] - D[2][0], cz = D[1][0] - D[0][1];
+      s = div / (div + cx * cx + cy * cy + cz * cz + eps);
+    }
+  }
+  b.sensor[id] = s;
+}
+
 } // namespace
 
 hipError_t launch_cv_from_bv_all(const DevBlock &b, hipStream_t s) {
@@ -1336,6 +1399,12 @@ hipError_t launch_dt_reduce(const double *part, long n, double *dt_dev, const do
 
 hipError_t launch_time_advance(double *time_dev, const double *dt_dev, hipStream_t s) {
   k_time_advance<<<1, 1, 0, s>>>(time_dev, dt_dev);
+  return hipGetLastError();
+}
+
+hipError_t launch_shock_sensor(const DevBlock &b, const Scheme &sc, hipStream_t s) {
+  if (!b.sensor) return hipSuccess;
+  k_shock_sensor<<<dim3((b.mx + 63) / 64, (b.my + 3) / 4, b.mz), dim3(64, 4, 1), 0, s>>>(b, sc.shock_sensor, sc.sensor_eps);
   return hipGetLastError();
 }
 

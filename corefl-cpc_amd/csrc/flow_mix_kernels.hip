@@ -923,9 +923,17 @@ __global__ __launch_bounds__(256) void k_mix_fsweno(DevBlock b, MixArrays x, Sch
   const long st = DIR == 0 ? 1 : (DIR == 1 ? b.d2 : b.d1);
   const long id = nid(b, i, j, k), sz = b.sz;
   constexpr int N = 2 * R;
-  auto weno = [](const double *vp, const double *vn, double eps) {
-    if constexpr (R == 4) return weno7_half(vp, eps) + weno7_half(vn, eps);
-    else return weno5_half(vp[0], vp[1], vp[2], vp[3], vp[4], eps) + weno5_half(vn[0], vn[1], vn[2], vn[3], vn[4], eps);
+  // the hybrid scheme's switch (WENO.cu:838-849): linear upwind weights where the stencil is free of shocks
+  const bool shock = wenodev::stencil_shock(b.sensor, sc.sensor_threshold, id, st, b.g);
+  auto weno = [shock](const double *vp, const double *vn, double eps) {
+    if constexpr (R == 4) {
+      return shock ? weno7_half(vp, eps) + weno7_half(vn, eps) : wenodev::ud7_half(vp) + wenodev::ud7_half(vn);
+    } else {
+      return shock ? weno5_half(vp[0], vp[1], vp[2], vp[3], vp[4], eps) +
+                         weno5_half(vn[0], vn[1], vn[2], vn[3], vn[4], eps)
+                   : wenodev::ud5_half(vp[0], vp[1], vp[2], vp[3], vp[4]) +
+                         wenodev::ud5_half(vn[0], vn[1], vn[2], vn[3], vn[4]);
+    }
   };
   // node quantities: U_k (unscaled metric), radius, half Jacobian
   double Uk[N], lam[N], hj[N], cg[N];

@@ -25,6 +25,8 @@ Parameter::Parameter() {
   reals["entropy_fix_factor"] = 0.125;
   ints["viscous_order"] = 2;
   reals["shockSensor_threshold"] = 0;
+  ints["shock_sensor"] = 0;           // 0 Ducros, 1 Jameson, 2 density/pressure jump (Parameter.cpp:384)
+  reals["shockSensor_epsilon"] = -1;  // the Ducros sensor's epsilon (Parameter.cpp:386)
   bools["positive_preserving"] = false;
   ints["species"] = 0;
   ints["reaction"] = 0;

@@ -68,5 +68,33 @@ __device__ __forceinline__ double weno7_half(const double *v, double eps) {
   return (1.0 / 12) * (a0 * q0 + a1 * q1 + a2 * q2 + a3 * q3) * rcp(a0 + a1 + a2 + a3);
 }
 
+// The linear upwind halves the hybrid scheme uses where the stencil is smooth: the WENO5 candidates with the
+// optimal weights (UD5, WENO.cu:3097-3107) and the WENO7 ones (UD7, WENO.cu:3205-3220), as the reference writes them.
+__device__ __forceinline__ double ud5_half(double v0, double v1, double v2, double v3, double v4) {
+  constexpr double o6 = 1.0 / 6;
+  const double p0 = o6 * (2 * v2 + 5 * v3 - v4);
+  const double p1 = o6 * (-v1 + 5 * v2 + 2 * v3);
+  const double p2 = o6 * (2 * v0 - 7 * v1 + 11 * v2);
+  return 0.3 * p0 + 0.6 * p1 + 0.1 * p2;
+}
+
+__device__ __forceinline__ double ud7_half(const double *v) {
+  constexpr double c0 = 1.0 / 35, c1 = 12.0 / 35, c2 = 18.0 / 35, c3 = 4.0 / 35;
+  const double q0 = -3 * v[0] + 13 * v[1] - 23 * v[2] + 25 * v[3];
+  const double q1 = v[1] - 5 * v[2] + 13 * v[3] + 3 * v[4];
+  const double q2 = -v[2] + 7 * v[3] + 7 * v[4] - v[5];
+  const double q3 = 3 * v[3] + 13 * v[4] - 5 * v[5] + v[6];
+  return (1.0 / 12) * (c0 * q0 + c1 * q1 + c2 * q2 + c3 * q3);
+}
+
+// The hybrid scheme's switch for the interface between node id and id + st (WENO.cu:838-849): a shock unless
+// the threshold is on and no node id - (g-1) st .. id + g st has a sensor above it.
+__device__ __forceinline__ bool stencil_shock(const double *sensor, double threshold, long id, long st, int g) {
+  if (!(threshold > 1e-10)) return true;
+  bool shock = false;
+  for (int ii = -g + 1; ii <= g; ++ii) shock = shock || sensor[id + ii * st] > threshold;
+  return shock;
+}
+
 } // namespace wenodev
 } // namespace corefl

@@ -47,7 +47,8 @@ enum {
   COREFL_METRIC = 5,
   COREFL_FV = 6,
   COREFL_GV = 7,
-  COREFL_HV = 8
+  COREFL_HV = 8,
+  COREFL_SENSOR = 9 /* shock sensor of the hybrid flux-splitting scheme (DZone::shock_sensor, src/Field.h:75) */
 };
 
 /* Replaces Parameter(&argc,&argv) + Mesh(parameter) + Driver<mix>(parameter, mesh)

@@ -36,12 +36,13 @@ def test_bad_input_reports_error():
 
 
 def test_flux_splitting_weno_scheme_validation():
-    """Schemes 51/52/71/72 are accepted; a positive shock-sensor threshold (the reference's hybrid
-    upwind/WENO switch, Parameter.cpp:236-255) and unknown scheme numbers fail before any device call."""
+    """Schemes 51/52/71/72 are accepted (with the hybrid scheme's shock sensors 0, 1, 2); unknown scheme and sensor
+    numbers fail before any device call."""
     import cases
     c = cases.sod(16, recon=4, inviscid=72)
-    bad = [(c["setup"].replace("int inviscid_scheme = 72", "int inviscid_scheme = 72\nreal shockSensor_threshold = 0.1"),
-            "shockSensor_threshold"),
+    bad = [(c["setup"].replace("int inviscid_scheme = 72",
+                               "int inviscid_scheme = 72\nreal shockSensor_threshold = 0.1\nint shock_sensor = 3"),
+            "shock_sensor"),
            (c["setup"].replace("int inviscid_scheme = 72", "int inviscid_scheme = 73"), "51/52/71/72")]
     for setup, msg in bad:
         with pytest.raises(corefl.CoreflError, match=msg):
