@@ -29,7 +29,7 @@ SYMBOLS = (
     "corefl_chem_autotune", "corefl_chem_synchronize", "corefl_chem_integrate_noalloc",
     "corefl_surrogate_n_params", "corefl_surrogate_create", "corefl_surrogate_destroy", "corefl_surrogate_set_params",
     "corefl_surrogate_get_params", "corefl_surrogate_set_scaling", "corefl_surrogate_set_transform",
-    "corefl_surrogate_set_gate",
+    "corefl_surrogate_set_gate", "corefl_surrogate_set_activation",
     "corefl_surrogate_set_data", "corefl_surrogate_train", "corefl_surrogate_grad", "corefl_surrogate_forward",
     "corefl_chem_surrogate_step",
 )
@@ -116,6 +116,7 @@ def load_library(path=LIB_PATH):
     lib.corefl_surrogate_get_params.argtypes = [P, P, LL]
     lib.corefl_surrogate_set_scaling.argtypes = [P, P, P, P, P]
     lib.corefl_surrogate_set_gate.argtypes = [P, P]
+    lib.corefl_surrogate_set_activation.argtypes = [P, I]
     lib.corefl_surrogate_set_transform.argtypes = [P, D, P]
     lib.corefl_surrogate_set_data.argtypes = [P, LL, P, P]
     lib.corefl_surrogate_train.argtypes = [P, I, I, F, ctypes.c_ulonglong, P]
@@ -497,6 +498,13 @@ class Surrogate:
     def set_gate(self, t_min, t_max, p_min, p_max, dt_min, dt_max, dT_max, dT_step=1e30):
         g = np.array([t_min, t_max, p_min, p_max, dt_min, dt_max, dT_max, dT_step], dtype=np.float64)
         _check(self._lib.corefl_surrogate_set_gate(self._ctx, g.ctypes.data))
+
+    ACT_GELU, ACT_HARD_SWISH = 0, 1
+
+    def set_activation(self, act):
+        """Hidden activation: ACT_GELU (tanh form, the default) or ACT_HARD_SWISH; set before training."""
+        _check(self._lib.corefl_surrogate_set_activation(self._ctx, int(act)))
+        self.act = int(act)
 
     def set_data(self, X, Y):
         X = np.ascontiguousarray(X, np.float32)

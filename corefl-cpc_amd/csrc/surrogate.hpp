@@ -18,9 +18,13 @@ constexpr int kMlpKp[kMlpLayers] = {16, 128, 128, 128, 128}; // K padded to the 
 constexpr int kMlpNp[kMlpLayers] = {128, 128, 128, 128, 32}; // N padded to the 32-wide MFMA tile
 
 // Device weights: BF16 W^T, [Np][Kp] row-major (k contiguous), zero padded; FP32 biases [N].
+// Hidden activation: GELU in its tanh form (the network the specification names) or hard swish.
+constexpr int kActGelu = 0, kActHardSwish = 1;
+
 struct MlpDev {
   const unsigned short *wt[kMlpLayers];
   const float *bias[kMlpLayers];
+  int act; // kActGelu or kActHardSwish
 };
 
 // Feature and target scaling.  Features (T, ln p, Y_1^a..Y_9^a, ln dt) -> (f - mu) / sig, with a = y_power
@@ -73,6 +77,7 @@ struct GemmArgs {
   const float *zprev;
   long ldz;
   int k_chunk; // K extent of one gridDim.z slice (multiple of 32); K for the non-atomic epilogues
+  int act;     // hidden activation of the forward and data-gradient epilogues
 };
 
 hipError_t launch_gemm(const GemmArgs &a, hipStream_t s);

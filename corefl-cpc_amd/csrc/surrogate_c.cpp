@@ -58,6 +58,7 @@ struct corefl_surrogate {
   bool have_scaling = false, have_gate = false, have_params = false;
   int cus = 0;
   double step_dt = 0; // uniform sub-step length of the current corefl_chem_surrogate_step call
+  int act = kActGelu;  // hidden activation
   ~corefl_surrogate();
 };
 
@@ -86,6 +87,7 @@ MlpDev net_of(const corefl_surrogate *s) {
     n.wt[l] = s->wt[l];
     n.bias[l] = s->p + kOff.b[l];
   }
+  n.act = s->act;
   return n;
 }
 
@@ -113,6 +115,7 @@ void forward(corefl_surrogate *s, int B) {
     GemmArgs a = gemm(B, kMlpN[l], kMlpKp[l], s->H[l], ld, 1, s->wt[l], 1, kMlpKp[l],
                       l < kMlpLayers - 1 ? kEpFwdHidden : kEpFwdOut);
     a.bias = s->p + kOff.b[l];
+    a.act = s->act;
     if (l < kMlpLayers - 1) {
       a.C = s->Z[l];
       a.ldc = 128;
@@ -146,6 +149,7 @@ void backward(corefl_surrogate *s, int B, float *loss_slot) {
       GemmArgs d = gemm(B, kMlpK[l], kMlpN[l], G, ldg, 1, s->wt[l], kMlpKp[l], 1, kEpBwdData);
       d.zprev = s->Z[l - 1];
       d.ldz = 128;
+      d.act = s->act;
       d.Cb = Gn;
       d.ldcb = 128;
       HIPCHK(launch_gemm(d, s->stream));
@@ -346,6 +350,14 @@ int corefl_surrogate_set_gate(corefl_surrogate *s, const double *gate) {
           s->gate.dT_max >= 0 && s->gate.dT_step >= 0))
       throw std::runtime_error("corefl_surrogate_set_gate: empty range");
     s->have_gate = true;
+  });
+}
+
+int corefl_surrogate_set_activation(corefl_surrogate *s, int act) {
+  return guarded([&] {
+    if (!s || (act != kActGelu && act != kActHardSwish))
+      throw std::runtime_error("corefl_surrogate_set_activation: act must be 0 (GELU) or 1 (hard swish)");
+    s->act = act;
   });
 }
 

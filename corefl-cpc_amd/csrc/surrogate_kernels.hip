@@ -31,17 +31,32 @@ __device__ __forceinline__ unsigned short f2bf(float f) {
   return (unsigned short)(u >> 16);
 }
 
-// Hidden activation: hard swish (Howard et al., "Searching for MobileNetV3", ICCV 2019),
-// act(z) = z clamp(z / 6 + 1/2, 0, 1): a GELU-like shape in three VALU operations without a transcendental.
-// The activations, not the matrix products, bound the inference kernel: the tanh-form GELU cost 0.271 ms per
-// 2^20 rows there, this form 0.194 ms.
-constexpr float kActSlope = 1.0f / 6.0f;
-__device__ __forceinline__ float act(float z) {
-  return z * __builtin_amdgcn_fmed3f(__builtin_fmaf(z, kActSlope, 0.5f), 0.0f, 1.0f);
+// Hidden activations.  GELU (Hendrycks and Gimpel, "Gaussian Error Linear Units", 2016) in its tanh form:
+// 0.5 z (1 + tanh(u)) = z / (1 + exp(-2u)), u = sqrt(2/pi) (z + 0.044715 z^3), with the hardware exp2 and
+// reciprocal (1 ulp each; the activation is rounded to BF16 next).  Hard swish (Howard et al., "Searching for
+// MobileNetV3", ICCV 2019), z clamp(z / 6 + 1/2, 0, 1): a GELU-like shape in three VALU operations without a
+// transcendental (0.194 ms per 2^20 rows against the tanh GELU's 0.271 ms in the inference kernel).
+constexpr float kGs = 0.7978845608028654f, kGa = 0.044715f;
+// -2u log2(e) = z (c1 + c3 z^2)
+constexpr float kG1 = -2.0f * 0.7978845608028654f * 1.4426950408889634f;
+constexpr float kG3 = -2.0f * 0.7978845608028654f * 0.044715f * 1.4426950408889634f;
+__device__ __forceinline__ float sig2u(float z) {
+  const float w = z * __builtin_fmaf(kG3, z * z, kG1);
+  return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(w)); // exp2 -> +inf gives 0, the z -> -inf limit
 }
-// d act / dz: 0 below -3, 1 above 3, z / 3 + 1/2 between
-__device__ __forceinline__ float act_grad(float z) {
-  return z <= -3.0f ? 0.0f : (z >= 3.0f ? 1.0f : __builtin_fmaf(z, 2.0f * kActSlope, 0.5f));
+constexpr float kActSlope = 1.0f / 6.0f;
+template<int ACT> __device__ __forceinline__ float act(float z) {
+  if constexpr (ACT == kActGelu) return z * sig2u(z);
+  else return z * __builtin_amdgcn_fmed3f(__builtin_fmaf(z, kActSlope, 0.5f), 0.0f, 1.0f);
+}
+// d act / dz (hard swish: 0 below -3, 1 above 3, z / 3 + 1/2 between)
+template<int ACT> __device__ __forceinline__ float act_grad(float z) {
+  if constexpr (ACT == kActGelu) {
+    const float s = sig2u(z);
+    return s + z * s * (1.0f - s) * 2.0f * kGs * (1.0f + 3.0f * kGa * z * z);
+  } else {
+    return z <= -3.0f ? 0.0f : (z >= 3.0f ? 1.0f : __builtin_fmaf(z, 2.0f * kActSlope, 0.5f));
+  }
 }
 
 // ------------------------------------------------------------------------------------------------ GEMM
@@ -94,11 +109,12 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs a) {
     if (a.ep == kEpFwdHidden) {
       const float z = v + a.bias[n];
       a.C[m * a.ldc + n] = z;
-      a.Cb[m * a.ldcb + n] = f2bf(act(z));
+      a.Cb[m * a.ldcb + n] = f2bf(a.act == kActGelu ? act<kActGelu>(z) : act<kActHardSwish>(z));
     } else if (a.ep == kEpFwdOut) {
       a.C[m * a.ldc + n] = v + a.bias[n];
     } else if (a.ep == kEpBwdData) {
-      a.Cb[m * a.ldcb + n] = f2bf(v * act_grad(a.zprev[m * a.ldz + n]));
+      const float z = a.zprev[m * a.ldz + n];
+      a.Cb[m * a.ldcb + n] = f2bf(v * (a.act == kActGelu ? act_grad<kActGelu>(z) : act_grad<kActHardSwish>(z)));
     } else {
       atomicAdd(&a.C[m * a.ldc + n], v);
     }
@@ -389,12 +405,13 @@ __device__ __forceinline__ unsigned pack_bf16(float a, float b) {
 
 // Activation of acc + bias for one 32-neuron block, packed as BF16 pairs: p[q][0..1] holds neurons
 // 8q + 4h + 0..3 of the block (h = lane / 32).
+template<int ACT>
 __device__ __forceinline__ void act_pack(const floatx16 &acc, const float *bias, int h, unsigned (&p)[4][2]) {
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     const float4 b = *reinterpret_cast<const float4 *>(&bias[8 * q + 4 * h]);
-    p[q][0] = pack_bf16(act(acc[4 * q] + b.x), act(acc[4 * q + 1] + b.y));
-    p[q][1] = pack_bf16(act(acc[4 * q + 2] + b.z), act(acc[4 * q + 3] + b.w));
+    p[q][0] = pack_bf16(act<ACT>(acc[4 * q] + b.x), act<ACT>(acc[4 * q + 1] + b.y));
+    p[q][1] = pack_bf16(act<ACT>(acc[4 * q + 2] + b.z), act<ACT>(acc[4 * q + 3] + b.w));
   }
 }
 
@@ -414,6 +431,7 @@ __device__ __forceinline__ void next_operand(const unsigned (&p)[4][2], int h, s
   }
 }
 
+template<int ACT>
 __global__ __launch_bounds__(64 * kMlpWaves) void k_mlp(MlpDev net, const unsigned short *__restrict__ feat,
                                                          float *__restrict__ out, long n) {
   __shared__ __attribute__((aligned(16))) unsigned short w0[128 * kS0];
@@ -464,7 +482,7 @@ __global__ __launch_bounds__(64 * kMlpWaves) void k_mlp(MlpDev net, const unsign
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
           unsigned p[4][2];
-          act_pack(acc[t][j], &bias[l][32 * j], h, p);
+          act_pack<ACT>(acc[t][j], &bias[l][32 * j], h, p);
           next_operand(p, h, bop[t][2 * j], bop[t][2 * j + 1]);
         }
       if (l == 3) break;
@@ -611,7 +629,8 @@ hipError_t launch_mlp_raw(const MlpDev &net, long n, const unsigned short *feat,
                           hipStream_t s) {
   if (n <= 0) return hipSuccess;
   const long units = (n + 32 * kMlpTiles * kMlpWaves - 1) / (32 * kMlpTiles * kMlpWaves);
-  k_mlp<<<(unsigned)std::min<long>(units, blocks), 64 * kMlpWaves, 0, s>>>(net, feat, y, n);
+  if (net.act == kActGelu) k_mlp<kActGelu><<<(unsigned)std::min<long>(units, blocks), 64 * kMlpWaves, 0, s>>>(net, feat, y, n);
+  else k_mlp<kActHardSwish><<<(unsigned)std::min<long>(units, blocks), 64 * kMlpWaves, 0, s>>>(net, feat, y, n);
   return hipGetLastError();
 }
 
@@ -633,7 +652,10 @@ hipError_t launch_mlp_chem(const MlpDev &net, const ChemMech *mech_dev, const Ch
   const unsigned g = grid1(b.n, 256);
   k_chem_feat<<<g, 256, 0, s>>>(a, w.feat, w.flag);
   const long units = (b.n + 32 * kMlpTiles * kMlpWaves - 1) / (32 * kMlpTiles * kMlpWaves);
-  k_mlp<<<(unsigned)std::min<long>(units, blocks), 64 * kMlpWaves, 0, s>>>(net, w.feat, w.out, b.n);
+  if (net.act == kActGelu)
+    k_mlp<kActGelu><<<(unsigned)std::min<long>(units, blocks), 64 * kMlpWaves, 0, s>>>(net, w.feat, w.out, b.n);
+  else
+    k_mlp<kActHardSwish><<<(unsigned)std::min<long>(units, blocks), 64 * kMlpWaves, 0, s>>>(net, w.feat, w.out, b.n);
   k_chem_post<<<g, 256, 0, s>>>(a, w.out, w.flag);
   return hipGetLastError();
 }

@@ -270,8 +270,8 @@ int corefl_chem_integrate_noalloc(corefl_chem *chem, long long n, const double *
                                   double *rhoY, double dt, double rtol, double atol, int max_attempts, void *stream);
 
 /* ---------------------------------------------------------------------------------------------------
- * Chemistry surrogate (no reference counterpart: a method change).  An MLP 12 -> 4 x 128 (hard swish,
- * z clamp(z/6 + 1/2, 0, 1)) -> 10 with BF16 weights and FP32 accumulation on the matrix cores maps the features
+ * Chemistry surrogate (no reference counterpart: a method change).  An MLP 12 -> 4 x 128 (GELU in its tanh form,
+ * or hard swish z clamp(z/6 + 1/2, 0, 1)) -> 10 with BF16 weights and FP32 accumulation on the matrix cores maps the features
  * (T, ln p, Y_1..Y_9, ln dt), each scaled as (f - mu) / sig, to the scaled rates (dY_k/dt, dT/dt).  It is
  * trained on the GPU (forward, backward and Adam kernels of this library) on states labelled by the exact
  * integrator, and replaces the exact chemistry sub-step in cells that pass its gate.
@@ -298,6 +298,9 @@ int corefl_surrogate_set_transform(corefl_surrogate *s, double y_power, const do
  * energy conservation differ by at most dT_max, and when its temperature changes by at most dT_step over
  * the sub-step. */
 int corefl_surrogate_set_gate(corefl_surrogate *s, const double *gate);
+/* Hidden activation: 0 GELU, tanh form 0.5 z (1 + tanh(sqrt(2/pi) (z + 0.044715 z^3))) (the default), 1 hard swish.
+ * It defines the network the parameters belong to: set it before training. */
+int corefl_surrogate_set_activation(corefl_surrogate *s, int act);
 /* Training set on the device: n rows of scaled features X (n x 12) and scaled targets Y (n x 10). */
 int corefl_surrogate_set_data(corefl_surrogate *s, long long n, const float *X, const float *Y);
 /* `steps` Adam steps (beta 0.9 / 0.999, eps 1e-8, learning rate lr) on mini-batches of `batch` rows drawn

@@ -5,8 +5,9 @@ checker.  The surrogate has no counterpart in the reference: it is pinned by fin
 checks of this restatement (tests/test_surrogate_oracle_cpu.py), and the GPU kernels are compared with it
 with the GPU's BF16 operand rounding emulated (`emulate=True`).
 
-Network: 12 -> 128 -> 128 -> 128 -> 128 -> 10, hard swish z clamp(z/6 + 1/2, 0, 1) (Howard et al.,
-"Searching for MobileNetV3", 2019) on the four hidden layers, linear output.
+Network: 12 -> 128 -> 128 -> 128 -> 128 -> 10 with a hidden activation on the four hidden layers, linear output:
+GELU in its tanh form 0.5 z (1 + tanh(sqrt(2/pi) (z + 0.044715 z^3))) (Hendrycks and Gimpel, 2016; act="gelu", the
+default) or hard swish z clamp(z/6 + 1/2, 0, 1) (Howard et al., "Searching for MobileNetV3", 2019; act="hswish").
 GPU numerics emulated here: every GEMM operand (layer inputs, weights, back-propagated gradients) is
 rounded to BF16 (round to nearest even), products are accumulated in FP32 on the GPU (FP64 here), and
 pre-activations, activations and the loss are FP32 on the GPU (FP64 here).
@@ -24,14 +25,38 @@ def bf16(x):
     return u.astype(np.uint32).view(np.float32).astype(np.float64)
 
 
-def act(z):
+GELU_S, GELU_A = np.sqrt(2.0 / np.pi), 0.044715
+
+
+def hswish(z):
     """Hard swish."""
     return z * np.clip(z / 6.0 + 0.5, 0.0, 1.0)
 
 
-def act_grad(z):
+def hswish_grad(z):
     """Its derivative: 0 below -3, 1 above 3, z/3 + 1/2 between (the kinks take the outer values)."""
     return np.where(z <= -3.0, 0.0, np.where(z >= 3.0, 1.0, z / 3.0 + 0.5))
+
+
+def gelu(z):
+    """GELU, tanh form."""
+    return 0.5 * z * (1.0 + np.tanh(GELU_S * (z + GELU_A * z ** 3)))
+
+
+def gelu_grad(z):
+    t = np.tanh(GELU_S * (z + GELU_A * z ** 3))
+    return 0.5 * (1.0 + t) + 0.5 * z * (1.0 - t * t) * GELU_S * (1.0 + 3.0 * GELU_A * z * z)
+
+
+ACTS = {"gelu": (gelu, gelu_grad), "hswish": (hswish, hswish_grad)}
+
+
+def act(z, kind="gelu"):
+    return ACTS[kind][0](z)
+
+
+def act_grad(z, kind="gelu"):
+    return ACTS[kind][1](z)
 
 
 def init_params(seed, sizes=SIZES):
@@ -45,7 +70,7 @@ def init_params(seed, sizes=SIZES):
     return params
 
 
-def forward(params, X, emulate=True):
+def forward(params, X, emulate=True, kind="gelu"):
     """Returns (outputs, cache).  X: (B, 12)."""
     r = bf16 if emulate else (lambda a: np.asarray(a, dtype=np.float64))
     H = r(X)
@@ -56,14 +81,14 @@ def forward(params, X, emulate=True):
         Z = H @ Wr + b
         cache.append((H, Wr, Z))
         if l < L - 1:
-            H = r(act(Z))
+            H = r(act(Z, kind))
     return Z, cache
 
 
-def loss_and_grads(params, X, Y, emulate=True):
+def loss_and_grads(params, X, Y, emulate=True, kind="gelu"):
     """Mean squared error over batch and outputs, and its gradients [(dW_l, db_l)]."""
     r = bf16 if emulate else (lambda a: np.asarray(a, dtype=np.float64))
-    out, cache = forward(params, X, emulate)
+    out, cache = forward(params, X, emulate, kind)
     B, N = out.shape
     loss = np.mean((out - Y) ** 2)
     G = r(2.0 * (out - Y) / (B * N))
@@ -73,7 +98,7 @@ def loss_and_grads(params, X, Y, emulate=True):
         grads[l] = (H.T @ G, G.sum(axis=0))
         if l > 0:
             dH = G @ Wr.T
-            G = r(dH * act_grad(cache[l - 1][2]))
+            G = r(dH * act_grad(cache[l - 1][2], kind))
     return loss, grads
 
 

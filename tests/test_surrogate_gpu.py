@@ -19,12 +19,16 @@ import mlp_oracle as M  # noqa: E402
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(scope="module")
-def net():
+KINDS = {"gelu": corefl.Surrogate.ACT_GELU, "hswish": corefl.Surrogate.ACT_HARD_SWISH}
+
+
+@pytest.fixture(scope="module", params=["gelu", "hswish"])
+def net(request):
     s = corefl.Surrogate(batch_capacity=4096)
+    s.set_activation(KINDS[request.param])
     params = M.init_params(5)
     s.set_params(params)
-    return s, params
+    return s, params, request.param
 
 
 def rel(a, b):
@@ -37,7 +41,7 @@ def cosine(a, b):
 
 
 def test_param_round_trip(net):
-    s, params = net
+    s, params, _ = net
     got = s.get_params()
     for (W, b), (Wg, bg) in zip(params, got):
         np.testing.assert_array_equal(Wg, W.astype(np.float32))
@@ -45,18 +49,19 @@ def test_param_round_trip(net):
 
 
 def test_forward_matches_oracle_and_fused_equals_unfused(net):
-    s, params = net
+    s, params, kind = net
     X = np.random.default_rng(1).standard_normal((3000, 12)).astype(np.float32)
     fused = s.forward(X, fused=True)
     unfused = s.forward(X, fused=False)
     np.testing.assert_array_equal(fused, unfused)
-    ref, _ = M.forward([(W.astype(np.float32).astype(np.float64), b) for W, b in params], X.astype(np.float64))
+    ref, _ = M.forward([(W.astype(np.float32).astype(np.float64), b) for W, b in params], X.astype(np.float64),
+                       kind=kind)
     assert rel(fused, ref) < 1e-2
     assert np.max(np.abs(fused - ref)) < 0.05 * np.max(np.abs(ref))
 
 
 def test_fused_handles_ragged_sizes(net):
-    s, _ = net
+    s, _, _ = net
     X = np.random.default_rng(2).standard_normal((300, 12)).astype(np.float32)
     full = s.forward(X)
     for n in (1, 31, 33, 127, 129, 255):
@@ -64,13 +69,13 @@ def test_fused_handles_ragged_sizes(net):
 
 
 def test_gradients_match_oracle(net):
-    s, params = net
+    s, params, kind = net
     rng = np.random.default_rng(3)
     X = rng.standard_normal((1024, 12)).astype(np.float32)
     Y = rng.standard_normal((1024, 10)).astype(np.float32)
     loss, g = s.grad(X, Y)
     p64 = [(W.astype(np.float32).astype(np.float64), b) for W, b in params]
-    loss_ref, g_ref = M.loss_and_grads(p64, X.astype(np.float64), Y.astype(np.float64))
+    loss_ref, g_ref = M.loss_and_grads(p64, X.astype(np.float64), Y.astype(np.float64), kind=kind)
     assert abs(loss - loss_ref) / loss_ref < 1e-4
     for l, ((gW, gb), (rW, rb)) in enumerate(zip(g, g_ref)):
         assert rel(gW, rW) < 1e-2 and cosine(gW, rW) > 0.9999, l
@@ -115,7 +120,7 @@ def chem_setup(n, seed):
 
 
 def test_surrogate_step_conserves_mass_elements_and_energy(net):
-    s, _ = net
+    s, _, _ = net
     ch, rho, T, Y = chem_setup(4000, 21)
     s.set_scaling(np.zeros(12), np.ones(12), np.zeros(10), np.full(10, 50.0))
     s.set_gate(0, 1e9, 0, 1e12, 0, 1, 1e9)  # everything accepted
@@ -138,7 +143,7 @@ def test_surrogate_step_conserves_mass_elements_and_energy(net):
 
 
 def test_closed_gate_falls_back_to_exact(net):
-    s, _ = net
+    s, _, _ = net
     ch, rho, T, Y = chem_setup(2000, 22)
     s.set_scaling(np.zeros(12), np.ones(12), np.zeros(10), np.full(10, 50.0))
     s.set_gate(0, 100, 0, 1e12, 0, 1, 1e9)  # no cell has T <= 100 K
@@ -155,7 +160,7 @@ def test_closed_gate_falls_back_to_exact(net):
 
 
 def test_partial_gate_mixes_paths(net):
-    s, _ = net
+    s, _, _ = net
     ch, rho, T, Y = chem_setup(3000, 23)
     s.set_scaling(np.zeros(12), np.ones(12), np.zeros(10), np.full(10, 50.0))
     s.set_gate(0, 1500, 0, 1e12, 0, 1, 1e9)
@@ -176,7 +181,7 @@ def test_transforms_match_host_definitions(net):
     """Features Y^a and the signed-logarithm targets: the fused chemistry step with a network whose output
     layer is zero reproduces the host's inverse transform of the output bias exactly in the rates."""
     import surrogate_train as ST
-    s, params = net
+    s, params, _ = net
     ch, rho, T, Y = chem_setup(500, 24)
     zero = [(np.zeros_like(W), np.zeros_like(b)) for W, b in params]
     zero[-1] = (np.zeros_like(params[-1][0]), np.zeros(10))

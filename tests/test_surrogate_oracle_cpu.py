@@ -5,6 +5,7 @@ import os
 import sys
 
 import numpy as np
+import pytest
 
 sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
 import mlp_oracle as M  # noqa: E402
@@ -19,25 +20,40 @@ def test_bf16_rounding():
     assert abs(got[6] - 1e-3) / 1e-3 < 2 ** -8
 
 
-def test_activation_and_grad():
+def test_hard_swish_and_grad():
     z = np.linspace(-6.05, 5.95, 101)  # no sample at the kinks z = +-3
     assert np.min(np.abs(np.abs(z) - 3.0)) > 1e-3
     h = 1e-6
-    fd = (M.act(z + h) - M.act(z - h)) / (2 * h)
-    np.testing.assert_allclose(M.act_grad(z), fd, rtol=1e-7, atol=1e-9)
+    fd = (M.hswish(z + h) - M.hswish(z - h)) / (2 * h)
+    np.testing.assert_allclose(M.hswish_grad(z), fd, rtol=1e-7, atol=1e-9)
     # closed forms: 0 at and below -3, identity at and above 3, z (z + 3) / 6 between
-    assert M.act(0.0) == 0.0 and M.act(-3.0) == 0.0 and M.act(3.0) == 3.0
-    assert M.act(-5.0) == 0.0 and M.act(10.0) == 10.0
-    assert abs(M.act(1.0) - 2.0 / 3.0) < 1e-15 and abs(M.act(-1.5) + 0.375) < 1e-15
+    assert M.hswish(0.0) == 0.0 and M.hswish(-3.0) == 0.0 and M.hswish(3.0) == 3.0
+    assert M.hswish(-5.0) == 0.0 and M.hswish(10.0) == 10.0
+    assert abs(M.hswish(1.0) - 2.0 / 3.0) < 1e-15 and abs(M.hswish(-1.5) + 0.375) < 1e-15
 
 
-def test_backward_matches_finite_differences():
+def test_gelu_and_grad():
+    """The tanh form against the exact GELU z Phi(z) (within its published 1e-3 accuracy), its derivative against
+    finite differences, and its limits."""
+    from scipy.special import erf
+    z = np.linspace(-6.0, 6.0, 241)
+    exact = 0.5 * z * (1 + erf(z / np.sqrt(2)))
+    assert np.max(np.abs(M.gelu(z) - exact)) < 1e-3
+    h = 1e-6
+    fd = (M.gelu(z + h) - M.gelu(z - h)) / (2 * h)
+    np.testing.assert_allclose(M.gelu_grad(z), fd, rtol=1e-7, atol=1e-9)
+    assert M.gelu(0.0) == 0.0 and abs(M.gelu(20.0) - 20.0) < 1e-12 and abs(M.gelu(-20.0)) < 1e-12
+    assert M.act(1.3) == M.gelu(1.3) and M.act(1.3, "hswish") == M.hswish(1.3)
+
+
+@pytest.mark.parametrize("kind", ["gelu", "hswish"])
+def test_backward_matches_finite_differences(kind):
     sizes = (12, 16, 16, 16, 16, 10)
     params = M.init_params(1, sizes)
     rng = np.random.default_rng(2)
     X = rng.standard_normal((7, 12))
     Y = rng.standard_normal((7, 10))
-    _, grads = M.loss_and_grads(params, X, Y, emulate=False)
+    _, grads = M.loss_and_grads(params, X, Y, emulate=False, kind=kind)
     h = 1e-6
     for l in range(len(params)):
         W, b = params[l]
@@ -47,7 +63,7 @@ def test_backward_matches_finite_differences():
             Wm[i, j] -= h
             pp = list(params); pp[l] = (Wp, b)
             pm = list(params); pm[l] = (Wm, b)
-            fd = (M.loss_and_grads(pp, X, Y, False)[0] - M.loss_and_grads(pm, X, Y, False)[0]) / (2 * h)
+            fd = (M.loss_and_grads(pp, X, Y, False, kind)[0] - M.loss_and_grads(pm, X, Y, False, kind)[0]) / (2 * h)
             assert abs(grads[l][0][i, j] - fd) <= 1e-6 * max(1e-3, abs(fd)), (l, i, j)
         for j in [0, b.size - 1]:
             bp, bm = b.copy(), b.copy()
@@ -55,7 +71,7 @@ def test_backward_matches_finite_differences():
             bm[j] -= h
             pp = list(params); pp[l] = (W, bp)
             pm = list(params); pm[l] = (W, bm)
-            fd = (M.loss_and_grads(pp, X, Y, False)[0] - M.loss_and_grads(pm, X, Y, False)[0]) / (2 * h)
+            fd = (M.loss_and_grads(pp, X, Y, False, kind)[0] - M.loss_and_grads(pm, X, Y, False, kind)[0]) / (2 * h)
             assert abs(grads[l][1][j] - fd) <= 1e-6 * max(1e-3, abs(fd)), (l, j)
 
 
