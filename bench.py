@@ -156,11 +156,12 @@ def reactive_bench(n=129, steps=50, warmup=5):
     return out
 
 
-def surrogate_bench(n_states=2000000, train_steps=30000, n_eval=256, dt=5e-8,
+def surrogate_bench(n_states=4000000, train_steps=30000, n_eval=256, dt=5e-8, act="gelu",
                     gates=((0.5, 1e30), (0.5, 20.0), (0.5, 5.0), (0.5, 2.0), (0.35, 3.0), (0.35, 2.0), (0.25, 3.0),
                            (0.25, 2.0), (1.0, 10.0))):
-    """Train the surrogate on exact-integrator states, then time one chemistry sub-step of n_eval^3 cells by
-    surrogate (with fallback, for each temperature gate) and by the exact integrator (autotuned launch)."""
+    """Train the surrogate (12 -> 4 x 128 -> 10; act "gelu" as specified, or "hswish") on exact-integrator states,
+    then time one chemistry sub-step of n_eval^3 cells by surrogate (with fallback, for each temperature gate) and
+    by the exact integrator (autotuned launch)."""
     import chem_cases as CC
     import corefl
     import surrogate_train as ST
@@ -171,6 +172,7 @@ def surrogate_bench(n_states=2000000, train_steps=30000, n_eval=256, dt=5e-8,
     t_data = time.perf_counter() - t0
     log(f"bench: surrogate dataset {data['T'].size} states in {t_data:.1f} s")
     sur = corefl.Surrogate(8192)
+    sur.set_activation(corefl.Surrogate.ACT_GELU if act == "gelu" else corefl.Surrogate.ACT_HARD_SWISH)
     t0 = time.perf_counter()
     sc, loss = ST.train(sur, data, mw, steps=train_steps)
     t_train = time.perf_counter() - t0
@@ -214,6 +216,8 @@ def surrogate_bench(n_states=2000000, train_steps=30000, n_eval=256, dt=5e-8,
     pick = max(ok, key=lambda r: r["speedup"]) if ok else min(sweep, key=lambda r: r["max_abs_dT_K"])
     out = {"workload": f"{n_eval}^3 cells of the hot-kernel H2/air box after 20 us of ignition, one chemistry "
                        f"sub-step of {dt:g} s",
+           "network": "12 -> 4 x 128 -> 10, " + ("GELU (tanh form)" if act == "gelu" else "hard swish") +
+                      ", BF16 weights, FP32 accumulation",
            "speedup": pick["speedup"], "gate_dT_K": pick["gate_dT_K"], "gate_dT_step_K": pick["gate_dT_step_K"],
            "accuracy_ok": bool(ok), "meets_targets": bool(ok) and pick["speedup"] >= 5.0,
            "max_abs_dY": pick["max_abs_dY"], "max_abs_dT_K": pick["max_abs_dT_K"],
@@ -357,7 +361,11 @@ def main():
     ap.add_argument("--only", choices=("reactive", "surrogate", "mixing"),
                     help="run one secondary section alone and print its JSON object (profiling runs)")
     ap.add_argument("--train-steps", type=int, default=30000)
-    ap.add_argument("--surrogate-states", type=int, default=2000000)
+    ap.add_argument("--surrogate-states", type=int, default=4000000)
+    ap.add_argument("--surrogate-act", choices=("gelu", "hswish"), default="gelu",
+                    help="hidden activation of the surrogate (GELU is the specified network)")
+    ap.add_argument("--surrogate-hardswish", action="store_true",
+                    help="also train and time the hard-swish network, reported as an extra object")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device copy (reference point for the HBM byte counters)")
     args = ap.parse_args()
@@ -385,7 +393,8 @@ def main():
               flush=True)
         return
     if args.only == "surrogate":
-        print(json.dumps(surrogate_bench(n_states=args.surrogate_states, train_steps=args.train_steps)), flush=True)
+        print(json.dumps(surrogate_bench(n_states=args.surrogate_states, train_steps=args.train_steps,
+                                         act=args.surrogate_act)), flush=True)
         return
     n = args.n
     log(f"bench: building TGV wavy grid {n}^3")
@@ -432,7 +441,9 @@ def main():
     S.close()
     reactive = None if args.no_reactive else reactive_bench(steps=args.reactive_steps)
     surrogate = None if args.no_surrogate else surrogate_bench(n_states=args.surrogate_states,
-                                                               train_steps=args.train_steps)
+                                                               train_steps=args.train_steps, act=args.surrogate_act)
+    surrogate_hs = (surrogate_bench(n_states=args.surrogate_states, train_steps=args.train_steps, act="hswish")
+                    if args.surrogate_hardswish and not args.no_surrogate else None)
     mixing = None if args.no_mixing else mixing_layer_bench(args.mixing_blocks, tuple(args.mixing_shape),
                                                              args.mixing_steps)
     cpu = None
@@ -456,6 +467,8 @@ def main():
         "surrogate": surrogate,
         "mixing_layer": mixing,
     }
+    if surrogate_hs is not None:
+        line["surrogate_hard_swish"] = surrogate_hs  # an alternative network, not the specified one
     print(json.dumps(line), flush=True)
 
 

@@ -15,6 +15,7 @@
 
 #include "chem.hpp"
 #include "chem_ctx.hpp"
+#include "surrogate.hpp"
 #include "chem_kernels.hpp"
 
 namespace {
@@ -78,6 +79,13 @@ void chem_ensure_device(corefl_chem *c) {
   HIPCHK(hipMalloc(&c->counter, sizeof(int)));
   HIPCHK(hipMalloc(&c->list_count, sizeof(int)));
   HIPCHK(hipMemcpy(c->mech_dev, &c->mech, sizeof(corefl::ChemMech), hipMemcpyHostToDevice));
+  // the surrogate's projection tables, ready before any sub-step is captured into a graph
+  corefl::ElementMatrix em{};
+  std::vector<double> table;
+  if (corefl::projection_tables(c->mech, em, table)) {
+    HIPCHK(hipMalloc(&c->proj_inv, sizeof(double) * table.size()));
+    HIPCHK(hipMemcpy(c->proj_inv, table.data(), sizeof(double) * table.size(), hipMemcpyHostToDevice));
+  }
   int cus = 0;
   HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0));
   c->cfg = corefl::ChemLaunch{3, 64, 4 * cus, 4}; // until autotuned: one-lane queue, stiff cells handed over
@@ -139,6 +147,7 @@ corefl_chem::~corefl_chem() {
   if (mech_dev) (void)hipFree(mech_dev);
   if (counter) (void)hipFree(counter);
   if (list_count) (void)hipFree(list_count);
+  if (proj_inv) (void)hipFree(proj_inv);
   if (stream) (void)hipStreamDestroy(stream);
 }
 

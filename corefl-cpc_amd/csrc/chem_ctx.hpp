@@ -31,6 +31,7 @@ struct corefl_chem {
   float *sout = nullptr;           // network outputs [cap][10],
   unsigned char *sflag = nullptr;  // input-range flags [cap]
   int *list_count = nullptr; // its device-side length
+  double *proj_inv = nullptr; // the surrogate projection's held-species inverse table (ElementMatrix::inv_table)
   corefl::ChemLaunch cfg{1, 64, 0};
   bool tuned = false;
   ~corefl_chem();

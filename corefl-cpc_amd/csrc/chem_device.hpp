@@ -17,26 +17,29 @@ constexpr double kRu = 8314.462618, kPatm = 101325.0;
 constexpr double kThird = 1.0 / 3.0, kSixth = 1.0 / 6.0, kTwelfth = 1.0 / 12.0;
 
 // h_k and cp_k per unit mass, linear extrapolation outside [t_low, t_high] (Thermo.cu:470-520).
-template<int NS> __device__ __forceinline__ void h_cp(const ChemMech &m, double t, double *h, double *cp) {
+// h and cp of species i at t (the body of h_cp for one species).
+__device__ __forceinline__ void h_cp_one(const ChemMech &m, int i, double t, double &h, double &cp) {
   const double t2 = t * t, t3 = t2 * t, t4 = t3 * t, t5 = t4 * t;
-#pragma unroll
-  for (int i = 0; i < NS; ++i) {
-    const double R = kRu * m.imw[i];
-    const bool below = t < m.t_low[i], above = t > m.t_high[i];
-    if (below || above) {
-      const double tt = below ? m.t_low[i] : m.t_high[i];
-      const double *c = below ? m.lo[i] : m.hi[i];
-      const double tt2 = tt * tt, tt3 = tt2 * tt, tt4 = tt3 * tt, tt5 = tt4 * tt;
-      const double cc = c[0] + c[1] * tt + c[2] * tt2 + c[3] * tt3 + c[4] * tt4;
-      const double hh = c[0] * tt + 0.5 * c[1] * tt2 + c[2] * tt3 * kThird + 0.25 * c[3] * tt4 + 0.2 * c[4] * tt5 + c[5];
-      h[i] = (hh + cc * (t - tt)) * R;
-      cp[i] = cc * R;
-    } else {
-      const double *c = t < m.t_mid[i] ? m.lo[i] : m.hi[i];
-      h[i] = (c[0] * t + 0.5 * c[1] * t2 + c[2] * t3 * kThird + 0.25 * c[3] * t4 + 0.2 * c[4] * t5 + c[5]) * R;
-      cp[i] = (c[0] + c[1] * t + c[2] * t2 + c[3] * t3 + c[4] * t4) * R;
-    }
+  const double R = kRu * m.imw[i];
+  const bool below = t < m.t_low[i], above = t > m.t_high[i];
+  if (below || above) {
+    const double tt = below ? m.t_low[i] : m.t_high[i];
+    const double *c = below ? m.lo[i] : m.hi[i];
+    const double tt2 = tt * tt, tt3 = tt2 * tt, tt4 = tt3 * tt, tt5 = tt4 * tt;
+    const double cc = c[0] + c[1] * tt + c[2] * tt2 + c[3] * tt3 + c[4] * tt4;
+    const double hh = c[0] * tt + 0.5 * c[1] * tt2 + c[2] * tt3 * kThird + 0.25 * c[3] * tt4 + 0.2 * c[4] * tt5 + c[5];
+    h = (hh + cc * (t - tt)) * R;
+    cp = cc * R;
+  } else {
+    const double *c = t < m.t_mid[i] ? m.lo[i] : m.hi[i];
+    h = (c[0] * t + 0.5 * c[1] * t2 + c[2] * t3 * kThird + 0.25 * c[3] * t4 + 0.2 * c[4] * t5 + c[5]) * R;
+    cp = (c[0] + c[1] * t + c[2] * t2 + c[3] * t3 + c[4] * t4) * R;
   }
+}
+
+template<int NS> __device__ __forceinline__ void h_cp(const ChemMech &m, double t, double *h, double *cp) {
+#pragma unroll
+  for (int i = 0; i < NS; ++i) h_cp_one(m, i, t, h[i], cp[i]);
 }
 
 // g_k / (R T), clamped to the fit range (Thermo.cu:574-598).  lnT = log(t): one logarithm serves every
@@ -264,13 +267,15 @@ template<int NS> __device__ double update_t(const ChemMech &m, double t, const d
   const double R = kRu * imw;
   double err = 1e+6;
   for (int it = 0; it < 100 && err > 1e-8; ++it) {
-    double h[NS], cp[NS];
-    h_cp<NS>(m, t, h, cp);
+    // h_cp species by species, summed as they come (the same operations in the same order as summing the
+    // arrays h_cp fills, without holding them)
     double cpt = 0, hh = 0;
 #pragma unroll
     for (int l = 0; l < NS; ++l) {
-      cpt += cp[l] * yk[l];
-      hh += h[l] * yk[l];
+      double h1, cp1;
+      h_cp_one(m, l, t, h1, cp1);
+      cpt += cp1 * yk[l];
+      hh += h1 * yk[l];
     }
     const double t1 = t - (hh - R * t - E) / (cpt - R);
     err = fabs(1 - t1 / t);

@@ -1,8 +1,10 @@
-// Chemistry surrogate: a 12 -> 4x128 (hard swish) -> 10 MLP with BF16 weights and FP32 accumulation on the
+// Chemistry surrogate: a 12 -> 4x128 (GELU or hard swish) -> 10 MLP with BF16 weights and FP32 accumulation on the
 // gfx950 matrix cores.  Training (forward, backward, Adam) runs as tiled MFMA GEMMs with fused epilogues;
 // inference runs as one persistent kernel holding every weight in LDS, fused with the feature scaling, the
 // element-conserving projection, the temperature recovery and the fallback gate.
 #pragma once
+
+#include <vector>
 
 #include <hip/hip_runtime.h>
 
@@ -52,7 +54,15 @@ struct ElementMatrix {
   // inverse of the projection's normal matrix E E^T (rows e >= ne: identity) when no species is held at
   // zero, the common case
   double M0inv[4][4];
+  // the same for every set of species held at zero (bit k: species k): 512 entries of 17 doubles, the 4 x 4
+  // inverse of the normal matrix over the other species and 1 (0 where that matrix is singular)
+  const double *inv_table;
 };
+
+constexpr int kProjTableEntries = 512, kProjTableStride = 17;
+// E, M0inv and the held-species inverse table of a mechanism (host); false if it has more than 4 elements or
+// E E^T is singular.
+bool projection_tables(const ChemMech &mech, ElementMatrix &em, std::vector<double> &table);
 
 // C = A B with A(m,k) = A[m*sam + k*sak], B(k,n) = B[k*sbk + n*sbn] in BF16; FP32 accumulation.
 enum GemmEpilogue {

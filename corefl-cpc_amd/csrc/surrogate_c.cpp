@@ -188,24 +188,19 @@ void need_params(const corefl_surrogate *s) {
 
 namespace corefl {
 
-void surrogate_substep(corefl_chem *c, corefl_surrogate *s, const ChemBatch &b, double rtol, double atol,
-                       int max_attempts, hipStream_t st) {
-  need_params(s);
-  if (!s->have_scaling || !s->have_gate) throw std::runtime_error("surrogate scaling and gate must be set first");
-  const ChemMech &mech = c->mech;
-  if (mech.ne > 4) throw std::runtime_error("surrogate projection: at most 4 elements are supported");
-  ElementMatrix em{};
-  em.ne = mech.ne;
-  for (int e = 0; e < mech.ne; ++e)
-    for (int k = 0; k < mech.ns; ++k) em.E[e][k] = mech.comp[k][e] / mech.mw[k];
-  // (E E^T)^-1 by Gauss-Jordan elimination with partial pivoting
+namespace {
+
+// Inverse of the normal matrix E E^T over the species not in `held` (rows and columns e >= ne: identity) by
+// Gauss-Jordan elimination with partial pivoting; false if it is singular.
+bool normal_inverse(const ElementMatrix &em, int ns, unsigned held, double (&inv)[4][4]) {
   double A[4][8] = {};
   for (int e = 0; e < 4; ++e) {
     for (int f = 0; f < 4; ++f) {
       double v = e == f ? 1.0 : 0.0;
       if (e < em.ne && f < em.ne) {
         v = 0;
-        for (int k = 0; k < mech.ns; ++k) v += em.E[e][k] * em.E[f][k];
+        for (int k = 0; k < ns; ++k)
+          if (!((held >> k) & 1u)) v += em.E[e][k] * em.E[f][k];
       }
       A[e][f] = v;
     }
@@ -215,10 +210,10 @@ void surrogate_substep(corefl_chem *c, corefl_surrogate *s, const ChemBatch &b, 
     int p = k;
     for (int r = k + 1; r < 4; ++r)
       if (std::fabs(A[r][k]) > std::fabs(A[p][k])) p = r;
-    if (!(std::fabs(A[p][k]) > 0)) throw std::runtime_error("surrogate projection: singular element matrix");
+    if (!(std::fabs(A[p][k]) > 1e-300)) return false;
     for (int c = 0; c < 8; ++c) std::swap(A[k][c], A[p][c]);
-    const double inv = 1.0 / A[k][k];
-    for (int c = 0; c < 8; ++c) A[k][c] *= inv;
+    const double iv = 1.0 / A[k][k];
+    for (int c = 0; c < 8; ++c) A[k][c] *= iv;
     for (int r = 0; r < 4; ++r) {
       if (r == k) continue;
       const double f = A[r][k];
@@ -226,7 +221,41 @@ void surrogate_substep(corefl_chem *c, corefl_surrogate *s, const ChemBatch &b, 
     }
   }
   for (int e = 0; e < 4; ++e)
-    for (int f = 0; f < 4; ++f) em.M0inv[e][f] = A[e][4 + f];
+    for (int f = 0; f < 4; ++f) inv[e][f] = A[e][4 + f];
+  return true;
+}
+
+} // namespace
+
+bool projection_tables(const ChemMech &mech, ElementMatrix &em, std::vector<double> &table) {
+  if (mech.ne > 4 || mech.ns > 9) return false;
+  em = ElementMatrix{};
+  em.ne = mech.ne;
+  for (int e = 0; e < mech.ne; ++e)
+    for (int k = 0; k < mech.ns; ++k) em.E[e][k] = mech.comp[k][e] / mech.mw[k];
+  if (!normal_inverse(em, mech.ns, 0u, em.M0inv)) return false;
+  table.assign((size_t)kProjTableEntries * kProjTableStride, 0.0);
+  for (unsigned held = 0; held < (unsigned)kProjTableEntries; ++held) {
+    double inv[4][4];
+    double *t = table.data() + (size_t)held * kProjTableStride;
+    if (!normal_inverse(em, mech.ns, held, inv)) continue;
+    for (int e = 0; e < 4; ++e)
+      for (int f = 0; f < 4; ++f) t[4 * e + f] = inv[e][f];
+    t[16] = 1.0;
+  }
+  return true;
+}
+
+void surrogate_substep(corefl_chem *c, corefl_surrogate *s, const ChemBatch &b, double rtol, double atol,
+                       int max_attempts, hipStream_t st) {
+  need_params(s);
+  if (!s->have_scaling || !s->have_gate) throw std::runtime_error("surrogate scaling and gate must be set first");
+  const ChemMech &mech = c->mech;
+  if (mech.ne > 4) throw std::runtime_error("surrogate projection: at most 4 elements are supported");
+  ElementMatrix em{};
+  std::vector<double> unused;
+  if (!projection_tables(mech, em, unused)) throw std::runtime_error("surrogate projection: singular element matrix");
+  em.inv_table = c->proj_inv;
   // dt of the launch: only used when the batch has neither per-cell nor device-side lengths
   const double dt = s->step_dt;
   const SurrogateWork w{c->sfeat, c->sout, c->sflag};

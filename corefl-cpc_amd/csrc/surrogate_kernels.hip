@@ -260,30 +260,6 @@ __device__ __forceinline__ bool chem_features(const FusedArgs &a, long c, float 
   return T >= g.t_min && T <= g.t_max && p >= g.p_min && p <= g.p_max && dt >= g.dt_min && dt <= g.dt_max;
 }
 
-// Solve the symmetric positive definite 4 x 4 system M x = r in place (elimination without pivoting, which
-// is stable for SPD matrices); false if a pivot is not positive.
-__device__ __forceinline__ bool spd_solve4(double (&M)[4][4], double (&r)[4]) {
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    if (!(M[k][k] > 1e-300)) return false;
-    const double inv = 1.0 / M[k][k];
-#pragma unroll
-    for (int i = k + 1; i < 4; ++i) {
-      const double f = M[i][k] * inv;
-#pragma unroll
-      for (int j = k; j < 4; ++j) M[i][j] -= f * M[k][j];
-      r[i] -= f * r[k];
-    }
-  }
-#pragma unroll
-  for (int k = 3; k >= 0; --k) {
-#pragma unroll
-    for (int j = k + 1; j < 4; ++j) r[k] -= M[k][j] * r[j];
-    r[k] /= M[k][k];
-  }
-  return true;
-}
-
 // Network output q -> rate (inverse of the signed-logarithm target transform).  FP32: the network's
 // outputs carry BF16-level error, and r0 (e^|v| - 1) is taken with the hardware exp2 (its absolute error
 // near v = 0, r0 * 1e-7, is far below the rates that matter).
@@ -296,9 +272,39 @@ __device__ __forceinline__ double rate_of(const SurrogateScaling &s, int q, floa
 
 // Network outputs o[10] of cell c -> projected mass fractions and energy-consistent temperature; writes the
 // cell if the gate passes, else appends it to the fallback list.
-__device__ void chem_finish(const FusedArgs &a, long c, const float *o, bool in_range) {
+// The NASA-7 data update_t reads, staged in LDS by each workgroup of k_chem_post.
+struct ThermoLds {
+  double lo[9][7], hi[9][7];
+  double t_low[9], t_mid[9], t_high[9], R[9], imw[9];
+};
+
+// h and cp of species i at t from the LDS table: chemdev::h_cp_one's operations in its order.
+__device__ __forceinline__ void h_cp_lds(const ThermoLds &th, int i, double t, double &h, double &cp) {
+  using chemdev::kThird;
+  const double t2 = t * t, t3 = t2 * t, t4 = t3 * t, t5 = t4 * t;
+  const double R = th.R[i];
+  const bool below = t < th.t_low[i], above = t > th.t_high[i];
+  if (below || above) {
+    const double tt = below ? th.t_low[i] : th.t_high[i];
+    const double *c = below ? th.lo[i] : th.hi[i];
+    const double tt2 = tt * tt, tt3 = tt2 * tt, tt4 = tt3 * tt, tt5 = tt4 * tt;
+    const double cc = c[0] + c[1] * tt + c[2] * tt2 + c[3] * tt3 + c[4] * tt4;
+    const double hh = c[0] * tt + 0.5 * c[1] * tt2 + c[2] * tt3 * kThird + 0.25 * c[3] * tt4 + 0.2 * c[4] * tt5 + c[5];
+    h = (hh + cc * (t - tt)) * R;
+    cp = cc * R;
+  } else {
+    const double *c = t < th.t_mid[i] ? th.lo[i] : th.hi[i];
+    h = (c[0] * t + 0.5 * c[1] * t2 + c[2] * t3 * kThird + 0.25 * c[3] * t4 + 0.2 * c[4] * t5 + c[5]) * R;
+    cp = (c[0] + c[1] * t + c[2] * t2 + c[3] * t3 + c[4] * t4) * R;
+  }
+}
+
+__device__ void chem_finish(const FusedArgs &a, long c, const float *o, bool in_range, const ThermoLds &th) {
+  // The projection's passes and the Newton iteration stay rolled up, the held-species inverses come from a
+  // table and the thermo coefficients from LDS, so that the register footprint (122 VGPRs) leaves four waves
+  // per SIMD to hide the memory latency (16.7M cells: 1.13 ms, against 1.71 ms with the passes unrolled, the
+  // held-species system solved in registers and the coefficients held in registers at 254 VGPRs).
   const ChemBatch &b = a.b;
-  const ChemMech &m = *a.mech;
   const double rho = b.rho[c], T = b.T[c], dt = cell_dt(b, c, a.dt);
   const double irho = 1.0 / rho;
   double y0[9], dy[9];
@@ -313,6 +319,7 @@ __device__ void chem_finish(const FusedArgs &a, long c, const float *o, bool in_
   unsigned clamp = 0;
   bool ok = in_range;
   double yn[9];
+#pragma unroll 1
   for (int pass = 0; pass < 4 && ok; ++pass) {
     double res[4];
 #pragma unroll
@@ -322,34 +329,22 @@ __device__ void chem_finish(const FusedArgs &a, long c, const float *o, bool in_
       for (int k = 0; k < 9; ++k) r += a.em.E[e][k] * ((clamp >> k) & 1u ? -y0[k] : dy[k]);
       res[e] = r;
     }
-    if (clamp == 0) {
-      // no species held at zero: the normal matrix is the mechanism's, inverted on the host
-      double x[4];
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        x[e] = 0.0;
-#pragma unroll
-        for (int f = 0; f < 4; ++f) x[e] += a.em.M0inv[e][f] * res[f];
-      }
-#pragma unroll
-      for (int e = 0; e < 4; ++e) res[e] = x[e];
-    } else {
-      double M[4][4];
-#pragma unroll
-      for (int e = 0; e < 4; ++e)
-#pragma unroll
-        for (int f = 0; f < 4; ++f) {
-          double s = 0.0;
-#pragma unroll
-          for (int k = 0; k < 9; ++k)
-            if (!((clamp >> k) & 1u)) s += a.em.E[e][k] * a.em.E[f][k];
-          M[e][f] = (e >= a.em.ne || f >= a.em.ne) ? (e == f ? 1.0 : 0.0) : s;
-        }
-      if (!spd_solve4(M, res)) {
-        ok = false;
-        break;
-      }
+    // the inverse normal matrix over the species not held at zero, inverted on the host: the mechanism's own
+    // when none is held, else the table's entry for the held set (singular: the cell falls back)
+    const double *Mi = clamp == 0 ? &a.em.M0inv[0][0] : a.em.inv_table + (long)clamp * kProjTableStride;
+    if (clamp != 0 && !(Mi[16] > 0)) {
+      ok = false;
+      break;
     }
+    double x[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      x[e] = 0.0;
+#pragma unroll
+      for (int f = 0; f < 4; ++f) x[e] += Mi[4 * e + f] * res[f];
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) res[e] = x[e];
     unsigned neg = 0;
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
@@ -369,7 +364,30 @@ __device__ void chem_finish(const FusedArgs &a, long c, const float *o, bool in_
   }
   double t_new = t_net;
   if (ok) {
-    t_new = chemdev::update_t<9>(m, fmin(fmax(t_net, 200.0), 5000.0), yn, b.e[c]);
+    // chemdev::update_t with the coefficients read from LDS in every iteration (the same operations in the same
+    // order; the empty asm statement with a memory clobber keeps the compiler from hoisting those reads, which
+    // would hold every coefficient in registers)
+    t_new = fmin(fmax(t_net, 200.0), 5000.0);
+    double imw = 0;
+#pragma unroll
+    for (int l = 0; l < 9; ++l) imw += yn[l] * th.imw[l];
+    const double R = chemdev::kRu * imw, E = b.e[c];
+    double err = 1e+6;
+#pragma unroll 1
+    for (int it = 0; it < 100 && err > 1e-8; ++it) {
+      asm volatile("" ::: "memory");
+      double cpt = 0, hh = 0;
+#pragma unroll
+      for (int l = 0; l < 9; ++l) {
+        double h1, cp1;
+        h_cp_lds(th, l, t_new, h1, cp1);
+        cpt += cp1 * yn[l];
+        hh += h1 * yn[l];
+      }
+      const double t1 = t_new - (hh - R * t_new - E) / (cpt - R);
+      err = fabs(1 - t1 / t_new);
+      t_new = t1;
+    }
     ok = isfinite(t_new) && fabs(t_new - t_net) <= a.gate.dT_max && fabs(t_new - T) <= a.gate.dT_step;
   }
   if (ok) {
@@ -387,12 +405,18 @@ __device__ void chem_finish(const FusedArgs &a, long c, const float *o, bool in_
 // Inference MLP, one pass over the batch.  The product of each layer is formed transposed, out^T = W^T h^T:
 // the weights (LDS-resident for the whole persistent launch) are the MFMA's A operand and the activations
 // its B operand.  The accumulator layout of out^T (lane = cell, 16 neurons per lane) then maps onto the next
-// layer's B operand with one exchange of half a neuron group between lanes l and l + 32, so activations stay
-// in registers from the features to the outputs.  A wave runs kMlpTiles 32-cell tiles at once (each weight
-// fragment read from LDS then feeds kMlpTiles MFMAs); eight waves per workgroup, two per SIMD, overlap one
-// wave's MFMAs with another's activation arithmetic (measured on 2^20 rows with the tanh GELU: 0.276 ms at
-// one tile and eight waves, 0.370 ms at two tiles and four waves).
-constexpr int kMlpTiles = 1, kMlpWaves = 8;
+// layer's B operand with one exchange of half a neuron group between lanes l and l + 32 (v_permlane32_swap),
+// so activations stay in registers from the features to the outputs.  Each layer's accumulators start from
+// its bias through one more MFMA: the FP32 bias split exactly into three BF16 parts (A operand, k = 0..2)
+// times ones (B operand), which takes the bias off the LDS read path and the VALU.  A wave runs kMlpTiles
+// 32-cell tiles at once (each weight fragment read from LDS then feeds kMlpTiles MFMAs); twelve waves per
+// workgroup, three per SIMD (one workgroup per CU: the weights fill 125 KB of LDS), overlap one wave's MFMAs
+// with another's activation arithmetic.  Measured on 16.7M cells with the tanh GELU: 3.29 ms at twelve waves,
+// 3.37 ms at eight, 4.07 ms with the bias added on the VALU from LDS and the half-group exchange through
+// ds_bpermute; issuing each block's k-steps of the next layer right after its activation (a software
+// pipeline within the wave) measured 3.49 ms.  Two tiles at four waves measured slower than one at eight.
+constexpr int kMlpTiles = 1, kMlpWaves = 12;
+constexpr int kBiasFrags = 4 * 4 + 1; // (layer, 32-neuron block) of the hidden layers, then the output layer
 
 typedef unsigned uintx4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
@@ -403,30 +427,37 @@ __device__ __forceinline__ unsigned pack_bf16(float a, float b) {
   return __builtin_bit_cast(unsigned, __builtin_convertvector(v, bf16x2)); // round to nearest even
 }
 
-// Activation of acc + bias for one 32-neuron block, packed as BF16 pairs: p[q][0..1] holds neurons
-// 8q + 4h + 0..3 of the block (h = lane / 32).
-template<int ACT>
-__device__ __forceinline__ void act_pack(const floatx16 &acc, const float *bias, int h, unsigned (&p)[4][2]) {
+// x = hi + mid + lo exactly, each part BF16 (round to nearest even; each remainder is exact in FP32), as the
+// A-operand dwords {hi | mid << 16, lo}.
+__device__ __forceinline__ uint2 split_bf16x3(float x) {
+  const unsigned short hi = f2bf(x);
+  const float r1 = x - __uint_as_float((unsigned)hi << 16);
+  const unsigned short mid = f2bf(r1);
+  const float r2 = r1 - __uint_as_float((unsigned)mid << 16);
+  return make_uint2((unsigned)hi | ((unsigned)mid << 16), (unsigned)f2bf(r2));
+}
+
+// Activation of one 32-neuron block of accumulators (bias included), packed as BF16 pairs: p[q][0..1] holds
+// neurons 8q + 4h + 0..3 of the block (h = lane / 32).
+template<int ACT> __device__ __forceinline__ void act_pack(const floatx16 &acc, unsigned (&p)[4][2]) {
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
-    const float4 b = *reinterpret_cast<const float4 *>(&bias[8 * q + 4 * h]);
-    p[q][0] = pack_bf16(act<ACT>(acc[4 * q] + b.x), act<ACT>(acc[4 * q + 1] + b.y));
-    p[q][1] = pack_bf16(act<ACT>(acc[4 * q + 2] + b.z), act<ACT>(acc[4 * q + 3] + b.w));
+    p[q][0] = pack_bf16(act<ACT>(acc[4 * q]), act<ACT>(acc[4 * q + 1]));
+    p[q][1] = pack_bf16(act<ACT>(acc[4 * q + 2]), act<ACT>(acc[4 * q + 3]));
   }
 }
 
 // B fragments of the next layer for the two k-steps 2j, 2j + 1 covered by block j (16 neurons each): lane
-// (cell, h) needs neurons 16p + 8h + 0..7 of the block, half of which its partner lane (l ^ 32) holds.
-__device__ __forceinline__ void next_operand(const unsigned (&p)[4][2], int h, shortx8 &b0, shortx8 &b1) {
+// (cell, h) needs group 2 par + h (neurons 16 par + 8h + 0..7 of the block), whose other half its partner lane
+// (l ^ 32) holds.  v_permlane32_swap on (A = p[2 par], B = p[2 par + 1]) swaps A's upper 32 lanes with B's
+// lower 32: afterwards A holds {own low half | partner's low half of group 2 par + 1} and B {partner's upper
+// half of group 2 par | own upper half}, which is {A, B} = group 2 par + h in neuron order on every lane.
+__device__ __forceinline__ void next_operand(const unsigned (&p)[4][2], shortx8 &b0, shortx8 &b1) {
 #pragma unroll
   for (int par = 0; par < 2; ++par) {
-    const int own = 2 * par + h; // group this lane needs: its own half, and the partner's
-    const unsigned s0 = h == 0 ? p[2 * par + 1][0] : p[2 * par][0];
-    const unsigned s1 = h == 0 ? p[2 * par + 1][1] : p[2 * par][1];
-    const unsigned r0 = __shfl_xor(s0, 32), r1 = __shfl_xor(s1, 32);
-    uintx4 v;
-    if (h == 0) v = uintx4{p[own][0], p[own][1], r0, r1};
-    else v = uintx4{r0, r1, p[own][0], p[own][1]};
+    const auto s0 = __builtin_amdgcn_permlane32_swap(p[2 * par][0], p[2 * par + 1][0], false, false);
+    const auto s1 = __builtin_amdgcn_permlane32_swap(p[2 * par][1], p[2 * par + 1][1], false, false);
+    const uintx4 v = {s0[0], s1[0], s0[1], s1[1]};
     (par == 0 ? b0 : b1) = __builtin_bit_cast(shortx8, v);
   }
 }
@@ -437,20 +468,29 @@ __global__ __launch_bounds__(64 * kMlpWaves) void k_mlp(MlpDev net, const unsign
   __shared__ __attribute__((aligned(16))) unsigned short w0[128 * kS0];
   __shared__ __attribute__((aligned(16))) unsigned short wh[3][128 * kSH];
   __shared__ __attribute__((aligned(16))) unsigned short w4[32 * kSH];
-  __shared__ __attribute__((aligned(16))) float bias[4][128];
-  __shared__ __attribute__((aligned(16))) float bias4[32];
+  // bias A fragments [frag][h][row]: the three BF16 parts on the h = 0 lanes, zeros on the h = 1 lanes
+  __shared__ __attribute__((aligned(16))) uint2 bfrag[kBiasFrags][2][32];
   load_rows(w0, kS0, net.wt[0], 128, 16);
   for (int l = 1; l < 4; ++l) load_rows(wh[l - 1], kSH, net.wt[l], 128, 128);
   load_rows(w4, kSH, net.wt[4], 32, 128);
-  for (int q = threadIdx.x; q < 4 * 128; q += blockDim.x) bias[q / 128][q % 128] = net.bias[q / 128][q % 128];
-  for (int q = threadIdx.x; q < 32; q += blockDim.x) bias4[q] = q < kMlpOut ? net.bias[4][q] : 0.0f;
+  for (int q = threadIdx.x; q < kBiasFrags * 32; q += blockDim.x) {
+    const int fr = q / 32, row = q % 32;
+    const float b = fr < 16 ? net.bias[fr / 4][(fr % 4) * 32 + row] : (row < kMlpOut ? net.bias[4][row] : 0.0f);
+    bfrag[fr][0][row] = split_bf16x3(b);
+    bfrag[fr][1][row] = make_uint2(0u, 0u);
+  }
   __syncthreads();
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int cell = lane & 31, h = lane >> 5, kh = 8 * h;
+  // B operand of the bias MFMA: ones at k = 0..2 (BF16 1.0 = 0x3f80), zero elsewhere
+  const shortx8 ones = __builtin_bit_cast(shortx8, h == 0 ? uintx4{0x3f803f80u, 0x3f80u, 0u, 0u} : uintx4{0u, 0u, 0u, 0u});
+  const floatx16 zero = {};
   constexpr long kCellsPerWave = 32 * kMlpTiles;
   const long n_units = (n + kCellsPerWave - 1) / kCellsPerWave;
   const int waves = gridDim.x * kMlpWaves;
   for (long unit = (long)blockIdx.x * kMlpWaves + wave; unit < n_units; unit += waves) {
+    // the bias and weight fragments are re-read from LDS for every tile, not hoisted into registers
+    asm volatile("" ::: "memory");
     const long c0 = unit * kCellsPerWave;
     floatx16 acc[kMlpTiles][4];
     shortx8 bop[kMlpTiles][8]; // B operands of the current layer, k-steps 0..7
@@ -465,13 +505,11 @@ __global__ __launch_bounds__(64 * kMlpWaves) void k_mlp(MlpDev net, const unsign
       }
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
+        const uint2 bf = bfrag[j][h][cell];
+        const shortx8 ab = __builtin_bit_cast(shortx8, uintx4{bf.x, bf.y, 0u, 0u});
         const shortx8 a = *reinterpret_cast<const shortx8 *>(&w0[(j * 32 + cell) * kS0 + kh]);
 #pragma unroll
-        for (int t = 0; t < kMlpTiles; ++t) {
-#pragma unroll
-          for (int r = 0; r < 16; ++r) acc[t][j][r] = 0.0f;
-          acc[t][j] = mfma(a, f[t], acc[t][j]);
-        }
+        for (int t = 0; t < kMlpTiles; ++t) acc[t][j] = mfma(a, f[t], mfma(ab, ones, zero));
       }
     }
 #pragma unroll
@@ -482,17 +520,18 @@ __global__ __launch_bounds__(64 * kMlpWaves) void k_mlp(MlpDev net, const unsign
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
           unsigned p[4][2];
-          act_pack<ACT>(acc[t][j], &bias[l][32 * j], h, p);
-          next_operand(p, h, bop[t][2 * j], bop[t][2 * j + 1]);
+          act_pack<ACT>(acc[t][j], p);
+          next_operand(p, bop[t][2 * j], bop[t][2 * j + 1]);
         }
       if (l == 3) break;
       const unsigned short *w = wh[l];
 #pragma unroll
-      for (int t = 0; t < kMlpTiles; ++t)
+      for (int j = 0; j < 4; ++j) {
+        const uint2 bf = bfrag[4 * (l + 1) + j][h][cell];
+        const shortx8 ab = __builtin_bit_cast(shortx8, uintx4{bf.x, bf.y, 0u, 0u});
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
-#pragma unroll
-          for (int r = 0; r < 16; ++r) acc[t][j][r] = 0.0f;
+        for (int t = 0; t < kMlpTiles; ++t) acc[t][j] = mfma(ab, ones, zero);
+      }
 #pragma unroll
       for (int ks = 0; ks < 8; ++ks)
 #pragma unroll
@@ -504,10 +543,12 @@ __global__ __launch_bounds__(64 * kMlpWaves) void k_mlp(MlpDev net, const unsign
     }
     // output layer: 128 -> 10 (32-wide tile)
     floatx16 o[kMlpTiles];
+    {
+      const uint2 bf = bfrag[16][h][cell];
+      const shortx8 ab = __builtin_bit_cast(shortx8, uintx4{bf.x, bf.y, 0u, 0u});
 #pragma unroll
-    for (int t = 0; t < kMlpTiles; ++t)
-#pragma unroll
-      for (int r = 0; r < 16; ++r) o[t][r] = 0.0f;
+      for (int t = 0; t < kMlpTiles; ++t) o[t] = mfma(ab, ones, zero);
+    }
 #pragma unroll
     for (int ks = 0; ks < 8; ++ks) {
       const shortx8 a = *reinterpret_cast<const shortx8 *>(&w4[cell * kSH + ks * 16 + kh]);
@@ -521,7 +562,7 @@ __global__ __launch_bounds__(64 * kMlpWaves) void k_mlp(MlpDev net, const unsign
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
         const int j = 8 * (r >> 2) + 4 * h + (r & 3);
-        if (j < kMlpOut) out[c * kMlpOut + j] = o[t][r] + bias4[j];
+        if (j < kMlpOut) out[c * kMlpOut + j] = o[t][r];
       }
     }
   }
@@ -549,12 +590,27 @@ __global__ __launch_bounds__(256) void k_chem_feat(FusedArgs a, unsigned short *
 // Projection, temperature, gate and write-back of every cell from the network outputs.
 __global__ __launch_bounds__(256) void k_chem_post(FusedArgs a, const float *__restrict__ out,
                                                    const unsigned char *__restrict__ flag) {
+  __shared__ ThermoLds th;
+  const ChemMech &m = *a.mech;
+  for (int q = threadIdx.x; q < 9 * 7; q += blockDim.x) {
+    th.lo[q / 7][q % 7] = m.lo[q / 7][q % 7];
+    th.hi[q / 7][q % 7] = m.hi[q / 7][q % 7];
+  }
+  if (threadIdx.x < 9) {
+    const int i = threadIdx.x;
+    th.t_low[i] = m.t_low[i];
+    th.t_mid[i] = m.t_mid[i];
+    th.t_high[i] = m.t_high[i];
+    th.R[i] = chemdev::kRu * m.imw[i];
+    th.imw[i] = m.imw[i];
+  }
+  __syncthreads();
   const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= a.n) return;
   float o[kMlpOut];
 #pragma unroll
   for (int q = 0; q < kMlpOut; ++q) o[q] = out[c * kMlpOut + q];
-  chem_finish(a, c, o, flag[c] != 0);
+  chem_finish(a, c, o, flag[c] != 0, th);
 }
 
 __global__ void k_rows_to_feat(const float *__restrict__ X, long n, unsigned short *__restrict__ feat) {
