@@ -55,7 +55,8 @@ struct ElementMatrix {
   // zero, the common case
   double M0inv[4][4];
   // the same for every set of species held at zero (bit k: species k): 512 entries of 17 doubles, the 4 x 4
-  // inverse of the normal matrix over the other species and 1 (0 where that matrix is singular)
+  // inverse of the normal matrix over the other species and 1 + m (0 where that matrix is singular), m the
+  // bit set of the elements no free species carries (their rows and columns are the identity's)
   const double *inv_table;
 };
 

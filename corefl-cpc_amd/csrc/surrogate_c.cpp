@@ -192,12 +192,23 @@ namespace {
 
 // Inverse of the normal matrix E E^T over the species not in `held` (rows and columns e >= ne: identity) by
 // Gauss-Jordan elimination with partial pivoting; false if it is singular.
-bool normal_inverse(const ElementMatrix &em, int ns, unsigned held, double (&inv)[4][4]) {
+// Inverse of the projection's normal matrix E E^T over the species not held at zero (rows e >= ne: identity).
+// An element that no free species carries has a zero row: that row and column are replaced by the identity and
+// the element's bit set in *dropped (the projection then leaves it alone, which conserves it only when the held
+// species carried none of it: the kernel checks that its residual is zero).  False if the matrix is singular
+// otherwise.
+bool normal_inverse(const ElementMatrix &em, int ns, unsigned held, double (&inv)[4][4], unsigned *dropped) {
+  unsigned drop = 0;
+  for (int e = 0; e < em.ne; ++e) {
+    bool carried = false;
+    for (int k = 0; k < ns; ++k) carried |= !((held >> k) & 1u) && em.E[e][k] != 0.0;
+    if (!carried) drop |= 1u << e;
+  }
   double A[4][8] = {};
   for (int e = 0; e < 4; ++e) {
     for (int f = 0; f < 4; ++f) {
       double v = e == f ? 1.0 : 0.0;
-      if (e < em.ne && f < em.ne) {
+      if (e < em.ne && f < em.ne && !((drop >> e) & 1u) && !((drop >> f) & 1u)) {
         v = 0;
         for (int k = 0; k < ns; ++k)
           if (!((held >> k) & 1u)) v += em.E[e][k] * em.E[f][k];
@@ -222,6 +233,7 @@ bool normal_inverse(const ElementMatrix &em, int ns, unsigned held, double (&inv
   }
   for (int e = 0; e < 4; ++e)
     for (int f = 0; f < 4; ++f) inv[e][f] = A[e][4 + f];
+  if (dropped) *dropped = drop;
   return true;
 }
 
@@ -233,15 +245,17 @@ bool projection_tables(const ChemMech &mech, ElementMatrix &em, std::vector<doub
   em.ne = mech.ne;
   for (int e = 0; e < mech.ne; ++e)
     for (int k = 0; k < mech.ns; ++k) em.E[e][k] = mech.comp[k][e] / mech.mw[k];
-  if (!normal_inverse(em, mech.ns, 0u, em.M0inv)) return false;
+  unsigned drop0 = 0;
+  if (!normal_inverse(em, mech.ns, 0u, em.M0inv, &drop0) || drop0 != 0) return false;
   table.assign((size_t)kProjTableEntries * kProjTableStride, 0.0);
   for (unsigned held = 0; held < (unsigned)kProjTableEntries; ++held) {
     double inv[4][4];
+    unsigned drop = 0;
     double *t = table.data() + (size_t)held * kProjTableStride;
-    if (!normal_inverse(em, mech.ns, held, inv)) continue;
+    if (!normal_inverse(em, mech.ns, held, inv, &drop)) continue;
     for (int e = 0; e < 4; ++e)
       for (int f = 0; f < 4; ++f) t[4 * e + f] = inv[e][f];
-    t[16] = 1.0;
+    t[16] = 1.0 + drop;
   }
   return true;
 }

@@ -320,7 +320,8 @@ __device__ void chem_finish(const FusedArgs &a, long c, const float *o, bool in_
   bool ok = in_range;
   double yn[9];
 #pragma unroll 1
-  for (int pass = 0; pass < 4 && ok; ++pass) {
+  // each pass that does not end the loop holds at least one more species, so nine passes always decide
+  for (int pass = 0; pass < 9 && ok; ++pass) {
     double res[4];
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
@@ -330,11 +331,20 @@ __device__ void chem_finish(const FusedArgs &a, long c, const float *o, bool in_
       res[e] = r;
     }
     // the inverse normal matrix over the species not held at zero, inverted on the host: the mechanism's own
-    // when none is held, else the table's entry for the held set (singular: the cell falls back)
+    // when none is held, else the table's entry for the held set (singular: the cell falls back).  An element
+    // that only held species carry is conserved only if they held none of it (a pure stream of the mixing
+    // layer holds no hydrogen, or no oxygen): else the cell falls back.
     const double *Mi = clamp == 0 ? &a.em.M0inv[0][0] : a.em.inv_table + (long)clamp * kProjTableStride;
-    if (clamp != 0 && !(Mi[16] > 0)) {
-      ok = false;
-      break;
+    if (clamp != 0) {
+      const double tag = Mi[16];
+      bool lost = !(tag > 0);
+      const unsigned drop = lost ? 0u : (unsigned)tag - 1u;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) lost |= ((drop >> e) & 1u) && fabs(res[e]) > 1e-14;
+      if (lost) {
+        ok = false;
+        break;
+      }
     }
     double x[4];
 #pragma unroll
@@ -360,7 +370,7 @@ __device__ void chem_finish(const FusedArgs &a, long c, const float *o, bool in_
     }
     if (!neg) break;
     clamp |= neg;
-    if (pass == 3) ok = false;
+    if (pass == 8) ok = false;
   }
   double t_new = t_net;
   if (ok) {

@@ -142,6 +142,40 @@ def test_surrogate_step_conserves_mass_elements_and_energy(net):
     assert np.max(np.abs(e1 - e0) / (np.abs(e0) + 1e5)) < 1e-9
 
 
+def test_pure_streams_project_without_fallback():
+    """Cells of one pure stream (air: no hydrogen; H2/N2: no oxygen) and of their mixture, with network rates of
+    either sign: the species the rates would drive negative are held at zero, an element that only held species
+    carry is left out of the projection (the held species carried none of it), and no cell falls back; mass
+    fractions stay non-negative and sum to one, every element is conserved, and a stream keeps none of the
+    element it lacks."""
+    s = corefl.Surrogate(batch_capacity=4096)
+    s.set_params(M.init_params(7))
+    s.set_scaling(np.zeros(12), np.ones(12), np.zeros(10), np.full(10, 1e3))
+    s.set_gate(0, 1e9, 0, 1e12, 0, 1, 1e9)  # everything accepted: only the projection can send a cell back
+    ch = corefl.Chemistry(CC.MECH, CC.THERM)
+    n = 600
+    rng = np.random.default_rng(31)
+    air, h2n2 = np.zeros(9), np.zeros(9)
+    air[2], air[8] = 0.233, 0.767  # species order H2 H O2 O OH HO2 H2O2 H2O N2
+    h2n2[0], h2n2[8] = 0.2, 0.8
+    Y = np.concatenate([np.repeat(air[:, None], n, 1), np.repeat(h2n2[:, None], n, 1),
+                        np.repeat((0.5 * air + 0.5 * h2n2)[:, None], n, 1)], axis=1)
+    T = rng.uniform(500.0, 1500.0, 3 * n)
+    rho = CC.density(CC.PATM, T, Y)
+    ch.set_cells(rho, T, rho * Y)
+    nf = s.chem_step(ch, 1e-7)
+    g = ch.get_cells()
+    Yn = g["rhoY"] / rho
+    assert nf == 0 and np.all(g["steps"] == 0)
+    assert np.max(np.abs(Yn - Y)) > 1e-6  # the network moved the cells
+    assert np.all(Yn >= 0) and np.max(np.abs(Yn.sum(axis=0) - 1)) < 1e-13
+    comp = ch.table("comp")
+    el0, el1 = CC.element_moles(Y, comp), CC.element_moles(Yn, comp)
+    assert np.max(np.abs(el1 - el0) / el0.max(axis=0)) < 1e-12
+    h_species, o_species = [0, 1, 4, 5, 6, 7], [2, 3, 4, 5, 6, 7]
+    assert np.max(Yn[h_species, :n]) < 1e-15 and np.max(Yn[o_species, n:2 * n]) < 1e-15
+
+
 def test_closed_gate_falls_back_to_exact(net):
     s, _, _ = net
     ch, rho, T, Y = chem_setup(2000, 22)
