@@ -316,8 +316,8 @@ def _contact_box(inviscid, pp, n=13):
 @pytest.mark.parametrize("inviscid", [51, 72])
 def test_positivity_limiter_acts_on_species_fluxes_only(inviscid):
     """The species positivity limiter (positive_preserving, WENO.cu:991-1087, 1471-1540) changes species fluxes
-    only, and does act where a jump drives a species bound negative.  The blend itself is restated from the
-    reference without an independent check (its formula is parity-unpinned)."""
+    only, and does act where a jump drives a species bound negative.  The blend's own closed forms are checked in
+    tests/test_positivity_pin_cpu.py."""
     res = []
     for pp in (0, 1):
         blk, bv, Yk = _contact_box(inviscid, pp)
