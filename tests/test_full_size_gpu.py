@@ -94,11 +94,12 @@ def test_reacting_box_workload_strang_step():
     T, To = interior(S.get(corefl.BV), 6, shape, g)[5], interior(B.get(1), 6, shape, g)[5]
     dT = np.max(np.abs(T - To) / To)
     print("max rel T err", dT, flush=True)
-    # the chemistry acted: on the periodic box the transport keeps the total H2O mass to rounding (1e-16), so its
-    # rise over the unique nodes is water made by the kernels (about 1e-10 of the total mass in one step)
-    made = (cv[12][:-1, :-1, :-1].sum() - c0[12][:-1, :-1, :-1].sum()) / c0[0][:-1, :-1, :-1].sum()
+    # the chemistry acted: the fresh mixture holds no water and transport makes none, so all of it was made by the
+    # kernels in their induction phase (measured 6.0e-13 of the total mass in this one step)
+    assert np.all(c0[12] == 0)
+    made = cv[12][:-1, :-1, :-1].sum() / c0[0][:-1, :-1, :-1].sum()
     print("H2O made per unit mass", made, flush=True)
-    assert made > 1e-12
+    assert made > 1e-13
     assert np.max(err) < 1e-12
     assert dT < 1e-12
     S.close()
