@@ -425,6 +425,9 @@ __device__ void chem_finish(const FusedArgs &a, long c, const float *o, bool in_
 // 3.37 ms at eight, 4.07 ms with the bias added on the VALU from LDS and the half-group exchange through
 // ds_bpermute; issuing each block's k-steps of the next layer right after its activation (a software
 // pipeline within the wave) measured 3.49 ms.  Two tiles at four waves measured slower than one at eight.
+// Computing the features inside this kernel from the chemistry batch (both lanes of a cell computing all twelve)
+// and the input-range check in the post-processing kernel measured 3.80 + 1.44 ms against 0.35 + 3.29 + 1.17 ms
+// with the separate feature kernel: the feature work is FP64 and VALU-heavy, and this kernel is VALU-bound.
 constexpr int kMlpTiles = 1, kMlpWaves = 12;
 constexpr int kBiasFrags = 4 * 4 + 1; // (layer, 32-neuron block) of the hidden layers, then the output layer
 
