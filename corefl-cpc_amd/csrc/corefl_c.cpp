@@ -199,6 +199,10 @@ struct corefl_ctx {
   corefl::Scheme scheme{};
   std::vector<Block> blocks;
   hipStream_t stream = nullptr;
+  // the viscous fluxes of a stage run on `side` while the inviscid fluxes run on the caller's stream (fork and
+  // join through the two events; inside a graph capture they become graph edges)
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   double *dt_dev = nullptr, *time_dev = nullptr, *parts = nullptr;
   long n_parts = 0;
   bool fixed_dt = false;
@@ -245,6 +249,9 @@ struct corefl_ctx {
     drop_graph();
     if (mon_host) (void)hipHostFree(mon_host);
     for (void *p : allocations) (void)hipFree(p);
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_join) (void)hipEventDestroy(ev_join);
+    if (side) (void)hipStreamDestroy(side);
     if (stream) (void)hipStreamDestroy(stream);
   }
 
@@ -346,34 +353,14 @@ struct corefl_ctx {
     // compute_shock_sensor once per step, before its first residual (RK.cuh:112-117)
     if (rk <= 0)
       for (auto &b : blocks) HIPCHK(corefl::launch_shock_sensor(b.dev, scheme, s));
-    if (scheme.viscous_order > 0) {
-      for (auto &b : blocks) {
-        if (ns) HIPCHK(corefl::launch_mix_viscous(b.dev, b.mix, scheme, s));
-        else HIPCHK(corefl::launch_viscous_flux(b.dev, scheme, s));
-      }
-      viscous_exchange(s);
-    }
-    for (auto &b : blocks) {
-      if (ns) {
-        if (scheme.inviscid_scheme == 2) HIPCHK(corefl::launch_mix_entropy_fix(b.dev, b.mix, scheme, s));
-        for (int d = 0; d < b.dev.dim; ++d) HIPCHK(corefl::launch_mix_inviscid_flux(b.dev, b.mix, scheme, d, s));
-        HIPCHK(corefl::launch_mix_update(b.dev, b.mix, rhs_only ? 0 : rk, dt_dev, rhs_only, s));
-        if (!rhs_only) {
-          limit(b, s);
-          apply_bcs(b, s);
-        }
-        continue;
-      }
-      if (scheme.inviscid_scheme == 2) HIPCHK(corefl::launch_entropy_fix(b.dev, scheme, s));
-      HIPCHK(corefl::launch_inviscid_flux(b.dev, scheme, 0, s));
-      HIPCHK(corefl::launch_inviscid_flux(b.dev, scheme, 1, s));
-      if (b.dev.dim == 3) HIPCHK(corefl::launch_inviscid_flux(b.dev, scheme, 2, s));
-      HIPCHK(corefl::launch_update(b.dev, scheme, rhs_only ? 0 : rk, dt_dev, rhs_only, s));
+    fluxes_then(s, [&](Block &b) {
+      if (ns) HIPCHK(corefl::launch_mix_update(b.dev, b.mix, rhs_only ? 0 : rk, dt_dev, rhs_only, s));
+      else HIPCHK(corefl::launch_update(b.dev, scheme, rhs_only ? 0 : rk, dt_dev, rhs_only, s));
       if (!rhs_only) {
         limit(b, s);
         apply_bcs(b, s);
       }
-    }
+    });
     if (!rhs_only) communicate(s);
   }
 
@@ -386,18 +373,33 @@ struct corefl_ctx {
     HIPCHK(corefl::launch_dt_reduce(parts, n_parts, dt_dev, time_dev, t_end, clip, s));
   }
 
-  // Viscous fluxes of every block (and their exchange for the collocated scheme), then per block the inviscid
-  // fluxes and `update`.
-  template<class F> void mix_fluxes(hipStream_t s, F &&update) {
-    if (scheme.viscous_order > 0) {
-      for (auto &b : blocks) HIPCHK(corefl::launch_mix_viscous(b.dev, b.mix, scheme, s));
-      viscous_exchange(s);
+  // The viscous fluxes of every block (and their exchange for the collocated scheme) on the side stream, the
+  // inviscid fluxes of every block on s at the same time (they read the same state and write other arrays),
+  // then `update` of every block on s once both are done.  The latency-bound viscous kernels fill the gaps the
+  // FP64-bound inviscid flux kernels leave.
+  template<class F> void fluxes_then(hipStream_t s, F &&update) {
+    const bool visc = scheme.viscous_order > 0;
+    if (visc) {
+      HIPCHK(hipEventRecord(ev_fork, s));
+      HIPCHK(hipStreamWaitEvent(side, ev_fork, 0));
+      for (auto &b : blocks) {
+        if (ns) HIPCHK(corefl::launch_mix_viscous(b.dev, b.mix, scheme, side));
+        else HIPCHK(corefl::launch_viscous_flux(b.dev, scheme, side));
+      }
+      viscous_exchange(side);
+      HIPCHK(hipEventRecord(ev_join, side));
     }
     for (auto &b : blocks) {
-      if (scheme.inviscid_scheme == 2) HIPCHK(corefl::launch_mix_entropy_fix(b.dev, b.mix, scheme, s));
-      for (int d = 0; d < b.dev.dim; ++d) HIPCHK(corefl::launch_mix_inviscid_flux(b.dev, b.mix, scheme, d, s));
-      update(b);
+      if (ns) {
+        if (scheme.inviscid_scheme == 2) HIPCHK(corefl::launch_mix_entropy_fix(b.dev, b.mix, scheme, s));
+        for (int d = 0; d < b.dev.dim; ++d) HIPCHK(corefl::launch_mix_inviscid_flux(b.dev, b.mix, scheme, d, s));
+      } else {
+        if (scheme.inviscid_scheme == 2) HIPCHK(corefl::launch_entropy_fix(b.dev, scheme, s));
+        for (int d = 0; d < b.dev.dim; ++d) HIPCHK(corefl::launch_inviscid_flux(b.dev, scheme, d, s));
+      }
     }
+    if (visc) HIPCHK(hipStreamWaitEvent(s, ev_join, 0));
+    for (auto &b : blocks) update(b);
   }
 
   // wu_splitting, one step (StrangSplitting.cu:683-861): the transport residual Cn at U^n (compute_cn) kept in
@@ -406,14 +408,14 @@ struct corefl_ctx {
   void wu_step(hipStream_t s) {
     // compute_cn computes the shock sensor (StrangSplitting.cu:19-24); the transport stages keep it
     for (auto &b : blocks) HIPCHK(corefl::launch_shock_sensor(b.dev, scheme, s));
-    mix_fluxes(s, [&](Block &b) { HIPCHK(corefl::launch_mix_update(b.dev, b.mix, 0, dt_dev, true, s)); });
+    fluxes_then(s, [&](Block &b) { HIPCHK(corefl::launch_mix_update(b.dev, b.mix, 0, dt_dev, true, s)); });
     for (auto &b : blocks) {
       HIPCHK(corefl::launch_mix_wu_reaction(b.dev, b.mix, dt_dev, wu_method, s));
       apply_bcs(b, s);
     }
     communicate(s);
     for (int rk = 0; rk < 3; ++rk) {
-      mix_fluxes(s, [&](Block &b) {
+      fluxes_then(s, [&](Block &b) {
         HIPCHK(corefl::launch_mix_wu_update(b.dev, b.mix, rk, dt_dev, s));
         apply_bcs(b, s);
       });
@@ -826,6 +828,9 @@ void build(corefl_ctx *ctx, std::vector<corefl::HostBlock> blocks) {
   }
 
   HIPCHK(hipStreamCreate(&ctx->stream));
+  HIPCHK(hipStreamCreate(&ctx->side));
+  HIPCHK(hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
+  HIPCHK(hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming));
   ctx->dt_dev = ctx->alloc(1);
   ctx->time_dev = ctx->alloc(1);
   ctx->sub_dt_dev = ctx->alloc(1);
