@@ -203,6 +203,8 @@ struct corefl_ctx {
   // join through the two events; inside a graph capture they become graph edges)
   hipStream_t side = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  static constexpr int kMaxSlabs = 64;
+  hipEvent_t ev_slab[kMaxSlabs] = {}; // the inviscid fluxes of z-slab q are done (air stages in slabs)
   double *dt_dev = nullptr, *time_dev = nullptr, *parts = nullptr;
   long n_parts = 0;
   bool fixed_dt = false;
@@ -249,6 +251,8 @@ struct corefl_ctx {
     drop_graph();
     if (mon_host) (void)hipHostFree(mon_host);
     for (void *p : allocations) (void)hipFree(p);
+    for (hipEvent_t e : ev_slab)
+      if (e) (void)hipEventDestroy(e);
     if (ev_fork) (void)hipEventDestroy(ev_fork);
     if (ev_join) (void)hipEventDestroy(ev_join);
     if (side) (void)hipStreamDestroy(side);
@@ -353,15 +357,60 @@ struct corefl_ctx {
     // compute_shock_sensor once per step, before its first residual (RK.cuh:112-117)
     if (rk <= 0)
       for (auto &b : blocks) HIPCHK(corefl::launch_shock_sensor(b.dev, scheme, s));
-    fluxes_then(s, [&](Block &b) {
-      if (ns) HIPCHK(corefl::launch_mix_update(b.dev, b.mix, rhs_only ? 0 : rk, dt_dev, rhs_only, s));
-      else HIPCHK(corefl::launch_update(b.dev, scheme, rhs_only ? 0 : rk, dt_dev, rhs_only, s));
+    if (!ns && slab_planes() > 0) {
+      air_stage_in_slabs(rk, s);
+    } else {
+      fluxes_then(s, [&](Block &b) {
+        if (ns) HIPCHK(corefl::launch_mix_update(b.dev, b.mix, rhs_only ? 0 : rk, dt_dev, rhs_only, s));
+        else HIPCHK(corefl::launch_update(b.dev, scheme, rhs_only ? 0 : rk, dt_dev, rhs_only, s));
+        if (!rhs_only) {
+          limit(b, s);
+          apply_bcs(b, s);
+        }
+      });
+    }
+    if (!rhs_only) communicate(s);
+  }
+
+  // Slab thickness (planes) of the air stages in z-slabs; 0 when a block is two-dimensional or too thin.
+  int slab_planes() const {
+    const int P = 4 * corefl::march_planes();
+    for (const auto &b : blocks)
+      if (b.dev.dim != 3 || b.dev.mz < 2 * P || (b.dev.mz + P - 1) / P > kMaxSlabs) return 0;
+    return P;
+  }
+
+  // One air stage with the blocks cut into z-slabs of P planes: the inviscid fluxes of the slabs run on the side
+  // stream, one slab after another, while s runs the viscous fluxes and then the update of each slab as soon as
+  // its fluxes are there, so the FP64-bound flux kernels and the HBM-bound update run at the same time.  The
+  // update of slab q overwrites its planes, which the z-fluxes of slab q + 1 still read (the WENO7 stencil of
+  // the interface k + 1/2 spans planes k - 3 .. k + 4), so it waits for slab q + 1's fluxes; the fluxes of later
+  // slabs read no plane of slab q.  The same kernels compute the same values as the unsliced stage.
+  void air_stage_in_slabs(int rk, hipStream_t s) {
+    const bool rhs_only = rk < 0;
+    const int P = slab_planes();
+    HIPCHK(hipEventRecord(ev_fork, s));
+    HIPCHK(hipStreamWaitEvent(side, ev_fork, 0));
+    if (scheme.viscous_order > 0) {
+      for (auto &b : blocks) HIPCHK(corefl::launch_viscous_flux(b.dev, scheme, s));
+      viscous_exchange(s);
+    }
+    for (auto &b : blocks) {
+      const int n_slab = (b.dev.mz + P - 1) / P;
+      if (scheme.inviscid_scheme == 2) HIPCHK(corefl::launch_entropy_fix(b.dev, scheme, side));
+      for (int q = 0; q < n_slab; ++q) {
+        for (int d = 0; d < 3; ++d) HIPCHK(corefl::launch_inviscid_flux(b.dev, scheme, d, side, q * P, (q + 1) * P));
+        HIPCHK(hipEventRecord(ev_slab[q], side));
+      }
+      for (int q = 0; q < n_slab; ++q) {
+        HIPCHK(hipStreamWaitEvent(s, ev_slab[std::min(q + 1, n_slab - 1)], 0));
+        HIPCHK(corefl::launch_update(b.dev, scheme, rhs_only ? 0 : rk, dt_dev, rhs_only, s, q * P, (q + 1) * P));
+      }
       if (!rhs_only) {
         limit(b, s);
         apply_bcs(b, s);
       }
-    });
-    if (!rhs_only) communicate(s);
+    }
   }
 
   void enqueue_dt(hipStream_t s, int clip) {
@@ -831,6 +880,7 @@ void build(corefl_ctx *ctx, std::vector<corefl::HostBlock> blocks) {
   HIPCHK(hipStreamCreate(&ctx->side));
   HIPCHK(hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
   HIPCHK(hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming));
+  for (hipEvent_t &e : ctx->ev_slab) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   ctx->dt_dev = ctx->alloc(1);
   ctx->time_dev = ctx->alloc(1);
   ctx->sub_dt_dev = ctx->alloc(1);

@@ -82,10 +82,15 @@ hipError_t launch_inner_viscous_exchange(const DevBlock &b, const DevBlock &t, c
 hipError_t launch_entropy_fix(const DevBlock &b, const Scheme &sc, hipStream_t s);
 // compute_shock_sensor (FieldOperation.cu:74-179) into b.sensor at the block's nodes; a no-op when b.sensor is null.
 hipError_t launch_shock_sensor(const DevBlock &b, const Scheme &sc, hipStream_t s);
-hipError_t launch_inviscid_flux(const DevBlock &b, const Scheme &sc, int dir, hipStream_t s);
-// Residual + SSP-RK3 stage update; rhs_only writes dq instead of updating.  dt is read from *dt_dev.
+// Interface fluxes along dir of the node planes [kb, ke) (the whole block by default).
+hipError_t launch_inviscid_flux(const DevBlock &b, const Scheme &sc, int dir, hipStream_t s, int kb = 0,
+                                int ke = 1 << 30);
+// Residual + SSP-RK3 stage update of the node planes [kb, ke) (kb a multiple of march_planes()); rhs_only writes dq
+// instead of updating.  dt is read from *dt_dev.
 hipError_t launch_update(const DevBlock &b, const Scheme &sc, int rk, const double *dt_dev, bool rhs_only,
-                         hipStream_t s);
+                         hipStream_t s, int kb = 0, int ke = 1 << 30);
+// Planes one thread of the z-marching node kernels walks along k.
+int march_planes();
 // CFL time step: per-workgroup partial minima into part[0 .. dt_partial_count(b)), then the global
 // minimum of n partials into dt_dev[0], clipped to t_end - time_dev[0] when clip != 0.
 int dt_partial_count(const DevBlock &b);

@@ -36,6 +36,7 @@ __device__ __forceinline__ long nid(const DevBlock &b, int i, int j, int k) {
 struct TileGrid {
   int gx, gy, gz; // tiles along i (64 wide), j (4 wide) and k (1 wide)
   int fast;       // 0: i then j then k (each XCD takes a contiguous slab of planes), 1: j fastest, 2: k fastest
+  int z0;         // first k tile (a launch over a slab of planes)
 };
 
 struct Tile {
@@ -63,6 +64,7 @@ __device__ __forceinline__ bool xcd_tile(const TileGrid &t, Tile &out) {
     out.x = r % t.gx;
     out.z = r / t.gx;
   }
+  out.z += t.z0;
   return true;
 }
 
@@ -1283,9 +1285,9 @@ hipError_t launch_entropy_fix(const DevBlock &b, const Scheme &sc, hipStream_t s
 }
 
 template<int DIR, int R>
-static void dispatch_rs(const DevBlock &b, const Scheme &sc, dim3 grid, double *F, hipStream_t s) {
+static void dispatch_rs(const DevBlock &b, const Scheme &sc, dim3 grid, double *F, hipStream_t s, int z0) {
   const dim3 tpb(64, 4, 1);
-  const TileGrid tg{(int)grid.x, (int)grid.y, (int)grid.z, DIR};
+  const TileGrid tg{(int)grid.x, (int)grid.y, (int)grid.z, DIR, z0};
   const unsigned nb = tile_blocks(tg);
   switch (sc.inviscid_scheme) {
     case 1: k_inviscid<DIR, R, 1><<<nb, tpb, 0, s>>>(b, sc, F, tg); break;
@@ -1296,9 +1298,9 @@ static void dispatch_rs(const DevBlock &b, const Scheme &sc, dim3 grid, double *
 }
 
 template<int DIR, int REC>
-static void dispatch_pv(const DevBlock &b, const Scheme &sc, dim3 grid, double *F, hipStream_t s) {
+static void dispatch_pv(const DevBlock &b, const Scheme &sc, dim3 grid, double *F, hipStream_t s, int z0) {
   const dim3 tpb(64, 4, 1);
-  const TileGrid tg{(int)grid.x, (int)grid.y, (int)grid.z, DIR};
+  const TileGrid tg{(int)grid.x, (int)grid.y, (int)grid.z, DIR, z0};
   const unsigned nb = tile_blocks(tg);
   switch (sc.inviscid_scheme) {
     case 1: k_inviscid_pv<DIR, REC, 1><<<nb, tpb, 0, s>>>(b, sc, F, tg); break;
@@ -1309,16 +1311,16 @@ static void dispatch_pv(const DevBlock &b, const Scheme &sc, dim3 grid, double *
 }
 
 template<int DIR>
-static void dispatch_pv_rec(const DevBlock &b, const Scheme &sc, dim3 grid, double *F, hipStream_t s) {
-  if (sc.reconstruction == 1) dispatch_pv<DIR, 1>(b, sc, grid, F, s);
-  else if (sc.reconstruction == 2) dispatch_pv<DIR, 2>(b, sc, grid, F, s);
-  else dispatch_pv<DIR, 3>(b, sc, grid, F, s);
+static void dispatch_pv_rec(const DevBlock &b, const Scheme &sc, dim3 grid, double *F, hipStream_t s, int z0) {
+  if (sc.reconstruction == 1) dispatch_pv<DIR, 1>(b, sc, grid, F, s, z0);
+  else if (sc.reconstruction == 2) dispatch_pv<DIR, 2>(b, sc, grid, F, s, z0);
+  else dispatch_pv<DIR, 3>(b, sc, grid, F, s, z0);
 }
 
 template<int DIR>
-static void dispatch_fs(const DevBlock &b, const Scheme &sc, dim3 grid, double *F, hipStream_t s) {
+static void dispatch_fs(const DevBlock &b, const Scheme &sc, dim3 grid, double *F, hipStream_t s, int z0) {
   const dim3 tpb(64, 4, 1);
-  const TileGrid tg{(int)grid.x, (int)grid.y, (int)grid.z, DIR};
+  const TileGrid tg{(int)grid.x, (int)grid.y, (int)grid.z, DIR, z0};
   const unsigned nb = tile_blocks(tg);
   switch (sc.inviscid_scheme) {
     case 51: k_fsweno<DIR, 3, false><<<nb, tpb, 0, s>>>(b, sc, F, tg); break;
@@ -1328,43 +1330,54 @@ static void dispatch_fs(const DevBlock &b, const Scheme &sc, dim3 grid, double *
   }
 }
 
-hipError_t launch_inviscid_flux(const DevBlock &b, const Scheme &sc, int dir, hipStream_t s) {
+hipError_t launch_inviscid_flux(const DevBlock &b, const Scheme &sc, int dir, hipStream_t s, int kb, int ke) {
+  // the interfaces of the node planes [kb, ke): F(i+1/2) stored at node i for i = -1 .. mx-1 (x), likewise in y; in
+  // z the interfaces k + 1/2 of planes kb .. ke-1, and also -1/2 when kb = 0 (tile z = k + 1)
+  const int me = ke < b.mz ? ke : b.mz;
   dim3 grid;
   double *F;
+  int z0 = kb;
   if (dir == 0) {
-    grid = dim3((b.mx + 1 + 63) / 64, (b.my + 3) / 4, b.mz);
+    grid = dim3((b.mx + 1 + 63) / 64, (b.my + 3) / 4, me - kb);
     F = b.fx;
   } else if (dir == 1) {
-    grid = dim3((b.mx + 63) / 64, (b.my + 1 + 3) / 4, b.mz);
+    grid = dim3((b.mx + 63) / 64, (b.my + 1 + 3) / 4, me - kb);
     F = b.fy;
   } else {
-    grid = dim3((b.mx + 63) / 64, (b.my + 3) / 4, b.mz + 1);
+    z0 = kb == 0 ? 0 : kb + 1;
+    grid = dim3((b.mx + 63) / 64, (b.my + 3) / 4, me + 1 - z0);
     F = b.fz;
   }
+  if ((int)grid.z <= 0) return hipSuccess;
   if (sc.inviscid_scheme > 50) {
-    if (dir == 0) dispatch_fs<0>(b, sc, grid, F, s);
-    else if (dir == 1) dispatch_fs<1>(b, sc, grid, F, s);
-    else dispatch_fs<2>(b, sc, grid, F, s);
+    if (dir == 0) dispatch_fs<0>(b, sc, grid, F, s, z0);
+    else if (dir == 1) dispatch_fs<1>(b, sc, grid, F, s, z0);
+    else dispatch_fs<2>(b, sc, grid, F, s, z0);
     return hipGetLastError();
   }
   if (sc.reconstruction >= 1 && sc.reconstruction <= 3) {
-    if (dir == 0) dispatch_pv_rec<0>(b, sc, grid, F, s);
-    else if (dir == 1) dispatch_pv_rec<1>(b, sc, grid, F, s);
-    else dispatch_pv_rec<2>(b, sc, grid, F, s);
+    if (dir == 0) dispatch_pv_rec<0>(b, sc, grid, F, s, z0);
+    else if (dir == 1) dispatch_pv_rec<1>(b, sc, grid, F, s, z0);
+    else dispatch_pv_rec<2>(b, sc, grid, F, s, z0);
     return hipGetLastError();
   }
   const bool w7 = sc.reconstruction == 5;
-  if (dir == 0) w7 ? dispatch_rs<0, 4>(b, sc, grid, F, s) : dispatch_rs<0, 3>(b, sc, grid, F, s);
-  else if (dir == 1) w7 ? dispatch_rs<1, 4>(b, sc, grid, F, s) : dispatch_rs<1, 3>(b, sc, grid, F, s);
-  else w7 ? dispatch_rs<2, 4>(b, sc, grid, F, s) : dispatch_rs<2, 3>(b, sc, grid, F, s);
+  if (dir == 0) w7 ? dispatch_rs<0, 4>(b, sc, grid, F, s, z0) : dispatch_rs<0, 3>(b, sc, grid, F, s, z0);
+  else if (dir == 1) w7 ? dispatch_rs<1, 4>(b, sc, grid, F, s, z0) : dispatch_rs<1, 3>(b, sc, grid, F, s, z0);
+  else w7 ? dispatch_rs<2, 4>(b, sc, grid, F, s, z0) : dispatch_rs<2, 3>(b, sc, grid, F, s, z0);
   return hipGetLastError();
 }
 
 hipError_t launch_update(const DevBlock &b, const Scheme &sc, int rk, const double *dt_dev, bool rhs_only,
-                         hipStream_t s) {
+                         hipStream_t s, int kb, int ke) {
   const dim3 tpb(64, 4, 1);
   const int kz = march_kz();
-  const TileGrid tg = march_tiles(b, kz);
+  TileGrid tg = march_tiles(b, kz);
+  // the node planes [kb, ke) (kb a multiple of the march length)
+  const int me = ke < b.mz ? ke : b.mz;
+  tg.z0 = kb / kz;
+  tg.gz = (me + kz - 1) / kz - tg.z0;
+  if (tg.gz <= 0) return hipSuccess;
   const unsigned grid = tile_blocks(tg);
   const int o = sc.viscous_order;
   if (rhs_only) {
@@ -1407,5 +1420,7 @@ hipError_t launch_shock_sensor(const DevBlock &b, const Scheme &sc, hipStream_t 
   k_shock_sensor<<<dim3((b.mx + 63) / 64, (b.my + 3) / 4, b.mz), dim3(64, 4, 1), 0, s>>>(b, sc.shock_sensor, sc.sensor_eps);
   return hipGetLastError();
 }
+
+int march_planes() { return march_kz(); }
 
 } // namespace corefl
