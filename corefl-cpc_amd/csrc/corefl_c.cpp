@@ -205,6 +205,7 @@ struct corefl_ctx {
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   static constexpr int kMaxSlabs = 64;
   hipEvent_t ev_slab[kMaxSlabs] = {}; // the inviscid fluxes of z-slab q are done (air stages in slabs)
+  int slab_setting = -1;              // stage_slab_planes
   double *dt_dev = nullptr, *time_dev = nullptr, *parts = nullptr;
   long n_parts = 0;
   bool fixed_dt = false;
@@ -372,9 +373,11 @@ struct corefl_ctx {
     if (!rhs_only) communicate(s);
   }
 
-  // Slab thickness (planes) of the air stages in z-slabs; 0 when a block is two-dimensional or too thin.
+  // Slab thickness (planes) of the air stages in z-slabs (setup key stage_slab_planes; -1: four march lengths);
+  // 0 when that is 0 or a block is two-dimensional or too thin.
   int slab_planes() const {
-    const int P = 4 * corefl::march_planes();
+    const int P = slab_setting < 0 ? 4 * corefl::march_planes() : slab_setting;
+    if (P == 0) return 0;
     for (const auto &b : blocks)
       if (b.dev.dim != 3 || b.dev.mz < 2 * P || (b.dev.mz + P - 1) / P > kMaxSlabs) return 0;
     return P;
@@ -878,6 +881,12 @@ void build(corefl_ctx *ctx, std::vector<corefl::HostBlock> blocks) {
 
   HIPCHK(hipStreamCreate(&ctx->stream));
   HIPCHK(hipStreamCreate(&ctx->side));
+  ctx->slab_setting = P.get_int("stage_slab_planes");
+  // a slab must be thicker than the 3 planes the WENO7 z-stencil reaches below an interface, and start on a
+  // march boundary of the update kernel
+  if (ctx->slab_setting > 0 && (ctx->slab_setting < 8 || ctx->slab_setting % corefl::march_planes() != 0))
+    throw std::runtime_error("stage_slab_planes must be -1, 0, or a multiple of " +
+                             std::to_string(corefl::march_planes()) + " of at least 8");
   HIPCHK(hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
   HIPCHK(hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming));
   for (hipEvent_t &e : ctx->ev_slab) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));

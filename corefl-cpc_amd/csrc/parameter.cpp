@@ -24,6 +24,9 @@ Parameter::Parameter() {
   ints["limiter"] = 0;
   reals["entropy_fix_factor"] = 0.125;
   ints["viscous_order"] = 2;
+  // this build's: thickness (planes) of the z-slabs an air stage is cut into so that flux kernels and updates
+  // overlap; -1 chooses 4 march lengths, 0 runs each stage unsliced
+  ints["stage_slab_planes"] = -1;
   reals["shockSensor_threshold"] = 0;
   ints["shock_sensor"] = 0;           // 0 Ducros, 1 Jameson, 2 density/pressure jump (Parameter.cpp:384)
   reals["shockSensor_epsilon"] = -1;  // the Ducros sensor's epsilon (Parameter.cpp:386)
