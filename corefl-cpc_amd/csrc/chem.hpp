@@ -46,6 +46,9 @@ struct ChemMech {
   int geometry[kMaxSpec];
   double lj_inv[kMaxSpec], vis_coeff[kMaxSpec], zrot[kMaxSpec];
   double wdw[kMaxSpec][kMaxSpec], sqw[kMaxSpec][kMaxSpec], bdc[kMaxSpec][kMaxSpec], kbe[kMaxSpec][kMaxSpec];
+  // ln(1 / (eps / k_B)), ln(k_B / eps_jk) and 1 / bdc: the logarithms turn the collision integrals' powers of
+  // T* = T k_B / eps into exponentials of ln T + ln(k_B / eps), with one ln T per node
+  double log_lj_inv[kMaxSpec], log_kbe[kMaxSpec][kMaxSpec], inv_bdc[kMaxSpec][kMaxSpec];
 };
 
 // h_k and cp_k per unit mass at t, linear extrapolation outside [t_low, t_high] (Thermo.cu:470-520).

@@ -284,8 +284,12 @@ template<int NS> __device__ double update_t(const ChemMech &m, double t, const d
   return t;
 }
 
-// Collision integral of binary diffusion at reduced temperature t (Transport.cu:188-190).
-__device__ __forceinline__ double omega_d(double t) { return 1.0548 * pow(t, -0.15504) + pow(t + 0.55909, -2.1705); }
+// Collision integral of binary diffusion at reduced temperature t (Transport.cu:188-190),
+// 1.0548 t^-0.15504 + (t + 0.55909)^-2.1705, given lt = ln t: t^a as exp(a ln t), so that a node's pairs share
+// its ln T (the powers agree with pow to a few ulps).
+__device__ __forceinline__ double omega_d_log(double t, double lt) {
+  return 1.0548 * exp(-0.15504 * lt) + exp(-2.1705 * log(t + 0.55909));
+}
 
 // compute_transport_property (Transport.cu:43-150) at temperature t and mass fractions y (cp: species cp per
 // unit mass at t): Chapman-Enskog species viscosities mixed by Wilke's rule; species conductivities from
@@ -294,19 +298,25 @@ __device__ __forceinline__ double omega_d(double t) { return 1.0548 * pow(t, -0.
 template<int NS>
 __device__ inline void transport(const ChemMech &m, double t, const double *y, const double *cp, double &mu,
                                  double &lambda, double *rhoD) {
+  // The reference's formulas with their powers of the reduced temperatures T k_B / eps written as exponentials
+  // of ln T + ln(k_B / eps) (one logarithm per node instead of one per power), 1/(x x) for x^-2, and products
+  // with tabulated or once-computed reciprocals in place of divisions: equal to the direct form to a few ulps.
   constexpr double pi = 3.14159265358979323846, Ru = 8314.462618;
   double x[NS], vs[NS];
   double imw = 0;
 #pragma unroll
   for (int l = 0; l < NS; ++l) imw += y[l] * m.imw[l];
   const double mwt = 1.0 / imw;
-  const double st = sqrt(t), t32 = t * st;
+  const double st = sqrt(t), t32 = t * st, lt = log(t), it32 = 1.0 / t32;
 #pragma unroll
   for (int l = 0; l < NS; ++l) {
     x[l] = y[l] * mwt * m.imw[l];
-    const double td = t * m.lj_inv[l];
-    vs[l] = m.vis_coeff[l] * st / (1.147 * pow(td, -0.145) + pow(td + 0.5, -2.0));
+    const double td = t * m.lj_inv[l], u = td + 0.5;
+    vs[l] = m.vis_coeff[l] * st / (1.147 * exp(-0.145 * (lt + m.log_lj_inv[l])) + 1.0 / (u * u));
   }
+  double isq[NS];
+#pragma unroll
+  for (int l = 0; l < NS; ++l) isq[l] = 1.0 / sqrt(vs[l]);
   mu = 0;
   lambda = 0;
 #pragma unroll
@@ -318,7 +328,7 @@ __device__ inline void transport(const ChemMech &m, double t, const double *y, c
       const double tr = t * m.kbe[a][a], tri = 1 / tr;
       const double q = sqrt(tri * pi) * pi;
       const double ft = 1 + 0.5 * q + (2 + 0.25 * pi * pi) * tri + q * tri;
-      const double rd = m.bdc[a][a] * t32 / (omega_d(tr) * R * t);
+      const double rd = m.bdc[a][a] * t32 / (omega_d_log(tr, lt + m.log_kbe[a][a]) * R * t);
       if (geom == 1) {
         const double ab = (2.5 * vs[a] - rd) / (pi * m.zrot[a] / ft * vs[a] + 10.0 / 3 * vs[a] + 2.0 * rd);
         lam += -5 * ab * vs[a] * R + rd * cp[a] + rd * (2 * ab - 2.5) * R;
@@ -333,7 +343,7 @@ __device__ inline void transport(const ChemMech &m, double t, const double *y, c
     for (int b = 0; b < NS; ++b) {
       double phi = 1.0;
       if (a != b) {
-        const double num = 1.0 + sa / sqrt(vs[b]) * m.wdw[a][b];
+        const double num = 1.0 + sa * isq[b] * m.wdw[a][b];
         phi = num * num * m.sqw[a][b];
       }
       den += phi * x[b];
@@ -354,7 +364,7 @@ __device__ inline void transport(const ChemMech &m, double t, const double *y, c
   for (int l = 0; l < NS; ++l)
 #pragma unroll
     for (int n = l + 1; n < NS; ++n) {
-      const double inv = omega_d(t * m.kbe[l][n]) / (m.bdc[l][n] * t32);
+      const double inv = omega_d_log(t * m.kbe[l][n], lt + m.log_kbe[l][n]) * m.inv_bdc[l][n] * it32;
       dsum[l] += (x[n] + eps) * inv;
       dsum[n] += (x[l] + eps) * inv;
     }

@@ -395,6 +395,13 @@ void parse_transport(const std::string &tran_text, ChemMech &m) {
       m.kbe[j][k] = m.kbe[k][j] = 1.0 / (xi * xi * std::sqrt(eps[j] * eps[k]));
     }
   }
+  for (int j = 0; j < ns; ++j) {
+    m.log_lj_inv[j] = std::log(m.lj_inv[j]);
+    for (int k = 0; k < ns; ++k) {
+      m.log_kbe[j][k] = std::log(m.kbe[j][k]);
+      m.inv_bdc[j][k] = 1.0 / m.bdc[j][k];
+    }
+  }
   m.has_transport = 1;
 }
 
