@@ -51,16 +51,10 @@ class ChemView(ctypes.Structure):
 _lib = None
 
 
-def load_library(path=LIB_PATH):
-    """Load the native library (once).  Raises OSError if it has not been built."""
-    global _lib
-    if _lib is not None:
-        return _lib
-    if not os.path.exists(path):
-        raise OSError(f"{path} is missing: build it with `make -C corefl-cpc_amd` (or __graft_entry__.build())")
-    lib = ctypes.CDLL(path)
+def _bind_flow(lib):
+    """argtypes of the flow entry points (the subset a single-block flow case uses, which the oracle behind the
+    ABI also exports)."""
     P, I, D = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
-    lib.corefl_create.argtypes = [ctypes.c_char_p, ctypes.POINTER(P)]
     lib.corefl_create_mem.argtypes = [ctypes.c_char_p, I, ctypes.POINTER(I), ctypes.POINTER(P), ctypes.c_char_p,
                                       ctypes.c_char_p, ctypes.POINTER(P)]
     lib.corefl_destroy.argtypes = [P]
@@ -72,6 +66,39 @@ def load_library(path=LIB_PATH):
     lib.corefl_set_state.argtypes = [P, I, P, ctypes.c_longlong]
     lib.corefl_rhs.argtypes = [P, I, P]
     lib.corefl_set_dt.argtypes = [P, D]
+    lib.corefl_compute_dt.argtypes = [P, ctypes.POINTER(D)]
+    lib.corefl_rk3_step.argtypes = [P, ctypes.POINTER(D), P]
+    lib.corefl_advance.argtypes = [P, I, I, P]
+    lib.corefl_get_time.argtypes = [P, ctypes.POINTER(D)]
+    lib.corefl_synchronize.argtypes = [P]
+    lib.corefl_get_int.argtypes = [P, ctypes.c_char_p, ctypes.POINTER(I)]
+    lib.corefl_get_real.argtypes = [P, ctypes.c_char_p, ctypes.POINTER(D)]
+    return lib
+
+
+_flow_libs = {}
+
+
+def load_flow_library(path):
+    """A library exporting the flow entry points of the C ABI (for example oracle/_build/liboracle_abi.so, the
+    CPU oracle behind the same ABI), loaded once per path."""
+    if path not in _flow_libs:
+        if not os.path.exists(path):
+            raise OSError(f"{path} is missing")
+        _flow_libs[path] = _bind_flow(ctypes.CDLL(path))
+    return _flow_libs[path]
+
+
+def load_library(path=LIB_PATH):
+    """Load the native library (once).  Raises OSError if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise OSError(f"{path} is missing: build it with `make -C corefl-cpc_amd` (or __graft_entry__.build())")
+    lib = _bind_flow(ctypes.CDLL(path))
+    P, I, D = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+    lib.corefl_create.argtypes = [ctypes.c_char_p, ctypes.POINTER(P)]
     lib.corefl_chem_substep.argtypes = [P, D, I, P]
     lib.corefl_compute_dt.argtypes = [P, ctypes.POINTER(D)]
     lib.corefl_rk3_step.argtypes = [P, ctypes.POINTER(D), P]
@@ -139,12 +166,15 @@ def _check(rc):
 class Solver:
     """One COREFL context: input contract in, device state and SSP-RK3 steps out."""
 
-    def __init__(self, input_dir=None, *, setup_text=None, blocks=None, boundary_text=None, inner_text=None):
-        lib = load_library()
+    def __init__(self, input_dir=None, *, setup_text=None, blocks=None, boundary_text=None, inner_text=None,
+                 library=None):
+        """library: path of another library exporting the same C ABI (the CPU oracle behind it,
+        oracle/_build/liboracle_abi.so), for a test that runs unchanged on either implementation."""
+        lib = load_library() if library is None else load_flow_library(library)
         self._lib = lib
         self._ctx = ctypes.c_void_p()
         if input_dir is not None:
-            _check(lib.corefl_create(os.fsencode(input_dir), ctypes.byref(self._ctx)))
+            self._ok(lib.corefl_create(os.fsencode(input_dir), ctypes.byref(self._ctx)))
         else:
             dims = (ctypes.c_int * (3 * len(blocks)))()
             ptrs = (ctypes.c_void_p * (3 * len(blocks)))()
@@ -156,8 +186,12 @@ class Solver:
                     flat = np.ascontiguousarray(np.asarray(a, dtype=np.float64).ravel(order="F"))
                     keep.append(flat)
                     ptrs[3 * b + q] = flat.ctypes.data
-            _check(lib.corefl_create_mem(setup_text.encode(), len(blocks), dims, ptrs, boundary_text.encode(),
+            self._ok(lib.corefl_create_mem(setup_text.encode(), len(blocks), dims, ptrs, boundary_text.encode(),
                                          inner_text.encode() if inner_text else None, ctypes.byref(self._ctx)))
+
+    def _ok(self, rc):
+        if rc != 0:
+            raise CoreflError(self._lib.corefl_last_error().decode())
 
     def close(self):
         if self._ctx:
@@ -176,7 +210,7 @@ class Solver:
 
     def view(self, b=0):
         v = BlockView()
-        _check(self._lib.corefl_block_view_get(self._ctx, b, ctypes.byref(v)))
+        self._ok(self._lib.corefl_block_view_get(self._ctx, b, ctypes.byref(v)))
         return v
 
     def get(self, which, b=0):
@@ -185,7 +219,7 @@ class Solver:
         count = {CV: v.n_var * v.sz, BV: 6 * v.sz, DQ: v.n_var * n_int, SV: v.n_spec * v.sz, JAC: v.sz,
                  METRIC: 9 * v.sz, FV: 5 * v.sz, GV: 5 * v.sz, HV: 5 * v.sz, SENSOR: v.sz}[which]
         out = np.empty(count, dtype=np.float64)
-        _check(self._lib.corefl_get_array(self._ctx, b, which, out.ctypes.data, count))
+        self._ok(self._lib.corefl_get_array(self._ctx, b, which, out.ctypes.data, count))
         return out
 
     def set_state(self, bv, b=0, sv=None):
@@ -193,85 +227,85 @@ class Solver:
         bv = np.ascontiguousarray(bv, dtype=np.float64).ravel()
         if sv is not None:
             bv = np.concatenate([bv, np.ascontiguousarray(sv, dtype=np.float64).ravel()])
-        _check(self._lib.corefl_set_state(self._ctx, b, bv.ctypes.data, bv.size))
+        self._ok(self._lib.corefl_set_state(self._ctx, b, bv.ctypes.data, bv.size))
 
     def chemistry(self, b=0):
         """Borrowed Chemistry view of a mixture setup, its batch holding block b's interior cells."""
         h = ctypes.c_void_p()
-        _check(self._lib.corefl_chemistry(self._ctx, b, ctypes.byref(h)))
+        self._ok(self._lib.corefl_chemistry(self._ctx, b, ctypes.byref(h)))
         return Chemistry._borrow(h, self)
 
     def set_chemistry_surrogate(self, surrogate):
         self._surrogate = surrogate
-        _check(self._lib.corefl_set_chemistry_surrogate(self._ctx, surrogate._ctx if surrogate else None))
+        self._ok(self._lib.corefl_set_chemistry_surrogate(self._ctx, surrogate._ctx if surrogate else None))
 
     def rhs(self, stage=-1):
         """stage -1: residual dq of the current state; 0, 1, 2: that SSP-RK3 stage (residual and update)."""
-        _check(self._lib.corefl_rhs(self._ctx, int(stage), None))
+        self._ok(self._lib.corefl_rhs(self._ctx, int(stage), None))
         self.sync()
 
     def set_dt(self, dt):
-        _check(self._lib.corefl_set_dt(self._ctx, float(dt)))
+        self._ok(self._lib.corefl_set_dt(self._ctx, float(dt)))
 
     def chem_substep(self, dt, mode=0):
         """One chemistry sub-step of length dt: mode 0 exact, 1 surrogate with gate and exact fallback."""
-        _check(self._lib.corefl_chem_substep(self._ctx, float(dt), int(mode), None))
+        self._ok(self._lib.corefl_chem_substep(self._ctx, float(dt), int(mode), None))
 
     def compute_dt(self):
         d = ctypes.c_double()
-        _check(self._lib.corefl_compute_dt(self._ctx, ctypes.byref(d)))
+        self._ok(self._lib.corefl_compute_dt(self._ctx, ctypes.byref(d)))
         return d.value
 
     def rk3_step(self, dt=0.0):
         d = ctypes.c_double(dt)
-        _check(self._lib.corefl_rk3_step(self._ctx, ctypes.byref(d), None))
+        self._ok(self._lib.corefl_rk3_step(self._ctx, ctypes.byref(d), None))
         return d.value
 
     def advance(self, n_steps, graph=True):
-        _check(self._lib.corefl_advance(self._ctx, int(n_steps), 1 if graph else 0, None))
+        self._ok(self._lib.corefl_advance(self._ctx, int(n_steps), 1 if graph else 0, None))
 
     def residual(self):
         """(res[4] of the last monitored step: rho, |V|, p, T; NaN flag; number of monitored steps)."""
         r = (ctypes.c_double * 4)()
         nan, cnt = ctypes.c_int(), ctypes.c_int()
-        _check(self._lib.corefl_residual(self._ctx, r, ctypes.byref(nan), ctypes.byref(cnt)))
+        self._ok(self._lib.corefl_residual(self._ctx, r, ctypes.byref(nan), ctypes.byref(cnt)))
         return np.array(list(r)), bool(nan.value), cnt.value
 
     def limit_flow(self):
-        _check(self._lib.corefl_limit_flow(self._ctx, None))
+        self._ok(self._lib.corefl_limit_flow(self._ctx, None))
 
     def time(self):
         d = ctypes.c_double()
-        _check(self._lib.corefl_get_time(self._ctx, ctypes.byref(d)))
+        self._ok(self._lib.corefl_get_time(self._ctx, ctypes.byref(d)))
         return d.value
 
     def sync(self):
-        _check(self._lib.corefl_synchronize(self._ctx))
+        self._ok(self._lib.corefl_synchronize(self._ctx))
 
     def write_flowfield(self, path):
         """Write the state and time as the reference's flowfield.plt (Tecplot binary, FieldIO.h:89-430, 709-825)."""
-        _check(self._lib.corefl_write_flowfield(self._ctx, os.fsencode(path)))
+        self._ok(self._lib.corefl_write_flowfield(self._ctx, os.fsencode(path)))
 
     def read_flowfield(self, path):
         """Continue from a flowfield.plt of the same blocks (read_flowfield, Initialize.cu:97-345)."""
-        _check(self._lib.corefl_read_flowfield(self._ctx, os.fsencode(path)))
+        self._ok(self._lib.corefl_read_flowfield(self._ctx, os.fsencode(path)))
 
     KERNELS = ("viscous_flux", "viscous_exchange", "flux_x", "flux_y", "flux_z", "update", "boundary", "cfl_dt")
 
     def profile_kernels(self, reps=5):
         """Average ms per launch of each kernel class (HIP events on the context stream)."""
         out = (ctypes.c_double * 8)()
-        _check(self._lib.corefl_profile_kernels(self._ctx, int(reps), out, 8))
+        self._ok(self._lib.corefl_profile_kernels(self._ctx, int(reps), out, 8))
         return dict(zip(self.KERNELS, list(out)))
 
     def get_int(self, name):
         v = ctypes.c_int()
-        _check(self._lib.corefl_get_int(self._ctx, name.encode(), ctypes.byref(v)))
+        self._ok(self._lib.corefl_get_int(self._ctx, name.encode(), ctypes.byref(v)))
         return v.value
 
     def get_real(self, name):
         v = ctypes.c_double()
-        _check(self._lib.corefl_get_real(self._ctx, name.encode(), ctypes.byref(v)))
+        self._ok(self._lib.corefl_get_real(self._ctx, name.encode(), ctypes.byref(v)))
         return v.value
 
 

@@ -106,6 +106,10 @@ double orc_sutherland(double T);
 
 /* ---- mesh (Mesh.cpp) ---- */
 
+/* The coordinates of every node of the ngg-ghost layout (sz each): the block's nodes and the mirrored ghost grid
+ * orc_metrics builds (Mesh.cpp:1013-1167). */
+void orc_ghost_coords(int mx, int my, int mz, int ngg, const double *x, const double *y, const double *z, double *gx,
+                      double *gy, double *gz);
 /* Mirror ghost coordinates (Mesh.cpp:1013-1167, ngg+1 layers) and 2nd-order metrics/Jacobian
  * (Mesh.cpp:244-528) for one block.  x,y,z: raw coordinates, mx*my*mz, i fastest, already
  * scaled by gridScale.  jac (sz) and metric (9*sz) are written in the ngg-ghost layout.

@@ -160,6 +160,37 @@ static void metric_copy(metric_ctx *m, int i, int j, int k, int si, int sj, int 
   for (int l = 0; l < 9; ++l) m->met[id + l * m->sz] = m->met[is + l * m->sz];
 }
 
+void orc_ghost_coords(int mx, int my, int mz, int ngg, const double *x, const double *y, const double *z, double *gx,
+                      double *gy, double *gz) {
+  const int dim = mz == 1 ? 2 : 3;
+  arr3 X, Y, Z;
+  arr3_init(&X, mx, my, mz, ngg + 1);
+  arr3_init(&Y, mx, my, mz, ngg + 1);
+  arr3_init(&Z, mx, my, mz, ngg + 1);
+  for (int k = 0; k < mz; ++k)
+    for (int j = 0; j < my; ++j)
+      for (int i = 0; i < mx; ++i) {
+        const long s = ((long)k * my + j) * mx + i;
+        *A3(&X, i, j, k) = x[s];
+        *A3(&Y, i, j, k) = y[s];
+        *A3(&Z, i, j, k) = z[s];
+      }
+  ghost_grid(&X, &Y, &Z, ngg, dim);
+  const long d2 = mx + 2 * ngg, d1 = (long)(my + 2 * ngg) * d2;
+  for (int k = -ngg; k < mz + ngg; ++k)
+    for (int j = -ngg; j < my + ngg; ++j)
+      for (int i = -ngg; i < mx + ngg; ++i) {
+        const long id = (long)(k + ngg) * d1 + (long)(j + ngg) * d2 + (i + ngg);
+        const int kk = dim == 2 ? 0 : k;
+        gx[id] = *A3(&X, i, j, kk);
+        gy[id] = *A3(&Y, i, j, kk);
+        gz[id] = *A3(&Z, i, j, kk);
+      }
+  free(X.v);
+  free(Y.v);
+  free(Z.v);
+}
+
 int orc_metrics(int mx, int my, int mz, int ngg, const double *x, const double *y, const double *z, double *jac,
                 double *metric) {
   const int dim = mz == 1 ? 2 : 3;
