@@ -29,8 +29,11 @@ __device__ __forceinline__ double weno5_half(double v0, double v1, double v2, do
   const double b0 = t12 * e0 * e0 + 0.25 * f0 * f0;
   const double b1 = t12 * e1 * e1 + 0.25 * f1 * f1;
   const double b2 = t12 * e2 * e2 + 0.25 * f2 * f2;
-  const double a0 = 0.3 * rcp((eps + b0) * (eps + b0));
-  const double a1 = 0.6 * rcp((eps + b1) * (eps + b1));
+  // 1/d0 and 1/d1 from one reciprocal of their product (each d >= eps^2, so the product stays normal)
+  const double d0 = (eps + b0) * (eps + b0), d1 = (eps + b1) * (eps + b1);
+  const double r01 = rcp(d0 * d1);
+  const double a0 = 0.3 * (d1 * r01);
+  const double a1 = 0.6 * (d0 * r01);
   const double a2 = 0.1 * rcp((eps + b2) * (eps + b2));
   return (a0 * p0 + a1 * p1 + a2 * p2) * rcp(a0 + a1 + a2);
 }
@@ -57,10 +60,14 @@ __device__ __forceinline__ double weno7_half(const double *v, double eps) {
   double tau = b0 + 3 * b1 - 3 * b2 - b3;
   tau *= tau;
   constexpr double c0 = 1.0 / 35, c1 = 12.0 / 35, c2 = 18.0 / 35, c3 = 4.0 / 35;
-  const double a0 = c0 + c0 * tau * rcp((eps + b0) * (eps + b0));
-  const double a1 = c1 + c1 * tau * rcp((eps + b1) * (eps + b1));
-  const double a2 = c2 + c2 * tau * rcp((eps + b2) * (eps + b2));
-  const double a3 = c3 + c3 * tau * rcp((eps + b3) * (eps + b3));
+  // the four 1/d_k from two reciprocals of pair products (each d >= eps^2, so a product stays normal)
+  const double d0 = (eps + b0) * (eps + b0), d1 = (eps + b1) * (eps + b1);
+  const double d2 = (eps + b2) * (eps + b2), d3 = (eps + b3) * (eps + b3);
+  const double r01 = rcp(d0 * d1), r23 = rcp(d2 * d3);
+  const double a0 = c0 + c0 * tau * (d1 * r01);
+  const double a1 = c1 + c1 * tau * (d0 * r01);
+  const double a2 = c2 + c2 * tau * (d3 * r23);
+  const double a3 = c3 + c3 * tau * (d2 * r23);
   const double q0 = -3 * v[0] + 13 * v[1] - 23 * v[2] + 25 * v[3];
   const double q1 = v[1] - 5 * v[2] + 13 * v[3] + 3 * v[4];
   const double q2 = -v[2] + 7 * v[3] + 7 * v[4] - v[5];
