@@ -82,11 +82,13 @@ __device__ __forceinline__ void flux_ausm(const double *L, const double *R, cons
   const double k1 = 0.5 * (jl * ml[0] + jr * mr[0]);
   const double k2 = 0.5 * (jl * ml[1] + jr * mr[1]);
   const double k3 = 0.5 * (jl * ml[2] + jr * mr[2]);
-  const double gk = sqrt(k1 * k1 + k2 * k2 + k3 * k3);
-  const double ul = (k1 * L[1] + k2 * L[2] + k3 * L[3]) / gk;
-  const double ur = (k1 * R[1] + k2 * R[2] + k3 * R[3]) / gk;
-  const double c = 0.5 * (sqrt(kGamma * L[4] / L[0]) + sqrt(kGamma * R[4] / R[0]));
-  const double Ml = ul / c, Mr = ur / c;
+  // divisions as products with the hardware reciprocal refined to the rounding level (wenodev::rcp)
+  const double gk = sqrt(k1 * k1 + k2 * k2 + k3 * k3), igk = wenodev::rcp(gk);
+  const double ul = (k1 * L[1] + k2 * L[2] + k3 * L[3]) * igk;
+  const double ur = (k1 * R[1] + k2 * R[2] + k3 * R[3]) * igk;
+  const double c = 0.5 * (sqrt(kGamma * L[4] * wenodev::rcp(L[0])) + sqrt(kGamma * R[4] * wenodev::rcp(R[0])));
+  const double ic = wenodev::rcp(c);
+  const double Ml = ul * ic, Mr = ur * ic;
   constexpr double alpha = 3 / 16.0;
   double mlp, plp, mrn, prn;
   if (fabs(Ml) > 1) {
@@ -114,7 +116,7 @@ __device__ __forceinline__ void flux_ausm(const double *L, const double *R, cons
   fc[1] = co * S[1] + pc * k1;
   fc[2] = co * S[2] + pc * k2;
   fc[3] = co * S[3] + pc * k3;
-  fc[4] = co * (S[5] + S[4]) / S[0];
+  fc[4] = co * (S[5] + S[4]) * wenodev::rcp(S[0]);
 }
 
 // HLLC with Roe-averaged wave-speed bounds (InviscidScheme.cu:471-599, single species).
@@ -241,7 +243,7 @@ __device__ __forceinline__ void flux_lf(const double *L, const double *R, const 
 }
 
 __device__ __forceinline__ void to_primitive(const double *q, double *p) {
-  const double ri = 1.0 / q[0];
+  const double ri = wenodev::rcp(q[0]);
   p[0] = q[0];
   p[1] = q[1] * ri;
   p[2] = q[2] * ri;
@@ -286,16 +288,16 @@ __global__ __launch_bounds__(256) void k_inviscid(DevBlock b, Scheme sc, double 
   const double *qa = w[R - 1], *qb = w[R];
   const double ra = qa[0], rb = qb[0];
   const double sab = sqrt(ra * rb);
-  const double ca = 1 / (ra + sab), cb = 1 / (sab + rb);
+  const double ca = wenodev::rcp(ra + sab), cb = wenodev::rcp(sab + rb);
   const double um = ca * qa[1] + cb * qb[1], vm = ca * qa[2] + cb * qb[2], wm = ca * qa[3] + cb * qb[3];
-  const double pa = (kGamma - 1) * (qa[4] - 0.5 * (qa[1] * qa[1] + qa[2] * qa[2] + qa[3] * qa[3]) / ra);
-  const double pb = (kGamma - 1) * (qb[4] - 0.5 * (qb[1] * qb[1] + qb[2] * qb[2] + qb[3] * qb[3]) / rb);
+  const double pa = (kGamma - 1) * (qa[4] - 0.5 * (qa[1] * qa[1] + qa[2] * qa[2] + qa[3] * qa[3]) * wenodev::rcp(ra));
+  const double pb = (kGamma - 1) * (qb[4] - 0.5 * (qb[1] * qb[1] + qb[2] * qb[2] + qb[3] * qb[3]) * wenodev::rcp(rb));
   const double H = ca * (qa[4] + pa) + cb * (qb[4] + pb);
   const double ek = 0.5 * (um * um + vm * vm + wm * wm);
   const double c = sqrt((kGamma - 1) * (H - ek));
   double kx = 0.5 * (ml[0] + mr[0]), ky = 0.5 * (ml[1] + mr[1]), kz = 0.5 * (ml[2] + mr[2]);
   {
-    const double r = 1.0 / sqrt(kx * kx + ky * ky + kz * kz);
+    const double r = wenodev::rcp(sqrt(kx * kx + ky * ky + kz * kz));
     kx *= r; ky *= r; kz *= r;
   }
   double nx = 0, ny = 0, nz = 0;
@@ -305,18 +307,18 @@ __global__ __launch_bounds__(256) void k_inviscid(DevBlock b, Scheme sc, double 
     ny = kz; nz = -ky;
   }
   {
-    const double r = 1.0 / sqrt(nx * nx + ny * ny + nz * nz);
+    const double r = wenodev::rcp(sqrt(nx * nx + ny * ny + nz * nz));
     nx *= r; ny *= r; nz *= r;
   }
   double qx = ky * nz - kz * ny, qy = kz * nx - kx * nz, qz = kx * ny - ky * nx;
   {
-    const double r = 1.0 / sqrt(qx * qx + qy * qy + qz * qz);
+    const double r = wenodev::rcp(sqrt(qx * qx + qy * qy + qz * qz));
     qx *= r; qy *= r; qz *= r;
   }
   const double Uk = kx * um + ky * vm + kz * wm;
   const double Un = nx * um + ny * vm + nz * wm;
   const double Uq = qx * um + qy * vm + qz * wm;
-  const double gm1 = kGamma - 1, ib = 1.0 / (c * c), al = gm1 * ek;
+  const double gm1 = kGamma - 1, ib = wenodev::rcp(c * c), al = gm1 * ek;
   double Lm[5][5] = {
       {(al + Uk * c) * ib * 0.5, -(gm1 * um + kx * c) * ib * 0.5, -(gm1 * vm + ky * c) * ib * 0.5,
        -(gm1 * wm + kz * c) * ib * 0.5, gm1 * ib * 0.5},
