@@ -318,34 +318,31 @@ __global__ __launch_bounds__(256) void k_inviscid(DevBlock b, Scheme sc, double 
   const double Uk = kx * um + ky * vm + kz * wm;
   const double Un = nx * um + ny * vm + nz * wm;
   const double Uq = qx * um + qy * vm + qz * wm;
-  const double gm1 = kGamma - 1, ib = wenodev::rcp(c * c), al = gm1 * ek;
-  double Lm[5][5] = {
-      {(al + Uk * c) * ib * 0.5, -(gm1 * um + kx * c) * ib * 0.5, -(gm1 * vm + ky * c) * ib * 0.5,
-       -(gm1 * wm + kz * c) * ib * 0.5, gm1 * ib * 0.5},
-      {-Un, nx, ny, nz, 0.0},
-      {-Uq, qx, qy, qz, 0.0},
-      {1 - al * ib, gm1 * um * ib, gm1 * vm * ib, gm1 * wm * ib, -gm1 * ib},
-      {(al - Uk * c) * ib * 0.5, -(gm1 * um - kx * c) * ib * 0.5, -(gm1 * vm - ky * c) * ib * 0.5,
-       -(gm1 * wm - kz * c) * ib * 0.5, gm1 * ib * 0.5}};
-  // project in place: w[m] <- L q[m]
+  // project in place: w[m] <- L q[m].  The rows of L (WENO.cu:73-124) in factored form, with
+  // P = u.(rho u), K = k.(rho u), X = ek rho + E - P, Y = Uk rho - K:
+  //   row 0 = (gm1 X + c Y) / 2c^2, row 4 = (gm1 X - c Y) / 2c^2, row 3 = rho - gm1 X / c^2,
+  //   row 1 = n.(rho u) - Un rho,   row 2 = q.(rho u) - Uq rho.
+  const double gm1 = kGamma - 1, ib = wenodev::rcp(c * c);
+  const double hg = 0.5 * gm1 * ib, hc = 0.5 * c * ib, g2 = -gm1 * ib;
 #pragma unroll
   for (int m = 0; m < N; ++m) {
-    double t[5];
-#pragma unroll
-    for (int l = 0; l < 5; ++l)
-      t[l] = Lm[l][0] * w[m][0] + Lm[l][1] * w[m][1] + Lm[l][2] * w[m][2] + Lm[l][3] * w[m][3] + Lm[l][4] * w[m][4];
-#pragma unroll
-    for (int l = 0; l < 5; ++l) w[m][l] = t[l];
+    const double r = w[m][0], mx = w[m][1], my = w[m][2], mz = w[m][3], e = w[m][4];
+    const double P = fma(um, mx, fma(vm, my, wm * mz));
+    const double K = fma(kx, mx, fma(ky, my, kz * mz));
+    const double X = fma(ek, r, e) - P, hY = hc * fma(Uk, r, -K);
+    w[m][0] = fma(hg, X, hY);
+    w[m][1] = fma(-Un, r, fma(nx, mx, fma(ny, my, nz * mz)));
+    w[m][2] = fma(-Uq, r, fma(qx, mx, fma(qy, my, qz * mz)));
+    w[m][3] = fma(g2, X, r);
+    w[m][4] = fma(hg, X, -hY);
   }
   double WL[5], WR[5];
   const double eps = sc.weno_eps;
 #pragma unroll
   for (int l = 0; l < 5; ++l) {
     if constexpr (R == 4) {
-      const double a[7] = {w[0][l], w[1][l], w[2][l], w[3][l], w[4][l], w[5][l], w[6][l]};
-      const double bb[7] = {w[7][l], w[6][l], w[5][l], w[4][l], w[3][l], w[2][l], w[1][l]};
-      WL[l] = weno7_half(a, eps);
-      WR[l] = weno7_half(bb, eps);
+      const double a[8] = {w[0][l], w[1][l], w[2][l], w[3][l], w[4][l], w[5][l], w[6][l], w[7][l]};
+      wenodev::weno7_pair(a, eps, WL[l], WR[l]);
     } else {
       WL[l] = weno5_half(w[0][l], w[1][l], w[2][l], w[3][l], w[4][l], eps);
       WR[l] = weno5_half(w[5][l], w[4][l], w[3][l], w[2][l], w[1][l], eps);
@@ -358,7 +355,7 @@ __global__ __launch_bounds__(256) void k_inviscid(DevBlock b, Scheme sc, double 
     Q[1] = um * s - kx * c * d + nx * W[1] + qx * W[2];
     Q[2] = vm * s - ky * c * d + ny * W[1] + qy * W[2];
     Q[3] = wm * s - kz * c * d + nz * W[1] + qz * W[2];
-    Q[4] = H * s - Uk * c * d + Un * W[1] + Uq * W[2] - c * c / gm1 * W[3];
+    Q[4] = H * s - Uk * c * d + Un * W[1] + Uq * W[2] - c * c * (1.0 / gm1) * W[3];
   };
   double QL[5], QR[5], PL[6], PR[6];
   back(WL, QL);

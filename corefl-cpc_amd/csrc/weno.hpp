@@ -38,41 +38,79 @@ __device__ __forceinline__ double weno5_half(double v0, double v1, double v2, do
   return (a0 * p0 + a1 * p1 + a2 * p2) * rcp(a0 + a1 + a2);
 }
 
-// Upwind-biased value at the right face of v[3] from v[0..6] (the fPlus part of WENO.cu:3116-3222,
-// Z-type weights with tau7 = (b0 + 3 b1 - 3 b2 - b3)^2).
-__device__ __forceinline__ double weno7_half(const double *v, double eps) {
-  constexpr double o6 = 1.0 / 6, d12 = 13.0 / 12.0, d13 = 1043.0 / 960, d14 = 1.0 / 12;
-  double s1 = o6 * (-2 * v[0] + 9 * v[1] - 18 * v[2] + 11 * v[3]);
-  double s2 = -v[0] + 4 * v[1] - 5 * v[2] + 2 * v[3];
-  double s3 = -v[0] + 3 * v[1] - 3 * v[2] + v[3];
-  const double b0 = s1 * s1 + d12 * s2 * s2 + d13 * s3 * s3 + d14 * s1 * s3;
-  s1 = o6 * (v[1] - 6 * v[2] + 3 * v[3] + 2 * v[4]);
-  s2 = v[2] - 2 * v[3] + v[4];
-  s3 = -v[1] + 3 * v[2] - 3 * v[3] + v[4];
-  const double b1 = s1 * s1 + d12 * s2 * s2 + d13 * s3 * s3 + d14 * s1 * s3;
-  s1 = o6 * (-2 * v[2] - 3 * v[3] + 6 * v[4] - v[5]);
-  s3 = -v[2] + 3 * v[3] - 3 * v[4] + v[5];
-  const double b2 = s1 * s1 + d12 * s2 * s2 + d13 * s3 * s3 + d14 * s1 * s3;
-  s1 = o6 * (-11 * v[3] + 18 * v[4] - 9 * v[5] + 2 * v[6]);
-  s2 = 2 * v[3] - 5 * v[4] + 4 * v[5] - v[6];
-  s3 = -v[3] + 3 * v[4] - 3 * v[5] + v[6];
-  const double b3 = s1 * s1 + d12 * s2 * s2 + d13 * s3 * s3 + d14 * s1 * s3;
-  double tau = b0 + 3 * b1 - 3 * b2 - b3;
+// ---------------------------------------------------------------- WENO7 (the fPlus part of WENO.cu:3116-3222)
+// The smoothness indicator of candidate window k (v[k..k+3]) is beta_k = s1^2 + 13/12 s2^2 + 1043/960 s3^2
+// + 1/12 s1 s3, with s1 a first-derivative estimate, s2 a second and s3 the third difference of the window.
+// Here every beta is carried times 36, with S1 = 6 s1:
+//   36 beta = S1 (S1 + 0.5 s3) + 39 s2^2 + 39.1125 s3^2,
+// and eps times 36 with it: the Z weights c_k (1 + tau / (eps + beta_k)^2), tau = (b0 + 3 b1 - 3 b2 - b3)^2,
+// do not change under a common scale of eps and the betas.  The third difference of a window read mirrored is
+// the negative of the one read forwards, so the two halves of an interface share three of them.
+
+// third difference of v[0..3]: -v0 + 3 v1 - 3 v2 + v3
+__device__ __forceinline__ double w7_third(double v0, double v1, double v2, double v3) {
+  return fma(3.0, v1 - v2, v3 - v0);
+}
+
+// 36 beta of one window from S1, s2, its third difference t (sign as read) and tt = 39.1125 t^2
+__device__ __forceinline__ double w7_beta36(double S1, double s2, double t, double tt) {
+  return fma(S1, fma(0.5, t, S1), fma(39.0 * s2, s2, tt));
+}
+
+// Upwind-biased value at the right face of v[3] from v[0..6], given the four windows' third differences
+// t[k] (of v[k..k+3], read forwards) and tt[k] = 39.1125 t[k]^2.
+__device__ __forceinline__ double weno7_half_t(const double *v, const double *t, const double *tt, double eps36) {
+  const double s2m = fma(-2.0, v[3], v[2] + v[4]);
+  const double b0 = w7_beta36(fma(-2.0, v[0], fma(9.0, v[1], fma(-18.0, v[2], 11.0 * v[3]))),
+                              fma(4.0, v[1], fma(-5.0, v[2], fma(2.0, v[3], -v[0]))), t[0], tt[0]);
+  const double b1 = w7_beta36(fma(2.0, v[4], fma(3.0, v[3], fma(-6.0, v[2], v[1]))), s2m, t[1], tt[1]);
+  const double b2 = w7_beta36(fma(6.0, v[4], fma(-3.0, v[3], fma(-2.0, v[2], -v[5]))), s2m, t[2], tt[2]);
+  const double b3 = w7_beta36(fma(2.0, v[6], fma(-9.0, v[5], fma(18.0, v[4], -11.0 * v[3]))),
+                              fma(2.0, v[3], fma(-5.0, v[4], fma(4.0, v[5], -v[6]))), t[3], tt[3]);
+  double tau = fma(3.0, b1 - b2, b0 - b3);
   tau *= tau;
   constexpr double c0 = 1.0 / 35, c1 = 12.0 / 35, c2 = 18.0 / 35, c3 = 4.0 / 35;
   // the four 1/d_k from two reciprocals of pair products (each d >= eps^2, so a product stays normal)
-  const double d0 = (eps + b0) * (eps + b0), d1 = (eps + b1) * (eps + b1);
-  const double d2 = (eps + b2) * (eps + b2), d3 = (eps + b3) * (eps + b3);
-  const double r01 = rcp(d0 * d1), r23 = rcp(d2 * d3);
-  const double a0 = c0 + c0 * tau * (d1 * r01);
-  const double a1 = c1 + c1 * tau * (d0 * r01);
-  const double a2 = c2 + c2 * tau * (d3 * r23);
-  const double a3 = c3 + c3 * tau * (d2 * r23);
+  const double d0 = (eps36 + b0) * (eps36 + b0), d1 = (eps36 + b1) * (eps36 + b1);
+  const double d2 = (eps36 + b2) * (eps36 + b2), d3 = (eps36 + b3) * (eps36 + b3);
+  const double r01 = tau * rcp(d0 * d1), r23 = tau * rcp(d2 * d3);
+  const double a0 = fma(c0 * d1, r01, c0);
+  const double a1 = fma(c1 * d0, r01, c1);
+  const double a2 = fma(c2 * d3, r23, c2);
+  const double a3 = fma(c3 * d2, r23, c3);
   const double q0 = -3 * v[0] + 13 * v[1] - 23 * v[2] + 25 * v[3];
   const double q1 = v[1] - 5 * v[2] + 13 * v[3] + 3 * v[4];
   const double q2 = -v[2] + 7 * v[3] + 7 * v[4] - v[5];
   const double q3 = 3 * v[3] + 13 * v[4] - 5 * v[5] + v[6];
   return (1.0 / 12) * (a0 * q0 + a1 * q1 + a2 * q2 + a3 * q3) * rcp(a0 + a1 + a2 + a3);
+}
+
+// Upwind-biased value at the right face of v[3] from v[0..6] (Z-type weights with tau7 = (b0 + 3 b1 - 3 b2 - b3)^2).
+__device__ __forceinline__ double weno7_half(const double *v, double eps) {
+  double t[4], tt[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    t[k] = w7_third(v[k], v[k + 1], v[k + 2], v[k + 3]);
+    tt[k] = (39.1125 * t[k]) * t[k];
+  }
+  return weno7_half_t(v, t, tt, 36.0 * eps);
+}
+
+// Both halves at the interface between w[3] and w[4] of the eight values w[0..7]: lo is the value upwind from
+// the left (v = w[0..6]), hi the one upwind from the right (v = w[7], w[6], .., w[1]).  The right half's
+// window k is w[4-k..7-k] read backwards, whose third difference is minus the forward one of that window.
+__device__ __forceinline__ void weno7_pair(const double *w, double eps, double &lo, double &hi) {
+  double T[5], TT[5];
+#pragma unroll
+  for (int j = 0; j < 5; ++j) {
+    T[j] = w7_third(w[j], w[j + 1], w[j + 2], w[j + 3]);
+    TT[j] = (39.1125 * T[j]) * T[j];
+  }
+  const double eps36 = 36.0 * eps;
+  lo = weno7_half_t(w, T, TT, eps36);
+  const double m[7] = {w[7], w[6], w[5], w[4], w[3], w[2], w[1]};
+  const double tm[4] = {-T[4], -T[3], -T[2], -T[1]}, ttm[4] = {TT[4], TT[3], TT[2], TT[1]};
+  hi = weno7_half_t(m, tm, ttm, eps36);
 }
 
 // The linear upwind halves the hybrid scheme uses where the stencil is smooth: the WENO5 candidates with the
