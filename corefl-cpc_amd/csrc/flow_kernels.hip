@@ -86,7 +86,8 @@ __device__ __forceinline__ void flux_ausm(const double *L, const double *R, cons
   const double gk = sqrt(k1 * k1 + k2 * k2 + k3 * k3), igk = wenodev::rcp(gk);
   const double ul = (k1 * L[1] + k2 * L[2] + k3 * L[3]) * igk;
   const double ur = (k1 * R[1] + k2 * R[2] + k3 * R[3]) * igk;
-  const double c = 0.5 * (sqrt(kGamma * L[4] * wenodev::rcp(L[0])) + sqrt(kGamma * R[4] * wenodev::rcp(R[0])));
+  const double il = wenodev::rcp(L[0]), ir = wenodev::rcp(R[0]);
+  const double c = 0.5 * (sqrt(kGamma * L[4] * il) + sqrt(kGamma * R[4] * ir));
   const double ic = wenodev::rcp(c);
   const double Ml = ul * ic, Mr = ur * ic;
   constexpr double alpha = 3 / 16.0;
@@ -116,7 +117,7 @@ __device__ __forceinline__ void flux_ausm(const double *L, const double *R, cons
   fc[1] = co * S[1] + pc * k1;
   fc[2] = co * S[2] + pc * k2;
   fc[3] = co * S[3] + pc * k3;
-  fc[4] = co * (S[5] + S[4]) * wenodev::rcp(S[0]);
+  fc[4] = co * (S[5] + S[4]) * (mass >= 0 ? il : ir);
 }
 
 // HLLC with Roe-averaged wave-speed bounds (InviscidScheme.cu:471-599, single species).
@@ -637,7 +638,9 @@ using stencil::dd_mid;
 // point.  kz trades that reuse against the number of waves in flight (COREFL_KZ overrides the default).
 
 __device__ __forceinline__ double sutherland(double T) {
-  return 1.7894e-5 * pow(T / 288.16, 1.5) * (288.16 + 110) / (T + 110); // Transport.cu:6-9
+  // Transport.cu:6-9, with (T/288.16)^1.5 as x sqrt(x) and the divisions as products with reciprocals
+  const double x = T * (1.0 / 288.16);
+  return 1.7894e-5 * (x * sqrt(x)) * (288.16 + 110) * wenodev::rcp(T + 110);
 }
 
 // compute_fv/gv/hv_2nd_order<Air> (ViscousScheme.cu:614-1010): viscous flux on the face between node t and
@@ -678,9 +681,9 @@ __global__ __launch_bounds__(256) void k_viscous_face(DevBlock b, Scheme sc, Til
 #pragma unroll
     for (int c = 0; c < 3; ++c) d[q][c] = g[q][0] * m[c] + g[q][1] * m[3 + c] + g[q][2] * m[6 + c];
   const double mu = 0.5 * (sutherland(b.bv[id + 5 * sz]) + sutherland(b.bv[ie + 5 * sz]));
-  const double txx = mu * (4 * d[0][0] - 2 * d[1][1] - 2 * d[2][2]) / 3.0;
-  const double tyy = mu * (4 * d[1][1] - 2 * d[0][0] - 2 * d[2][2]) / 3.0;
-  const double tzz = mu * (4 * d[2][2] - 2 * d[0][0] - 2 * d[1][1]) / 3.0;
+  const double txx = mu * (4 * d[0][0] - 2 * d[1][1] - 2 * d[2][2]) * (1.0 / 3);
+  const double tyy = mu * (4 * d[1][1] - 2 * d[0][0] - 2 * d[2][2]) * (1.0 / 3);
+  const double tzz = mu * (4 * d[2][2] - 2 * d[0][0] - 2 * d[1][1]) * (1.0 / 3);
   const double txy = mu * (d[0][1] + d[1][0]), txz = mu * (d[0][2] + d[2][0]), tyz = mu * (d[1][2] + d[2][1]);
   double kj[3];
 #pragma unroll
@@ -693,7 +696,7 @@ __global__ __launch_bounds__(256) void k_viscous_face(DevBlock b, Scheme sc, Til
   const double vm = 0.5 * (b.bv[id + 2 * sz] + b.bv[ie + 2 * sz]);
   const double wm = 0.5 * (b.bv[id + 3 * sz] + b.bv[ie + 3 * sz]);
   constexpr double cp = kGamma * kRair / (kGamma - 1);
-  const double kc = mu / sc.prandtl * cp;
+  const double kc = mu * (cp / sc.prandtl);
   double *F = DIR == 0 ? b.fv : (DIR == 1 ? b.gv : b.hv);
   F[id + sz] = f1;
   F[id + 2 * sz] = f2;
@@ -752,12 +755,12 @@ __global__ __launch_bounds__(256) void k_viscous_flux(DevBlock b, Scheme sc, Til
     const double u = z3 ? wz[0][H] : b.bv[id + sz], v = z3 ? wz[1][H] : b.bv[id + 2 * sz];
     const double w = z3 ? wz[2][H] : b.bv[id + 3 * sz], T = z3 ? wz[3][H] : b.bv[id + 5 * sz];
     const double mu = sutherland(T);
-    const double txx = mu * (4 * d[0][0] - 2 * d[1][1] - 2 * d[2][2]) / 3.0;
-    const double tyy = mu * (4 * d[1][1] - 2 * d[0][0] - 2 * d[2][2]) / 3.0;
-    const double tzz = mu * (4 * d[2][2] - 2 * d[0][0] - 2 * d[1][1]) / 3.0;
+    const double txx = mu * (4 * d[0][0] - 2 * d[1][1] - 2 * d[2][2]) * (1.0 / 3);
+    const double tyy = mu * (4 * d[1][1] - 2 * d[0][0] - 2 * d[2][2]) * (1.0 / 3);
+    const double tzz = mu * (4 * d[2][2] - 2 * d[0][0] - 2 * d[1][1]) * (1.0 / 3);
     const double txy = mu * (d[0][1] + d[1][0]), txz = mu * (d[0][2] + d[2][0]), tyz = mu * (d[1][2] + d[2][1]);
     constexpr double cp = kGamma * kRair / (kGamma - 1);
-    const double kc = mu / sc.prandtl * cp;
+    const double kc = mu * (cp / sc.prandtl);
     const double Ex = u * txx + v * txy + w * txz + kc * d[3][0];
     const double Ey = u * txy + v * tyy + w * tyz + kc * d[3][1];
     const double Ez = u * txz + v * tyz + w * tzz + kc * d[3][2];
@@ -956,7 +959,7 @@ __global__ __launch_bounds__(256) void k_update(DevBlock b, int rk, const double
       for (int l = 0; l < 5; ++l) b.dq[iq + l * b.n] = r[l];
       continue;
     }
-    const double dj = dt / b.jac[id];
+    const double dj = dt * wenodev::rcp(b.jac[id]);
     double q[5];
     if (rk == 0) {
 #pragma unroll
@@ -973,7 +976,7 @@ __global__ __launch_bounds__(256) void k_update(DevBlock b, int rk, const double
     if (b.mz == 1) q[3] = 0;
 #pragma unroll
     for (int l = 0; l < 5; ++l) b.cv[id + l * sz] = q[l];
-    const double di = 1.0 / q[0];
+    const double di = wenodev::rcp(q[0]);
     const double u = q[1] * di, v = q[2] * di, w = q[3] * di;
     const double p = (kGamma - 1) * (q[4] - 0.5 * q[0] * (u * u + v * v + w * w));
     b.bv[id] = q[0];
@@ -981,7 +984,7 @@ __global__ __launch_bounds__(256) void k_update(DevBlock b, int rk, const double
     b.bv[id + 2 * sz] = v;
     b.bv[id + 3 * sz] = w;
     b.bv[id + 4 * sz] = p;
-    b.bv[id + 5 * sz] = p * kMwAir * di / kRu;
+    b.bv[id + 5 * sz] = p * kMwAir * di * (1.0 / kRu);
   }
 }
 

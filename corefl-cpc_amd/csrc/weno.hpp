@@ -15,6 +15,13 @@ __device__ __forceinline__ double rcp(double d) {
   return r;
 }
 
+// The hardware reciprocal refined by one Newton step: relative error below 2.3e-15 (the hardware value is good to
+// about 5e-8 and one step squares that).  Used where it scales only the nonlinear part of a WENO weight.
+__device__ __forceinline__ double rcp1(double d) {
+  const double r = __builtin_amdgcn_rcp(d);
+  return fma(fma(-d, r, 1.0), r, r);
+}
+
 // ---------------------------------------------------------------- WENO halves
 // Upwind-biased value at the right face of v[2] from v[0..4] (the fPlus part of WENO.cu:3059-3114).
 // The downwind-biased value is the same function of the mirrored stencil.
@@ -73,7 +80,7 @@ __device__ __forceinline__ double weno7_half_t(const double *v, const double *t,
   // the four 1/d_k from two reciprocals of pair products (each d >= eps^2, so a product stays normal)
   const double d0 = (eps36 + b0) * (eps36 + b0), d1 = (eps36 + b1) * (eps36 + b1);
   const double d2 = (eps36 + b2) * (eps36 + b2), d3 = (eps36 + b3) * (eps36 + b3);
-  const double r01 = tau * rcp(d0 * d1), r23 = tau * rcp(d2 * d3);
+  const double r01 = tau * rcp1(d0 * d1), r23 = tau * rcp1(d2 * d3);
   const double a0 = fma(c0 * d1, r01, c0);
   const double a1 = fma(c1 * d0, r01, c1);
   const double a2 = fma(c2 * d3, r23, c2);
