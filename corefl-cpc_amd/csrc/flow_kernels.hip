@@ -31,45 +31,51 @@ __device__ __forceinline__ long nid(const DevBlock &b, int i, int j, int k) {
 
 // XCD-aware tile order.  Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.
 // A 1-D launch of 8*P workgroups is remapped so that the workgroups landing on one XCD take a contiguous
-// run of tiles in which the sweep axis varies fastest: neighbouring planes of a stencil are then read by
-// workgroups sharing an L2.  The map b -> (b % 8) * P + b / 8 is a bijection on [0, 8P).
+// run of tiles: along the plane (consecutive rows, the i and j stencils) or, for the z sweep, along k, so that
+// the nodes of a stencil are read by workgroups sharing an L2.  The map b -> (b % 8) * P + b / 8 is a
+// bijection on [0, 8P).
+// Within a plane, a workgroup of 256 threads takes 256 consecutive nodes of the launch's nx x ny nodes in row-major
+// order (i fastest), so that only the last workgroup of a plane has idle lanes whatever nx is: a 64-wide tiling of
+// 257 nodes leaves a fifth wave per row with one active lane, a quarter more waves than nodes need.
 struct TileGrid {
-  int gx, gy, gz; // tiles along i (64 wide), j (4 wide) and k (1 wide)
-  int fast;       // 0: i then j then k (each XCD takes a contiguous slab of planes), 1: j fastest, 2: k fastest
+  int gx, gy, gz; // workgroups per plane (gy = 1), and planes (or plane groups) along k
+  int fast;       // 0: plane chunk then k (each XCD takes a contiguous run), 2: k fastest
   int z0;         // first k tile (a launch over a slab of planes)
+  int nx, ny;     // nodes of a plane covered by the launch
 };
 
 struct Tile {
-  int x, y, z;
+  int z;    // plane (or plane group) index
+  int i, j; // this thread's node, before the kernel's own offsets
 };
 
+// A launch over the nx x ny nodes of gz planes (or plane groups) starting at z0.
+inline TileGrid plane_grid(int nx, int ny, int gz, int fast, int z0) {
+  return TileGrid{(nx * ny + 255) / 256, 1, gz, fast, z0, nx, ny};
+}
+
 __device__ __forceinline__ bool xcd_tile(const TileGrid &t, Tile &out) {
-  const unsigned total = (unsigned)t.gx * t.gy * t.gz;
+  const unsigned total = (unsigned)t.gx * t.gz;
   const unsigned per = (total + 7) / 8;
   const unsigned n = (blockIdx.x % 8) * per + blockIdx.x / 8;
   if (n >= total) return false;
-  if (t.fast == 0) {
-    out.x = n % t.gx;
-    const unsigned r = n / t.gx;
-    out.y = r % t.gy;
-    out.z = r / t.gy;
-  } else if (t.fast == 2) {
+  unsigned x;
+  if (t.fast == 2) {
     out.z = n % t.gz;
-    const unsigned r = n / t.gz;
-    out.x = r % t.gx;
-    out.y = r / t.gx;
+    x = n / t.gz;
   } else {
-    out.y = n % t.gy;
-    const unsigned r = n / t.gy;
-    out.x = r % t.gx;
-    out.z = r / t.gx;
+    x = n % t.gx;
+    out.z = n / t.gx;
   }
   out.z += t.z0;
-  return true;
+  const unsigned p = x * 256u + threadIdx.y * 64u + threadIdx.x, nx = t.nx;
+  out.j = (int)(p / nx);
+  out.i = (int)(p - (unsigned)out.j * nx);
+  return out.j < t.ny;
 }
 
 inline unsigned tile_blocks(const TileGrid &t) {
-  const unsigned total = (unsigned)t.gx * t.gy * t.gz;
+  const unsigned total = (unsigned)t.gx * t.gz;
   return 8 * ((total + 7) / 8);
 }
 
@@ -262,7 +268,7 @@ template<int DIR, int R, int RS>
 __global__ __launch_bounds__(256) void k_inviscid(DevBlock b, Scheme sc, double *F, TileGrid tg) {
   Tile t;
   if (!xcd_tile(tg, t)) return;
-  int i = t.x * 64 + threadIdx.x, j = t.y * 4 + threadIdx.y, k = t.z;
+  int i = t.i, j = t.j, k = t.z;
   if constexpr (DIR == 0) i -= 1;
   else if constexpr (DIR == 1) j -= 1;
   else k -= 1;
@@ -395,7 +401,7 @@ template<int DIR, int REC, int RS>
 __global__ __launch_bounds__(256) void k_inviscid_pv(DevBlock b, Scheme sc, double *F, TileGrid tg) {
   Tile t;
   if (!xcd_tile(tg, t)) return;
-  int i = t.x * 64 + threadIdx.x, j = t.y * 4 + threadIdx.y, k = t.z;
+  int i = t.i, j = t.j, k = t.z;
   if constexpr (DIR == 0) i -= 1;
   else if constexpr (DIR == 1) j -= 1;
   else k -= 1;
@@ -452,7 +458,7 @@ template<int DIR, int R, bool CH>
 __global__ __launch_bounds__(256, 2) void k_fsweno(DevBlock b, Scheme sc, double *F, TileGrid tg) {
   Tile t;
   if (!xcd_tile(tg, t)) return;
-  int i = t.x * 64 + threadIdx.x, j = t.y * 4 + threadIdx.y, k = t.z;
+  int i = t.i, j = t.j, k = t.z;
   if constexpr (DIR == 0) i -= 1;
   else if constexpr (DIR == 1) j -= 1;
   else k -= 1;
@@ -651,7 +657,7 @@ template<int DIR>
 __global__ __launch_bounds__(256) void k_viscous_face(DevBlock b, Scheme sc, TileGrid tg) {
   Tile t;
   if (!xcd_tile(tg, t)) return;
-  int i = t.x * 64 + threadIdx.x, j = t.y * 4 + threadIdx.y, k = t.z;
+  int i = t.i, j = t.j, k = t.z;
   if constexpr (DIR == 0) i -= 1;
   else if constexpr (DIR == 1) j -= 1;
   else k -= 1;
@@ -709,7 +715,7 @@ template<int ORDER>
 __global__ __launch_bounds__(256) void k_viscous_flux(DevBlock b, Scheme sc, TileGrid tg, int kz) {
   Tile t;
   if (!xcd_tile(tg, t)) return;
-  const int i = t.x * 64 + threadIdx.x, j = t.y * 4 + threadIdx.y;
+  const int i = t.i, j = t.j;
   if (i >= b.mx || j >= b.my) return;
   const int k0 = t.z * kz, k1 = min(k0 + kz, b.mz);
   const long sz = b.sz, d1 = b.d1, id0 = nid(b, i, j, k0);
@@ -876,7 +882,7 @@ __global__ __launch_bounds__(256) void k_update(DevBlock b, int rk, const double
                                                 int kz) {
   Tile t;
   if (!xcd_tile(tg, t)) return;
-  const int i = t.x * 64 + threadIdx.x, j = t.y * 4 + threadIdx.y;
+  const int i = t.i, j = t.j;
   if (i >= b.mx || j >= b.my) return;
   const int k0 = t.z * kz, k1 = min(k0 + kz, b.mz);
   const long sz = b.sz, d1 = b.d1, id0 = nid(b, i, j, k0);
@@ -1167,7 +1173,7 @@ int march_kz() {
 
 // Tiles of the z-marching kernels: 64 x 4 columns of kz planes each.
 TileGrid march_tiles(const DevBlock &b, int kz) {
-  return TileGrid{(b.mx + 63) / 64, (b.my + 3) / 4, (b.mz + kz - 1) / kz, 0};
+  return plane_grid(b.mx, b.my, (b.mz + kz - 1) / kz, 0, 0);
 }
 
 // compute_shock_sensor (FieldOperation.cu:74-179) at the block's nodes, air and mixtures alike (it reads bv and the
@@ -1242,12 +1248,12 @@ hipError_t launch_viscous_flux(const DevBlock &b, const Scheme &sc, hipStream_t 
   const dim3 tpb(64, 4, 1);
   if (sc.viscous_order == 2) {
     // one launch per face direction, over the faces -1/2 .. m-1/2 along it
-    const TileGrid tx{(b.mx + 1 + 63) / 64, (b.my + 3) / 4, b.mz, 0};
-    const TileGrid ty{(b.mx + 63) / 64, (b.my + 1 + 3) / 4, b.mz, 1};
+    const TileGrid tx = plane_grid(b.mx + 1, b.my, b.mz, 0, 0);
+    const TileGrid ty = plane_grid(b.mx, b.my + 1, b.mz, 0, 0);
     k_viscous_face<0><<<tile_blocks(tx), tpb, 0, s>>>(b, sc, tx);
     k_viscous_face<1><<<tile_blocks(ty), tpb, 0, s>>>(b, sc, ty);
     if (b.dim == 3) {
-      const TileGrid tz{(b.mx + 63) / 64, (b.my + 3) / 4, b.mz + 1, 2};
+      const TileGrid tz = plane_grid(b.mx, b.my, b.mz + 1, 2, 0);
       k_viscous_face<2><<<tile_blocks(tz), tpb, 0, s>>>(b, sc, tz);
     }
     return hipGetLastError();
@@ -1289,7 +1295,7 @@ hipError_t launch_entropy_fix(const DevBlock &b, const Scheme &sc, hipStream_t s
 template<int DIR, int R>
 static void dispatch_rs(const DevBlock &b, const Scheme &sc, dim3 grid, double *F, hipStream_t s, int z0) {
   const dim3 tpb(64, 4, 1);
-  const TileGrid tg{(int)grid.x, (int)grid.y, (int)grid.z, DIR, z0};
+  const TileGrid tg = plane_grid((int)grid.x, (int)grid.y, (int)grid.z, DIR == 2 ? 2 : 0, z0);
   const unsigned nb = tile_blocks(tg);
   switch (sc.inviscid_scheme) {
     case 1: k_inviscid<DIR, R, 1><<<nb, tpb, 0, s>>>(b, sc, F, tg); break;
@@ -1302,7 +1308,7 @@ static void dispatch_rs(const DevBlock &b, const Scheme &sc, dim3 grid, double *
 template<int DIR, int REC>
 static void dispatch_pv(const DevBlock &b, const Scheme &sc, dim3 grid, double *F, hipStream_t s, int z0) {
   const dim3 tpb(64, 4, 1);
-  const TileGrid tg{(int)grid.x, (int)grid.y, (int)grid.z, DIR, z0};
+  const TileGrid tg = plane_grid((int)grid.x, (int)grid.y, (int)grid.z, DIR == 2 ? 2 : 0, z0);
   const unsigned nb = tile_blocks(tg);
   switch (sc.inviscid_scheme) {
     case 1: k_inviscid_pv<DIR, REC, 1><<<nb, tpb, 0, s>>>(b, sc, F, tg); break;
@@ -1322,7 +1328,7 @@ static void dispatch_pv_rec(const DevBlock &b, const Scheme &sc, dim3 grid, doub
 template<int DIR>
 static void dispatch_fs(const DevBlock &b, const Scheme &sc, dim3 grid, double *F, hipStream_t s, int z0) {
   const dim3 tpb(64, 4, 1);
-  const TileGrid tg{(int)grid.x, (int)grid.y, (int)grid.z, DIR, z0};
+  const TileGrid tg = plane_grid((int)grid.x, (int)grid.y, (int)grid.z, DIR == 2 ? 2 : 0, z0);
   const unsigned nb = tile_blocks(tg);
   switch (sc.inviscid_scheme) {
     case 51: k_fsweno<DIR, 3, false><<<nb, tpb, 0, s>>>(b, sc, F, tg); break;
@@ -1336,18 +1342,18 @@ hipError_t launch_inviscid_flux(const DevBlock &b, const Scheme &sc, int dir, hi
   // the interfaces of the node planes [kb, ke): F(i+1/2) stored at node i for i = -1 .. mx-1 (x), likewise in y; in
   // z the interfaces k + 1/2 of planes kb .. ke-1, and also -1/2 when kb = 0 (tile z = k + 1)
   const int me = ke < b.mz ? ke : b.mz;
-  dim3 grid;
+  dim3 grid; // nodes along i and j, planes along k
   double *F;
   int z0 = kb;
   if (dir == 0) {
-    grid = dim3((b.mx + 1 + 63) / 64, (b.my + 3) / 4, me - kb);
+    grid = dim3(b.mx + 1, b.my, me - kb);
     F = b.fx;
   } else if (dir == 1) {
-    grid = dim3((b.mx + 63) / 64, (b.my + 1 + 3) / 4, me - kb);
+    grid = dim3(b.mx, b.my + 1, me - kb);
     F = b.fy;
   } else {
     z0 = kb == 0 ? 0 : kb + 1;
-    grid = dim3((b.mx + 63) / 64, (b.my + 3) / 4, me + 1 - z0);
+    grid = dim3(b.mx, b.my, me + 1 - z0);
     F = b.fz;
   }
   if ((int)grid.z <= 0) return hipSuccess;
