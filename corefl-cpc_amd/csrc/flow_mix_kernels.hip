@@ -125,11 +125,25 @@ __global__ __launch_bounds__(256) void k_mix_boundary(DevBlock b, MixArrays x, D
   mix_cons(b, x, ig);
 }
 
+// Node (i, j) of this thread in a launch over the nx x ny nodes of each plane: every 256-thread block takes 256
+// consecutive nodes in row-major order (grid.x blocks per plane, grid.z planes), so that only the last block of a
+// plane has idle lanes (a 64-wide tiling of 129 nodes leaves a third wave per row with one active lane).
+__device__ __forceinline__ bool chunk_node(int nx, int ny, int &i, int &j) {
+  const unsigned p = blockIdx.x * 256u + threadIdx.y * 64u + threadIdx.x;
+  j = (int)(p / (unsigned)nx);
+  i = (int)(p - (unsigned)j * (unsigned)nx);
+  return j < ny;
+}
+
+inline dim3 chunk_grid(int nx, int ny, int nz) { return dim3((unsigned)((nx * ny + 255) / 256), 1, nz); }
+
 // compute_entropy_fix_delta<Mixture> (InviscidScheme.cu:269-304), interior plus one layer.
 __global__ __launch_bounds__(256) void k_mix_entropy_fix(DevBlock b, MixArrays x, Scheme sc) {
-  const int i = blockIdx.x * 64 + threadIdx.x - 1, j = blockIdx.y * 4 + threadIdx.y - 1;
+  int i, j;
+  if (!chunk_node(b.mx + 2, b.my + 2, i, j)) return;
+  i -= 1;
+  j -= 1;
   const int k = b.dim == 3 ? (int)blockIdx.z - 1 : 0;
-  if (i > b.mx || j > b.my) return;
   const long id = nid(b, i, j, k), sz = b.sz;
   const double *M = b.met + id;
   const double u = b.bv[id + sz], v = b.bv[id + 2 * sz], w = b.bv[id + 3 * sz];
@@ -435,7 +449,8 @@ __device__ void hllc(const ChemMech &m, const double *pl, const double *pr, cons
 template<int DIR, int REC, int RS>
 __global__ __launch_bounds__(256) void k_mix_flux(DevBlock b, MixArrays x, Scheme sc, double *F) {
   constexpr int R = REC >= 4 ? 3 : 2;
-  int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+  int i, j, k = blockIdx.z;
+  if (!chunk_node(b.mx + (DIR == 0), b.my + (DIR == 1), i, j)) return;
   if constexpr (DIR == 0) i -= 1;
   else if constexpr (DIR == 1) j -= 1;
   else k -= 1;
@@ -501,8 +516,9 @@ __global__ __launch_bounds__(256) void k_mix_transport(DevBlock b, MixArrays x) 
 // velocity.  Gradients by the collocated ORDER-th central difference with one-sided rows at physical faces.
 template<int ORDER>
 __global__ __launch_bounds__(256) void k_mix_viscous_node(DevBlock b, MixArrays x) {
-  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
-  if (i >= b.mx || j >= b.my) return;
+  int i, j;
+  const int k = blockIdx.z;
+  if (!chunk_node(b.mx, b.my, i, j)) return;
   const long id = nid(b, i, j, k), sz = b.sz;
   const bool z3 = b.dim == 3;
   const double *M = b.met + id;
@@ -588,7 +604,8 @@ __global__ __launch_bounds__(256) void k_mix_viscous_node(DevBlock b, MixArrays 
 // stored at node t for the face t+1/2 along DIR, t = -1 .. m-1, components 1..4 and 5..5+NS-1.
 template<int DIR>
 __global__ __launch_bounds__(256) void k_mix_viscous_face(DevBlock b, MixArrays x, Scheme sc) {
-  int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+  int i, j, k = blockIdx.z;
+  if (!chunk_node(b.mx + (DIR == 0), b.my + (DIR == 1), i, j)) return;
   if constexpr (DIR == 0) i -= 1;
   else if constexpr (DIR == 1) j -= 1;
   else k -= 1;
@@ -711,8 +728,9 @@ __device__ __forceinline__ double mix_state(const ChemMech &m, double *q, double
 template<bool RHS_ONLY>
 __global__ __launch_bounds__(256) void k_mix_update(DevBlock b, MixArrays x, int rk, const double *__restrict__ dtp,
                                                     const double *__restrict__ cn, double dts) {
-  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
-  if (i >= b.mx || j >= b.my) return;
+  int i, j;
+  const int k = blockIdx.z;
+  if (!chunk_node(b.mx, b.my, i, j)) return;
   const long id = nid(b, i, j, k), sz = b.sz;
   double r[NV];
 #pragma unroll
@@ -857,8 +875,9 @@ __global__ __launch_bounds__(256) void k_mix_local_dt(DevBlock b, MixArrays x, S
 }
 
 __global__ __launch_bounds__(256) void k_mix_chem_gather(DevBlock b, MixArrays x, ChemBatch c) {
-  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
-  if (i >= b.mx || j >= b.my) return;
+  int i, j;
+  const int k = blockIdx.z;
+  if (!chunk_node(b.mx, b.my, i, j)) return;
   const long id = nid(b, i, j, k), sz = b.sz;
   const long q = ((long)k * b.my + j) * b.mx + i;
   const double rho = b.cv[id];
@@ -871,8 +890,9 @@ __global__ __launch_bounds__(256) void k_mix_chem_gather(DevBlock b, MixArrays x
 }
 
 __global__ __launch_bounds__(256) void k_mix_chem_scatter(DevBlock b, MixArrays x, ChemBatch c) {
-  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
-  if (i >= b.mx || j >= b.my) return;
+  int i, j;
+  const int k = blockIdx.z;
+  if (!chunk_node(b.mx, b.my, i, j)) return;
   const long id = nid(b, i, j, k), sz = b.sz;
   const long q = ((long)k * b.my + j) * b.mx + i;
   const ChemMech &m = *x.mech;
@@ -915,7 +935,8 @@ void dispatch_rs(const DevBlock &b, const MixArrays &x, const Scheme &sc, dim3 g
 // one at a time, to stay within the register file.
 template<int DIR, int R, bool CH>
 __global__ __launch_bounds__(256) void k_mix_fsweno(DevBlock b, MixArrays x, Scheme sc, double *F) {
-  int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+  int i, j, k = blockIdx.z;
+  if (!chunk_node(b.mx + (DIR == 0), b.my + (DIR == 1), i, j)) return;
   if constexpr (DIR == 0) i -= 1;
   else if constexpr (DIR == 1) j -= 1;
   else k -= 1;
@@ -1176,16 +1197,16 @@ hipError_t launch_mix_boundary(const DevBlock &b, const MixArrays &x, const DevF
 }
 
 hipError_t launch_mix_entropy_fix(const DevBlock &b, const MixArrays &x, const Scheme &sc, hipStream_t s) {
-  const dim3 grid((b.mx + 2 + 63) / 64, (b.my + 2 + 3) / 4, b.dim == 3 ? b.mz + 2 : 1);
+  const dim3 grid = chunk_grid(b.mx + 2, b.my + 2, b.dim == 3 ? b.mz + 2 : 1);
   k_mix_entropy_fix<<<grid, dim3(64, 4, 1), 0, s>>>(b, x, sc);
   return hipGetLastError();
 }
 
 hipError_t launch_mix_inviscid_flux(const DevBlock &b, const MixArrays &x, const Scheme &sc, int dir, hipStream_t s) {
   if (x.ns != NS) return hipErrorNotSupported;
-  if (dir == 0) dispatch_rec<0>(b, x, sc, dim3((b.mx + 1 + 63) / 64, (b.my + 3) / 4, b.mz), b.fx, s);
-  else if (dir == 1) dispatch_rec<1>(b, x, sc, dim3((b.mx + 63) / 64, (b.my + 1 + 3) / 4, b.mz), b.fy, s);
-  else dispatch_rec<2>(b, x, sc, dim3((b.mx + 63) / 64, (b.my + 3) / 4, b.mz + 1), b.fz, s);
+  if (dir == 0) dispatch_rec<0>(b, x, sc, chunk_grid(b.mx + 1, b.my, b.mz), b.fx, s);
+  else if (dir == 1) dispatch_rec<1>(b, x, sc, chunk_grid(b.mx, b.my + 1, b.mz), b.fy, s);
+  else dispatch_rec<2>(b, x, sc, chunk_grid(b.mx, b.my, b.mz + 1), b.fz, s);
   return hipGetLastError();
 }
 
@@ -1198,8 +1219,9 @@ hipError_t launch_mix_inviscid_flux(const DevBlock &b, const MixArrays &x, const
 // source (METHOD 3), then clips and renormalises the species and solves for T.  n_sub is capped at 10^6.
 template<int METHOD>
 __global__ __launch_bounds__(256) void k_mix_wu_reaction(DevBlock b, MixArrays x, const double *__restrict__ dtp) {
-  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
-  if (i >= b.mx || j >= b.my) return;
+  int i, j;
+  const int k = blockIdx.z;
+  if (!chunk_node(b.mx, b.my, i, j)) return;
   const long id = nid(b, i, j, k), sz = b.sz;
   const long iq = ((long)k * b.my + j) * b.mx + i;
   const ChemMech &m = *x.mech;
@@ -1291,36 +1313,36 @@ hipError_t launch_mix_viscous(const DevBlock &b, const MixArrays &x, const Schem
   k_mix_transport<<<(unsigned)((b.sz + 255) / 256), 256, 0, s>>>(b, x);
   const dim3 tpb(64, 4, 1);
   if (sc.viscous_order == 8) {
-    k_mix_viscous_node<8><<<dim3((b.mx + 63) / 64, (b.my + 3) / 4, b.mz), tpb, 0, s>>>(b, x);
+    k_mix_viscous_node<8><<<chunk_grid(b.mx, b.my, b.mz), tpb, 0, s>>>(b, x);
     return hipGetLastError();
   }
   if (sc.viscous_order != 2) return hipErrorNotSupported;
-  k_mix_viscous_face<0><<<dim3((b.mx + 1 + 63) / 64, (b.my + 3) / 4, b.mz), tpb, 0, s>>>(b, x, sc);
-  k_mix_viscous_face<1><<<dim3((b.mx + 63) / 64, (b.my + 1 + 3) / 4, b.mz), tpb, 0, s>>>(b, x, sc);
-  if (b.dim == 3) k_mix_viscous_face<2><<<dim3((b.mx + 63) / 64, (b.my + 3) / 4, b.mz + 1), tpb, 0, s>>>(b, x, sc);
+  k_mix_viscous_face<0><<<chunk_grid(b.mx + 1, b.my, b.mz), tpb, 0, s>>>(b, x, sc);
+  k_mix_viscous_face<1><<<chunk_grid(b.mx, b.my + 1, b.mz), tpb, 0, s>>>(b, x, sc);
+  if (b.dim == 3) k_mix_viscous_face<2><<<chunk_grid(b.mx, b.my, b.mz + 1), tpb, 0, s>>>(b, x, sc);
   return hipGetLastError();
 }
 
 hipError_t launch_mix_wu_reaction(const DevBlock &b, const MixArrays &x, const double *dt_dev, int method,
                                   hipStream_t s) {
   if (x.ns != NS) return hipErrorNotSupported;
-  if (method == 3) k_mix_wu_reaction<3><<<node_grid(b), dim3(64, 4, 1), 0, s>>>(b, x, dt_dev);
-  else if (method == 1) k_mix_wu_reaction<1><<<node_grid(b), dim3(64, 4, 1), 0, s>>>(b, x, dt_dev);
-  else k_mix_wu_reaction<0><<<node_grid(b), dim3(64, 4, 1), 0, s>>>(b, x, dt_dev);
+  if (method == 3) k_mix_wu_reaction<3><<<chunk_grid(b.mx, b.my, b.mz), dim3(64, 4, 1), 0, s>>>(b, x, dt_dev);
+  else if (method == 1) k_mix_wu_reaction<1><<<chunk_grid(b.mx, b.my, b.mz), dim3(64, 4, 1), 0, s>>>(b, x, dt_dev);
+  else k_mix_wu_reaction<0><<<chunk_grid(b.mx, b.my, b.mz), dim3(64, 4, 1), 0, s>>>(b, x, dt_dev);
   return hipGetLastError();
 }
 
 hipError_t launch_mix_wu_update(const DevBlock &b, const MixArrays &x, int rk, const double *dt_dev, hipStream_t s) {
   if (x.ns != NS) return hipErrorNotSupported;
-  k_mix_update<false><<<node_grid(b), dim3(64, 4, 1), 0, s>>>(b, x, rk, dt_dev, b.dq, 0.5);
+  k_mix_update<false><<<chunk_grid(b.mx, b.my, b.mz), dim3(64, 4, 1), 0, s>>>(b, x, rk, dt_dev, b.dq, 0.5);
   return hipGetLastError();
 }
 
 hipError_t launch_mix_update(const DevBlock &b, const MixArrays &x, int rk, const double *dt_dev, bool rhs_only,
                              hipStream_t s) {
   if (x.ns != NS) return hipErrorNotSupported;
-  if (rhs_only) k_mix_update<true><<<node_grid(b), dim3(64, 4, 1), 0, s>>>(b, x, rk, dt_dev, nullptr, 1.0);
-  else k_mix_update<false><<<node_grid(b), dim3(64, 4, 1), 0, s>>>(b, x, rk, dt_dev, nullptr, 1.0);
+  if (rhs_only) k_mix_update<true><<<chunk_grid(b.mx, b.my, b.mz), dim3(64, 4, 1), 0, s>>>(b, x, rk, dt_dev, nullptr, 1.0);
+  else k_mix_update<false><<<chunk_grid(b.mx, b.my, b.mz), dim3(64, 4, 1), 0, s>>>(b, x, rk, dt_dev, nullptr, 1.0);
   return hipGetLastError();
 }
 
@@ -1331,13 +1353,13 @@ hipError_t launch_mix_local_dt(const DevBlock &b, const MixArrays &x, const Sche
 
 hipError_t launch_mix_chem_gather(const DevBlock &b, const MixArrays &x, const ChemBatch &c, hipStream_t s) {
   if (x.ns != NS || c.n != b.n) return hipErrorInvalidValue;
-  k_mix_chem_gather<<<node_grid(b), dim3(64, 4, 1), 0, s>>>(b, x, c);
+  k_mix_chem_gather<<<chunk_grid(b.mx, b.my, b.mz), dim3(64, 4, 1), 0, s>>>(b, x, c);
   return hipGetLastError();
 }
 
 hipError_t launch_mix_chem_scatter(const DevBlock &b, const MixArrays &x, const ChemBatch &c, hipStream_t s) {
   if (x.ns != NS || c.n != b.n) return hipErrorInvalidValue;
-  k_mix_chem_scatter<<<node_grid(b), dim3(64, 4, 1), 0, s>>>(b, x, c);
+  k_mix_chem_scatter<<<chunk_grid(b.mx, b.my, b.mz), dim3(64, 4, 1), 0, s>>>(b, x, c);
   return hipGetLastError();
 }
 
