@@ -1107,9 +1107,10 @@ __global__ void k_cv_from_bv(DevBlock b) {
 // ---------------------------------------------------------------- CFL time step
 // local_time_step<Air> (TimeAdvanceFunc.cuh:28-115) fused with a workgroup minimum.
 __global__ __launch_bounds__(256) void k_local_dt(DevBlock b, Scheme sc, double *part) {
-  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+  const unsigned pn = blockIdx.x * 256u + threadIdx.y * 64u + threadIdx.x;
+  const int j = (int)(pn / (unsigned)b.mx), i = (int)(pn - (unsigned)j * (unsigned)b.mx), k = blockIdx.z;
   double dt = 1e+6;
-  if (i < b.mx && j < b.my) {
+  if (j < b.my) {
     const long id = nid(b, i, j, k), sz = b.sz;
     const double *M = b.met + id;
     const double gx = sqrt(M[0] * M[0] + M[sz] * M[sz] + M[2 * sz] * M[2 * sz]);
@@ -1159,7 +1160,9 @@ __global__ __launch_bounds__(1024) void k_dt_reduce(const double *part, long n, 
 
 __global__ void k_time_advance(double *time, const double *dt) { time[0] += dt[0]; }
 
-dim3 node_grid(const DevBlock &b) { return dim3((b.mx + 63) / 64, (b.my + 3) / 4, b.mz); }
+// one workgroup per 256 consecutive nodes of a plane (row-major), grid.z planes; the per-workgroup minima of the time
+// step are indexed by workgroup in this grid
+dim3 node_grid(const DevBlock &b) { return dim3((unsigned)((b.mx * b.my + 255) / 256), 1, b.mz); }
 
 // Planes per column of the z-marching kernels.
 int march_kz() {

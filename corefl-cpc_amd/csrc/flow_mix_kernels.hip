@@ -828,9 +828,11 @@ __global__ __launch_bounds__(256) void k_mix_update(DevBlock b, MixArrays x, int
 
 // local_time_step<Mixture> (TimeAdvanceFunc.cuh:28-115, inviscid) fused with a workgroup minimum.
 __global__ __launch_bounds__(256) void k_mix_local_dt(DevBlock b, MixArrays x, Scheme sc, double *part) {
-  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+  int i, j;
+  const int k = blockIdx.z;
+  const bool in = chunk_node(b.mx, b.my, i, j);
   double dt = 1e+6;
-  if (i < b.mx && j < b.my) {
+  if (in) {
     const long id = nid(b, i, j, k), sz = b.sz;
     const double *M = b.met + id;
     const double gx = sqrt(M[0] * M[0] + M[sz] * M[sz] + M[2 * sz] * M[2 * sz]);
@@ -915,7 +917,8 @@ __global__ __launch_bounds__(256) void k_mix_chem_scatter(DevBlock b, MixArrays 
   x.snd[id] = sqrt(cpt / (cpt - R) * R * T);
 }
 
-dim3 node_grid(const DevBlock &b) { return dim3((b.mx + 63) / 64, (b.my + 3) / 4, b.mz); }
+// the grid of k_mix_local_dt: the same workgroups as the single-species time-step kernel's
+dim3 node_grid(const DevBlock &b) { return chunk_grid(b.mx, b.my, b.mz); }
 
 template<int DIR, int REC>
 void dispatch_rs(const DevBlock &b, const MixArrays &x, const Scheme &sc, dim3 grid, double *F, hipStream_t s) {
