@@ -36,9 +36,9 @@ STEP_BYTES = 560.0
 # (40 B) written
 FLUX_BYTES = 112.0
 # flux kernel, per interface: FP64 operations (FMA counted twice) of WENO7-characteristic + AUSM+, as counted
-# by SQ_INSTS_VALU_{FMA,MUL,ADD}_F64 on the 256^3 workload (60.9 GFLOP per launch over 257 x 257 x 258
+# by SQ_INSTS_VALU_{FMA,MUL,ADD}_F64 on the 256^3 workload (42.3 GFLOP per launch over 257 x 257 x 258
 # interfaces; profiles/flow_tgv256_fp64_valu.md)
-FLUX_FLOP = 3574.0
+FLUX_FLOP = 2480.0
 FP64_PEAK_TFLOPS = 78.6
 SCHEME_NAMES = {1: "WENO7-ch + LF", 2: "WENO7-ch + Roe", 3: "WENO7-ch + AUSM+", 4: "WENO7-ch + HLLC",
                 51: "WENO5-cp LLF flux splitting", 52: "WENO5-ch LLF flux splitting",
