@@ -1174,7 +1174,7 @@ int march_kz() {
   return kz;
 }
 
-// Tiles of the z-marching kernels: 64 x 4 columns of kz planes each.
+// Work-groups of the z-marching kernels: 256 consecutive (i, j) columns of a plane, kz planes each.
 TileGrid march_tiles(const DevBlock &b, int kz) {
   return plane_grid(b.mx, b.my, (b.mz + kz - 1) / kz, 0, 0);
 }
