@@ -373,10 +373,10 @@ struct corefl_ctx {
     if (!rhs_only) communicate(s);
   }
 
-  // Slab thickness (planes) of the air stages in z-slabs (setup key stage_slab_planes; -1: four march lengths);
+  // Slab thickness (planes) of the air stages in z-slabs (setup key stage_slab_planes; -1: two march lengths);
   // 0 when that is 0 or a block is two-dimensional or too thin.
   int slab_planes() const {
-    const int P = slab_setting < 0 ? 4 * corefl::march_planes() : slab_setting;
+    const int P = slab_setting < 0 ? 2 * corefl::march_planes() : slab_setting;
     if (P == 0) return 0;
     for (const auto &b : blocks)
       if (b.dev.dim != 3 || b.dev.mz < 2 * P || (b.dev.mz + P - 1) / P > kMaxSlabs) return 0;
