@@ -171,11 +171,15 @@ __device__ __forceinline__ double weno5_left(const double *v, double eps) {
                     0.25 * (v[1] - v[3]) * (v[1] - v[3]);
   const double b2 = 13.0 / 12 * (v[2] - 2 * v[3] + v[4]) * (v[2] - 2 * v[3] + v[4]) +
                     0.25 * (3 * v[2] - 4 * v[3] + v[4]) * (3 * v[2] - 4 * v[3] + v[4]);
-  const double a0 = 0.1 / ((b0 + eps) * (b0 + eps)), a1 = 0.6 / ((b1 + eps) * (b1 + eps)),
-               a2 = 0.3 / ((b2 + eps) * (b2 + eps));
-  const double q0 = (2 * v[0] - 7 * v[1] + 11 * v[2]) / 6, q1 = (-v[1] + 5 * v[2] + 2 * v[3]) / 6,
-               q2 = (2 * v[2] + 5 * v[3] - v[4]) / 6;
-  return (a0 * q0 + a1 * q1 + a2 * q2) / (a0 + a1 + a2);
+  // the divisions as products with refined reciprocals (wenodev::rcp); 1/d0 and 1/d1 from one reciprocal of
+  // their product (each d >= eps^2, so the product stays normal)
+  const double d0 = (b0 + eps) * (b0 + eps), d1 = (b1 + eps) * (b1 + eps), d2 = (b2 + eps) * (b2 + eps);
+  const double r01 = wenodev::rcp(d0 * d1);
+  const double a0 = 0.1 * (d1 * r01), a1 = 0.6 * (d0 * r01), a2 = 0.3 * wenodev::rcp(d2);
+  constexpr double o6 = 1.0 / 6;
+  const double q0 = (2 * v[0] - 7 * v[1] + 11 * v[2]) * o6, q1 = (-v[1] + 5 * v[2] + 2 * v[3]) * o6,
+               q2 = (2 * v[2] + 5 * v[3] - v[4]) * o6;
+  return (a0 * q0 + a1 * q1 + a2 * q2) * wenodev::rcp(a0 + a1 + a2);
 }
 
 // Primitive reconstruction at the interface between stencil points R-1 and R of pv[2R][NV].
